@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark: pressure-solve time to 1e-8 true relative residual on the
+128^3 perturbed, permuted hex-mesh pressure Poisson system (2,097,152 cells,
+14,581,760 nonzeros), mixed FP32/FP64 against plain FP64 on one MI355X, plus
+SpMV bandwidth against the 8 TB/s HBM peak.
+
+One "step" is one full pressure solve (x0 = 0, FP64 true residual <= 1e-8)
+of the mixed-precision solver on a system assembled on the GPU from the SoA
+mesh.  Inputs (matrix, rhs) are resident in HBM before the timed region.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n, nnz, vbytes, ibytes=4):
+    # values + column indices + row pointers + x read once + y written once
+    return nnz * (vbytes + ibytes) + (n + 1) * 4 + 2 * n * vbytes
+
+
+def cpu_baseline(A_rp, A_ci, A_vals, b, iterations_full, sample_iters):
+    """Times the oracle's serial Jacobi-PCG (the reference algorithm, 1 core)
+    for `sample_iters` iterations and projects to `iterations_full`."""
+    import oracle
+
+    Ao = oracle.Matrix(A_rp, A_ci, A_vals)
+    t0 = time.perf_counter()
+    _, it, _, _ = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-30, sample_iters)
+    dt = time.perf_counter() - t0
+    per_it = dt / max(it, 1)
+    return per_it * iterations_full * 1e3, dt, it
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="cells per axis")
+    ap.add_argument("--pc", default="jacobi", choices=["jacobi", "amg"])
+    ap.add_argument("--cpu-iters", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from simall_amd import capi
+    from simall_amd import workload as wl
+
+    lib = capi.load()
+    capi.check(lib.mi_init(0))
+    t0 = time.perf_counter()
+    w = wl.poisson3d(args.n)
+    m = w.mesh
+    log(f"mesh {args.n}^3: {m.n_cells} cells, {m.n_faces} faces ({time.perf_counter() - t0:.1f}s)")
+    dm = capi.DeviceMesh(m, w.bc_of_zone)
+    A = dm.create_matrix()
+    d_rhs = capi.DeviceArray(m.n_cells)
+    d_gamma = capi.DeviceArray.from_host(w.gamma)
+    pinned = dm.assemble_pressure(A, d_rhs, d_gamma=d_gamma)
+    b = w.b.copy()
+    if pinned:
+        b[0] = 0.0
+    n, nnz = A.n, A.nnz
+    log(f"assembled on GPU: n={n} nnz={nnz} pinned={pinned} ({time.perf_counter() - t0:.1f}s)")
+    d_b = capi.DeviceArray.from_host(b)
+    d_x = capi.DeviceArray(n)
+    pc = capi.MI_PC_AMG if args.pc == "amg" else capi.MI_PC_JACOBI
+
+    def make_solver(precision):
+        s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=pc, tolerance=1e-8,
+                        max_iterations=20000, precision=precision, true_residual=1)
+        s.setup(A)
+        return s
+
+    results = {}
+    for name, prec in (("fp64", capi.MI_PRECISION_FP64), ("mixed", capi.MI_PRECISION_MIXED)):
+        s = make_solver(prec)
+        for _ in range(args.warmup):
+            st, rc = s.solve_device(d_b, d_x)
+        lib.mi_device_synchronize()
+        times, stats = [], None
+        for _ in range(args.steps):
+            lib.mi_device_synchronize()
+            t1 = time.perf_counter()
+            st, rc = s.solve_device(d_b, d_x)
+            lib.mi_device_synchronize()
+            times.append(time.perf_counter() - t1)
+            stats = st.as_dict()
+        x = d_x.download()
+        results[name] = dict(ms=1e3 * float(np.median(times)), total_s=float(sum(times)),
+                             stats=stats, rc=rc, x=x)
+        log(f"{name}: {results[name]['ms']:.2f} ms/solve, its={stats['iterations']} "
+            f"outer={stats['outer_iterations']} rel={stats['true_relative_residual']:.3e} "
+            f"setup={stats['setup_seconds']:.3f}s rc={rc}")
+
+    # SpMV bandwidth (the dominant kernel), FP64 and the FP32 inner form
+    dx = capi.DeviceArray.from_host(np.ones(n))
+    dy = capi.DeviceArray(n)
+    us64 = A.time_spmv(dx, dy, capi.MI_PRECISION_FP64, reps=50)
+    dx32 = capi.DeviceArray.from_host(np.ones(n, np.float32))
+    dy32 = capi.DeviceArray(n, np.float32)
+    us32 = A.time_spmv(dx32, dy32, capi.MI_PRECISION_MIXED, reps=50)
+    by64, by32 = spmv_bytes(n, nnz, 8), spmv_bytes(n, nnz, 4)
+    gbs64, gbs32 = by64 / us64 / 1e3, by32 / us32 / 1e3
+    log(f"SpMV fp64 {us64:.1f} us {gbs64:.0f} GB/s ({100 * gbs64 / HBM_PEAK_GBS:.1f}%); "
+        f"fp32 {us32:.1f} us {gbs32:.0f} GB/s")
+
+    x64, xm = results["fp64"]["x"], results["mixed"]["x"]
+    err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+
+    cpu = None
+    if not args.no_cpu:
+        rp, ci = A.pattern()
+        vals = A.values()
+        its = results["fp64"]["stats"]["iterations"]
+        proj_ms, dt, done = cpu_baseline(rp, ci, vals, b, its, args.cpu_iters)
+        cpu = {"value": round(proj_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+               "sample": f"{done} iterations of the oracle's serial Jacobi-PCG on the same "
+                         f"{args.n}^3 system ({dt:.1f} s), projected to the {its} iterations "
+                         f"of the FP64 GPU solve"}
+        log(f"cpu baseline: {cpu}")
+
+    mixed_ms = results["mixed"]["ms"]
+    fp64_ms = results["fp64"]["ms"]
+    out = {
+        "metric": "pressure-solve time to 1e-8 rel residual, mixed vs FP64 GPU; SpMV HBM GB/s",
+        "value": round(mixed_ms, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 inner / fp64 refinement",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.n}^3 perturbed+permuted hex pressure Poisson, log-uniform "
+                        f"conductance, {args.pc}-PCG to 1e-8 true residual",
+            "n": n, "nnz": nnz, "preconditioner": args.pc,
+        },
+        "fp64_ms": round(fp64_ms, 3),
+        "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
+        "iterations": {"fp64": results["fp64"]["stats"]["iterations"],
+                       "mixed_inner": results["mixed"]["stats"]["iterations"],
+                       "mixed_outer": results["mixed"]["stats"]["outer_iterations"]},
+        "true_relative_residual": {"fp64": results["fp64"]["stats"]["true_relative_residual"],
+                                   "mixed": results["mixed"]["stats"]["true_relative_residual"]},
+        "mixed_vs_fp64_solution_rel_diff": err,
+        "spmv": {"fp64_us": round(us64, 2), "fp64_GBps": round(gbs64, 1),
+                 "fp32_us": round(us32, 2), "fp32_GBps": round(gbs32, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,261 @@
+/*
+ * simall_amd.h — C ABI of the MI355X (gfx950) pressure-correction solve backend.
+ *
+ * Plain C: pointers, sizes and integer status codes.  No STL or torch types
+ * cross this boundary.  Each group of entry points names the reference
+ * interface it replaces (paths relative to the SimAll source tree).
+ *
+ * Conventions
+ *   - Every function returns an mi_status (MI_OK == 0).  On failure the
+ *     thread-local mi_last_error() describes the cause.
+ *   - "host" arguments are ordinary CPU memory, borrowed for the call.
+ *     "device" arguments are HIP device pointers (mi_device_alloc, hipMalloc
+ *     or a torch CUDA tensor's data_ptr()).
+ *   - Handles (mi_csr, mi_solver, mi_mesh) own their device memory and are
+ *     released by the matching *_destroy.  A handle is used by one host
+ *     thread at a time; distinct handles may be used concurrently.
+ *   - CSR: int32 rowPtr[n+1], int32 colIdx[nnz] (sorted per row), double
+ *     values[nnz], as solver::CSRMatrix (src/solver/CSRMatrix.hpp:16-23).
+ */
+#ifndef SIMALL_AMD_H
+#define SIMALL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(_WIN32)
+#define MI_API __declspec(dllexport)
+#else
+#define MI_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MI_ABI_VERSION 1
+
+/* ------------------------------------------------------------- status codes
+ * The reference's solvers never throw: non-convergence is the returned
+ * iteration count plus last_residual (src/solver/LinearSolvers.cpp:451),
+ * GPU failures are logged with default stats (src/gpu/CgGpu.cpp:105-116).
+ * These codes carry the same distinctions. */
+typedef enum {
+    MI_OK = 0,
+    MI_ERR_NOT_CONVERGED = 1, /* max iterations reached; x holds the last iterate */
+    MI_ERR_BREAKDOWN = 2,     /* Krylov breakdown (p.Ap == 0, rho == 0, ...) */
+    MI_ERR_BAD_ARG = 3,
+    MI_ERR_DEVICE = 4,        /* HIP runtime error */
+    MI_ERR_NO_DEVICE = 5,     /* no usable gfx950 device */
+    MI_ERR_UNSUPPORTED = 6,   /* solver/preconditioner combination not on device */
+    MI_ERR_ZERO_PIVOT = 7     /* ILU(0) met a zero pivot */
+} mi_status;
+
+/* Same ordinals as solver::LinearSolverKind / PreconditionerKind
+ * (src/solver/Solver.hpp:67-81). */
+enum { MI_SOLVER_GMRES = 0, MI_SOLVER_BICGSTAB = 1, MI_SOLVER_CG = 2, MI_SOLVER_TFQMR = 3 };
+enum { MI_PC_JACOBI = 0, MI_PC_ILU = 1, MI_PC_SSOR = 2, MI_PC_AMG = 3, MI_PC_NONE = 4 };
+/* Arithmetic of the Krylov iterations. */
+enum { MI_PRECISION_FP64 = 0, MI_PRECISION_MIXED = 1 };
+
+/* ---------------------------------------------------------------- runtime */
+/* Replaces CudaContext (src/gpu/CudaContext.hpp:76-127) and
+ * Device::is_available (src/gpu/Device.cpp:6-9). */
+MI_API int mi_get_abi_version(void);
+MI_API int mi_init(int device);
+MI_API int mi_device_count(int* count);
+MI_API const char* mi_last_error(void);
+MI_API int mi_device_synchronize(void);
+
+/* Device memory: DeviceMemoryPool::allocate/release
+ * (src/gpu/DeviceMemoryPool.hpp:64-104) and HostDeviceMirror::to_device /
+ * to_host (src/gpu/HostDeviceMirror.hpp:121-151). */
+MI_API int mi_device_alloc(size_t bytes, void** ptr);
+MI_API int mi_device_free(void* ptr);
+MI_API int mi_memcpy_h2d(void* dst_device, const void* src_host, size_t bytes);
+MI_API int mi_memcpy_d2h(void* dst_host, const void* src_device, size_t bytes);
+MI_API int mi_memcpy_d2d(void* dst_device, const void* src_device, size_t bytes);
+MI_API int mi_memset_device(void* dst_device, int value, size_t bytes);
+
+/* ------------------------------------------------- vector and sparse kernels
+ * The free functions of src/gpu/GpuKernels.hpp:47-108 on device pointers,
+ * double precision, element counts.  `stream` is a hipStream_t or NULL (the
+ * library's stream).  dot / nrm2 block until the result is on the host,
+ * like the reference. */
+MI_API int mi_axpy(double alpha, const void* x, void* y, size_t n, void* stream);
+MI_API int mi_scal(double alpha, void* x, size_t n, void* stream);
+MI_API int mi_copy(const void* src, void* dst, size_t n, void* stream);
+MI_API int mi_dot(const void* x, const void* y, size_t n, double* result, void* stream);
+MI_API int mi_nrm2(const void* x, size_t n, double* result, void* stream);
+MI_API int mi_axpby(double a, const void* x, double b, void* y, size_t n, void* stream);
+MI_API int mi_xpby(const void* x, double beta, void* y, size_t n, void* stream);
+MI_API int mi_spmv_csr(size_t n, const void* rowPtr, const void* colIdx, const void* values,
+                       const void* x, void* y, void* stream);
+/* z[i] = r[i] / diag[i] (GpuKernels.hpp:85) */
+MI_API int mi_jacobi_apply(const void* diag, const void* r, void* z, size_t n, void* stream);
+
+/* ------------------------------------------------------------ CSR handles
+ * Device-resident copy of a solver::CSRMatrix (src/solver/CSRMatrix.hpp:16-36);
+ * the role of CgGpu::set_matrix's HostCsrView upload (src/gpu/CgGpu.hpp:78-96).
+ * Arrays are copied; the caller keeps ownership of its host arrays. */
+typedef struct mi_csr mi_csr;
+MI_API int mi_csr_create(int64_t n, int64_t nnz, const int32_t* rowPtr, const int32_t* colIdx,
+                         const double* values, mi_csr** out);
+/* New values on the same pattern (the SIMPLE pattern is built once and
+ * re-filled every outer iteration: src/solver/SimpleAlgorithm.cpp:49-98).
+ * Solvers set up on this matrix rebuild their preconditioner on next solve. */
+MI_API int mi_csr_update_values(mi_csr* A, const double* values_host);
+MI_API int mi_csr_update_values_device(mi_csr* A, const double* values_device);
+MI_API int mi_csr_get_values(const mi_csr* A, double* values_host);
+MI_API int mi_csr_get_pattern(const mi_csr* A, int32_t* rowPtr_host, int32_t* colIdx_host);
+MI_API int mi_csr_info(const mi_csr* A, int64_t* n, int64_t* nnz);
+/* Device pointer to the FP64 values (for kernels that fill them in place). */
+MI_API int mi_csr_values_device(mi_csr* A, double** values_device);
+/* y = A x with host arrays (CSRMatrix::spmv, src/solver/CSRMatrix.hpp:25-35). */
+MI_API int mi_csr_spmv(mi_csr* A, const double* x_host, double* y_host);
+/* y = A x on device.  precision MI_PRECISION_FP64: double x,y;
+ * MI_PRECISION_MIXED: float values, float x,y (the inner-iteration form). */
+MI_API int mi_csr_spmv_device(mi_csr* A, int precision, const void* x_device, void* y_device,
+                              void* stream);
+/* Times `reps` back-to-back SpMV launches on one stream with HIP events and
+ * returns the average launch duration in microseconds. */
+MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x_device, void* y_device,
+                            int reps, double* avg_us);
+MI_API int mi_csr_destroy(mi_csr* A);
+
+/* ---------------------------------------------------------------- solvers
+ * Replaces ILinearSolver / make_linear_solver (src/solver/Solver.hpp:83-102,
+ * src/solver/Solver.cpp:25-39) and the unwired CgGpu / BicgstabGpu
+ * (src/gpu/CgGpu.hpp:49-125).  The first five fields are LinearSolverConfig
+ * with the same defaults (tolerance 1e-8, 1000 iterations, restart 30). */
+typedef struct {
+    int kind;           /* MI_SOLVER_* */
+    int preconditioner; /* MI_PC_* (None and unknown kinds act as Jacobi, as
+                           make_preconditioner: src/solver/LinearSolvers.cpp:240-254) */
+    double tolerance;   /* stop when ||r||/max(||b||,1e-30) < tolerance */
+    int max_iterations;
+    int restart;
+    int precision;      /* MI_PRECISION_FP64 or MI_PRECISION_MIXED */
+    int warm_start;     /* 0: x0 = 0 as the CPU solvers (LinearSolvers.cpp:424); 1: use x */
+    int true_residual;  /* 0: stop on the recurrence residual (reference); 1: the FP64
+                           true residual b - Ax must meet the tolerance */
+    double inner_tolerance; /* mixed: relative reduction of each FP32 inner solve */
+    int max_outer;      /* mixed: refinement sweeps */
+    int reorder;        /* -1 auto, 0 none, 1 reverse Cuthill-McKee inside setup */
+    int check_interval; /* iterations between host convergence polls (0 = auto) */
+    /* smoothed-aggregation AMG */
+    int amg_max_levels;
+    int amg_coarse_size;
+    double amg_strength;   /* aggregation strength threshold */
+    double amg_jacobi_omega; /* prolongator smoothing weight (scaled by 1/rho(D^-1A)) */
+    int amg_smoother;      /* 0 = Chebyshev, 1 = damped Jacobi */
+    int amg_sweeps;        /* pre = post sweeps / Chebyshev degree */
+} mi_solver_cfg;
+
+/* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */
+typedef struct {
+    int iterations;        /* Krylov iterations (all inner iterations when mixed) */
+    int outer_iterations;  /* refinement sweeps (mixed); 1 otherwise */
+    int converged;
+    int status;            /* mi_status of the solve */
+    double initial_residual; /* ||b - A x0|| */
+    double final_residual;   /* ||r|| at exit (recurrence or true, see cfg) */
+    double relative_residual; /* final_residual / max(||b||,1e-30) == last_residual() */
+    double true_relative_residual; /* FP64 ||b - A x|| / ||b|| computed on device at exit */
+    double setup_seconds;
+    double solve_seconds;
+} mi_stats;
+
+typedef struct mi_solver mi_solver;
+MI_API void mi_solver_cfg_default(mi_solver_cfg* cfg);
+MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out);
+/* Binds A; builds the preconditioner now (the reference rebuilds it inside
+ * every solve: src/solver/LinearSolvers.cpp:422; here it is rebuilt only when
+ * A's values changed). */
+MI_API int mi_solver_setup(mi_solver* s, mi_csr* A);
+/* Host b, x (x is the initial guess only when warm_start). Returns MI_OK,
+ * MI_ERR_NOT_CONVERGED or MI_ERR_BREAKDOWN; x and stats are written in all
+ * three cases. */
+MI_API int mi_solver_solve(mi_solver* s, const double* b_host, double* x_host, mi_stats* stats);
+MI_API int mi_solver_solve_device(mi_solver* s, const double* b_device, double* x_device,
+                                  mi_stats* stats);
+/* ILinearSolver::last_residual (src/solver/Solver.hpp:99). */
+MI_API double mi_solver_last_residual(const mi_solver* s);
+/* Relative residual after each Krylov iteration of the last solve. */
+MI_API int mi_solver_history(const mi_solver* s, double* out, int capacity, int* count);
+MI_API int mi_solver_destroy(mi_solver* s);
+
+/* ------------------------------------------------------ mesh and assembly
+ * The SoA mesh of meshing::Mesh (src/meshing/MeshStorage.hpp:37-74): uint64
+ * owner/neighbour with UINT64_MAX (kBoundaryCell) on boundary faces, FP64
+ * area vectors and centroids, uint32 boundary zones, cell->face CSR with
+ * int32 offsets and uint64 face ids.  Host arrays, copied at creation. */
+typedef struct {
+    int64_t n_cells;
+    int64_t n_faces;
+    const uint64_t* owner;
+    const uint64_t* neighbor;
+    const double* areaX;
+    const double* areaY;
+    const double* areaZ;
+    const double* faceCx;
+    const double* faceCy;
+    const double* faceCz;
+    const uint32_t* boundaryZone;
+    const double* cellCx;
+    const double* cellCy;
+    const double* cellCz;
+    const double* volume;
+    const int32_t* cellFaceOffsets;
+    const uint64_t* cellFaceIndices;
+} mi_mesh_soa;
+
+/* Boundary treatment of a zone, grouping BCType (src/solver/Solver.hpp:105-123)
+ * by what the SIMPLE face loops do with it:
+ *   WALL            Wall, NoSlipWall: U = 0; zero-gradient p'
+ *   PRESSURE        PressureInlet/Outlet: Dirichlet p' = 0; outflow for U
+ *   SYMMETRY        Symmetry: no contribution
+ *   VELOCITY_INLET  VelocityInlet, MovingWall: U = zone value
+ *   OUTFLOW         every other type and zones without a BoundarySpec */
+enum { MI_BC_WALL = 0, MI_BC_PRESSURE = 1, MI_BC_SYMMETRY = 2, MI_BC_VELOCITY_INLET = 3,
+       MI_BC_OUTFLOW = 4 };
+
+typedef struct mi_mesh mi_mesh;
+/* Uploads the mesh and builds the CSR pattern of SimpleAlgorithm::build_sparsity
+ * (src/solver/SimpleAlgorithm.cpp:49-98; periodic twins not supported).
+ * bc_of_zone[z] is the MI_BC_* class of zone z (zones >= n_zones: OUTFLOW). */
+MI_API int mi_mesh_create(const mi_mesh_soa* mesh, const int32_t* bc_of_zone, int n_zones,
+                          mi_mesh** out);
+MI_API int mi_mesh_info(const mi_mesh* m, int64_t* n_cells, int64_t* n_faces, int64_t* nnz);
+/* A zero-valued matrix on the mesh's pattern (A_p_ / A_mom_ storage). */
+MI_API int mi_mesh_create_matrix(mi_mesh* m, mi_csr** out);
+/* SimpleAlgorithm::assemble_pressure_correction (src/solver/SimpleAlgorithm.cpp:358-447),
+ * SIMPLE denominators.  Device arrays: aP[n_cells] (nullable: aP_f = 1),
+ * gamma[n_faces] (nullable: 1) scaling each face coefficient
+ * rho*gamma_f*|S|^2/max(aP_f*|d.S|,1e-30), phi[n_faces] (nullable: rhs 0).
+ * Writes A's values and rhs[n_cells]; with no pressure-zone face the system
+ * is pinned (A_00 *= 1e6, rhs_0 = 0) and *pinned = 1. */
+MI_API int mi_assemble_pressure(mi_mesh* m, double rho, const double* aP_device,
+                                const double* gamma_device, const double* phi_device, mi_csr* A,
+                                double* rhs_device, int* pinned);
+/* SimpleAlgorithm::assemble_momentum (src/solver/SimpleAlgorithm.cpp:240-355),
+ * first-order upwind + diffusion for velocity component comp.  Device arrays:
+ * phi[n_faces]; pgradV[n_cells] (nullable) = grad(p)_comp * V subtracted from
+ * the rhs; dt[n_cells] (nullable: steady) adds rho*V/dt implicit Euler with
+ * Uold; Ucomp the current component for under-relaxation urf.  Host
+ * bc_value[n_zones*3] holds inlet velocities. */
+MI_API int mi_assemble_momentum(mi_mesh* m, int comp, double mu, double rho,
+                                const double* phi_device, const double* pgradV_device,
+                                const double* dt_device, const double* Uold_device,
+                                const double* Ucomp_device, double urf,
+                                const double* bc_value_host, mi_csr* A, double* rhs_device);
+/* SimpleAlgorithm::compute_face_fluxes (src/solver/SimpleAlgorithm.cpp:201-237). */
+MI_API int mi_face_flux(mi_mesh* m, double rho, const double* Ux_device, const double* Uy_device,
+                        const double* Uz_device, const double* bc_value_host, double* phi_device);
+MI_API int mi_mesh_destroy(mi_mesh* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIMALL_AMD_H */

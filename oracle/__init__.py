@@ -1,0 +1,299 @@
+"""CPU oracle of the SimAll pressure-correction solve path (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker: the product (libsimall_amd.so) never
+calls it.  ``liboracle.so`` is built from oracle.c by oracle/Makefile.
+
+Each function restates one reference routine (file:line in oracle.h/oracle.c).
+How each is pinned (tests/test_oracle.py):
+
+  spmv               reference KAT y=[2,4,6,13] (tests/unit/gpu/test_gpu_models.cpp:144-178),
+                     dense brute force, A.1 = 0 on Neumann rows
+  dot/norm2/axpy     KAT dot(1..5,1..5)=55, nrm2=sqrt(55) (test_gpu_models.cpp:117-141),
+                     compensated sums
+  jacobi             closed form 1/a_ii, missing/zero diagonal -> 1
+  cg                 KAT tridiag(-1,4,-1) n=10 (<50 its, ||r||<1e-8); exact answer vs
+                     numpy.linalg.solve; finite termination in <= n steps; Thomas algorithm
+  bicgstab + ilu0    KAT n=8 nonsymmetric (||r||<1e-7, test_gpu_models.cpp:238-287); ILU(0) of a
+                     tridiagonal is its exact LU (scipy.linalg.lu); (LU)_ij = a_ij on the pattern
+  ssor               identity -> r (tests/unit/solver/test_ssor_preconditioner.cpp:44-63);
+                     CG+SSOR 1-D Laplacian n=50 (:65-88)
+  gmres              exact solution vs numpy on random nonsymmetric systems
+  gmres_transpose    KAT psi=(0.5,1/6) (tests/unit/longtail/test_longtail_modules.cpp:32-54)
+  build_sparsity     brute-force neighbour sets from the face lists
+  assemble_pressure  uniform grid, unit gamma: diagonal = #interior neighbours, off-diag -1,
+                     zero row sums before the pin, symmetry; coefficient formula restated
+                     independently in numpy
+  assemble_momentum  zero-flux case equals pressure-style diffusion; upwind row sums equal net
+                     outflow; M-matrix sign pattern
+  face_flux          closed form for a uniform velocity field (phi = U.S)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+PC_JACOBI, PC_ILU, PC_SSOR, PC_AMG, PC_NONE = 0, 1, 2, 3, 4
+
+
+class OrcCsr(C.Structure):
+    _fields_ = [("n", C.c_int), ("rowPtr", C.c_void_p), ("colIdx", C.c_void_p), ("values", C.c_void_p)]
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("n_faces", C.c_int64)] + [
+        (k, C.c_void_p)
+        for k in ("owner", "neighbor", "areaX", "areaY", "areaZ", "faceCx", "faceCy", "faceCz",
+                  "boundaryZone", "cellCx", "cellCy", "cellCz", "volume", "cellFaceOffsets",
+                  "cellFaceIndices")
+    ]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        lib = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        sig = {
+            "orc_spmv": (None, [P(OrcCsr), vp, vp]),
+            "orc_spmv_transpose": (None, [P(OrcCsr), vp, vp]),
+            "orc_dot": (d, [vp, vp, i64]),
+            "orc_norm2": (d, [vp, i64]),
+            "orc_axpy": (None, [d, vp, vp, i64]),
+            "orc_scal": (None, [d, vp, i64]),
+            "orc_jacobi_setup": (None, [P(OrcCsr), vp]),
+            "orc_ilu0_setup": (None, [P(OrcCsr), vp, vp]),
+            "orc_ilu0_apply": (None, [P(OrcCsr), vp, vp, vp, vp]),
+            "orc_ssor_apply": (None, [P(OrcCsr), vp, d, vp, vp]),
+            "orc_cg_solve": (i, [P(OrcCsr), i, d, i, vp, vp, P(d), vp, P(i)]),
+            "orc_bicgstab_solve": (i, [P(OrcCsr), i, d, i, vp, vp, P(d), vp, P(i)]),
+            "orc_gmres_solve": (i, [P(OrcCsr), i, d, i, i, vp, vp, P(d)]),
+            "orc_gmres_transpose": (i, [P(OrcCsr), vp, vp, i, i, d, P(i), P(d)]),
+            "orc_build_sparsity": (i64, [P(OrcMesh), vp, vp]),
+            "orc_assemble_pressure": (i, [P(OrcMesh), vp, i, d, vp, vp, vp, P(OrcCsr), vp, vp]),
+            "orc_assemble_momentum": (None, [P(OrcMesh), vp, i, vp, i, d, d, vp, vp, vp, vp, d,
+                                             P(OrcCsr), vp, vp]),
+            "orc_face_flux": (None, [P(OrcMesh), vp, i, vp, d, vp, vp, vp, vp]),
+            "orc_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
+            "orc_uniform": (d, [P(C.c_uint64)]),
+            "orc_normal": (d, [P(C.c_uint64)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+class Matrix:
+    """CSR triple kept alive for the C structs."""
+
+    def __init__(self, rowPtr, colIdx, values):
+        self.rowPtr = np.ascontiguousarray(rowPtr, dtype=np.int32)
+        self.colIdx = np.ascontiguousarray(colIdx, dtype=np.int32)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        self.n = self.rowPtr.size - 1
+        self.s = OrcCsr(self.n, _p(self.rowPtr), _p(self.colIdx), _p(self.values))
+
+    @classmethod
+    def from_dense(cls, A):
+        A = np.asarray(A, dtype=np.float64)
+        rp, ci, va = [0], [], []
+        for i in range(A.shape[0]):
+            for j in range(A.shape[1]):
+                if A[i, j] != 0.0 or i == j:
+                    ci.append(j)
+                    va.append(A[i, j])
+            rp.append(len(ci))
+        return cls(rp, ci, va)
+
+    def dense(self):
+        D = np.zeros((self.n, self.n))
+        for i in range(self.n):
+            for k in range(self.rowPtr[i], self.rowPtr[i + 1]):
+                D[i, self.colIdx[k]] += self.values[k]
+        return D
+
+
+def _vec(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def spmv(A, x):
+    x = _vec(x)
+    y = np.empty(A.n)
+    load().orc_spmv(C.byref(A.s), _p(x), _p(y))
+    return y
+
+
+def spmv_transpose(A, x):
+    x = _vec(x)
+    y = np.empty(A.n)
+    load().orc_spmv_transpose(C.byref(A.s), _p(x), _p(y))
+    return y
+
+
+def dot(a, b):
+    a, b = _vec(a), _vec(b)
+    return load().orc_dot(_p(a), _p(b), a.size)
+
+
+def norm2(a):
+    a = _vec(a)
+    return load().orc_norm2(_p(a), a.size)
+
+
+def axpy(alpha, x, y):
+    x = _vec(x)
+    y = np.array(y, dtype=np.float64, copy=True)
+    load().orc_axpy(alpha, _p(x), _p(y), x.size)
+    return y
+
+
+def scal(a, x):
+    x = np.array(x, dtype=np.float64, copy=True)
+    load().orc_scal(a, _p(x), x.size)
+    return x
+
+
+def jacobi_inv_diag(A):
+    out = np.empty(A.n)
+    load().orc_jacobi_setup(C.byref(A.s), _p(out))
+    return out
+
+
+def ilu0(A):
+    lu = np.empty(A.values.size)
+    dp = np.empty(A.n, np.int32)
+    load().orc_ilu0_setup(C.byref(A.s), _p(lu), _p(dp))
+    return lu, dp
+
+
+def ilu0_apply(A, lu, dp, r):
+    r = _vec(r)
+    z = np.empty(A.n)
+    load().orc_ilu0_apply(C.byref(A.s), _p(lu), _p(dp), _p(r), _p(z))
+    return z
+
+
+def ssor_apply(A, r, omega=1.0):
+    dp = np.full(A.n, -1, np.int32)
+    for i in range(A.n):
+        for k in range(A.rowPtr[i], A.rowPtr[i + 1]):
+            if A.colIdx[k] == i:
+                dp[i] = k
+                break
+    r = _vec(r)
+    z = np.empty(A.n)
+    load().orc_ssor_apply(C.byref(A.s), _p(dp), omega, _p(r), _p(z))
+    return z
+
+
+def _krylov(fn, A, b, pc, tol, max_iter):
+    b = _vec(b)
+    x = np.zeros(A.n)
+    last = C.c_double()
+    hist = np.zeros(max(max_iter, 1) + 1)
+    nh = C.c_int()
+    it = fn(C.byref(A.s), pc, tol, max_iter, _p(b), _p(x), C.byref(last), _p(hist), C.byref(nh))
+    if it < 0:
+        raise NotImplementedError("preconditioner not restated in the oracle")
+    return x, it, last.value, hist[: nh.value].copy()
+
+
+def cg(A, b, pc=PC_JACOBI, tol=1e-8, max_iter=1000):
+    """CGSolver::solve -> (x, iterations, last_residual, history)."""
+    return _krylov(load().orc_cg_solve, A, b, pc, tol, max_iter)
+
+
+def bicgstab(A, b, pc=PC_ILU, tol=1e-8, max_iter=1000):
+    """BiCGSTABSolver::solve -> (x, iterations, last_residual, history)."""
+    return _krylov(load().orc_bicgstab_solve, A, b, pc, tol, max_iter)
+
+
+def gmres(A, b, pc=PC_ILU, tol=1e-8, max_iter=1000, restart=30):
+    b = _vec(b)
+    x = np.zeros(A.n)
+    last = C.c_double()
+    it = load().orc_gmres_solve(C.byref(A.s), pc, tol, max_iter, restart, _p(b), _p(x), C.byref(last))
+    return x, it, last.value
+
+
+def gmres_transpose(A, b, max_iter=500, restart=30, tol=1e-10):
+    """gmres_transpose (DiscreteAdjoint.cpp:54-146) -> (x, ok, iterations, residual)."""
+    b = _vec(b)
+    x = np.zeros(A.n)
+    its = C.c_int()
+    res = C.c_double()
+    ok = load().orc_gmres_transpose(C.byref(A.s), _p(b), _p(x), max_iter, restart, tol, C.byref(its), C.byref(res))
+    return x, bool(ok), its.value, res.value
+
+
+# ------------------------------------------------------------- assembly
+def mesh_struct(mesh):
+    """OrcMesh over a simall_amd.workload.Mesh (arrays must stay alive)."""
+    s = OrcMesh()
+    s.n_cells, s.n_faces = mesh.n_cells, mesh.n_faces
+    for k in ("owner", "neighbor", "areaX", "areaY", "areaZ", "faceCx", "faceCy", "faceCz",
+              "boundaryZone", "cellCx", "cellCy", "cellCz", "volume", "cellFaceOffsets",
+              "cellFaceIndices"):
+        setattr(s, k, getattr(mesh, k).ctypes.data)
+    return s
+
+
+def build_sparsity(mesh):
+    ms = mesh_struct(mesh)
+    rp = np.empty(mesh.n_cells + 1, np.int32)
+    nnz = load().orc_build_sparsity(C.byref(ms), _p(rp), None)
+    ci = np.empty(nnz, np.int32)
+    load().orc_build_sparsity(C.byref(ms), _p(rp), _p(ci))
+    return rp, ci
+
+
+def assemble_pressure(mesh, pattern, bc_of_zone, rho=1.0, aP=None, gamma=None, phi=None):
+    """-> (values, rhs, pinned)."""
+    ms = mesh_struct(mesh)
+    bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    vals = np.empty(pattern.colIdx.size)
+    rhs = np.empty(mesh.n_cells)
+    pinned = load().orc_assemble_pressure(C.byref(ms), _p(bc), bc.size, rho, _p(aP), _p(gamma), _p(phi),
+                                          C.byref(pattern.s), _p(vals), _p(rhs))
+    return vals, rhs, bool(pinned)
+
+
+def assemble_momentum(mesh, pattern, bc_of_zone, comp, mu, rho, phi, dt=None, Uold=None,
+                      Ucomp=None, urf=1.0, bc_value=None):
+    ms = mesh_struct(mesh)
+    bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    vals = np.empty(pattern.colIdx.size)
+    rhs = np.empty(mesh.n_cells)
+    bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+    load().orc_assemble_momentum(C.byref(ms), _p(bc), bc.size, _p(bv), comp, mu, rho, _p(_vec(phi)),
+                                 _p(dt), _p(Uold), _p(Ucomp), urf, C.byref(pattern.s), _p(vals), _p(rhs))
+    return vals, rhs
+
+
+def face_flux(mesh, bc_of_zone, rho, U, bc_value=None):
+    ms = mesh_struct(mesh)
+    bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    phi = np.empty(mesh.n_faces)
+    bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+    Ux, Uy, Uz = (_vec(u) for u in U)
+    load().orc_face_flux(C.byref(ms), _p(bc), bc.size, _p(bv), rho, _p(Ux), _p(Uy), _p(Uz), _p(phi))
+    return phi

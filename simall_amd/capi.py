@@ -1,0 +1,414 @@
+"""ctypes binding of the C ABI declared in include/simall_amd.h.
+
+The library is the in-tree ``simall_amd/libsimall_amd.so`` built by
+``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsimall_amd.so")
+
+MI_OK = 0
+MI_ERR_NOT_CONVERGED = 1
+MI_ERR_BREAKDOWN = 2
+MI_ERR_BAD_ARG = 3
+MI_ERR_DEVICE = 4
+MI_ERR_NO_DEVICE = 5
+MI_ERR_UNSUPPORTED = 6
+MI_ERR_ZERO_PIVOT = 7
+
+MI_SOLVER_GMRES, MI_SOLVER_BICGSTAB, MI_SOLVER_CG, MI_SOLVER_TFQMR = 0, 1, 2, 3
+MI_PC_JACOBI, MI_PC_ILU, MI_PC_SSOR, MI_PC_AMG, MI_PC_NONE = 0, 1, 2, 3, 4
+MI_PRECISION_FP64, MI_PRECISION_MIXED = 0, 1
+
+
+class SolverCfg(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("preconditioner", C.c_int),
+        ("tolerance", C.c_double),
+        ("max_iterations", C.c_int),
+        ("restart", C.c_int),
+        ("precision", C.c_int),
+        ("warm_start", C.c_int),
+        ("true_residual", C.c_int),
+        ("inner_tolerance", C.c_double),
+        ("max_outer", C.c_int),
+        ("reorder", C.c_int),
+        ("check_interval", C.c_int),
+        ("amg_max_levels", C.c_int),
+        ("amg_coarse_size", C.c_int),
+        ("amg_strength", C.c_double),
+        ("amg_jacobi_omega", C.c_double),
+        ("amg_smoother", C.c_int),
+        ("amg_sweeps", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("outer_iterations", C.c_int),
+        ("converged", C.c_int),
+        ("status", C.c_int),
+        ("initial_residual", C.c_double),
+        ("final_residual", C.c_double),
+        ("relative_residual", C.c_double),
+        ("true_relative_residual", C.c_double),
+        ("setup_seconds", C.c_double),
+        ("solve_seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_i64 = C.c_int64
+_sz = C.c_size_t
+_d = C.c_double
+_pd = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); the list of every exported entry point.
+PROTOTYPES = {
+    "mi_get_abi_version": (_i, []),
+    "mi_init": (_i, [_i]),
+    "mi_device_count": (_i, [C.POINTER(_i)]),
+    "mi_last_error": (C.c_char_p, []),
+    "mi_device_synchronize": (_i, []),
+    "mi_device_alloc": (_i, [_sz, C.POINTER(_vp)]),
+    "mi_device_free": (_i, [_vp]),
+    "mi_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "mi_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "mi_memcpy_d2d": (_i, [_vp, _vp, _sz]),
+    "mi_memset_device": (_i, [_vp, _i, _sz]),
+    "mi_axpy": (_i, [_d, _vp, _vp, _sz, _vp]),
+    "mi_scal": (_i, [_d, _vp, _sz, _vp]),
+    "mi_copy": (_i, [_vp, _vp, _sz, _vp]),
+    "mi_dot": (_i, [_vp, _vp, _sz, _pd, _vp]),
+    "mi_nrm2": (_i, [_vp, _sz, _pd, _vp]),
+    "mi_axpby": (_i, [_d, _vp, _d, _vp, _sz, _vp]),
+    "mi_xpby": (_i, [_vp, _d, _vp, _sz, _vp]),
+    "mi_spmv_csr": (_i, [_sz, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mi_jacobi_apply": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "mi_csr_create": (_i, [_i64, _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "mi_csr_update_values": (_i, [_vp, _vp]),
+    "mi_csr_update_values_device": (_i, [_vp, _vp]),
+    "mi_csr_get_values": (_i, [_vp, _vp]),
+    "mi_csr_get_pattern": (_i, [_vp, _vp, _vp]),
+    "mi_csr_info": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
+    "mi_csr_values_device": (_i, [_vp, C.POINTER(_vp)]),
+    "mi_csr_spmv": (_i, [_vp, _vp, _vp]),
+    "mi_csr_spmv_device": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "mi_csr_time_spmv": (_i, [_vp, _i, _vp, _vp, _i, _pd]),
+    "mi_csr_destroy": (_i, [_vp]),
+    "mi_solver_cfg_default": (None, [C.POINTER(SolverCfg)]),
+    "mi_solver_create": (_i, [C.POINTER(SolverCfg), C.POINTER(_vp)]),
+    "mi_solver_setup": (_i, [_vp, _vp]),
+    "mi_solver_solve": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
+    "mi_solver_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
+    "mi_solver_last_residual": (_d, [_vp]),
+    "mi_solver_history": (_i, [_vp, _pd, _i, C.POINTER(_i)]),
+    "mi_solver_destroy": (_i, [_vp]),
+    "mi_mesh_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
+    "mi_mesh_info": (_i, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "mi_mesh_create_matrix": (_i, [_vp, C.POINTER(_vp)]),
+    "mi_assemble_pressure": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i)]),
+    "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
+    "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "mi_mesh_destroy": (_i, [_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libsimall_amd.so and binds every prototype (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+        )
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class MiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mi status {code}: {msg}")
+        self.code = code
+
+
+def check(rc, allowed=(MI_OK,)):
+    if rc not in allowed:
+        raise MiError(rc, load().mi_last_error().decode(errors="replace"))
+    return rc
+
+
+def ptr(a):
+    """Host pointer of a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data
+
+
+# --------------------------------------------------------------- device arrays
+class DeviceArray:
+    """A device allocation owned by Python (freed on close / garbage collection)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.lib = load()
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        check(self.lib.mi_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(self.lib.mi_memcpy_h2d(self.ptr, ptr(a), self.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        check(self.lib.mi_memcpy_d2h(ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(self.lib.mi_memset_device(self.ptr, 0, self.nbytes))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.mi_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ handles
+class Csr:
+    """mi_csr handle over host CSR arrays (int32 rowPtr/colIdx, float64 values)."""
+
+    def __init__(self, rowPtr, colIdx, values):
+        self.lib = load()
+        self.rowPtr = np.ascontiguousarray(rowPtr, dtype=np.int32)
+        self.colIdx = np.ascontiguousarray(colIdx, dtype=np.int32)
+        vals = np.ascontiguousarray(values, dtype=np.float64)
+        self.n = self.rowPtr.size - 1
+        self.nnz = self.colIdx.size
+        h = C.c_void_p()
+        check(self.lib.mi_csr_create(self.n, self.nnz, ptr(self.rowPtr), ptr(self.colIdx), ptr(vals), C.byref(h)))
+        self.h = h.value
+
+    def update_values(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(self.lib.mi_csr_update_values(self.h, ptr(v)))
+
+    def values(self):
+        out = np.empty(self.nnz, np.float64)
+        check(self.lib.mi_csr_get_values(self.h, ptr(out)))
+        return out
+
+    def spmv(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty(self.n, np.float64)
+        check(self.lib.mi_csr_spmv(self.h, ptr(x), ptr(y)))
+        return y
+
+    def spmv_device(self, dx, dy, precision=MI_PRECISION_FP64):
+        check(self.lib.mi_csr_spmv_device(self.h, precision, dx.ptr, dy.ptr, None))
+
+    def time_spmv(self, dx, dy, precision=MI_PRECISION_FP64, reps=20):
+        us = C.c_double()
+        check(self.lib.mi_csr_time_spmv(self.h, precision, dx.ptr, dy.ptr, reps, C.byref(us)))
+        return us.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_cfg(**kw):
+    cfg = SolverCfg()
+    load().mi_solver_cfg_default(C.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise AttributeError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Solver:
+    """mi_solver handle (ILinearSolver semantics: solve returns iterations)."""
+
+    def __init__(self, cfg=None, **kw):
+        self.lib = load()
+        self.cfg = cfg if cfg is not None else default_cfg(**kw)
+        h = C.c_void_p()
+        check(self.lib.mi_solver_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h.value
+        self.A = None
+
+    def setup(self, A):
+        self.A = A
+        check(self.lib.mi_solver_setup(self.h, A.h))
+
+    def solve(self, b, x=None):
+        """Returns (x, stats, status)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros_like(b) if x is None else np.array(x, dtype=np.float64, copy=True)
+        st = Stats()
+        rc = self.lib.mi_solver_solve(self.h, ptr(b), ptr(x), C.byref(st))
+        check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED, MI_ERR_BREAKDOWN))
+        return x, st, rc
+
+    def solve_device(self, db, dx):
+        st = Stats()
+        rc = self.lib.mi_solver_solve_device(self.h, db.ptr, dx.ptr, C.byref(st))
+        check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED, MI_ERR_BREAKDOWN))
+        return st, rc
+
+    def last_residual(self):
+        return self.lib.mi_solver_last_residual(self.h)
+
+    def history(self):
+        cnt = C.c_int()
+        check(self.lib.mi_solver_history(self.h, None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value, np.float64)
+        if cnt.value:
+            check(self.lib.mi_solver_history(self.h, out.ctypes.data_as(_pd), cnt.value, C.byref(cnt)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MeshSoa(C.Structure):
+    _fields_ = [("n_cells", C.c_int64), ("n_faces", C.c_int64)] + [
+        (k, C.c_void_p)
+        for k in ("owner", "neighbor", "areaX", "areaY", "areaZ", "faceCx", "faceCy", "faceCz",
+                  "boundaryZone", "cellCx", "cellCy", "cellCz", "volume", "cellFaceOffsets",
+                  "cellFaceIndices")
+    ]
+
+
+class CsrView:
+    """An mi_csr created by the library (e.g. on a mesh pattern)."""
+
+    def __init__(self, handle):
+        self.lib = load()
+        self.h = handle
+        n, nnz = C.c_int64(), C.c_int64()
+        check(self.lib.mi_csr_info(self.h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    values = Csr.values
+    update_values = Csr.update_values
+    spmv = Csr.spmv
+    spmv_device = Csr.spmv_device
+    time_spmv = Csr.time_spmv
+    close = Csr.close
+    __del__ = Csr.__del__
+
+    def pattern(self):
+        rp = np.empty(self.n + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        check(self.lib.mi_csr_get_pattern(self.h, ptr(rp), ptr(ci)))
+        return rp, ci
+
+
+def _dptr(d):
+    return None if d is None else d.ptr
+
+
+class DeviceMesh:
+    """mi_mesh handle built from a simall_amd.workload.Mesh (host SoA arrays)."""
+
+    def __init__(self, mesh, bc_of_zone):
+        self.lib = load()
+        s = MeshSoa()
+        s.n_cells, s.n_faces = mesh.n_cells, mesh.n_faces
+        for k, _ in MeshSoa._fields_[2:]:
+            setattr(s, k, getattr(mesh, k).ctypes.data)
+        self.bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+        h = C.c_void_p()
+        check(self.lib.mi_mesh_create(C.byref(s), ptr(self.bc), self.bc.size, C.byref(h)))
+        self.h = h.value
+        nc, nf, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.lib.mi_mesh_info(self.h, C.byref(nc), C.byref(nf), C.byref(nnz)))
+        self.n_cells, self.n_faces, self.nnz = nc.value, nf.value, nnz.value
+
+    def create_matrix(self):
+        h = C.c_void_p()
+        check(self.lib.mi_mesh_create_matrix(self.h, C.byref(h)))
+        return CsrView(h.value)
+
+    def assemble_pressure(self, A, d_rhs, rho=1.0, d_aP=None, d_gamma=None, d_phi=None):
+        pinned = C.c_int()
+        check(self.lib.mi_assemble_pressure(self.h, rho, _dptr(d_aP), _dptr(d_gamma), _dptr(d_phi),
+                                            A.h, d_rhs.ptr, C.byref(pinned)))
+        return bool(pinned.value)
+
+    def assemble_momentum(self, A, d_rhs, comp, mu, rho, d_phi, d_pgradV=None, d_dt=None,
+                          d_Uold=None, d_Ucomp=None, urf=1.0, bc_value=None):
+        bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+        check(self.lib.mi_assemble_momentum(self.h, comp, mu, rho, d_phi.ptr, _dptr(d_pgradV),
+                                            _dptr(d_dt), _dptr(d_Uold), _dptr(d_Ucomp), urf,
+                                            ptr(bv), A.h, d_rhs.ptr))
+
+    def face_flux(self, d_Ux, d_Uy, d_Uz, d_phi, rho=1.0, bc_value=None):
+        bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+        check(self.lib.mi_face_flux(self.h, rho, d_Ux.ptr, d_Uy.ptr, d_Uz.ptr, ptr(bv), d_phi.ptr))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,347 @@
+// csr.hip — mi_csr handles: upload, value updates, SpMV entry points.
+#include "csr.hpp"
+#include "krylov.hpp"
+#include "spmv.hpp"
+
+#include <cstring>
+
+using namespace mi;
+
+namespace mi
+{
+
+__global__ void k_diag_pos(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           int* __restrict__ diag)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    int d = -1;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i) {
+            d = k;
+            break;
+        }
+    diag[i] = d;
+}
+
+template <class T>
+__global__ void k_inv_diag(int n, const int* __restrict__ diag, const double* __restrict__ val,
+                           T* __restrict__ invD)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const int d = diag[i];
+    double v = 1.0;
+    if (d >= 0 && val[d] != 0.0)
+        v = 1.0 / val[d];
+    invD[i] = T(v);
+}
+
+__global__ void k_convert_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        dst[i] = float(src[i]);
+}
+
+int make_pattern(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
+                 std::shared_ptr<Pattern>& out, hipStream_t s)
+{
+    if (n < 0 || nnz < 0 || n > INT32_MAX - 1 || nnz > INT32_MAX)
+        return set_error(MI_ERR_BAD_ARG, "sizes out of int32 range (n=%lld nnz=%lld)",
+                         (long long)n, (long long)nnz);
+    if ((n > 0 && !rp) || (nnz > 0 && !ci))
+        return set_error(MI_ERR_BAD_ARG, "null CSR array");
+    if (n > 0 && (rp[0] != 0 || rp[n] != nnz))
+        return set_error(MI_ERR_BAD_ARG, "rowPtr[0]=%d rowPtr[n]=%d, expected 0 and nnz=%lld",
+                         rp[0], rp[n], (long long)nnz);
+    for (int64_t i = 0; i < n; ++i) {
+        if (rp[i + 1] < rp[i])
+            return set_error(MI_ERR_BAD_ARG, "rowPtr decreases at row %lld", (long long)i);
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] < 0 || ci[k] >= n)
+                return set_error(MI_ERR_BAD_ARG, "column %d out of range in row %lld", ci[k],
+                                 (long long)i);
+    }
+    auto p = std::make_shared<Pattern>();
+    p->n = n;
+    p->nnz = nnz;
+    if (n > 0) {
+        p->h_rp.assign(rp, rp + n + 1);
+        p->h_ci.assign(ci, ci + nnz);
+        MI_TRY(p->rp.upload(p->h_rp.data(), size_t(n + 1), s));
+        MI_TRY(p->ci.upload(p->h_ci.data(), size_t(nnz), s));
+        MI_TRY(p->diag.alloc(size_t(n)));
+        k_diag_pos<<<grid_for(n, kBlock), kBlock, 0, s>>>(int(n), p->rp.p, p->ci.p, p->diag.p);
+        MI_HIP(hipGetLastError());
+    } else {
+        p->h_rp.assign(1, 0);
+    }
+    p->lanes = lanes_for(n, nnz);
+    out = std::move(p);
+    return MI_OK;
+}
+
+int make_csr_on_pattern(const std::shared_ptr<Pattern>& pat, mi_csr** out)
+{
+    auto* A = new (std::nothrow) mi_csr();
+    if (!A)
+        return set_error(MI_ERR_DEVICE, "out of host memory");
+    A->pat = pat;
+    int rc = A->val.alloc(size_t(pat->nnz));
+    if (rc == MI_OK && pat->nnz)
+        rc = (hipMemset(A->val.p, 0, size_t(pat->nnz) * sizeof(double)) == hipSuccess)
+                 ? MI_OK
+                 : set_error(MI_ERR_DEVICE, "hipMemset failed");
+    if (rc != MI_OK) {
+        delete A;
+        return rc;
+    }
+    A->h_val.assign(size_t(pat->nnz), 0.0);
+    A->host_valid = true;
+    *out = A;
+    return MI_OK;
+}
+
+int launch_inv_diag(const Pattern& p, const double* val, double* invD, hipStream_t s)
+{
+    if (p.n == 0)
+        return MI_OK;
+    k_inv_diag<double><<<grid_for(p.n, kBlock), kBlock, 0, s>>>(int(p.n), p.diag.p, val, invD);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+int launch_inv_diag32(const Pattern& p, const double* val, float* invD, hipStream_t s)
+{
+    if (p.n == 0)
+        return MI_OK;
+    k_inv_diag<float><<<grid_for(p.n, kBlock), kBlock, 0, s>>>(int(p.n), p.diag.p, val, invD);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s)
+{
+    if (n <= 0)
+        return MI_OK;
+    k_convert_f32<<<reduce_grid(n, 4), kBlock, 0, s>>>(n, src, dst);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+} // namespace mi
+
+int mi_csr::sync_host()
+{
+    if (host_valid)
+        return MI_OK;
+    h_val.resize(size_t(nnz()));
+    if (nnz()) {
+        hipStream_t s = default_stream();
+        MI_HIP(hipMemcpyAsync(h_val.data(), val.p, size_t(nnz()) * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+    }
+    host_valid = true;
+    return MI_OK;
+}
+
+int mi_csr::ensure_fp32(hipStream_t s)
+{
+    if (val32_valid)
+        return MI_OK;
+    MI_TRY(val32.alloc(size_t(nnz())));
+    MI_TRY(launch_convert_f32(nnz(), val.p, val32.p, s));
+    val32_valid = true;
+    return MI_OK;
+}
+
+extern "C" {
+
+MI_API int mi_csr_create(int64_t n, int64_t nnz, const int32_t* rowPtr, const int32_t* colIdx,
+                         const double* values, mi_csr** out)
+{
+    clear_error();
+    if (!out)
+        return set_error(MI_ERR_BAD_ARG, "null output handle");
+    *out = nullptr;
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    std::shared_ptr<Pattern> pat;
+    MI_TRY(make_pattern(n, nnz, rowPtr, colIdx, pat, s));
+    mi_csr* A = nullptr;
+    MI_TRY(make_csr_on_pattern(pat, &A));
+    if (nnz > 0 && values) {
+        A->h_val.assign(values, values + nnz);
+        int rc = A->val.upload(A->h_val.data(), size_t(nnz), s);
+        if (rc != MI_OK) {
+            delete A;
+            return rc;
+        }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        delete A;
+        return set_error(MI_ERR_DEVICE, "upload failed");
+    }
+    *out = A;
+    return MI_OK;
+}
+
+MI_API int mi_csr_update_values(mi_csr* A, const double* values)
+{
+    if (!A || (A->nnz() && !values))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    A->h_val.assign(values, values + A->nnz());
+    MI_TRY(A->val.upload(A->h_val.data(), size_t(A->nnz()), s));
+    MI_HIP(hipStreamSynchronize(s));
+    A->host_valid = true;
+    A->val32_valid = false;
+    ++A->version;
+    return MI_OK;
+}
+
+MI_API int mi_csr_update_values_device(mi_csr* A, const double* values_device)
+{
+    if (!A || (A->nnz() && !values_device))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    if (A->nnz() && values_device != A->val.p)
+        MI_HIP(hipMemcpyAsync(A->val.p, values_device, size_t(A->nnz()) * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+    MI_HIP(hipStreamSynchronize(s));
+    A->host_valid = false;
+    A->val32_valid = false;
+    ++A->version;
+    return MI_OK;
+}
+
+MI_API int mi_csr_get_values(const mi_csr* A, double* values_host)
+{
+    if (!A || (A->nnz() && !values_host))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    MI_TRY(const_cast<mi_csr*>(A)->sync_host());
+    std::memcpy(values_host, A->h_val.data(), size_t(A->nnz()) * sizeof(double));
+    return MI_OK;
+}
+
+MI_API int mi_csr_get_pattern(const mi_csr* A, int32_t* rowPtr_host, int32_t* colIdx_host)
+{
+    if (!A)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    if (rowPtr_host)
+        std::memcpy(rowPtr_host, A->pat->h_rp.data(), size_t(A->n() + 1) * sizeof(int32_t));
+    if (colIdx_host && A->nnz())
+        std::memcpy(colIdx_host, A->pat->h_ci.data(), size_t(A->nnz()) * sizeof(int32_t));
+    return MI_OK;
+}
+
+MI_API int mi_csr_info(const mi_csr* A, int64_t* n, int64_t* nnz)
+{
+    if (!A)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    if (n)
+        *n = A->n();
+    if (nnz)
+        *nnz = A->nnz();
+    return MI_OK;
+}
+
+MI_API int mi_csr_values_device(mi_csr* A, double** values_device)
+{
+    if (!A || !values_device)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    *values_device = A->val.p;
+    // The caller may write through this pointer: host and FP32 copies go stale.
+    A->host_valid = false;
+    A->val32_valid = false;
+    ++A->version;
+    return MI_OK;
+}
+
+MI_API int mi_csr_spmv(mi_csr* A, const double* x, double* y)
+{
+    if (!A || (A->n() && (!x || !y)))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    const int64_t n = A->n();
+    if (n == 0)
+        return MI_OK;
+    hipStream_t s = default_stream();
+    DevBuf<double> dx, dy;
+    MI_TRY(dx.upload(x, size_t(n), s));
+    MI_TRY(dy.alloc(size_t(n)));
+    MI_TRY(launch_spmv_f64(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val.p, dx.p, dy.p, s,
+                           A->pat->lanes));
+    MI_HIP(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+static int spmv_device_impl(mi_csr* A, int precision, const void* x, void* y, hipStream_t s)
+{
+    const int64_t n = A->n();
+    if (n == 0)
+        return MI_OK;
+    if (precision == MI_PRECISION_FP64)
+        return launch_spmv_f64(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val.p,
+                               static_cast<const double*>(x), static_cast<double*>(y), s,
+                               A->pat->lanes);
+    if (precision == MI_PRECISION_MIXED) {
+        MI_TRY(A->ensure_fp32(s));
+        return launch_spmv_f32(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val32.p,
+                               static_cast<const float*>(x), static_cast<float*>(y), s,
+                               A->pat->lanes);
+    }
+    return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);
+}
+
+MI_API int mi_csr_spmv_device(mi_csr* A, int precision, const void* x, void* y, void* stream)
+{
+    if (!A || (A->n() && (!x || !y)))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : default_stream();
+    MI_TRY(spmv_device_impl(A, precision, x, y, s));
+    if (!stream)
+        MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x, void* y, int reps,
+                            double* avg_us)
+{
+    if (!A || !avg_us || reps < 1 || (A->n() && (!x || !y)))
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    MI_TRY(spmv_device_impl(A, precision, x, y, s)); // warm + FP32 copy outside the timing
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    MI_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i)
+        MI_TRY(spmv_device_impl(A, precision, x, y, s));
+    MI_HIP(hipEventRecord(e1, s));
+    MI_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MI_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_us = double(ms) * 1000.0 / reps;
+    return MI_OK;
+}
+
+MI_API int mi_csr_destroy(mi_csr* A)
+{
+    delete A;
+    return MI_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,311 @@
+// krylov.hip — fused preconditioned CG kernels.
+//
+// One CG iteration of CGSolver::solve (src/solver/LinearSolvers.cpp:430-450):
+//   Ap = A p ; pAp = p.Ap ; alpha = rz / pAp           k_cg_spmv_dot
+//   x += alpha p ; r -= alpha Ap ; ||r|| ; stop test     k_cg_update[_jacobi]
+//   z = M r ; rzNew = r.z ; beta = rzNew / rz            (fused for Jacobi)
+//   p = z + beta p                                       k_cg_p[_jacobi]
+// Scalars live in KState on the device; each reduction is finished by the last
+// block of its launch, so the host only polls `done` every few iterations.
+#include "krylov.hpp"
+
+namespace mi
+{
+
+unsigned range_grid(int64_t rows, int min_rows_per_block)
+{
+    int64_t g = (rows + min_rows_per_block - 1) / min_rows_per_block;
+    g = (g + 7) / 8 * 8;
+    if (g < 8)
+        g = 8;
+    if (g > kMaxReduceBlocks)
+        g = kMaxReduceBlocks;
+    return unsigned(g);
+}
+
+// ------------------------------------------------------------------- init
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_init_jacobi(CgArgs<T> a, int warm, double tol,
+                                                           int max_iter)
+{
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[2] = {0.0, 0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T bi = a.b[i];
+        T ri;
+        if (warm) {
+            ri = a.r[i];
+        } else {
+            ri = bi;
+            a.x[i] = T(0);
+            a.r[i] = ri;
+        }
+        const T zi = a.invD[i] * ri;
+        a.p[i] = zi;
+        v[0] += double(ri) * double(zi);
+        v[1] += double(bi) * double(bi);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        st->rz = tot[0];
+        const double bn = sqrt(tot[1]);
+        st->bnorm = bn > 1e-30 ? bn : 1e-30;
+        st->tol = tol;
+        st->max_iter = max_iter;
+        st->iter = 0;
+        st->done = max_iter > 0 ? 0 : 2;
+        st->last = 0.0;
+        st->alpha = st->beta = 0.0;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_init_r(CgArgs<T> a)
+{
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        a.x[i] = T(0);
+        a.r[i] = a.b[i];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_init_p(CgArgs<T> a, double tol, int max_iter)
+{
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[2] = {0.0, 0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T ri = a.r[i], zi = a.z[i], bi = a.b[i];
+        a.p[i] = zi;
+        v[0] += double(ri) * double(zi);
+        v[1] += double(bi) * double(bi);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        st->rz = tot[0];
+        const double bn = sqrt(tot[1]);
+        st->bnorm = bn > 1e-30 ? bn : 1e-30;
+        st->tol = tol;
+        st->max_iter = max_iter;
+        st->iter = 0;
+        st->done = max_iter > 0 ? 0 : 2;
+        st->last = 0.0;
+        st->alpha = st->beta = 0.0;
+    }
+}
+
+// -------------------------------------------------------------- iteration
+template <int LANES, class T>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    constexpr int kRows = kBlock / LANES;
+    const RowRange rr = block_rows(a.n, gridDim.x, kRows);
+    const int sub = threadIdx.x / LANES;
+    const int lane = threadIdx.x & (LANES - 1);
+    double v[1] = {0.0};
+    for (int base = rr.beg; base < rr.end; base += kRows) {
+        const int row = base + sub;
+        const bool mine = row < rr.end;
+        const auto s = csr_row_dot<LANES>(mine ? row : a.n, a.n, lane, a.rp, a.ci, a.val, a.p);
+        if (mine && lane == 0) {
+            const T ap = T(s);
+            a.Ap[row] = ap;
+            v[0] += double(a.p[row]) * double(ap);
+        }
+    }
+    double tot[1];
+    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        const double pAp = tot[0];
+        if (pAp == 0.0) {
+            // src/solver/LinearSolvers.cpp:433-436
+            st->last = sqrt(st->rz) / st->bnorm;
+            st->done = 3;
+        } else {
+            st->alpha = st->rz / pAp;
+        }
+    }
+}
+
+__device__ __forceinline__ void cg_stop_test(KState* st, double rr, double* history)
+{
+    st->last = sqrt(rr) / st->bnorm;
+    const int it = st->iter + 1;
+    st->iter = it;
+    if (history)
+        history[it - 1] = st->last;
+    if (st->last < st->tol)
+        st->done = 1;
+    else if (it >= st->max_iter)
+        st->done = 2;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update_jacobi(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T al = T(a.st->alpha);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[2] = {0.0, 0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T xi = a.x[i] + al * a.p[i];
+        const T ri = a.r[i] - al * a.Ap[i];
+        a.x[i] = xi;
+        a.r[i] = ri;
+        const T zi = a.invD[i] * ri;
+        v[0] += double(ri) * double(ri);
+        v[1] += double(ri) * double(zi);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        cg_stop_test(st, tot[0], a.history);
+        if (!st->done) {
+            st->beta = tot[1] / st->rz;
+            st->rz = tot[1];
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_p_jacobi(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T be = T(a.st->beta);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        a.p[i] = a.invD[i] * a.r[i] + be * a.p[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T al = T(a.st->alpha);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[1] = {0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T xi = a.x[i] + al * a.p[i];
+        const T ri = a.r[i] - al * a.Ap[i];
+        a.x[i] = xi;
+        a.r[i] = ri;
+        v[0] += double(ri) * double(ri);
+    }
+    double tot[1];
+    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0)
+        cg_stop_test(a.st, tot[0], a.history);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_rz(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[1] = {0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        v[0] += double(a.r[i]) * double(a.z[i]);
+    double tot[1];
+    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        st->beta = tot[0] / st->rz;
+        st->rz = tot[0];
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T be = T(a.st->beta);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        a.p[i] = a.z[i] + be * a.p[i];
+}
+
+// -------------------------------------------------------------- launchers
+template <class T> int cg_init_jacobi(const CgArgs<T>& a, bool warm, double tol, int max_iter,
+                                       hipStream_t s)
+{
+    k_cg_init_jacobi<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0, tol, max_iter);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int cg_init_r(const CgArgs<T>& a, bool warm, hipStream_t s)
+{
+    if (!warm) {
+        k_cg_init_r<T><<<a.grid, kBlock, 0, s>>>(a);
+        MI_HIP(hipGetLastError());
+    }
+    return MI_OK;
+}
+
+template <class T> int cg_init_p(const CgArgs<T>& a, double tol, int max_iter, hipStream_t s)
+{
+    k_cg_init_p<T><<<a.grid, kBlock, 0, s>>>(a, tol, max_iter);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> static int launch_spmv_dot(const CgArgs<T>& a, hipStream_t s)
+{
+    switch (a.lanes) {
+    case 2: k_cg_spmv_dot<2, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    case 4: k_cg_spmv_dot<4, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    case 8: k_cg_spmv_dot<8, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    case 16: k_cg_spmv_dot<16, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    case 32: k_cg_spmv_dot<32, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    default: k_cg_spmv_dot<64, T><<<a.grid, kBlock, 0, s>>>(a); break;
+    }
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s)
+{
+    MI_TRY(launch_spmv_dot(a, s));
+    k_cg_update_jacobi<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    k_cg_p_jacobi<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s)
+{
+    MI_TRY(launch_spmv_dot(a, s));
+    k_cg_update<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s)
+{
+    k_cg_rz<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    k_cg_p<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+#define MI_INSTANTIATE_CG(T)                                                                     \
+    template int cg_init_jacobi<T>(const CgArgs<T>&, bool, double, int, hipStream_t);            \
+    template int cg_init_r<T>(const CgArgs<T>&, bool, hipStream_t);                              \
+    template int cg_init_p<T>(const CgArgs<T>&, double, int, hipStream_t);                       \
+    template int cg_iteration_jacobi<T>(const CgArgs<T>&, hipStream_t);                          \
+    template int cg_iteration_part1<T>(const CgArgs<T>&, hipStream_t);                           \
+    template int cg_iteration_part2<T>(const CgArgs<T>&, hipStream_t);
+
+MI_INSTANTIATE_CG(double)
+MI_INSTANTIATE_CG(float)
+
+} // namespace mi

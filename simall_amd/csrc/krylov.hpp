@@ -1,0 +1,111 @@
+// krylov.hpp — device state and launchers of the fused Krylov kernels.
+#pragma once
+
+#include "blas1.hpp"
+#include "spmv.hpp"
+
+namespace mi
+{
+
+// Scalars of one Krylov solve, resident on the device.  Kernels read the
+// step lengths from here and the last block of each reduction writes them, so
+// an iteration needs no host round trip.  `done`: 0 running, 1 converged,
+// 2 iteration limit, 3 breakdown.
+struct KState
+{
+    double rz;     // r.z (CG) / rho (BiCGStab)
+    double alpha;
+    double beta;
+    double omega;  // BiCGStab
+    double bnorm;  // max(||b||, 1e-30)
+    double tol;
+    double last;   // last relative residual
+    double scratch[4];
+    int iter;
+    int max_iter;
+    int done;
+    int pad;
+};
+
+// Row range of logical block `lb` when `G` blocks split `n` rows into
+// contiguous ranges (rounded to `align` rows).
+struct RowRange
+{
+    int beg, end;
+};
+
+#if defined(__HIPCC__)
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip; this maps
+// hardware block b to a logical block so that each XCD walks one contiguous
+// slice of rows and keeps its gathered x in its own L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int G)
+{
+    constexpr int kXcd = 8;
+    if (G % kXcd != 0)
+        return b;
+    return (b % kXcd) * (G / kXcd) + b / kXcd;
+}
+
+__device__ __forceinline__ RowRange block_rows(int n, int G, int align)
+{
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int64_t per = ((int64_t(n) + G - 1) / G + align - 1) / align * align;
+    int64_t beg = int64_t(lb) * per, end = beg + per;
+    if (beg > n)
+        beg = n;
+    if (end > n)
+        end = n;
+    return {int(beg), int(end)};
+}
+#endif
+
+// Grid size for the row-range kernels: a multiple of 8 (one share per XCD),
+// at most kMaxReduceBlocks, at least one block per `min_rows_per_block` rows.
+unsigned range_grid(int64_t rows, int min_rows_per_block);
+
+// Workspace of a CG / BiCGStab solve in element type T.
+template <class T> struct KrylovVectors
+{
+    DevBuf<T> r, z, p, q, s, t, y, rh, v;
+};
+
+// --------------------------------------------------------------- CG launchers
+// Stage tags let one set of kernels serve both precisions: T is the vector
+// and matrix-value type (double, or float for the inner iterations of the
+// mixed solve); every reduction accumulates in double.
+template <class T> struct CgArgs
+{
+    int n;
+    int64_t nnz;
+    const int* rp;
+    const int* ci;
+    const T* val;
+    const T* invD; // Jacobi: inverse diagonal; nullptr for an external preconditioner
+    const T* b;
+    T* x;
+    T* r;
+    T* z;
+    T* p;
+    T* Ap;
+    KState* st;
+    double* partials;
+    unsigned* counter;
+    double* history; // [max_iter], may be null
+    int lanes;
+    unsigned grid;   // range_grid(n, ...)
+};
+
+// r = b (x0 = 0) or r = b - A x (warm); z = M r (Jacobi) ; p = z; rz, ||b||.
+// For an external preconditioner call cg_init_r, apply M, then cg_init_p.
+template <class T> int cg_init_jacobi(const CgArgs<T>& a, bool warm, double tol, int max_iter,
+                                       hipStream_t s);
+template <class T> int cg_init_r(const CgArgs<T>& a, bool warm, hipStream_t s);
+template <class T> int cg_init_p(const CgArgs<T>& a, double tol, int max_iter, hipStream_t s);
+// One CG iteration (three to four launches).  With a Jacobi preconditioner
+// the apply is fused; otherwise `apply` is invoked between the update and the
+// r.z reduction.
+template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s);
+template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s);  // Ap, alpha, x, r, ||r||
+template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s);  // rz, beta, p
+
+} // namespace mi

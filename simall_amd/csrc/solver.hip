@@ -1,0 +1,590 @@
+// solver.hip — mi_solver: device-resident CG with FP64 or mixed FP32/FP64
+// arithmetic behind the ILinearSolver contract (src/solver/Solver.hpp:92-100).
+//
+// FP64 path: CGSolver::solve (src/solver/LinearSolvers.cpp:420-452) with the
+// iteration kept on the device.  Mixed path: iterative refinement — the FP64
+// true residual r = b - A x is formed on the device, an FP32 preconditioned
+// CG solves A d = r to `inner_tolerance`, x += d in FP64, repeat until the
+// FP64 true residual meets `tolerance`.
+#include "solver.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+using namespace mi;
+
+namespace mi
+{
+
+// ----------------------------------------------------------------- Jacobi
+namespace
+{
+struct JacobiPrecond final : Precond
+{
+    DevBuf<double> inv64;
+    DevBuf<float> inv32;
+    bool has64 = false, has32 = false;
+    int setup(mi_csr* A, bool need64, bool need32, hipStream_t s) override
+    {
+        const Pattern& p = *A->pat;
+        has64 = has32 = false;
+        if (need64) {
+            MI_TRY(inv64.alloc(size_t(p.n)));
+            MI_TRY(launch_inv_diag(p, A->val.p, inv64.p, s));
+            has64 = true;
+        }
+        if (need32) {
+            MI_TRY(inv32.alloc(size_t(p.n)));
+            MI_TRY(launch_inv_diag32(p, A->val.p, inv32.p, s));
+            has32 = true;
+        }
+        return MI_OK;
+    }
+    const double* inv_diag64() const override { return has64 ? inv64.p : nullptr; }
+    const float* inv_diag32() const override { return has32 ? inv32.p : nullptr; }
+    const char* name() const override { return "jacobi"; }
+};
+} // namespace
+
+std::unique_ptr<Precond> make_jacobi() { return std::make_unique<JacobiPrecond>(); }
+
+// r32 = float(b - A x), dscal[0] = ||b - A x||  (FP64 arithmetic)
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_true_residual(int n, const int* __restrict__ rp,
+                                                          const int* __restrict__ ci,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ b,
+                                                          double* __restrict__ r64,
+                                                          float* __restrict__ r32,
+                                                          double* partials, unsigned* counter,
+                                                          double* out)
+{
+    constexpr int kRows = kBlock / LANES;
+    const RowRange rr = block_rows(n, gridDim.x, kRows);
+    const int sub = threadIdx.x / LANES;
+    const int lane = threadIdx.x & (LANES - 1);
+    double v[1] = {0.0};
+    for (int base = rr.beg; base < rr.end; base += kRows) {
+        const int row = base + sub;
+        const bool mine = row < rr.end;
+        const double s = csr_row_dot<LANES>(mine ? row : n, n, lane, rp, ci, val, x);
+        if (mine && lane == 0) {
+            const double ri = b[row] - s;
+            if (r64)
+                r64[row] = ri;
+            if (r32)
+                r32[row] = float(ri);
+            v[0] += ri * ri;
+        }
+    }
+    double tot[1];
+    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
+        out[0] = sqrt(tot[0]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int n, const float* __restrict__ d,
+                                                           double* __restrict__ x)
+{
+    const RowRange rr = block_rows(n, gridDim.x, 1);
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        x[i] += double(d[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_norm(int n, const double* __restrict__ b,
+                                                 double* partials, unsigned* counter, double* out)
+{
+    const RowRange rr = block_rows(n, gridDim.x, 1);
+    double v[1] = {0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        v[0] += b[i] * b[i];
+    double tot[1];
+    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
+        out[0] = sqrt(tot[0]);
+}
+
+} // namespace mi
+
+mi_solver::~mi_solver()
+{
+    if (h_st)
+        (void)hipHostFree(h_st);
+    if (h_scal)
+        (void)hipHostFree(h_scal);
+    if (stream)
+        (void)hipStreamDestroy(stream);
+}
+
+namespace
+{
+
+using Clock = std::chrono::steady_clock;
+
+int true_residual(mi_solver* S, const double* b, const double* x, double* r64, float* r32,
+                  double* out_dev)
+{
+    const Pattern& P = *S->A->pat;
+    const int n = int(P.n);
+    const unsigned grid = range_grid(n, kBlock);
+    hipStream_t s = S->stream;
+    switch (P.lanes) {
+#define MI_CASE(L)                                                                               \
+    case L:                                                                                      \
+        k_true_residual<L><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->A->val.p, x, b, r64, r32, \
+                                                   S->partials.p, S->counter.p, out_dev);        \
+        break;
+        MI_CASE(2)
+        MI_CASE(4)
+        MI_CASE(8)
+        MI_CASE(16)
+        MI_CASE(32)
+    default:
+        k_true_residual<64><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->A->val.p, x, b, r64, r32,
+                                                    S->partials.p, S->counter.p, out_dev);
+        break;
+#undef MI_CASE
+    }
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+int read_scalars(mi_solver* S, int count)
+{
+    MI_HIP(hipMemcpyAsync(S->h_scal, S->dscal.p, sizeof(double) * size_t(count),
+                          hipMemcpyDeviceToHost, S->stream));
+    MI_HIP(hipStreamSynchronize(S->stream));
+    return MI_OK;
+}
+
+int ensure_workspace(mi_solver* S)
+{
+    const size_t n = size_t(S->A->n());
+    MI_TRY(S->r.alloc(n));
+    MI_TRY(S->z.alloc(n));
+    MI_TRY(S->p.alloc(n));
+    MI_TRY(S->Ap.alloc(n));
+    MI_TRY(S->st.alloc(1));
+    MI_TRY(S->partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
+    if (!S->counter.p) {
+        MI_TRY(S->counter.alloc(1));
+        MI_HIP(hipMemsetAsync(S->counter.p, 0, sizeof(unsigned), S->stream));
+    }
+    MI_TRY(S->dscal.alloc(8));
+    if (!S->h_st)
+        MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->h_st), sizeof(KState)));
+    if (!S->h_scal)
+        MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->h_scal), 8 * sizeof(double)));
+    const int hist = S->cfg.max_iterations > 0 ? S->cfg.max_iterations : 1;
+    MI_TRY(S->hist.alloc(size_t(hist)));
+    if (S->cfg.precision == MI_PRECISION_MIXED) {
+        MI_TRY(S->r32.alloc(n));
+        MI_TRY(S->z32.alloc(n));
+        MI_TRY(S->p32.alloc(n));
+        MI_TRY(S->Ap32.alloc(n));
+        MI_TRY(S->x32.alloc(n));
+    }
+    return MI_OK;
+}
+
+int build(mi_solver* S)
+{
+    mi_csr* A = S->A;
+    const bool mixed = S->cfg.precision == MI_PRECISION_MIXED;
+    if (S->pc && S->built_version == A->version)
+        return MI_OK;
+    const auto t0 = Clock::now();
+    if (mixed)
+        MI_TRY(A->ensure_fp32(S->stream));
+    if (!S->pc) {
+        switch (S->cfg.preconditioner) {
+        case MI_PC_ILU:
+        case MI_PC_SSOR:
+        case MI_PC_AMG:
+            return set_error(MI_ERR_UNSUPPORTED, "preconditioner %d not available on device yet",
+                             S->cfg.preconditioner);
+        default:
+            S->pc = make_jacobi(); // Jacobi, None and unknown kinds (LinearSolvers.cpp:240-254)
+        }
+    }
+    MI_TRY(S->pc->setup(A, !mixed, mixed, S->stream));
+    MI_TRY(ensure_workspace(S));
+    MI_HIP(hipStreamSynchronize(S->stream));
+    S->built_version = A->version;
+    S->setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+    return MI_OK;
+}
+
+int check_interval(const mi_solver* S, int n)
+{
+    if (S->cfg.check_interval > 0)
+        return S->cfg.check_interval;
+    return n < 200000 ? 32 : 8;
+}
+
+// Runs CG iterations on the device until KState::done; returns the final
+// state in S->h_st.
+template <class T>
+int run_cg(mi_solver* S, const CgArgs<T>& a, Precond* pc, bool jacobi, double tol, int max_iter,
+           bool warm)
+{
+    hipStream_t s = S->stream;
+    if (jacobi) {
+        MI_TRY(cg_init_jacobi<T>(a, warm, tol, max_iter, s));
+    } else {
+        MI_TRY(cg_init_r<T>(a, warm, s));
+        if constexpr (sizeof(T) == 8)
+            MI_TRY(pc->apply64(a.r, a.z, s));
+        else
+            MI_TRY(pc->apply32(a.r, a.z, s));
+        MI_TRY(cg_init_p<T>(a, tol, max_iter, s));
+    }
+    const int chunk = check_interval(S, a.n);
+    int issued = 0;
+    for (;;) {
+        const int k = std::min(chunk, max_iter - issued);
+        for (int i = 0; i < k; ++i) {
+            if (jacobi) {
+                MI_TRY(cg_iteration_jacobi<T>(a, s));
+            } else {
+                MI_TRY(cg_iteration_part1<T>(a, s));
+                if constexpr (sizeof(T) == 8)
+                    MI_TRY(pc->apply64(a.r, a.z, s));
+                else
+                    MI_TRY(pc->apply32(a.r, a.z, s));
+                MI_TRY(cg_iteration_part2<T>(a, s));
+            }
+        }
+        issued += k;
+        MI_HIP(hipMemcpyAsync(S->h_st, a.st, sizeof(KState), hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        if (S->h_st->done || issued >= max_iter)
+            break;
+    }
+    return MI_OK;
+}
+
+template <class T>
+CgArgs<T> make_args(mi_solver* S, const T* val, const T* invD, const T* b, T* x, T* r, T* z, T* p,
+                    T* Ap, double* history)
+{
+    const Pattern& P = *S->A->pat;
+    CgArgs<T> a{};
+    a.n = int(P.n);
+    a.nnz = P.nnz;
+    a.rp = P.rp.p;
+    a.ci = P.ci.p;
+    a.val = val;
+    a.invD = invD;
+    a.b = b;
+    a.x = x;
+    a.r = r;
+    a.z = z;
+    a.p = p;
+    a.Ap = Ap;
+    a.st = S->st.p;
+    a.partials = S->partials.p;
+    a.counter = S->counter.p;
+    a.history = history;
+    a.lanes = P.lanes;
+    a.grid = range_grid(P.n, 256);
+    return a;
+}
+
+void copy_history(mi_solver* S, int count, int offset)
+{
+    if (count <= 0)
+        return;
+    const size_t old = S->history.size();
+    (void)offset;
+    S->history.resize(old + size_t(count));
+    (void)hipMemcpy(S->history.data() + old, S->hist.p, sizeof(double) * size_t(count),
+                    hipMemcpyDeviceToHost);
+}
+
+int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
+{
+    const mi_solver_cfg& c = S->cfg;
+    Precond* pc = S->pc.get();
+    const bool jac = pc->inv_diag64() != nullptr;
+    auto a = make_args<double>(S, S->A->val.p, pc->inv_diag64(), b, x, S->r.p, S->z.p, S->p.p,
+                               S->Ap.p, S->hist.p);
+    const int n = a.n;
+    bool warm = c.warm_start != 0;
+    int total = 0, outer = 0;
+    int status = MI_OK;
+    double bnorm = 0.0, rel_true = 0.0, init_res = 0.0;
+    // ||b|| and the initial residual
+    k_norm<<<range_grid(n, kBlock), kBlock, 0, S->stream>>>(n, b, S->partials.p, S->counter.p,
+                                                            S->dscal.p + 1);
+    MI_HIP(hipGetLastError());
+    if (warm)
+        MI_TRY(true_residual(S, b, x, S->r.p, nullptr, S->dscal.p + 2));
+    MI_TRY(read_scalars(S, 3));
+    bnorm = std::max(S->h_scal[1], 1e-30);
+    init_res = warm ? S->h_scal[2] : S->h_scal[1];
+    for (;;) {
+        MI_TRY(run_cg<double>(S, a, pc, jac, c.tolerance, c.max_iterations - total, warm));
+        const KState st = *S->h_st;
+        copy_history(S, st.iter, total);
+        total += st.iter;
+        ++outer;
+        S->last = st.last;
+        MI_TRY(true_residual(S, b, x, nullptr, nullptr, S->dscal.p));
+        MI_TRY(read_scalars(S, 1));
+        rel_true = S->h_scal[0] / bnorm;
+        if (st.done == 3) {
+            status = (st.last < c.tolerance) ? MI_OK : MI_ERR_BREAKDOWN;
+            break;
+        }
+        if (st.done != 1) {
+            status = MI_ERR_NOT_CONVERGED;
+            break;
+        }
+        if (!c.true_residual || rel_true < c.tolerance)
+            break;
+        if (total >= c.max_iterations || outer >= std::max(c.max_outer, 1)) {
+            status = MI_ERR_NOT_CONVERGED;
+            break;
+        }
+        // Recurrence residual converged but the true residual did not:
+        // restart from the current x (refinement in FP64).
+        MI_TRY(true_residual(S, b, x, S->r.p, nullptr, S->dscal.p));
+        warm = true;
+    }
+    if (c.true_residual)
+        S->last = rel_true;
+    out->iterations = total;
+    out->outer_iterations = outer;
+    out->status = status;
+    out->converged = status == MI_OK;
+    out->initial_residual = init_res;
+    out->relative_residual = S->last;
+    out->final_residual = S->last * bnorm;
+    out->true_relative_residual = rel_true;
+    return status;
+}
+
+int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
+{
+    const mi_solver_cfg& c = S->cfg;
+    Precond* pc = S->pc.get();
+    const bool jac = pc->inv_diag32() != nullptr;
+    const int n = int(S->A->n());
+    hipStream_t s = S->stream;
+    if (!jac)
+        return set_error(MI_ERR_UNSUPPORTED, "mixed precision needs an FP32 preconditioner");
+    // Inner system A d = r32: b = r32 (outer residual, FP32), x = x32 (the
+    // correction d), r = b32 (inner residual), z = z32, p = p32, Ap = Ap32.
+    MI_TRY(S->b32.alloc(size_t(n)));
+    auto a = make_args<float>(S, S->A->val32.p, pc->inv_diag32(), S->r32.p, S->x32.p, S->b32.p,
+                              S->z32.p, S->p32.p, S->Ap32.p, S->hist.p);
+
+    const bool warm = c.warm_start != 0;
+    if (!warm)
+        MI_HIP(hipMemsetAsync(x, 0, sizeof(double) * size_t(n), s));
+    k_norm<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, b, S->partials.p, S->counter.p,
+                                                    S->dscal.p + 1);
+    MI_HIP(hipGetLastError());
+    MI_TRY(true_residual(S, b, x, nullptr, S->r32.p, S->dscal.p));
+    MI_TRY(read_scalars(S, 2));
+    const double bnorm = std::max(S->h_scal[1], 1e-30);
+    const double init_res = S->h_scal[0];
+    double rn = init_res;
+    int total = 0, outer = 0, status = MI_OK;
+    const double inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-5;
+    while (rn / bnorm >= c.tolerance) {
+        if (outer >= std::max(c.max_outer, 1) || total >= c.max_iterations) {
+            status = MI_ERR_NOT_CONVERGED;
+            break;
+        }
+        // inner: A d = r32 from d = 0, stop at inner_tol relative to ||r32||,
+        // or earlier if that already meets the outer target.
+        const double tol_in = std::max(inner_tol, 0.5 * c.tolerance * bnorm / rn);
+        MI_TRY(run_cg<float>(S, a, pc, jac, tol_in, c.max_iterations - total, false));
+        const KState st = *S->h_st;
+        copy_history(S, st.iter, total);
+        total += st.iter;
+        ++outer;
+        k_add_f32_to_f64<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, S->x32.p, x);
+        MI_HIP(hipGetLastError());
+        MI_TRY(true_residual(S, b, x, nullptr, S->r32.p, S->dscal.p));
+        MI_TRY(read_scalars(S, 1));
+        const double rn_new = S->h_scal[0];
+        if (st.done == 3 && st.iter == 0) {
+            status = MI_ERR_BREAKDOWN;
+            rn = rn_new;
+            break;
+        }
+        rn = rn_new;
+    }
+    S->last = rn / bnorm;
+    out->iterations = total;
+    out->outer_iterations = outer;
+    out->status = status;
+    out->converged = status == MI_OK;
+    out->initial_residual = init_res;
+    out->final_residual = rn;
+    out->relative_residual = S->last;
+    out->true_relative_residual = S->last;
+    return status;
+}
+
+int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
+{
+    mi_stats local{};
+    mi_stats* out = stats ? stats : &local;
+    std::memset(out, 0, sizeof(*out));
+    if (!S->A)
+        return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
+    if (S->cfg.kind != MI_SOLVER_CG)
+        return set_error(MI_ERR_UNSUPPORTED, "solver kind %d not available on device yet",
+                         S->cfg.kind);
+    MI_TRY(build(S));
+    out->setup_seconds = S->setup_seconds;
+    S->history.clear();
+    if (S->A->n() == 0) {
+        out->converged = 1;
+        S->last = 0.0;
+        return MI_OK;
+    }
+    const auto t0 = Clock::now();
+    int rc;
+    if (S->cfg.precision == MI_PRECISION_MIXED)
+        rc = solve_mixed(S, b, x, out);
+    else
+        rc = solve_fp64(S, b, x, out);
+    if (rc != MI_OK && rc != MI_ERR_NOT_CONVERGED && rc != MI_ERR_BREAKDOWN)
+        return rc;
+    MI_HIP(hipStreamSynchronize(S->stream));
+    out->solve_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+    return rc;
+}
+
+} // namespace
+
+extern "C" {
+
+MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
+{
+    if (!c)
+        return;
+    std::memset(c, 0, sizeof(*c));
+    c->kind = MI_SOLVER_CG;
+    c->preconditioner = MI_PC_JACOBI;
+    c->tolerance = 1.0e-8;
+    c->max_iterations = 1000;
+    c->restart = 30;
+    c->precision = MI_PRECISION_FP64;
+    c->warm_start = 0;
+    c->true_residual = 0;
+    c->inner_tolerance = 1.0e-5;
+    c->max_outer = 20;
+    c->reorder = -1;
+    c->check_interval = 0;
+    c->amg_max_levels = 10;
+    c->amg_coarse_size = 256;
+    c->amg_strength = 0.08;
+    c->amg_jacobi_omega = 4.0 / 3.0;
+    c->amg_smoother = 0;
+    c->amg_sweeps = 2;
+}
+
+MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out)
+{
+    clear_error();
+    if (!out)
+        return set_error(MI_ERR_BAD_ARG, "null output handle");
+    *out = nullptr;
+    MI_TRY(ensure_device());
+    auto* S = new (std::nothrow) mi_solver();
+    if (!S)
+        return set_error(MI_ERR_DEVICE, "out of host memory");
+    if (cfg)
+        S->cfg = *cfg;
+    else
+        mi_solver_cfg_default(&S->cfg);
+    if (S->cfg.max_iterations < 0 || !(S->cfg.tolerance >= 0.0)) {
+        delete S;
+        return set_error(MI_ERR_BAD_ARG, "bad tolerance or iteration limit");
+    }
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete S;
+        return set_error(MI_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    *out = S;
+    return MI_OK;
+}
+
+MI_API int mi_solver_setup(mi_solver* S, mi_csr* A)
+{
+    clear_error();
+    if (!S || !A)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    if (S->A != A) {
+        S->A = A;
+        S->pc.reset();
+        S->built_version = 0;
+    }
+    return build(S);
+}
+
+MI_API int mi_solver_solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
+{
+    clear_error();
+    if (!S || (S->A && S->A->n() && (!b || !x)))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    return solve_device(S, b, x, stats);
+}
+
+MI_API int mi_solver_solve(mi_solver* S, const double* b, double* x, mi_stats* stats)
+{
+    clear_error();
+    if (!S || !S->A)
+        return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
+    const size_t n = size_t(S->A->n());
+    if (n && (!b || !x))
+        return set_error(MI_ERR_BAD_ARG, "null vector");
+    MI_TRY(ensure_device());
+    MI_TRY(S->b.alloc(n));
+    MI_TRY(S->x.alloc(n));
+    if (n) {
+        MI_HIP(hipMemcpyAsync(S->b.p, b, n * sizeof(double), hipMemcpyHostToDevice, S->stream));
+        if (S->cfg.warm_start)
+            MI_HIP(hipMemcpyAsync(S->x.p, x, n * sizeof(double), hipMemcpyHostToDevice, S->stream));
+    }
+    const int rc = solve_device(S, S->b.p, S->x.p, stats);
+    if (rc != MI_OK && rc != MI_ERR_NOT_CONVERGED && rc != MI_ERR_BREAKDOWN)
+        return rc;
+    if (n) {
+        MI_HIP(hipMemcpyAsync(x, S->x.p, n * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+        MI_HIP(hipStreamSynchronize(S->stream));
+    }
+    return rc;
+}
+
+MI_API double mi_solver_last_residual(const mi_solver* S) { return S ? S->last : 0.0; }
+
+MI_API int mi_solver_history(const mi_solver* S, double* out, int capacity, int* count)
+{
+    if (!S)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    const int n = int(S->history.size());
+    if (count)
+        *count = n;
+    if (out)
+        std::memcpy(out, S->history.data(), sizeof(double) * size_t(std::min(n, capacity)));
+    return MI_OK;
+}
+
+MI_API int mi_solver_destroy(mi_solver* S)
+{
+    if (S && S->stream)
+        (void)hipStreamSynchronize(S->stream);
+    delete S;
+    return MI_OK;
+}
+
+} // extern "C"

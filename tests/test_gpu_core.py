@@ -1,0 +1,221 @@
+"""GPU parity of the vector kernels, SpMV and Jacobi-PCG against the oracle.
+
+Tolerances: SpMV and dots are FP64 with a different summation order than the
+serial reference (lane-strided partial sums), so they agree to a few ulps of
+the row magnitude (rtol 1e-13 against sums of |a_ij x_j|).  Krylov iterates
+agree to the iteration count +-2 and to the solution within 1e-6 relative at a
+1e-8 residual tolerance (both runs stop below 1e-8; the difference is bounded
+by the conditioning of the pinned system).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from simall_amd import capi
+from simall_amd import workload as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def tridiag_csr(n, lo, d, up):
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        if i > 0:
+            ci.append(i - 1)
+            va.append(lo)
+        ci.append(i)
+        va.append(d)
+        if i + 1 < n:
+            ci.append(i + 1)
+            va.append(up)
+        rp.append(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), np.array(va)
+
+
+def random_csr(n, seed, min_len=0, max_len=9, with_diag=True):
+    rng = np.random.default_rng(seed)
+    rp, ci = [0], []
+    for i in range(n):
+        k = int(rng.integers(min_len, max_len + 1))
+        cols = set(rng.choice(n, size=min(k, n), replace=False).tolist())
+        if with_diag:
+            cols.add(i)
+        ci.extend(sorted(cols))
+        rp.append(len(ci))
+    va = rng.standard_normal(len(ci))
+    return np.array(rp, np.int32), np.array(ci, np.int32), va
+
+
+def test_blas1_known_answers(gpu):
+    # tests/unit/gpu/test_gpu_models.cpp:117-141 on device pointers
+    x = capi.DeviceArray.from_host(np.arange(1.0, 6.0))
+    y = capi.DeviceArray.from_host(np.ones(5))
+    capi.check(gpu.mi_axpy(2.0, x.ptr, y.ptr, 5, None))
+    gpu.mi_device_synchronize()
+    assert np.allclose(y.download(), 1.0 + 2.0 * np.arange(1.0, 6.0), atol=1e-12)
+    capi.check(gpu.mi_scal(0.5, y.ptr, 5, None))
+    gpu.mi_device_synchronize()
+    assert abs(y.download()[0] - 1.5) < 1e-12
+    d = capi.C.c_double()
+    capi.check(gpu.mi_dot(x.ptr, x.ptr, 5, capi.C.byref(d), None))
+    assert d.value == 55.0
+    capi.check(gpu.mi_nrm2(x.ptr, 5, capi.C.byref(d), None))
+    assert abs(d.value - np.sqrt(55.0)) < 1e-12 * np.sqrt(55.0)
+    # axpby / xpby / copy / jacobi
+    capi.check(gpu.mi_axpby(2.0, x.ptr, -1.0, y.ptr, 5, None))
+    gpu.mi_device_synchronize()
+    yy = 2.0 * np.arange(1.0, 6.0) - 0.5 * (1.0 + 2.0 * np.arange(1.0, 6.0))
+    assert np.allclose(y.download(), yy, atol=1e-12)
+    capi.check(gpu.mi_xpby(x.ptr, 3.0, y.ptr, 5, None))
+    gpu.mi_device_synchronize()
+    assert np.allclose(y.download(), np.arange(1.0, 6.0) + 3.0 * yy, atol=1e-12)
+    z = capi.DeviceArray(5)
+    capi.check(gpu.mi_jacobi_apply(x.ptr, y.ptr, z.ptr, 5, None))
+    gpu.mi_device_synchronize()
+    assert np.allclose(z.download(), y.download() / np.arange(1.0, 6.0), rtol=1e-15)
+
+
+@pytest.mark.parametrize("n", [1, 1000, 100003, 3000000])
+def test_dot_large_deterministic(gpu, n):
+    rng = np.random.default_rng(n)
+    a, b = rng.standard_normal(n), rng.standard_normal(n)
+    da, db = capi.DeviceArray.from_host(a), capi.DeviceArray.from_host(b)
+    d1, d2 = capi.C.c_double(), capi.C.c_double()
+    capi.check(gpu.mi_dot(da.ptr, db.ptr, n, capi.C.byref(d1), None))
+    capi.check(gpu.mi_dot(da.ptr, db.ptr, n, capi.C.byref(d2), None))
+    assert d1.value == d2.value  # fixed reduction order
+    exact = float(np.sum(np.multiply(a, b, dtype=np.longdouble)))
+    assert abs(d1.value - exact) <= 1e-13 * np.abs(a * b).sum()
+
+
+def test_spmv_known_answer(gpu):
+    # tests/unit/gpu/test_gpu_models.cpp:144-178, through both entry points
+    rp = np.array([0, 2, 5, 8, 10], np.int32)
+    ci = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3], np.int32)
+    va = np.array([4, -1, -1, 4, -1, -1, 4, -1, -1, 4], np.float64)
+    A = capi.Csr(rp, ci, va)
+    assert np.array_equal(A.spmv([1, 2, 3, 4]), [2, 4, 6, 13])
+    drp, dci, dva = (capi.DeviceArray.from_host(v) for v in (rp, ci, va))
+    dx = capi.DeviceArray.from_host(np.arange(1.0, 5.0))
+    dy = capi.DeviceArray(4)
+    capi.check(gpu.mi_spmv_csr(4, drp.ptr, dci.ptr, dva.ptr, dx.ptr, dy.ptr, None))
+    gpu.mi_device_synchronize()
+    assert np.array_equal(dy.download(), [2, 4, 6, 13])
+
+
+@pytest.mark.parametrize("max_len", [0, 3, 7, 15, 30, 60, 200])
+def test_spmv_random_row_lengths(gpu, max_len):
+    n = 5000
+    rp, ci, va = random_csr(n, max_len, 0, max_len, with_diag=max_len > 0)
+    A = capi.Csr(rp, ci, va)
+    x = np.random.default_rng(1).standard_normal(n)
+    y = A.spmv(x)
+    Ao = oracle.Matrix(rp, ci, va)
+    yo = oracle.spmv(Ao, x)
+    scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x)) + 1e-300
+    assert np.all(np.abs(y - yo) <= 1e-13 * scale)
+    # FP32 values and vectors
+    dx = capi.DeviceArray.from_host(x.astype(np.float32))
+    dy = capi.DeviceArray(n, np.float32)
+    A.spmv_device(dx, dy, capi.MI_PRECISION_MIXED)
+    y32 = dy.download().astype(np.float64)
+    assert np.all(np.abs(y32 - yo) <= 2e-6 * scale)
+
+
+def test_spmv_empty_matrix(gpu):
+    A = capi.Csr(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert A.spmv(np.zeros(0)).size == 0
+
+
+def test_csr_rejects_bad_pattern(gpu):
+    with pytest.raises(capi.MiError) as e:
+        capi.Csr([0, 2, 1], [0, 1], [1.0, 2.0])
+    assert e.value.code == capi.MI_ERR_BAD_ARG
+    with pytest.raises(capi.MiError):
+        capi.Csr([0, 1], [5], [1.0])
+
+
+def test_cg_tridiagonal_known_answer(gpu):
+    # tests/unit/gpu/test_gpu_models.cpp:181-235
+    rp, ci, va = tridiag_csr(10, -1.0, 4.0, -1.0)
+    b = np.zeros(10)
+    b[0] = b[-1] = 1.0
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-10,
+                    max_iterations=200)
+    A = capi.Csr(rp, ci, va)
+    s.setup(A)
+    x, st, rc = s.solve(b)
+    assert rc == capi.MI_OK and st.converged and st.iterations < 50
+    D = oracle.Matrix(rp, ci, va).dense()
+    assert np.linalg.norm(b - D @ x) < 1e-8
+    xo, ito, lasto, ho = oracle.cg(oracle.Matrix(rp, ci, va), b, oracle.PC_JACOBI, 1e-10, 200)
+    assert st.iterations == ito
+    assert np.allclose(s.history(), ho, rtol=1e-10, atol=1e-300)
+
+
+def test_cg_zero_rhs_and_limits(gpu):
+    rp, ci, va = tridiag_csr(50, -1.0, 2.0, -1.0)
+    A = capi.Csr(rp, ci, va)
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, tolerance=1e-8, max_iterations=100)
+    s.setup(A)
+    x, st, rc = s.solve(np.zeros(50))
+    # reference: pAp == 0 on the first iteration -> returns 0, last = 0
+    assert st.iterations == 0 and not x.any() and rc == capi.MI_OK
+    s2 = capi.Solver(kind=capi.MI_SOLVER_CG, tolerance=1e-14, max_iterations=5)
+    s2.setup(A)
+    x, st, rc = s2.solve(np.ones(50))
+    xo, ito, lasto, _ = oracle.cg(oracle.Matrix(rp, ci, va), np.ones(50), oracle.PC_JACOBI, 1e-14, 5)
+    assert rc == capi.MI_ERR_NOT_CONVERGED and st.iterations == ito == 5
+    assert abs(s2.last_residual() - lasto) < 1e-12 * lasto
+    assert np.allclose(x, xo, rtol=1e-12, atol=1e-14)
+
+
+@pytest.fixture(scope="module")
+def cavity():
+    w = wl.cavity2d(128)
+    rp, ci = oracle.build_sparsity(w.mesh)
+    P = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    vals, _, pinned = oracle.assemble_pressure(w.mesh, P, w.bc_of_zone)
+    b = w.b.copy()
+    b[0] = 0.0
+    return w, oracle.Matrix(rp, ci, vals), b
+
+
+def test_cavity_jacobi_pcg_parity(gpu, cavity):
+    w, Ao, b = cavity
+    xo, ito, lasto, ho = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-8, 10000)
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                    max_iterations=10000)
+    A = capi.Csr(Ao.rowPtr, Ao.colIdx, Ao.values)
+    s.setup(A)
+    x, st, rc = s.solve(b)
+    print(f"cavity 128^2 Jacobi-PCG: GPU {st.iterations} its, oracle {ito}; "
+          f"true rel res {st.true_relative_residual:.3e}")
+    assert rc == capi.MI_OK and abs(st.iterations - ito) <= 2
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-6
+    h = s.history()
+    k = min(len(h), len(ho)) // 2
+    assert np.allclose(h[:k], ho[:k], rtol=1e-6)
+
+
+def test_cavity_mixed_precision_true_residual(gpu, cavity):
+    w, Ao, b = cavity
+    xt, _, _, _ = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-13, 20000)
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                    max_iterations=20000, precision=capi.MI_PRECISION_MIXED)
+    s.setup(capi.Csr(Ao.rowPtr, Ao.colIdx, Ao.values))
+    x, st, rc = s.solve(b)
+    r = b - oracle.spmv(Ao, x)
+    rel = np.linalg.norm(r) / np.linalg.norm(b)
+    print(f"cavity mixed: {st.iterations} inner its, {st.outer_iterations} sweeps, rel res {rel:.3e}")
+    assert rc == capi.MI_OK and rel <= 1e-8
+    assert abs(rel - st.true_relative_residual) <= 1e-3 * rel + 1e-15
+
+
+def test_fp64_true_residual_mode(gpu, cavity):
+    w, Ao, b = cavity
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, tolerance=1e-8, max_iterations=20000, true_residual=1)
+    s.setup(capi.Csr(Ao.rowPtr, Ao.colIdx, Ao.values))
+    x, st, rc = s.solve(b)
+    rel = np.linalg.norm(b - oracle.spmv(Ao, x)) / np.linalg.norm(b)
+    assert rc == capi.MI_OK and rel < 1e-8
