@@ -46,14 +46,15 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(int64_t n, const double* __re
         z[i] = r[i] / diag[i];
 }
 
-int ReduceScratch::ensure()
+int ReduceScratch::ensure(hipStream_t s)
 {
     if (partials.p)
         return MI_OK;
     MI_TRY(partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
     MI_TRY(counter.alloc(1));
     MI_TRY(result.alloc(kMaxReduceValues));
-    MI_HIP(hipMemset(counter.p, 0, sizeof(unsigned)));
+    // The ticket counter must be zero before the first grid_sum launch on s.
+    MI_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned), s));
     return MI_OK;
 }
 
@@ -69,7 +70,7 @@ int launch_vec(int mode, int64_t n, double a, const double* x, double b, double*
 int launch_dot(int64_t n, const double* x, const double* y, ReduceScratch& rs, double* out_dev,
                hipStream_t s)
 {
-    MI_TRY(rs.ensure());
+    MI_TRY(rs.ensure(s));
     k_dot<<<reduce_grid(n, 8), kBlock, 0, s>>>(n, x, y, rs.partials.p, rs.counter.p, out_dev);
     MI_HIP(hipGetLastError());
     return MI_OK;
@@ -141,7 +142,7 @@ MI_API int mi_dot(const void* x, const void* y, size_t n, double* result, void* 
         return set_error(MI_ERR_BAD_ARG, "null argument");
     ReduceScratch& rs = facade_scratch();
     hipStream_t s = as_stream(stream);
-    MI_TRY(rs.ensure());
+    MI_TRY(rs.ensure(s));
     MI_TRY(launch_dot(int64_t(n), static_cast<const double*>(x), static_cast<const double*>(y), rs,
                       rs.result.p, s));
     MI_HIP(hipMemcpyAsync(result, rs.result.p, sizeof(double), hipMemcpyDeviceToHost, s));

@@ -16,7 +16,7 @@ struct ReduceScratch
     DevBuf<double> partials;
     DevBuf<unsigned> counter;
     DevBuf<double> result;
-    int ensure();
+    int ensure(hipStream_t s);
 };
 
 // mode: 0 axpy y+=a*x, 1 scal y*=a, 2 copy y=x, 3 axpby y=a*x+b*y, 4 xpby y=x+b*y

@@ -91,10 +91,13 @@ int make_csr_on_pattern(const std::shared_ptr<Pattern>& pat, mi_csr** out)
         return set_error(MI_ERR_DEVICE, "out of host memory");
     A->pat = pat;
     int rc = A->val.alloc(size_t(pat->nnz));
-    if (rc == MI_OK && pat->nnz)
-        rc = (hipMemset(A->val.p, 0, size_t(pat->nnz) * sizeof(double)) == hipSuccess)
+    if (rc == MI_OK && pat->nnz) {
+        hipStream_t s = default_stream();
+        rc = (hipMemsetAsync(A->val.p, 0, size_t(pat->nnz) * sizeof(double), s) == hipSuccess &&
+              hipStreamSynchronize(s) == hipSuccess)
                  ? MI_OK
-                 : set_error(MI_ERR_DEVICE, "hipMemset failed");
+                 : set_error(MI_ERR_DEVICE, "hipMemsetAsync failed");
+    }
     if (rc != MI_OK) {
         delete A;
         return rc;
