@@ -179,11 +179,41 @@ MI_API int mi_solver_setup(mi_solver* s, mi_csr* A);
 MI_API int mi_solver_solve(mi_solver* s, const double* b_host, double* x_host, mi_stats* stats);
 MI_API int mi_solver_solve_device(mi_solver* s, const double* b_device, double* x_device,
                                   mi_stats* stats);
+/* IPreconditioner::apply (src/solver/LinearSolvers.hpp:28-35): z = M^-1 r
+ * with the preconditioner mi_solver_setup built (FP32 copy when the solver's
+ * precision is MI_PRECISION_MIXED).  Host arrays in the caller's numbering. */
+MI_API int mi_solver_precond_apply(mi_solver* s, const double* r_host, double* z_host);
+/* Preconditioner levels / operator complexity and the pattern bandwidth
+ * before and after the solver's internal reordering (-1: not reordered). */
+MI_API int mi_solver_info(const mi_solver* s, int* levels, double* operator_complexity,
+                          int* bandwidth_before, int* bandwidth_after);
 /* ILinearSolver::last_residual (src/solver/Solver.hpp:99). */
 MI_API double mi_solver_last_residual(const mi_solver* s);
 /* Relative residual after each Krylov iteration of the last solve. */
 MI_API int mi_solver_history(const mi_solver* s, double* out, int capacity, int* count);
 MI_API int mi_solver_destroy(mi_solver* s);
+
+/* ------------------------------------------------- host-side inspection
+ * The smoothed-aggregation hierarchy and the reverse Cuthill-McKee ordering
+ * that mi_solver_setup builds, computed on the host for inspection and tests
+ * (no device needed).  The reference's AMG (AMGPreconditioner::setup,
+ * src/solver/AMGPreconditioner.cpp:248-326) keeps its levels private; these
+ * entries have no reference counterpart. */
+typedef struct mi_amg_host mi_amg_host;
+MI_API int mi_amg_host_build(int64_t n, const int32_t* rowPtr, const int32_t* colIdx,
+                             const double* values, const mi_solver_cfg* cfg, mi_amg_host** out);
+MI_API int mi_amg_host_levels(const mi_amg_host* h, int* levels, double* operator_complexity,
+                              double* grid_complexity);
+/* which: 0 = A_l, 1 = P_l (prolongation from level l+1), 2 = R_l = P_l^T,
+ * 3 = dense inverse of the coarsest operator (row-major, values only). */
+MI_API int mi_amg_host_level_info(const mi_amg_host* h, int level, int which, int64_t* nrows,
+                                  int64_t* ncols, int64_t* nnz, double* lambda_max);
+MI_API int mi_amg_host_level_get(const mi_amg_host* h, int level, int which, int32_t* rowPtr,
+                                 int32_t* colIdx, double* values);
+MI_API int mi_amg_host_destroy(mi_amg_host* h);
+/* perm[new] = old, reverse Cuthill-McKee of the symmetrised pattern. */
+MI_API int mi_rcm_permutation(int64_t n, const int32_t* rowPtr, const int32_t* colIdx,
+                              int32_t* perm, int32_t* bandwidth_after);
 
 /* ------------------------------------------------------ mesh and assembly
  * The SoA mesh of meshing::Mesh (src/meshing/MeshStorage.hpp:37-74): uint64

@@ -115,6 +115,8 @@ PROTOTYPES = {
     "mi_solver_solve": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
     "mi_solver_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
     "mi_solver_last_residual": (_d, [_vp]),
+    "mi_solver_precond_apply": (_i, [_vp, _vp, _vp]),
+    "mi_solver_info": (_i, [_vp, C.POINTER(_i), _pd, C.POINTER(_i), C.POINTER(_i)]),
     "mi_solver_history": (_i, [_vp, _pd, _i, C.POINTER(_i)]),
     "mi_solver_destroy": (_i, [_vp]),
     "mi_mesh_create": (_i, [_vp, _vp, _i, C.POINTER(_vp)]),
@@ -124,6 +126,12 @@ PROTOTYPES = {
     "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
     "mi_mesh_destroy": (_i, [_vp]),
+    "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
+    "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
+    "mi_amg_host_level_info": (_i, [_vp, _i, _i, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _pd]),
+    "mi_amg_host_level_get": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
+    "mi_amg_host_destroy": (_i, [_vp]),
+    "mi_rcm_permutation": (_i, [_i64, _vp, _vp, _vp, C.POINTER(C.c_int32)]),
 }
 
 _lib = None
@@ -305,6 +313,18 @@ class Solver:
     def last_residual(self):
         return self.lib.mi_solver_last_residual(self.h)
 
+    def precond_apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        check(self.lib.mi_solver_precond_apply(self.h, ptr(r), ptr(z)))
+        return z
+
+    def info(self):
+        lv, oc, b0, b1 = C.c_int(), C.c_double(), C.c_int(), C.c_int()
+        check(self.lib.mi_solver_info(self.h, C.byref(lv), C.byref(oc), C.byref(b0), C.byref(b1)))
+        return {"levels": lv.value, "operator_complexity": oc.value, "bandwidth_before": b0.value,
+                "bandwidth_after": b1.value}
+
     def history(self):
         cnt = C.c_int()
         check(self.lib.mi_solver_history(self.h, None, 0, C.byref(cnt)))
@@ -412,3 +432,65 @@ class DeviceMesh:
             self.close()
         except Exception:
             pass
+
+
+class AmgHost:
+    """The SA-AMG hierarchy built on the host (inspection; no device needed)."""
+
+    A, P, R, COARSE_INV = 0, 1, 2, 3
+
+    def __init__(self, rowPtr, colIdx, values, cfg=None):
+        self.lib = load()
+        rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
+        ci = np.ascontiguousarray(colIdx, dtype=np.int32)
+        va = np.ascontiguousarray(values, dtype=np.float64)
+        h = C.c_void_p()
+        cfg = cfg if cfg is not None else default_cfg()
+        check(self.lib.mi_amg_host_build(rp.size - 1, ptr(rp), ptr(ci), ptr(va), C.byref(cfg), C.byref(h)))
+        self.h = h.value
+        nl, oc, gc = C.c_int(), C.c_double(), C.c_double()
+        check(self.lib.mi_amg_host_levels(self.h, C.byref(nl), C.byref(oc), C.byref(gc)))
+        self.levels, self.operator_complexity, self.grid_complexity = nl.value, oc.value, gc.value
+
+    def info(self, level, which):
+        nr, nc, nnz, lam = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        check(self.lib.mi_amg_host_level_info(self.h, level, which, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(lam)))
+        return nr.value, nc.value, nnz.value, lam.value
+
+    def matrix(self, level, which):
+        """(rowPtr, colIdx, values, ncols) of A_l / P_l / R_l, or the dense coarse inverse."""
+        nr, nc, nnz, _ = self.info(level, which)
+        if which == self.COARSE_INV:
+            v = np.empty(nnz)
+            check(self.lib.mi_amg_host_level_get(self.h, level, which, None, None, ptr(v)))
+            return v.reshape(nr, nc)
+        rp = np.empty(nr + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        v = np.empty(nnz)
+        check(self.lib.mi_amg_host_level_get(self.h, level, which, ptr(rp), ptr(ci), ptr(v)))
+        return rp, ci, v, nc
+
+    def scipy(self, level, which):
+        import scipy.sparse as sp
+        rp, ci, v, nc = self.matrix(level, which)
+        return sp.csr_matrix((v, ci, rp), shape=(rp.size - 1, nc))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_amg_host_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rcm_permutation(rowPtr, colIdx):
+    rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
+    ci = np.ascontiguousarray(colIdx, dtype=np.int32)
+    perm = np.empty(rp.size - 1, np.int32)
+    bw = C.c_int32()
+    check(load().mi_rcm_permutation(rp.size - 1, ptr(rp), ptr(ci), ptr(perm), C.byref(bw)))
+    return perm, bw.value

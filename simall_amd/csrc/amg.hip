@@ -1,0 +1,348 @@
+// amg.hip — smoothed-aggregation AMG V-cycle on the device.
+//
+// Solve-phase pieces of the hierarchy built by amg_setup.cpp, as gfx950
+// kernels in T = double (FP64 solves) or float (inner iterations of the mixed
+// solve; the coarsest solve stays FP64):
+//   smoother     Chebyshev polynomial in D^-1 A of degree k (k = 1 is damped
+//                Jacobi); each step is one fused kernel
+//                r -= A d ; d' = beta d + alpha D^-1 r ; x += d'
+//   restriction  b_c = R r        (R = P^T stored explicitly: gather, no atomics)
+//   prolongation e = P x_c ; x += e
+//   coarsest     x = A_c^-1 b with the dense FP64 inverse (one wavefront per row)
+// The reference's V-cycle (src/solver/AMGPreconditioner.cpp:388-426) uses
+// sequential symmetric Gauss-Seidel sweeps; the polynomial smoother is its
+// data-parallel counterpart and keeps the cycle symmetric for CG.
+#include "amg.hpp"
+#include "krylov.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mi
+{
+
+// ------------------------------------------------------------------ kernels
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cheb_init(int n, const T* __restrict__ b,
+                                                      const T* __restrict__ invD, T a0,
+                                                      T* __restrict__ x, T* __restrict__ r,
+                                                      T* __restrict__ d)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int i = lb * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const T bi = b[i];
+    const T di = a0 * invD[i] * bi;
+    r[i] = bi;
+    d[i] = di;
+    x[i] = di;
+}
+
+// mode 0: r -= A din ; dout = beta din + alpha D^-1 r ; x += dout
+// mode 1: r -= A din
+template <int LANES, class T>
+__global__ __launch_bounds__(kBlock) void k_cheb_step(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci,
+                                                      const T* __restrict__ val,
+                                                      const T* __restrict__ din,
+                                                      T* __restrict__ dout, T* __restrict__ r,
+                                                      const T* __restrict__ invD,
+                                                      T* __restrict__ x, T beta, T alpha, int mode)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
+    const int row = int(gid / LANES);
+    const int lane = int(gid & (LANES - 1));
+    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, din);
+    if (row < n && lane == 0) {
+        const T ri = r[row] - T(s);
+        r[row] = ri;
+        if (mode == 0) {
+            const T dn = beta * din[row] + alpha * invD[row] * ri;
+            dout[row] = dn;
+            x[row] += dn;
+        }
+    }
+}
+
+template <int LANES, class T>
+__global__ __launch_bounds__(kBlock) void k_prolong(int n, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci,
+                                                    const T* __restrict__ val,
+                                                    const T* __restrict__ xc, T* __restrict__ e,
+                                                    T* __restrict__ x)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
+    const int row = int(gid / LANES);
+    const int lane = int(gid & (LANES - 1));
+    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, xc);
+    if (row < n && lane == 0) {
+        e[row] = T(s);
+        x[row] += T(s);
+    }
+}
+
+template <int LANES, class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(int n, const int* __restrict__ rp,
+                                                     const int* __restrict__ ci,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ r, T* __restrict__ bc)
+{
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
+    const int row = int(gid / LANES);
+    const int lane = int(gid & (LANES - 1));
+    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, r);
+    if (row < n && lane == 0)
+        bc[row] = T(s);
+}
+
+// x = Ainv b, dense FP64, one wavefront per row.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_coarse_solve(int nc, const double* __restrict__ inv,
+                                                         const T* __restrict__ b,
+                                                         T* __restrict__ x)
+{
+    const int row = int(blockIdx.x) * (kBlock / kWave) + int(threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    double s = 0.0;
+    if (row < nc) {
+        const double* a = inv + size_t(row) * size_t(nc);
+        for (int j = lane; j < nc; j += kWave)
+            s += a[j] * double(b[j]);
+    }
+    s = wave_sum(s);
+    if (row < nc && lane == 0)
+        x[row] = T(s);
+}
+
+// ------------------------------------------------------------------ launch
+// 1-D grid padded to a multiple of 8 blocks so that xcd_logical_block gives
+// each XCD one contiguous slice of rows.
+static unsigned rows_grid(int n, int lanes)
+{
+    unsigned g = grid_for(int64_t(n) * lanes, kBlock);
+    return (g + 7) / 8 * 8;
+}
+
+#define MI_LANES_SWITCH(lanes, KNAME, T, ...)                                                      \
+    switch (lanes) {                                                                               \
+    case 2: KNAME<2, T><<<rows_grid(n, 2), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    case 4: KNAME<4, T><<<rows_grid(n, 4), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    case 8: KNAME<8, T><<<rows_grid(n, 8), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
+    case 16: KNAME<16, T><<<rows_grid(n, 16), kBlock, 0, s>>>(__VA_ARGS__); break;                \
+    case 32: KNAME<32, T><<<rows_grid(n, 32), kBlock, 0, s>>>(__VA_ARGS__); break;                \
+    default: KNAME<64, T><<<rows_grid(n, 64), kBlock, 0, s>>>(__VA_ARGS__); break;                \
+    }
+
+template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s)
+{
+    nr = h.nr;
+    nc = h.nc;
+    nnz = h.nnz();
+    lanes = lanes_for(nr, nnz);
+    MI_TRY(rp.upload(h.rp.data(), h.rp.size(), s));
+    MI_TRY(ci.upload(h.ci.data(), h.ci.size(), s));
+    std::vector<T> tv(h.v.begin(), h.v.end());
+    MI_TRY(v.upload(tv.data(), tv.size(), s));
+    MI_HIP(hipStreamSynchronize(s)); // tv is released on return
+    return MI_OK;
+}
+
+template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r, const T* invD,
+                                   T* x, double beta, double alpha, int mode, hipStream_t s)
+{
+    const int n = A.nr;
+    if (n == 0)
+        return MI_OK;
+    MI_LANES_SWITCH(A.lanes, k_cheb_step, T, n, A.rp.p, A.ci.p, A.v.p, din, dout, r, invD, x,
+                    T(beta), T(alpha), mode)
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree, double lmin_ratio,
+                                            hipStream_t s)
+{
+    const int nl = int(H.levels.size());
+    L.clear();
+    L.resize(size_t(nl));
+    deg = std::max(degree, 1);
+    for (int l = 0; l < nl; ++l) {
+        const AmgLevelHost& h = H.levels[size_t(l)];
+        DLevel<T>& d = L[size_t(l)];
+        const int n = h.A.nr;
+        MI_TRY(d.A.upload(h.A, s));
+        if (l + 1 < nl) {
+            MI_TRY(d.P.upload(h.P, s));
+            MI_TRY(d.R.upload(h.R, s));
+        }
+        std::vector<T> inv(size_t(n), T(1));
+        for (int i = 0; i < n; ++i)
+            for (int k = h.A.rp[size_t(i)]; k < h.A.rp[size_t(i) + 1]; ++k)
+                if (h.A.ci[size_t(k)] == i && h.A.v[size_t(k)] != 0.0)
+                    inv[size_t(i)] = T(1.0 / h.A.v[size_t(k)]);
+        MI_TRY(d.invD.upload(inv.data(), inv.size(), s));
+        MI_HIP(hipStreamSynchronize(s));
+        for (DevBuf<T>* b : {&d.x, &d.b, &d.r, &d.d, &d.d2, &d.e})
+            MI_TRY(b->alloc(size_t(std::max(n, 1))));
+        // Chebyshev coefficients on [lmin, lmax] of D^-1 A
+        const double lmax = h.lmax > 0 ? h.lmax : 2.0;
+        const double ratio = deg == 1 ? 0.5 : lmin_ratio;
+        const double lmin = lmax * ratio;
+        const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+        const double sigma = theta / delta;
+        d.alpha.assign(size_t(deg), 0.0);
+        d.beta.assign(size_t(deg), 0.0);
+        d.alpha[0] = 1.0 / theta;
+        double rho = 1.0 / sigma;
+        for (int k = 1; k < deg; ++k) {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            d.beta[size_t(k)] = rn * rho;
+            d.alpha[size_t(k)] = 2.0 * rn / delta;
+            rho = rn;
+        }
+    }
+    nc = H.coarse_n;
+    MI_TRY(cinv.upload(H.coarse_inverse.data(), H.coarse_inverse.size(), s));
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s)
+{
+    const int nl = int(L.size());
+    auto bptr = [&](int l) -> const T* { return l == 0 ? b0 : L[size_t(l)].b.p; };
+    auto xptr = [&](int l) -> T* { return l == 0 ? x0 : L[size_t(l)].x.p; };
+    // downward leg: pre-smooth from x = 0, restrict the residual
+    for (int l = 0; l + 1 < nl; ++l) {
+        DLevel<T>& d = L[size_t(l)];
+        const int n = d.A.nr;
+        const T* b = bptr(l);
+        T* x = xptr(l);
+        k_cheb_init<T><<<rows_grid(n, 1), kBlock, 0, s>>>(n, b, d.invD.p, T(d.alpha[0]), x, d.r.p,
+                                                         d.d.p);
+        MI_HIP(hipGetLastError());
+        T* din = d.d.p;
+        T* dout = d.d2.p;
+        for (int k = 1; k < deg; ++k) {
+            MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
+                           0, s));
+            std::swap(din, dout);
+        }
+        MI_TRY(step<T>(d.A, din, nullptr, d.r.p, d.invD.p, x, 0.0, 0.0, 1, s));
+        const int nr = d.R.nr;
+        {
+            const int n = nr;
+            MI_LANES_SWITCH(d.R.lanes, k_restrict, T, nr, d.R.rp.p, d.R.ci.p, d.R.v.p, d.r.p,
+                            L[size_t(l) + 1].b.p)
+        }
+        MI_HIP(hipGetLastError());
+    }
+    // coarsest level
+    {
+        const T* b = bptr(nl - 1);
+        T* x = xptr(nl - 1);
+        const unsigned grid = unsigned((nc + (kBlock / kWave) - 1) / (kBlock / kWave));
+        if (nc > 0) {
+            k_coarse_solve<T><<<grid, kBlock, 0, s>>>(nc, cinv.p, b, x);
+            MI_HIP(hipGetLastError());
+        }
+    }
+    // upward leg: prolongate, post-smooth
+    for (int l = nl - 2; l >= 0; --l) {
+        DLevel<T>& d = L[size_t(l)];
+        const int n = d.A.nr;
+        T* x = xptr(l);
+        const T* xc = xptr(l + 1);
+        MI_LANES_SWITCH(d.P.lanes, k_prolong, T, n, d.P.rp.p, d.P.ci.p, d.P.v.p, xc, d.e.p, x)
+        MI_HIP(hipGetLastError());
+        // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps
+        MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, s));
+        T* din = d.d.p;
+        T* dout = d.d2.p;
+        for (int k = 1; k < deg; ++k) {
+            MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
+                           0, s));
+            std::swap(din, dout);
+        }
+    }
+    return MI_OK;
+}
+
+template struct DHierarchy<double>;
+template struct DHierarchy<float>;
+template struct DMat<double>;
+template struct DMat<float>;
+
+// --------------------------------------------------------------- precond
+namespace
+{
+
+struct AmgPrecond final : Precond
+{
+    AmgParams params;
+    int degree = 2;
+    double lmin_ratio = 0.3;
+    AmgHierarchyHost host;
+    DHierarchy<double> h64;
+    DHierarchy<float> h32;
+    bool has64 = false, has32 = false;
+
+    int setup(mi_csr* A, bool need64, bool need32, hipStream_t s) override
+    {
+        MI_TRY(A->sync_host());
+        HCsr H;
+        H.nr = H.nc = int(A->n());
+        H.rp = A->pat->h_rp;
+        H.ci = A->pat->h_ci;
+        H.v = A->h_val;
+        amg_build(H, params, host);
+        has64 = has32 = false;
+        if (need64) {
+            MI_TRY(h64.build(host, degree, lmin_ratio, s));
+            has64 = true;
+        }
+        if (need32) {
+            MI_TRY(h32.build(host, degree, lmin_ratio, s));
+            has32 = true;
+        }
+        return MI_OK;
+    }
+    int apply64(const double* r, double* z, hipStream_t s) override
+    {
+        if (!has64)
+            return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP64 copy");
+        return h64.vcycle(r, z, s);
+    }
+    int apply32(const float* r, float* z, hipStream_t s) override
+    {
+        if (!has32)
+            return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP32 copy");
+        return h32.vcycle(r, z, s);
+    }
+    const char* name() const override { return "sa-amg"; }
+    int levels() const override { return int(host.levels.size()); }
+    double complexity() const override { return host.operator_complexity; }
+};
+
+} // namespace
+
+std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c)
+{
+    auto p = std::make_unique<AmgPrecond>();
+    if (c.amg_max_levels > 0)
+        p->params.max_levels = c.amg_max_levels;
+    if (c.amg_coarse_size > 0)
+        p->params.coarse_size = c.amg_coarse_size;
+    if (c.amg_strength >= 0)
+        p->params.theta = c.amg_strength;
+    if (c.amg_jacobi_omega > 0)
+        p->params.omega_scale = c.amg_jacobi_omega;
+    p->degree = c.amg_sweeps > 0 ? c.amg_sweeps : 2;
+    return p;
+}
+
+} // namespace mi

@@ -1,0 +1,43 @@
+// amg.hpp — device-side smoothed-aggregation hierarchy.
+#pragma once
+
+#include "amg_setup.hpp"
+#include "solver.hpp"
+
+#include <vector>
+
+namespace mi
+{
+
+template <class T> struct DMat
+{
+    int nr = 0, nc = 0;
+    int64_t nnz = 0;
+    int lanes = 8;
+    DevBuf<int> rp, ci;
+    DevBuf<T> v;
+    int upload(const HCsr& h, hipStream_t s);
+};
+
+template <class T> struct DLevel
+{
+    DMat<T> A, P, R;
+    DevBuf<T> invD;
+    DevBuf<T> x, b, r, d, d2, e; // x and b of level 0 are the caller's vectors
+    std::vector<double> alpha, beta; // Chebyshev step coefficients
+};
+
+template <class T> struct DHierarchy
+{
+    std::vector<DLevel<T>> L;
+    DevBuf<double> cinv; // dense FP64 inverse of the coarsest operator
+    int nc = 0;
+    int deg = 2;
+    int build(const AmgHierarchyHost& H, int degree, double lmin_ratio, hipStream_t s);
+    // x0 = V-cycle(b0); x0 needs no initialisation.
+    int vcycle(const T* b0, T* x0, hipStream_t s);
+};
+
+std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c);
+
+} // namespace mi

@@ -1,0 +1,469 @@
+// amg_setup.cpp — smoothed-aggregation AMG setup (Vanek, Mandel, Brezina 1996)
+// on the host in FP64.
+//
+// The reference preconditioner is classical Ruge-Stueben AMG
+// (src/solver/AMGPreconditioner.cpp:43-437) whose C/F splitting rescans all
+// rows for every coarse point; it cannot build a 2M-row hierarchy.  This
+// setup replaces it with smoothed aggregation, whose pieces map onto
+// data-parallel GPU kernels in the solve phase:
+//   strength    |a_ij| >= theta sqrt(|a_ii a_jj|)
+//   aggregates  greedy root + strong neighbourhood, leftovers joined to the
+//               most strongly connected aggregate, isolated rows as singletons
+//   T           piecewise-constant tentative prolongator (near null space 1)
+//   P           (I - omega D^-1 A) T, omega = 4/3 / lambda_max(D^-1 A)
+//   A_c         R A P with R = P^T (symmetric V-cycle for CG)
+// and the coarsest operator is inverted densely in FP64.
+#include "amg_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+namespace mi
+{
+
+namespace
+{
+
+int thread_count(int requested)
+{
+    int t = requested > 0 ? requested : int(std::thread::hardware_concurrency());
+    if (t < 1)
+        t = 1;
+    if (t > 16)
+        t = 16;
+    return t;
+}
+
+template <class F> void parallel_for(int n, int threads, F&& f)
+{
+    if (threads <= 1 || n < 4096) {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int chunk = (n + threads - 1) / threads;
+    for (int t = 0; t < threads; ++t) {
+        const int b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e)
+            break;
+        pool.emplace_back([&, b, e, t] { f(b, e, t); });
+    }
+    for (auto& th : pool)
+        th.join();
+}
+
+std::vector<double> diagonal(const HCsr& A)
+{
+    std::vector<double> d(size_t(A.nr), 0.0);
+    for (int i = 0; i < A.nr; ++i)
+        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k)
+            if (A.ci[size_t(k)] == i)
+                d[size_t(i)] += A.v[size_t(k)];
+    return d;
+}
+
+} // namespace
+
+void amg_strength(const HCsr& A, double theta, std::vector<int>& srp, std::vector<int>& sci)
+{
+    const std::vector<double> d = diagonal(A);
+    srp.assign(size_t(A.nr) + 1, 0);
+    sci.clear();
+    sci.reserve(size_t(A.nnz()));
+    for (int i = 0; i < A.nr; ++i) {
+        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+            const int j = A.ci[size_t(k)];
+            if (j == i)
+                continue;
+            const double a = std::abs(A.v[size_t(k)]);
+            if (a > 0.0 && a >= theta * std::sqrt(std::abs(d[size_t(i)] * d[size_t(j)])))
+                sci.push_back(j);
+        }
+        srp[size_t(i) + 1] = int(sci.size());
+    }
+}
+
+int amg_aggregate(int n, const std::vector<int>& srp, const std::vector<int>& sci, const HCsr& A,
+                  std::vector<int>& agg)
+{
+    agg.assign(size_t(n), -1);
+    int nagg = 0;
+    // pass 1: a node whose strong neighbours are all free roots an aggregate
+    for (int i = 0; i < n; ++i) {
+        if (agg[size_t(i)] >= 0)
+            continue;
+        bool free = true;
+        for (int k = srp[size_t(i)]; k < srp[size_t(i) + 1]; ++k)
+            if (agg[size_t(sci[size_t(k)])] >= 0) {
+                free = false;
+                break;
+            }
+        if (!free)
+            continue;
+        agg[size_t(i)] = nagg;
+        for (int k = srp[size_t(i)]; k < srp[size_t(i) + 1]; ++k)
+            agg[size_t(sci[size_t(k)])] = nagg;
+        ++nagg;
+    }
+    // pass 2: join the aggregate of the most strongly connected neighbour
+    std::vector<int> joined(size_t(n), -1);
+    for (int i = 0; i < n; ++i) {
+        if (agg[size_t(i)] >= 0)
+            continue;
+        double best = -1.0;
+        int target = -1;
+        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+            const int j = A.ci[size_t(k)];
+            if (j == i || agg[size_t(j)] < 0)
+                continue;
+            bool strong = false;
+            for (int s = srp[size_t(i)]; s < srp[size_t(i) + 1]; ++s)
+                if (sci[size_t(s)] == j) {
+                    strong = true;
+                    break;
+                }
+            const double a = std::abs(A.v[size_t(k)]);
+            if (strong && a > best) {
+                best = a;
+                target = agg[size_t(j)];
+            }
+        }
+        joined[size_t(i)] = target;
+    }
+    for (int i = 0; i < n; ++i)
+        if (agg[size_t(i)] < 0 && joined[size_t(i)] >= 0)
+            agg[size_t(i)] = joined[size_t(i)];
+    // pass 3: what is left forms new aggregates with its free strong neighbours
+    for (int i = 0; i < n; ++i) {
+        if (agg[size_t(i)] >= 0)
+            continue;
+        agg[size_t(i)] = nagg;
+        for (int k = srp[size_t(i)]; k < srp[size_t(i) + 1]; ++k)
+            if (agg[size_t(sci[size_t(k)])] < 0)
+                agg[size_t(sci[size_t(k)])] = nagg;
+        ++nagg;
+    }
+    return nagg;
+}
+
+double amg_lambda_max(const HCsr& A, int iters)
+{
+    // Power iteration on the symmetric scaling D^-1/2 A D^-1/2 (same spectrum
+    // as D^-1 A) with a Rayleigh quotient estimate.
+    const int n = A.nr;
+    if (n == 0)
+        return 1.0;
+    const std::vector<double> d = diagonal(A);
+    std::vector<double> s(static_cast<size_t>(n)), x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+        s[size_t(i)] = d[size_t(i)] != 0.0 ? 1.0 / std::sqrt(std::abs(d[size_t(i)])) : 1.0;
+    uint64_t st = 0x9E3779B97F4A7C15ULL;
+    for (int i = 0; i < n; ++i) {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        x[size_t(i)] = double(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    const int threads = thread_count(0);
+    double lam = 1.0;
+    for (int it = 0; it < iters; ++it) {
+        parallel_for(n, threads, [&](int b, int e, int) {
+            for (int i = b; i < e; ++i) {
+                double acc = 0.0;
+                for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                    const int j = A.ci[size_t(k)];
+                    acc += A.v[size_t(k)] * s[size_t(j)] * x[size_t(j)];
+                }
+                y[size_t(i)] = s[size_t(i)] * acc;
+            }
+        });
+        double xy = 0.0, xx = 0.0, yy = 0.0;
+        for (int i = 0; i < n; ++i) {
+            xy += x[size_t(i)] * y[size_t(i)];
+            xx += x[size_t(i)] * x[size_t(i)];
+            yy += y[size_t(i)] * y[size_t(i)];
+        }
+        lam = xx > 0 ? std::abs(xy / xx) : 1.0;
+        const double inv = yy > 0 ? 1.0 / std::sqrt(yy) : 1.0;
+        for (int i = 0; i < n; ++i)
+            x[size_t(i)] = y[size_t(i)] * inv;
+    }
+    return lam;
+}
+
+void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int nagg, double omega,
+                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci)
+{
+    // With a strength pattern, the smoothing uses the filtered operator A_F:
+    // weak off-diagonal entries are dropped and added to the diagonal (row
+    // sums, hence A_F 1 = A 1, are preserved), so P spreads only along strong
+    // couplings and the Galerkin operators stay sparse.
+    const int n = A.nr;
+    const std::vector<double> d = diagonal(A);
+    P.nr = n;
+    P.nc = nagg;
+    P.rp.assign(size_t(n) + 1, 0);
+    P.ci.clear();
+    P.v.clear();
+    P.ci.reserve(size_t(n) * 8);
+    P.v.reserve(size_t(n) * 8);
+    std::vector<int> cols;
+    std::vector<double> acc(static_cast<size_t>(nagg), 0.0);
+    std::vector<char> used(static_cast<size_t>(nagg), 0);
+    std::vector<char> strong(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; ++i) {
+        cols.clear();
+        auto add = [&](int c, double v) {
+            if (!used[size_t(c)]) {
+                used[size_t(c)] = 1;
+                acc[size_t(c)] = 0.0;
+                cols.push_back(c);
+            }
+            acc[size_t(c)] += v;
+        };
+        double dF = d[size_t(i)];
+        if (srp) {
+            for (int s = (*srp)[size_t(i)]; s < (*srp)[size_t(i) + 1]; ++s)
+                strong[size_t((*sci)[size_t(s)])] = 1;
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                const int j = A.ci[size_t(k)];
+                if (j != i && !strong[size_t(j)])
+                    dF += A.v[size_t(k)];
+            }
+        }
+        if (agg[size_t(i)] >= 0)
+            add(agg[size_t(i)], 1.0);
+        const double w = dF != 0.0 ? omega / dF : 0.0;
+        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+            const int j = A.ci[size_t(k)];
+            const int a = agg[size_t(j)];
+            if (a < 0)
+                continue;
+            if (j == i)
+                add(a, -w * dF);
+            else if (!srp || strong[size_t(j)])
+                add(a, -w * A.v[size_t(k)]);
+        }
+        if (srp)
+            for (int s = (*srp)[size_t(i)]; s < (*srp)[size_t(i) + 1]; ++s)
+                strong[size_t((*sci)[size_t(s)])] = 0;
+        std::sort(cols.begin(), cols.end());
+        for (int c : cols) {
+            used[size_t(c)] = 0;
+            if (acc[size_t(c)] != 0.0) {
+                P.ci.push_back(c);
+                P.v.push_back(acc[size_t(c)]);
+            }
+        }
+        P.rp[size_t(i) + 1] = int(P.ci.size());
+    }
+}
+
+void csr_transpose(const HCsr& A, HCsr& T)
+{
+    T.nr = A.nc;
+    T.nc = A.nr;
+    T.rp.assign(size_t(A.nc) + 1, 0);
+    for (int c : A.ci)
+        ++T.rp[size_t(c) + 1];
+    for (int i = 0; i < A.nc; ++i)
+        T.rp[size_t(i) + 1] += T.rp[size_t(i)];
+    T.ci.resize(A.ci.size());
+    T.v.resize(A.v.size());
+    std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
+    for (int i = 0; i < A.nr; ++i)
+        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+            const int c = A.ci[size_t(k)];
+            const int w = pos[size_t(c)]++;
+            T.ci[size_t(w)] = i;
+            T.v[size_t(w)] = A.v[size_t(k)];
+        }
+}
+
+void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads)
+{
+    // Row-wise Gustavson product with a sparse accumulator per thread; rows
+    // keep their columns sorted.
+    threads = thread_count(threads);
+    const int n = A.nr;
+    struct Part
+    {
+        std::vector<int> rp, ci;
+        std::vector<double> v;
+    };
+    const int parts = (threads > 1 && n >= 4096) ? threads : 1;
+    std::vector<Part> part(static_cast<size_t>(parts));
+    const int chunk = (n + parts - 1) / parts;
+    auto work = [&](int b, int e, int t) {
+        Part& P = part[size_t(t)];
+        P.rp.assign(1, 0);
+        std::vector<int> mark(size_t(B.nc), -1);
+        std::vector<double> acc(size_t(B.nc), 0.0);
+        std::vector<int> cols;
+        for (int i = b; i < e; ++i) {
+            cols.clear();
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                const int r = A.ci[size_t(k)];
+                const double a = A.v[size_t(k)];
+                for (int q = B.rp[size_t(r)]; q < B.rp[size_t(r) + 1]; ++q) {
+                    const int c = B.ci[size_t(q)];
+                    if (mark[size_t(c)] != i) {
+                        mark[size_t(c)] = i;
+                        acc[size_t(c)] = 0.0;
+                        cols.push_back(c);
+                    }
+                    acc[size_t(c)] += a * B.v[size_t(q)];
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            for (int c : cols) {
+                P.ci.push_back(c);
+                P.v.push_back(acc[size_t(c)]);
+            }
+            P.rp.push_back(int(P.ci.size()));
+        }
+    };
+    if (parts == 1) {
+        work(0, n, 0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < parts; ++t) {
+            const int b = t * chunk, e = std::min(n, b + chunk);
+            pool.emplace_back([&, b, e, t] {
+                if (b < e)
+                    work(b, e, t);
+                else
+                    part[size_t(t)].rp.assign(1, 0);
+            });
+        }
+        for (auto& th : pool)
+            th.join();
+    }
+    C.nr = n;
+    C.nc = B.nc;
+    size_t total = 0;
+    for (auto& p : part)
+        total += p.ci.size();
+    C.rp.assign(size_t(n) + 1, 0);
+    C.ci.resize(total);
+    C.v.resize(total);
+    size_t off = 0;
+    int row = 0;
+    for (auto& p : part) {
+        const int rows = int(p.rp.size()) - 1;
+        for (int r = 0; r < rows; ++r)
+            C.rp[size_t(row + r) + 1] = int(off) + p.rp[size_t(r) + 1];
+        std::memcpy(C.ci.data() + off, p.ci.data(), p.ci.size() * sizeof(int));
+        std::memcpy(C.v.data() + off, p.v.data(), p.v.size() * sizeof(double));
+        off += p.ci.size();
+        row += rows;
+    }
+}
+
+int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv)
+{
+    // Gauss-Jordan elimination with partial pivoting on [A | I].  A pivot that
+    // vanishes (a singular coarse operator, e.g. an unpinned Neumann problem)
+    // is replaced by a tiny multiple of the largest diagonal so the coarse
+    // correction stays finite; the Krylov outer iteration absorbs it.
+    inv.assign(size_t(n) * size_t(n), 0.0);
+    for (int i = 0; i < n; ++i)
+        inv[size_t(i) * size_t(n) + size_t(i)] = 1.0;
+    double amax = 0.0;
+    for (int i = 0; i < n; ++i)
+        amax = std::max(amax, std::abs(a[size_t(i) * size_t(n) + size_t(i)]));
+    int regularised = 0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::abs(a[size_t(r) * size_t(n) + size_t(c)]) >
+                std::abs(a[size_t(piv) * size_t(n) + size_t(c)]))
+                piv = r;
+        if (piv != c) {
+            for (int k = 0; k < n; ++k) {
+                std::swap(a[size_t(c) * size_t(n) + size_t(k)], a[size_t(piv) * size_t(n) + size_t(k)]);
+                std::swap(inv[size_t(c) * size_t(n) + size_t(k)],
+                          inv[size_t(piv) * size_t(n) + size_t(k)]);
+            }
+        }
+        double p = a[size_t(c) * size_t(n) + size_t(c)];
+        if (std::abs(p) <= 1e-13 * amax) {
+            p = (p >= 0 ? 1.0 : -1.0) * 1e-10 * (amax > 0 ? amax : 1.0);
+            a[size_t(c) * size_t(n) + size_t(c)] = p;
+            ++regularised;
+        }
+        const double ip = 1.0 / p;
+        for (int k = 0; k < n; ++k) {
+            a[size_t(c) * size_t(n) + size_t(k)] *= ip;
+            inv[size_t(c) * size_t(n) + size_t(k)] *= ip;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c)
+                continue;
+            const double f = a[size_t(r) * size_t(n) + size_t(c)];
+            if (f == 0.0)
+                continue;
+            for (int k = 0; k < n; ++k) {
+                a[size_t(r) * size_t(n) + size_t(k)] -= f * a[size_t(c) * size_t(n) + size_t(k)];
+                inv[size_t(r) * size_t(n) + size_t(k)] -= f * inv[size_t(c) * size_t(n) + size_t(k)];
+            }
+        }
+    }
+    return regularised;
+}
+
+int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
+{
+    out.levels.clear();
+    out.levels.emplace_back();
+    out.levels.back().A = A0;
+    const int threads = thread_count(p.threads);
+    int64_t nnz_total = A0.nnz(), rows_total = A0.nr;
+    for (int lev = 0; lev + 1 < p.max_levels; ++lev) {
+        AmgLevelHost& L = out.levels.back();
+        const HCsr& A = L.A;
+        if (A.nr <= p.coarse_size)
+            break;
+        std::vector<int> srp, sci, agg;
+        amg_strength(A, p.theta, srp, sci);
+        int nagg = amg_aggregate(A.nr, srp, sci, A, agg);
+        if (nagg >= A.nr * 9 / 10 && p.theta > 0.0) {
+            // too few strong couplings to coarsen: treat every coupling as strong
+            amg_strength(A, 0.0, srp, sci);
+            nagg = amg_aggregate(A.nr, srp, sci, A, agg);
+        }
+        if (nagg <= 0 || nagg >= A.nr * 9 / 10)
+            break; // no useful coarsening
+        L.lmax = amg_lambda_max(A, 20) * 1.05;
+        L.n_aggregates = nagg;
+        const double omega = p.omega_scale / L.lmax;
+        if (p.filter)
+            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, &srp, &sci);
+        else
+            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, nullptr, nullptr);
+        csr_transpose(L.P, L.R);
+        HCsr AP, Ac;
+        csr_multiply(A, L.P, AP, threads);
+        csr_multiply(L.R, AP, Ac, threads);
+        out.levels.emplace_back();
+        out.levels.back().A = std::move(Ac);
+        nnz_total += out.levels.back().A.nnz();
+        rows_total += out.levels.back().A.nr;
+    }
+    AmgLevelHost& C = out.levels.back();
+    C.lmax = amg_lambda_max(C.A, 20) * 1.05;
+    const int nc = C.A.nr;
+    std::vector<double> dense(size_t(nc) * size_t(nc), 0.0);
+    for (int i = 0; i < nc; ++i)
+        for (int k = C.A.rp[size_t(i)]; k < C.A.rp[size_t(i) + 1]; ++k)
+            dense[size_t(i) * size_t(nc) + size_t(C.A.ci[size_t(k)])] += C.A.v[size_t(k)];
+    dense_inverse(nc, dense, out.coarse_inverse);
+    out.coarse_n = nc;
+    out.operator_complexity = double(nnz_total) / double(std::max<int64_t>(A0.nnz(), 1));
+    out.grid_complexity = double(rows_total) / double(std::max(A0.nr, 1));
+    return 0;
+}
+
+} // namespace mi

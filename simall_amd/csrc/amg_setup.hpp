@@ -1,0 +1,61 @@
+// amg_setup.hpp — smoothed-aggregation AMG hierarchy construction (host, FP64).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace mi
+{
+
+// Host CSR with nr rows and nc columns.
+struct HCsr
+{
+    int nr = 0, nc = 0;
+    std::vector<int> rp{0};
+    std::vector<int> ci;
+    std::vector<double> v;
+    int64_t nnz() const { return int64_t(ci.size()); }
+};
+
+struct AmgParams
+{
+    int max_levels = 10;
+    int coarse_size = 400;
+    double theta = 0.02;        // strength: |a_ij| >= theta * sqrt(|a_ii a_jj|)
+    double omega_scale = 4.0 / 3.0; // prolongator smoothing weight * lambda_max(D^-1 A)
+    int threads = 0;            // 0: hardware concurrency (at most 16)
+    bool filter = true;         // smooth P with the filtered operator (weak entries lumped)
+};
+
+struct AmgLevelHost
+{
+    HCsr A;          // level operator
+    HCsr P, R;       // prolongator to this level from the next, R = P^T (empty on the coarsest)
+    double lmax = 0; // estimate of lambda_max(D^-1 A)
+    int n_aggregates = 0;
+};
+
+struct AmgHierarchyHost
+{
+    std::vector<AmgLevelHost> levels;
+    std::vector<double> coarse_inverse; // dense row-major inverse of the coarsest A
+    int coarse_n = 0;
+    double operator_complexity = 0.0;
+    double grid_complexity = 0.0;
+};
+
+// Builds the hierarchy from a square FP64 matrix.  Returns 0 on success.
+int amg_build(const HCsr& A, const AmgParams& p, AmgHierarchyHost& out);
+
+// Pieces, exposed for tests of the construction.
+void amg_strength(const HCsr& A, double theta, std::vector<int>& srp, std::vector<int>& sci);
+int amg_aggregate(int n, const std::vector<int>& srp, const std::vector<int>& sci,
+                  const HCsr& A, std::vector<int>& agg);
+void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int nagg, double omega,
+                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci);
+void csr_transpose(const HCsr& A, HCsr& T);
+void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads);
+double amg_lambda_max(const HCsr& A, int iters);
+int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv);
+
+} // namespace mi

@@ -7,6 +7,8 @@
 // CG solves A d = r to `inner_tolerance`, x += d in FP64, repeat until the
 // FP64 true residual meets `tolerance`.
 #include "solver.hpp"
+#include "amg.hpp"
+#include "reorder.hpp"
 
 #include <chrono>
 #include <cmath>
@@ -18,6 +20,31 @@ namespace mi
 {
 
 // ----------------------------------------------------------------- Jacobi
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_scale(int n, const T* __restrict__ a,
+                                                  const T* __restrict__ r, T* __restrict__ z)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+        z[i] = a[i] * r[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int n, const double* __restrict__ a,
+                                                   float* __restrict__ b)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+        b[i] = float(a[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f64(int n, const float* __restrict__ a,
+                                                   double* __restrict__ b)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+        b[i] = double(a[i]);
+}
+
 namespace
 {
 struct JacobiPrecond final : Precond
@@ -43,6 +70,24 @@ struct JacobiPrecond final : Precond
     }
     const double* inv_diag64() const override { return has64 ? inv64.p : nullptr; }
     const float* inv_diag32() const override { return has32 ? inv32.p : nullptr; }
+    int apply64(const double* r, double* z, hipStream_t s) override
+    {
+        if (!has64)
+            return set_error(MI_ERR_UNSUPPORTED, "no FP64 inverse diagonal");
+        const int n = int(inv64.n);
+        k_scale<double><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, inv64.p, r, z);
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
+    int apply32(const float* r, float* z, hipStream_t s) override
+    {
+        if (!has32)
+            return set_error(MI_ERR_UNSUPPORTED, "no FP32 inverse diagonal");
+        const int n = int(inv32.n);
+        k_scale<float><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, inv32.p, r, z);
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
     const char* name() const override { return "jacobi"; }
 };
 } // namespace
@@ -104,6 +149,35 @@ __global__ __launch_bounds__(kBlock) void k_norm(int n, const double* __restrict
         out[0] = sqrt(tot[0]);
 }
 
+// dst[i] = src[perm[i]]
+__global__ __launch_bounds__(kBlock) void k_gather(int n, const int* __restrict__ perm,
+                                                   const double* __restrict__ src,
+                                                   double* __restrict__ dst)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+        dst[i] = src[perm[i]];
+}
+
+// dst[perm[i]] = src[i]
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, const int* __restrict__ perm,
+                                                    const double* __restrict__ src,
+                                                    double* __restrict__ dst)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n)
+        dst[perm[i]] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_values(int64_t nnz, const int* __restrict__ vmap,
+                                                          const double* __restrict__ src,
+                                                          double* __restrict__ dst)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t k = int64_t(blockIdx.x) * kBlock + threadIdx.x; k < nnz; k += stride)
+        dst[k] = src[vmap[k]];
+}
+
 } // namespace mi
 
 mi_solver::~mi_solver()
@@ -124,14 +198,14 @@ using Clock = std::chrono::steady_clock;
 int true_residual(mi_solver* S, const double* b, const double* x, double* r64, float* r32,
                   double* out_dev)
 {
-    const Pattern& P = *S->A->pat;
+    const Pattern& P = *S->M->pat;
     const int n = int(P.n);
     const unsigned grid = range_grid(n, kBlock);
     hipStream_t s = S->stream;
     switch (P.lanes) {
 #define MI_CASE(L)                                                                               \
     case L:                                                                                      \
-        k_true_residual<L><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->A->val.p, x, b, r64, r32, \
+        k_true_residual<L><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->M->val.p, x, b, r64, r32, \
                                                    S->partials.p, S->counter.p, out_dev);        \
         break;
         MI_CASE(2)
@@ -140,7 +214,7 @@ int true_residual(mi_solver* S, const double* b, const double* x, double* r64, f
         MI_CASE(16)
         MI_CASE(32)
     default:
-        k_true_residual<64><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->A->val.p, x, b, r64, r32,
+        k_true_residual<64><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->M->val.p, x, b, r64, r32,
                                                     S->partials.p, S->counter.p, out_dev);
         break;
 #undef MI_CASE
@@ -159,7 +233,7 @@ int read_scalars(mi_solver* S, int count)
 
 int ensure_workspace(mi_solver* S)
 {
-    const size_t n = size_t(S->A->n());
+    const size_t n = size_t(S->M->n());
     MI_TRY(S->r.alloc(n));
     MI_TRY(S->z.alloc(n));
     MI_TRY(S->p.alloc(n));
@@ -187,27 +261,95 @@ int ensure_workspace(mi_solver* S)
     return MI_OK;
 }
 
+bool want_reorder(const mi_solver* S)
+{
+    const int r = S->cfg.reorder;
+    return r > 0 || (r < 0 && S->A->n() >= 100000);
+}
+
+// Computes the reverse Cuthill-McKee permutation of A's pattern and the
+// reordered working matrix (once per pattern).
+int build_reordering(mi_solver* S)
+{
+    mi_csr* A = S->A;
+    const Pattern& src = *A->pat;
+    auto ro = std::make_unique<Reordering>();
+    ro->source = A->pat;
+    const int n = int(src.n);
+    rcm_order(n, src.h_rp.data(), src.h_ci.data(), ro->perm);
+    std::vector<int> brp, bci;
+    permute_pattern(n, src.h_rp.data(), src.h_ci.data(), ro->perm, brp, bci, ro->vmap);
+    ro->bandwidth_before = bandwidth(n, src.h_rp.data(), src.h_ci.data());
+    ro->bandwidth_after = bandwidth(n, brp.data(), bci.data());
+    MI_TRY(make_pattern(n, src.nnz, brp.data(), bci.data(), ro->pat, S->stream));
+    MI_TRY(ro->d_perm.upload(ro->perm.data(), ro->perm.size(), S->stream));
+    MI_TRY(ro->d_vmap.upload(ro->vmap.data(), ro->vmap.size(), S->stream));
+    mi_csr* Mo = nullptr;
+    MI_HIP(hipStreamSynchronize(S->stream));
+    MI_TRY(make_csr_on_pattern(ro->pat, &Mo));
+    S->Mown.reset(Mo);
+    S->ro = std::move(ro);
+    S->M = S->Mown.get();
+    S->pc.reset();
+    S->built_version = 0;
+    return MI_OK;
+}
+
 int build(mi_solver* S)
 {
     mi_csr* A = S->A;
     const bool mixed = S->cfg.precision == MI_PRECISION_MIXED;
+    const auto t0 = Clock::now();
+    if (want_reorder(S)) {
+        if (!S->ro || S->ro->source != A->pat)
+            MI_TRY(build_reordering(S));
+    } else if (S->M != A) {
+        S->ro.reset();
+        S->Mown.reset();
+        S->M = A;
+        S->pc.reset();
+        S->built_version = 0;
+    }
     if (S->pc && S->built_version == A->version)
         return MI_OK;
-    const auto t0 = Clock::now();
+    mi_csr* M = S->M;
+    if (S->ro) {
+        // M = P A P^T: gather the values on the device (assembly leaves them there)
+        const int64_t nnz = A->nnz();
+        if (nnz) {
+            k_gather_values<<<reduce_grid(nnz, 4), kBlock, 0, S->stream>>>(nnz, S->ro->d_vmap.p,
+                                                                          A->val.p, M->val.p);
+            MI_HIP(hipGetLastError());
+        }
+        M->val32_valid = false;
+        ++M->version;
+        if (A->host_valid) {
+            M->h_val.resize(size_t(nnz));
+            for (int64_t k = 0; k < nnz; ++k)
+                M->h_val[size_t(k)] = A->h_val[size_t(S->ro->vmap[size_t(k)])];
+            M->host_valid = true;
+        } else {
+            MI_HIP(hipStreamSynchronize(S->stream));
+            M->host_valid = false;
+        }
+    }
     if (mixed)
-        MI_TRY(A->ensure_fp32(S->stream));
+        MI_TRY(M->ensure_fp32(S->stream));
     if (!S->pc) {
         switch (S->cfg.preconditioner) {
+        case MI_PC_AMG:
+            S->pc = make_amg(S->cfg);
+            break;
         case MI_PC_ILU:
         case MI_PC_SSOR:
-        case MI_PC_AMG:
-            return set_error(MI_ERR_UNSUPPORTED, "preconditioner %d not available on device yet",
+            return set_error(MI_ERR_UNSUPPORTED, "preconditioner %d not available with CG on device yet",
                              S->cfg.preconditioner);
         default:
             S->pc = make_jacobi(); // Jacobi, None and unknown kinds (LinearSolvers.cpp:240-254)
         }
     }
-    MI_TRY(S->pc->setup(A, !mixed, mixed, S->stream));
+    MI_HIP(hipStreamSynchronize(S->stream));
+    MI_TRY(S->pc->setup(M, !mixed, mixed, S->stream));
     MI_TRY(ensure_workspace(S));
     MI_HIP(hipStreamSynchronize(S->stream));
     S->built_version = A->version;
@@ -268,7 +410,7 @@ template <class T>
 CgArgs<T> make_args(mi_solver* S, const T* val, const T* invD, const T* b, T* x, T* r, T* z, T* p,
                     T* Ap, double* history)
 {
-    const Pattern& P = *S->A->pat;
+    const Pattern& P = *S->M->pat;
     CgArgs<T> a{};
     a.n = int(P.n);
     a.nnz = P.nnz;
@@ -307,7 +449,7 @@ int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
     const mi_solver_cfg& c = S->cfg;
     Precond* pc = S->pc.get();
     const bool jac = pc->inv_diag64() != nullptr;
-    auto a = make_args<double>(S, S->A->val.p, pc->inv_diag64(), b, x, S->r.p, S->z.p, S->p.p,
+    auto a = make_args<double>(S, S->M->val.p, pc->inv_diag64(), b, x, S->r.p, S->z.p, S->p.p,
                                S->Ap.p, S->hist.p);
     const int n = a.n;
     bool warm = c.warm_start != 0;
@@ -370,14 +512,12 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     const mi_solver_cfg& c = S->cfg;
     Precond* pc = S->pc.get();
     const bool jac = pc->inv_diag32() != nullptr;
-    const int n = int(S->A->n());
+    const int n = int(S->M->n());
     hipStream_t s = S->stream;
-    if (!jac)
-        return set_error(MI_ERR_UNSUPPORTED, "mixed precision needs an FP32 preconditioner");
     // Inner system A d = r32: b = r32 (outer residual, FP32), x = x32 (the
     // correction d), r = b32 (inner residual), z = z32, p = p32, Ap = Ap32.
     MI_TRY(S->b32.alloc(size_t(n)));
-    auto a = make_args<float>(S, S->A->val32.p, pc->inv_diag32(), S->r32.p, S->x32.p, S->b32.p,
+    auto a = make_args<float>(S, S->M->val32.p, pc->inv_diag32(), S->r32.p, S->x32.p, S->b32.p,
                               S->z32.p, S->p32.p, S->Ap32.p, S->hist.p);
 
     const bool warm = c.warm_start != 0;
@@ -449,13 +589,31 @@ int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
         return MI_OK;
     }
     const auto t0 = Clock::now();
+    const int n = int(S->A->n());
+    const double* bw = b;
+    double* xw = x;
+    if (S->ro) {
+        // solve in the reordered numbering: b_w = P b, x_w = P x
+        MI_TRY(S->bw.alloc(size_t(n)));
+        MI_TRY(S->xw.alloc(size_t(n)));
+        k_gather<<<grid_for(n, kBlock), kBlock, 0, S->stream>>>(n, S->ro->d_perm.p, b, S->bw.p);
+        if (S->cfg.warm_start)
+            k_gather<<<grid_for(n, kBlock), kBlock, 0, S->stream>>>(n, S->ro->d_perm.p, x, S->xw.p);
+        MI_HIP(hipGetLastError());
+        bw = S->bw.p;
+        xw = S->xw.p;
+    }
     int rc;
     if (S->cfg.precision == MI_PRECISION_MIXED)
-        rc = solve_mixed(S, b, x, out);
+        rc = solve_mixed(S, bw, xw, out);
     else
-        rc = solve_fp64(S, b, x, out);
+        rc = solve_fp64(S, bw, xw, out);
     if (rc != MI_OK && rc != MI_ERR_NOT_CONVERGED && rc != MI_ERR_BREAKDOWN)
         return rc;
+    if (S->ro) {
+        k_scatter<<<grid_for(n, kBlock), kBlock, 0, S->stream>>>(n, S->ro->d_perm.p, S->xw.p, x);
+        MI_HIP(hipGetLastError());
+    }
     MI_HIP(hipStreamSynchronize(S->stream));
     out->solve_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
     return rc;
@@ -484,7 +642,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->check_interval = 0;
     c->amg_max_levels = 10;
     c->amg_coarse_size = 256;
-    c->amg_strength = 0.08;
+    c->amg_strength = 0.02;
     c->amg_jacobi_omega = 4.0 / 3.0;
     c->amg_smoother = 0;
     c->amg_sweeps = 2;
@@ -563,6 +721,65 @@ MI_API int mi_solver_solve(mi_solver* S, const double* b, double* x, mi_stats* s
         MI_HIP(hipStreamSynchronize(S->stream));
     }
     return rc;
+}
+
+MI_API int mi_solver_precond_apply(mi_solver* S, const double* r, double* z)
+{
+    clear_error();
+    if (!S || !S->A)
+        return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
+    const int n = int(S->A->n());
+    if (n && (!r || !z))
+        return set_error(MI_ERR_BAD_ARG, "null vector");
+    MI_TRY(ensure_device());
+    MI_TRY(build(S));
+    if (n == 0)
+        return MI_OK;
+    hipStream_t s = S->stream;
+    MI_TRY(S->b.alloc(size_t(n)));
+    MI_TRY(S->x.alloc(size_t(n)));
+    MI_TRY(S->bw.alloc(size_t(n)));
+    MI_TRY(S->xw.alloc(size_t(n)));
+    MI_HIP(hipMemcpyAsync(S->b.p, r, size_t(n) * sizeof(double), hipMemcpyHostToDevice, s));
+    const double* rw = S->b.p;
+    double* zw = S->ro ? S->xw.p : S->x.p;
+    if (S->ro) {
+        k_gather<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, S->ro->d_perm.p, S->b.p, S->bw.p);
+        rw = S->bw.p;
+    }
+    if (S->cfg.precision == MI_PRECISION_MIXED) {
+        DevBuf<float> r32, z32;
+        MI_TRY(r32.alloc(size_t(n)));
+        MI_TRY(z32.alloc(size_t(n)));
+        k_to_f32<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, rw, r32.p);
+        MI_TRY(S->pc->apply32(r32.p, z32.p, s));
+        k_to_f64<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, z32.p, zw);
+        MI_HIP(hipStreamSynchronize(s));
+    } else {
+        MI_TRY(S->pc->apply64(rw, zw, s));
+    }
+    if (S->ro)
+        k_scatter<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, S->ro->d_perm.p, S->xw.p, S->x.p);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipMemcpyAsync(z, S->x.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+MI_API int mi_solver_info(const mi_solver* S, int* levels, double* operator_complexity,
+                          int* bandwidth_before, int* bandwidth_after)
+{
+    if (!S)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    if (levels)
+        *levels = S->pc ? S->pc->levels() : 0;
+    if (operator_complexity)
+        *operator_complexity = S->pc ? S->pc->complexity() : 0.0;
+    if (bandwidth_before)
+        *bandwidth_before = S->ro ? S->ro->bandwidth_before : -1;
+    if (bandwidth_after)
+        *bandwidth_after = S->ro ? S->ro->bandwidth_after : -1;
+    return MI_OK;
 }
 
 MI_API double mi_solver_last_residual(const mi_solver* S) { return S ? S->last : 0.0; }

@@ -27,23 +27,40 @@ struct Precond
         return set_error(MI_ERR_UNSUPPORTED, "preconditioner has no FP32 apply");
     }
     virtual const char* name() const = 0;
+    virtual int levels() const { return 1; }
+    virtual double complexity() const { return 1.0; }
 };
 
 std::unique_ptr<Precond> make_jacobi();
+
+// Symmetric reordering of a pattern for the solver's working matrix.
+struct Reordering
+{
+    std::shared_ptr<Pattern> source;  // pattern the permutation was computed for
+    std::vector<int> perm;            // perm[new] = old
+    std::shared_ptr<Pattern> pat;     // reordered pattern
+    DevBuf<int> d_perm;
+    DevBuf<int> d_vmap;               // reordered entry k comes from source entry vmap[k]
+    std::vector<int> vmap;
+    int bandwidth_before = 0, bandwidth_after = 0;
+};
 
 } // namespace mi
 
 struct mi_solver
 {
     mi_solver_cfg cfg{};
-    mi_csr* A = nullptr;
+    mi_csr* A = nullptr;                 // the caller's matrix
+    mi_csr* M = nullptr;                 // working matrix: A, or Mown (reordered A)
+    std::unique_ptr<mi_csr> Mown;
+    std::unique_ptr<mi::Reordering> ro;
     uint64_t built_version = 0;
-    bool built64 = false, built32 = false;
     std::unique_ptr<mi::Precond> pc;
     hipStream_t stream = nullptr;
 
     // FP64 vectors
-    mi::DevBuf<double> b, x, r, z, p, Ap, hist, tmp;
+    mi::DevBuf<double> b, x, r, z, p, Ap, hist;
+    mi::DevBuf<double> bw, xw; // b and x in the working ordering
     // FP32 vectors of the inner iterations
     mi::DevBuf<float> r32, z32, p32, Ap32, x32, b32;
     mi::DevBuf<mi::KState> st;

@@ -1,0 +1,116 @@
+"""Invariants of the SA-AMG hierarchy and the RCM ordering (CPU only).
+
+The north-star SA-AMG has no reference counterpart (the reference's AMG is
+Ruge-Stueben, src/solver/AMGPreconditioner.cpp); it is pinned by what the
+construction must satisfy: R = P^T, A_c = P^T A P (scipy product), P 1 = 1 for
+an operator with A 1 = 0, aggregates covering every row, the coarse inverse,
+symmetry of the cycle, and h-robust PCG convergence.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from simall_amd import capi
+from simall_amd import workload as wl
+
+from amg_model import VCycle, pcg
+
+
+def system(N, pin=True, permuted=True):
+    w = wl.poisson3d(N)
+    m = w.mesh
+    rp, ci = oracle.build_sparsity(m)
+    P = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    vals, _, _ = oracle.assemble_pressure(m, P, w.bc_of_zone, gamma=w.gamma)
+    A = sp.csr_matrix((vals, ci, rp), shape=(m.n_cells, m.n_cells))
+    if not pin:
+        A = A.tolil()
+        A[0, 0] = A[0, 0] / 1e6
+        A = A.tocsr()
+    b = w.b.copy()
+    b[0] = 0.0
+    return A, b
+
+
+@pytest.fixture(scope="module")
+def hier():
+    A, b = system(16)
+    return A, b, capi.AmgHost(A.indptr, A.indices, A.data)
+
+
+def test_restriction_is_prolongation_transpose(hier):
+    A, b, H = hier
+    assert H.levels >= 2
+    for l in range(H.levels - 1):
+        P, R = H.scipy(l, H.P), H.scipy(l, H.R)
+        assert (abs(P.T - R)).max() == 0.0
+
+
+def test_galerkin_operator(hier):
+    A, b, H = hier
+    assert abs(H.scipy(0, H.A) - A).max() == 0.0
+    for l in range(H.levels - 1):
+        Al, P, Ac = H.scipy(l, H.A), H.scipy(l, H.P), H.scipy(l + 1, H.A)
+        ref = (P.T @ Al @ P).toarray()
+        assert np.allclose(Ac.toarray(), ref, rtol=1e-12, atol=1e-12 * abs(ref).max())
+
+
+def test_coarse_inverse(hier):
+    A, b, H = hier
+    Ac = H.scipy(H.levels - 1, H.A).toarray()
+    inv = H.matrix(H.levels - 1, H.COARSE_INV)
+    assert np.allclose(inv @ Ac, np.eye(Ac.shape[0]), atol=1e-9)
+
+
+def test_prolongator_preserves_constants_on_neumann_operator():
+    # unpinned Neumann operator: A 1 = 0, so the smoothed prolongator keeps
+    # the constant vector: P 1 = 1 (every row aggregated)
+    A, _ = system(12, pin=False)
+    assert abs(A @ np.ones(A.shape[0])).max() < 1e-12
+    H = capi.AmgHost(A.indptr, A.indices, A.data)
+    P = H.scipy(0, H.P)
+    assert np.allclose(P @ np.ones(P.shape[1]), 1.0, atol=1e-12)
+    # every fine row has a nonzero in P (aggregates cover all rows)
+    assert (np.diff(P.indptr) > 0).all()
+
+
+def test_lambda_max_estimate(hier):
+    A, b, H = hier
+    d = A.diagonal()
+    S = sp.diags(1 / np.sqrt(d)) @ A @ sp.diags(1 / np.sqrt(d))
+    lam = spla.eigsh(S, k=1, which="LA", return_eigenvectors=False)[0]
+    est = H.info(0, H.A)[3]
+    assert 0.95 * lam <= est <= 1.2 * lam
+
+
+def test_vcycle_symmetric_and_spd(hier):
+    A, b, H = hier
+    M = VCycle(H, degree=2)
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal((2, A.shape[0]))
+    assert abs(u @ M(v) - v @ M(u)) <= 1e-12 * abs(u @ M(u))
+    assert u @ M(u) > 0
+
+
+@pytest.mark.parametrize("N", [12, 24])
+def test_amg_pcg_converges_fast(N):
+    A, b = system(N)
+    H = capi.AmgHost(A.indptr, A.indices, A.data)
+    x, its = pcg(A, b, VCycle(H, 2), tol=1e-8)
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-8
+    # Jacobi-PCG needs hundreds of iterations here
+    assert its <= 30
+
+
+def test_rcm_permutation():
+    A, _ = system(10)
+    perm, bw = capi.rcm_permutation(A.indptr, A.indices)
+    assert np.array_equal(np.sort(perm), np.arange(A.shape[0]))
+    B = A[perm][:, perm].tocoo()
+    assert np.abs(B.row - B.col).max() == bw
+    # a permuted 10^3 grid has bandwidth ~ n; RCM brings it near a level set
+    Ac = A.tocoo()
+    assert np.abs(Ac.row - Ac.col).max() > 5 * bw
+    assert bw <= 3 * 10 * 10
