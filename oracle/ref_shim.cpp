@@ -1,0 +1,363 @@
+// ref_shim.cpp — flat C entry points over the reference's own sources, built
+// only in a container where /root/reference exists (oracle/ref.mk), into
+// oracle/_ref/libsimall_ref.so.  TEST INFRASTRUCTURE: used by tests/ to pin
+// the oracle and the workload generator against the real code; never part of
+// the product.
+//
+// Reference code called (not copied): meshing::CartesianMesher / Mesh
+// (src/meshing/CartesianMesher.cpp, Connectivity.cpp, MeshStorage.cpp),
+// solver::SimpleAlgorithm assembly (src/solver/SimpleAlgorithm.cpp, through a
+// subclass that reaches its protected hooks), solver::CSRMatrix::spmv,
+// adjoint::solve_discrete_adjoint (src/adjoint/DiscreteAdjoint.cpp) and the
+// Ruge-Stueben AMGPreconditioner (src/solver/AMGPreconditioner.cpp).
+#include "adjoint/DiscreteAdjoint.hpp"
+#include "meshing/CartesianMesher.hpp"
+#include "solver/AMGPreconditioner.hpp"
+#include "solver/SimpleAlgorithm.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+
+using namespace simall;
+
+namespace
+{
+
+uint64_t splitmix64(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+double uniform01(uint64_t& s) { return double(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// A mesh from the reference's generator; interior nodes optionally moved by
+// the same seeded sequence as the build's workload generator, then the
+// reference's geometry pass.
+std::unique_ptr<meshing::Mesh> make_mesh(int Nx, int Ny, int Nz, const double* o, const double* L,
+                                         double perturb, uint64_t seed)
+{
+    meshing::CartesianGridSpec spec;
+    spec.origin = util::Vec3d{o[0], o[1], o[2]};
+    spec.extent = util::Vec3d{L[0], L[1], L[2]};
+    spec.Nx = Nx;
+    spec.Ny = Ny;
+    spec.Nz = Nz;
+    auto m = std::make_unique<meshing::Mesh>();
+    meshing::CartesianMesher(spec).generate(*m);
+    if (perturb > 0.0) {
+        const double dx = L[0] / Nx, dy = L[1] / Ny, dz = L[2] / Nz;
+        auto& N = m->nodes();
+        uint64_t s = seed;
+        const int Px = Nx + 1, Py = Ny + 1, Pz = Nz + 1;
+        for (int k = 0; k < Pz; ++k)
+            for (int j = 0; j < Py; ++j)
+                for (int i = 0; i < Px; ++i) {
+                    if (!(i > 0 && i < Nx && j > 0 && j < Ny && k > 0 && k < Nz))
+                        continue;
+                    const size_t id = size_t((k * Py + j) * Px + i);
+                    N.x[id] += (2.0 * uniform01(s) - 1.0) * perturb * dx;
+                    N.y[id] += (2.0 * uniform01(s) - 1.0) * perturb * dy;
+                    N.z[id] += (2.0 * uniform01(s) - 1.0) * perturb * dz;
+                }
+        m->compute_geometry();
+    }
+    return m;
+}
+
+class NullSolver final : public solver::ILinearSolver
+{
+public:
+    int solve(const solver::CSRMatrix&, const util::aligned_vector<double>&,
+              util::aligned_vector<double>&) override
+    {
+        return 0;
+    }
+    double last_residual() const override { return 0.0; }
+};
+
+// Reaches SimpleAlgorithm's protected assembly hooks and storage.
+class Probe final : public solver::SimpleAlgorithm
+{
+public:
+    using SimpleAlgorithm::SimpleAlgorithm;
+    void sparsity() { build_sparsity(); }
+    void pressure(const double* aP, const double* phi)
+    {
+        std::memcpy(aP_.data(), aP, aP_.size() * sizeof(double));
+        phiFlux_.assign(phi, phi + mesh_.faces().size());
+        assemble_pressure_correction();
+    }
+    void momentum(int comp, const double* phi, const double* Uold)
+    {
+        phiFlux_.assign(phi, phi + mesh_.faces().size());
+        auto& Un = comp == 0 ? Ux_n_ : comp == 1 ? Uy_n_ : Uz_n_;
+        if (Uold)
+            std::memcpy(Un.data(), Uold, Un.size() * sizeof(double));
+        assemble_momentum(comp);
+    }
+    void flux() { compute_face_fluxes(); }
+    const solver::CSRMatrix& Ap() const { return A_p_; }
+    const solver::CSRMatrix& Am() const { return A_mom_; }
+    const util::aligned_vector<double>& rhs() const { return rhs_; }
+    const util::aligned_vector<double>& phi() const { return phiFlux_; }
+};
+
+solver::BCType bc_type(int cls)
+{
+    switch (cls) {
+    case 0: return solver::BCType::Wall;
+    case 1: return solver::BCType::PressureOutlet;
+    case 2: return solver::BCType::Symmetry;
+    case 3: return solver::BCType::VelocityInlet;
+    default: return solver::BCType::OutFlow;
+    }
+}
+
+struct Case
+{
+    std::unique_ptr<meshing::Mesh> mesh;
+    solver::FieldRegistry fields;
+    std::vector<solver::BoundarySpec> bcs;
+    NullSolver s1, s2;
+    std::unique_ptr<Probe> probe;
+};
+
+std::unique_ptr<Case> make_case(int Nx, int Ny, int Nz, const double* o, const double* L,
+                                double perturb, uint64_t seed, const int* bc_of_zone, int nz,
+                                const double* bc_value, double rho, double mu, double dt, double urf,
+                                const double* U)
+{
+    auto c = std::make_unique<Case>();
+    c->mesh = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
+    const size_t nC = c->mesh->cells().size();
+    for (int z = 1; z < nz; ++z) {
+        solver::BoundarySpec b;
+        b.zone = meshing::ZoneId(z);
+        b.type = bc_type(bc_of_zone[z]);
+        for (int d = 0; d < 3; ++d)
+            b.vectorValue[d] = bc_value ? bc_value[z * 3 + d] : 0.0;
+        c->bcs.push_back(b);
+    }
+    auto& Uf = c->fields.vector("U", nC);
+    c->fields.scalar("p", nC);
+    if (U) {
+        std::memcpy(Uf.x.data(), U, nC * sizeof(double));
+        std::memcpy(Uf.y.data(), U + nC, nC * sizeof(double));
+        std::memcpy(Uf.z.data(), U + 2 * nC, nC * sizeof(double));
+    }
+    solver::SimpleOptions opt;
+    opt.rho = rho;
+    opt.mu = mu;
+    opt.dt = dt;
+    opt.urfU = urf;
+    opt.timeScheme = solver::TemporalScheme::ImplicitEuler;
+    c->probe = std::make_unique<Probe>(*c->mesh, c->fields, c->bcs, c->s1, c->s2, opt);
+    c->probe->sparsity();
+    return c;
+}
+
+void copy_csr(const solver::CSRMatrix& A, int32_t* rp, int32_t* ci, double* v)
+{
+    if (rp)
+        std::memcpy(rp, A.rowPtr.data(), A.rowPtr.size() * sizeof(int32_t));
+    if (ci)
+        std::memcpy(ci, A.colIdx.data(), A.colIdx.size() * sizeof(int32_t));
+    if (v)
+        std::memcpy(v, A.values.data(), A.values.size() * sizeof(double));
+}
+
+} // namespace
+
+extern "C" {
+
+// Sizes of the reference mesh: cells, faces, nnz of the SIMPLE pattern.
+int ref_mesh_sizes(int Nx, int Ny, int Nz, const double* o, const double* L, int64_t* nC,
+                   int64_t* nF, int64_t* nnz)
+{
+    const int bc[7] = {0, 0, 0, 0, 0, 0, 0};
+    auto c = make_case(Nx, Ny, Nz, o, L, 0.0, 0, bc, 7, nullptr, 1, 1, 0, 1, nullptr);
+    *nC = int64_t(c->mesh->cells().size());
+    *nF = int64_t(c->mesh->faces().size());
+    *nnz = int64_t(c->probe->Ap().colIdx.size());
+    return 0;
+}
+
+// Mesh arrays after the reference generator (+ optional perturbation and the
+// reference geometry pass).
+int ref_mesh_arrays(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                    uint64_t seed, uint64_t* owner, uint64_t* neighbor, double* area, double* fc,
+                    uint32_t* zone, double* vol, double* cc, int32_t* cfo, uint64_t* cfi)
+{
+    auto m = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
+    const auto& F = m->faces();
+    const auto& C = m->cells();
+    const size_t nF = F.size(), nC = C.size();
+    std::memcpy(owner, F.owner.data(), nF * 8);
+    std::memcpy(neighbor, F.neighbor.data(), nF * 8);
+    std::memcpy(area, F.areaX.data(), nF * 8);
+    std::memcpy(area + nF, F.areaY.data(), nF * 8);
+    std::memcpy(area + 2 * nF, F.areaZ.data(), nF * 8);
+    std::memcpy(fc, F.centroidX.data(), nF * 8);
+    std::memcpy(fc + nF, F.centroidY.data(), nF * 8);
+    std::memcpy(fc + 2 * nF, F.centroidZ.data(), nF * 8);
+    std::memcpy(zone, F.boundaryZone.data(), nF * 4);
+    std::memcpy(vol, C.volume.data(), nC * 8);
+    std::memcpy(cc, C.centroidX.data(), nC * 8);
+    std::memcpy(cc + nC, C.centroidY.data(), nC * 8);
+    std::memcpy(cc + 2 * nC, C.centroidZ.data(), nC * 8);
+    std::memcpy(cfo, C.faceOffsets.data(), (nC + 1) * 4);
+    std::memcpy(cfi, C.faceIndices.data(), size_t(C.faceOffsets[nC]) * 8);
+    return 0;
+}
+
+// SimpleAlgorithm::assemble_pressure_correction on the reference mesh.
+int ref_assemble_pressure(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                          uint64_t seed, const int* bc_of_zone, int nz, double rho,
+                          const double* aP, const double* phi, int32_t* rp, int32_t* ci,
+                          double* vals, double* rhs)
+{
+    auto c = make_case(Nx, Ny, Nz, o, L, perturb, seed, bc_of_zone, nz, nullptr, rho, 1e-3, 0.0,
+                       1.0, nullptr);
+    c->probe->pressure(aP, phi);
+    copy_csr(c->probe->Ap(), rp, ci, vals);
+    std::memcpy(rhs, c->probe->rhs().data(), c->probe->rhs().size() * sizeof(double));
+    return 0;
+}
+
+// compute_face_fluxes + assemble_momentum(comp) (p = 0) on the reference mesh.
+int ref_assemble_momentum(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                          uint64_t seed, const int* bc_of_zone, int nz, const double* bc_value,
+                          double rho, double mu, double dt, double urf, const double* U, int comp,
+                          const double* Uold, double* phi, double* vals, double* rhs)
+{
+    auto c = make_case(Nx, Ny, Nz, o, L, perturb, seed, bc_of_zone, nz, bc_value, rho, mu, dt, urf,
+                       U);
+    c->probe->flux();
+    const size_t nF = c->mesh->faces().size();
+    std::memcpy(phi, c->probe->phi().data(), nF * sizeof(double));
+    c->probe->momentum(comp, phi, Uold);
+    copy_csr(c->probe->Am(), nullptr, nullptr, vals);
+    std::memcpy(rhs, c->probe->rhs().data(), c->probe->rhs().size() * sizeof(double));
+    return 0;
+}
+
+// solver::CSRMatrix::spmv
+int ref_spmv(int n, const int32_t* rp, const int32_t* ci, const double* v, const double* x, double* y)
+{
+    solver::CSRMatrix A;
+    A.rowPtr.assign(rp, rp + n + 1);
+    A.colIdx.assign(ci, ci + rp[n]);
+    A.values.assign(v, v + rp[n]);
+    util::aligned_vector<double> X(x, x + n), Y;
+    A.spmv(X, Y);
+    std::memcpy(y, Y.data(), size_t(n) * sizeof(double));
+    return 0;
+}
+
+// adjoint::solve_discrete_adjoint with (dR/dalpha)^T psi supplied as a
+// dense m x n row-major matrix G: out = G psi.
+int ref_discrete_adjoint(int n, const int32_t* rp, const int32_t* ci, const double* v,
+                         const double* dJdQ, int m, const double* dJdAlpha, const double* G,
+                         int maxIter, int restart, double tol, double* psi, double* grad,
+                         int* iterations, double* residual)
+{
+    adjoint::CsrMatrix A;
+    A.n = size_t(n);
+    A.rowPtr.assign(rp, rp + n + 1);
+    A.colIdx.assign(ci, ci + rp[n]);
+    A.values.assign(v, v + rp[n]);
+    adjoint::DiscreteAdjointInputs in;
+    in.A = &A;
+    in.dJdQ.assign(dJdQ, dJdQ + n);
+    if (dJdAlpha)
+        in.dJdAlpha.assign(dJdAlpha, dJdAlpha + m);
+    // G psi needs psi: the reference calls the callable after solving, and
+    // the result's psi is only returned afterwards, so the callable reads it
+    // through this pointer once the solve has filled it.
+    const std::vector<double>* psiRef = nullptr;
+    if (G)
+        in.dRdAlphaTransposeTimes = [&](std::vector<double>& out) {
+            for (int i = 0; i < m; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < n; ++j)
+                    s += G[size_t(i) * size_t(n) + size_t(j)] * (psiRef ? (*psiRef)[size_t(j)] : 0.0);
+                out[size_t(i)] = s;
+            }
+        };
+    adjoint::DiscreteAdjointOptions opt;
+    opt.maxIter = size_t(maxIter);
+    opt.restart = size_t(restart);
+    opt.tol = tol;
+    // First pass solves for psi; the gradient callable then sees psi.
+    auto r1 = adjoint::solve_discrete_adjoint(in, size_t(m), opt);
+    psiRef = &r1.psi;
+    auto r = adjoint::solve_discrete_adjoint(in, size_t(m), opt);
+    std::memcpy(psi, r.psi.data(), size_t(n) * sizeof(double));
+    if (grad)
+        std::memcpy(grad, r.gradient.data(), size_t(m) * sizeof(double));
+    *iterations = int(r.iterations);
+    *residual = r.residual;
+    return r.ok ? 0 : 1;
+}
+
+// CG (the loop of CGSolver::solve, src/solver/LinearSolvers.cpp:420-452)
+// preconditioned by the reference's Ruge-Stueben AMGPreconditioner.
+int ref_rs_amg_pcg(int n, const int32_t* rp, const int32_t* ci, const double* v, const double* b,
+                   double tol, int maxIter, double* x, double* last, double* setup_seconds)
+{
+    solver::CSRMatrix A;
+    A.rowPtr.assign(rp, rp + n + 1);
+    A.colIdx.assign(ci, ci + rp[n]);
+    A.values.assign(v, v + rp[n]);
+    solver::AMGPreconditioner M;
+    const auto t0 = std::chrono::steady_clock::now();
+    M.setup(A);
+    *setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    using Vec = util::aligned_vector<double>;
+    const size_t nn = static_cast<size_t>(n);
+    Vec X(nn, 0.0), r(b, b + n), z(nn), p, Ap(nn);
+    auto dot = [&](const Vec& a, const Vec& c) {
+        double s = 0;
+        for (int i = 0; i < n; ++i)
+            s += a[size_t(i)] * c[size_t(i)];
+        return s;
+    };
+    M.apply(r, z);
+    p = z;
+    double rz = dot(r, z);
+    Vec B(b, b + n);
+    const double bnorm = std::max(std::sqrt(dot(B, B)), 1e-30);
+    int it = 0;
+    for (; it < maxIter; ++it) {
+        A.spmv(p, Ap);
+        const double pAp = dot(p, Ap);
+        if (pAp == 0)
+            break;
+        const double alpha = rz / pAp;
+        for (int i = 0; i < n; ++i) {
+            X[size_t(i)] += alpha * p[size_t(i)];
+            r[size_t(i)] -= alpha * Ap[size_t(i)];
+        }
+        *last = std::sqrt(dot(r, r)) / bnorm;
+        if (*last < tol) {
+            ++it;
+            break;
+        }
+        M.apply(r, z);
+        const double rzNew = dot(r, z);
+        const double beta = rzNew / rz;
+        for (int i = 0; i < n; ++i)
+            p[size_t(i)] = z[size_t(i)] + beta * p[size_t(i)];
+        rz = rzNew;
+    }
+    std::memcpy(x, X.data(), size_t(n) * sizeof(double));
+    return it;
+}
+
+} // extern "C"
