@@ -98,6 +98,8 @@ PROTOTYPES = {
     "mi_xpby": (_i, [_vp, _d, _vp, _sz, _vp]),
     "mi_spmv_csr": (_i, [_sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_jacobi_apply": (_i, [_vp, _vp, _vp, _sz, _vp]),
+    "mi_ilu0_factor_inplace": (_i, [_sz, _vp, _vp, _vp, C.POINTER(_i)]),
+    "mi_ilu0_apply": (_i, [_sz, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mi_csr_create": (_i, [_i64, _i64, _vp, _vp, _vp, C.POINTER(_vp)]),
     "mi_csr_update_values": (_i, [_vp, _vp]),
     "mi_csr_update_values_device": (_i, [_vp, _vp]),

@@ -1,0 +1,40 @@
+// ilu0.hpp — ILU(0) factorization and level-scheduled device solves.
+#pragma once
+
+#include "solver.hpp"
+
+#include <vector>
+
+namespace mi
+{
+
+// In-place ILU(0) of CSR values (sorted columns) on the host; rows with a
+// zero or missing pivot are skipped as in the reference.  zero_pivots counts
+// zero pivots met (including zero final diagonals).
+int ilu0_factor_host(int n, const int* rp, const int* ci, std::vector<double>& lu,
+                     std::vector<int>& diag, int* zero_pivots);
+
+// Dependency levels of the L and U sweeps: rows of level l are
+// rows[off[l] .. off[l+1]).  lend[i] = first position of row i with col >= i.
+void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& lend,
+                 std::vector<int>& lrows, std::vector<int>& loff, std::vector<int>& urows,
+                 std::vector<int>& uoff);
+
+template <class T> struct Ilu0
+{
+    int n = 0;
+    int zero_pivots = 0;
+    const int* rp_dev = nullptr;
+    const int* ci_dev = nullptr;
+    DevBuf<T> d_lu, d_y;
+    DevBuf<int> d_lrows, d_urows, d_lend, d_diag;
+    std::vector<int> h_loff, h_uoff;
+    bool level_built = false;
+    const Pattern* levels_for = nullptr;
+    int setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s);
+    int apply(const T* r, T* z, hipStream_t s);
+};
+
+std::unique_ptr<Precond> make_ilu0();
+
+} // namespace mi

@@ -297,6 +297,218 @@ template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s)
     return MI_OK;
 }
 
+// --------------------------------------------------------------- BiCGStab
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bi_init(BiArgs<T> a, int warm, double tol, int max_iter)
+{
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[2] = {0.0, 0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T bi = a.b[i];
+        T ri;
+        if (warm) {
+            ri = a.r[i];
+        } else {
+            ri = bi;
+            a.x[i] = T(0);
+            a.r[i] = ri;
+        }
+        a.rh[i] = ri;
+        a.p[i] = T(0);
+        a.v[i] = T(0);
+        v[0] += double(ri) * double(ri);
+        v[1] += double(bi) * double(bi);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        const double bn = sqrt(tot[1]);
+        st->bnorm = bn > 1e-30 ? bn : 1e-30;
+        st->tol = tol;
+        st->max_iter = max_iter;
+        st->iter = 0;
+        st->rz = tot[0];          // rho_new = rh.r with rh = r
+        st->scratch[0] = 1.0;     // rho
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        st->beta = tot[0];        // (rho_new / rho) * (alpha / omega)
+        st->last = sqrt(tot[0]) / st->bnorm;
+        st->done = max_iter > 0 ? 0 : 2;
+        if (tot[0] == 0.0 && max_iter > 0)
+            st->done = 3; // rho_new == 0 at it = 0 (LinearSolvers.cpp:364-367)
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bi_p(BiArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T be = T(a.st->beta), om = T(a.st->omega);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
+        a.p[i] = a.r[i] + be * (a.p[i] - om * a.v[i]);
+}
+
+// out = A in ; dots in.w? : mode 0: v = A y, accumulate rh.v ; mode 1: t = A z, accumulate t.s, t.t
+template <int LANES, class T>
+__global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
+{
+    if (a.st->done)
+        return;
+    constexpr int kRows = kBlock / LANES;
+    const RowRange rr = block_rows(a.n, gridDim.x, kRows);
+    const int sub = threadIdx.x / LANES;
+    const int lane = threadIdx.x & (LANES - 1);
+    const T* in = mode == 0 ? a.y : a.z;
+    T* out = mode == 0 ? a.v : a.t;
+    double v[2] = {0.0, 0.0};
+    for (int base = rr.beg; base < rr.end; base += kRows) {
+        const int row = base + sub;
+        const bool mine = row < rr.end;
+        const auto sres = csr_row_dot<LANES>(mine ? row : a.n, a.n, lane, a.rp, a.ci, a.val, in);
+        if (mine && lane == 0) {
+            const T o = T(sres);
+            out[row] = o;
+            if (mode == 0) {
+                v[0] += double(a.rh[row]) * double(o);
+            } else {
+                v[0] += double(o) * double(a.s[row]);
+                v[1] += double(o) * double(o);
+            }
+        }
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        if (mode == 0) {
+            st->alpha = st->rz / tot[0];
+        } else {
+            st->omega = tot[0] / (tot[1] > 1e-30 ? tot[1] : 1e-30);
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bi_s(BiArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T al = T(a.st->alpha);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[1] = {0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        const T si = a.r[i] - al * a.v[i];
+        a.s[i] = si;
+        a.x[i] += al * a.y[i];
+        v[0] += double(si) * double(si);
+    }
+    double tot[1];
+    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        const double sn = sqrt(tot[0]);
+        if (sn < st->tol * st->bnorm) {
+            // early exit on ||s|| (LinearSolvers.cpp:386-391)
+            st->last = sn / st->bnorm;
+            st->iter += 1;
+            if (a.history)
+                a.history[st->iter - 1] = st->last;
+            st->done = 1;
+        }
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_bi_update(BiArgs<T> a)
+{
+    if (a.st->done)
+        return;
+    const T om = T(a.st->omega);
+    const RowRange rr = block_rows(a.n, gridDim.x, 1);
+    double v[2] = {0.0, 0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        a.x[i] += om * a.z[i];
+        const T ri = a.s[i] - om * a.t[i];
+        a.r[i] = ri;
+        v[0] += double(ri) * double(ri);
+        v[1] += double(a.rh[i]) * double(ri);
+    }
+    double tot[2];
+    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
+        KState* st = a.st;
+        const double rho = st->rz;
+        st->scratch[0] = rho;
+        st->rz = tot[1];
+        st->last = sqrt(tot[0]) / st->bnorm;
+        const int it = st->iter + 1;
+        st->iter = it;
+        if (a.history)
+            a.history[it - 1] = st->last;
+        if (st->last < st->tol)
+            st->done = 1;
+        else if (it >= st->max_iter)
+            st->done = 2;
+        else if (tot[1] == 0.0)
+            st->done = 3;
+        else
+            st->beta = (tot[1] / rho) * (st->alpha / st->omega);
+    }
+}
+
+template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,
+                                  hipStream_t s)
+{
+    k_bi_init<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0, tol, max_iter);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s)
+{
+    k_bi_p<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> static int bi_spmv(const BiArgs<T>& a, int mode, hipStream_t s)
+{
+    switch (a.lanes) {
+    case 2: k_bi_spmv<2, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    case 4: k_bi_spmv<4, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    case 8: k_bi_spmv<8, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    case 16: k_bi_spmv<16, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    case 32: k_bi_spmv<32, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    default: k_bi_spmv<64, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
+    }
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int bicg_stage2(const BiArgs<T>& a, hipStream_t s)
+{
+    MI_TRY(bi_spmv(a, 0, s));
+    k_bi_s<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template <class T> int bicg_stage3(const BiArgs<T>& a, hipStream_t s)
+{
+    MI_TRY(bi_spmv(a, 1, s));
+    k_bi_update<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+#define MI_INSTANTIATE_BI(T)                                                                     \
+    template int bicg_init<T>(const BiArgs<T>&, bool, double, int, hipStream_t);                 \
+    template int bicg_stage1<T>(const BiArgs<T>&, hipStream_t);                                  \
+    template int bicg_stage2<T>(const BiArgs<T>&, hipStream_t);                                  \
+    template int bicg_stage3<T>(const BiArgs<T>&, hipStream_t);
+
+MI_INSTANTIATE_BI(double)
+MI_INSTANTIATE_BI(float)
+
 #define MI_INSTANTIATE_CG(T)                                                                     \
     template int cg_init_jacobi<T>(const CgArgs<T>&, bool, double, int, hipStream_t);            \
     template int cg_init_r<T>(const CgArgs<T>&, bool, hipStream_t);                              \

@@ -108,4 +108,31 @@ template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s);
 template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s);  // Ap, alpha, x, r, ||r||
 template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s);  // rz, beta, p
 
+// --------------------------------------------------------- BiCGStab
+// Right-preconditioned BiCGStab of BiCGSTABSolver::solve
+// (src/solver/LinearSolvers.cpp:360-404).  KState: rz = rho_new (rh.r),
+// scratch[0] = rho, alpha, omega, beta as in the reference.
+template <class T> struct BiArgs
+{
+    int n;
+    int64_t nnz;
+    const int* rp;
+    const int* ci;
+    const T* val;
+    const T* b;
+    T *x, *r, *rh, *p, *v, *s, *t, *y, *z;
+    KState* st;
+    double* partials;
+    unsigned* counter;
+    double* history;
+    int lanes;
+    unsigned grid;
+};
+
+template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,
+                                  hipStream_t s);
+template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s); // p update
+template <class T> int bicg_stage2(const BiArgs<T>& a, hipStream_t s); // v = A y, alpha, s, x += alpha y
+template <class T> int bicg_stage3(const BiArgs<T>& a, hipStream_t s); // t = A z, omega, x, r, rho
+
 } // namespace mi

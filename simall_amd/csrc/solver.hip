@@ -8,6 +8,7 @@
 // FP64 true residual meets `tolerance`.
 #include "solver.hpp"
 #include "amg.hpp"
+#include "ilu0.hpp"
 #include "reorder.hpp"
 
 #include <chrono>
@@ -234,10 +235,19 @@ int read_scalars(mi_solver* S, int count)
 int ensure_workspace(mi_solver* S)
 {
     const size_t n = size_t(S->M->n());
-    MI_TRY(S->r.alloc(n));
-    MI_TRY(S->z.alloc(n));
-    MI_TRY(S->p.alloc(n));
-    MI_TRY(S->Ap.alloc(n));
+    const bool mixed = S->cfg.precision == MI_PRECISION_MIXED;
+    // Krylov vectors: CG uses slots 0-3 (r z p Ap), BiCGStab 0-7
+    // (r rh p v s t y z); the FP32 pool adds 8 = outer residual, 9 = correction.
+    const int nv = S->cfg.kind == MI_SOLVER_BICGSTAB ? 8 : 4;
+    for (int k = 0; k < nv; ++k)
+        MI_TRY(S->w64[k].alloc(n));
+    if (mixed)
+        for (int k = 0; k < nv; ++k)
+            MI_TRY(S->w32[k].alloc(n));
+    if (mixed) {
+        MI_TRY(S->w32[8].alloc(n));
+        MI_TRY(S->w32[9].alloc(n));
+    }
     MI_TRY(S->st.alloc(1));
     MI_TRY(S->partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
     if (!S->counter.p) {
@@ -251,13 +261,6 @@ int ensure_workspace(mi_solver* S)
         MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->h_scal), 8 * sizeof(double)));
     const int hist = S->cfg.max_iterations > 0 ? S->cfg.max_iterations : 1;
     MI_TRY(S->hist.alloc(size_t(hist)));
-    if (S->cfg.precision == MI_PRECISION_MIXED) {
-        MI_TRY(S->r32.alloc(n));
-        MI_TRY(S->z32.alloc(n));
-        MI_TRY(S->p32.alloc(n));
-        MI_TRY(S->Ap32.alloc(n));
-        MI_TRY(S->x32.alloc(n));
-    }
     return MI_OK;
 }
 
@@ -341,9 +344,10 @@ int build(mi_solver* S)
             S->pc = make_amg(S->cfg);
             break;
         case MI_PC_ILU:
+            S->pc = make_ilu0();
+            break;
         case MI_PC_SSOR:
-            return set_error(MI_ERR_UNSUPPORTED, "preconditioner %d not available with CG on device yet",
-                             S->cfg.preconditioner);
+            return set_error(MI_ERR_UNSUPPORTED, "SSOR is not available on device");
         default:
             S->pc = make_jacobi(); // Jacobi, None and unknown kinds (LinearSolvers.cpp:240-254)
         }
@@ -364,55 +368,76 @@ int check_interval(const mi_solver* S, int n)
     return n < 200000 ? 32 : 8;
 }
 
-// Runs CG iterations on the device until KState::done; returns the final
-// state in S->h_st.
-template <class T>
-int run_cg(mi_solver* S, const CgArgs<T>& a, Precond* pc, bool jacobi, double tol, int max_iter,
-           bool warm)
+template <class T> int precond(Precond* pc, const T* r, T* z, hipStream_t s)
 {
-    hipStream_t s = S->stream;
-    if (jacobi) {
-        MI_TRY(cg_init_jacobi<T>(a, warm, tol, max_iter, s));
-    } else {
-        MI_TRY(cg_init_r<T>(a, warm, s));
-        if constexpr (sizeof(T) == 8)
-            MI_TRY(pc->apply64(a.r, a.z, s));
-        else
-            MI_TRY(pc->apply32(a.r, a.z, s));
-        MI_TRY(cg_init_p<T>(a, tol, max_iter, s));
-    }
-    const int chunk = check_interval(S, a.n);
+    if constexpr (sizeof(T) == 8)
+        return pc->apply64(r, z, s);
+    else
+        return pc->apply32(r, z, s);
+}
+
+// Polls KState every `chunk` issued iterations until the device says done.
+template <class Body> int iterate(mi_solver* S, KState* st, int max_iter, int chunk, Body&& body)
+{
     int issued = 0;
     for (;;) {
-        const int k = std::min(chunk, max_iter - issued);
-        for (int i = 0; i < k; ++i) {
-            if (jacobi) {
-                MI_TRY(cg_iteration_jacobi<T>(a, s));
-            } else {
-                MI_TRY(cg_iteration_part1<T>(a, s));
-                if constexpr (sizeof(T) == 8)
-                    MI_TRY(pc->apply64(a.r, a.z, s));
-                else
-                    MI_TRY(pc->apply32(a.r, a.z, s));
-                MI_TRY(cg_iteration_part2<T>(a, s));
-            }
-        }
-        issued += k;
-        MI_HIP(hipMemcpyAsync(S->h_st, a.st, sizeof(KState), hipMemcpyDeviceToHost, s));
-        MI_HIP(hipStreamSynchronize(s));
+        MI_HIP(hipMemcpyAsync(S->h_st, st, sizeof(KState), hipMemcpyDeviceToHost, S->stream));
+        MI_HIP(hipStreamSynchronize(S->stream));
         if (S->h_st->done || issued >= max_iter)
             break;
+        const int k = std::min(chunk, max_iter - issued);
+        for (int i = 0; i < k; ++i)
+            MI_TRY(body());
+        issued += k;
     }
     return MI_OK;
 }
 
+// One Krylov solve A x = b in element type T on the working matrix with the
+// vector pool w; the final KState is left in S->h_st.
 template <class T>
-CgArgs<T> make_args(mi_solver* S, const T* val, const T* invD, const T* b, T* x, T* r, T* z, T* p,
-                    T* Ap, double* history)
+int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevBuf<T>* w,
+               double tol, int max_iter, bool warm)
 {
     const Pattern& P = *S->M->pat;
+    hipStream_t s = S->stream;
+    Precond* pc = S->pc.get();
+    const int n = int(P.n);
+    const int chunk = check_interval(S, n);
+    if (S->cfg.kind == MI_SOLVER_BICGSTAB) {
+        BiArgs<T> a{};
+        a.n = n;
+        a.nnz = P.nnz;
+        a.rp = P.rp.p;
+        a.ci = P.ci.p;
+        a.val = val;
+        a.b = b;
+        a.x = x;
+        a.r = w[0].p;
+        a.rh = w[1].p;
+        a.p = w[2].p;
+        a.v = w[3].p;
+        a.s = w[4].p;
+        a.t = w[5].p;
+        a.y = w[6].p;
+        a.z = w[7].p;
+        a.st = S->st.p;
+        a.partials = S->partials.p;
+        a.counter = S->counter.p;
+        a.history = S->hist.p;
+        a.lanes = P.lanes;
+        a.grid = range_grid(n, 256);
+        MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
+        return iterate(S, a.st, max_iter, chunk, [&]() -> int {
+            MI_TRY(bicg_stage1<T>(a, s));
+            MI_TRY(precond<T>(pc, a.p, a.y, s));
+            MI_TRY(bicg_stage2<T>(a, s));
+            MI_TRY(precond<T>(pc, a.s, a.z, s));
+            return bicg_stage3<T>(a, s);
+        });
+    }
     CgArgs<T> a{};
-    a.n = int(P.n);
+    a.n = n;
     a.nnz = P.nnz;
     a.rp = P.rp.p;
     a.ci = P.ci.p;
@@ -420,25 +445,38 @@ CgArgs<T> make_args(mi_solver* S, const T* val, const T* invD, const T* b, T* x,
     a.invD = invD;
     a.b = b;
     a.x = x;
-    a.r = r;
-    a.z = z;
-    a.p = p;
-    a.Ap = Ap;
+    a.r = w[0].p;
+    a.z = w[1].p;
+    a.p = w[2].p;
+    a.Ap = w[3].p;
     a.st = S->st.p;
     a.partials = S->partials.p;
     a.counter = S->counter.p;
-    a.history = history;
+    a.history = S->hist.p;
     a.lanes = P.lanes;
-    a.grid = range_grid(P.n, 256);
-    return a;
+    a.grid = range_grid(n, 256);
+    const bool jacobi = invD != nullptr;
+    if (jacobi) {
+        MI_TRY(cg_init_jacobi<T>(a, warm, tol, max_iter, s));
+    } else {
+        MI_TRY(cg_init_r<T>(a, warm, s));
+        MI_TRY(precond<T>(pc, a.r, a.z, s));
+        MI_TRY(cg_init_p<T>(a, tol, max_iter, s));
+    }
+    return iterate(S, a.st, max_iter, chunk, [&]() -> int {
+        if (jacobi)
+            return cg_iteration_jacobi<T>(a, s);
+        MI_TRY(cg_iteration_part1<T>(a, s));
+        MI_TRY(precond<T>(pc, a.r, a.z, s));
+        return cg_iteration_part2<T>(a, s);
+    });
 }
 
-void copy_history(mi_solver* S, int count, int offset)
+void copy_history(mi_solver* S, int count)
 {
     if (count <= 0)
         return;
     const size_t old = S->history.size();
-    (void)offset;
     S->history.resize(old + size_t(count));
     (void)hipMemcpy(S->history.data() + old, S->hist.p, sizeof(double) * size_t(count),
                     hipMemcpyDeviceToHost);
@@ -448,27 +486,25 @@ int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
 {
     const mi_solver_cfg& c = S->cfg;
     Precond* pc = S->pc.get();
-    const bool jac = pc->inv_diag64() != nullptr;
-    auto a = make_args<double>(S, S->M->val.p, pc->inv_diag64(), b, x, S->r.p, S->z.p, S->p.p,
-                               S->Ap.p, S->hist.p);
-    const int n = a.n;
+    const double* invD = c.kind == MI_SOLVER_CG ? pc->inv_diag64() : nullptr;
+    const int n = int(S->M->n());
     bool warm = c.warm_start != 0;
     int total = 0, outer = 0;
     int status = MI_OK;
-    double bnorm = 0.0, rel_true = 0.0, init_res = 0.0;
-    // ||b|| and the initial residual
+    double rel_true = 0.0;
     k_norm<<<range_grid(n, kBlock), kBlock, 0, S->stream>>>(n, b, S->partials.p, S->counter.p,
                                                             S->dscal.p + 1);
     MI_HIP(hipGetLastError());
     if (warm)
-        MI_TRY(true_residual(S, b, x, S->r.p, nullptr, S->dscal.p + 2));
+        MI_TRY(true_residual(S, b, x, S->w64[0].p, nullptr, S->dscal.p + 2));
     MI_TRY(read_scalars(S, 3));
-    bnorm = std::max(S->h_scal[1], 1e-30);
-    init_res = warm ? S->h_scal[2] : S->h_scal[1];
+    const double bnorm = std::max(S->h_scal[1], 1e-30);
+    const double init_res = warm ? S->h_scal[2] : S->h_scal[1];
     for (;;) {
-        MI_TRY(run_cg<double>(S, a, pc, jac, c.tolerance, c.max_iterations - total, warm));
+        MI_TRY(run_krylov<double>(S, S->M->val.p, invD, b, x, S->w64, c.tolerance,
+                                  c.max_iterations - total, warm));
         const KState st = *S->h_st;
-        copy_history(S, st.iter, total);
+        copy_history(S, st.iter);
         total += st.iter;
         ++outer;
         S->last = st.last;
@@ -489,9 +525,9 @@ int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
             status = MI_ERR_NOT_CONVERGED;
             break;
         }
-        // Recurrence residual converged but the true residual did not:
-        // restart from the current x (refinement in FP64).
-        MI_TRY(true_residual(S, b, x, S->r.p, nullptr, S->dscal.p));
+        // The recurrence residual met the tolerance but the true residual did
+        // not: restart from the current x (FP64 refinement).
+        MI_TRY(true_residual(S, b, x, S->w64[0].p, nullptr, S->dscal.p));
         warm = true;
     }
     if (c.true_residual)
@@ -511,22 +547,18 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
 {
     const mi_solver_cfg& c = S->cfg;
     Precond* pc = S->pc.get();
-    const bool jac = pc->inv_diag32() != nullptr;
+    const float* invD = c.kind == MI_SOLVER_CG ? pc->inv_diag32() : nullptr;
     const int n = int(S->M->n());
     hipStream_t s = S->stream;
-    // Inner system A d = r32: b = r32 (outer residual, FP32), x = x32 (the
-    // correction d), r = b32 (inner residual), z = z32, p = p32, Ap = Ap32.
-    MI_TRY(S->b32.alloc(size_t(n)));
-    auto a = make_args<float>(S, S->M->val32.p, pc->inv_diag32(), S->r32.p, S->x32.p, S->b32.p,
-                              S->z32.p, S->p32.p, S->Ap32.p, S->hist.p);
-
+    float* r32 = S->w32[8].p; // outer residual b - A x in FP32
+    float* d32 = S->w32[9].p; // correction from the inner solve
     const bool warm = c.warm_start != 0;
     if (!warm)
         MI_HIP(hipMemsetAsync(x, 0, sizeof(double) * size_t(n), s));
     k_norm<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, b, S->partials.p, S->counter.p,
                                                     S->dscal.p + 1);
     MI_HIP(hipGetLastError());
-    MI_TRY(true_residual(S, b, x, nullptr, S->r32.p, S->dscal.p));
+    MI_TRY(true_residual(S, b, x, nullptr, r32, S->dscal.p));
     MI_TRY(read_scalars(S, 2));
     const double bnorm = std::max(S->h_scal[1], 1e-30);
     const double init_res = S->h_scal[0];
@@ -538,25 +570,24 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
             status = MI_ERR_NOT_CONVERGED;
             break;
         }
-        // inner: A d = r32 from d = 0, stop at inner_tol relative to ||r32||,
-        // or earlier if that already meets the outer target.
+        // inner: A d = r32 from d = 0 to inner_tol relative to ||r32||, or
+        // less when that already meets the outer target
         const double tol_in = std::max(inner_tol, 0.5 * c.tolerance * bnorm / rn);
-        MI_TRY(run_cg<float>(S, a, pc, jac, tol_in, c.max_iterations - total, false));
+        MI_TRY(run_krylov<float>(S, S->M->val32.p, invD, r32, d32, S->w32, tol_in,
+                                 c.max_iterations - total, false));
         const KState st = *S->h_st;
-        copy_history(S, st.iter, total);
+        copy_history(S, st.iter);
         total += st.iter;
         ++outer;
-        k_add_f32_to_f64<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, S->x32.p, x);
+        k_add_f32_to_f64<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, d32, x);
         MI_HIP(hipGetLastError());
-        MI_TRY(true_residual(S, b, x, nullptr, S->r32.p, S->dscal.p));
+        MI_TRY(true_residual(S, b, x, nullptr, r32, S->dscal.p));
         MI_TRY(read_scalars(S, 1));
-        const double rn_new = S->h_scal[0];
+        rn = S->h_scal[0];
         if (st.done == 3 && st.iter == 0) {
             status = MI_ERR_BREAKDOWN;
-            rn = rn_new;
             break;
         }
-        rn = rn_new;
     }
     S->last = rn / bnorm;
     out->iterations = total;
@@ -577,7 +608,7 @@ int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
     std::memset(out, 0, sizeof(*out));
     if (!S->A)
         return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
-    if (S->cfg.kind != MI_SOLVER_CG)
+    if (S->cfg.kind != MI_SOLVER_CG && S->cfg.kind != MI_SOLVER_BICGSTAB)
         return set_error(MI_ERR_UNSUPPORTED, "solver kind %d not available on device yet",
                          S->cfg.kind);
     MI_TRY(build(S));

@@ -58,11 +58,10 @@ struct mi_solver
     std::unique_ptr<mi::Precond> pc;
     hipStream_t stream = nullptr;
 
-    // FP64 vectors
-    mi::DevBuf<double> b, x, r, z, p, Ap, hist;
-    mi::DevBuf<double> bw, xw; // b and x in the working ordering
-    // FP32 vectors of the inner iterations
-    mi::DevBuf<float> r32, z32, p32, Ap32, x32, b32;
+    mi::DevBuf<double> b, x, hist; // host-API staging of b and x; residual history
+    mi::DevBuf<double> bw, xw;     // b and x in the working ordering
+    mi::DevBuf<double> w64[8];     // FP64 Krylov vectors
+    mi::DevBuf<float> w32[10];     // FP32 Krylov vectors of the inner iterations
     mi::DevBuf<mi::KState> st;
     mi::KState* h_st = nullptr; // pinned mirror of st
     mi::DevBuf<double> partials;
