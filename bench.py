@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--pc", default="jacobi", choices=["jacobi", "amg"])
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")
+    ap.add_argument("--degree", type=int, default=0, help="AMG smoother degree (0: default)")
+    ap.add_argument("--inner-tol", type=float, default=0.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -87,14 +90,22 @@ def main():
     pc = capi.MI_PC_AMG if args.pc == "amg" else capi.MI_PC_JACOBI
 
     def make_solver(precision):
+        kw = {}
+        if args.degree:
+            kw["amg_sweeps"] = args.degree
+        if args.inner_tol:
+            kw["inner_tolerance"] = args.inner_tol
         s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=pc, tolerance=1e-8,
-                        max_iterations=20000, precision=precision, true_residual=1)
+                        max_iterations=20000, precision=precision, true_residual=1, **kw)
         s.setup(A)
         return s
 
-    results = {}
-    for name, prec in (("fp64", capi.MI_PRECISION_FP64), ("mixed", capi.MI_PRECISION_MIXED)):
+    results, solvers = {}, {}
+    modes = [m for m in (("fp64", capi.MI_PRECISION_FP64), ("mixed", capi.MI_PRECISION_MIXED))
+             if m[0] in args.modes.split(",")]
+    for name, prec in modes:
         s = make_solver(prec)
+        solvers[name] = s
         for _ in range(args.warmup):
             st, rc = s.solve_device(d_b, d_x)
         lib.mi_device_synchronize()
@@ -113,23 +124,28 @@ def main():
             f"outer={stats['outer_iterations']} rel={stats['true_relative_residual']:.3e} "
             f"setup={stats['setup_seconds']:.3f}s rc={rc}")
 
-    # SpMV bandwidth (the dominant kernel), FP64 and the FP32 inner form
+    # SpMV bandwidth on the solver's working (reordered) matrix, FP64 and the
+    # FP32 inner form; and on the caller's (randomly numbered) matrix
+    ws = next(iter(solvers.values()))
+    us64, by64 = ws.time_spmv(capi.MI_PRECISION_FP64, reps=50)
+    us32, by32 = ws.time_spmv(capi.MI_PRECISION_MIXED, reps=50)
+    gbs64, gbs32 = by64 / us64 / 1e3, by32 / us32 / 1e3
     dx = capi.DeviceArray.from_host(np.ones(n))
     dy = capi.DeviceArray(n)
-    us64 = A.time_spmv(dx, dy, capi.MI_PRECISION_FP64, reps=50)
-    dx32 = capi.DeviceArray.from_host(np.ones(n, np.float32))
-    dy32 = capi.DeviceArray(n, np.float32)
-    us32 = A.time_spmv(dx32, dy32, capi.MI_PRECISION_MIXED, reps=50)
-    by64, by32 = spmv_bytes(n, nnz, 8), spmv_bytes(n, nnz, 4)
-    gbs64, gbs32 = by64 / us64 / 1e3, by32 / us32 / 1e3
-    log(f"SpMV fp64 {us64:.1f} us {gbs64:.0f} GB/s ({100 * gbs64 / HBM_PEAK_GBS:.1f}%); "
-        f"fp32 {us32:.1f} us {gbs32:.0f} GB/s")
-
-    x64, xm = results["fp64"]["x"], results["mixed"]["x"]
-    err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    us_raw = A.time_spmv(dx, dy, capi.MI_PRECISION_FP64, reps=20)
+    gbs_raw = spmv_bytes(n, nnz, 8) / us_raw / 1e3
+    info = ws.info()
+    log(f"SpMV (reordered, bandwidth {info['bandwidth_before']} -> {info['bandwidth_after']}) "
+        f"fp64 {us64:.1f} us {gbs64:.0f} GB/s ({100 * gbs64 / HBM_PEAK_GBS:.1f}%); fp32 {us32:.1f} us "
+        f"{gbs32:.0f} GB/s; caller ordering fp64 {us_raw:.1f} us {gbs_raw:.0f} GB/s; "
+        f"AMG levels {info['levels']} complexity {info['operator_complexity']:.2f}")
+    err = None
+    if "fp64" in results and "mixed" in results:
+        x64, xm = results["fp64"]["x"], results["mixed"]["x"]
+        err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and "fp64" in results:
         rp, ci = A.pattern()
         vals = A.values()
         its = results["fp64"]["stats"]["iterations"]
@@ -140,8 +156,9 @@ def main():
                          f"of the FP64 GPU solve"}
         log(f"cpu baseline: {cpu}")
 
-    mixed_ms = results["mixed"]["ms"]
-    fp64_ms = results["fp64"]["ms"]
+    mixed_ms = results["mixed"]["ms"] if "mixed" in results else float("nan")
+    fp64_ms = results["fp64"]["ms"] if "fp64" in results else float("nan")
+    get = lambda m, k: results[m]["stats"][k] if m in results else None
     out = {
         "metric": "pressure-solve time to 1e-8 rel residual, mixed vs FP64 GPU; SpMV HBM GB/s",
         "value": round(mixed_ms, 3),
@@ -162,14 +179,17 @@ def main():
         },
         "fp64_ms": round(fp64_ms, 3),
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
-        "iterations": {"fp64": results["fp64"]["stats"]["iterations"],
-                       "mixed_inner": results["mixed"]["stats"]["iterations"],
-                       "mixed_outer": results["mixed"]["stats"]["outer_iterations"]},
-        "true_relative_residual": {"fp64": results["fp64"]["stats"]["true_relative_residual"],
-                                   "mixed": results["mixed"]["stats"]["true_relative_residual"]},
+        "iterations": {"fp64": get("fp64", "iterations"), "mixed_inner": get("mixed", "iterations"),
+                       "mixed_outer": get("mixed", "outer_iterations")},
+        "true_relative_residual": {"fp64": get("fp64", "true_relative_residual"),
+                                   "mixed": get("mixed", "true_relative_residual")},
+        "setup_s": {"fp64": get("fp64", "setup_seconds"), "mixed": get("mixed", "setup_seconds")},
         "mixed_vs_fp64_solution_rel_diff": err,
+        "amg": {"levels": info["levels"], "operator_complexity": info["operator_complexity"]},
         "spmv": {"fp64_us": round(us64, 2), "fp64_GBps": round(gbs64, 1),
-                 "fp32_us": round(us32, 2), "fp32_GBps": round(gbs32, 1)},
+                 "fp32_us": round(us32, 2), "fp32_GBps": round(gbs32, 1),
+                 "caller_order_fp64_GBps": round(gbs_raw, 1),
+                 "bandwidth": [info["bandwidth_before"], info["bandwidth_after"]]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

@@ -195,6 +195,12 @@ MI_API int mi_solver_precond_apply(mi_solver* s, const double* r_host, double* z
  * before and after the solver's internal reordering (-1: not reordered). */
 MI_API int mi_solver_info(const mi_solver* s, int* levels, double* operator_complexity,
                           int* bandwidth_before, int* bandwidth_after);
+/* Times `reps` SpMVs with the solver's working matrix (after its internal
+ * reordering) on the library stream with HIP events: average microseconds per
+ * launch and the algorithmic bytes of one launch (values + columns + row
+ * pointers + x once + y). */
+MI_API int mi_solver_time_spmv(mi_solver* s, int precision, int reps, double* avg_us,
+                               double* bytes);
 /* ILinearSolver::last_residual (src/solver/Solver.hpp:99). */
 MI_API double mi_solver_last_residual(const mi_solver* s);
 /* Relative residual after each Krylov iteration of the last solve. */

@@ -118,6 +118,7 @@ PROTOTYPES = {
     "mi_solver_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
     "mi_solver_last_residual": (_d, [_vp]),
     "mi_solver_precond_apply": (_i, [_vp, _vp, _vp]),
+    "mi_solver_time_spmv": (_i, [_vp, _i, _i, _pd, _pd]),
     "mi_solver_info": (_i, [_vp, C.POINTER(_i), _pd, C.POINTER(_i), C.POINTER(_i)]),
     "mi_solver_history": (_i, [_vp, _pd, _i, C.POINTER(_i)]),
     "mi_solver_destroy": (_i, [_vp]),
@@ -320,6 +321,11 @@ class Solver:
         z = np.empty_like(r)
         check(self.lib.mi_solver_precond_apply(self.h, ptr(r), ptr(z)))
         return z
+
+    def time_spmv(self, precision=MI_PRECISION_FP64, reps=50):
+        us, by = C.c_double(), C.c_double()
+        check(self.lib.mi_solver_time_spmv(self.h, precision, reps, C.byref(us), C.byref(by)))
+        return us.value, by.value
 
     def info(self):
         lv, oc, b0, b1 = C.c_int(), C.c_double(), C.c_int(), C.c_int()

@@ -41,62 +41,70 @@ __global__ __launch_bounds__(kBlock) void k_cheb_init(int n, const T* __restrict
 
 // mode 0: r -= A din ; dout = beta din + alpha D^-1 r ; x += dout
 // mode 1: r -= A din
-template <int LANES, class T>
-__global__ __launch_bounds__(kBlock) void k_cheb_step(int n, const int* __restrict__ rp,
-                                                      const int* __restrict__ ci,
-                                                      const T* __restrict__ val,
-                                                      const T* __restrict__ din,
+template <class T> struct EpiCheb
+{
+    const T* din;
+    T* dout;
+    T* r;
+    const T* invD;
+    T* x;
+    T beta, alpha;
+    int mode;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        const T ri = r[i] - T(s);
+        r[i] = ri;
+        if (mode == 0) {
+            const T dn = beta * din[i] + alpha * invD[i] * ri;
+            dout[i] = dn;
+            x[i] += dn;
+        }
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cheb_step(StreamMat<T> A, const T* __restrict__ din,
                                                       T* __restrict__ dout, T* __restrict__ r,
                                                       const T* __restrict__ invD,
                                                       T* __restrict__ x, T beta, T alpha, int mode)
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
-    const int row = int(gid / LANES);
-    const int lane = int(gid & (LANES - 1));
-    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, din);
-    if (row < n && lane == 0) {
-        const T ri = r[row] - T(s);
-        r[row] = ri;
-        if (mode == 0) {
-            const T dn = beta * din[row] + alpha * invD[row] * ri;
-            dout[row] = dn;
-            x[row] += dn;
-        }
-    }
+    EpiCheb<T> e{din, dout, r, invD, x, beta, alpha, mode};
+    stream_rows(A, din, e);
 }
 
-template <int LANES, class T>
-__global__ __launch_bounds__(kBlock) void k_prolong(int n, const int* __restrict__ rp,
-                                                    const int* __restrict__ ci,
-                                                    const T* __restrict__ val,
-                                                    const T* __restrict__ xc, T* __restrict__ e,
-                                                    T* __restrict__ x)
+// e = P x_c ; x += e
+template <class T> struct EpiProlong
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
-    const int row = int(gid / LANES);
-    const int lane = int(gid & (LANES - 1));
-    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, xc);
-    if (row < n && lane == 0) {
-        e[row] = T(s);
-        x[row] += T(s);
+    T* e;
+    T* x;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        e[i] = T(s);
+        x[i] += T(s);
     }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong(StreamMat<T> P, const T* __restrict__ xc,
+                                                    T* __restrict__ e, T* __restrict__ x)
+{
+    EpiProlong<T> ep{e, x};
+    stream_rows(P, xc, ep);
 }
 
-template <int LANES, class T>
-__global__ __launch_bounds__(kBlock) void k_restrict(int n, const int* __restrict__ rp,
-                                                     const int* __restrict__ ci,
-                                                     const T* __restrict__ val,
-                                                     const T* __restrict__ r, T* __restrict__ bc)
+// b_c = R r
+template <class T> struct EpiStoreT
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int64_t gid = int64_t(lb) * kBlock + threadIdx.x;
-    const int row = int(gid / LANES);
-    const int lane = int(gid & (LANES - 1));
-    const auto s = csr_row_dot<LANES>(row, n, lane, rp, ci, val, r);
-    if (row < n && lane == 0)
-        bc[row] = T(s);
+    T* y;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(StreamMat<T> R, const T* __restrict__ r,
+                                                     T* __restrict__ bc)
+{
+    EpiStoreT<T> e{bc};
+    stream_rows(R, r, e);
 }
 
 // x = Ainv b, dense FP64, one wavefront per row.
@@ -127,16 +135,6 @@ static unsigned rows_grid(int n, int lanes)
     return (g + 7) / 8 * 8;
 }
 
-#define MI_LANES_SWITCH(lanes, KNAME, T, ...)                                                      \
-    switch (lanes) {                                                                               \
-    case 2: KNAME<2, T><<<rows_grid(n, 2), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
-    case 4: KNAME<4, T><<<rows_grid(n, 4), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
-    case 8: KNAME<8, T><<<rows_grid(n, 8), kBlock, 0, s>>>(__VA_ARGS__); break;                   \
-    case 16: KNAME<16, T><<<rows_grid(n, 16), kBlock, 0, s>>>(__VA_ARGS__); break;                \
-    case 32: KNAME<32, T><<<rows_grid(n, 32), kBlock, 0, s>>>(__VA_ARGS__); break;                \
-    default: KNAME<64, T><<<rows_grid(n, 64), kBlock, 0, s>>>(__VA_ARGS__); break;                \
-    }
-
 template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s)
 {
     nr = h.nr;
@@ -147,18 +145,17 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s)
     MI_TRY(ci.upload(h.ci.data(), h.ci.size(), s));
     std::vector<T> tv(h.v.begin(), h.v.end());
     MI_TRY(v.upload(tv.data(), tv.size(), s));
-    MI_HIP(hipStreamSynchronize(s)); // tv is released on return
+    MI_TRY(rb.upload(nr, h.rp.data(), s)); // synchronizes: tv is released on return
     return MI_OK;
 }
 
 template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r, const T* invD,
                                    T* x, double beta, double alpha, int mode, hipStream_t s)
 {
-    const int n = A.nr;
-    if (n == 0)
+    if (A.nr == 0)
         return MI_OK;
-    MI_LANES_SWITCH(A.lanes, k_cheb_step, T, n, A.rp.p, A.ci.p, A.v.p, din, dout, r, invD, x,
-                    T(beta), T(alpha), mode)
+    k_cheb_step<T><<<stream_grid(A.rb.nrb), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
+                                                              T(alpha), mode);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -233,12 +230,9 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s)
             std::swap(din, dout);
         }
         MI_TRY(step<T>(d.A, din, nullptr, d.r.p, d.invD.p, x, 0.0, 0.0, 1, s));
-        const int nr = d.R.nr;
-        {
-            const int n = nr;
-            MI_LANES_SWITCH(d.R.lanes, k_restrict, T, nr, d.R.rp.p, d.R.ci.p, d.R.v.p, d.r.p,
-                            L[size_t(l) + 1].b.p)
-        }
+        if (d.R.nr > 0)
+            k_restrict<T><<<stream_grid(d.R.rb.nrb), kBlock, 0, s>>>(d.R.sm(), d.r.p,
+                                                                      L[size_t(l) + 1].b.p);
         MI_HIP(hipGetLastError());
     }
     // coarsest level
@@ -257,7 +251,8 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s)
         const int n = d.A.nr;
         T* x = xptr(l);
         const T* xc = xptr(l + 1);
-        MI_LANES_SWITCH(d.P.lanes, k_prolong, T, n, d.P.rp.p, d.P.ci.p, d.P.v.p, xc, d.e.p, x)
+        if (n > 0)
+            k_prolong<T><<<stream_grid(d.P.rb.nrb), kBlock, 0, s>>>(d.P.sm(), xc, d.e.p, x);
         MI_HIP(hipGetLastError());
         // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps
         MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, s));

@@ -3,6 +3,7 @@
 
 #include "amg_setup.hpp"
 #include "solver.hpp"
+#include "stream.hpp"
 
 #include <vector>
 
@@ -16,7 +17,11 @@ template <class T> struct DMat
     int lanes = 8;
     DevBuf<int> rp, ci;
     DevBuf<T> v;
+    RowBlocks rb;
     int upload(const HCsr& h, hipStream_t s);
+#if defined(__HIPCC__)
+    StreamMat<T> sm() const { return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p}; }
+#endif
 };
 
 template <class T> struct DLevel

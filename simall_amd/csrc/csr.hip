@@ -80,6 +80,8 @@ int make_pattern(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
         p->h_rp.assign(1, 0);
     }
     p->lanes = lanes_for(n, nnz);
+    if (n > 0)
+        MI_TRY(p->rb.upload(int(n), p->h_rp.data(), s));
     out = std::move(p);
     return MI_OK;
 }
@@ -280,8 +282,7 @@ MI_API int mi_csr_spmv(mi_csr* A, const double* x, double* y)
     DevBuf<double> dx, dy;
     MI_TRY(dx.upload(x, size_t(n), s));
     MI_TRY(dy.alloc(size_t(n)));
-    MI_TRY(launch_spmv_f64(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val.p, dx.p, dy.p, s,
-                           A->pat->lanes));
+    MI_TRY(launch_spmv_stream<double>(*A->pat, A->val.p, dx.p, dy.p, s));
     MI_HIP(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
@@ -293,14 +294,12 @@ static int spmv_device_impl(mi_csr* A, int precision, const void* x, void* y, hi
     if (n == 0)
         return MI_OK;
     if (precision == MI_PRECISION_FP64)
-        return launch_spmv_f64(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val.p,
-                               static_cast<const double*>(x), static_cast<double*>(y), s,
-                               A->pat->lanes);
+        return launch_spmv_stream<double>(*A->pat, A->val.p, static_cast<const double*>(x),
+                                          static_cast<double*>(y), s);
     if (precision == MI_PRECISION_MIXED) {
         MI_TRY(A->ensure_fp32(s));
-        return launch_spmv_f32(int(n), A->nnz(), A->pat->rp.p, A->pat->ci.p, A->val32.p,
-                               static_cast<const float*>(x), static_cast<float*>(y), s,
-                               A->pat->lanes);
+        return launch_spmv_stream<float>(*A->pat, A->val32.p, static_cast<const float*>(x),
+                                         static_cast<float*>(y), s);
     }
     return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);
 }

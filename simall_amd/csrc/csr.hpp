@@ -2,6 +2,7 @@
 #pragma once
 
 #include "mi_common.hpp"
+#include "stream.hpp"
 
 #include <memory>
 #include <vector>
@@ -16,6 +17,7 @@ struct Pattern
     std::vector<int> h_rp, h_ci;
     DevBuf<int> rp, ci;
     DevBuf<int> diag; // position of a_ii in each row, -1 if absent
+    RowBlocks rb;     // CSR-stream row blocks
     int lanes = 8;
 };
 } // namespace mi

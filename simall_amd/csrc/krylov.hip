@@ -8,6 +8,7 @@
 // Scalars live in KState on the device; each reduction is finished by the last
 // block of its launch, so the host only polls `done` every few iterations.
 #include "krylov.hpp"
+#include "stream.hpp"
 
 namespace mi
 {
@@ -97,26 +98,29 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_p(CgArgs<T> a, double tol, i
 }
 
 // -------------------------------------------------------------- iteration
-template <int LANES, class T>
+template <class T> struct EpiCgAp
+{
+    const T* p;
+    T* Ap;
+    double acc = 0.0;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        const T ap = T(s);
+        Ap[i] = ap;
+        acc += double(p[i]) * double(ap);
+    }
+};
+
+// Ap = A p ; alpha = rz / p.Ap
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
 {
     if (a.st->done)
         return;
-    constexpr int kRows = kBlock / LANES;
-    const RowRange rr = block_rows(a.n, gridDim.x, kRows);
-    const int sub = threadIdx.x / LANES;
-    const int lane = threadIdx.x & (LANES - 1);
-    double v[1] = {0.0};
-    for (int base = rr.beg; base < rr.end; base += kRows) {
-        const int row = base + sub;
-        const bool mine = row < rr.end;
-        const auto s = csr_row_dot<LANES>(mine ? row : a.n, a.n, lane, a.rp, a.ci, a.val, a.p);
-        if (mine && lane == 0) {
-            const T ap = T(s);
-            a.Ap[row] = ap;
-            v[0] += double(a.p[row]) * double(ap);
-        }
-    }
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val};
+    EpiCgAp<T> e{a.p, a.Ap};
+    stream_rows(A, a.p, e);
+    double v[1] = {e.acc};
     double tot[1];
     if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
         KState* st = a.st;
@@ -258,14 +262,7 @@ template <class T> int cg_init_p(const CgArgs<T>& a, double tol, int max_iter, h
 
 template <class T> static int launch_spmv_dot(const CgArgs<T>& a, hipStream_t s)
 {
-    switch (a.lanes) {
-    case 2: k_cg_spmv_dot<2, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    case 4: k_cg_spmv_dot<4, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    case 8: k_cg_spmv_dot<8, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    case 16: k_cg_spmv_dot<16, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    case 32: k_cg_spmv_dot<32, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    default: k_cg_spmv_dot<64, T><<<a.grid, kBlock, 0, s>>>(a); break;
-    }
+    k_cg_spmv_dot<T><<<stream_grid(a.nrb), kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -350,42 +347,42 @@ __global__ __launch_bounds__(kBlock) void k_bi_p(BiArgs<T> a)
         a.p[i] = a.r[i] + be * (a.p[i] - om * a.v[i]);
 }
 
-// out = A in ; dots in.w? : mode 0: v = A y, accumulate rh.v ; mode 1: t = A z, accumulate t.s, t.t
-template <int LANES, class T>
+template <class T> struct EpiBi
+{
+    const T* w1; // rh (mode 0) or s (mode 1)
+    T* out;
+    int mode;
+    double acc0 = 0.0, acc1 = 0.0;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type sres)
+    {
+        const T o = T(sres);
+        out[i] = o;
+        if (mode == 0) {
+            acc0 += double(w1[i]) * double(o);
+        } else {
+            acc0 += double(o) * double(w1[i]);
+            acc1 += double(o) * double(o);
+        }
+    }
+};
+
+// mode 0: v = A y, alpha = rho_new / rh.v ; mode 1: t = A z, omega = t.s / max(t.t, 1e-30)
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
 {
     if (a.st->done)
         return;
-    constexpr int kRows = kBlock / LANES;
-    const RowRange rr = block_rows(a.n, gridDim.x, kRows);
-    const int sub = threadIdx.x / LANES;
-    const int lane = threadIdx.x & (LANES - 1);
-    const T* in = mode == 0 ? a.y : a.z;
-    T* out = mode == 0 ? a.v : a.t;
-    double v[2] = {0.0, 0.0};
-    for (int base = rr.beg; base < rr.end; base += kRows) {
-        const int row = base + sub;
-        const bool mine = row < rr.end;
-        const auto sres = csr_row_dot<LANES>(mine ? row : a.n, a.n, lane, a.rp, a.ci, a.val, in);
-        if (mine && lane == 0) {
-            const T o = T(sres);
-            out[row] = o;
-            if (mode == 0) {
-                v[0] += double(a.rh[row]) * double(o);
-            } else {
-                v[0] += double(o) * double(a.s[row]);
-                v[1] += double(o) * double(o);
-            }
-        }
-    }
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val};
+    EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
+    stream_rows(A, mode == 0 ? a.y : a.z, e);
+    double v[2] = {e.acc0, e.acc1};
     double tot[2];
     if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
         KState* st = a.st;
-        if (mode == 0) {
+        if (mode == 0)
             st->alpha = st->rz / tot[0];
-        } else {
+        else
             st->omega = tot[0] / (tot[1] > 1e-30 ? tot[1] : 1e-30);
-        }
     }
 }
 
@@ -472,14 +469,7 @@ template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s)
 
 template <class T> static int bi_spmv(const BiArgs<T>& a, int mode, hipStream_t s)
 {
-    switch (a.lanes) {
-    case 2: k_bi_spmv<2, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    case 4: k_bi_spmv<4, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    case 8: k_bi_spmv<8, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    case 16: k_bi_spmv<16, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    case 32: k_bi_spmv<32, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    default: k_bi_spmv<64, T><<<a.grid, kBlock, 0, s>>>(a, mode); break;
-    }
+    k_bi_spmv<T><<<stream_grid(a.nrb), kBlock, 0, s>>>(a, mode);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }

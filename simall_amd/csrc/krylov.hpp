@@ -93,6 +93,8 @@ template <class T> struct CgArgs
     double* history; // [max_iter], may be null
     int lanes;
     unsigned grid;   // range_grid(n, ...)
+    const int* blk;  // CSR-stream row blocks of the matrix
+    int nrb;
 };
 
 // r = b (x0 = 0) or r = b - A x (warm); z = M r (Jacobi) ; p = z; rz, ||b||.
@@ -127,6 +129,8 @@ template <class T> struct BiArgs
     double* history;
     int lanes;
     unsigned grid;
+    const int* blk;
+    int nrb;
 };
 
 template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,

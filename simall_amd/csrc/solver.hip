@@ -7,6 +7,7 @@
 // CG solves A d = r to `inner_tolerance`, x += d in FP64, repeat until the
 // FP64 true residual meets `tolerance`.
 #include "solver.hpp"
+#include "stream.hpp"
 #include "amg.hpp"
 #include "ilu0.hpp"
 #include "reorder.hpp"
@@ -95,11 +96,25 @@ struct JacobiPrecond final : Precond
 
 std::unique_ptr<Precond> make_jacobi() { return std::make_unique<JacobiPrecond>(); }
 
-// r32 = float(b - A x), dscal[0] = ||b - A x||  (FP64 arithmetic)
-template <int LANES>
-__global__ __launch_bounds__(kBlock) void k_true_residual(int n, const int* __restrict__ rp,
-                                                          const int* __restrict__ ci,
-                                                          const double* __restrict__ val,
+// r = b - A x in FP64 (stored as r64 and/or rounded to r32), out = ||r||
+struct EpiResidual
+{
+    const double* b;
+    double* r64;
+    float* r32;
+    double acc = 0.0;
+    __device__ __forceinline__ void row(int i, double s)
+    {
+        const double ri = b[i] - s;
+        if (r64)
+            r64[i] = ri;
+        if (r32)
+            r32[i] = float(ri);
+        acc += ri * ri;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void k_true_residual(StreamMat<double> A,
                                                           const double* __restrict__ x,
                                                           const double* __restrict__ b,
                                                           double* __restrict__ r64,
@@ -107,24 +122,9 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(int n, const int* __re
                                                           double* partials, unsigned* counter,
                                                           double* out)
 {
-    constexpr int kRows = kBlock / LANES;
-    const RowRange rr = block_rows(n, gridDim.x, kRows);
-    const int sub = threadIdx.x / LANES;
-    const int lane = threadIdx.x & (LANES - 1);
-    double v[1] = {0.0};
-    for (int base = rr.beg; base < rr.end; base += kRows) {
-        const int row = base + sub;
-        const bool mine = row < rr.end;
-        const double s = csr_row_dot<LANES>(mine ? row : n, n, lane, rp, ci, val, x);
-        if (mine && lane == 0) {
-            const double ri = b[row] - s;
-            if (r64)
-                r64[row] = ri;
-            if (r32)
-                r32[row] = float(ri);
-            v[0] += ri * ri;
-        }
-    }
+    EpiResidual e{b, r64, r32};
+    stream_rows(A, x, e);
+    double v[1] = {e.acc};
     double tot[1];
     if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
         out[0] = sqrt(tot[0]);
@@ -200,26 +200,9 @@ int true_residual(mi_solver* S, const double* b, const double* x, double* r64, f
                   double* out_dev)
 {
     const Pattern& P = *S->M->pat;
-    const int n = int(P.n);
-    const unsigned grid = range_grid(n, kBlock);
-    hipStream_t s = S->stream;
-    switch (P.lanes) {
-#define MI_CASE(L)                                                                               \
-    case L:                                                                                      \
-        k_true_residual<L><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->M->val.p, x, b, r64, r32, \
-                                                   S->partials.p, S->counter.p, out_dev);        \
-        break;
-        MI_CASE(2)
-        MI_CASE(4)
-        MI_CASE(8)
-        MI_CASE(16)
-        MI_CASE(32)
-    default:
-        k_true_residual<64><<<grid, kBlock, 0, s>>>(n, P.rp.p, P.ci.p, S->M->val.p, x, b, r64, r32,
-                                                    S->partials.p, S->counter.p, out_dev);
-        break;
-#undef MI_CASE
-    }
+    StreamMat<double> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, S->M->val.p};
+    k_true_residual<<<stream_grid(P.rb.nrb), kBlock, 0, S->stream>>>(A, x, b, r64, r32, S->partials.p,
+                                                                     S->counter.p, out_dev);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -427,6 +410,8 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
         a.history = S->hist.p;
         a.lanes = P.lanes;
         a.grid = range_grid(n, 256);
+        a.blk = P.rb.blk.p;
+        a.nrb = P.rb.nrb;
         MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
         return iterate(S, a.st, max_iter, chunk, [&]() -> int {
             MI_TRY(bicg_stage1<T>(a, s));
@@ -455,6 +440,8 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
     a.history = S->hist.p;
     a.lanes = P.lanes;
     a.grid = range_grid(n, 256);
+    a.blk = P.rb.blk.p;
+    a.nrb = P.rb.nrb;
     const bool jacobi = invD != nullptr;
     if (jacobi) {
         MI_TRY(cg_init_jacobi<T>(a, warm, tol, max_iter, s));
@@ -811,6 +798,31 @@ MI_API int mi_solver_info(const mi_solver* S, int* levels, double* operator_comp
     if (bandwidth_after)
         *bandwidth_after = S->ro ? S->ro->bandwidth_after : -1;
     return MI_OK;
+}
+
+MI_API int mi_solver_time_spmv(mi_solver* S, int precision, int reps, double* avg_us,
+                               double* bytes)
+{
+    clear_error();
+    if (!S || !S->A || reps < 1 || !avg_us)
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(ensure_device());
+    MI_TRY(build(S));
+    mi_csr* M = S->M;
+    const int64_t n = M->n(), nnz = M->nnz();
+    const size_t vb = precision == MI_PRECISION_MIXED ? 4 : 8;
+    if (bytes)
+        *bytes = double(nnz) * double(vb + 4) + double(n + 1) * 4.0 + 2.0 * double(n) * double(vb);
+    if (n == 0) {
+        *avg_us = 0.0;
+        return MI_OK;
+    }
+    DevBuf<char> xb, yb;
+    MI_TRY(xb.alloc(size_t(n) * vb));
+    MI_TRY(yb.alloc(size_t(n) * vb));
+    MI_HIP(hipMemsetAsync(xb.p, 0, size_t(n) * vb, default_stream()));
+    MI_HIP(hipStreamSynchronize(default_stream()));
+    return mi_csr_time_spmv(M, precision, xb.p, yb.p, reps, avg_us);
 }
 
 MI_API double mi_solver_last_residual(const mi_solver* S) { return S ? S->last : 0.0; }

@@ -1,5 +1,9 @@
-// spmv.hip — CSR SpMV kernels (vector-CSR with LANES lanes per row).
+// spmv.hip — CSR SpMV: the CSR-stream engine for matrices held in handles,
+// and vector-CSR (LANES lanes per row, LANES = 64: one wavefront per row) for
+// the raw device-pointer entry point, which has no host-side row partition.
+#include "csr.hpp"
 #include "spmv.hpp"
+#include "stream.hpp"
 
 namespace mi
 {
@@ -33,6 +37,61 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
     if (row < n && lane == 0)
         r[row] = b[row] - s;
 }
+
+void build_row_blocks(int n, const int* rp, std::vector<int>& blk)
+{
+    blk.clear();
+    blk.push_back(0);
+    int r = 0;
+    while (r < n) {
+        const int start = r;
+        while (r < n && r - start < kStreamRows && rp[r + 1] - rp[start] <= kStreamNnz)
+            ++r;
+        if (r == start)
+            ++r; // a row longer than kStreamNnz gets a block of its own
+        blk.push_back(r);
+    }
+}
+
+int RowBlocks::upload(int n, const int* rp_host, hipStream_t s)
+{
+    std::vector<int> b;
+    build_row_blocks(n, rp_host, b);
+    nrb = int(b.size()) - 1;
+    MI_TRY(blk.upload(b.data(), b.size(), s));
+    return hipStreamSynchronize(s) == hipSuccess ? MI_OK
+                                                  : set_error(MI_ERR_DEVICE, "row block upload");
+}
+
+template <class T> struct EpiStore
+{
+    T* y;
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_stream_spmv(StreamMat<T> A, const T* __restrict__ x,
+                                                        T* __restrict__ y)
+{
+    EpiStore<T> e{y};
+    stream_rows(A, x, e);
+}
+
+template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const T* x, T* y,
+                                          hipStream_t s)
+{
+    if (P.n == 0)
+        return MI_OK;
+    StreamMat<T> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, val};
+    k_stream_spmv<T><<<stream_grid(P.rb.nrb), kBlock, 0, s>>>(A, x, y);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+template int launch_spmv_stream<double>(const Pattern&, const double*, const double*, double*,
+                                        hipStream_t);
+template int launch_spmv_stream<float>(const Pattern&, const float*, const float*, float*,
+                                       hipStream_t);
 
 template <class VT, class XT, class YT>
 static int spmv_dispatch(int n, int64_t nnz, const int* rp, const int* ci, const VT* val,

@@ -74,6 +74,10 @@ int launch_spmv_f64(int n, int64_t nnz, const int* rp, const int* ci, const doub
                     const double* x, double* y, hipStream_t s, int lanes = 0);
 int launch_spmv_f32(int n, int64_t nnz, const int* rp, const int* ci, const float* val,
                     const float* x, float* y, hipStream_t s, int lanes = 0);
+// y = A x with the CSR-stream engine on a pattern's row blocks.
+struct Pattern;
+template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const T* x, T* y,
+                                          hipStream_t s);
 // r = b - A x  (FP64)
 int launch_residual_f64(int n, int64_t nnz, const int* rp, const int* ci, const double* val,
                         const double* x, const double* b, double* r, hipStream_t s, int lanes = 0);

@@ -112,14 +112,43 @@ def test_spmv_random_row_lengths(gpu, max_len):
     y = A.spmv(x)
     Ao = oracle.Matrix(rp, ci, va)
     yo = oracle.spmv(Ao, x)
+    # the CSR-stream engine rounds each product and adds in column order, as
+    # CSRMatrix::spmv does: identical results
+    assert np.array_equal(y, yo)
     scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x)) + 1e-300
-    assert np.all(np.abs(y - yo) <= 1e-13 * scale)
+    # the raw-pointer entry point (vector-CSR, lane-strided sums): a few ulps
+    drp, dci, dva = (capi.DeviceArray.from_host(v) for v in (rp, ci, va))
+    dxx, dyy = capi.DeviceArray.from_host(x), capi.DeviceArray(n)
+    capi.check(gpu.mi_spmv_csr(n, drp.ptr, dci.ptr, dva.ptr, dxx.ptr, dyy.ptr, None))
+    gpu.mi_device_synchronize()
+    assert np.all(np.abs(dyy.download() - yo) <= 1e-13 * scale)
     # FP32 values and vectors
     dx = capi.DeviceArray.from_host(x.astype(np.float32))
     dy = capi.DeviceArray(n, np.float32)
     A.spmv_device(dx, dy, capi.MI_PRECISION_MIXED)
     y32 = dy.download().astype(np.float64)
     assert np.all(np.abs(y32 - yo) <= 2e-6 * scale)
+
+
+def test_spmv_long_rows(gpu):
+    # rows longer than one LDS row block take the block-reduction path
+    rng = np.random.default_rng(9)
+    n = 6000
+    rp, ci = [0], []
+    for i in range(n):
+        k = 5000 if i in (3, 4000) else int(rng.integers(1, 9))
+        ci.extend(sorted(rng.choice(n, size=k, replace=False).tolist()))
+        rp.append(len(ci))
+    rp, ci = np.array(rp, np.int32), np.array(ci, np.int32)
+    va = rng.standard_normal(ci.size)
+    x = rng.standard_normal(n)
+    y = capi.Csr(rp, ci, va).spmv(x)
+    yo = oracle.spmv(oracle.Matrix(rp, ci, va), x)
+    long_rows = np.array([3, 4000])
+    short = np.setdiff1d(np.arange(n), long_rows)
+    assert np.array_equal(y[short], yo[short])
+    scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x))
+    assert np.all(np.abs(y - yo) <= 1e-13 * scale)
 
 
 def test_spmv_empty_matrix(gpu):
