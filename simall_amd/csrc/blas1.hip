@@ -2,6 +2,7 @@
 // as gfx950 kernels on device pointers.  Reductions run as one deterministic
 // two-level launch (block partials, last block folds them in fixed order).
 #include "blas1.hpp"
+#include "krylov.hpp"
 #include "spmv.hpp"
 
 namespace mi
@@ -24,17 +25,13 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const dou
 
 __global__ __launch_bounds__(kBlock) void k_dot(int64_t n, const double* __restrict__ x,
                                                 const double* __restrict__ y,
-                                                double* __restrict__ partials,
-                                                unsigned* __restrict__ counter,
-                                                double* __restrict__ out)
+                                                double* __restrict__ partials)
 {
     const int64_t stride = int64_t(gridDim.x) * kBlock;
     double v[1] = {0.0};
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
         v[0] += x[i] * y[i];
-    double tot[1];
-    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
-        *out = tot[0];
+    block_partials<1>(v, partials);
 }
 
 __global__ __launch_bounds__(kBlock) void k_jacobi(int64_t n, const double* __restrict__ diag,
@@ -51,10 +48,8 @@ int ReduceScratch::ensure(hipStream_t s)
     if (partials.p)
         return MI_OK;
     MI_TRY(partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
-    MI_TRY(counter.alloc(1));
     MI_TRY(result.alloc(kMaxReduceValues));
-    // The ticket counter must be zero before the first grid_sum launch on s.
-    MI_HIP(hipMemsetAsync(counter.p, 0, sizeof(unsigned), s));
+    (void)s;
     return MI_OK;
 }
 
@@ -71,9 +66,10 @@ int launch_dot(int64_t n, const double* x, const double* y, ReduceScratch& rs, d
                hipStream_t s)
 {
     MI_TRY(rs.ensure(s));
-    k_dot<<<reduce_grid(n, 8), kBlock, 0, s>>>(n, x, y, rs.partials.p, rs.counter.p, out_dev);
+    const unsigned G = reduce_grid(n, 8);
+    k_dot<<<G, kBlock, 0, s>>>(n, x, y, rs.partials.p);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return launch_fin_sum(rs.partials.p, G, false, out_dev, s);
 }
 
 static ReduceScratch& facade_scratch()
@@ -163,7 +159,7 @@ MI_API int mi_spmv_csr(size_t n, const void* rowPtr, const void* colIdx, const v
     MI_TRY(ensure_device());
     if (n == 0)
         return MI_OK;
-    if (!rowPtr || !colIdx || !values || !x || !y)
+    if (!rowPtr || !x || !y)
         return set_error(MI_ERR_BAD_ARG, "null argument");
     if (n > size_t(INT32_MAX))
         return set_error(MI_ERR_BAD_ARG, "n exceeds int32 range");
@@ -174,6 +170,8 @@ MI_API int mi_spmv_csr(size_t n, const void* rowPtr, const void* colIdx, const v
     MI_HIP(hipMemcpyAsync(&nnz, static_cast<const int*>(rowPtr) + n, sizeof(int),
                           hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
+    if (nnz > 0 && (!colIdx || !values))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
     return launch_spmv_f64(int(n), nnz, static_cast<const int*>(rowPtr),
                            static_cast<const int*>(colIdx), static_cast<const double*>(values),
                            static_cast<const double*>(x), static_cast<double*>(y), s);

@@ -10,11 +10,10 @@ namespace mi
 
 constexpr int kMaxReduceValues = 4;
 
-// Partial-sum buffer + ticket counter for grid_sum, one per stream user.
+// Block-partial buffer of the two-launch reductions, one per stream user.
 struct ReduceScratch
 {
     DevBuf<double> partials;
-    DevBuf<unsigned> counter;
     DevBuf<double> result;
     int ensure(hipStream_t s);
 };

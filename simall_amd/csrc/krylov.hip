@@ -1,12 +1,16 @@
-// krylov.hip — fused preconditioned CG kernels.
+// krylov.hip — fused preconditioned CG and BiCGStab kernels.
 //
 // One CG iteration of CGSolver::solve (src/solver/LinearSolvers.cpp:430-450):
-//   Ap = A p ; pAp = p.Ap ; alpha = rz / pAp           k_cg_spmv_dot
-//   x += alpha p ; r -= alpha Ap ; ||r|| ; stop test     k_cg_update[_jacobi]
+//   Ap = A p ; pAp = p.Ap ; alpha = rz / pAp           k_cg_spmv_dot + fin ALPHA
+//   x += alpha p ; r -= alpha Ap ; ||r|| ; stop test     k_cg_update[_jacobi] + fin
 //   z = M r ; rzNew = r.z ; beta = rzNew / rz            (fused for Jacobi)
 //   p = z + beta p                                       k_cg_p[_jacobi]
-// Scalars live in KState on the device; each reduction is finished by the last
-// block of its launch, so the host only polls `done` every few iterations.
+// and of BiCGSTABSolver::solve (src/solver/LinearSolvers.cpp:360-404).
+// Scalars live in KState on the device.  Every reduction is a producing
+// kernel that stores block partials and a one-block finalize kernel (k_fin)
+// that folds them in a fixed order and advances the scalar recurrence, so an
+// iteration needs no host round trip and results are run-to-run identical;
+// the host only polls `done` every few iterations.
 #include "krylov.hpp"
 #include "stream.hpp"
 
@@ -24,10 +28,164 @@ unsigned range_grid(int64_t rows, int min_rows_per_block)
     return unsigned(g);
 }
 
-// ------------------------------------------------------------------- init
+// ------------------------------------------------------------- finalize
+enum FinOp {
+    FIN_CG_INIT,       // tot: r.z, b.b
+    FIN_CG_ALPHA,      // tot: p.Ap
+    FIN_CG_UPDATE_JAC, // tot: r.r, r.z
+    FIN_CG_UPDATE,     // tot: r.r
+    FIN_CG_RZ,         // tot: r.z
+    FIN_BI_INIT,       // tot: r.r, b.b
+    FIN_BI_ALPHA,      // tot: rh.v
+    FIN_BI_OMEGA,      // tot: t.s, t.t
+    FIN_BI_S,          // tot: s.s
+    FIN_BI_UPDATE      // tot: r.r, rh.r
+};
+
+struct FinArgs
+{
+    KState* st;
+    double* history;
+    double tol;
+    int max_iter;
+};
+
+__device__ __forceinline__ void cg_stop_test(KState* st, double rr, double* history)
+{
+    st->last = sqrt(rr) / st->bnorm;
+    const int it = st->iter + 1;
+    st->iter = it;
+    if (history)
+        history[it - 1] = st->last;
+    if (st->last < st->tol)
+        st->done = 1;
+    else if (it >= st->max_iter)
+        st->done = 2;
+}
+
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_fin(const double* __restrict__ partials, int G, int op,
+                                                FinArgs f)
+{
+    KState* st = f.st;
+    const bool init = op == FIN_CG_INIT || op == FIN_BI_INIT;
+    if (!init && st->done)
+        return;
+    double t[NV];
+    fold_partials<NV>(partials, G, t);
+    if (threadIdx.x != 0)
+        return;
+    double tot[2] = {t[0], NV > 1 ? t[NV - 1] : 0.0};
+    switch (op) {
+    case FIN_CG_INIT: {
+        st->rz = tot[0];
+        const double bn = sqrt(tot[1]);
+        st->bnorm = bn > 1e-30 ? bn : 1e-30;
+        st->tol = f.tol;
+        st->max_iter = f.max_iter;
+        st->iter = 0;
+        st->done = f.max_iter > 0 ? 0 : 2;
+        st->last = 0.0;
+        st->alpha = st->beta = 0.0;
+        break;
+    }
+    case FIN_CG_ALPHA:
+        if (tot[0] == 0.0) {
+            // src/solver/LinearSolvers.cpp:433-436
+            st->last = sqrt(st->rz) / st->bnorm;
+            st->done = 3;
+        } else {
+            st->alpha = st->rz / tot[0];
+        }
+        break;
+    case FIN_CG_UPDATE_JAC:
+        cg_stop_test(st, tot[0], f.history);
+        if (!st->done) {
+            st->beta = tot[1] / st->rz;
+            st->rz = tot[1];
+        }
+        break;
+    case FIN_CG_UPDATE:
+        cg_stop_test(st, tot[0], f.history);
+        break;
+    case FIN_CG_RZ:
+        st->beta = tot[0] / st->rz;
+        st->rz = tot[0];
+        break;
+    case FIN_BI_INIT: {
+        const double bn = sqrt(tot[1]);
+        st->bnorm = bn > 1e-30 ? bn : 1e-30;
+        st->tol = f.tol;
+        st->max_iter = f.max_iter;
+        st->iter = 0;
+        st->rz = tot[0];      // rho_new = rh.r with rh = r
+        st->scratch[0] = 1.0; // rho
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        st->beta = tot[0];    // (rho_new / rho) * (alpha / omega)
+        st->last = sqrt(tot[0]) / st->bnorm;
+        st->done = f.max_iter > 0 ? 0 : 2;
+        if (tot[0] == 0.0 && f.max_iter > 0)
+            st->done = 3; // rho_new == 0 at it = 0 (LinearSolvers.cpp:364-367)
+        break;
+    }
+    case FIN_BI_ALPHA:
+        st->alpha = st->rz / tot[0];
+        break;
+    case FIN_BI_OMEGA:
+        st->omega = tot[0] / (tot[1] > 1e-30 ? tot[1] : 1e-30);
+        break;
+    case FIN_BI_S: {
+        const double sn = sqrt(tot[0]);
+        if (sn < st->tol * st->bnorm) {
+            // early exit on ||s|| (LinearSolvers.cpp:386-391); x += alpha y was applied
+            st->last = sn / st->bnorm;
+            st->iter += 1;
+            if (f.history)
+                f.history[st->iter - 1] = st->last;
+            st->done = 1;
+        }
+        break;
+    }
+    case FIN_BI_UPDATE: {
+        const double rho = st->rz;
+        st->scratch[0] = rho;
+        st->rz = tot[1];
+        st->last = sqrt(tot[0]) / st->bnorm;
+        const int it = st->iter + 1;
+        st->iter = it;
+        if (f.history)
+            f.history[it - 1] = st->last;
+        if (st->last < st->tol)
+            st->done = 1;
+        else if (it >= st->max_iter)
+            st->done = 2;
+        else if (tot[1] == 0.0)
+            st->done = 3;
+        else
+            st->beta = (tot[1] / rho) * (st->alpha / st->omega);
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+static int fin(int nv, const double* partials, unsigned G, int op, KState* st, double* history,
+               double tol, int max_iter, hipStream_t s)
+{
+    FinArgs f{st, history, tol, max_iter};
+    if (nv == 1)
+        k_fin<1><<<1, kBlock, 0, s>>>(partials, int(G), op, f);
+    else
+        k_fin<2><<<1, kBlock, 0, s>>>(partials, int(G), op, f);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+// ------------------------------------------------------------------- CG
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_init_jacobi(CgArgs<T> a, int warm, double tol,
-                                                           int max_iter)
+__global__ __launch_bounds__(kBlock) void k_cg_init_jacobi(CgArgs<T> a, int warm)
 {
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[2] = {0.0, 0.0};
@@ -46,19 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_jacobi(CgArgs<T> a, int warm
         v[0] += double(ri) * double(zi);
         v[1] += double(bi) * double(bi);
     }
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        st->rz = tot[0];
-        const double bn = sqrt(tot[1]);
-        st->bnorm = bn > 1e-30 ? bn : 1e-30;
-        st->tol = tol;
-        st->max_iter = max_iter;
-        st->iter = 0;
-        st->done = max_iter > 0 ? 0 : 2;
-        st->last = 0.0;
-        st->alpha = st->beta = 0.0;
-    }
+    block_partials<2>(v, a.partials);
 }
 
 template <class T>
@@ -72,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_r(CgArgs<T> a)
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_cg_init_p(CgArgs<T> a, double tol, int max_iter)
+__global__ __launch_bounds__(kBlock) void k_cg_init_p(CgArgs<T> a)
 {
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[2] = {0.0, 0.0};
@@ -82,22 +228,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_p(CgArgs<T> a, double tol, i
         v[0] += double(ri) * double(zi);
         v[1] += double(bi) * double(bi);
     }
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        st->rz = tot[0];
-        const double bn = sqrt(tot[1]);
-        st->bnorm = bn > 1e-30 ? bn : 1e-30;
-        st->tol = tol;
-        st->max_iter = max_iter;
-        st->iter = 0;
-        st->done = max_iter > 0 ? 0 : 2;
-        st->last = 0.0;
-        st->alpha = st->beta = 0.0;
-    }
+    block_partials<2>(v, a.partials);
 }
 
-// -------------------------------------------------------------- iteration
 template <class T> struct EpiCgAp
 {
     const T* p;
@@ -111,7 +244,7 @@ template <class T> struct EpiCgAp
     }
 };
 
-// Ap = A p ; alpha = rz / p.Ap
+// Ap = A p ; partials of p.Ap
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
 {
@@ -121,31 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
     EpiCgAp<T> e{a.p, a.Ap};
     stream_rows(A, a.p, e);
     double v[1] = {e.acc};
-    double tot[1];
-    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        const double pAp = tot[0];
-        if (pAp == 0.0) {
-            // src/solver/LinearSolvers.cpp:433-436
-            st->last = sqrt(st->rz) / st->bnorm;
-            st->done = 3;
-        } else {
-            st->alpha = st->rz / pAp;
-        }
-    }
-}
-
-__device__ __forceinline__ void cg_stop_test(KState* st, double rr, double* history)
-{
-    st->last = sqrt(rr) / st->bnorm;
-    const int it = st->iter + 1;
-    st->iter = it;
-    if (history)
-        history[it - 1] = st->last;
-    if (st->last < st->tol)
-        st->done = 1;
-    else if (it >= st->max_iter)
-        st->done = 2;
+    block_partials<1>(v, a.partials);
 }
 
 template <class T>
@@ -165,15 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_jacobi(CgArgs<T> a)
         v[0] += double(ri) * double(ri);
         v[1] += double(ri) * double(zi);
     }
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        cg_stop_test(st, tot[0], a.history);
-        if (!st->done) {
-            st->beta = tot[1] / st->rz;
-            st->rz = tot[1];
-        }
-    }
+    block_partials<2>(v, a.partials);
 }
 
 template <class T>
@@ -202,9 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a)
         a.r[i] = ri;
         v[0] += double(ri) * double(ri);
     }
-    double tot[1];
-    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0)
-        cg_stop_test(a.st, tot[0], a.history);
+    block_partials<1>(v, a.partials);
 }
 
 template <class T>
@@ -216,12 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_rz(CgArgs<T> a)
     double v[1] = {0.0};
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
         v[0] += double(a.r[i]) * double(a.z[i]);
-    double tot[1];
-    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        st->beta = tot[0] / st->rz;
-        st->rz = tot[0];
-    }
+    block_partials<1>(v, a.partials);
 }
 
 template <class T>
@@ -235,13 +329,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(CgArgs<T> a)
         a.p[i] = a.z[i] + be * a.p[i];
 }
 
-// -------------------------------------------------------------- launchers
 template <class T> int cg_init_jacobi(const CgArgs<T>& a, bool warm, double tol, int max_iter,
                                        hipStream_t s)
 {
-    k_cg_init_jacobi<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0, tol, max_iter);
+    k_cg_init_jacobi<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(2, a.partials, a.grid, FIN_CG_INIT, a.st, a.history, tol, max_iter, s);
 }
 
 template <class T> int cg_init_r(const CgArgs<T>& a, bool warm, hipStream_t s)
@@ -255,23 +348,25 @@ template <class T> int cg_init_r(const CgArgs<T>& a, bool warm, hipStream_t s)
 
 template <class T> int cg_init_p(const CgArgs<T>& a, double tol, int max_iter, hipStream_t s)
 {
-    k_cg_init_p<T><<<a.grid, kBlock, 0, s>>>(a, tol, max_iter);
+    k_cg_init_p<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(2, a.partials, a.grid, FIN_CG_INIT, a.st, a.history, tol, max_iter, s);
 }
 
-template <class T> static int launch_spmv_dot(const CgArgs<T>& a, hipStream_t s)
+template <class T> static int spmv_dot(const CgArgs<T>& a, hipStream_t s)
 {
-    k_cg_spmv_dot<T><<<stream_grid(a.nrb), kBlock, 0, s>>>(a);
+    const unsigned G = stream_grid(a.nrb);
+    k_cg_spmv_dot<T><<<G, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(1, a.partials, G, FIN_CG_ALPHA, a.st, a.history, 0, 0, s);
 }
 
 template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s)
 {
-    MI_TRY(launch_spmv_dot(a, s));
+    MI_TRY(spmv_dot(a, s));
     k_cg_update_jacobi<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
+    MI_TRY(fin(2, a.partials, a.grid, FIN_CG_UPDATE_JAC, a.st, a.history, 0, 0, s));
     k_cg_p_jacobi<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
     return MI_OK;
@@ -279,16 +374,17 @@ template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s)
 
 template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s)
 {
-    MI_TRY(launch_spmv_dot(a, s));
+    MI_TRY(spmv_dot(a, s));
     k_cg_update<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(1, a.partials, a.grid, FIN_CG_UPDATE, a.st, a.history, 0, 0, s);
 }
 
 template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s)
 {
     k_cg_rz<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
+    MI_TRY(fin(1, a.partials, a.grid, FIN_CG_RZ, a.st, a.history, 0, 0, s));
     k_cg_p<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
     return MI_OK;
@@ -296,7 +392,7 @@ template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s)
 
 // --------------------------------------------------------------- BiCGStab
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_bi_init(BiArgs<T> a, int warm, double tol, int max_iter)
+__global__ __launch_bounds__(kBlock) void k_bi_init(BiArgs<T> a, int warm)
 {
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[2] = {0.0, 0.0};
@@ -316,24 +412,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(BiArgs<T> a, int warm, doubl
         v[0] += double(ri) * double(ri);
         v[1] += double(bi) * double(bi);
     }
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        const double bn = sqrt(tot[1]);
-        st->bnorm = bn > 1e-30 ? bn : 1e-30;
-        st->tol = tol;
-        st->max_iter = max_iter;
-        st->iter = 0;
-        st->rz = tot[0];          // rho_new = rh.r with rh = r
-        st->scratch[0] = 1.0;     // rho
-        st->alpha = 1.0;
-        st->omega = 1.0;
-        st->beta = tot[0];        // (rho_new / rho) * (alpha / omega)
-        st->last = sqrt(tot[0]) / st->bnorm;
-        st->done = max_iter > 0 ? 0 : 2;
-        if (tot[0] == 0.0 && max_iter > 0)
-            st->done = 3; // rho_new == 0 at it = 0 (LinearSolvers.cpp:364-367)
-    }
+    block_partials<2>(v, a.partials);
 }
 
 template <class T>
@@ -357,16 +436,13 @@ template <class T> struct EpiBi
     {
         const T o = T(sres);
         out[i] = o;
-        if (mode == 0) {
-            acc0 += double(w1[i]) * double(o);
-        } else {
-            acc0 += double(o) * double(w1[i]);
+        acc0 += double(w1[i]) * double(o);
+        if (mode == 1)
             acc1 += double(o) * double(o);
-        }
     }
 };
 
-// mode 0: v = A y, alpha = rho_new / rh.v ; mode 1: t = A z, omega = t.s / max(t.t, 1e-30)
+// mode 0: v = A y, partials of rh.v ; mode 1: t = A z, partials of t.s, t.t
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
 {
@@ -376,14 +452,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
     EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
     stream_rows(A, mode == 0 ? a.y : a.z, e);
     double v[2] = {e.acc0, e.acc1};
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        if (mode == 0)
-            st->alpha = st->rz / tot[0];
-        else
-            st->omega = tot[0] / (tot[1] > 1e-30 ? tot[1] : 1e-30);
-    }
+    block_partials<2>(v, a.partials);
 }
 
 template <class T>
@@ -400,19 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(BiArgs<T> a)
         a.x[i] += al * a.y[i];
         v[0] += double(si) * double(si);
     }
-    double tot[1];
-    if (grid_sum<1>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        const double sn = sqrt(tot[0]);
-        if (sn < st->tol * st->bnorm) {
-            // early exit on ||s|| (LinearSolvers.cpp:386-391)
-            st->last = sn / st->bnorm;
-            st->iter += 1;
-            if (a.history)
-                a.history[st->iter - 1] = st->last;
-            st->done = 1;
-        }
-    }
+    block_partials<1>(v, a.partials);
 }
 
 template <class T>
@@ -430,34 +487,15 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(BiArgs<T> a)
         v[0] += double(ri) * double(ri);
         v[1] += double(a.rh[i]) * double(ri);
     }
-    double tot[2];
-    if (grid_sum<2>(v, a.partials, a.counter, tot) && threadIdx.x == 0) {
-        KState* st = a.st;
-        const double rho = st->rz;
-        st->scratch[0] = rho;
-        st->rz = tot[1];
-        st->last = sqrt(tot[0]) / st->bnorm;
-        const int it = st->iter + 1;
-        st->iter = it;
-        if (a.history)
-            a.history[it - 1] = st->last;
-        if (st->last < st->tol)
-            st->done = 1;
-        else if (it >= st->max_iter)
-            st->done = 2;
-        else if (tot[1] == 0.0)
-            st->done = 3;
-        else
-            st->beta = (tot[1] / rho) * (st->alpha / st->omega);
-    }
+    block_partials<2>(v, a.partials);
 }
 
 template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,
                                   hipStream_t s)
 {
-    k_bi_init<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0, tol, max_iter);
+    k_bi_init<T><<<a.grid, kBlock, 0, s>>>(a, warm ? 1 : 0);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(2, a.partials, a.grid, FIN_BI_INIT, a.st, a.history, tol, max_iter, s);
 }
 
 template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s)
@@ -467,47 +505,58 @@ template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s)
     return MI_OK;
 }
 
-template <class T> static int bi_spmv(const BiArgs<T>& a, int mode, hipStream_t s)
-{
-    k_bi_spmv<T><<<stream_grid(a.nrb), kBlock, 0, s>>>(a, mode);
-    MI_HIP(hipGetLastError());
-    return MI_OK;
-}
-
 template <class T> int bicg_stage2(const BiArgs<T>& a, hipStream_t s)
 {
-    MI_TRY(bi_spmv(a, 0, s));
+    const unsigned G = stream_grid(a.nrb);
+    k_bi_spmv<T><<<G, kBlock, 0, s>>>(a, 0);
+    MI_HIP(hipGetLastError());
+    MI_TRY(fin(2, a.partials, G, FIN_BI_ALPHA, a.st, a.history, 0, 0, s));
     k_bi_s<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(1, a.partials, a.grid, FIN_BI_S, a.st, a.history, 0, 0, s);
 }
 
 template <class T> int bicg_stage3(const BiArgs<T>& a, hipStream_t s)
 {
-    MI_TRY(bi_spmv(a, 1, s));
+    const unsigned G = stream_grid(a.nrb);
+    k_bi_spmv<T><<<G, kBlock, 0, s>>>(a, 1);
+    MI_HIP(hipGetLastError());
+    MI_TRY(fin(2, a.partials, G, FIN_BI_OMEGA, a.st, a.history, 0, 0, s));
     k_bi_update<T><<<a.grid, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return fin(2, a.partials, a.grid, FIN_BI_UPDATE, a.st, a.history, 0, 0, s);
 }
 
-#define MI_INSTANTIATE_BI(T)                                                                     \
-    template int bicg_init<T>(const BiArgs<T>&, bool, double, int, hipStream_t);                 \
-    template int bicg_stage1<T>(const BiArgs<T>&, hipStream_t);                                  \
-    template int bicg_stage2<T>(const BiArgs<T>&, hipStream_t);                                  \
-    template int bicg_stage3<T>(const BiArgs<T>&, hipStream_t);
-
-MI_INSTANTIATE_BI(double)
-MI_INSTANTIATE_BI(float)
-
-#define MI_INSTANTIATE_CG(T)                                                                     \
+#define MI_INSTANTIATE(T)                                                                        \
     template int cg_init_jacobi<T>(const CgArgs<T>&, bool, double, int, hipStream_t);            \
     template int cg_init_r<T>(const CgArgs<T>&, bool, hipStream_t);                              \
     template int cg_init_p<T>(const CgArgs<T>&, double, int, hipStream_t);                       \
     template int cg_iteration_jacobi<T>(const CgArgs<T>&, hipStream_t);                          \
     template int cg_iteration_part1<T>(const CgArgs<T>&, hipStream_t);                           \
-    template int cg_iteration_part2<T>(const CgArgs<T>&, hipStream_t);
+    template int cg_iteration_part2<T>(const CgArgs<T>&, hipStream_t);                           \
+    template int bicg_init<T>(const BiArgs<T>&, bool, double, int, hipStream_t);                 \
+    template int bicg_stage1<T>(const BiArgs<T>&, hipStream_t);                                  \
+    template int bicg_stage2<T>(const BiArgs<T>&, hipStream_t);                                  \
+    template int bicg_stage3<T>(const BiArgs<T>&, hipStream_t);
 
-MI_INSTANTIATE_CG(double)
-MI_INSTANTIATE_CG(float)
+MI_INSTANTIATE(double)
+MI_INSTANTIATE(float)
+
+// ------------------------------------------------------------ plain sums
+__global__ __launch_bounds__(kBlock) void k_fin_sum(const double* __restrict__ partials, int G,
+                                                    int take_sqrt, double* __restrict__ out)
+{
+    double tot[1];
+    fold_partials<1>(partials, G, tot);
+    if (threadIdx.x == 0)
+        out[0] = take_sqrt ? sqrt(tot[0]) : tot[0];
+}
+
+int launch_fin_sum(const double* partials, unsigned G, bool take_sqrt, double* out, hipStream_t s)
+{
+    k_fin_sum<<<1, kBlock, 0, s>>>(partials, int(G), take_sqrt ? 1 : 0, out);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
 
 } // namespace mi

@@ -59,6 +59,9 @@ __device__ __forceinline__ RowRange block_rows(int n, int G, int align)
 }
 #endif
 
+// out[0] = sum (or its square root) of G one-value block partials.
+int launch_fin_sum(const double* partials, unsigned G, bool take_sqrt, double* out, hipStream_t s);
+
 // Grid size for the row-range kernels: a multiple of 8 (one share per XCD),
 // at most kMaxReduceBlocks, at least one block per `min_rows_per_block` rows.
 unsigned range_grid(int64_t rows, int min_rows_per_block);
@@ -89,7 +92,6 @@ template <class T> struct CgArgs
     T* Ap;
     KState* st;
     double* partials;
-    unsigned* counter;
     double* history; // [max_iter], may be null
     int lanes;
     unsigned grid;   // range_grid(n, ...)
@@ -125,7 +127,6 @@ template <class T> struct BiArgs
     T *x, *r, *rh, *p, *v, *s, *t, *y, *z;
     KState* st;
     double* partials;
-    unsigned* counter;
     double* history;
     int lanes;
     unsigned grid;

@@ -166,43 +166,35 @@ template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], dou
     __syncthreads();
 }
 
-// Grid-wide deterministic reduction.  Each block stores its NV partial sums;
-// the last block to arrive (ticket counter) adds all partials in a fixed
-// order and returns true in every thread of that block with the totals in
-// `tot` (valid in thread 0).  The counter is re-armed for the next launch.
-template <int NV>
-__device__ __forceinline__ bool grid_sum(double (&v)[NV], double* partials, unsigned* counter,
-                                         double (&tot)[NV])
+// Grid-wide deterministic reductions are split in two launches: each block of
+// the producing kernel stores its NV block sums (block_partials), and a
+// one-block finalize kernel folds the G partials in a fixed order
+// (fold_partials).  The kernel boundary publishes the partials; no block
+// issues a device-scope fence, which on a multi-XCD device would write back
+// the XCD's whole L2 every time.
+template <int NV> __device__ __forceinline__ void block_partials(double (&v)[NV], double* partials)
 {
     __shared__ double lds[NV * (kBlock / kWave)];
-    __shared__ bool last;
     block_sum<NV>(v, lds);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int j = 0; j < NV; ++j)
             partials[size_t(j) * gridDim.x + blockIdx.x] = v[j];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last)
-        return false;
-    __threadfence();
-    double s[NV];
+}
+
+// Totals over G partial blocks, valid in thread 0 of a one-block launch.
+template <int NV>
+__device__ __forceinline__ void fold_partials(const double* partials, int G, double (&tot)[NV])
+{
+    __shared__ double lds[NV * (kBlock / kWave)];
 #pragma unroll
     for (int j = 0; j < NV; ++j) {
-        s[j] = 0.0;
-        for (unsigned b = threadIdx.x; b < gridDim.x; b += kBlock)
-            s[j] += partials[size_t(j) * gridDim.x + b];
+        double s = 0.0;
+        for (int b = threadIdx.x; b < G; b += kBlock)
+            s += partials[size_t(j) * G + b];
+        tot[j] = s;
     }
-    block_sum<NV>(s, lds);
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-        tot[j] = s[j];
-    if (threadIdx.x == 0)
-        *counter = 0u;
-    return true;
+    block_sum<NV>(tot, lds);
 }
 
 #endif

@@ -119,15 +119,12 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(StreamMat<double> A,
                                                           const double* __restrict__ b,
                                                           double* __restrict__ r64,
                                                           float* __restrict__ r32,
-                                                          double* partials, unsigned* counter,
-                                                          double* out)
+                                                          double* partials)
 {
     EpiResidual e{b, r64, r32};
     stream_rows(A, x, e);
     double v[1] = {e.acc};
-    double tot[1];
-    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
-        out[0] = sqrt(tot[0]);
+    block_partials<1>(v, partials);
 }
 
 __global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int n, const float* __restrict__ d,
@@ -139,15 +136,13 @@ __global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int n, const float* _
 }
 
 __global__ __launch_bounds__(kBlock) void k_norm(int n, const double* __restrict__ b,
-                                                 double* partials, unsigned* counter, double* out)
+                                                 double* partials)
 {
     const RowRange rr = block_rows(n, gridDim.x, 1);
     double v[1] = {0.0};
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
         v[0] += b[i] * b[i];
-    double tot[1];
-    if (grid_sum<1>(v, partials, counter, tot) && threadIdx.x == 0)
-        out[0] = sqrt(tot[0]);
+    block_partials<1>(v, partials);
 }
 
 // dst[i] = src[perm[i]]
@@ -201,10 +196,18 @@ int true_residual(mi_solver* S, const double* b, const double* x, double* r64, f
 {
     const Pattern& P = *S->M->pat;
     StreamMat<double> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, S->M->val.p};
-    k_true_residual<<<stream_grid(P.rb.nrb), kBlock, 0, S->stream>>>(A, x, b, r64, r32, S->partials.p,
-                                                                     S->counter.p, out_dev);
+    const unsigned G = stream_grid(P.rb.nrb);
+    k_true_residual<<<G, kBlock, 0, S->stream>>>(A, x, b, r64, r32, S->partials.p);
     MI_HIP(hipGetLastError());
-    return MI_OK;
+    return launch_fin_sum(S->partials.p, G, true, out_dev, S->stream);
+}
+
+int norm2(mi_solver* S, int n, const double* v, double* out_dev)
+{
+    const unsigned G = range_grid(n, kBlock);
+    k_norm<<<G, kBlock, 0, S->stream>>>(n, v, S->partials.p);
+    MI_HIP(hipGetLastError());
+    return launch_fin_sum(S->partials.p, G, true, out_dev, S->stream);
 }
 
 int read_scalars(mi_solver* S, int count)
@@ -233,10 +236,6 @@ int ensure_workspace(mi_solver* S)
     }
     MI_TRY(S->st.alloc(1));
     MI_TRY(S->partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
-    if (!S->counter.p) {
-        MI_TRY(S->counter.alloc(1));
-        MI_HIP(hipMemsetAsync(S->counter.p, 0, sizeof(unsigned), S->stream));
-    }
     MI_TRY(S->dscal.alloc(8));
     if (!S->h_st)
         MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->h_st), sizeof(KState)));
@@ -406,7 +405,6 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
         a.z = w[7].p;
         a.st = S->st.p;
         a.partials = S->partials.p;
-        a.counter = S->counter.p;
         a.history = S->hist.p;
         a.lanes = P.lanes;
         a.grid = range_grid(n, 256);
@@ -436,7 +434,6 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
     a.Ap = w[3].p;
     a.st = S->st.p;
     a.partials = S->partials.p;
-    a.counter = S->counter.p;
     a.history = S->hist.p;
     a.lanes = P.lanes;
     a.grid = range_grid(n, 256);
@@ -479,9 +476,7 @@ int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
     int total = 0, outer = 0;
     int status = MI_OK;
     double rel_true = 0.0;
-    k_norm<<<range_grid(n, kBlock), kBlock, 0, S->stream>>>(n, b, S->partials.p, S->counter.p,
-                                                            S->dscal.p + 1);
-    MI_HIP(hipGetLastError());
+    MI_TRY(norm2(S, n, b, S->dscal.p + 1));
     if (warm)
         MI_TRY(true_residual(S, b, x, S->w64[0].p, nullptr, S->dscal.p + 2));
     MI_TRY(read_scalars(S, 3));
@@ -542,9 +537,7 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     const bool warm = c.warm_start != 0;
     if (!warm)
         MI_HIP(hipMemsetAsync(x, 0, sizeof(double) * size_t(n), s));
-    k_norm<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, b, S->partials.p, S->counter.p,
-                                                    S->dscal.p + 1);
-    MI_HIP(hipGetLastError());
+    MI_TRY(norm2(S, n, b, S->dscal.p + 1));
     MI_TRY(true_residual(S, b, x, nullptr, r32, S->dscal.p));
     MI_TRY(read_scalars(S, 2));
     const double bnorm = std::max(S->h_scal[1], 1e-30);

@@ -65,7 +65,6 @@ struct mi_solver
     mi::DevBuf<mi::KState> st;
     mi::KState* h_st = nullptr; // pinned mirror of st
     mi::DevBuf<double> partials;
-    mi::DevBuf<unsigned> counter;
     mi::DevBuf<double> dscal;   // device scalars for norms
     double* h_scal = nullptr;   // pinned
     std::vector<double> history;
