@@ -52,7 +52,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
-    ap.add_argument("--pc", default="jacobi", choices=["jacobi", "amg"])
+    ap.add_argument("--pc", default="amg", choices=["jacobi", "amg"])
     ap.add_argument("--cpu-iters", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")

@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace mi
 {
@@ -135,7 +136,7 @@ static unsigned rows_grid(int n, int lanes)
     return (g + 7) / 8 * 8;
 }
 
-template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s)
+template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_form)
 {
     nr = h.nr;
     nc = h.nc;
@@ -144,6 +145,28 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s)
     MI_TRY(rp.upload(h.rp.data(), h.rp.size(), s));
     MI_TRY(ci.upload(h.ci.data(), h.ci.size(), s));
     std::vector<T> tv(h.v.begin(), h.v.end());
+    std::vector<int> kd;
+    if (diff_form && std::is_same<T, float>::value && nr == nc) {
+        // off-diagonals rounded, diagonal slot = FP64 row sum rounded once
+        kd.assign(static_cast<size_t>(nr), -1);
+        for (int i = 0; i < nr; ++i) {
+            double rs = 0.0;
+            for (int k = h.rp[size_t(i)]; k < h.rp[size_t(i) + 1]; ++k) {
+                rs += h.v[size_t(k)];
+                if (h.ci[size_t(k)] == i && kd[size_t(i)] < 0)
+                    kd[size_t(i)] = k;
+            }
+            if (kd[size_t(i)] < 0) {
+                kd.clear(); // a row without its diagonal: keep the rounded values
+                break;
+            }
+            tv[size_t(kd[size_t(i)])] = T(rs);
+        }
+        if (kd.empty())
+            tv.assign(h.v.begin(), h.v.end());
+    }
+    if (!kd.empty())
+        MI_TRY(dg.upload(kd.data(), kd.size(), s));
     MI_TRY(v.upload(tv.data(), tv.size(), s));
     MI_TRY(rb.upload(nr, h.rp.data(), s)); // synchronizes: tv is released on return
     return MI_OK;
@@ -171,7 +194,7 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
         const AmgLevelHost& h = H.levels[size_t(l)];
         DLevel<T>& d = L[size_t(l)];
         const int n = h.A.nr;
-        MI_TRY(d.A.upload(h.A, s));
+        MI_TRY(d.A.upload(h.A, s, true));
         if (l + 1 < nl) {
             MI_TRY(d.P.upload(h.P, s));
             MI_TRY(d.R.upload(h.R, s));

@@ -17,10 +17,16 @@ template <class T> struct DMat
     int lanes = 8;
     DevBuf<int> rp, ci;
     DevBuf<T> v;
+    DevBuf<int> dg; // diagonal positions when v holds the FP32 difference form
     RowBlocks rb;
-    int upload(const HCsr& h, hipStream_t s);
+    // diff_form: store the FP32 difference form (stream_rows_diff) when every
+    // row holds its diagonal; ignored for FP64.
+    int upload(const HCsr& h, hipStream_t s, bool diff_form = false);
 #if defined(__HIPCC__)
-    StreamMat<T> sm() const { return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p}; }
+    StreamMat<T> sm() const
+    {
+        return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr};
+    }
 #endif
 };
 

@@ -39,6 +39,22 @@ __global__ void k_inv_diag(int n, const int* __restrict__ diag, const double* __
     invD[i] = T(v);
 }
 
+// Off-diagonals rounded to FP32; the diagonal slot receives the FP64 row sum.
+__global__ void k_diff_f32(int n, const int* __restrict__ rp, const int* __restrict__ diag,
+                           const double* __restrict__ val, float* __restrict__ out)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    double d = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        d += val[k];
+        out[k] = float(val[k]);
+    }
+    if (diag[i] >= 0)
+        out[diag[i]] = float(d);
+}
+
 __global__ void k_convert_f32(int64_t n, const double* __restrict__ src, float* __restrict__ dst)
 {
     const int64_t stride = int64_t(gridDim.x) * kBlock;
@@ -71,6 +87,13 @@ int make_pattern(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
     if (n > 0) {
         p->h_rp.assign(rp, rp + n + 1);
         p->h_ci.assign(ci, ci + nnz);
+        p->all_diag = true;
+        for (int64_t i = 0; i < n && p->all_diag; ++i) {
+            bool has = false;
+            for (int k = rp[i]; k < rp[i + 1] && !has; ++k)
+                has = ci[k] == i;
+            p->all_diag = has;
+        }
         MI_TRY(p->rp.upload(p->h_rp.data(), size_t(n + 1), s));
         MI_TRY(p->ci.upload(p->h_ci.data(), size_t(nnz), s));
         MI_TRY(p->diag.alloc(size_t(n)));
@@ -164,6 +187,25 @@ int mi_csr::ensure_fp32(hipStream_t s)
     return MI_OK;
 }
 
+int mi_csr::ensure_fp32_diff(hipStream_t s)
+{
+    if (val32d_valid)
+        return MI_OK;
+    MI_TRY(val32d.alloc(size_t(nnz())));
+    if (n() > 0) {
+        k_diff_f32<<<grid_for(n(), kBlock), kBlock, 0, s>>>(int(n()), pat->rp.p, pat->diag.p, val.p,
+                                                            val32d.p);
+        MI_HIP(hipGetLastError());
+    }
+    val32d_valid = true;
+    return MI_OK;
+}
+
+int mi_csr::ensure_fp32_op(hipStream_t s)
+{
+    return pat->all_diag ? ensure_fp32_diff(s) : ensure_fp32(s);
+}
+
 extern "C" {
 
 MI_API int mi_csr_create(int64_t n, int64_t nnz, const int32_t* rowPtr, const int32_t* colIdx,
@@ -206,6 +248,7 @@ MI_API int mi_csr_update_values(mi_csr* A, const double* values)
     MI_HIP(hipStreamSynchronize(s));
     A->host_valid = true;
     A->val32_valid = false;
+    A->val32d_valid = false;
     ++A->version;
     return MI_OK;
 }
@@ -222,6 +265,7 @@ MI_API int mi_csr_update_values_device(mi_csr* A, const double* values_device)
     MI_HIP(hipStreamSynchronize(s));
     A->host_valid = false;
     A->val32_valid = false;
+    A->val32d_valid = false;
     ++A->version;
     return MI_OK;
 }
@@ -266,6 +310,7 @@ MI_API int mi_csr_values_device(mi_csr* A, double** values_device)
     // The caller may write through this pointer: host and FP32 copies go stale.
     A->host_valid = false;
     A->val32_valid = false;
+    A->val32d_valid = false;
     ++A->version;
     return MI_OK;
 }

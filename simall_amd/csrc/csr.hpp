@@ -17,6 +17,7 @@ struct Pattern
     std::vector<int> h_rp, h_ci;
     DevBuf<int> rp, ci;
     DevBuf<int> diag; // position of a_ii in each row, -1 if absent
+    bool all_diag = false; // every row stores its diagonal entry
     RowBlocks rb;     // CSR-stream row blocks
     int lanes = 8;
 };
@@ -27,6 +28,8 @@ struct mi_csr
     std::shared_ptr<mi::Pattern> pat;
     mi::DevBuf<double> val;
     mi::DevBuf<float> val32;  // FP32 copy for the inner iterations, built lazily
+    mi::DevBuf<float> val32d; // FP32 difference form: diagonal slot = row sum
+    bool val32d_valid = false;
     std::vector<double> h_val;
     bool host_valid = false;  // h_val mirrors val
     bool val32_valid = false;
@@ -38,6 +41,14 @@ struct mi_csr
     int sync_host();
     // Makes val32 current.
     int ensure_fp32(hipStream_t s);
+    // Makes val32d current (requires every row to hold its diagonal).
+    int ensure_fp32_diff(hipStream_t s);
+    // The FP32 operator of the inner iterations: the difference form when
+    // every row holds its diagonal, else the rounded values.  diag32() is
+    // the matching StreamMat::diag (nullptr for the rounded values).
+    int ensure_fp32_op(hipStream_t s);
+    const float* op32() const { return pat->all_diag ? val32d.p : val32.p; }
+    const int* diag32() const { return pat->all_diag ? pat->diag.p : nullptr; }
 };
 
 namespace mi

@@ -196,7 +196,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_jacobi(CgArgs<T> a, int warm
             ri = a.r[i];
         } else {
             ri = bi;
-            a.x[i] = T(0);
+            if (!a.xacc)
+                a.x[i] = T(0);
             a.r[i] = ri;
         }
         const T zi = a.invD[i] * ri;
@@ -212,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_r(CgArgs<T> a)
 {
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
-        a.x[i] = T(0);
+        if (!a.xacc)
+            a.x[i] = T(0);
         a.r[i] = a.b[i];
     }
 }
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag};
     EpiCgAp<T> e{a.p, a.Ap};
     stream_rows(A, a.p, e);
     double v[1] = {e.acc};
@@ -265,10 +267,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_jacobi(CgArgs<T> a)
     const T al = T(a.st->alpha);
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[2] = {0.0, 0.0};
+    const double al64 = a.st->alpha;
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
-        const T xi = a.x[i] + al * a.p[i];
+        if (a.xacc)
+            a.xacc[i] += al64 * double(a.p[i]);
+        else
+            a.x[i] = a.x[i] + al * a.p[i];
         const T ri = a.r[i] - al * a.Ap[i];
-        a.x[i] = xi;
         a.r[i] = ri;
         const T zi = a.invD[i] * ri;
         v[0] += double(ri) * double(ri);
@@ -296,10 +301,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(CgArgs<T> a)
     const T al = T(a.st->alpha);
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[1] = {0.0};
+    const double al64 = a.st->alpha;
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
-        const T xi = a.x[i] + al * a.p[i];
+        if (a.xacc)
+            a.xacc[i] += al64 * double(a.p[i]);
+        else
+            a.x[i] = a.x[i] + al * a.p[i];
         const T ri = a.r[i] - al * a.Ap[i];
-        a.x[i] = xi;
         a.r[i] = ri;
         v[0] += double(ri) * double(ri);
     }
@@ -403,7 +411,8 @@ __global__ __launch_bounds__(kBlock) void k_bi_init(BiArgs<T> a, int warm)
             ri = a.r[i];
         } else {
             ri = bi;
-            a.x[i] = T(0);
+            if (!a.xacc)
+                a.x[i] = T(0);
             a.r[i] = ri;
         }
         a.rh[i] = ri;
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag};
     EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
     stream_rows(A, mode == 0 ? a.y : a.z, e);
     double v[2] = {e.acc0, e.acc1};
@@ -466,7 +475,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_s(BiArgs<T> a)
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
         const T si = a.r[i] - al * a.v[i];
         a.s[i] = si;
-        a.x[i] += al * a.y[i];
+        if (a.xacc)
+            a.xacc[i] += a.st->alpha * double(a.y[i]);
+        else
+            a.x[i] += al * a.y[i];
         v[0] += double(si) * double(si);
     }
     block_partials<1>(v, a.partials);
@@ -481,7 +493,10 @@ __global__ __launch_bounds__(kBlock) void k_bi_update(BiArgs<T> a)
     const RowRange rr = block_rows(a.n, gridDim.x, 1);
     double v[2] = {0.0, 0.0};
     for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
-        a.x[i] += om * a.z[i];
+        if (a.xacc)
+            a.xacc[i] += a.st->omega * double(a.z[i]);
+        else
+            a.x[i] += om * a.z[i];
         const T ri = a.s[i] - om * a.t[i];
         a.r[i] = ri;
         v[0] += double(ri) * double(ri);

@@ -97,6 +97,12 @@ template <class T> struct CgArgs
     unsigned grid;   // range_grid(n, ...)
     const int* blk;  // CSR-stream row blocks of the matrix
     int nrb;
+    const int* diag; // FP32 difference form: diagonal positions, else nullptr
+    // FP32 inner solves of the mixed form: when set, the iterate is
+    // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
+    // then unused.  The caller's FP64 solution takes the correction directly
+    // and the FP32 rounding of a stored correction never reaches it.
+    double* xacc;
 };
 
 // r = b (x0 = 0) or r = b - A x (warm); z = M r (Jacobi) ; p = z; rz, ||b||.
@@ -132,6 +138,8 @@ template <class T> struct BiArgs
     unsigned grid;
     const int* blk;
     int nrb;
+    const int* diag; // as in CgArgs
+    double* xacc;    // as in CgArgs
 };
 
 template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,

@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 using namespace mi;
 
@@ -125,14 +126,6 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(StreamMat<double> A,
     stream_rows(A, x, e);
     double v[1] = {e.acc};
     block_partials<1>(v, partials);
-}
-
-__global__ __launch_bounds__(kBlock) void k_add_f32_to_f64(int n, const float* __restrict__ d,
-                                                           double* __restrict__ x)
-{
-    const RowRange rr = block_rows(n, gridDim.x, 1);
-    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock)
-        x[i] += double(d[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_norm(int n, const double* __restrict__ b,
@@ -307,6 +300,7 @@ int build(mi_solver* S)
             MI_HIP(hipGetLastError());
         }
         M->val32_valid = false;
+        M->val32d_valid = false;
         ++M->version;
         if (A->host_valid) {
             M->h_val.resize(size_t(nnz));
@@ -319,7 +313,7 @@ int build(mi_solver* S)
         }
     }
     if (mixed)
-        MI_TRY(M->ensure_fp32(S->stream));
+        MI_TRY(M->ensure_fp32_op(S->stream));
     if (!S->pc) {
         switch (S->cfg.preconditioner) {
         case MI_PC_AMG:
@@ -376,10 +370,11 @@ template <class Body> int iterate(mi_solver* S, KState* st, int max_iter, int ch
 }
 
 // One Krylov solve A x = b in element type T on the working matrix with the
-// vector pool w; the final KState is left in S->h_st.
+// vector pool w; the final KState is left in S->h_st.  With xacc set (FP32
+// inner solves) the iterate is added to that FP64 vector instead of x.
 template <class T>
 int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevBuf<T>* w,
-               double tol, int max_iter, bool warm)
+               double tol, int max_iter, bool warm, double* xacc = nullptr)
 {
     const Pattern& P = *S->M->pat;
     hipStream_t s = S->stream;
@@ -409,7 +404,9 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
         a.lanes = P.lanes;
         a.grid = range_grid(n, 256);
         a.blk = P.rb.blk.p;
+        a.diag = std::is_same<T, float>::value ? S->M->diag32() : nullptr;
         a.nrb = P.rb.nrb;
+        a.xacc = xacc;
         MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
         return iterate(S, a.st, max_iter, chunk, [&]() -> int {
             MI_TRY(bicg_stage1<T>(a, s));
@@ -439,6 +436,8 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
     a.grid = range_grid(n, 256);
     a.blk = P.rb.blk.p;
     a.nrb = P.rb.nrb;
+    a.diag = std::is_same<T, float>::value ? S->M->diag32() : nullptr;
+    a.xacc = xacc;
     const bool jacobi = invD != nullptr;
     if (jacobi) {
         MI_TRY(cg_init_jacobi<T>(a, warm, tol, max_iter, s));
@@ -533,7 +532,7 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     const int n = int(S->M->n());
     hipStream_t s = S->stream;
     float* r32 = S->w32[8].p; // outer residual b - A x in FP32
-    float* d32 = S->w32[9].p; // correction from the inner solve
+    float* d32 = S->w32[9].p; // unused FP32 iterate slot of the inner solve
     const bool warm = c.warm_start != 0;
     if (!warm)
         MI_HIP(hipMemsetAsync(x, 0, sizeof(double) * size_t(n), s));
@@ -551,16 +550,15 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
             break;
         }
         // inner: A d = r32 from d = 0 to inner_tol relative to ||r32||, or
-        // less when that already meets the outer target
+        // less when that already meets the outer target; x += d is done by
+        // the inner update kernels in FP64 (xacc)
         const double tol_in = std::max(inner_tol, 0.5 * c.tolerance * bnorm / rn);
-        MI_TRY(run_krylov<float>(S, S->M->val32.p, invD, r32, d32, S->w32, tol_in,
-                                 c.max_iterations - total, false));
+        MI_TRY(run_krylov<float>(S, S->M->op32(), invD, r32, d32, S->w32, tol_in,
+                                 c.max_iterations - total, false, x));
         const KState st = *S->h_st;
         copy_history(S, st.iter);
         total += st.iter;
         ++outer;
-        k_add_f32_to_f64<<<range_grid(n, kBlock), kBlock, 0, s>>>(n, d32, x);
-        MI_HIP(hipGetLastError());
         MI_TRY(true_residual(S, b, x, nullptr, r32, S->dscal.p));
         MI_TRY(read_scalars(S, 1));
         rn = S->h_scal[0];

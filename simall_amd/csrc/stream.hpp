@@ -12,7 +12,8 @@
 // CSRMatrix::spmv (src/solver/CSRMatrix.hpp:25-35) — so with contraction off
 // the FP64 results are identical to the reference's serial loop.  A row
 // longer than kStreamNnz is one block of its own, summed with a block-wide
-// reduction (fixed order, deterministic).
+// reduction (fixed order, deterministic).  FP32 matrices may instead carry
+// the difference form (stream_rows_diff), selected by StreamMat::diag.
 //
 // Workgroups are persistent over a contiguous range of row blocks; the range
 // of hardware block b follows xcd_logical_block so each XCD walks one slice
@@ -21,6 +22,7 @@
 
 #include "krylov.hpp"
 
+#include <type_traits>
 #include <vector>
 
 namespace mi
@@ -64,14 +66,91 @@ template <class T> struct StreamMat
     const int* rp;
     const int* ci;
     const T* val;
+    // FP32 only: position of the diagonal in each row when val holds the
+    // difference form (see stream_rows_diff); nullptr for the plain form.
+    const int* diag = nullptr;
 };
+
+// One 16 KB LDS buffer per workgroup shared by both row forms: 2048 FP64
+// products, or 2048 FP32 values plus 2048 gathered FP32 x entries.
+__device__ __forceinline__ double* stream_lds()
+{
+    __shared__ double buf[kStreamNnz];
+    return buf;
+}
+
+// Difference form for FP32 operators: A x = sum_{j != i} a_ij (x_j - x_i)
+// + d_i x_i, where the diagonal slot holds d_i = sum_j a_ij (the FP64 row
+// sum, rounded once).  Rounding the coefficients then perturbs the
+// conductances and the row sums separately instead of destroying the zero
+// row sums of a Neumann Laplacian: the FP32 operator stays spectrally within
+// FP32 rounding of the FP64 one, and smooth error is not amplified.
+template <class Epi>
+__device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
+                                                 const float* __restrict__ x, Epi& epi)
+{
+    float* av = reinterpret_cast<float*>(stream_lds());
+    float* xv = av + kStreamNnz;
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int per = (A.nrb + G - 1) / G;
+    const int rb0 = lb * per;
+    const int rb1 = min(A.nrb, rb0 + per);
+    const int tid = threadIdx.x;
+    for (int rb = rb0; rb < rb1; ++rb) {
+        const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
+        const int k0 = A.rp[r0], k1 = A.rp[r1];
+        if (k1 - k0 > kStreamNnz) {
+            const int kd = A.diag[r0];
+            const float xi = x[r0];
+            float s = 0.f;
+            for (int k = k0 + tid; k < k1; k += kBlock) {
+                const float a = __builtin_nontemporal_load(&A.val[k]);
+                const float xj = x[__builtin_nontemporal_load(&A.ci[k])];
+                s += (k == kd) ? a * xi : a * (xj - xi);
+            }
+            av[tid] = s;
+            __syncthreads();
+            if (tid == 0) {
+                float t = 0.f;
+                for (int j = 0; j < kBlock; ++j)
+                    t += av[j];
+                epi.row(r0, t);
+            }
+            __syncthreads();
+            continue;
+        }
+        for (int k = k0 + tid; k < k1; k += kBlock) {
+            av[k - k0] = __builtin_nontemporal_load(&A.val[k]);
+            xv[k - k0] = x[__builtin_nontemporal_load(&A.ci[k])];
+        }
+        __syncthreads();
+        const int i = r0 + tid;
+        if (i < r1) {
+            const int kd = A.diag[i] - k0;
+            const float xi = xv[kd];
+            float s = 0.f;
+            const int e = A.rp[i + 1] - k0;
+            for (int k = A.rp[i] - k0; k < e; ++k)
+                s += (k == kd) ? av[k] * xi : av[k] * (xv[k] - xi);
+            epi.row(i, s);
+        }
+        __syncthreads();
+    }
+}
 
 // Runs epi.row(i, sum_i) for every row of A handled by this workgroup.
 template <class T, class X, class Epi>
 __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __restrict__ x, Epi& epi)
 {
+    if constexpr (std::is_same<T, float>::value && std::is_same<X, float>::value) {
+        if (A.diag) {
+            stream_rows_diff(A, x, epi);
+            return;
+        }
+    }
     using Acc = typename acc_type<T>::type;
-    __shared__ Acc prod[kStreamNnz];
+    Acc* prod = reinterpret_cast<Acc*>(stream_lds());
     const int G = gridDim.x;
     const int lb = xcd_logical_block(blockIdx.x, G);
     const int per = (A.nrb + G - 1) / G;
