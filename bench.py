@@ -22,6 +22,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the FP64 SpMV from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes (scaled by a copy kernel of known size), written next to
+# the profile it came from.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_stream_spmv_fp64_poisson128_traffic.md")
+COPY_DOUBLES = 1 << 26  # calibration copy: 512 MiB read + 512 MiB written
 
 
 def log(*a):
@@ -31,6 +36,18 @@ def log(*a):
 def spmv_bytes(n, nnz, vbytes, ibytes=4):
     # values + column indices + row pointers + x read once + y written once
     return nnz * (vbytes + ibytes) + (n + 1) * 4 + 2 * n * vbytes
+
+
+def read_traffic():
+    """traffic_bytes_per_launch recorded from the counter passes, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(A_rp, A_ci, A_vals, b, iterations_full, sample_iters):
@@ -58,6 +75,11 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")
     ap.add_argument("--degree", type=int, default=0, help="AMG smoother degree (0: default)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--caller-order", action="store_true",
+                    help="also time SpMV on the caller's (randomly numbered) matrix")
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="time the working-matrix SpMV and a calibration copy only "
+                         "(short run for hardware-counter passes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -88,6 +110,20 @@ def main():
     d_b = capi.DeviceArray.from_host(b)
     d_x = capi.DeviceArray(n)
     pc = capi.MI_PC_AMG if args.pc == "amg" else capi.MI_PC_JACOBI
+    if args.spmv_only:
+        s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                        max_iterations=10, precision=capi.MI_PRECISION_FP64)
+        s.setup(A)
+        us64, by64 = s.time_spmv(capi.MI_PRECISION_FP64, reps=10)
+        src = capi.DeviceArray(COPY_DOUBLES)
+        dst = capi.DeviceArray(COPY_DOUBLES)
+        for _ in range(3):
+            capi.check(lib.mi_copy(src.ptr, dst.ptr, COPY_DOUBLES, None))
+        lib.mi_device_synchronize()
+        print(json.dumps({"spmv_fp64_us": us64, "spmv_fp64_algorithmic_bytes": by64,
+                          "copy_bytes_read": 8 * COPY_DOUBLES,
+                          "copy_bytes_written": 8 * COPY_DOUBLES}), flush=True)
+        return
 
     def make_solver(precision):
         kw = {}
@@ -130,14 +166,16 @@ def main():
     us64, by64 = ws.time_spmv(capi.MI_PRECISION_FP64, reps=50)
     us32, by32 = ws.time_spmv(capi.MI_PRECISION_MIXED, reps=50)
     gbs64, gbs32 = by64 / us64 / 1e3, by32 / us32 / 1e3
-    dx = capi.DeviceArray.from_host(np.ones(n))
-    dy = capi.DeviceArray(n)
-    us_raw = A.time_spmv(dx, dy, capi.MI_PRECISION_FP64, reps=20)
-    gbs_raw = spmv_bytes(n, nnz, 8) / us_raw / 1e3
+    gbs_raw = None
+    if args.caller_order:
+        dx = capi.DeviceArray.from_host(np.ones(n))
+        dy = capi.DeviceArray(n)
+        us_raw = A.time_spmv(dx, dy, capi.MI_PRECISION_FP64, reps=20)
+        gbs_raw = round(spmv_bytes(n, nnz, 8) / us_raw / 1e3, 1)
     info = ws.info()
     log(f"SpMV (reordered, bandwidth {info['bandwidth_before']} -> {info['bandwidth_after']}) "
         f"fp64 {us64:.1f} us {gbs64:.0f} GB/s ({100 * gbs64 / HBM_PEAK_GBS:.1f}%); fp32 {us32:.1f} us "
-        f"{gbs32:.0f} GB/s; caller ordering fp64 {us_raw:.1f} us {gbs_raw:.0f} GB/s; "
+        f"{gbs32:.0f} GB/s; caller ordering fp64 {gbs_raw} GB/s; "
         f"AMG levels {info['levels']} complexity {info['operator_complexity']:.2f}")
     err = None
     if "fp64" in results and "mixed" in results:
@@ -188,10 +226,12 @@ def main():
         "amg": {"levels": info["levels"], "operator_complexity": info["operator_complexity"]},
         "spmv": {"fp64_us": round(us64, 2), "fp64_GBps": round(gbs64, 1),
                  "fp32_us": round(us32, 2), "fp32_GBps": round(gbs32, 1),
-                 "caller_order_fp64_GBps": round(gbs_raw, 1),
+                 "caller_order_fp64_GBps": gbs_raw,
                  "bandwidth": [info["bandwidth_before"], info["bandwidth_after"]]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(), "kernel": "k_stream_spmv<double>",
+                     "algorithmic_bytes": by64, "avg_us": round(us64, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

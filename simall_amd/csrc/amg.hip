@@ -168,7 +168,8 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
     if (!kd.empty())
         MI_TRY(dg.upload(kd.data(), kd.size(), s));
     MI_TRY(v.upload(tv.data(), tv.size(), s));
-    MI_TRY(rb.upload(nr, h.rp.data(), s)); // synchronizes: tv is released on return
+    // the hierarchy's operators need no column-order sums: long rows get lanes
+    MI_TRY(rb.upload(nr, h.rp.data(), s, stream_lanes_per_row(nr, nnz, true))); // synchronizes
     return MI_OK;
 }
 
@@ -317,7 +318,10 @@ struct AmgPrecond final : Precond
         H.rp = A->pat->h_rp;
         H.ci = A->pat->h_ci;
         H.v = A->h_val;
-        amg_build(H, params, host);
+        if (amg_build(H, params, host) != 0)
+            return set_error(MI_ERR_UNSUPPORTED,
+                             "AMG coarsening stalled above %d rows (no dense coarsest solve)",
+                             kAmgMaxDense);
         has64 = has32 = false;
         if (need64) {
             MI_TRY(h64.build(host, degree, lmin_ratio, s));

@@ -25,7 +25,7 @@ template <class T> struct DMat
 #if defined(__HIPCC__)
     StreamMat<T> sm() const
     {
-        return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr};
+        return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr, rb.lpr};
     }
 #endif
 };

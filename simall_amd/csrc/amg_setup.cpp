@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <barrier>
 #include <thread>
 
 namespace mi
@@ -362,55 +363,79 @@ void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads)
     }
 }
 
-int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv)
+int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv, int threads)
 {
     // Gauss-Jordan elimination with partial pivoting on [A | I].  A pivot that
     // vanishes (a singular coarse operator, e.g. an unpinned Neumann problem)
     // is replaced by a tiny multiple of the largest diagonal so the coarse
     // correction stays finite; the Krylov outer iteration absorbs it.
+    // One team of threads splits the row updates of every pivot step; thread
+    // 0 picks, swaps and scales the pivot row between two barriers.
     inv.assign(size_t(n) * size_t(n), 0.0);
-    for (int i = 0; i < n; ++i)
-        inv[size_t(i) * size_t(n) + size_t(i)] = 1.0;
+    const size_t N = size_t(n);
+    for (size_t i = 0; i < N; ++i)
+        inv[i * N + i] = 1.0;
     double amax = 0.0;
-    for (int i = 0; i < n; ++i)
-        amax = std::max(amax, std::abs(a[size_t(i) * size_t(n) + size_t(i)]));
+    for (size_t i = 0; i < N; ++i)
+        amax = std::max(amax, std::abs(a[i * N + i]));
     int regularised = 0;
-    for (int c = 0; c < n; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < n; ++r)
-            if (std::abs(a[size_t(r) * size_t(n) + size_t(c)]) >
-                std::abs(a[size_t(piv) * size_t(n) + size_t(c)]))
+    const int team = (n >= 256) ? std::max(1, std::min(threads, 16)) : 1;
+    std::barrier sync(team);
+    auto pivot_step = [&](size_t c) {
+        size_t piv = c;
+        for (size_t r = c + 1; r < N; ++r)
+            if (std::abs(a[r * N + c]) > std::abs(a[piv * N + c]))
                 piv = r;
         if (piv != c) {
-            for (int k = 0; k < n; ++k) {
-                std::swap(a[size_t(c) * size_t(n) + size_t(k)], a[size_t(piv) * size_t(n) + size_t(k)]);
-                std::swap(inv[size_t(c) * size_t(n) + size_t(k)],
-                          inv[size_t(piv) * size_t(n) + size_t(k)]);
+            for (size_t k = 0; k < N; ++k) {
+                std::swap(a[c * N + k], a[piv * N + k]);
+                std::swap(inv[c * N + k], inv[piv * N + k]);
             }
         }
-        double p = a[size_t(c) * size_t(n) + size_t(c)];
+        double p = a[c * N + c];
         if (std::abs(p) <= 1e-13 * amax) {
             p = (p >= 0 ? 1.0 : -1.0) * 1e-10 * (amax > 0 ? amax : 1.0);
-            a[size_t(c) * size_t(n) + size_t(c)] = p;
+            a[c * N + c] = p;
             ++regularised;
         }
         const double ip = 1.0 / p;
-        for (int k = 0; k < n; ++k) {
-            a[size_t(c) * size_t(n) + size_t(k)] *= ip;
-            inv[size_t(c) * size_t(n) + size_t(k)] *= ip;
+        for (size_t k = 0; k < N; ++k) {
+            a[c * N + k] *= ip;
+            inv[c * N + k] *= ip;
         }
-        for (int r = 0; r < n; ++r) {
-            if (r == c)
-                continue;
-            const double f = a[size_t(r) * size_t(n) + size_t(c)];
-            if (f == 0.0)
-                continue;
-            for (int k = 0; k < n; ++k) {
-                a[size_t(r) * size_t(n) + size_t(k)] -= f * a[size_t(c) * size_t(n) + size_t(k)];
-                inv[size_t(r) * size_t(n) + size_t(k)] -= f * inv[size_t(c) * size_t(n) + size_t(k)];
+    };
+    auto worker = [&](int t) {
+        const size_t chunk = (N + size_t(team) - 1) / size_t(team);
+        const size_t r0 = size_t(t) * chunk, r1 = std::min(N, r0 + chunk);
+        for (size_t c = 0; c < N; ++c) {
+            if (t == 0)
+                pivot_step(c);
+            sync.arrive_and_wait();
+            const double* ac = &a[c * N];
+            const double* ic = &inv[c * N];
+            for (size_t r = r0; r < r1; ++r) {
+                if (r == c)
+                    continue;
+                double* ar = &a[r * N];
+                const double f = ar[c];
+                if (f == 0.0)
+                    continue;
+                // columns < c of the pivot row are already zero
+                for (size_t k = c; k < N; ++k)
+                    ar[k] -= f * ac[k];
+                double* ir = &inv[r * N];
+                for (size_t k = 0; k < N; ++k)
+                    ir[k] -= f * ic[k];
             }
+            sync.arrive_and_wait();
         }
-    }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < team; ++t)
+        pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool)
+        th.join();
     return regularised;
 }
 
@@ -424,7 +449,7 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
     for (int lev = 0; lev + 1 < p.max_levels; ++lev) {
         AmgLevelHost& L = out.levels.back();
         const HCsr& A = L.A;
-        if (A.nr <= p.coarse_size)
+        if (A.nr <= std::min(p.coarse_size, kAmgMaxDense))
             break;
         std::vector<int> srp, sci, agg;
         amg_strength(A, p.theta, srp, sci);
@@ -455,11 +480,13 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
     AmgLevelHost& C = out.levels.back();
     C.lmax = amg_lambda_max(C.A, 20) * 1.05;
     const int nc = C.A.nr;
+    if (nc > kAmgMaxDense)
+        return 1; // coarsening stalled: the coarsest level is too large for a dense solve
     std::vector<double> dense(size_t(nc) * size_t(nc), 0.0);
     for (int i = 0; i < nc; ++i)
         for (int k = C.A.rp[size_t(i)]; k < C.A.rp[size_t(i) + 1]; ++k)
             dense[size_t(i) * size_t(nc) + size_t(C.A.ci[size_t(k)])] += C.A.v[size_t(k)];
-    dense_inverse(nc, dense, out.coarse_inverse);
+    dense_inverse(nc, dense, out.coarse_inverse, threads);
     out.coarse_n = nc;
     out.operator_complexity = double(nnz_total) / double(std::max<int64_t>(A0.nnz(), 1));
     out.grid_complexity = double(rows_total) / double(std::max(A0.nr, 1));

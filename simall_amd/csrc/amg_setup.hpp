@@ -17,10 +17,13 @@ struct HCsr
     int64_t nnz() const { return int64_t(ci.size()); }
 };
 
+// Largest coarsest level inverted densely (128 MiB of FP64).
+constexpr int kAmgMaxDense = 4096;
+
 struct AmgParams
 {
     int max_levels = 10;
-    int coarse_size = 400;
+    int coarse_size = 2048; // stop coarsening at this many rows (<= kAmgMaxDense)
     double theta = 0.02;        // strength: |a_ij| >= theta * sqrt(|a_ii a_jj|)
     double omega_scale = 4.0 / 3.0; // prolongator smoothing weight * lambda_max(D^-1 A)
     int threads = 0;            // 0: hardware concurrency (at most 16)
@@ -56,6 +59,6 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
 void csr_transpose(const HCsr& A, HCsr& T);
 void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads);
 double amg_lambda_max(const HCsr& A, int iters);
-int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv);
+int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv, int threads = 1);
 
 } // namespace mi

@@ -650,7 +650,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->reorder = -1;
     c->check_interval = 0;
     c->amg_max_levels = 10;
-    c->amg_coarse_size = 256;
+    c->amg_coarse_size = 2048;
     c->amg_strength = 0.02;
     c->amg_jacobi_omega = 4.0 / 3.0;
     c->amg_smoother = 0;

@@ -38,14 +38,14 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, const int* __restric
         r[row] = b[row] - s;
 }
 
-void build_row_blocks(int n, const int* rp, std::vector<int>& blk)
+void build_row_blocks(int n, const int* rp, std::vector<int>& blk, int max_rows)
 {
     blk.clear();
     blk.push_back(0);
     int r = 0;
     while (r < n) {
         const int start = r;
-        while (r < n && r - start < kStreamRows && rp[r + 1] - rp[start] <= kStreamNnz)
+        while (r < n && r - start < max_rows && rp[r + 1] - rp[start] <= kStreamNnz)
             ++r;
         if (r == start)
             ++r; // a row longer than kStreamNnz gets a block of its own
@@ -53,10 +53,27 @@ void build_row_blocks(int n, const int* rp, std::vector<int>& blk)
     }
 }
 
-int RowBlocks::upload(int n, const int* rp_host, hipStream_t s)
+int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide)
 {
+    if (!wide || n <= 0)
+        return 1;
+    const double m = double(nnz) / double(n);
+    if (m < 10.0)
+        return 1;
+    int l = 1;
+    while (l < 8 && 2 * l <= m / 3.0)
+        l *= 2;
+    return l;
+}
+
+int RowBlocks::upload(int n, const int* rp_host, hipStream_t s, int lanes_per_row)
+{
+    if (lanes_per_row < 1 || lanes_per_row > 64 || (lanes_per_row & (lanes_per_row - 1)))
+        return set_error(MI_ERR_BAD_ARG, "lanes per row %d is not a power of two in [1, 64]",
+                         lanes_per_row);
+    lpr = lanes_per_row;
     std::vector<int> b;
-    build_row_blocks(n, rp_host, b);
+    build_row_blocks(n, rp_host, b, kStreamRows / lpr);
     nrb = int(b.size()) - 1;
     MI_TRY(blk.upload(b.data(), b.size(), s));
     return hipStreamSynchronize(s) == hipSuccess ? MI_OK

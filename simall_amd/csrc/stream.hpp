@@ -7,7 +7,9 @@
 //      and parks it in LDS;
 //   2. gives each row to one thread, which adds its products in column order
 //      and hands (row, sum) to an epilogue (store y, update a smoother or
-//      Krylov vector, accumulate a dot product).
+//      Krylov vector, accumulate a dot product).  Matrices with long rows
+//      that need no bit-compatibility (the AMG hierarchy's own operators)
+//      give each row lpr lanes instead (StreamMat::lpr).
 // Products are rounded and then added left to right — the arithmetic of
 // CSRMatrix::spmv (src/solver/CSRMatrix.hpp:25-35) — so with contraction off
 // the FP64 results are identical to the reference's serial loop.  A row
@@ -30,15 +32,24 @@ namespace mi
 
 constexpr int kStreamRows = 256; // rows per row block (one per thread)
 constexpr int kStreamNnz = 2048; // nonzeros per row block (LDS products)
+constexpr int kLoadsPerThread = kStreamNnz / kBlock; // loads per thread per row block
 
-// Host partition of rows into row blocks: rows blk[b] .. blk[b+1].
-void build_row_blocks(int n, const int* rp, std::vector<int>& blk);
+// Host partition of rows into row blocks of at most max_rows rows and
+// kStreamNnz nonzeros: rows blk[b] .. blk[b+1].
+void build_row_blocks(int n, const int* rp, std::vector<int>& blk, int max_rows = kStreamRows);
+
+// Lanes per row for the summation phase: 1 (column-order sums, the
+// reference's arithmetic) unless `wide` and the mean row holds 10 or more
+// nonzeros, then the largest power of two <= mean/3, at most 8 (measured
+// best on the SA-AMG levels of a 2M-row 3D pressure system).
+int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide);
 
 struct RowBlocks
 {
     DevBuf<int> blk;
     int nrb = 0;
-    int upload(int n, const int* rp_host, hipStream_t s);
+    int lpr = 1; // lanes per row (a power of two); rows per block <= kStreamRows / lpr
+    int upload(int n, const int* rp_host, hipStream_t s, int lanes_per_row = 1);
 };
 
 // Workgroups for a stream launch: enough to cover the row blocks, a multiple
@@ -69,7 +80,19 @@ template <class T> struct StreamMat
     // FP32 only: position of the diagonal in each row when val holds the
     // difference form (see stream_rows_diff); nullptr for the plain form.
     const int* diag = nullptr;
+    // Lanes per row of the summation phase (power of two, the RowBlocks'
+    // lpr).  1 sums each row in column order on one thread; more lanes take
+    // every lpr-th product and combine with xor shuffles (fixed order).
+    int lpr = 1;
 };
+
+// Sum over the lpr lanes of a row group (aligned within the wavefront).
+template <class Acc> __device__ __forceinline__ Acc lane_group_sum(Acc s, int lpr)
+{
+    for (int off = lpr >> 1; off >= 1; off >>= 1)
+        s += __shfl_xor(s, off);
+    return s;
+}
 
 // One 16 KB LDS buffer per workgroup shared by both row forms: 2048 FP64
 // products, or 2048 FP32 values plus 2048 gathered FP32 x entries.
@@ -77,6 +100,38 @@ __device__ __forceinline__ double* stream_lds()
 {
     __shared__ double buf[kStreamNnz];
     return buf;
+}
+
+// Column/value streams of one row block (nonzeros k0..k1, at most
+// kStreamNnz) into registers: kLoadsPerThread independent loads per thread,
+// c[j] = -1 past the end.
+template <class T>
+__device__ __forceinline__ void stream_load(const StreamMat<T>& A, int k0, int k1, int tid,
+                                            int (&c)[kLoadsPerThread], T (&a)[kLoadsPerThread])
+{
+#pragma unroll
+    for (int j = 0; j < kLoadsPerThread; ++j) {
+        const int k = k0 + tid + j * kBlock;
+        c[j] = k < k1 ? __builtin_nontemporal_load(&A.ci[k]) : -1;
+        a[j] = k < k1 ? __builtin_nontemporal_load(&A.val[k]) : T(0);
+    }
+}
+
+// Prefetch of the block after rb (same workgroup range) when it is a regular
+// block; returns whether c/a now hold it.  The loads stay in flight while
+// the current block's rows are summed.
+template <class T>
+__device__ __forceinline__ bool stream_prefetch(const StreamMat<T>& A, int rb, int rb1, int k1,
+                                                int tid, int (&c)[kLoadsPerThread],
+                                                T (&a)[kLoadsPerThread])
+{
+    if (rb + 1 >= rb1)
+        return false;
+    const int q1 = A.rp[A.blk[rb + 2]];
+    if (q1 - k1 > kStreamNnz)
+        return false;
+    stream_load(A, k1, q1, tid, c, a);
+    return true;
 }
 
 // Difference form for FP32 operators: A x = sum_{j != i} a_ij (x_j - x_i)
@@ -97,6 +152,9 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
     const int rb0 = lb * per;
     const int rb1 = min(A.nrb, rb0 + per);
     const int tid = threadIdx.x;
+    int c[kLoadsPerThread];
+    float a[kLoadsPerThread];
+    bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
         const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
         const int k0 = A.rp[r0], k1 = A.rp[r1];
@@ -105,9 +163,9 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             const float xi = x[r0];
             float s = 0.f;
             for (int k = k0 + tid; k < k1; k += kBlock) {
-                const float a = __builtin_nontemporal_load(&A.val[k]);
+                const float v = __builtin_nontemporal_load(&A.val[k]);
                 const float xj = x[__builtin_nontemporal_load(&A.ci[k])];
-                s += (k == kd) ? a * xi : a * (xj - xi);
+                s += (k == kd) ? v * xi : v * (xj - xi);
             }
             av[tid] = s;
             __syncthreads();
@@ -118,23 +176,35 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
                 epi.row(r0, t);
             }
             __syncthreads();
+            have = false;
             continue;
         }
-        for (int k = k0 + tid; k < k1; k += kBlock) {
-            av[k - k0] = __builtin_nontemporal_load(&A.val[k]);
-            xv[k - k0] = x[__builtin_nontemporal_load(&A.ci[k])];
+        if (!have)
+            stream_load(A, k0, k1, tid, c, a);
+#pragma unroll
+        for (int j = 0; j < kLoadsPerThread; ++j) {
+            if (c[j] >= 0) {
+                av[tid + j * kBlock] = a[j];
+                xv[tid + j * kBlock] = x[c[j]];
+            }
         }
         __syncthreads();
-        const int i = r0 + tid;
+        have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
+        const int lpr = A.lpr;
+        const int sub = tid & (lpr - 1);
+        const int i = r0 + tid / lpr;
+        float s = 0.f;
         if (i < r1) {
             const int kd = A.diag[i] - k0;
             const float xi = xv[kd];
-            float s = 0.f;
             const int e = A.rp[i + 1] - k0;
-            for (int k = A.rp[i] - k0; k < e; ++k)
+            for (int k = A.rp[i] - k0 + sub; k < e; k += lpr)
                 s += (k == kd) ? av[k] * xi : av[k] * (xv[k] - xi);
-            epi.row(i, s);
         }
+        if (lpr > 1)
+            s = lane_group_sum(s, lpr);
+        if (i < r1 && sub == 0)
+            epi.row(i, s);
         __syncthreads();
     }
 }
@@ -157,6 +227,9 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
     const int rb0 = lb * per;
     const int rb1 = min(A.nrb, rb0 + per);
     const int tid = threadIdx.x;
+    int c[kLoadsPerThread];
+    T a[kLoadsPerThread];
+    bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
         const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
         const int k0 = A.rp[r0], k1 = A.rp[r1];
@@ -175,20 +248,34 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
                 epi.row(r0, t);
             }
             __syncthreads();
+            have = false;
             continue;
         }
-        for (int k = k0 + tid; k < k1; k += kBlock)
-            prod[k - k0] = Acc(__builtin_nontemporal_load(&A.val[k])) *
-                           Acc(x[__builtin_nontemporal_load(&A.ci[k])]);
+        if (!have)
+            stream_load(A, k0, k1, tid, c, a);
+#pragma unroll
+        for (int j = 0; j < kLoadsPerThread; ++j)
+            if (c[j] >= 0)
+                prod[tid + j * kBlock] = Acc(a[j]) * Acc(x[c[j]]);
         __syncthreads();
-        const int i = r0 + tid;
+        // FP32 blocks are short on bytes per synchronisation, so the next
+        // block's streams are prefetched; for FP64 the extra live registers
+        // cost more occupancy than the overlap gains (measured)
+        if constexpr (sizeof(T) == 4)
+            have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
+        const int lpr = A.lpr;
+        const int sub = tid & (lpr - 1);
+        const int i = r0 + tid / lpr;
+        Acc s = 0;
         if (i < r1) {
-            Acc s = 0;
             const int e = A.rp[i + 1] - k0;
-            for (int k = A.rp[i] - k0; k < e; ++k)
+            for (int k = A.rp[i] - k0 + sub; k < e; k += lpr)
                 s += prod[k];
-            epi.row(i, s);
         }
+        if (lpr > 1)
+            s = lane_group_sum(s, lpr);
+        if (i < r1 && sub == 0)
+            epi.row(i, s);
         __syncthreads();
     }
 }
