@@ -51,14 +51,24 @@ template <class T> struct EpiCheb
     T* x;
     T beta, alpha;
     int mode;
+    T ri0 = 0, di = 0, id = 0, xi = 0;
+    __device__ __forceinline__ void pre(int i)
+    {
+        ri0 = r[i];
+        if (mode == 0) {
+            di = din[i];
+            id = invD[i];
+            xi = x[i];
+        }
+    }
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
     {
-        const T ri = r[i] - T(s);
+        const T ri = ri0 - T(s);
         r[i] = ri;
         if (mode == 0) {
-            const T dn = beta * din[i] + alpha * invD[i] * ri;
+            const T dn = beta * di + alpha * id * ri;
             dout[i] = dn;
-            x[i] += dn;
+            x[i] = xi + dn;
         }
     }
 };
@@ -78,10 +88,12 @@ template <class T> struct EpiProlong
 {
     T* e;
     T* x;
+    T xi = 0;
+    __device__ __forceinline__ void pre(int i) { xi = x[i]; }
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
     {
         e[i] = T(s);
-        x[i] += T(s);
+        x[i] = xi + T(s);
     }
 };
 
@@ -97,6 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong(StreamMat<T> P, const T* __r
 template <class T> struct EpiStoreT
 {
     T* y;
+    __device__ __forceinline__ void pre(int) {}
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
 };
 

@@ -238,11 +238,13 @@ template <class T> struct EpiCgAp
     const T* p;
     T* Ap;
     double acc = 0.0;
+    T pi = 0;
+    __device__ __forceinline__ void pre(int i) { pi = p[i]; }
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
     {
         const T ap = T(s);
         Ap[i] = ap;
-        acc += double(p[i]) * double(ap);
+        acc += double(pi) * double(ap);
     }
 };
 
@@ -441,11 +443,13 @@ template <class T> struct EpiBi
     T* out;
     int mode;
     double acc0 = 0.0, acc1 = 0.0;
+    T wi = 0;
+    __device__ __forceinline__ void pre(int i) { wi = w1[i]; }
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type sres)
     {
         const T o = T(sres);
         out[i] = o;
-        acc0 += double(w1[i]) * double(o);
+        acc0 += double(wi) * double(o);
         if (mode == 1)
             acc1 += double(o) * double(o);
     }

@@ -104,9 +104,11 @@ struct EpiResidual
     double* r64;
     float* r32;
     double acc = 0.0;
+    double bi = 0.0;
+    __device__ __forceinline__ void pre(int i) { bi = b[i]; }
     __device__ __forceinline__ void row(int i, double s)
     {
-        const double ri = b[i] - s;
+        const double ri = bi - s;
         if (r64)
             r64[i] = ri;
         if (r32)

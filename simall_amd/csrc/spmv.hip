@@ -83,6 +83,7 @@ int RowBlocks::upload(int n, const int* rp_host, hipStream_t s, int lanes_per_ro
 template <class T> struct EpiStore
 {
     T* y;
+    __device__ __forceinline__ void pre(int) {}
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
 };
 

@@ -159,6 +159,8 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
         const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
         const int k0 = A.rp[r0], k1 = A.rp[r1];
         if (k1 - k0 > kStreamNnz) {
+            if (tid == 0)
+                epi.pre(r0);
             const int kd = A.diag[r0];
             const float xi = x[r0];
             float s = 0.f;
@@ -179,6 +181,14 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             have = false;
             continue;
         }
+        // this thread's row and its epilogue operands, loaded before the
+        // products so their latency overlaps the gathers and the barrier
+        const int lpr = A.lpr;
+        const int sub = tid & (lpr - 1);
+        const int i = r0 + tid / lpr;
+        const bool mine = i < r1 && sub == 0;
+        if (mine)
+            epi.pre(i);
         if (!have)
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
@@ -190,9 +200,6 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
         }
         __syncthreads();
         have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
-        const int lpr = A.lpr;
-        const int sub = tid & (lpr - 1);
-        const int i = r0 + tid / lpr;
         float s = 0.f;
         if (i < r1) {
             const int kd = A.diag[i] - k0;
@@ -203,13 +210,15 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
         }
         if (lpr > 1)
             s = lane_group_sum(s, lpr);
-        if (i < r1 && sub == 0)
+        if (mine)
             epi.row(i, s);
         __syncthreads();
     }
 }
 
-// Runs epi.row(i, sum_i) for every row of A handled by this workgroup.
+// Runs epi.row(i, sum_i) for every row of A handled by this workgroup.  The
+// epilogue's pre(i) is called first, on the same thread, at the start of the
+// row's block: it loads the per-row operands row(i, .) will use.
 template <class T, class X, class Epi>
 __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __restrict__ x, Epi& epi)
 {
@@ -235,6 +244,8 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
         const int k0 = A.rp[r0], k1 = A.rp[r1];
         if (k1 - k0 > kStreamNnz) {
             // one long row: strided partial sums, fixed-order block reduction
+            if (tid == 0)
+                epi.pre(r0);
             Acc s = 0;
             for (int k = k0 + tid; k < k1; k += kBlock)
                 s += Acc(__builtin_nontemporal_load(&A.val[k])) *
@@ -251,6 +262,14 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
             have = false;
             continue;
         }
+        // this thread's row and its epilogue operands, loaded before the
+        // products so their latency overlaps the gathers and the barrier
+        const int lpr = A.lpr;
+        const int sub = tid & (lpr - 1);
+        const int i = r0 + tid / lpr;
+        const bool mine = i < r1 && sub == 0;
+        if (mine)
+            epi.pre(i);
         if (!have)
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
@@ -263,9 +282,6 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
         // cost more occupancy than the overlap gains (measured)
         if constexpr (sizeof(T) == 4)
             have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
-        const int lpr = A.lpr;
-        const int sub = tid & (lpr - 1);
-        const int i = r0 + tid / lpr;
         Acc s = 0;
         if (i < r1) {
             const int e = A.rp[i + 1] - k0;
@@ -274,7 +290,7 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
         }
         if (lpr > 1)
             s = lane_group_sum(s, lpr);
-        if (i < r1 && sub == 0)
+        if (mine)
             epi.row(i, s);
         __syncthreads();
     }
