@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--pc", default="amg", choices=["jacobi", "amg"])
-    ap.add_argument("--cpu-iters", type=int, default=150)
+    ap.add_argument("--cpu-iters", type=int, default=900)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")
     ap.add_argument("--degree", type=int, default=0, help="AMG smoother degree (0: default)")
