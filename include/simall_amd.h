@@ -129,6 +129,13 @@ MI_API int mi_csr_spmv_device(mi_csr* A, int precision, const void* x_device, vo
  * returns the average launch duration in microseconds. */
 MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x_device, void* y_device,
                             int reps, double* avg_us);
+/* Explicit transpose A^T as a new handle: the transposed pattern is built on
+ * the host once, values are gathered on the device.  Replaces the scatter
+ * product adjoint::spmv_transpose (src/adjoint/DiscreteAdjoint.cpp:13-22)
+ * with a CSR of A^T that every SpMV-shaped kernel can stream.  At must not
+ * outlive A; mi_csr_transpose_values refreshes At after A's values change. */
+MI_API int mi_csr_transpose(const mi_csr* A, mi_csr** At);
+MI_API int mi_csr_transpose_values(const mi_csr* A, mi_csr* At);
 MI_API int mi_csr_destroy(mi_csr* A);
 
 /* ---------------------------------------------------------------- solvers
@@ -158,6 +165,8 @@ typedef struct {
     double amg_jacobi_omega; /* prolongator smoothing weight (scaled by 1/rho(D^-1A)) */
     int amg_smoother;      /* 0 = Chebyshev, 1 = damped Jacobi */
     int amg_sweeps;        /* pre = post sweeps / Chebyshev degree */
+    int keep_preconditioner; /* 1: new values of the bound matrix keep the preconditioner
+                                built on earlier values (the operator is always current) */
 } mi_solver_cfg;
 
 /* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */
@@ -201,6 +210,14 @@ MI_API int mi_solver_info(const mi_solver* s, int* levels, double* operator_comp
  * pointers + x once + y). */
 MI_API int mi_solver_time_spmv(mi_solver* s, int precision, int reps, double* avg_us,
                                double* bytes);
+/* Adjoint solve A^T lambda = g (device vectors) with the solver's
+ * configuration on the explicit transpose of the matrix bound by
+ * mi_solver_setup; the transpose and its preconditioner are rebuilt when the
+ * matrix's values change.  The adjoint-state solve of
+ * adjoint::solve_discrete_adjoint (src/adjoint/DiscreteAdjoint.cpp:150-168),
+ * which uses GMRES(30) on A^T there. */
+MI_API int mi_adjoint_solve(mi_solver* s, const double* g_device, double* lambda_device,
+                            mi_stats* stats);
 /* ILinearSolver::last_residual (src/solver/Solver.hpp:99). */
 MI_API double mi_solver_last_residual(const mi_solver* s);
 /* Relative residual after each Krylov iteration of the last solve. */
@@ -293,6 +310,19 @@ MI_API int mi_assemble_momentum(mi_mesh* m, int comp, double mu, double rho,
                                 const double* dt_device, const double* Uold_device,
                                 const double* Ucomp_device, double urf,
                                 const double* bc_value_host, mi_csr* A, double* rhs_device);
+/* Per-face gradient of an objective J(x) through the pressure system of
+ * mi_assemble_pressure with face scales gamma: grad[f] = dJ/dgamma_f =
+ * -lambda^T (dA/dgamma_f) x, where A^T lambda = dJ/dx (mi_adjoint_solve).
+ * An interior face adds gamma_f G_f (e_o - e_n)(e_o - e_n)^T to A, a
+ * pressure-boundary face gamma_f G_f e_o e_o^T, G_f = rho |S_f|^2 /
+ * max(aP_f |d_f . S_f|, 1e-30) (src/solver/SimpleAlgorithm.cpp:373-397,
+ * 420-431); with `pinned` the pinned cell's diagonal carries the factor 1e6
+ * (:441-446).  Other faces get 0.  The gradient step
+ * dJ/dalpha - (dR/dalpha)^T psi of src/adjoint/DiscreteAdjoint.cpp:170-182.
+ * Device arrays: aP[n_cells] (nullable), x, lambda [n_cells], grad[n_faces]. */
+MI_API int mi_face_gradient(mi_mesh* m, double rho, const double* aP_device, int pinned,
+                            const double* x_device, const double* lambda_device,
+                            double* grad_device);
 /* SimpleAlgorithm::compute_face_fluxes (src/solver/SimpleAlgorithm.cpp:201-237). */
 MI_API int mi_face_flux(mi_mesh* m, double rho, const double* Ux_device, const double* Uy_device,
                         const double* Uz_device, const double* bc_value_host, double* phi_device);

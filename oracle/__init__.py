@@ -27,6 +27,9 @@ How each is pinned (tests/test_oracle.py):
   assemble_momentum  zero-flux case equals pressure-style diffusion; upwind row sums equal net
                      outflow; M-matrix sign pattern
   face_flux          closed form for a uniform velocity field (phi = U.S)
+  face_gradient      central finite differences of J = sum w x^2 through assemble_pressure and a
+                     dense solve (pinned and pressure-outlet cases); closed form
+                     -G (lam_o - lam_n)(x_o - x_n) on unpinned faces (tests/test_adjoint_oracle.py)
 """
 from __future__ import annotations
 
@@ -85,6 +88,7 @@ def load():
             "orc_assemble_momentum": (None, [P(OrcMesh), vp, i, vp, i, d, d, vp, vp, vp, vp, d,
                                              P(OrcCsr), vp, vp]),
             "orc_face_flux": (None, [P(OrcMesh), vp, i, vp, d, vp, vp, vp, vp]),
+            "orc_face_gradient": (None, [P(OrcMesh), vp, i, d, vp, i, vp, vp, vp]),
             "orc_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
             "orc_uniform": (d, [P(C.c_uint64)]),
             "orc_normal": (d, [P(C.c_uint64)]),
@@ -275,6 +279,19 @@ def assemble_pressure(mesh, pattern, bc_of_zone, rho=1.0, aP=None, gamma=None, p
     pinned = load().orc_assemble_pressure(C.byref(ms), _p(bc), bc.size, rho, _p(aP), _p(gamma), _p(phi),
                                           C.byref(pattern.s), _p(vals), _p(rhs))
     return vals, rhs, bool(pinned)
+
+
+def face_gradient(mesh, bc_of_zone, x, lam, pinned, rho=1.0, aP=None):
+    """dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for every face (pressure matrix)."""
+    ms = mesh_struct(mesh)
+    bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    x = _vec(x)
+    lam = _vec(lam)
+    aP = None if aP is None else _vec(aP)
+    grad = np.empty(mesh.n_faces)
+    load().orc_face_gradient(C.byref(ms), _p(bc), bc.size, rho, _p(aP), int(pinned), _p(x),
+                             _p(lam), _p(grad))
+    return grad
 
 
 def assemble_momentum(mesh, pattern, bc_of_zone, comp, mu, rho, phi, dt=None, Uold=None,

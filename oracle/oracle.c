@@ -797,6 +797,66 @@ int orc_assemble_pressure(const orc_mesh* m, const int* bcOfZone, int nZones,
     return 0;
 }
 
+/* Gradient step of the discrete adjoint for the pressure matrix:
+ * grad[f] = -lambda^T (dA/dgamma_f) x, the term -(dR/dalpha)^T psi of
+ * adjoint::solve_discrete_adjoint (src/adjoint/DiscreteAdjoint.cpp:170-182)
+ * with R = A(gamma) x - b and alpha = gamma.  dA/dgamma_f is the face's
+ * contribution in orc_assemble_pressure with gamma_f = 1 (the reference's
+ * coefficient, src/solver/SimpleAlgorithm.cpp:373-397, 420-431), entry by
+ * entry, and the pinned diagonal scaled by 1e6 (:441-446). */
+void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, double rho,
+                       const double* aP, int pinned, const double* x, const double* lam,
+                       double* grad)
+{
+    for (int64_t f = 0; f < m->n_faces; ++f) {
+        const uint64_t o = m->owner[f];
+        const uint64_t nb = m->neighbor[f];
+        const double Ax = m->areaX[f], Ay = m->areaY[f], Az = m->areaZ[f];
+        const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
+        /* entries (row, col, d value / d gamma_f) */
+        int rows[4], cols[4];
+        double dv[4];
+        int ne = 0;
+        if (nb != UINT64_MAX) {
+            const double dx = m->cellCx[nb] - m->cellCx[o];
+            const double dy = m->cellCy[nb] - m->cellCy[o];
+            const double dz = m->cellCz[nb] - m->cellCz[o];
+            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+            double aPf = 1.0;
+            if (aP) {
+                const double dO = aP[o] > 1e-30 ? aP[o] : 1e-30;
+                const double dN = aP[nb] > 1e-30 ? aP[nb] : 1e-30;
+                aPf = 0.5 * (dO + dN);
+            }
+            const double den = aPf * ddotA;
+            const double dcoef = rho * Amag2 / (den > 1e-30 ? den : 1e-30);
+            rows[0] = (int)o;  cols[0] = (int)o;  dv[0] = dcoef;
+            rows[1] = (int)o;  cols[1] = (int)nb; dv[1] = -dcoef;
+            rows[2] = (int)nb; cols[2] = (int)nb; dv[2] = dcoef;
+            rows[3] = (int)nb; cols[3] = (int)o;  dv[3] = -dcoef;
+            ne = 4;
+        } else if (zone_bc(bcOfZone, nZones, m->boundaryZone[f]) == ORC_BC_PRESSURE) {
+            const double dx = m->faceCx[f] - m->cellCx[o];
+            const double dy = m->faceCy[f] - m->cellCy[o];
+            const double dz = m->faceCz[f] - m->cellCz[o];
+            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+            const double dO = aP ? (aP[o] > 1e-30 ? aP[o] : 1e-30) : 1.0;
+            const double den = dO * ddotA;
+            rows[0] = (int)o; cols[0] = (int)o;
+            dv[0] = rho * Amag2 / (den > 1e-30 ? den : 1e-30);
+            ne = 1;
+        }
+        double t = 0.0;
+        for (int e = 0; e < ne; ++e) {
+            double v = dv[e];
+            if (pinned && rows[e] == 0 && cols[e] == 0)
+                v *= 1.0e6;
+            t += lam[rows[e]] * v * x[cols[e]];
+        }
+        grad[f] = -t;
+    }
+}
+
 /* SimpleAlgorithm::assemble_momentum, src/solver/SimpleAlgorithm.cpp:240-355,
  * face loop :253-303, transient :320-337 (implicit Euler), URF :340-348. */
 void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,

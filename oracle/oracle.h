@@ -141,6 +141,12 @@ int orc_assemble_pressure(const orc_mesh* m, const int* bcOfZone, int nZones,
                           double rho, const double* aP, const double* gamma,
                           const double* phi, const orc_csr* pattern,
                           double* values, double* rhs);
+/* grad[f] = dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for the matrix of
+ * orc_assemble_pressure, where A^T lambda = dJ/dx.  pinned: the value
+ * orc_assemble_pressure returned. */
+void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, double rho,
+                       const double* aP, int pinned, const double* x, const double* lam,
+                       double* grad);
 
 /* Momentum assembly, first-order upwind + diffusion, one component
  * (src/solver/SimpleAlgorithm.cpp:240-355) without the pressure-gradient

@@ -48,6 +48,7 @@ class SolverCfg(C.Structure):
         ("amg_jacobi_omega", C.c_double),
         ("amg_smoother", C.c_int),
         ("amg_sweeps", C.c_int),
+        ("keep_preconditioner", C.c_int),
     ]
 
 
@@ -110,12 +111,15 @@ PROTOTYPES = {
     "mi_csr_spmv": (_i, [_vp, _vp, _vp]),
     "mi_csr_spmv_device": (_i, [_vp, _i, _vp, _vp, _vp]),
     "mi_csr_time_spmv": (_i, [_vp, _i, _vp, _vp, _i, _pd]),
+    "mi_csr_transpose": (_i, [_vp, C.POINTER(_vp)]),
+    "mi_csr_transpose_values": (_i, [_vp, _vp]),
     "mi_csr_destroy": (_i, [_vp]),
     "mi_solver_cfg_default": (None, [C.POINTER(SolverCfg)]),
     "mi_solver_create": (_i, [C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_solver_setup": (_i, [_vp, _vp]),
     "mi_solver_solve": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
     "mi_solver_solve_device": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
+    "mi_adjoint_solve": (_i, [_vp, _vp, _vp, C.POINTER(Stats)]),
     "mi_solver_last_residual": (_d, [_vp]),
     "mi_solver_precond_apply": (_i, [_vp, _vp, _vp]),
     "mi_solver_time_spmv": (_i, [_vp, _i, _i, _pd, _pd]),
@@ -128,6 +132,7 @@ PROTOTYPES = {
     "mi_assemble_pressure": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i)]),
     "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "mi_face_gradient": (_i, [_vp, _d, _vp, _i, _vp, _vp, _vp]),
     "mi_mesh_destroy": (_i, [_vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
@@ -261,6 +266,17 @@ class Csr:
         check(self.lib.mi_csr_time_spmv(self.h, precision, dx.ptr, dy.ptr, reps, C.byref(us)))
         return us.value
 
+    def transpose(self):
+        """Explicit A^T as a new handle (keeps this matrix alive)."""
+        h = C.c_void_p()
+        check(self.lib.mi_csr_transpose(self.h, C.byref(h)))
+        At = CsrView(h.value)
+        At._src = self
+        return At
+
+    def refresh_transpose(self, At):
+        check(self.lib.mi_csr_transpose_values(self.h, At.h))
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.mi_csr_destroy(self.h)
@@ -310,6 +326,13 @@ class Solver:
     def solve_device(self, db, dx):
         st = Stats()
         rc = self.lib.mi_solver_solve_device(self.h, db.ptr, dx.ptr, C.byref(st))
+        check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED, MI_ERR_BREAKDOWN))
+        return st, rc
+
+    def adjoint_solve_device(self, dg, dlam):
+        """A^T lambda = g on the matrix bound by setup (device vectors)."""
+        st = Stats()
+        rc = self.lib.mi_adjoint_solve(self.h, dg.ptr, dlam.ptr, C.byref(st))
         check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED, MI_ERR_BREAKDOWN))
         return st, rc
 
@@ -374,6 +397,8 @@ class CsrView:
 
     values = Csr.values
     update_values = Csr.update_values
+    transpose = Csr.transpose
+    refresh_transpose = Csr.refresh_transpose
     spmv = Csr.spmv
     spmv_device = Csr.spmv_device
     time_spmv = Csr.time_spmv
@@ -425,6 +450,11 @@ class DeviceMesh:
         check(self.lib.mi_assemble_momentum(self.h, comp, mu, rho, d_phi.ptr, _dptr(d_pgradV),
                                             _dptr(d_dt), _dptr(d_Uold), _dptr(d_Ucomp), urf,
                                             ptr(bv), A.h, d_rhs.ptr))
+
+    def face_gradient(self, d_x, d_lam, d_grad, pinned, rho=1.0, d_aP=None):
+        """d_grad[f] = dJ/dgamma_f = -lambda^T dA/dgamma_f x (pressure matrix)."""
+        check(self.lib.mi_face_gradient(self.h, rho, _dptr(d_aP), int(bool(pinned)), d_x.ptr,
+                                        d_lam.ptr, d_grad.ptr))
 
     def face_flux(self, d_Ux, d_Uy, d_Uz, d_phi, rho=1.0, bc_value=None):
         bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)

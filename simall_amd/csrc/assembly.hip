@@ -147,6 +147,52 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
     rhs[c] = b;
 }
 
+// dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for the pressure matrix of
+// k_assemble_pressure: an interior face adds k_f (e_o - e_n)(e_o - e_n)^T, a
+// pressure-boundary face k_f e_o e_o^T, with k_f = gamma_f * G_f; the pinned
+// cell's diagonal carries the factor 1e6.  The rhs does not depend on gamma.
+// (Gradient = dJ/dalpha - (dR/dalpha)^T psi: src/adjoint/DiscreteAdjoint.cpp:170-182,
+// with R = A(gamma) x - b.)
+__global__ __launch_bounds__(kBlock) void k_face_gradient(MeshDev m, double rho,
+                                                          const double* __restrict__ aP, int pin,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ lam,
+                                                          double* __restrict__ grad)
+{
+    const int f = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (f >= m.n_faces)
+        return;
+    const int o = m.owner[f];
+    const int nb = m.nbr[f];
+    const double Ax = m.ax[f], Ay = m.ay[f], Az = m.az[f];
+    const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
+    const double so = (pin && o == 0) ? 1.0e6 : 1.0;
+    double g = 0.0;
+    if (nb >= 0) {
+        const double dx = m.ccx[nb] - m.ccx[o];
+        const double dy = m.ccy[nb] - m.ccy[o];
+        const double dz = m.ccz[nb] - m.ccz[o];
+        const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        double aPf = 1.0;
+        if (aP)
+            aPf = 0.5 * (fmax(aP[o], 1e-30) + fmax(aP[nb], 1e-30));
+        const double G = rho * Amag2 / fmax(aPf * ddotA, 1e-30);
+        const double sn = (pin && nb == 0) ? 1.0e6 : 1.0;
+        const double lo = lam[o], ln = lam[nb], xo = x[o], xn = x[nb];
+        const double t = (lo - ln) * (xo - xn) + (so - 1.0) * lo * xo + (sn - 1.0) * ln * xn;
+        g = -G * t;
+    } else if (m.fbc[f] == BC_PRESSURE) {
+        const double dx = m.fcx[f] - m.ccx[o];
+        const double dy = m.fcy[f] - m.ccy[o];
+        const double dz = m.fcz[f] - m.ccz[o];
+        const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        const double dO = aP ? fmax(aP[o], 1e-30) : 1.0;
+        const double G = rho * Amag2 / fmax(dO * ddotA, 1e-30);
+        g = -G * so * lam[o] * x[o];
+    }
+    grad[f] = g;
+}
+
 // Momentum row of cell c, one velocity component, first-order upwind +
 // diffusion (src/solver/SimpleAlgorithm.cpp:240-355).  pgradV (nullable) is
 // the cell's grad(p)_comp * V_c, subtracted from the rhs (:307-315).
@@ -441,9 +487,7 @@ MI_API int mi_assemble_pressure(mi_mesh* M, double rho, const double* aP, const 
         MI_HIP(hipGetLastError());
     }
     MI_HIP(hipStreamSynchronize(s));
-    A->host_valid = false;
-    A->val32_valid = false;
-    ++A->version;
+    A->values_changed(false);
     if (pinned)
         *pinned = pin;
     return MI_OK;
@@ -468,9 +512,24 @@ MI_API int mi_assemble_momentum(mi_mesh* M, int comp, double mu, double rho, con
         MI_HIP(hipGetLastError());
     }
     MI_HIP(hipStreamSynchronize(s));
-    A->host_valid = false;
-    A->val32_valid = false;
-    ++A->version;
+    A->values_changed(false);
+    return MI_OK;
+}
+
+MI_API int mi_face_gradient(mi_mesh* M, double rho, const double* aP, int pinned, const double* x,
+                            const double* lambda, double* grad)
+{
+    if (!M || (M->n_faces && (!x || !lambda || !grad)))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    if (M->n_faces > 0) {
+        k_face_gradient<<<grid_for(M->n_faces, kBlock), kBlock, 0, s>>>(M->dev(), rho, aP,
+                                                                        pinned ? 1 : 0, x, lambda,
+                                                                        grad);
+        MI_HIP(hipGetLastError());
+    }
+    MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }
 

@@ -201,6 +201,14 @@ int mi_csr::ensure_fp32_diff(hipStream_t s)
     return MI_OK;
 }
 
+__global__ void k_gather_vals(int64_t nnz, const int* __restrict__ map,
+                              const double* __restrict__ src, double* __restrict__ dst)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t k = int64_t(blockIdx.x) * kBlock + threadIdx.x; k < nnz; k += stride)
+        dst[k] = src[map[k]];
+}
+
 int mi_csr::ensure_fp32_op(hipStream_t s)
 {
     return pat->all_diag ? ensure_fp32_diff(s) : ensure_fp32(s);
@@ -246,10 +254,7 @@ MI_API int mi_csr_update_values(mi_csr* A, const double* values)
     A->h_val.assign(values, values + A->nnz());
     MI_TRY(A->val.upload(A->h_val.data(), size_t(A->nnz()), s));
     MI_HIP(hipStreamSynchronize(s));
-    A->host_valid = true;
-    A->val32_valid = false;
-    A->val32d_valid = false;
-    ++A->version;
+    A->values_changed(true);
     return MI_OK;
 }
 
@@ -263,10 +268,7 @@ MI_API int mi_csr_update_values_device(mi_csr* A, const double* values_device)
         MI_HIP(hipMemcpyAsync(A->val.p, values_device, size_t(A->nnz()) * sizeof(double),
                               hipMemcpyDeviceToDevice, s));
     MI_HIP(hipStreamSynchronize(s));
-    A->host_valid = false;
-    A->val32_valid = false;
-    A->val32d_valid = false;
-    ++A->version;
+    A->values_changed(false);
     return MI_OK;
 }
 
@@ -308,10 +310,7 @@ MI_API int mi_csr_values_device(mi_csr* A, double** values_device)
         return set_error(MI_ERR_BAD_ARG, "null argument");
     *values_device = A->val.p;
     // The caller may write through this pointer: host and FP32 copies go stale.
-    A->host_valid = false;
-    A->val32_valid = false;
-    A->val32d_valid = false;
-    ++A->version;
+    A->values_changed(false);
     return MI_OK;
 }
 
@@ -382,6 +381,61 @@ MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x, void* y, in
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *avg_us = double(ms) * 1000.0 / reps;
+    return MI_OK;
+}
+
+MI_API int mi_csr_transpose(const mi_csr* A, mi_csr** out)
+{
+    if (!A || !out)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    const int64_t n = A->n(), nnz = A->nnz();
+    const std::vector<int>& rp = A->pat->h_rp;
+    const std::vector<int>& ci = A->pat->h_ci;
+    // counting sort by column: rows of A^T come out with ascending columns
+    std::vector<int32_t> trp(static_cast<size_t>(n + 1), 0), tci(static_cast<size_t>(nnz));
+    std::vector<int> map(static_cast<size_t>(nnz));
+    for (int64_t k = 0; k < nnz; ++k)
+        ++trp[size_t(ci[size_t(k)]) + 1];
+    for (int64_t i = 0; i < n; ++i)
+        trp[size_t(i) + 1] += trp[size_t(i)];
+    std::vector<int32_t> next(trp.begin(), trp.end() - 1);
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = rp[size_t(i)]; k < rp[size_t(i) + 1]; ++k) {
+            const int p = next[size_t(ci[size_t(k)])]++;
+            tci[size_t(p)] = int32_t(i);
+            map[size_t(p)] = k;
+        }
+    std::shared_ptr<Pattern> pat;
+    MI_TRY(make_pattern(n, nnz, trp.data(), tci.data(), pat, s));
+    mi_csr* T = nullptr;
+    MI_TRY(make_csr_on_pattern(pat, &T));
+    std::unique_ptr<mi_csr> guard(T);
+    if (nnz) {
+        MI_TRY(T->tmap.upload(map.data(), map.size(), s));
+        MI_HIP(hipStreamSynchronize(s));
+    }
+    T->tsrc = A;
+    *out = guard.release();
+    return mi_csr_transpose_values(A, *out);
+}
+
+MI_API int mi_csr_transpose_values(const mi_csr* A, mi_csr* At)
+{
+    if (!A || !At)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    if (At->tsrc != A || At->nnz() != A->nnz())
+        return set_error(MI_ERR_BAD_ARG, "matrix is not a transpose created from this matrix");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    if (A->nnz()) {
+        k_gather_vals<<<reduce_grid(A->nnz(), 4), kBlock, 0, s>>>(A->nnz(), At->tmap.p, A->val.p,
+                                                                  At->val.p);
+        MI_HIP(hipGetLastError());
+    }
+    MI_HIP(hipStreamSynchronize(s));
+    At->values_changed(false);
     return MI_OK;
 }
 

@@ -35,7 +35,21 @@ struct mi_csr
     bool val32_valid = false;
     uint64_t version = 1;     // bumped on every value update
 
+    // Transposed copy (mi_csr_transpose): tmap[k] = position in the source
+    // matrix of this matrix's k-th value.
+    mi::DevBuf<int> tmap;
+    const mi_csr* tsrc = nullptr;
+
     int64_t n() const { return pat->n; }
+    // Values were replaced on the device (host_valid: h_val already holds
+    // them): derived copies are stale and solvers rebuild on next use.
+    void values_changed(bool host_valid_now)
+    {
+        host_valid = host_valid_now;
+        val32_valid = false;
+        val32d_valid = false;
+        ++version;
+    }
     int64_t nnz() const { return pat->nnz; }
     // Makes h_val current (device -> host copy when values were set on device).
     int sync_host();

@@ -301,9 +301,7 @@ int build(mi_solver* S)
                                                                           A->val.p, M->val.p);
             MI_HIP(hipGetLastError());
         }
-        M->val32_valid = false;
-        M->val32d_valid = false;
-        ++M->version;
+        M->values_changed(false);
         if (A->host_valid) {
             M->h_val.resize(size_t(nnz));
             for (int64_t k = 0; k < nnz; ++k)
@@ -316,6 +314,13 @@ int build(mi_solver* S)
     }
     if (mixed)
         MI_TRY(M->ensure_fp32_op(S->stream));
+    if (S->pc && S->cfg.keep_preconditioner) {
+        // same pattern, new values: keep the preconditioner built on earlier values
+        MI_HIP(hipStreamSynchronize(S->stream));
+        S->built_version = A->version;
+        S->setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+        return MI_OK;
+    }
     if (!S->pc) {
         switch (S->cfg.preconditioner) {
         case MI_PC_AMG:
@@ -657,6 +662,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->amg_jacobi_omega = 4.0 / 3.0;
     c->amg_smoother = 0;
     c->amg_sweeps = 2;
+    c->keep_preconditioner = 0;
 }
 
 MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out)
@@ -816,6 +822,33 @@ MI_API int mi_solver_time_spmv(mi_solver* S, int precision, int reps, double* av
     MI_HIP(hipMemsetAsync(xb.p, 0, size_t(n) * vb, default_stream()));
     MI_HIP(hipStreamSynchronize(default_stream()));
     return mi_csr_time_spmv(M, precision, xb.p, yb.p, reps, avg_us);
+}
+
+MI_API int mi_adjoint_solve(mi_solver* S, const double* g, double* lambda, mi_stats* stats)
+{
+    clear_error();
+    if (!S || !S->A)
+        return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
+    if (S->A->n() && (!g || !lambda))
+        return set_error(MI_ERR_BAD_ARG, "null vector");
+    MI_TRY(ensure_device());
+    if (!S->At || !S->adj || S->at_src != S->A) {
+        S->adj.reset();
+        S->At.reset();
+        mi_csr* T = nullptr;
+        MI_TRY(mi_csr_transpose(S->A, &T));
+        S->At.reset(T);
+        mi_solver* a = nullptr;
+        MI_TRY(mi_solver_create(&S->cfg, &a));
+        S->adj.reset(a);
+        MI_TRY(mi_solver_setup(a, T));
+        S->at_src = S->A;
+        S->at_version = S->A->version;
+    } else if (S->at_version != S->A->version) {
+        MI_TRY(mi_csr_transpose_values(S->A, S->At.get()));
+        S->at_version = S->A->version;
+    }
+    return solve_device(S->adj.get(), g, lambda, stats);
 }
 
 MI_API double mi_solver_last_residual(const mi_solver* S) { return S ? S->last : 0.0; }

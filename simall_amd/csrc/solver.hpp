@@ -71,5 +71,12 @@ struct mi_solver
     double last = 0.0;
     double setup_seconds = 0.0;
 
+    // Discrete adjoint: a solver with this configuration on the explicit
+    // transpose of A, refreshed when A's values change (mi_adjoint_solve).
+    std::unique_ptr<mi_csr> At;
+    std::unique_ptr<mi_solver> adj;
+    const mi_csr* at_src = nullptr;
+    uint64_t at_version = 0;
+
     ~mi_solver();
 };

@@ -14,7 +14,19 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Initialises the HIP backend on device 0 (fails the test when absent)."""
+    """Initialises the HIP backend on device 0 (fails the test when absent).
+
+    torch is imported first when available: it carries its own HIP runtime,
+    and the backend library then binds to that same runtime, so device
+    pointers of torch tensors (the autograd harness) and of the backend are
+    interchangeable.  Loaded the other way round, two runtimes would coexist
+    and torch would see no device."""
+    try:
+        import torch
+
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     from simall_amd import capi
 
     lib = capi.load()
