@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--caller-order", action="store_true",
                     help="also time SpMV on the caller's (randomly numbered) matrix")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the discrete-adjoint timing (J = sum w x^2 on the same system)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the working-matrix SpMV and a calibration copy only "
                          "(short run for hardware-counter passes)")
@@ -177,6 +179,38 @@ def main():
         f"fp64 {us64:.1f} us {gbs64:.0f} GB/s ({100 * gbs64 / HBM_PEAK_GBS:.1f}%); fp32 {us32:.1f} us "
         f"{gbs32:.0f} GB/s; caller ordering fp64 {gbs_raw} GB/s; "
         f"AMG levels {info['levels']} complexity {info['operator_complexity']:.2f}")
+    adjoint = None
+    if "fp64" in solvers and not args.no_adjoint:
+        # discrete adjoint of the FP64 solve: A^T lambda = 2 w x, then dJ/dgamma_f per face
+        s64 = solvers["fp64"]
+        wts = wl.uniform(wl.seed(4, 5), 0.0, 1.0, n)
+        d_g = capi.DeviceArray.from_host(2.0 * wts * results["fp64"]["x"])
+        d_l = capi.DeviceArray(n)
+        d_xs = capi.DeviceArray.from_host(results["fp64"]["x"])
+        d_grad = capi.DeviceArray(m.n_faces)
+        t1 = time.perf_counter()
+        st, rc = s64.adjoint_solve_device(d_g, d_l)
+        lib.mi_device_synchronize()
+        first = time.perf_counter() - t1
+        ta = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            st, rc = s64.adjoint_solve_device(d_g, d_l)
+            lib.mi_device_synchronize()
+            ta.append(time.perf_counter() - t1)
+        tg = []
+        for _ in range(5):
+            t1 = time.perf_counter()
+            dm.face_gradient(d_xs, d_l, d_grad, pinned)
+            tg.append(time.perf_counter() - t1)
+        sa = st.as_dict()
+        adjoint = {"solve_ms": round(1e3 * float(np.median(ta)), 3),
+                   "first_call_s": round(first, 3), "iterations": sa["iterations"],
+                   "true_relative_residual": sa["true_relative_residual"],
+                   "face_gradient_ms": round(1e3 * float(np.median(tg)), 3),
+                   "faces": m.n_faces}
+        log(f"adjoint: {adjoint}")
+
     err = None
     if "fp64" in results and "mixed" in results:
         x64, xm = results["fp64"]["x"], results["mixed"]["x"]
@@ -233,6 +267,7 @@ def main():
                      "traffic": read_traffic(), "kernel": "k_stream_spmv<double>",
                      "algorithmic_bytes": by64, "avg_us": round(us64, 2)},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -167,6 +167,8 @@ typedef struct {
     int amg_sweeps;        /* pre = post sweeps / Chebyshev degree */
     int keep_preconditioner; /* 1: new values of the bound matrix keep the preconditioner
                                 built on earlier values (the operator is always current) */
+    int use_graph;           /* batched solve: 1 replays a captured HIP graph of
+                                check_interval iterations, 0 launches each kernel */
 } mi_solver_cfg;
 
 /* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */
@@ -223,6 +225,38 @@ MI_API double mi_solver_last_residual(const mi_solver* s);
 /* Relative residual after each Krylov iteration of the last solve. */
 MI_API int mi_solver_history(const mi_solver* s, double* out, int capacity, int* count);
 MI_API int mi_solver_destroy(mi_solver* s);
+
+/* ---------------------------------------------------- batched equations
+ * n_eq (1..64) systems A_e x_e = b_e on one CSR pattern, solved together.
+ * SimAll solves each scalar transport equation (energy, k/omega, species)
+ * with its own ILinearSolver::solve on an identical pattern
+ * (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232); one
+ * mi_batch_solve replaces those n_eq calls.  Per equation the iteration is
+ * Jacobi-PCG from x = 0 (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452)
+ * in FP32 with FP64 dot products, norms and iterate, refined on the FP64 true
+ * residual until ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations
+ * converge and drop out independently.  cfg (nullable: defaults) supplies
+ * kind = CG, preconditioner = Jacobi or None, tolerance, max_iterations (per
+ * equation, all sweeps), inner_tolerance, max_outer and check_interval.
+ * Vectors are equation-major: b[e * n + i].  The rows per workgroup of the
+ * batched SpMV are chosen by timing candidates at creation (mi_batch_tuning). */
+typedef struct mi_batch mi_batch;
+MI_API int mi_batch_create(const mi_csr* pattern, int n_eq, mi_batch** out);
+/* Values of equation eq from a matrix with the batch's pattern. */
+MI_API int mi_batch_set_equation(mi_batch* b, int eq, const mi_csr* A);
+/* All values, equation-major: values_host[e * nnz + k]. */
+MI_API int mi_batch_set_values(mi_batch* b, const double* values_host);
+/* Returns MI_OK when every equation converged, else MI_ERR_NOT_CONVERGED;
+ * iterations[e] and rel_residual[e] (FP64 true) are written in both cases. */
+MI_API int mi_batch_solve(mi_batch* b, const mi_solver_cfg* cfg, const double* b_host,
+                          double* x_host, int* iterations, double* rel_residual);
+MI_API int mi_batch_solve_device(mi_batch* b, const mi_solver_cfg* cfg, const double* b_device,
+                                 double* x_device, int* iterations, double* rel_residual);
+/* Autotuner result: rows per workgroup, time of one batched SpMV (all
+ * equations active) and its algorithmic bytes. */
+MI_API int mi_batch_tuning(const mi_batch* b, int* rows_per_block, double* spmm_us,
+                           double* spmm_bytes);
+MI_API int mi_batch_destroy(mi_batch* b);
 
 /* ------------------------------------------------- host-side inspection
  * The smoothed-aggregation hierarchy and the reverse Cuthill-McKee ordering

@@ -49,6 +49,7 @@ class SolverCfg(C.Structure):
         ("amg_smoother", C.c_int),
         ("amg_sweeps", C.c_int),
         ("keep_preconditioner", C.c_int),
+        ("use_graph", C.c_int),
     ]
 
 
@@ -133,6 +134,13 @@ PROTOTYPES = {
     "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
     "mi_face_gradient": (_i, [_vp, _d, _vp, _i, _vp, _vp, _vp]),
+    "mi_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "mi_batch_set_equation": (_i, [_vp, _i, _vp]),
+    "mi_batch_set_values": (_i, [_vp, _vp]),
+    "mi_batch_solve": (_i, [_vp, C.POINTER(SolverCfg), _vp, _vp, _vp, _vp]),
+    "mi_batch_solve_device": (_i, [_vp, C.POINTER(SolverCfg), _vp, _vp, _vp, _vp]),
+    "mi_batch_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_d), C.POINTER(_d)]),
+    "mi_batch_destroy": (_i, [_vp]),
     "mi_mesh_destroy": (_i, [_vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
@@ -532,3 +540,63 @@ def rcm_permutation(rowPtr, colIdx):
     bw = C.c_int32()
     check(load().mi_rcm_permutation(rp.size - 1, ptr(rp), ptr(ci), ptr(perm), C.byref(bw)))
     return perm, bw.value
+
+
+class Batch:
+    """mi_batch: n_eq equations sharing the pattern of `A` (Csr or CsrView)."""
+
+    def __init__(self, A, n_eq):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.mi_batch_create(A.h, int(n_eq), C.byref(h)))
+        self.h = h.value
+        self.A = A
+        self.n_eq, self.n, self.nnz = int(n_eq), A.n, A.nnz
+
+    def set_equation(self, e, A):
+        check(self.lib.mi_batch_set_equation(self.h, int(e), A.h))
+
+    def set_values(self, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n_eq, self.nnz)
+        check(self.lib.mi_batch_set_values(self.h, ptr(v)))
+
+    def _cfg(self, cfg, kw):
+        c = cfg if cfg is not None else default_cfg(kind=MI_SOLVER_CG, preconditioner=MI_PC_JACOBI)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+    def solve(self, b, cfg=None, **kw):
+        """b: (n_eq, n) -> (x, iterations, rel_residual, status)."""
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(self.n_eq, self.n)
+        x = np.empty_like(b)
+        its = np.empty(self.n_eq, np.int32)
+        rel = np.empty(self.n_eq, np.float64)
+        c = self._cfg(cfg, kw)
+        rc = self.lib.mi_batch_solve(self.h, C.byref(c), ptr(b), ptr(x), ptr(its), ptr(rel))
+        check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED))
+        return x, its, rel, rc
+
+    def solve_device(self, db, dx, cfg=None, **kw):
+        its = np.empty(self.n_eq, np.int32)
+        rel = np.empty(self.n_eq, np.float64)
+        c = self._cfg(cfg, kw)
+        rc = self.lib.mi_batch_solve_device(self.h, C.byref(c), db.ptr, dx.ptr, ptr(its), ptr(rel))
+        check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED))
+        return its, rel, rc
+
+    def tuning(self):
+        rb, us, by = C.c_int(), C.c_double(), C.c_double()
+        check(self.lib.mi_batch_tuning(self.h, C.byref(rb), C.byref(us), C.byref(by)))
+        return {"rows_per_block": rb.value, "spmm_us": us.value, "spmm_bytes": by.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -663,6 +663,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->amg_smoother = 0;
     c->amg_sweeps = 2;
     c->keep_preconditioner = 0;
+    c->use_graph = 1;
 }
 
 MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out)
