@@ -6,25 +6,29 @@
 // E equations of one pattern are solved together: each stage of the
 // iteration is one launch for all equations.
 //
-// Layout: values, vectors and inverse diagonals are interleaved by equation
-// (element (k, e) at k * Ep + e, Ep = E rounded up to a power of two).  A
-// workgroup of 256 threads covers 256 / Ep rows x Ep equations, thread
-// (row slot, e) owns row i of equation e and walks the row in column order.
-// For each nonzero the Ep lanes of a row read Ep consecutive values and
-// gather Ep consecutive x entries: one 4*Ep-byte segment each, and the
-// shared column index once for all equations.
+// Layout: equation-major.  Values v[e * nnz + k] (FP32 for the iteration,
+// FP64 for the true residual), vectors x[e * n + i].  A persistent workgroup
+// walks its range of the pattern's row blocks (<= 256 rows, <= 2048
+// nonzeros); for each row block it stages the shared column indices in LDS
+// once, then for every equation still iterating streams that equation's
+// values, gathers its x, and sums each row in column order (thread per row).
+// An equation that has converged is skipped for the whole launch, so the
+// traffic of a launch is that of the active equations plus one pass over the
+// shared pattern.
 //
 // Arithmetic: Jacobi-PCG per equation (CGSolver::solve,
 // src/solver/LinearSolvers.cpp:420-452) with FP32 values and vectors and FP64
-// dot products, norms and iterate (the iterate is accumulated in FP64 as
-// in the mixed single-equation solver); an FP64 true residual with the FP64
-// values refines each equation until ||b_e - A_e x_e|| <= tol ||b_e||.
-// Each equation has its own scalars and convergence flag: converged
-// equations drop out of every launch.  One inner iteration is five launches
-// (batched SpMV + p.Ap, finalize, update, finalize, direction) captured once
-// into a HIP graph of `chunk` iterations that the host replays between polls.
+// dot products, norms and iterate (accumulated in FP64 as in the mixed
+// single-equation solver); an FP64 true residual with the FP64 values
+// refines each equation until ||b_e - A_e x_e|| <= tol ||b_e||.  Each
+// equation has its own scalars and convergence flag.  One inner iteration is
+// five launches (batched SpMV + p.Ap, finalize, update, finalize, direction)
+// captured once into a HIP graph of `chunk` iterations that the host replays
+// between polls.
 #include "csr.hpp"
 #include "krylov.hpp"
+#include "reorder.hpp"
+#include "stream.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -35,214 +39,323 @@
 namespace mi
 {
 
+constexpr int kMaxEq = 64;
+constexpr int kWaves = kBlock / kWave;
+
 struct BEq
 {
     double rz, alpha, beta, r0, bnorm, tol_in, rel, last;
     int iter, total, outer, done, fin, status, pad[2];
 };
 
-enum BOp : int { B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4 };
+enum BOp : int { B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4, B_P = 5 };
 
 struct BArgs
 {
-    int n, E, Ep, RB;
+    int n, E, nrb;
+    int64_t nnz;
+    const int* blk; // pattern row blocks
     const int* rp;
     const int* ci;
-    const float* v32;
-    const double* v64;
-    const float* invD;
-    const double* b;
+    const float* v32;  // [E][nnz]
+    const double* v64; // [E][nnz]
+    const float* invD; // [E][n]
+    const double* b;   // [E][n]
     double* x;
     double* rt;
     float* r;
     float* p;
     float* Ap;
-    BEq* st;
-    double* partials; // [2][G][Ep]
+    BEq* st;           // nullptr: every equation active (autotuner)
+    double* partials;  // [2][E][G]
 };
 
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ bool b_active(const BArgs& a, int e, bool inner)
+__device__ __forceinline__ bool b_iterating(const BArgs& a, int e)
 {
-    return e < a.E && (a.st == nullptr || (inner ? !a.st[e].done : !a.st[e].fin));
+    return a.st == nullptr || !a.st[e].done;
 }
 
-// Sum of v over the row slots of each equation into partials[j][block][e].
-template <int NV>
-__device__ __forceinline__ void b_partials(const BArgs& a, double (&v)[NV])
+// Per-workgroup FP64 sums of up to two values per equation: each wavefront
+// adds its share into its own slot (fixed order), the slots are folded at
+// the end and stored as partials[v][e][block].
+struct BSums
 {
-    __shared__ double red[NV][kBlock];
-    const int t = threadIdx.x;
-#pragma unroll
-    for (int j = 0; j < NV; ++j)
-        red[j][t] = v[j];
-    __syncthreads();
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    if (t < Ep) {
-#pragma unroll
-        for (int j = 0; j < NV; ++j) {
+    double (*w)[kMaxEq][kWaves];
+    __device__ __forceinline__ void clear(int E)
+    {
+        for (int t = threadIdx.x; t < 2 * kMaxEq * kWaves; t += kBlock)
+            (&w[0][0][0])[t] = 0.0;
+        __syncthreads();
+        (void)E;
+    }
+    __device__ __forceinline__ void add(int v, int e, double s)
+    {
+        s = wave_sum(s);
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            w[v][e][threadIdx.x / kWave] += s;
+    }
+    __device__ __forceinline__ void store(const BArgs& a, int nv)
+    {
+        __syncthreads();
+        const int G = gridDim.x;
+        for (int t = threadIdx.x; t < nv * a.E; t += kBlock) {
+            const int v = t / a.E, e = t % a.E;
             double s = 0.0;
-            for (int q = 0; q < RG; ++q)
-                s += red[j][q * Ep + t];
-            a.partials[(size_t(j) * gridDim.x + blockIdx.x) * Ep + t] = s;
+            for (int q = 0; q < kWaves; ++q)
+                s += w[v][e][q];
+            a.partials[(size_t(v) * a.E + e) * G + blockIdx.x] = s;
         }
     }
-}
+};
 
 // ------------------------------------------------------------------ kernels
-// Ap = A p for the active equations; partials of p.Ap.
+// Work of a launch: units (equation, row block) for SpMV-shaped stages and
+// (equation, row chunk) for vector stages, over the equations the stage
+// applies to.  Every workgroup builds the same list of those equations, then
+// walks its contiguous range of units, so units of one equation are
+// consecutive and each is processed exactly like one block of the
+// single-equation CSR-stream engine.
+constexpr int kChunkRows = kBlock * 8;
+
+__device__ __forceinline__ int b_list(const BArgs& a, int op, int* act)
+{
+    __shared__ int cnt;
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int e = 0; e < a.E; ++e) {
+            bool on = true;
+            if (op == B_TRUE)
+                on = !a.st[e].fin;
+            else if (op != B_BNORM)
+                on = b_iterating(a, e);
+            if (on)
+                act[c++] = e;
+        }
+        cnt = c;
+    }
+    __syncthreads();
+    return cnt;
+}
+
+// Ap_e = A_e p_e for the iterating equations (FP32); partials of p_e.Ap_e.
 __global__ __launch_bounds__(kBlock) void k_bspmv_dot(BArgs a)
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    double v[1] = {0.0};
-    if (b_active(a, e, true)) {
-        const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-        for (int i = r0 + slot; i < r1; i += RG) {
+    __shared__ float prod[kStreamNnz];
+    __shared__ int act[kMaxEq];
+    __shared__ double wsum[2][kMaxEq][kWaves];
+    BSums S{wsum};
+    S.clear(a.E);
+    const int nact = b_list(a, B_ALPHA, act);
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int64_t units = int64_t(nact) * a.nrb;
+    const int64_t per = (units + G - 1) / G;
+    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
+    const int tid = threadIdx.x;
+    for (int64_t u = u0; u < u1; ++u) {
+        const int e = act[u / a.nrb];
+        const int rb = int(u % a.nrb);
+        const float* __restrict__ ve = a.v32 + size_t(e) * a.nnz;
+        const float* __restrict__ pe = a.p + size_t(e) * a.n;
+        float* __restrict__ Ae = a.Ap + size_t(e) * a.n;
+        const int r0 = a.blk[rb], r1 = a.blk[rb + 1];
+        const int k0 = a.rp[r0], k1 = a.rp[r1];
+        if (k1 - k0 > kStreamNnz) {
+            // one long row: strided sums, fixed-order fold over the block
             float s = 0.f;
-            for (int k = a.rp[i]; k < a.rp[i + 1]; ++k) {
-                const int c = a.ci[k];
-                s += __builtin_nontemporal_load(&a.v32[size_t(k) * Ep + e]) *
-                     a.p[size_t(c) * Ep + e];
+            for (int k = k0 + tid; k < k1; k += kBlock)
+                s += ve[k] * pe[a.ci[k]];
+            prod[tid] = s;
+            __syncthreads();
+            double d = 0.0;
+            if (tid == 0) {
+                float t = 0.f;
+                for (int j = 0; j < kBlock; ++j)
+                    t += prod[j];
+                Ae[r0] = t;
+                d = double(pe[r0]) * double(t);
             }
-            const size_t q = size_t(i) * Ep + e;
-            a.Ap[q] = s;
-            v[0] += double(a.p[q]) * double(s);
+            S.add(0, e, d);
+            __syncthreads();
+            continue;
         }
+        const int i = r0 + tid;
+        float pi = 0.f;
+        if (i < r1)
+            pi = pe[i];
+        int c[kLoadsPerThread];
+        float v[kLoadsPerThread];
+#pragma unroll
+        for (int j = 0; j < kLoadsPerThread; ++j) {
+            const int k = k0 + tid + j * kBlock;
+            c[j] = k < k1 ? __builtin_nontemporal_load(&a.ci[k]) : -1;
+            v[j] = k < k1 ? __builtin_nontemporal_load(&ve[k]) : 0.f;
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadsPerThread; ++j)
+            if (c[j] >= 0)
+                prod[tid + j * kBlock] = v[j] * pe[c[j]];
+        __syncthreads();
+        double d = 0.0;
+        if (i < r1) {
+            float s = 0.f;
+            const int end = a.rp[i + 1] - k0;
+            for (int k = a.rp[i] - k0; k < end; ++k)
+                s += prod[k];
+            Ae[i] = s;
+            d = double(pi) * double(s);
+        }
+        S.add(0, e, d);
+        __syncthreads();
     }
-    b_partials<1>(a, v);
+    S.store(a, 1);
 }
 
-// rt = b - A x in FP64 (FP64 values), r = float(rt); partials of ||rt||^2.
+// rt_e = b_e - A_e x_e in FP64 for the unfinished equations, r_e = float(rt_e);
+// partials of ||rt_e||^2.
 __global__ __launch_bounds__(kBlock) void k_btrue(BArgs a)
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    double v[1] = {0.0};
-    if (b_active(a, e, false)) {
-        const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-        for (int i = r0 + slot; i < r1; i += RG) {
+    __shared__ double prod[kStreamNnz];
+    __shared__ int act[kMaxEq];
+    __shared__ double wsum[2][kMaxEq][kWaves];
+    BSums S{wsum};
+    S.clear(a.E);
+    const int nact = b_list(a, B_TRUE, act);
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int64_t units = int64_t(nact) * a.nrb;
+    const int64_t per = (units + G - 1) / G;
+    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
+    const int tid = threadIdx.x;
+    for (int64_t u = u0; u < u1; ++u) {
+        const int e = act[u / a.nrb];
+        const int rb = int(u % a.nrb);
+        const double* __restrict__ ve = a.v64 + size_t(e) * a.nnz;
+        const double* __restrict__ xe = a.x + size_t(e) * a.n;
+        const size_t off = size_t(e) * a.n;
+        const int r0 = a.blk[rb], r1 = a.blk[rb + 1];
+        const int k0 = a.rp[r0], k1 = a.rp[r1];
+        if (k1 - k0 > kStreamNnz) {
             double s = 0.0;
-            for (int k = a.rp[i]; k < a.rp[i + 1]; ++k)
-                s += a.v64[size_t(k) * Ep + e] * a.x[size_t(a.ci[k]) * Ep + e];
-            const size_t q = size_t(i) * Ep + e;
-            const double ri = a.b[q] - s;
-            a.rt[q] = ri;
-            a.r[q] = float(ri);
-            v[0] += ri * ri;
+            for (int k = k0 + tid; k < k1; k += kBlock)
+                s += ve[k] * xe[a.ci[k]];
+            prod[tid] = s;
+            __syncthreads();
+            double rr = 0.0;
+            if (tid == 0) {
+                double t = 0.0;
+                for (int j = 0; j < kBlock; ++j)
+                    t += prod[j];
+                const double ri = a.b[off + r0] - t;
+                a.rt[off + r0] = ri;
+                a.r[off + r0] = float(ri);
+                rr = ri * ri;
+            }
+            S.add(0, e, rr);
+            __syncthreads();
+            continue;
         }
+        for (int k = k0 + tid; k < k1; k += kBlock)
+            prod[k - k0] = ve[k] * xe[a.ci[k]];
+        __syncthreads();
+        const int i = r0 + tid;
+        double rr = 0.0;
+        if (i < r1) {
+            double s = 0.0;
+            const int end = a.rp[i + 1] - k0;
+            for (int k = a.rp[i] - k0; k < end; ++k)
+                s += prod[k];
+            const double ri = a.b[off + i] - s;
+            a.rt[off + i] = ri;
+            a.r[off + i] = float(ri);
+            rr = ri * ri;
+        }
+        S.add(0, e, rr);
+        __syncthreads();
     }
-    b_partials<1>(a, v);
+    S.store(a, 1);
 }
 
-// partials of ||b||^2 (all equations)
-__global__ __launch_bounds__(kBlock) void k_bnorm(BArgs a)
+// Vector stages over units (equation, chunk of kChunkRows rows):
+//   B_BNORM  partials ||b_e||^2 (all equations)
+//   B_INIT   p = D^-1 r; partials r.z, r.r
+//   B_UPDATE x += alpha p (FP64), r -= alpha Ap; partials r.r, r.z
+//   B_P      p = D^-1 r + beta p
+__global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
 {
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    double v[1] = {0.0};
-    if (e < a.E) {
-        const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-        for (int i = r0 + slot; i < r1; i += RG) {
-            const double bi = a.b[size_t(i) * Ep + e];
-            v[0] += bi * bi;
+    __shared__ int act[kMaxEq];
+    __shared__ double wsum[2][kMaxEq][kWaves];
+    BSums S{wsum};
+    S.clear(a.E);
+    const int nact = b_list(a, op, act);
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int nch = (a.n + kChunkRows - 1) / kChunkRows;
+    const int64_t units = int64_t(nact) * nch;
+    const int64_t per = (units + G - 1) / G;
+    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
+    for (int64_t u = u0; u < u1; ++u) {
+        const int e = act[u / nch];
+        const int c0 = int(u % nch) * kChunkRows;
+        const int c1 = min(a.n, c0 + kChunkRows);
+        const size_t off = size_t(e) * a.n;
+        double s0 = 0.0, s1 = 0.0;
+        if (op == B_BNORM) {
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
+                s0 += a.b[off + i] * a.b[off + i];
+        } else if (op == B_INIT) {
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                const float ri = a.r[off + i], zi = a.invD[off + i] * ri;
+                a.p[off + i] = zi;
+                s0 += double(ri) * double(zi);
+                s1 += double(ri) * double(ri);
+            }
+        } else if (op == B_UPDATE) {
+            const double al = a.st[e].alpha;
+            const float alf = float(al);
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                a.x[off + i] += al * double(a.p[off + i]);
+                const float ri = a.r[off + i] - alf * a.Ap[off + i];
+                a.r[off + i] = ri;
+                const float zi = a.invD[off + i] * ri;
+                s0 += double(ri) * double(ri);
+                s1 += double(ri) * double(zi);
+            }
+        } else { // B_P
+            const float be = float(a.st[e].beta);
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
+                a.p[off + i] = a.invD[off + i] * a.r[off + i] + be * a.p[off + i];
+            continue;
         }
+        S.add(0, e, s0);
+        if (op != B_BNORM)
+            S.add(1, e, s1);
     }
-    b_partials<1>(a, v);
-}
-
-// inner start from d = 0: p = D^-1 r; partials of r.z, r.r
-__global__ __launch_bounds__(kBlock) void k_binit(BArgs a)
-{
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    double v[2] = {0.0, 0.0};
-    if (b_active(a, e, true)) {
-        const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-        for (int i = r0 + slot; i < r1; i += RG) {
-            const size_t q = size_t(i) * Ep + e;
-            const float ri = a.r[q], zi = a.invD[q] * ri;
-            a.p[q] = zi;
-            v[0] += double(ri) * double(zi);
-            v[1] += double(ri) * double(ri);
-        }
-    }
-    b_partials<2>(a, v);
-}
-
-// x += alpha p (FP64), r -= alpha Ap; partials of r.r, r.z
-__global__ __launch_bounds__(kBlock) void k_bupdate(BArgs a)
-{
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    double v[2] = {0.0, 0.0};
-    if (b_active(a, e, true)) {
-        const double al = a.st[e].alpha;
-        const float alf = float(al);
-        const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-        for (int i = r0 + slot; i < r1; i += RG) {
-            const size_t q = size_t(i) * Ep + e;
-            a.x[q] += al * double(a.p[q]);
-            const float ri = a.r[q] - alf * a.Ap[q];
-            a.r[q] = ri;
-            const float zi = a.invD[q] * ri;
-            v[0] += double(ri) * double(ri);
-            v[1] += double(ri) * double(zi);
-        }
-    }
-    b_partials<2>(a, v);
-}
-
-// p = D^-1 r + beta p
-__global__ __launch_bounds__(kBlock) void k_bp(BArgs a)
-{
-    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int Ep = a.Ep, RG = kBlock / Ep;
-    const int e = threadIdx.x & (Ep - 1), slot = threadIdx.x / Ep;
-    if (!b_active(a, e, true))
-        return;
-    const float be = float(a.st[e].beta);
-    const int r0 = lb * a.RB, r1 = min(a.n, r0 + a.RB);
-    for (int i = r0 + slot; i < r1; i += RG) {
-        const size_t q = size_t(i) * Ep + e;
-        a.p[q] = a.invD[q] * a.r[q] + be * a.p[q];
-    }
+    if (op != B_P)
+        S.store(a, op == B_BNORM ? 1 : 2);
 }
 
 struct BFinArgs
 {
-    int E, Ep, G, max_iter, max_outer;
+    int E, G, max_iter, max_outer;
     double tol, inner_tol;
 };
 
-// One block: fold the G partials of each equation (fixed order) and update
+// One workgroup per equation: fold the G partials (fixed order) and update
 // the equation's scalars.
 __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ partials, BEq* st,
                                                  int op, BFinArgs f)
 {
-    __shared__ double red[2][kBlock];
-    const int Ep = f.Ep, RG = kBlock / Ep;
-    const int t = threadIdx.x, e = t & (Ep - 1), j = t / Ep;
+    const int e = blockIdx.x;
     const int nv = (op == B_INIT || op == B_UPDATE) ? 2 : 1;
-    for (int v = 0; v < 2; ++v) {
-        double s = 0.0;
-        if (v < nv)
-            for (int g = j; g < f.G; g += RG)
-                s += partials[(size_t(v) * f.G + g) * Ep + e];
-        red[v][t] = s;
-    }
-    __syncthreads();
-    if (t >= Ep || e >= f.E)
-        return;
     double tot[2] = {0.0, 0.0};
     for (int v = 0; v < nv; ++v)
-        for (int q = 0; q < RG; ++q)
-            tot[v] += red[v][q * Ep + e];
+        for (int g = threadIdx.x; g < f.G; g += kBlock)
+            tot[v] += partials[(size_t(v) * f.E + e) * f.G + g];
+    __shared__ double lds[2 * kWaves];
+    block_sum<2>(tot, lds);
+    if (threadIdx.x != 0)
+        return;
     BEq& s = st[e];
     switch (op) {
     case B_BNORM:
@@ -314,40 +427,49 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
     }
 }
 
-// interleave: dst[i * Ep + e] = src[i] (FP64 and optional FP32 copy)
-__global__ __launch_bounds__(kBlock) void k_bscatter(int64_t m, const double* __restrict__ src,
-                                                     int Ep, int e, double* __restrict__ d64,
+// d[k] = src[map[k]] (map null: identity) in FP64 and FP32
+__global__ __launch_bounds__(kBlock) void k_bconvert(int64_t m, const int* __restrict__ map,
+                                                     const double* __restrict__ src,
+                                                     double* __restrict__ d64,
                                                      float* __restrict__ d32)
 {
     const int64_t stride = int64_t(gridDim.x) * kBlock;
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
-        const double v = src[i];
-        if (d64)
-            d64[i * Ep + e] = v;
-        if (d32)
-            d32[i * Ep + e] = float(v);
+        const double v = src[map ? map[i] : i];
+        d64[i] = v;
+        d32[i] = float(v);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_bgather(int64_t m, const double* __restrict__ src,
-                                                    int Ep, int e, double* __restrict__ dst)
+// Equation-major vectors between the caller's numbering and the batch's:
+// dir 0: dst[e n + i] = src[e n + perm[i]]; dir 1: dst[e n + perm[i]] = src[e n + i].
+__global__ __launch_bounds__(kBlock) void k_bpermute(int64_t n, int E, const int* __restrict__ perm,
+                                                     const double* __restrict__ src,
+                                                     double* __restrict__ dst, int dir)
 {
+    const int64_t m = n * E;
     const int64_t stride = int64_t(gridDim.x) * kBlock;
-    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride)
-        dst[i] = src[i * Ep + e];
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < m; t += stride) {
+        const int64_t e = t / n, i = t - e * n;
+        const int64_t j = e * n + perm[i];
+        if (dir == 0)
+            dst[t] = src[j];
+        else
+            dst[j] = src[t];
+    }
 }
 
-// invD[i * Ep + e] = 1 / a_ii (1 when missing or zero: LinearSolvers.cpp:54-65)
+// invD_e[i] = 1 / a_ii (1 when missing or zero: LinearSolvers.cpp:54-65)
 __global__ __launch_bounds__(kBlock) void k_binvdiag(int n, const int* __restrict__ diag,
-                                                     const double* __restrict__ val, int Ep, int e,
+                                                     const double* __restrict__ val,
                                                      float* __restrict__ invD)
 {
     const int i = int(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n)
         return;
     const int k = diag[i];
-    const double a = k >= 0 ? val[k] : 0.0;
-    invD[size_t(i) * Ep + e] = float(a != 0.0 ? 1.0 / a : 1.0);
+    const double v = k >= 0 ? val[k] : 0.0;
+    invD[i] = float(v != 0.0 ? 1.0 / v : 1.0);
 }
 
 __global__ __launch_bounds__(kBlock) void k_bfill(int64_t m, double* __restrict__ x)
@@ -363,11 +485,14 @@ using namespace mi;
 
 struct mi_batch
 {
-    std::shared_ptr<Pattern> pat;
-    int E = 0, Ep = 1, RB = 256, G = 8;
+    std::shared_ptr<Pattern> src; // the caller's pattern
+    std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM)
+    DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
+    bool reordered = false;
+    int E = 0, G = 8;
     int64_t n = 0, nnz = 0;
     DevBuf<float> v32, invD, r, p, Ap;
-    DevBuf<double> v64, b, x, rt, partials, io;
+    DevBuf<double> v64, b, x, rt, partials;
     DevBuf<BEq> st;
     BEq* h_st = nullptr;
     hipStream_t stream = nullptr;
@@ -375,8 +500,6 @@ struct mi_batch
     int graph_chunk = 0;
     BFinArgs graph_f{}; // finalize arguments baked into the captured graph
     double tune_us = 0.0;
-    std::vector<int> tried_rb;
-    std::vector<double> tried_us;
 
     ~mi_batch()
     {
@@ -392,8 +515,9 @@ struct mi_batch
         BArgs a{};
         a.n = int(n);
         a.E = E;
-        a.Ep = Ep;
-        a.RB = RB;
+        a.nrb = pat->rb.nrb;
+        a.nnz = nnz;
+        a.blk = pat->rb.blk.p;
         a.rp = pat->rp.p;
         a.ci = pat->ci.p;
         a.v32 = v32.p;
@@ -409,63 +533,49 @@ struct mi_batch
         a.partials = partials.p;
         return a;
     }
-    unsigned grid_for_rb(int rb) const
-    {
-        const int64_t g = (n + rb - 1) / rb;
-        return unsigned((g + 7) / 8 * 8);
-    }
 };
 
 namespace
 {
 
-int launch_all(mi_batch* B, void (*k)(BArgs), const BArgs& a)
-{
-    k<<<B->G, kBlock, 0, B->stream>>>(a);
-    MI_HIP(hipGetLastError());
-    return MI_OK;
-}
+constexpr int kBatchMaxGrid = 4096;
 
 int fin(mi_batch* B, int op, const BFinArgs& f)
 {
-    k_bfin<<<1, kBlock, 0, B->stream>>>(B->partials.p, B->st.p, op, f);
+    k_bfin<<<B->E, kBlock, 0, B->stream>>>(B->partials.p, B->st.p, op, f);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
 
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f)
 {
-    MI_TRY(launch_all(B, k_bspmv_dot, a));
+    k_bspmv_dot<<<B->G, kBlock, 0, B->stream>>>(a);
+    MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_ALPHA, f));
-    MI_TRY(launch_all(B, k_bupdate, a));
+    k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_UPDATE);
+    MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_UPDATE, f));
-    return launch_all(B, k_bp, a);
+    k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_P);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
 }
 
-// Autotuner: rows per workgroup for the batched SpMV, timed on this pattern
-// with all equations active.
+// Autotuner: workgroups of the persistent batched launches, timed on this
+// pattern with every equation active.
 int autotune(mi_batch* B)
 {
+    B->G = 8;
     if (B->n == 0)
         return MI_OK;
-    const int RG = kBlock / B->Ep;
     hipEvent_t e0, e1;
     MI_HIP(hipEventCreate(&e0));
     MI_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    int best_rb = RG;
-    B->tried_rb.clear();
-    B->tried_us.clear();
-    for (int mult : {1, 2, 4, 8, 16, 32, 64}) {
-        const int rb = RG * mult;
-        if (mult > 1 && int64_t(rb) > B->n)
+    BArgs a = B->args();
+    a.st = nullptr; // all equations active
+    for (int g : {512, 1024, 2048, 4096}) {
+        if (g > kBatchMaxGrid)
             break;
-        BArgs a = B->args();
-        a.RB = rb;
-        a.st = nullptr; // all equations active
-        const unsigned g = B->grid_for_rb(rb);
-        if (size_t(g) * size_t(B->Ep) * 2 > B->partials.n)
-            continue;
         k_bspmv_dot<<<g, kBlock, 0, B->stream>>>(a);
         MI_HIP(hipGetLastError());
         MI_HIP(hipEventRecord(e0, B->stream));
@@ -477,17 +587,13 @@ int autotune(mi_batch* B)
         float ms = 0.f;
         MI_HIP(hipEventElapsedTime(&ms, e0, e1));
         const double us = 1000.0 * ms / reps;
-        B->tried_rb.push_back(rb);
-        B->tried_us.push_back(us);
         if (us < best) {
             best = us;
-            best_rb = rb;
+            B->G = g;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    B->RB = best_rb;
-    B->G = int(B->grid_for_rb(best_rb));
     B->tune_us = best;
     return MI_OK;
 }
@@ -520,26 +626,28 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     }
     BFinArgs f{};
     f.E = E;
-    f.Ep = B->Ep;
     f.G = B->G;
     f.max_iter = c.max_iterations;
     f.max_outer = c.max_outer > 0 ? c.max_outer : 20;
     f.tol = c.tolerance;
     f.inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-5;
-    const unsigned vg = reduce_grid(n, 4);
-    // b, x = 0 into the interleaved layout
-    for (int e = 0; e < E; ++e)
-        if (n)
-            k_bscatter<<<vg, kBlock, 0, s>>>(n, b_dev + size_t(e) * n, B->Ep, e, B->b.p, nullptr);
-    k_bfill<<<reduce_grid(n * B->Ep, 4), kBlock, 0, s>>>(n * B->Ep, B->x.p);
+    const size_t m = size_t(n) * size_t(E);
+    if (B->reordered) {
+        k_bpermute<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(n, E, B->d_perm.p, b_dev, B->b.p, 0);
+        MI_HIP(hipGetLastError());
+    } else {
+        MI_HIP(hipMemcpyAsync(B->b.p, b_dev, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    k_bfill<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(int64_t(m), B->x.p);
     MI_HIP(hipGetLastError());
     const BArgs a = B->args();
-    MI_TRY(launch_all(B, k_bnorm, a));
+    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BNORM);
+    MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_BNORM, f));
     const int chunk = c.check_interval > 0 ? c.check_interval : 8;
     const bool use_graph = c.use_graph != 0;
     if (use_graph && (!B->graph || B->graph_chunk != chunk ||
-        std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0)) {
+                      std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0)) {
         if (B->graph) {
             (void)hipGraphExecDestroy(B->graph);
             B->graph = nullptr;
@@ -563,7 +671,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         B->graph_f = f;
     }
     for (;;) {
-        MI_TRY(launch_all(B, k_btrue, a));
+        k_btrue<<<B->G, kBlock, 0, s>>>(a);
+        MI_HIP(hipGetLastError());
         MI_TRY(fin(B, B_TRUE, f));
         MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(E), hipMemcpyDeviceToHost, s));
         MI_HIP(hipStreamSynchronize(s));
@@ -572,7 +681,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             all_fin = all_fin && B->h_st[e].fin;
         if (all_fin)
             break;
-        MI_TRY(launch_all(B, k_binit, a));
+        k_bvec<<<B->G, kBlock, 0, s>>>(a, B_INIT);
+        MI_HIP(hipGetLastError());
         MI_TRY(fin(B, B_INIT, f));
         for (;;) {
             if (use_graph) {
@@ -600,10 +710,12 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         if (B->h_st[e].status)
             status = MI_ERR_NOT_CONVERGED;
     }
-    for (int e = 0; e < E; ++e)
-        if (n)
-            k_bgather<<<vg, kBlock, 0, s>>>(n, B->x.p, B->Ep, e, x_dev + size_t(e) * n);
-    MI_HIP(hipGetLastError());
+    if (B->reordered) {
+        k_bpermute<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(n, E, B->d_perm.p, B->x.p, x_dev, 1);
+        MI_HIP(hipGetLastError());
+    } else {
+        MI_HIP(hipMemcpyAsync(x_dev, B->x.p, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     MI_HIP(hipStreamSynchronize(s));
     if (status != MI_OK)
         set_error(MI_ERR_NOT_CONVERGED, "batched solve: some equations did not converge");
@@ -616,33 +728,42 @@ extern "C" {
 
 MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
 {
-    if (!A || !out || n_eq < 1 || n_eq > 64)
-        return set_error(MI_ERR_BAD_ARG, "need a pattern matrix and 1..64 equations");
+    if (!A || !out || n_eq < 1 || n_eq > kMaxEq)
+        return set_error(MI_ERR_BAD_ARG, "need a pattern matrix and 1..%d equations", kMaxEq);
     MI_TRY(ensure_device());
     auto B = std::make_unique<mi_batch>();
+    B->src = A->pat;
     B->pat = A->pat;
     B->E = n_eq;
-    while (B->Ep < n_eq)
-        B->Ep *= 2;
     B->n = A->n();
     B->nnz = A->nnz();
     MI_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_st), sizeof(BEq) * 64, 0));
-    const size_t nE = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Ep);
-    const size_t zE = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(B->Ep);
+    if (B->n >= 100000) {
+        // the same bandwidth reduction as the single-equation solver's setup
+        const Pattern& src = *A->pat;
+        std::vector<int> perm, brp, bci, vmap;
+        rcm_order(int(src.n), src.h_rp.data(), src.h_ci.data(), perm);
+        permute_pattern(int(src.n), src.h_rp.data(), src.h_ci.data(), perm, brp, bci, vmap);
+        MI_TRY(make_pattern(src.n, src.nnz, brp.data(), bci.data(), B->pat, B->stream));
+        MI_TRY(B->d_perm.upload(perm.data(), perm.size(), B->stream));
+        MI_TRY(B->d_vmap.upload(vmap.data(), vmap.size(), B->stream));
+        MI_HIP(hipStreamSynchronize(B->stream));
+        B->reordered = true;
+    }
+    MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_st), sizeof(BEq) * kMaxEq, 0));
+    const size_t nE = size_t(std::max<int64_t>(B->n, 1)) * size_t(n_eq);
+    const size_t zE = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(n_eq);
     MI_TRY(B->v32.alloc(zE));
     MI_TRY(B->v64.alloc(zE));
     for (DevBuf<float>* v : {&B->invD, &B->r, &B->p, &B->Ap})
         MI_TRY(v->alloc(nE));
     for (DevBuf<double>* v : {&B->b, &B->x, &B->rt})
         MI_TRY(v->alloc(nE));
-    // partials for the smallest candidate row block (the autotuner's first)
-    const int RG = kBlock / B->Ep;
-    MI_TRY(B->partials.alloc(size_t(2) * B->grid_for_rb(RG) * size_t(B->Ep)));
-    MI_TRY(B->st.alloc(size_t(B->Ep)));
+    MI_TRY(B->partials.alloc(size_t(2) * kBatchMaxGrid * size_t(n_eq)));
+    MI_TRY(B->st.alloc(size_t(n_eq)));
     MI_HIP(hipMemsetAsync(B->v32.p, 0, zE * sizeof(float), B->stream));
     MI_HIP(hipMemsetAsync(B->p.p, 0, nE * sizeof(float), B->stream));
-    MI_HIP(hipMemsetAsync(B->st.p, 0, sizeof(BEq) * size_t(B->Ep), B->stream));
+    MI_HIP(hipMemsetAsync(B->st.p, 0, sizeof(BEq) * size_t(n_eq), B->stream));
     MI_HIP(hipStreamSynchronize(B->stream));
     MI_TRY(autotune(B.get()));
     *out = B.release();
@@ -653,17 +774,19 @@ MI_API int mi_batch_set_equation(mi_batch* B, int eq, const mi_csr* A)
 {
     if (!B || !A || eq < 0 || eq >= B->E)
         return set_error(MI_ERR_BAD_ARG, "bad argument");
-    if (A->pat != B->pat && (A->n() != B->n || A->nnz() != B->nnz ||
-                             A->pat->h_rp != B->pat->h_rp || A->pat->h_ci != B->pat->h_ci))
+    if (A->pat != B->src && (A->n() != B->n || A->nnz() != B->nnz ||
+                             A->pat->h_rp != B->src->h_rp || A->pat->h_ci != B->src->h_ci))
         return set_error(MI_ERR_BAD_ARG, "matrix does not have the batch's pattern");
     MI_TRY(ensure_device());
     hipStream_t s = B->stream;
+    double* v64 = B->v64.p + size_t(eq) * size_t(B->nnz);
     if (B->nnz)
-        k_bscatter<<<reduce_grid(B->nnz, 4), kBlock, 0, s>>>(B->nnz, A->val.p, B->Ep, eq, B->v64.p,
-                                                            B->v32.p);
+        k_bconvert<<<reduce_grid(B->nnz, 4), kBlock, 0, s>>>(
+            B->nnz, B->reordered ? B->d_vmap.p : nullptr, A->val.p, v64,
+            B->v32.p + size_t(eq) * size_t(B->nnz));
     if (B->n)
-        k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(int(B->n), B->pat->diag.p, A->val.p,
-                                                            B->Ep, eq, B->invD.p);
+        k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
+            int(B->n), B->pat->diag.p, v64, B->invD.p + size_t(eq) * size_t(B->n));
     MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
@@ -675,7 +798,7 @@ MI_API int mi_batch_set_values(mi_batch* B, const double* values_host)
         return set_error(MI_ERR_BAD_ARG, "bad argument");
     MI_TRY(ensure_device());
     mi_csr tmp;
-    tmp.pat = B->pat;
+    tmp.pat = B->src;
     MI_TRY(tmp.val.alloc(size_t(std::max<int64_t>(B->nnz, 1))));
     for (int e = 0; e < B->E; ++e) {
         MI_TRY(tmp.val.upload(values_host + size_t(e) * size_t(B->nnz), size_t(B->nnz), B->stream));
@@ -710,6 +833,7 @@ MI_API int mi_batch_solve(mi_batch* B, const mi_solver_cfg* cfg, const double* b
     DevBuf<double> db, dx;
     MI_TRY(db.upload(b_host, m, B->stream));
     MI_TRY(dx.alloc(std::max<size_t>(m, 1)));
+    MI_HIP(hipStreamSynchronize(B->stream));
     const int rc = mi_batch_solve_device(B, cfg, db.p, dx.p, iterations, rel_residual);
     if (rc != MI_OK && rc != MI_ERR_NOT_CONVERGED)
         return rc;
@@ -722,8 +846,8 @@ MI_API int mi_batch_tuning(const mi_batch* B, int* rows_per_block, double* spmm_
 {
     if (!B)
         return set_error(MI_ERR_BAD_ARG, "null handle");
-    if (rows_per_block)
-        *rows_per_block = B->RB;
+    if (rows_per_block) // rows of one persistent workgroup
+        *rows_per_block = int((B->n + B->G - 1) / B->G);
     if (spmm_us)
         *spmm_us = B->tune_us;
     if (spmm_bytes) // FP32 values per equation, shared columns and row pointers, p once, Ap once
