@@ -201,6 +201,7 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
                                             hipStream_t s)
 {
     const int nl = int(H.levels.size());
+    graphs.clear();
     L.clear();
     L.resize(size_t(nl));
     deg = std::max(degree, 1);
@@ -245,8 +246,10 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
     return MI_OK;
 }
 
-template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s)
+template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph)
 {
+    if (use_graph)
+        return graphs.run(b0, x0, s, [&]() { return vcycle(b0, x0, s, false); });
     const int nl = int(L.size());
     auto bptr = [&](int l) -> const T* { return l == 0 ? b0 : L[size_t(l)].b.p; };
     auto xptr = [&](int l) -> T* { return l == 0 ? x0 : L[size_t(l)].x.p; };
@@ -350,13 +353,13 @@ struct AmgPrecond final : Precond
     {
         if (!has64)
             return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP64 copy");
-        return h64.vcycle(r, z, s);
+        return h64.vcycle(r, z, s, use_graph);
     }
     int apply32(const float* r, float* z, hipStream_t s) override
     {
         if (!has32)
             return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP32 copy");
-        return h32.vcycle(r, z, s);
+        return h32.vcycle(r, z, s, use_graph);
     }
     const char* name() const override { return "sa-amg"; }
     int levels() const override { return int(host.levels.size()); }

@@ -44,9 +44,11 @@ template <class T> struct DHierarchy
     DevBuf<double> cinv; // dense FP64 inverse of the coarsest operator
     int nc = 0;
     int deg = 2;
+    GraphCache graphs; // captured V-cycles
     int build(const AmgHierarchyHost& H, int degree, double lmin_ratio, hipStream_t s);
-    // x0 = V-cycle(b0); x0 needs no initialisation.
-    int vcycle(const T* b0, T* x0, hipStream_t s);
+    // x0 = V-cycle(b0); x0 needs no initialisation.  With use_graph the
+    // cycle's launches are replayed from a graph captured for (b0, x0, s).
+    int vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph = false);
 };
 
 std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c);

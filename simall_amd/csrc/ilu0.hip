@@ -151,6 +151,7 @@ void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& le
 
 template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s)
 {
+    graphs.clear();
     n = int(P.n);
     const int* rp = P.h_rp.data();
     const int* ci = P.h_ci.data();
@@ -188,8 +189,12 @@ template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double
     return MI_OK;
 }
 
-template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s)
+template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_graph)
 {
+    // two sweeps of one launch per level: replayed from a captured graph
+    // when asked (the level count is in the hundreds for 3-D meshes)
+    if (use_graph)
+        return graphs.run(r, z, s, [&]() { return apply(r, z, s, false); });
     for (size_t l = 0; l + 1 < h_loff.size(); ++l) {
         const int beg = h_loff[l], cnt = h_loff[l + 1] - beg;
         k_lower_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(cnt, d_lrows.p + beg, rp_dev,
@@ -230,11 +235,13 @@ struct IluPrecond final : Precond
     }
     int apply64(const double* r, double* z, hipStream_t s) override
     {
-        return has64 ? f64.apply(r, z, s) : set_error(MI_ERR_UNSUPPORTED, "no FP64 ILU(0)");
+        return has64 ? f64.apply(r, z, s, use_graph)
+                     : set_error(MI_ERR_UNSUPPORTED, "no FP64 ILU(0)");
     }
     int apply32(const float* r, float* z, hipStream_t s) override
     {
-        return has32 ? f32.apply(r, z, s) : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
+        return has32 ? f32.apply(r, z, s, use_graph)
+                     : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
     }
     const char* name() const override { return "ilu0"; }
     int levels() const override

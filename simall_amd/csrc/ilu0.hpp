@@ -31,8 +31,9 @@ template <class T> struct Ilu0
     std::vector<int> h_loff, h_uoff;
     bool level_built = false;
     const Pattern* levels_for = nullptr;
+    GraphCache graphs; // captured applies (one level kernel per launch)
     int setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s);
-    int apply(const T* r, T* z, hipStream_t s);
+    int apply(const T* r, T* z, hipStream_t s, bool use_graph = false);
 };
 
 std::unique_ptr<Precond> make_ilu0();

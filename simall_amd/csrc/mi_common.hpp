@@ -1,6 +1,8 @@
 // mi_common.hpp — shared host/device helpers for the gfx950 solve backend.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -198,5 +200,55 @@ __device__ __forceinline__ void fold_partials(const double* partials, int G, dou
 }
 
 #endif
+
+// Replays of a fixed launch sequence (a preconditioner apply): one
+// executable HIP graph per (input, output, stream), captured on first use.
+// The owner clears the cache whenever the buffers or constants the launches
+// use change.  The sequences captured here contain kernel launches only.
+struct GraphCache
+{
+    struct Entry
+    {
+        const void* in;
+        const void* out;
+        hipStream_t s;
+        hipGraphExec_t exec;
+    };
+    std::vector<Entry> entries;
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() { clear(); }
+    void clear()
+    {
+        for (Entry& e : entries)
+            (void)hipGraphExecDestroy(e.exec);
+        entries.clear();
+    }
+    template <class F> int run(const void* in, const void* out, hipStream_t s, F&& launches)
+    {
+        for (const Entry& e : entries)
+            if (e.in == in && e.out == out && e.s == s) {
+                MI_HIP(hipGraphLaunch(e.exec, s));
+                return MI_OK;
+            }
+        hipGraph_t g = nullptr;
+        MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = launches();
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != MI_OK || ec != hipSuccess || !g) {
+            if (g)
+                (void)hipGraphDestroy(g);
+            return rc != MI_OK ? rc : set_error(MI_ERR_DEVICE, "graph capture failed");
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MI_HIP(ei);
+        entries.push_back(Entry{in, out, s, exec});
+        MI_HIP(hipGraphLaunch(exec, s));
+        return MI_OK;
+    }
+};
 
 } // namespace mi

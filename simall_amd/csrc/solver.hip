@@ -334,6 +334,7 @@ int build(mi_solver* S)
         default:
             S->pc = make_jacobi(); // Jacobi, None and unknown kinds (LinearSolvers.cpp:240-254)
         }
+        S->pc->use_graph = S->cfg.use_graph != 0;
     }
     MI_HIP(hipStreamSynchronize(S->stream));
     MI_TRY(S->pc->setup(M, !mixed, mixed, S->stream));

@@ -29,6 +29,7 @@ struct Precond
     virtual const char* name() const = 0;
     virtual int levels() const { return 1; }
     virtual double complexity() const { return 1.0; }
+    bool use_graph = true; // replay applies from captured HIP graphs
 };
 
 std::unique_ptr<Precond> make_jacobi();
