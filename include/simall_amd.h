@@ -154,7 +154,7 @@ typedef struct {
     int warm_start;     /* 0: x0 = 0 as the CPU solvers (LinearSolvers.cpp:424); 1: use x */
     int true_residual;  /* 0: stop on the recurrence residual (reference); 1: the FP64
                            true residual b - Ax must meet the tolerance */
-    double inner_tolerance; /* mixed: relative reduction of each FP32 inner solve */
+    double inner_tolerance; /* mixed: relative reduction of each FP32 inner solve (1e-4) */
     int max_outer;      /* mixed: refinement sweeps */
     int reorder;        /* -1 auto, 0 none, 1 reverse Cuthill-McKee inside setup */
     int check_interval; /* iterations between host convergence polls (0 = auto) */

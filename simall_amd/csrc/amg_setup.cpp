@@ -14,6 +14,7 @@
 //   A_c         R A P with R = P^T (symmetric V-cycle for CG)
 // and the coarsest operator is inverted densely in FP64.
 #include "amg_setup.hpp"
+#include "reorder.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -439,6 +440,45 @@ int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv, int t
     return regularised;
 }
 
+// Renumbers the coarse unknowns of a new level with reverse Cuthill-McKee:
+// A_c <- Q A_c Q^T and P <- P Q^T (R is rebuilt as P^T).  An equivalent
+// hierarchy whose coarse operators have a narrow band, so the device gathers
+// of x on those levels touch few cache lines.
+void amg_renumber_coarse(HCsr& Ac, HCsr& P)
+{
+    const int nc = Ac.nr;
+    if (nc < 2)
+        return;
+    std::vector<int> perm, brp, bci, vmap;
+    rcm_order(nc, Ac.rp.data(), Ac.ci.data(), perm);
+    permute_pattern(nc, Ac.rp.data(), Ac.ci.data(), perm, brp, bci, vmap);
+    std::vector<double> bv(vmap.size());
+    for (size_t k = 0; k < vmap.size(); ++k)
+        bv[k] = Ac.v[size_t(vmap[k])];
+    Ac.rp = std::move(brp);
+    Ac.ci = std::move(bci);
+    Ac.v = std::move(bv);
+    std::vector<int> inv(static_cast<size_t>(nc));
+    for (int i = 0; i < nc; ++i)
+        inv[size_t(perm[size_t(i)])] = i;
+    std::vector<std::pair<int, double>> row;
+    for (int i = 0; i < P.nr; ++i) {
+        row.clear();
+        for (int k = P.rp[size_t(i)]; k < P.rp[size_t(i) + 1]; ++k)
+            row.emplace_back(inv[size_t(P.ci[size_t(k)])], P.v[size_t(k)]);
+        std::sort(row.begin(), row.end(),
+                  [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+                      return a.first < b.first;
+                  });
+        int k = P.rp[size_t(i)];
+        for (const auto& e : row) {
+            P.ci[size_t(k)] = e.first;
+            P.v[size_t(k)] = e.second;
+            ++k;
+        }
+    }
+}
+
 int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
 {
     out.levels.clear();
@@ -468,10 +508,16 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
             amg_smoothed_prolongator(A, agg, nagg, omega, L.P, &srp, &sci);
         else
             amg_smoothed_prolongator(A, agg, nagg, omega, L.P, nullptr, nullptr);
-        csr_transpose(L.P, L.R);
         HCsr AP, Ac;
-        csr_multiply(A, L.P, AP, threads);
-        csr_multiply(L.R, AP, Ac, threads);
+        {
+            HCsr R0;
+            csr_transpose(L.P, R0);
+            csr_multiply(A, L.P, AP, threads);
+            csr_multiply(R0, AP, Ac, threads);
+        }
+        if (p.renumber)
+            amg_renumber_coarse(Ac, L.P);
+        csr_transpose(L.P, L.R);
         out.levels.emplace_back();
         out.levels.back().A = std::move(Ac);
         nnz_total += out.levels.back().A.nnz();

@@ -24,6 +24,7 @@ struct AmgParams
 {
     int max_levels = 10;
     int coarse_size = 2048; // stop coarsening at this many rows (<= kAmgMaxDense)
+    bool renumber = true;   // RCM-renumber every coarse level (amg_renumber_coarse)
     double theta = 0.02;        // strength: |a_ij| >= theta * sqrt(|a_ii a_jj|)
     double omega_scale = 4.0 / 3.0; // prolongator smoothing weight * lambda_max(D^-1 A)
     int threads = 0;            // 0: hardware concurrency (at most 16)

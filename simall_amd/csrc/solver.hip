@@ -551,7 +551,7 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     const double init_res = S->h_scal[0];
     double rn = init_res;
     int total = 0, outer = 0, status = MI_OK;
-    const double inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-5;
+    const double inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-4;
     while (rn / bnorm >= c.tolerance) {
         if (outer >= std::max(c.max_outer, 1) || total >= c.max_iterations) {
             status = MI_ERR_NOT_CONVERGED;
@@ -653,7 +653,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->precision = MI_PRECISION_FP64;
     c->warm_start = 0;
     c->true_residual = 0;
-    c->inner_tolerance = 1.0e-5;
+    c->inner_tolerance = 1.0e-4;
     c->max_outer = 20;
     c->reorder = -1;
     c->check_interval = 0;
