@@ -361,15 +361,35 @@ template <class T> int precond(Precond* pc, const T* r, T* z, hipStream_t s)
 }
 
 // Polls KState every `chunk` issued iterations until the device says done.
+// Issues iterations in chunks between polls of the device state.  Kernels of
+// an iteration after convergence return at once, but a preconditioner apply
+// (a V-cycle, the ILU sweeps) runs regardless, so each chunk is sized from
+// the convergence rate observed between the last two polls to the
+// iterations still needed (at most `chunk`).
 template <class Body> int iterate(mi_solver* S, KState* st, int max_iter, int chunk, Body&& body)
 {
     int issued = 0;
+    int prev_iter = -1;
+    double prev_last = 0.0;
+    int next = std::min(chunk, 2);
     for (;;) {
         MI_HIP(hipMemcpyAsync(S->h_st, st, sizeof(KState), hipMemcpyDeviceToHost, S->stream));
         MI_HIP(hipStreamSynchronize(S->stream));
-        if (S->h_st->done || issued >= max_iter)
+        const KState& h = *S->h_st;
+        if (h.done || issued >= max_iter)
             break;
-        const int k = std::min(chunk, max_iter - issued);
+        if (prev_iter >= 0 && h.iter > prev_iter && h.last > 0.0 && prev_last > 0.0) {
+            const double rho = std::pow(h.last / prev_last, 1.0 / double(h.iter - prev_iter));
+            next = chunk;
+            if (rho > 0.0 && rho < 1.0 && h.tol > 0.0 && h.tol < h.last) {
+                const double need = std::log(h.tol / h.last) / std::log(rho);
+                if (need < double(chunk))
+                    next = std::max(1, int(std::ceil(need)));
+            }
+        }
+        prev_iter = h.iter;
+        prev_last = h.last;
+        const int k = std::min(next, max_iter - issued);
         for (int i = 0; i < k; ++i)
             MI_TRY(body());
         issued += k;
