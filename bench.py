@@ -189,6 +189,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")
     ap.add_argument("--degree", type=int, default=0, help="AMG smoother degree (0: default)")
+    ap.add_argument("--coarse-degree", type=int, default=0,
+                    help="AMG smoother degree below the finest level (0: default)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--caller-order", action="store_true",
                     help="also time SpMV on the caller's (randomly numbered) matrix")
@@ -250,6 +252,8 @@ def main():
         kw = {}
         if args.degree:
             kw["amg_sweeps"] = args.degree
+        if args.coarse_degree:
+            kw["amg_coarse_sweeps"] = args.coarse_degree
         if args.inner_tol:
             kw["inner_tolerance"] = args.inner_tol
         s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=pc, tolerance=1e-8,

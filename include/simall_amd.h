@@ -165,6 +165,7 @@ typedef struct {
     double amg_jacobi_omega; /* prolongator smoothing weight (scaled by 1/rho(D^-1A)) */
     int amg_smoother;      /* 0 = Chebyshev, 1 = damped Jacobi */
     int amg_sweeps;        /* pre = post sweeps / Chebyshev degree */
+    int amg_coarse_sweeps; /* the same on levels below the finest (default 1; 0: amg_sweeps) */
     int keep_preconditioner; /* 1: new values of the bound matrix keep the preconditioner
                                 built on earlier values (the operator is always current) */
     int use_graph;           /* batched solve: 1 replays a captured HIP graph of

@@ -48,6 +48,7 @@ class SolverCfg(C.Structure):
         ("amg_jacobi_omega", C.c_double),
         ("amg_smoother", C.c_int),
         ("amg_sweeps", C.c_int),
+        ("amg_coarse_sweeps", C.c_int),
         ("keep_preconditioner", C.c_int),
         ("use_graph", C.c_int),
     ]

@@ -197,8 +197,8 @@ template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r
     return MI_OK;
 }
 
-template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree, double lmin_ratio,
-                                            hipStream_t s)
+template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree,
+                                            int coarse_degree, double lmin_ratio, hipStream_t s)
 {
     const int nl = int(H.levels.size());
     graphs.clear();
@@ -209,6 +209,8 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
         const AmgLevelHost& h = H.levels[size_t(l)];
         DLevel<T>& d = L[size_t(l)];
         const int n = h.A.nr;
+        const int ldeg = l == 0 || coarse_degree < 1 ? deg : coarse_degree;
+        d.deg = ldeg;
         MI_TRY(d.A.upload(h.A, s, true));
         if (l + 1 < nl) {
             MI_TRY(d.P.upload(h.P, s));
@@ -225,15 +227,15 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
             MI_TRY(b->alloc(size_t(std::max(n, 1))));
         // Chebyshev coefficients on [lmin, lmax] of D^-1 A
         const double lmax = h.lmax > 0 ? h.lmax : 2.0;
-        const double ratio = deg == 1 ? 0.5 : lmin_ratio;
+        const double ratio = ldeg == 1 ? 0.5 : lmin_ratio;
         const double lmin = lmax * ratio;
         const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
         const double sigma = theta / delta;
-        d.alpha.assign(size_t(deg), 0.0);
-        d.beta.assign(size_t(deg), 0.0);
+        d.alpha.assign(size_t(ldeg), 0.0);
+        d.beta.assign(size_t(ldeg), 0.0);
         d.alpha[0] = 1.0 / theta;
         double rho = 1.0 / sigma;
-        for (int k = 1; k < deg; ++k) {
+        for (int k = 1; k < ldeg; ++k) {
             const double rn = 1.0 / (2.0 * sigma - rho);
             d.beta[size_t(k)] = rn * rho;
             d.alpha[size_t(k)] = 2.0 * rn / delta;
@@ -264,7 +266,7 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         MI_HIP(hipGetLastError());
         T* din = d.d.p;
         T* dout = d.d2.p;
-        for (int k = 1; k < deg; ++k) {
+        for (int k = 1; k < d.deg; ++k) {
             MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
                            0, s));
             std::swap(din, dout);
@@ -298,7 +300,7 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, s));
         T* din = d.d.p;
         T* dout = d.d2.p;
-        for (int k = 1; k < deg; ++k) {
+        for (int k = 1; k < d.deg; ++k) {
             MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
                            0, s));
             std::swap(din, dout);
@@ -320,6 +322,7 @@ struct AmgPrecond final : Precond
 {
     AmgParams params;
     int degree = 2;
+    int coarse_degree = 0;
     double lmin_ratio = 0.3;
     AmgHierarchyHost host;
     DHierarchy<double> h64;
@@ -340,11 +343,11 @@ struct AmgPrecond final : Precond
                              kAmgMaxDense);
         has64 = has32 = false;
         if (need64) {
-            MI_TRY(h64.build(host, degree, lmin_ratio, s));
+            MI_TRY(h64.build(host, degree, coarse_degree, lmin_ratio, s));
             has64 = true;
         }
         if (need32) {
-            MI_TRY(h32.build(host, degree, lmin_ratio, s));
+            MI_TRY(h32.build(host, degree, coarse_degree, lmin_ratio, s));
             has32 = true;
         }
         return MI_OK;
@@ -380,6 +383,7 @@ std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c)
     if (c.amg_jacobi_omega > 0)
         p->params.omega_scale = c.amg_jacobi_omega;
     p->degree = c.amg_sweeps > 0 ? c.amg_sweeps : 2;
+    p->coarse_degree = c.amg_coarse_sweeps;
     return p;
 }
 

@@ -36,6 +36,7 @@ template <class T> struct DLevel
     DevBuf<T> invD;
     DevBuf<T> x, b, r, d, d2, e; // x and b of level 0 are the caller's vectors
     std::vector<double> alpha, beta; // Chebyshev step coefficients
+    int deg = 2;                     // smoother degree on this level
 };
 
 template <class T> struct DHierarchy
@@ -45,7 +46,10 @@ template <class T> struct DHierarchy
     int nc = 0;
     int deg = 2;
     GraphCache graphs; // captured V-cycles
-    int build(const AmgHierarchyHost& H, int degree, double lmin_ratio, hipStream_t s);
+    // degree: Chebyshev degree on level 0; coarse_degree on the coarser
+    // levels (< 1: the same as level 0).
+    int build(const AmgHierarchyHost& H, int degree, int coarse_degree, double lmin_ratio,
+              hipStream_t s);
     // x0 = V-cycle(b0); x0 needs no initialisation.  With use_graph the
     // cycle's launches are replayed from a graph captured for (b0, x0, s).
     int vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph = false);

@@ -683,6 +683,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->amg_jacobi_omega = 4.0 / 3.0;
     c->amg_smoother = 0;
     c->amg_sweeps = 2;
+    c->amg_coarse_sweeps = 1;
     c->keep_preconditioner = 0;
     c->use_graph = 1;
 }

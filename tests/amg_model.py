@@ -2,8 +2,10 @@
 
 Applies, with scipy sparse matrices, the same cycle the device runs on a
 hierarchy obtained from ``capi.AmgHost``: Chebyshev smoothing of degree k on
-[lmin, lmax] of D^-1 A (k = 1: lmin = lmax/2, damped Jacobi), restriction by
-R = P^T, prolongation by P, dense inverse on the coarsest level.
+[lmin, lmax] of D^-1 A (k = 1: lmin = lmax/2, damped Jacobi; `degree` on the
+finest level, `coarse_degree` below it, as the solver's amg_sweeps and
+amg_coarse_sweeps with their defaults 2 and 1), restriction by R = P^T,
+prolongation by P, dense inverse on the coarsest level.
 """
 import numpy as np
 
@@ -23,7 +25,7 @@ def chebyshev_coefficients(lmax, degree, ratio=0.3):
 
 
 class VCycle:
-    def __init__(self, host, degree=2):
+    def __init__(self, host, degree=2, coarse_degree=1):
         self.levels = []
         self.degree = degree
         for l in range(host.levels):
@@ -34,14 +36,14 @@ class VCycle:
             R = None if last else host.scipy(l, host.R)
             d = A.diagonal()
             inv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 1.0)
-            self.levels.append((A, P, R, inv, *chebyshev_coefficients(lam, degree)))
+            k = degree if l == 0 else coarse_degree
+            self.levels.append((A, P, R, inv, k, *chebyshev_coefficients(lam, k)))
         self.coarse_inverse = host.matrix(host.levels - 1, host.COARSE_INV)
 
     def __call__(self, b, level=0):
-        A, P, R, inv, al, be = self.levels[level]
+        A, P, R, inv, k, al, be = self.levels[level]
         if level == len(self.levels) - 1:
             return self.coarse_inverse @ b
-        k = self.degree
         r = b.copy()
         d = al[0] * inv * b
         x = d.copy()
