@@ -1,4 +1,4 @@
-"""Batched multi-RHS solve of equations sharing one pattern (config-4 form).
+"""Batched multi-RHS solve of transport equations sharing one pattern.
 
 Systems: A_e = V/dt I + Gamma_e L on a perturbed, permuted hex mesh with
 symmetry boundaries, assembled on the GPU by the momentum assembly with zero
