@@ -156,7 +156,8 @@ typedef struct {
                            true residual b - Ax must meet the tolerance */
     double inner_tolerance; /* mixed: relative reduction of each FP32 inner solve (1e-4) */
     int max_outer;      /* mixed: refinement sweeps */
-    int reorder;        /* -1 auto, 0 none, 1 reverse Cuthill-McKee inside setup */
+    int reorder;        /* -1 auto (n >= 100000: multicolour for ILU(0), else reverse
+                           Cuthill-McKee), 0 none, 1 reverse Cuthill-McKee, 2 multicolour */
     int check_interval; /* iterations between host convergence polls (0 = auto) */
     /* smoothed-aggregation AMG */
     int amg_max_levels;

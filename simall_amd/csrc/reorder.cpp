@@ -173,4 +173,44 @@ int bandwidth(int n, const int* rp, const int* ci)
     return bw;
 }
 
+void color_order(int n, const int* rp, const int* ci, std::vector<int>& perm, int* colors)
+{
+    std::vector<int> rcm;
+    rcm_order(n, rp, ci, rcm);
+    std::vector<int> arp, aci;
+    symmetrise(n, rp, ci, arp, aci);
+    // greedy colouring in RCM order: the smallest colour unused by coloured neighbours
+    std::vector<int> col(static_cast<size_t>(n), -1), mark;
+    int nc = 0;
+    for (int t = 0; t < n; ++t) {
+        const int u = rcm[size_t(t)];
+        for (int k = arp[size_t(u)]; k < arp[size_t(u) + 1]; ++k) {
+            const int c = col[size_t(aci[size_t(k)])];
+            if (c >= 0) {
+                if (c >= int(mark.size()))
+                    mark.resize(size_t(c) + 1, -1);
+                mark[size_t(c)] = u;
+            }
+        }
+        int c = 0;
+        while (c < int(mark.size()) && mark[size_t(c)] == u)
+            ++c;
+        col[size_t(u)] = c;
+        nc = std::max(nc, c + 1);
+    }
+    // rows grouped by colour, RCM order within each colour
+    std::vector<int> start(static_cast<size_t>(nc) + 1, 0);
+    for (int u = 0; u < n; ++u)
+        ++start[size_t(col[size_t(u)]) + 1];
+    for (int c = 0; c < nc; ++c)
+        start[size_t(c) + 1] += start[size_t(c)];
+    perm.assign(static_cast<size_t>(n), 0);
+    for (int t = 0; t < n; ++t) {
+        const int u = rcm[size_t(t)];
+        perm[size_t(start[size_t(col[size_t(u)])]++)] = u;
+    }
+    if (colors)
+        *colors = nc;
+}
+
 } // namespace mi

@@ -247,6 +247,15 @@ bool want_reorder(const mi_solver* S)
     return r > 0 || (r < 0 && S->A->n() >= 100000);
 }
 
+// Ordering of the working matrix: multicolour for ILU(0) when asked (2) or by
+// default at large sizes (its triangular sweeps then have one level per
+// colour instead of hundreds), reverse Cuthill-McKee otherwise.
+bool want_colors(const mi_solver* S)
+{
+    const int r = S->cfg.reorder;
+    return r == 2 || (r < 0 && S->cfg.preconditioner == MI_PC_ILU);
+}
+
 // Computes the reverse Cuthill-McKee permutation of A's pattern and the
 // reordered working matrix (once per pattern).
 int build_reordering(mi_solver* S)
@@ -256,7 +265,10 @@ int build_reordering(mi_solver* S)
     auto ro = std::make_unique<Reordering>();
     ro->source = A->pat;
     const int n = int(src.n);
-    rcm_order(n, src.h_rp.data(), src.h_ci.data(), ro->perm);
+    if (want_colors(S))
+        color_order(n, src.h_rp.data(), src.h_ci.data(), ro->perm, &ro->colors);
+    else
+        rcm_order(n, src.h_rp.data(), src.h_ci.data(), ro->perm);
     std::vector<int> brp, bci;
     permute_pattern(n, src.h_rp.data(), src.h_ci.data(), ro->perm, brp, bci, ro->vmap);
     ro->bandwidth_before = bandwidth(n, src.h_rp.data(), src.h_ci.data());

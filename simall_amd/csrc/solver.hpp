@@ -44,6 +44,7 @@ struct Reordering
     DevBuf<int> d_vmap;               // reordered entry k comes from source entry vmap[k]
     std::vector<int> vmap;
     int bandwidth_before = 0, bandwidth_after = 0;
+    int colors = 0; // multicolour ordering: colour count (0 for RCM)
 };
 
 } // namespace mi

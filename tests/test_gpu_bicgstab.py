@@ -115,3 +115,17 @@ def test_bicgstab_mixed_and_true_residual(gpu, momentum):
         rel = np.linalg.norm(b - oracle.spmv(A, x)) / np.linalg.norm(b)
         print(f"BiCGStab-ILU precision {prec}: {st.iterations} its, {st.outer_iterations} sweeps, rel {rel:.2e}")
         assert rc == capi.MI_OK and rel <= 1e-8
+
+
+@pytest.mark.parametrize("prec", [capi.MI_PRECISION_FP64, capi.MI_PRECISION_MIXED])
+def test_bicgstab_ilu_multicolour_ordering(gpu, momentum, prec):
+    """reorder = 2: ILU(0) of the multicolour-ordered matrix (one sweep level
+    per colour); the solution comes back in the caller's numbering."""
+    A, b = momentum
+    s = solver(A, precision=prec, true_residual=1, reorder=2)
+    x, st, rc = s.solve(b)
+    info = s.info()
+    rel = np.linalg.norm(b - oracle.spmv(A, x)) / np.linalg.norm(b)
+    print(f"multicolour ILU-BiCGStab prec {prec}: {st.iterations} its, levels {info['levels']}, rel {rel:.2e}")
+    assert rc == capi.MI_OK and rel <= 1e-8
+    assert info["levels"] <= 8  # colours of a 7-point stencil under greedy colouring
