@@ -5,10 +5,13 @@
 // iteration kept on the device.  Mixed path: iterative refinement — the FP64
 // true residual r = b - A x is formed on the device, an FP32 preconditioned
 // CG solves A d = r to `inner_tolerance`, x += d in FP64, repeat until the
-// FP64 true residual meets `tolerance`.
+// FP64 true residual meets `tolerance`.  GMRES(m) (LinearSolvers.cpp:266-340)
+// restarts every cycle from the FP64 residual; its mixed form runs the
+// Arnoldi steps in FP32.
 #include "solver.hpp"
 #include "stream.hpp"
 #include "amg.hpp"
+#include "gmres.hpp"
 #include "ilu0.hpp"
 #include "reorder.hpp"
 
@@ -219,13 +222,15 @@ int ensure_workspace(mi_solver* S)
     const bool mixed = S->cfg.precision == MI_PRECISION_MIXED;
     // Krylov vectors: CG uses slots 0-3 (r z p Ap), BiCGStab 0-7
     // (r rh p v s t y z); the FP32 pool adds 8 = outer residual, 9 = correction.
-    const int nv = S->cfg.kind == MI_SOLVER_BICGSTAB ? 8 : 4;
+    // GMRES keeps its basis in its own workspace.
+    const bool gmres = S->cfg.kind == MI_SOLVER_GMRES;
+    const int nv = gmres ? 0 : S->cfg.kind == MI_SOLVER_BICGSTAB ? 8 : 4;
     for (int k = 0; k < nv; ++k)
         MI_TRY(S->w64[k].alloc(n));
     if (mixed)
         for (int k = 0; k < nv; ++k)
             MI_TRY(S->w32[k].alloc(n));
-    if (mixed) {
+    if (mixed && !gmres) {
         MI_TRY(S->w32[8].alloc(n));
         MI_TRY(S->w32[9].alloc(n));
     }
@@ -619,6 +624,193 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     return status;
 }
 
+// ------------------------------------------------------------- GMRES(m)
+int ensure_gmres(mi_solver* S)
+{
+    const int m = std::max(1, S->cfg.restart);
+    if (m > kGmMaxRestart)
+        return set_error(MI_ERR_BAD_ARG, "GMRES restart %d exceeds the device limit %d", m,
+                         kGmMaxRestart);
+    if (!S->gm)
+        S->gm = std::make_unique<GmWork>();
+    GmWork& W = *S->gm;
+    const int64_t n = S->M->n();
+    const bool mixed = S->cfg.precision == MI_PRECISION_MIXED;
+    if (W.m == m && W.n == n && (mixed ? W.V32.p != nullptr : W.V64.p != nullptr))
+        return MI_OK;
+    W.m = m;
+    W.n = n;
+    W.ld = size_t((n + 63) / 64 * 64); // 512-byte aligned basis vectors
+    W.G = gm_grid(n);
+    const size_t nb = size_t(m + 1) * W.ld;
+    if (mixed) {
+        W.V64.release();
+        W.w64.release();
+        MI_TRY(W.V32.alloc(nb));
+        MI_TRY(W.w32.alloc(size_t(n)));
+    } else {
+        W.V32.release();
+        W.w32.release();
+        MI_TRY(W.V64.alloc(nb));
+        MI_TRY(W.w64.alloc(size_t(n)));
+    }
+    MI_TRY(W.small.alloc(gm_small_size(m)));
+    MI_TRY(W.partials.alloc(size_t(m + 1) * W.G));
+    MI_TRY(W.st.alloc(1));
+    if (!W.h_st)
+        MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&W.h_st), sizeof(GmState)));
+    return MI_OK;
+}
+
+// Restarted GMRES(m) of GMRESmSolver::solve (src/solver/LinearSolvers.cpp:266-340)
+// on the working matrix, from the iterate in x (FP64).  T is the precision
+// of the Arnoldi steps; every cycle starts from the FP64 residual b - A x.
+// With `mixed` the in-cycle estimate only ends a cycle and the FP64 residual
+// test at the next cycle start decides convergence.  The final state is left
+// in the pinned mirror of the workspace.
+template <class T>
+int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
+{
+    GmWork& W = *S->gm;
+    mi_csr* M = S->M;
+    const Pattern& P = *M->pat;
+    hipStream_t s = S->stream;
+    Precond* pc = S->pc.get();
+    const int n = int(P.n), m = W.m;
+    GmArgs<T> a{};
+    a.n = n;
+    a.m = m;
+    a.ld = W.ld;
+    a.A64 = StreamMat<double>{n, P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, M->val.p, nullptr, P.rb.lpr};
+    if constexpr (sizeof(T) == 8) {
+        a.V = W.V64.p;
+        a.w = W.w64.p;
+        a.invD = pc->inv_diag64();
+        a.A = a.A64;
+    } else {
+        a.V = W.V32.p;
+        a.w = W.w32.p;
+        a.invD = pc->inv_diag32();
+        a.A = StreamMat<float>{n, P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, M->op32(), M->diag32(),
+                               P.rb.lpr};
+    }
+    a.b = b;
+    a.x = x;
+    a.g = W.st.p;
+    double* sm = W.small.p;
+    a.H = sm;
+    sm += size_t(m + 1) * size_t(m);
+    a.cs = sm;
+    sm += m;
+    a.sn = sm;
+    sm += m;
+    a.y = sm;
+    sm += m;
+    a.sv = sm;
+    sm += m + 1;
+    a.h1 = sm;
+    sm += m + 1;
+    a.h2 = sm;
+    a.partials = W.partials.p;
+    a.history = S->hist.p;
+    a.hist_cap = int(S->hist.n);
+    a.G = int(W.G);
+    a.Gs = stream_grid(P.rb.nrb);
+    MI_TRY(gm_init<T>(a, S->dscal.p + 1, S->cfg.tolerance, S->cfg.max_iterations, mixed, s));
+    const int chunk = check_interval(S, n);
+    auto poll = [&]() -> int {
+        MI_HIP(hipMemcpyAsync(W.h_st, W.st.p, sizeof(GmState), hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        return MI_OK;
+    };
+    for (;;) {
+        // cycle start: v_0 = M^-1 (b - A x), beta, tests, v_0 /= beta
+        MI_TRY(gm_cycle_residual<T>(a, s));
+        if (!a.invD)
+            MI_TRY(precond<T>(pc, a.w, a.V, s));
+        MI_TRY(gm_cycle_begin<T>(a, s));
+        MI_TRY(poll());
+        if (W.h_st->k.done)
+            break;
+        ++*cycles;
+        const int room = std::min(m, W.h_st->k.max_iter - W.h_st->k.iter);
+        // Arnoldi steps in groups between polls, sized from the observed
+        // convergence rate as in the CG / BiCGStab driver
+        int j = 0, prev_j = -1, next = std::min(chunk, 2);
+        double prev_last = 0.0;
+        while (j < room) {
+            const int k = std::min(next, room - j);
+            for (int q = 0; q < k; ++q, ++j) {
+                MI_TRY(gm_step_spmv<T>(a, j, s));
+                if (!a.invD)
+                    MI_TRY(precond<T>(pc, a.w, a.V + size_t(j + 1) * a.ld, s));
+                MI_TRY(gm_step_orth<T>(a, j, s));
+            }
+            MI_TRY(poll());
+            const GmState& h = *W.h_st;
+            if (h.k.done || h.cycle_end)
+                break;
+            next = chunk;
+            if (prev_j >= 0 && h.j > prev_j && h.k.last > 0.0 && prev_last > 0.0) {
+                const double rho = std::pow(h.k.last / prev_last, 1.0 / double(h.j - prev_j));
+                if (rho > 0.0 && rho < 1.0 && h.k.tol > 0.0 && h.k.tol < h.k.last) {
+                    const double need = std::log(h.k.tol / h.k.last) / std::log(rho);
+                    if (need < double(chunk))
+                        next = std::max(1, int(std::ceil(need)));
+                }
+            }
+            prev_j = h.j;
+            prev_last = h.k.last;
+        }
+        // cycle end: y = H^-1 s, x += V y
+        MI_TRY(gm_cycle_end<T>(a, s));
+        if (W.h_st->k.done) {
+            MI_HIP(hipStreamSynchronize(s));
+            break;
+        }
+    }
+    return MI_OK;
+}
+
+int solve_gmres(mi_solver* S, const double* b, double* x, mi_stats* out)
+{
+    const mi_solver_cfg& c = S->cfg;
+    const bool mixed = c.precision == MI_PRECISION_MIXED;
+    const int n = int(S->M->n());
+    const bool warm = c.warm_start != 0;
+    MI_TRY(ensure_gmres(S));
+    if (!warm)
+        MI_HIP(hipMemsetAsync(x, 0, sizeof(double) * size_t(n), S->stream));
+    MI_TRY(norm2(S, n, b, S->dscal.p + 1));
+    if (warm)
+        MI_TRY(true_residual(S, b, x, nullptr, nullptr, S->dscal.p + 2));
+    MI_TRY(read_scalars(S, 3));
+    const double bnorm = std::max(S->h_scal[1], 1e-30);
+    const double init_res = warm ? S->h_scal[2] : S->h_scal[1];
+    int cycles = 0;
+    if (mixed)
+        MI_TRY(run_gmres<float>(S, b, x, true, &cycles));
+    else
+        MI_TRY(run_gmres<double>(S, b, x, false, &cycles));
+    const GmState h = *S->gm->h_st;
+    copy_history(S, std::min<int>(h.steps, int(S->hist.n)));
+    MI_TRY(true_residual(S, b, x, nullptr, nullptr, S->dscal.p));
+    MI_TRY(read_scalars(S, 1));
+    const int status = h.k.done == 1   ? MI_OK
+                       : h.k.done == 3 ? MI_ERR_BREAKDOWN
+                                       : MI_ERR_NOT_CONVERGED;
+    S->last = h.k.last;
+    out->iterations = h.k.iter;
+    out->outer_iterations = cycles;
+    out->status = status;
+    out->converged = status == MI_OK;
+    out->initial_residual = init_res;
+    out->relative_residual = h.k.last;
+    out->final_residual = h.k.last * bnorm;
+    out->true_relative_residual = S->h_scal[0] / bnorm;
+    return status;
+}
+
 int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
 {
     mi_stats local{};
@@ -626,8 +818,9 @@ int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
     std::memset(out, 0, sizeof(*out));
     if (!S->A)
         return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
-    if (S->cfg.kind != MI_SOLVER_CG && S->cfg.kind != MI_SOLVER_BICGSTAB)
-        return set_error(MI_ERR_UNSUPPORTED, "solver kind %d not available on device yet",
+    if (S->cfg.kind != MI_SOLVER_CG && S->cfg.kind != MI_SOLVER_BICGSTAB &&
+        S->cfg.kind != MI_SOLVER_GMRES)
+        return set_error(MI_ERR_UNSUPPORTED, "solver kind %d not available on device",
                          S->cfg.kind);
     MI_TRY(build(S));
     out->setup_seconds = S->setup_seconds;
@@ -653,7 +846,9 @@ int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
         xw = S->xw.p;
     }
     int rc;
-    if (S->cfg.precision == MI_PRECISION_MIXED)
+    if (S->cfg.kind == MI_SOLVER_GMRES)
+        rc = solve_gmres(S, bw, xw, out);
+    else if (S->cfg.precision == MI_PRECISION_MIXED)
         rc = solve_mixed(S, bw, xw, out);
     else
         rc = solve_fp64(S, bw, xw, out);
@@ -717,6 +912,12 @@ MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out)
     if (S->cfg.max_iterations < 0 || !(S->cfg.tolerance >= 0.0)) {
         delete S;
         return set_error(MI_ERR_BAD_ARG, "bad tolerance or iteration limit");
+    }
+    if (S->cfg.kind == MI_SOLVER_GMRES && S->cfg.restart > kGmMaxRestart) {
+        const int m = S->cfg.restart;
+        delete S;
+        return set_error(MI_ERR_BAD_ARG, "GMRES restart %d exceeds the device limit %d", m,
+                         kGmMaxRestart);
     }
     if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
         delete S;

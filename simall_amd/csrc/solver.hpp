@@ -34,6 +34,8 @@ struct Precond
 
 std::unique_ptr<Precond> make_jacobi();
 
+struct GmWork; // GMRES(m) workspace (gmres.hpp)
+
 // Symmetric reordering of a pattern for the solver's working matrix.
 struct Reordering
 {
@@ -69,6 +71,7 @@ struct mi_solver
     mi::DevBuf<double> partials;
     mi::DevBuf<double> dscal;   // device scalars for norms
     double* h_scal = nullptr;   // pinned
+    std::unique_ptr<mi::GmWork> gm; // GMRES(m) basis and state, allocated on first use
     std::vector<double> history;
     double last = 0.0;
     double setup_seconds = 0.0;

@@ -8,12 +8,17 @@ namespace simall::hip
 
 bool HipLinearSolver::supported(const solver::LinearSolverConfig& cfg)
 {
-    // On device: CG with Jacobi / None (Jacobi, as make_preconditioner's
-    // default branch: src/solver/LinearSolvers.cpp:240-254) or SA-AMG.
-    if (cfg.kind != solver::LinearSolverKind::CG)
+    // On device: CG, BiCGStab and GMRES(m) with Jacobi / None (Jacobi, as
+    // make_preconditioner's default branch: src/solver/LinearSolvers.cpp:240-254),
+    // ILU(0) or SA-AMG.  TFQMR and SSOR stay on the host.
+    if (cfg.kind != solver::LinearSolverKind::CG && cfg.kind != solver::LinearSolverKind::BiCGSTAB &&
+        cfg.kind != solver::LinearSolverKind::GMRES)
         return false;
+    if (cfg.kind == solver::LinearSolverKind::GMRES && cfg.restart > 256)
+        return false; // longest device restart cycle
     return cfg.preconditioner == solver::PreconditionerKind::Jacobi ||
            cfg.preconditioner == solver::PreconditionerKind::None ||
+           cfg.preconditioner == solver::PreconditionerKind::ILU ||
            cfg.preconditioner == solver::PreconditionerKind::AMG;
 }
 
@@ -87,8 +92,9 @@ int HipLinearSolver::solve(const solver::CSRMatrix& A, const util::aligned_vecto
     if (rc == MI_OK)
         return stats_.iterations;
     if (rc == MI_ERR_BREAKDOWN)
-        return stats_.iterations; // p.Ap == 0 returns the current count (LinearSolvers.cpp:433-436)
-    return maxIterations_;
+        return stats_.iterations; // CG p.Ap == 0 / BiCGStab rho == 0 return the current count
+                                  // (LinearSolvers.cpp:375-378, 433-436)
+    return maxIterations_;        // the iteration limit (GMRES counts up to it: :285-340)
 }
 
 } // namespace simall::hip

@@ -105,9 +105,42 @@ int main(int argc, char** argv)
         if (!g_factory)
             return fail("on_load did not register the factory");
         solver::LinearSolverConfig cfg;
-        cfg.kind = solver::LinearSolverKind::GMRES; // no device path: host keeps its own
+        cfg.kind = solver::LinearSolverKind::TFQMR; // no device path: host keeps its own
         if (g_factory(&cfg) != nullptr)
-            return fail("GMRES should not be claimed");
+            return fail("TFQMR should not be claimed");
+        cfg.kind = solver::LinearSolverKind::CG;
+        cfg.preconditioner = solver::PreconditionerKind::SSOR;
+        if (g_factory(&cfg) != nullptr)
+            return fail("SSOR should not be claimed");
+        {
+            // GMRES(20) + ILU(0) as the turbulence models configure it
+            // (src/turbulence/KOmegaSST.cpp:32-37), on a nonsymmetric system
+            solver::LinearSolverConfig g;
+            g.kind = solver::LinearSolverKind::GMRES;
+            g.preconditioner = solver::PreconditionerKind::ILU;
+            g.tolerance = 1e-6;
+            g.maxIterations = 100;
+            g.restart = 20;
+            std::unique_ptr<solver::ILinearSolver> sg(g_factory(&g));
+            if (!sg)
+                return fail("GMRES/ILU factory");
+            const auto N = tridiag(500, -1.3, 2.5, -0.7);
+            util::aligned_vector<double> bg(500, 1.0), xg;
+            const int itg = sg->solve(N, bg, xg);
+            const double rg = residual(N, bg, xg) / std::sqrt(500.0);
+            std::printf("GMRES(20)/ILU n=500: %d iterations, rel %.2e, last_residual %.3e\n", itg,
+                        rg, sg->last_residual());
+            if (!(itg < 100 && rg < 1e-5 && sg->last_residual() < 1e-6))
+                return fail("GMRES solve through ILinearSolver");
+            g.kind = solver::LinearSolverKind::BiCGSTAB;
+            std::unique_ptr<solver::ILinearSolver> sb(g_factory(&g));
+            util::aligned_vector<double> xb;
+            const int itb = sb ? sb->solve(N, bg, xb) : -1;
+            const double rb = residual(N, bg, xb) / std::sqrt(500.0);
+            std::printf("BiCGStab/ILU n=500: %d iterations, rel %.2e\n", itb, rb);
+            if (!(itb >= 0 && itb < 100 && rb < 1e-5))
+                return fail("BiCGStab solve through ILinearSolver");
+        }
         // tests/unit/gpu/test_gpu_models.cpp:181-235 through ILinearSolver
         cfg.kind = solver::LinearSolverKind::CG;
         cfg.preconditioner = solver::PreconditionerKind::Jacobi;
