@@ -144,16 +144,20 @@ MI_API int mi_csr_destroy(mi_csr* A);
  * (src/gpu/CgGpu.hpp:49-125).  The first five fields are LinearSolverConfig
  * with the same defaults (tolerance 1e-8, 1000 iterations, restart 30). */
 typedef struct {
-    int kind;           /* MI_SOLVER_* */
+    int kind;           /* MI_SOLVER_CG, _BICGSTAB or _GMRES on the device (TFQMR: unsupported) */
     int preconditioner; /* MI_PC_* (None and unknown kinds act as Jacobi, as
                            make_preconditioner: src/solver/LinearSolvers.cpp:240-254) */
-    double tolerance;   /* stop when ||r||/max(||b||,1e-30) < tolerance */
+    double tolerance;   /* stop when ||r||/max(||b||,1e-30) < tolerance; GMRES measures
+                           the preconditioned residual ||M^-1 r|| as GMRESmSolver::solve
+                           (src/solver/LinearSolvers.cpp:281-326) */
     int max_iterations;
-    int restart;
-    int precision;      /* MI_PRECISION_FP64 or MI_PRECISION_MIXED */
+    int restart;        /* GMRES cycle length m, max(1, restart), at most 256 on the device */
+    int precision;      /* MI_PRECISION_FP64 or MI_PRECISION_MIXED (GMRES: FP32 Arnoldi
+                           cycles, each restarted and tested on the FP64 residual) */
     int warm_start;     /* 0: x0 = 0 as the CPU solvers (LinearSolvers.cpp:424); 1: use x */
     int true_residual;  /* 0: stop on the recurrence residual (reference); 1: the FP64
-                           true residual b - Ax must meet the tolerance */
+                           true residual b - Ax must meet the tolerance (CG, BiCGStab;
+                           GMRES cycles always restart from it) */
     double inner_tolerance; /* mixed: relative reduction of each FP32 inner solve (1e-4) */
     int max_outer;      /* mixed: refinement sweeps */
     int reorder;        /* -1 auto (n >= 100000: multicolour for ILU(0), else reverse
@@ -169,14 +173,15 @@ typedef struct {
     int amg_coarse_sweeps; /* the same on levels below the finest (default 1; 0: amg_sweeps) */
     int keep_preconditioner; /* 1: new values of the bound matrix keep the preconditioner
                                 built on earlier values (the operator is always current) */
-    int use_graph;           /* batched solve: 1 replays a captured HIP graph of
-                                check_interval iterations, 0 launches each kernel */
+    int use_graph;           /* 1: replay captured HIP graphs (AMG V-cycles; the batched
+                                solve's check_interval iterations), 0: launch each kernel */
 } mi_solver_cfg;
 
 /* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */
 typedef struct {
-    int iterations;        /* Krylov iterations (all inner iterations when mixed) */
-    int outer_iterations;  /* refinement sweeps (mixed); 1 otherwise */
+    int iterations;        /* Krylov iterations (all inner iterations when mixed); GMRES
+                              counts as the reference: the step that met the tolerance not */
+    int outer_iterations;  /* refinement sweeps (mixed); GMRES restart cycles; 1 otherwise */
     int converged;
     int status;            /* mi_status of the solve */
     double initial_residual; /* ||b - A x0|| */

@@ -109,6 +109,87 @@ __device__ __forceinline__ void gm_store_partials(double* lds, int nv, double* p
     }
 }
 
+constexpr int R = kGmRowsPerThread;
+
+// This thread's rows of the block's range: row k_r = beg + tid + r * kBlock.
+// Loads use the clamped index kk[r] (always inside the vector), so the basis
+// loops are straight-line code whose loads the compiler can issue together;
+// rows outside the range carry z = 0 and are never stored.
+struct GmRows
+{
+    int kk[R];
+    bool in[R];
+    __device__ __forceinline__ GmRows(const RowRange& rr, int n)
+    {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int k = rr.beg + int(threadIdx.x) + r * kBlock;
+            in[r] = k < rr.end;
+            kk[r] = min(k, n - 1);
+        }
+    }
+};
+
+// lds[i][wave] = this wave's share of v_i . z for NG vectors from i0
+template <class T, int NG>
+__device__ __forceinline__ void gm_dot_group(const GmArgs<T>& a, int i0, const GmRows& rw,
+                                             const double (&zr)[R], double* lds)
+{
+    T v[NG][R];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[g][r] = a.V[size_t(i0 + g) * a.ld + rw.kk[r]];
+    double s[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        s[g] = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            s[g] += double(v[g][r]) * zr[r];
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        s[g] = wave_sum(s[g]);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+            lds[(i0 + g) * (kBlock / kWave) + threadIdx.x / kWave] = s[g];
+}
+
+template <class T>
+__device__ __forceinline__ void gm_dots(const GmArgs<T>& a, int nv, const GmRows& rw,
+                                        const double (&zr)[R], double* lds)
+{
+    int i = 0;
+    for (; i + 4 <= nv; i += 4)
+        gm_dot_group<T, 4>(a, i, rw, zr, lds);
+    for (; i < nv; ++i)
+        gm_dot_group<T, 1>(a, i, rw, zr, lds);
+    gm_store_partials(lds, nv, a.partials);
+}
+
+// z -= h_i v_i for NG vectors from i0, in ascending i per row
+template <class T, int NG>
+__device__ __forceinline__ void gm_update_group(const GmArgs<T>& a, int i0, const double* h,
+                                                const GmRows& rw, double (&zr)[R])
+{
+    T v[NG][R];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[g][r] = a.V[size_t(i0 + g) * a.ld + rw.kk[r]];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const double hg = h[i0 + g];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            zr[r] -= hg * double(v[g][r]);
+    }
+}
+
 // partials of v_i . z for i < nv, z = v_zslot.  in_cycle selects which
 // flags make the launch a no-op.
 template <class T>
@@ -117,29 +198,13 @@ __global__ __launch_bounds__(kBlock) void k_gm_dots(GmArgs<T> a, int nv, int zsl
     if (gm_idle(a.g, in_cycle != 0))
         return;
     __shared__ double lds[(kGmMaxRestart + 1) * (kBlock / kWave)];
-    const RowRange rr = block_rows(a.n, gridDim.x, 1);
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const GmRows rw(block_rows(a.n, gridDim.x, 1), a.n);
     const T* z = a.V + size_t(zslot) * a.ld;
-    double zr[kGmRowsPerThread];
+    double zr[R];
 #pragma unroll
-    for (int r = 0; r < kGmRowsPerThread; ++r) {
-        const int k = rr.beg + tid + r * kBlock;
-        zr[r] = k < rr.end ? double(z[k]) : 0.0;
-    }
-    for (int i = 0; i < nv; ++i) {
-        const T* v = a.V + size_t(i) * a.ld;
-        double s = 0.0;
-#pragma unroll
-        for (int r = 0; r < kGmRowsPerThread; ++r) {
-            const int k = rr.beg + tid + r * kBlock;
-            if (k < rr.end)
-                s += double(v[k]) * zr[r];
-        }
-        s = wave_sum(s);
-        if (lane == 0)
-            lds[i * (kBlock / kWave) + wid] = s;
-    }
-    gm_store_partials(lds, nv, a.partials);
+    for (int r = 0; r < R; ++r)
+        zr[r] = rw.in[r] ? double(z[rw.kk[r]]) : 0.0;
+    gm_dots(a, nv, rw, zr, lds);
 }
 
 // z = v_zslot; z -= sum_i h_i v_i (i < nv), stored back; then either the
@@ -151,59 +216,39 @@ __global__ __launch_bounds__(kBlock) void k_gm_orth(GmArgs<T> a, int nv, int zsl
     if (gm_idle(a.g, true))
         return;
     __shared__ double lds[(kGmMaxRestart + 1) * (kBlock / kWave)];
-    const RowRange rr = block_rows(a.n, gridDim.x, 1);
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const GmRows rw(block_rows(a.n, gridDim.x, 1), a.n);
     T* z = a.V + size_t(zslot) * a.ld;
-    double zr[kGmRowsPerThread];
+    double zr[R];
 #pragma unroll
-    for (int r = 0; r < kGmRowsPerThread; ++r) {
-        const int k = rr.beg + tid + r * kBlock;
-        zr[r] = k < rr.end ? double(z[k]) : 0.0;
-    }
-    for (int i = 0; i < nv; ++i) {
-        const T* v = a.V + size_t(i) * a.ld;
-        const double hi = h[i];
+    for (int r = 0; r < R; ++r)
+        zr[r] = rw.in[r] ? double(z[rw.kk[r]]) : 0.0;
+    int i = 0;
+    for (; i + 4 <= nv; i += 4)
+        gm_update_group<T, 4>(a, i, h, rw, zr);
+    for (; i < nv; ++i)
+        gm_update_group<T, 1>(a, i, h, rw, zr);
 #pragma unroll
-        for (int r = 0; r < kGmRowsPerThread; ++r) {
-            const int k = rr.beg + tid + r * kBlock;
-            if (k < rr.end)
-                zr[r] -= hi * double(v[k]);
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < kGmRowsPerThread; ++r) {
-        const int k = rr.beg + tid + r * kBlock;
-        if (k < rr.end) {
+    for (int r = 0; r < R; ++r) {
+        if (rw.in[r]) {
             const T zt = T(zr[r]);
-            z[k] = zt;
+            z[rw.kk[r]] = zt;
             zr[r] = double(zt);
+        } else {
+            zr[r] = 0.0; // the update ran on the clamped row: drop it
         }
     }
     if (norm) {
         double s = 0.0;
 #pragma unroll
-        for (int r = 0; r < kGmRowsPerThread; ++r)
+        for (int r = 0; r < R; ++r)
             s += zr[r] * zr[r];
         s = wave_sum(s);
-        if (lane == 0)
-            lds[wid] = s;
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            lds[threadIdx.x / kWave] = s;
         gm_store_partials(lds, 1, a.partials);
         return;
     }
-    for (int i = 0; i < nv; ++i) {
-        const T* v = a.V + size_t(i) * a.ld;
-        double s = 0.0;
-#pragma unroll
-        for (int r = 0; r < kGmRowsPerThread; ++r) {
-            const int k = rr.beg + tid + r * kBlock;
-            if (k < rr.end)
-                s += double(v[k]) * zr[r];
-        }
-        s = wave_sum(s);
-        if (lane == 0)
-            lds[i * (kBlock / kWave) + wid] = s;
-    }
-    gm_store_partials(lds, nv, a.partials);
+    gm_dots(a, nv, rw, zr, lds);
 }
 
 // out[i] = fold of partials[i * G ...], one block per value

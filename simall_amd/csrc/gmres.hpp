@@ -18,7 +18,7 @@ namespace mi
 constexpr int kGmMaxRestart = 256;
 // Rows per thread of the basis kernels: one block covers at most
 // kGmRowsPerThread * kBlock rows, which fixes the smallest grid.
-constexpr int kGmRowsPerThread = 8;
+constexpr int kGmRowsPerThread = 4;
 
 struct GmState
 {

@@ -10,8 +10,13 @@
 // and the upper backward sweep.  A row of L depends only on rows in earlier
 // dependency levels, so each level is one launch with one thread per row;
 // within a row the products are accumulated in column order with contraction
-// off, which reproduces the serial sweeps' arithmetic exactly.
+// off, which reproduces the serial sweeps' arithmetic exactly.  Orderings
+// whose rows form a few long runs of mutually independent rows (multicolour)
+// sweep run by run instead, each run one CSR-stream launch over the split
+// strictly lower or upper factor; there a row's products are summed first and
+// then subtracted, a different rounding of the same sweep.
 #include "ilu0.hpp"
+#include "stream.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -59,6 +64,83 @@ __global__ __launch_bounds__(kBlock) void k_upper_level(int cnt, const int* __re
         s -= lu[k] * z[ci[k]];
     const T d = lu[kd];
     z[i] = (d != T(0)) ? s / d : s;
+}
+
+// y_i = r_i - sum_{j < i} l_ij y_j over the rows of one run
+template <class T> struct EpiIluLower
+{
+    const T* r;
+    T* y;
+    T ri = T(0);
+    __device__ __forceinline__ void pre(int i) { ri = r[i]; }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = ri - T(s); }
+};
+
+// z_i = (y_i - sum_{j > i} u_ij z_j) / u_ii over the rows of one run
+template <class T> struct EpiIluUpper
+{
+    const T* y;
+    const T* d;
+    T* z;
+    T yi = T(0), di = T(1);
+    __device__ __forceinline__ void pre(int i)
+    {
+        yi = y[i];
+        di = d[i];
+    }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        const T t = yi - T(s);
+        z[i] = (di != T(0)) ? t / di : t;
+    }
+};
+
+// z_i = (r_i - sum_j m_ij x_j) / u_ii: a row whose other triangle is empty
+// (y_i = r_i - sum l_ij y_j with no U entries, or y_i = r_i with no L
+// entries), in the operations of the two sweeps
+template <class T> struct EpiIluRow
+{
+    const T* r;
+    const T* d;
+    T* z;
+    T ri = T(0), di = T(1);
+    __device__ __forceinline__ void pre(int i)
+    {
+        ri = r[i];
+        di = d[i];
+    }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        const T t = ri - T(s);
+        z[i] = (di != T(0)) ? t / di : t;
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ilu_two_run(StreamMat<T> M, const T* x,
+                                                        const T* __restrict__ r,
+                                                        const T* __restrict__ d, T* z)
+{
+    EpiIluRow<T> e{r, d, z};
+    stream_rows(M, x, e);
+}
+
+// The rows written by a run are never read by the same launch (a run has no
+// dependency inside it), so y may be both the gathered vector and the output.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ilu_lower_run(StreamMat<T> L, const T* __restrict__ r,
+                                                          T* y)
+{
+    EpiIluLower<T> e{r, y};
+    stream_rows(L, y, e);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ilu_upper_run(StreamMat<T> U, const T* __restrict__ y,
+                                                          const T* __restrict__ d, T* z)
+{
+    EpiIluUpper<T> e{y, d, z};
+    stream_rows(U, z, e);
 }
 
 #pragma clang fp contract(on)
@@ -149,6 +231,76 @@ void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& le
     bucket(n > 0 ? nu : 0, urows, uoff);
 }
 
+void ilu0_segments(int n, const int* rp, const int* ci, const std::vector<int>& lend,
+                   const std::vector<int>& diag, std::vector<int>& lbeg, std::vector<int>& ubeg)
+{
+    lbeg.assign(1, 0);
+    int start = 0;
+    for (int i = 0; i < n; ++i) {
+        bool dep = false;
+        for (int k = rp[i]; k < lend[size_t(i)] && !dep; ++k)
+            dep = ci[k] >= start;
+        if (dep) {
+            lbeg.push_back(i);
+            start = i;
+        }
+    }
+    lbeg.push_back(n);
+    ubeg.assign(1, n);
+    int end = n;
+    for (int i = n - 1; i >= 0; --i) {
+        bool dep = false;
+        for (int k = diag[size_t(i)] + 1; k < rp[i + 1] && !dep; ++k)
+            dep = ci[k] < end;
+        if (dep) {
+            ubeg.push_back(i + 1);
+            end = i + 1;
+        }
+    }
+    ubeg.push_back(0);
+}
+
+namespace
+{
+// Split pattern of one triangle: rows i keep the entries k in [kb(i), ke(i)).
+template <class KB, class KE>
+void split_triangle(int n, const int* rp, const int* ci, KB kb, KE ke, std::vector<int>& trp,
+                    std::vector<int>& tci, std::vector<int>& tmap)
+{
+    trp.assign(size_t(n) + 1, 0);
+    tci.clear();
+    tmap.clear();
+    for (int i = 0; i < n; ++i) {
+        for (int k = kb(i); k < ke(i); ++k) {
+            tci.push_back(ci[k]);
+            tmap.push_back(k);
+        }
+        trp[size_t(i) + 1] = int(tci.size());
+    }
+    (void)rp;
+}
+
+// Row blocks of the runs [b[t], b[t+1]) (ascending) or [b[t+1], b[t])
+// (descending): per run, its block boundaries appended to blk and the pair
+// (offset, block count) to sb.
+void run_blocks(const std::vector<int>& bounds, bool descending, const std::vector<int>& trp,
+                std::vector<int>& blk, std::vector<int>& sb)
+{
+    blk.clear();
+    sb.clear();
+    std::vector<int> b;
+    for (size_t t = 0; t + 1 < bounds.size(); ++t) {
+        const int r0 = descending ? bounds[t + 1] : bounds[t];
+        const int r1 = descending ? bounds[t] : bounds[t + 1];
+        build_row_blocks(r1 - r0, trp.data() + r0, b);
+        sb.push_back(int(blk.size()));
+        sb.push_back(int(b.size()) - 1);
+        for (int v : b)
+            blk.push_back(v + r0);
+    }
+}
+} // namespace
+
 template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s)
 {
     graphs.clear();
@@ -176,9 +328,51 @@ template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double
         MI_TRY(d_diag.upload(diag.data(), diag.size(), s));
         h_loff = loff;
         h_uoff = uoff;
+        // run sweeps when every run holds many rows (each launch has the
+        // work of a large SpMV); otherwise the dependency levels
+        std::vector<int> lbeg, ubeg;
+        ilu0_segments(n, rp, ci, lend, diag, lbeg, ubeg);
+        const int runs = int(std::max(lbeg.size(), ubeg.size())) - 1;
+        seg = n > 0 && int64_t(n) >= int64_t(2048) * runs;
+        two = false;
+        if (seg) {
+            std::vector<int> trp, tci, blk;
+            split_triangle(n, rp, ci, [&](int i) { return rp[i]; },
+                           [&](int i) { return lend[size_t(i)]; }, trp, tci, Lmap);
+            run_blocks(lbeg, false, trp, blk, Lsb);
+            const std::vector<int> lrp_host = trp;
+            auto lrp_at = [&](int i) { return lrp_host[size_t(i)]; };
+            MI_TRY(Lrp.upload(trp.data(), trp.size(), s));
+            MI_TRY(Lci.upload(tci.data(), tci.size(), s));
+            MI_TRY(Lblk.upload(blk.data(), blk.size(), s));
+            MI_HIP(hipStreamSynchronize(s));
+            split_triangle(n, rp, ci, [&](int i) { return diag[size_t(i)] + 1; },
+                           [&](int i) { return rp[i + 1]; }, trp, tci, Umap);
+            run_blocks(ubeg, true, trp, blk, Usb);
+            // two runs [0, b) and [b, n) where the first has no L entries and
+            // the second no U entries (a red-black ordering): one launch per run
+            two = lbeg.size() == 3 && ubeg.size() == 3 && lbeg[1] == ubeg[1] &&
+                  lrp_at(lbeg[1]) == 0 && trp[size_t(n)] == trp[size_t(ubeg[1])];
+            MI_TRY(Urp.upload(trp.data(), trp.size(), s));
+            MI_TRY(Uci.upload(tci.data(), tci.size(), s));
+            MI_TRY(Ublk.upload(blk.data(), blk.size(), s));
+        }
         MI_HIP(hipStreamSynchronize(s));
         level_built = true;
         levels_for = &P;
+    }
+    if (seg) {
+        std::vector<T> lv(Lmap.size()), uv(Umap.size()), dv(static_cast<size_t>(n));
+        for (size_t j = 0; j < Lmap.size(); ++j)
+            lv[j] = T(lu[size_t(Lmap[j])]);
+        for (size_t j = 0; j < Umap.size(); ++j)
+            uv[j] = T(lu[size_t(Umap[j])]);
+        for (int i = 0; i < n; ++i)
+            dv[size_t(i)] = T(lu[size_t(diag[size_t(i)])]);
+        MI_TRY(Lval.upload(lv.data(), lv.size(), s));
+        MI_TRY(Uval.upload(uv.data(), uv.size(), s));
+        MI_TRY(Ud.upload(dv.data(), dv.size(), s));
+        MI_HIP(hipStreamSynchronize(s));
     }
     std::vector<T> tv(lu.begin(), lu.end());
     MI_TRY(d_lu.upload(tv.data(), tv.size(), s));
@@ -195,6 +389,38 @@ template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_
     // when asked (the level count is in the hundreds for 3-D meshes)
     if (use_graph)
         return graphs.run(r, z, s, [&]() { return apply(r, z, s, false); });
+    if (seg && two) {
+        // rows [b, n): z = (r - L r) / d ; rows [0, b): z = (r - U z) / d
+        const int nl = Lsb[3], nu = Usb[3];
+        if (nl > 0) {
+            StreamMat<T> L{n, nl, Lblk.p + Lsb[2], Lrp.p, Lci.p, Lval.p};
+            k_ilu_two_run<T><<<stream_grid(nl), kBlock, 0, s>>>(L, r, r, Ud.p, z);
+        }
+        if (nu > 0) {
+            StreamMat<T> U{n, nu, Ublk.p + Usb[2], Urp.p, Uci.p, Uval.p};
+            k_ilu_two_run<T><<<stream_grid(nu), kBlock, 0, s>>>(U, z, r, Ud.p, z);
+        }
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
+    if (seg) {
+        for (size_t t = 0; t + 1 < Lsb.size(); t += 2) {
+            const int nrb = Lsb[t + 1];
+            if (nrb == 0)
+                continue;
+            StreamMat<T> L{n, nrb, Lblk.p + Lsb[t], Lrp.p, Lci.p, Lval.p};
+            k_ilu_lower_run<T><<<stream_grid(nrb), kBlock, 0, s>>>(L, r, d_y.p);
+        }
+        for (size_t t = 0; t + 1 < Usb.size(); t += 2) {
+            const int nrb = Usb[t + 1];
+            if (nrb == 0)
+                continue;
+            StreamMat<T> U{n, nrb, Ublk.p + Usb[t], Urp.p, Uci.p, Uval.p};
+            k_ilu_upper_run<T><<<stream_grid(nrb), kBlock, 0, s>>>(U, d_y.p, Ud.p, z);
+        }
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
     for (size_t l = 0; l + 1 < h_loff.size(); ++l) {
         const int beg = h_loff[l], cnt = h_loff[l + 1] - beg;
         k_lower_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(cnt, d_lrows.p + beg, rp_dev,
@@ -247,11 +473,7 @@ struct IluPrecond final : Precond
                      : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
     }
     const char* name() const override { return "ilu0"; }
-    int levels() const override
-    {
-        const auto& o = has64 ? f64.h_loff : f32.h_loff;
-        return o.empty() ? 0 : int(o.size()) - 1;
-    }
+    int levels() const override { return has64 ? f64.sweep_launches() : f32.sweep_launches(); }
 };
 } // namespace
 

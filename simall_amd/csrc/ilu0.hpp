@@ -14,6 +14,13 @@ namespace mi
 int ilu0_factor_host(int n, const int* rp, const int* ci, std::vector<double>& lu,
                      std::vector<int>& diag, int* zero_pivots);
 
+// Runs of consecutive rows with no dependency inside a run: for the L sweep,
+// row i starts a new run when one of its lower entries lies in the current
+// run; for the U sweep the same scanning from the last row.  lbeg / ubeg
+// hold the run boundaries in sweep order (ubeg descending).
+void ilu0_segments(int n, const int* rp, const int* ci, const std::vector<int>& lend,
+                   const std::vector<int>& diag, std::vector<int>& lbeg, std::vector<int>& ubeg);
+
 // Dependency levels of the L and U sweeps: rows of level l are
 // rows[off[l] .. off[l+1]).  lend[i] = first position of row i with col >= i.
 void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& lend,
@@ -32,8 +39,27 @@ template <class T> struct Ilu0
     bool level_built = false;
     const Pattern* levels_for = nullptr;
     GraphCache graphs; // captured applies (one level kernel per launch)
+
+    // Segment sweeps: when the rows fall into a few long runs of consecutive
+    // rows that do not depend on each other (a multicolour ordering: one run
+    // per colour), each run is one CSR-stream launch over the split strictly
+    // lower (L) or strictly upper (U) factor, with coalesced factor loads.
+    bool seg = false;
+    bool two = false; // two runs, the first without L and the second without U entries
+    DevBuf<int> Lrp, Lci, Lblk, Urp, Uci, Ublk;
+    DevBuf<T> Lval, Uval, Ud;
+    std::vector<int> Lsb, Usb; // per run in sweep order: offset into L/Ublk, row blocks
+    std::vector<int> Lmap, Umap; // position in the LU factor of each L / U entry
+
     int setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s);
     int apply(const T* r, T* z, hipStream_t s, bool use_graph = false);
+    // launches per sweep: runs (segment sweeps) or dependency levels
+    int sweep_launches() const
+    {
+        if (seg)
+            return two ? 1 : int(Lsb.size() / 2);
+        return h_loff.empty() ? 0 : int(h_loff.size()) - 1;
+    }
 };
 
 std::unique_ptr<Precond> make_ilu0();
