@@ -369,6 +369,67 @@ MI_API int mi_face_flux(mi_mesh* m, double rho, const double* Ux_device, const d
                         const double* Uz_device, const double* bc_value_host, double* phi_device);
 MI_API int mi_mesh_destroy(mi_mesh* m);
 
+/* ----------------------------------------------------- gradient, correction
+ * LeastSquaresGradient::evaluate (src/solver/Gradient.cpp:93-136) of a cell
+ * field: device phi[n_cells] -> gx, gy, gz [n_cells].  The inverted moment
+ * matrices (Gradient.cpp:42-91, built from the mesh geometry) are made on
+ * first use and kept with the mesh.  Bit-exact to the reference. */
+MI_API int mi_ls_gradient(mi_mesh* m, const double* phi_device, double* gx_device,
+                          double* gy_device, double* gz_device);
+/* SimpleAlgorithm::correct_fields (src/solver/SimpleAlgorithm.cpp:450-489):
+ * p += urfP p'; U -= V / max(aP, 1e-30) grad(p') with the least-squares
+ * gradient; on interior faces phi += rho |S|^2 / max(aP_f |d.S|, 1e-30) *
+ * (p'_owner - p'_neighbour), aP_f = (aP_owner + aP_neighbour) / 2.  Device
+ * arrays of n_cells (phi: n_faces, nullable to skip the flux correction). */
+MI_API int mi_correct_fields(mi_mesh* m, double urfP, double rho, const double* aP_device,
+                             const double* pprime_device, double* p_device, double* Ux_device,
+                             double* Uy_device, double* Uz_device, double* phi_device);
+
+/* ------------------------------------------------------- SIMPLE / PISO step
+ * A device-resident SimpleAlgorithm (src/solver/SimpleAlgorithm.cpp:521-575)
+ * or PisoAlgorithm (src/solver/PisoAlgorithm.cpp:27-70) on a mi_mesh: U, p,
+ * the face flux phi, the momentum diagonal aP and both matrices stay in HBM
+ * between iterations.  One iteration is: face fluxes; per velocity component
+ * momentum assembly (least-squares pressure-gradient source, time term,
+ * under-relaxation), solve and residual ||A U - b||; face fluxes; pressure
+ * assembly, solve from p' = 0, ||rhs|| and correct_fields (PISO: n_correctors
+ * times (n_nonorth + 1) pressure passes after the predictor).  Solves start
+ * from x = 0 as the reference's CPU solvers do.  Fields start at zero; write
+ * initial values through the pointers of mi_flow_fields. */
+enum { MI_FLOW_SIMPLE = 0, MI_FLOW_PISO = 1 };
+enum { MI_TIME_EULER = 0, MI_TIME_BDF2 = 1 }; /* TemporalScheme ImplicitEuler, BDF2 */
+typedef struct {
+    int algorithm;       /* MI_FLOW_SIMPLE or MI_FLOW_PISO */
+    double rho, mu;      /* density, dynamic viscosity (1, 1e-3) */
+    double urfU, urfP;   /* under-relaxation: SIMPLE 0.7, 0.3; PISO 1, 1 */
+    double dt;           /* > 0: transient term; SIMPLE 0 (steady), PISO 1e-3 */
+    int time_scheme;     /* MI_TIME_EULER (SIMPLE) or MI_TIME_BDF2 (PISO); BDF2 from
+                            the second time step on, implicit Euler before */
+    int n_correctors;    /* PISO pressure correctors (2) */
+    int n_nonorth;       /* PISO extra pressure passes per corrector (0) */
+    mi_solver_cfg momentum; /* momentum solves: GMRES(30) + ILU(0), 1e-8 (LinearSolverConfig) */
+    mi_solver_cfg pressure; /* pressure-correction solves: CG + SA-AMG, 1e-8 */
+} mi_flow_cfg;
+/* SimpleResiduals (src/solver/SimpleAlgorithm.hpp:59-63). */
+typedef struct {
+    double mom[3];
+    double cont;
+} mi_flow_residuals;
+typedef struct mi_flow mi_flow;
+/* Defaults of SimpleOptions / PisoOptions for the algorithm. */
+MI_API void mi_flow_cfg_default(int algorithm, mi_flow_cfg* cfg);
+/* bc_value_host[n_zones * 3]: zone velocities (inlets), nullable. */
+MI_API int mi_flow_create(mi_mesh* m, const double* bc_value_host, const mi_flow_cfg* cfg,
+                          mi_flow** out);
+/* Device pointers of the resident fields (each output nullable). */
+MI_API int mi_flow_fields(mi_flow* f, double** Ux, double** Uy, double** Uz, double** p,
+                          double** phi);
+/* One outer iteration: SimpleAlgorithm::iterate / PisoAlgorithm::iterate. */
+MI_API int mi_flow_iterate(mi_flow* f, mi_flow_residuals* res);
+/* SimpleAlgorithm::advance_time_step: roll U^n (and U^{n-1}), n_inner iterations. */
+MI_API int mi_flow_advance_time_step(mi_flow* f, int n_inner, mi_flow_residuals* res);
+MI_API int mi_flow_destroy(mi_flow* f);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,12 @@ How each is pinned (tests/test_oracle.py):
   face_gradient      central finite differences of J = sum w x^2 through assemble_pressure and a
                      dense solve (pinned and pressure-outlet cases); closed form
                      -G (lam_o - lam_n)(x_o - x_n) on unpinned faces (tests/test_adjoint_oracle.py)
+  ls_gradient        exact gradient of a linear field on cells without boundary faces;
+                     bit-exact vs LeastSquaresGradient compiled from the reference
+  correct_fields     constant p' leaves U and phi unchanged; bit-exact vs correct_fields
+                     compiled from the reference
+  Flow               SIMPLE / PISO iterations vs SimpleAlgorithm / PisoAlgorithm compiled from
+                     the reference with a dense direct solve as their linear solver
 """
 from __future__ import annotations
 
@@ -85,8 +91,10 @@ def load():
             "orc_gmres_transpose": (i, [P(OrcCsr), vp, vp, i, i, d, P(i), P(d)]),
             "orc_build_sparsity": (i64, [P(OrcMesh), vp, vp]),
             "orc_assemble_pressure": (i, [P(OrcMesh), vp, i, d, vp, vp, vp, P(OrcCsr), vp, vp]),
-            "orc_assemble_momentum": (None, [P(OrcMesh), vp, i, vp, i, d, d, vp, vp, vp, vp, d,
-                                             P(OrcCsr), vp, vp]),
+            "orc_assemble_momentum": (None, [P(OrcMesh), vp, i, vp, i, d, d, vp, vp, vp, vp, vp,
+                                             vp, d, P(OrcCsr), vp, vp]),
+            "orc_ls_gradient": (None, [P(OrcMesh), vp, vp, vp, vp]),
+            "orc_correct_fields": (None, [P(OrcMesh), d, d, vp, vp, vp, vp, vp, vp, vp]),
             "orc_face_flux": (None, [P(OrcMesh), vp, i, vp, d, vp, vp, vp, vp]),
             "orc_face_gradient": (None, [P(OrcMesh), vp, i, d, vp, i, vp, vp, vp]),
             "orc_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
@@ -126,6 +134,13 @@ class Matrix:
                     va.append(A[i, j])
             rp.append(len(ci))
         return cls(rp, ci, va)
+
+    def diag_positions(self):
+        """Position of a_ii in each row (every row must hold it)."""
+        rows = np.repeat(np.arange(self.n), np.diff(self.rowPtr))
+        pos = np.flatnonzero(self.colIdx == rows)
+        assert pos.size == self.n, "a row without its diagonal"
+        return pos
 
     def dense(self):
         D = np.zeros((self.n, self.n))
@@ -295,15 +310,121 @@ def face_gradient(mesh, bc_of_zone, x, lam, pinned, rho=1.0, aP=None):
 
 
 def assemble_momentum(mesh, pattern, bc_of_zone, comp, mu, rho, phi, dt=None, Uold=None,
-                      Ucomp=None, urf=1.0, bc_value=None):
+                      Ucomp=None, urf=1.0, bc_value=None, pgradV=None, Unm1=None):
     ms = mesh_struct(mesh)
     bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
     vals = np.empty(pattern.colIdx.size)
     rhs = np.empty(mesh.n_cells)
     bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+    opt = [None if a is None else _vec(a) for a in (pgradV, dt, Uold, Unm1, Ucomp)]
     load().orc_assemble_momentum(C.byref(ms), _p(bc), bc.size, _p(bv), comp, mu, rho, _p(_vec(phi)),
-                                 _p(dt), _p(Uold), _p(Ucomp), urf, C.byref(pattern.s), _p(vals), _p(rhs))
+                                 *(_p(a) for a in opt), urf, C.byref(pattern.s), _p(vals), _p(rhs))
     return vals, rhs
+
+
+def ls_gradient(mesh, phi):
+    """LeastSquaresGradient::evaluate (Gradient.cpp:93-136) -> (gx, gy, gz)."""
+    ms = mesh_struct(mesh)
+    g = np.empty((3, mesh.n_cells))
+    load().orc_ls_gradient(C.byref(ms), _p(_vec(phi)), _p(g[0]), _p(g[1]), _p(g[2]))
+    return g[0].copy(), g[1].copy(), g[2].copy()
+
+
+def correct_fields(mesh, urfP, rho, aP, pp, p, U, phi):
+    """SimpleAlgorithm::correct_fields (SimpleAlgorithm.cpp:450-489) on copies
+    -> (p, (Ux, Uy, Uz), phi)."""
+    ms = mesh_struct(mesh)
+    p = np.array(p, dtype=np.float64)
+    U = [np.array(u, dtype=np.float64) for u in U]
+    phi = None if phi is None else np.array(phi, dtype=np.float64)
+    load().orc_correct_fields(C.byref(ms), urfP, rho, _p(_vec(aP)), _p(_vec(pp)), _p(p), _p(U[0]),
+                              _p(U[1]), _p(U[2]), _p(phi))
+    return p, tuple(U), phi
+
+
+class Flow:
+    """SimpleAlgorithm / PisoAlgorithm state and outer iteration
+    (SimpleAlgorithm.cpp:49-98, 491-575; PisoAlgorithm.cpp:27-70) over the
+    oracle's assembly, gradient and correction, with exact sparse solves in
+    place of the linear solvers."""
+
+    def __init__(self, mesh, bc_of_zone, bc_value=None, algorithm="simple", rho=1.0, mu=1e-3,
+                 urfU=None, urfP=None, dt=None, bdf2=None, n_correctors=2, n_nonorth=0):
+        piso = algorithm == "piso"
+        self.mesh, self.bc, self.bc_value = mesh, list(bc_of_zone), bc_value
+        self.piso, self.rho, self.mu = piso, rho, mu
+        self.urfU = (1.0 if piso else 0.7) if urfU is None else urfU
+        self.urfP = (1.0 if piso else 0.3) if urfP is None else urfP
+        self.dt = (1e-3 if piso else 0.0) if dt is None else dt
+        self.bdf2 = piso if bdf2 is None else bdf2
+        self.n_correctors, self.n_nonorth = max(1, n_correctors), n_nonorth
+        n = mesh.n_cells
+        rp, ci = build_sparsity(mesh)
+        self.pattern = Matrix(rp, ci, np.zeros(ci.size))
+        self.U = [np.zeros(n) for _ in range(3)]
+        self.Un = [np.zeros(n) for _ in range(3)]
+        self.Unm1 = [np.zeros(n) for _ in range(3)]
+        self.p = np.zeros(n)
+        self.aP = np.zeros(n)
+        self.phi = np.zeros(mesh.n_faces)
+        self.step = 0
+
+    def _solve(self, vals, b):
+        import scipy.sparse as sp
+        import scipy.sparse.linalg as spl
+        P = self.pattern
+        A = sp.csr_matrix((vals, P.colIdx, P.rowPtr), shape=(P.n, P.n)).tocsc()
+        return spl.spsolve(A, b), A
+
+    def _momentum(self, res):
+        m = self.mesh
+        self.phi = face_flux(m, self.bc, self.rho, self.U, self.bc_value)
+        g = ls_gradient(m, self.p)
+        dtc = np.full(m.n_cells, self.dt) if self.dt > 0 else None
+        use_bdf2 = self.dt > 0 and self.bdf2 and self.step >= 1
+        for k in range(3):
+            vals, rhs = assemble_momentum(m, self.pattern, self.bc, k, self.mu, self.rho, self.phi,
+                                          dt=dtc, Uold=self.Un[k] if dtc is not None else None,
+                                          Ucomp=self.U[k], urf=self.urfU, bc_value=self.bc_value,
+                                          pgradV=g[k] * m.volume,
+                                          Unm1=self.Unm1[k] if use_bdf2 else None)
+            if k == 0:
+                self.aP = vals[self.pattern.diag_positions()].copy()
+            x, A = self._solve(vals, rhs)
+            self.U[k] = x
+            res["mom"][k] = float(np.linalg.norm(A @ x - rhs))
+
+    def _pressure(self, res):
+        m = self.mesh
+        vals, rhs, _ = assemble_pressure(m, self.pattern, self.bc, self.rho, aP=self.aP, phi=self.phi)
+        pp, _ = self._solve(vals, rhs)
+        res["cont"] = float(np.linalg.norm(rhs))
+        self.p, U, self.phi = correct_fields(m, self.urfP, self.rho, self.aP, pp, self.p, self.U,
+                                             self.phi)
+        self.U = list(U)
+
+    def iterate(self):
+        res = {"mom": [0.0, 0.0, 0.0], "cont": 0.0}
+        self._momentum(res)
+        if self.piso:
+            for _ in range(self.n_correctors):
+                self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value)
+                for _ in range(self.n_nonorth + 1):
+                    self._pressure(res)
+        else:
+            self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value)
+            self._pressure(res)
+        return res
+
+    def advance_time_step(self, n_inner=1):
+        if self.step >= 1:
+            self.Unm1 = [u.copy() for u in self.Un]
+        self.Un = [u.copy() for u in self.U]
+        res = None
+        for _ in range(max(1, n_inner)):
+            res = self.iterate()
+        self.step += 1
+        return res
 
 
 def face_flux(mesh, bc_of_zone, rho, U, bc_value=None):

@@ -861,9 +861,9 @@ void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, doubl
  * face loop :253-303, transient :320-337 (implicit Euler), URF :340-348. */
 void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,
                            const double* bcValue, int comp, double mu, double rho,
-                           const double* phi, const double* dtCell, const double* Uold,
-                           const double* Ucomp, double urfU, const orc_csr* pat,
-                           double* values, double* rhs)
+                           const double* phi, const double* pgradV, const double* dtCell,
+                           const double* Uold, const double* Unm1, const double* Ucomp,
+                           double urfU, const orc_csr* pat, double* values, double* rhs)
 {
     const int64_t nC = m->n_cells;
     const int64_t nnz = pat->rowPtr[nC];
@@ -909,14 +909,25 @@ void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,
             }
         }
     }
+    /* pressure-gradient source, :306-315 (pgradV = dp/dx_comp * V) */
+    if (pgradV)
+        for (int64_t c = 0; c < nC; ++c)
+            rhs[c] -= pgradV[c];
+    /* time term, :317-336: BDF2 when U^{n-1} is given, else implicit Euler */
     if (dtCell) {
         for (int64_t c = 0; c < nC; ++c) {
             const double Vrho_dt = rho * m->volume[c] / dtCell[c];
             const int kd = find_col(pat, (int)c, (int)c);
             if (kd < 0)
                 continue;
-            values[kd] += Vrho_dt;
-            rhs[c] += (Uold ? Uold[c] : 0.0) * Vrho_dt;
+            const double un = Uold ? Uold[c] : 0.0;
+            if (Unm1) {
+                values[kd] += 1.5 * Vrho_dt;
+                rhs[c] += (2.0 * un - 0.5 * Unm1[c]) * Vrho_dt;
+            } else {
+                values[kd] += Vrho_dt;
+                rhs[c] += un * Vrho_dt;
+            }
         }
     }
     for (int64_t c = 0; c < nC; ++c) {
@@ -961,3 +972,123 @@ void orc_face_flux(const orc_mesh* m, const int* bcOfZone, int nZones,
         phi[f] = rho * (ux * m->areaX[f] + uy * m->areaY[f] + uz * m->areaZ[f]);
     }
 }
+
+/* LeastSquaresGradient (src/solver/Gradient.cpp:42-136): per cell the
+ * symmetric moment matrix sum w d d^T over its faces, w = 1/(|d|^2 + 1e-30),
+ * d the offset to the neighbour centroid (boundary faces: to the face
+ * centroid), inverted by its adjugate (zero when |det| < 1e-30); then
+ * grad = M^-1 sum w d (phi_n - phi_o) over interior faces (boundary faces
+ * contribute w d * 0).  Face loops in face order, as the reference. */
+static void ls_acc(double* a, double dx, double dy, double dz)
+{
+    const double w2 = 1.0 / (dx * dx + dy * dy + dz * dz + 1e-30);
+    a[0] += w2 * dx * dx;
+    a[1] += w2 * dx * dy;
+    a[2] += w2 * dx * dz;
+    a[3] += w2 * dy * dy;
+    a[4] += w2 * dy * dz;
+    a[5] += w2 * dz * dz;
+}
+
+void orc_ls_gradient(const orc_mesh* m, const double* phi, double* gx, double* gy, double* gz)
+{
+    const int64_t nC = m->n_cells;
+    double* A = (double*)calloc((size_t)(nC > 0 ? nC : 1) * 6, sizeof(double));
+    double* B = (double*)calloc((size_t)(nC > 0 ? nC : 1) * 3, sizeof(double));
+    for (int64_t f = 0; f < m->n_faces; ++f) {
+        const uint64_t o = m->owner[f], nb = m->neighbor[f];
+        if (nb == UINT64_MAX) {
+            ls_acc(A + o * 6, m->faceCx[f] - m->cellCx[o], m->faceCy[f] - m->cellCy[o],
+                   m->faceCz[f] - m->cellCz[o]);
+        } else {
+            const double dx = m->cellCx[nb] - m->cellCx[o];
+            const double dy = m->cellCy[nb] - m->cellCy[o];
+            const double dz = m->cellCz[nb] - m->cellCz[o];
+            ls_acc(A + o * 6, dx, dy, dz);
+            ls_acc(A + nb * 6, -dx, -dy, -dz);
+        }
+    }
+    for (int64_t f = 0; f < m->n_faces; ++f) {
+        const uint64_t o = m->owner[f], nb = m->neighbor[f];
+        double dx, dy, dz, dphi;
+        if (nb == UINT64_MAX) {
+            dx = m->faceCx[f] - m->cellCx[o];
+            dy = m->faceCy[f] - m->cellCy[o];
+            dz = m->faceCz[f] - m->cellCz[o];
+            dphi = 0.0;
+        } else {
+            dx = m->cellCx[nb] - m->cellCx[o];
+            dy = m->cellCy[nb] - m->cellCy[o];
+            dz = m->cellCz[nb] - m->cellCz[o];
+            dphi = phi[nb] - phi[o];
+        }
+        const double w2 = 1.0 / (dx * dx + dy * dy + dz * dz + 1e-30);
+        B[o * 3 + 0] += w2 * dx * dphi;
+        B[o * 3 + 1] += w2 * dy * dphi;
+        B[o * 3 + 2] += w2 * dz * dphi;
+        if (nb != UINT64_MAX) {
+            B[nb * 3 + 0] += w2 * dx * -dphi;
+            B[nb * 3 + 1] += w2 * dy * -dphi;
+            B[nb * 3 + 2] += w2 * dz * -dphi;
+        }
+    }
+    for (int64_t c = 0; c < nC; ++c) {
+        const double* a = A + c * 6;
+        const double xx = a[0], xy = a[1], xz = a[2], yy = a[3], yz = a[4], zz = a[5];
+        const double det =
+            xx * (yy * zz - yz * yz) - xy * (xy * zz - yz * xz) + xz * (xy * yz - yy * xz);
+        double mv[6] = {0, 0, 0, 0, 0, 0};
+        if (fabs(det) >= 1e-30) {
+            const double inv = 1.0 / det;
+            mv[0] = (yy * zz - yz * yz) * inv;
+            mv[1] = -(xy * zz - yz * xz) * inv;
+            mv[2] = (xy * yz - yy * xz) * inv;
+            mv[3] = (xx * zz - xz * xz) * inv;
+            mv[4] = -(xx * yz - xy * xz) * inv;
+            mv[5] = (xx * yy - xy * xy) * inv;
+        }
+        const double bx = B[c * 3], by = B[c * 3 + 1], bz = B[c * 3 + 2];
+        gx[c] = mv[0] * bx + mv[1] * by + mv[2] * bz;
+        gy[c] = mv[1] * bx + mv[3] * by + mv[4] * bz;
+        gz[c] = mv[2] * bx + mv[4] * by + mv[5] * bz;
+    }
+    free(A);
+    free(B);
+}
+
+/* SimpleAlgorithm::correct_fields, src/solver/SimpleAlgorithm.cpp:450-489 */
+void orc_correct_fields(const orc_mesh* m, double urfP, double rho, const double* aP,
+                        const double* pp, double* p, double* Ux, double* Uy, double* Uz,
+                        double* phi)
+{
+    const int64_t nC = m->n_cells;
+    for (int64_t c = 0; c < nC; ++c)
+        p[c] += urfP * pp[c];
+    double* g = (double*)malloc(sizeof(double) * (size_t)(nC > 0 ? nC : 1) * 3);
+    orc_ls_gradient(m, pp, g, g + nC, g + 2 * nC);
+    for (int64_t c = 0; c < nC; ++c) {
+        const double w = m->volume[c] / (aP[c] > 1e-30 ? aP[c] : 1e-30);
+        Ux[c] -= w * g[c];
+        Uy[c] -= w * g[nC + c];
+        Uz[c] -= w * g[2 * nC + c];
+    }
+    free(g);
+    if (!phi)
+        return;
+    for (int64_t f = 0; f < m->n_faces; ++f) {
+        const uint64_t o = m->owner[f], nb = m->neighbor[f];
+        if (nb == UINT64_MAX)
+            continue;
+        const double Ax = m->areaX[f], Ay = m->areaY[f], Az = m->areaZ[f];
+        const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
+        const double dx = m->cellCx[nb] - m->cellCx[o];
+        const double dy = m->cellCy[nb] - m->cellCy[o];
+        const double dz = m->cellCz[nb] - m->cellCz[o];
+        const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        const double aPf = 0.5 * (aP[o] + aP[nb]);
+        const double den = aPf * ddotA;
+        const double coef = rho * Amag2 / (den > 1e-30 ? den : 1e-30);
+        phi[f] += coef * (pp[o] - pp[nb]);
+    }
+}
+

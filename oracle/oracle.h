@@ -149,16 +149,27 @@ void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, doubl
                        double* grad);
 
 /* Momentum assembly, first-order upwind + diffusion, one component
- * (src/solver/SimpleAlgorithm.cpp:240-355) without the pressure-gradient
- * source.  dtCell (nullable) adds rho*V/dt_c to the diagonal and
- * rho*V/dt_c*Uold_c to the rhs (implicit Euler); urfU then scales the
- * diagonal as Patankar (rhs += (a/urf - a)*Ucomp).  bcValue[zone*3+comp]
- * gives the fixed velocity of inlet zones. */
+ * (src/solver/SimpleAlgorithm.cpp:240-355).  pgradV (nullable) is the
+ * pressure-gradient source dp/dx_comp * V subtracted from the rhs.  dtCell
+ * (nullable) adds the time term: implicit Euler rho*V/dt_c on the diagonal and
+ * rho*V/dt_c*Uold_c on the rhs, or with Unm1 BDF2 (1.5 rho V/dt;
+ * (2 Uold - 0.5 Unm1) rho V/dt); urfU then scales the diagonal as Patankar
+ * (rhs += (a/urf - a)*Ucomp).  bcValue[zone*3+comp] gives the fixed velocity
+ * of inlet zones. */
 void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,
                            const double* bcValue, int comp, double mu, double rho,
-                           const double* phi, const double* dtCell, const double* Uold,
-                           const double* Ucomp, double urfU, const orc_csr* pattern,
-                           double* values, double* rhs);
+                           const double* phi, const double* pgradV, const double* dtCell,
+                           const double* Uold, const double* Unm1, const double* Ucomp,
+                           double urfU, const orc_csr* pattern, double* values, double* rhs);
+
+/* Least-squares cell gradient (src/solver/Gradient.cpp:42-136). */
+void orc_ls_gradient(const orc_mesh* m, const double* phi, double* gx, double* gy, double* gz);
+/* Field correction after the pressure solve (src/solver/SimpleAlgorithm.cpp:450-489):
+ * p += urfP p'; U -= V/max(aP,1e-30) grad p'; interior-face flux correction
+ * (phi nullable: skipped). */
+void orc_correct_fields(const orc_mesh* m, double urfP, double rho, const double* aP,
+                        const double* pp, double* p, double* Ux, double* Uy, double* Uz,
+                        double* phi);
 
 /* Face mass flux phi_f = rho * 0.5*(U_o+U_n) . S_f on interior faces
  * (src/solver/SimpleAlgorithm.cpp:201-237). */

@@ -7,7 +7,8 @@ CXX ?= g++
 OUT := _ref
 SRCS := $(REF)/src/meshing/CartesianMesher.cpp $(REF)/src/meshing/Connectivity.cpp \
         $(REF)/src/meshing/MeshStorage.cpp $(REF)/src/solver/SimpleAlgorithm.cpp \
-        $(REF)/src/solver/Gradient.cpp $(REF)/src/adjoint/DiscreteAdjoint.cpp \
+        $(REF)/src/solver/Gradient.cpp $(REF)/src/solver/PisoAlgorithm.cpp \
+        $(REF)/src/adjoint/DiscreteAdjoint.cpp \
         $(REF)/src/solver/AMGPreconditioner.cpp
 # -ffp-contract=off: x86-64 code without fused multiply-add, as the
 # reference's default build.

@@ -9,10 +9,15 @@
 // solver::SimpleAlgorithm assembly (src/solver/SimpleAlgorithm.cpp, through a
 // subclass that reaches its protected hooks), solver::CSRMatrix::spmv,
 // adjoint::solve_discrete_adjoint (src/adjoint/DiscreteAdjoint.cpp) and the
-// Ruge-Stueben AMGPreconditioner (src/solver/AMGPreconditioner.cpp).
+// Ruge-Stueben AMGPreconditioner (src/solver/AMGPreconditioner.cpp),
+// LeastSquaresGradient (src/solver/Gradient.cpp) and the SIMPLE / PISO outer
+// iterations (SimpleAlgorithm.cpp, PisoAlgorithm.cpp) with a dense direct
+// solve standing in for the linear solvers.
 #include "adjoint/DiscreteAdjoint.hpp"
 #include "meshing/CartesianMesher.hpp"
 #include "solver/AMGPreconditioner.hpp"
+#include "solver/Gradient.hpp"
+#include "solver/PisoAlgorithm.hpp"
 #include "solver/SimpleAlgorithm.hpp"
 
 #include <chrono>
@@ -102,9 +107,77 @@ public:
         assemble_momentum(comp);
     }
     void flux() { compute_face_fluxes(); }
+    void correct(const double* aP, const double* pp, const double* phi)
+    {
+        std::memcpy(aP_.data(), aP, aP_.size() * sizeof(double));
+        std::memcpy(pPrime_.data(), pp, pPrime_.size() * sizeof(double));
+        phiFlux_.assign(phi, phi + mesh_.faces().size());
+        correct_fields();
+    }
     const solver::CSRMatrix& Ap() const { return A_p_; }
     const solver::CSRMatrix& Am() const { return A_mom_; }
     const util::aligned_vector<double>& rhs() const { return rhs_; }
+    const util::aligned_vector<double>& phi() const { return phiFlux_; }
+};
+
+// A dense LU solve with partial pivoting as the ILinearSolver of the outer
+// iterations: their results then depend on the assembly and correction only.
+class DenseSolver final : public solver::ILinearSolver
+{
+public:
+    int solve(const solver::CSRMatrix& A, const util::aligned_vector<double>& b,
+              util::aligned_vector<double>& x) override
+    {
+        const size_t n = A.rows();
+        std::vector<double> M(n * n, 0.0);
+        for (size_t i = 0; i < n; ++i)
+            for (int k = A.rowPtr[i]; k < A.rowPtr[i + 1]; ++k)
+                M[i * n + size_t(A.colIdx[size_t(k)])] += A.values[size_t(k)];
+        std::vector<double> y(b.begin(), b.end());
+        for (size_t c = 0; c < n; ++c) {
+            size_t piv = c;
+            for (size_t r = c + 1; r < n; ++r)
+                if (std::fabs(M[r * n + c]) > std::fabs(M[piv * n + c]))
+                    piv = r;
+            if (piv != c) {
+                for (size_t k = 0; k < n; ++k)
+                    std::swap(M[c * n + k], M[piv * n + k]);
+                std::swap(y[c], y[piv]);
+            }
+            const double d = M[c * n + c];
+            for (size_t r = c + 1; r < n; ++r) {
+                const double f = M[r * n + c] / d;
+                if (f == 0.0)
+                    continue;
+                for (size_t k = c; k < n; ++k)
+                    M[r * n + k] -= f * M[c * n + k];
+                y[r] -= f * y[c];
+            }
+        }
+        x.assign(n, 0.0);
+        for (size_t i = n; i-- > 0;) {
+            double s = y[i];
+            for (size_t k = i + 1; k < n; ++k)
+                s -= M[i * n + k] * x[k];
+            x[i] = s / M[i * n + i];
+        }
+        return 1;
+    }
+    double last_residual() const override { return 0.0; }
+};
+
+// PisoAlgorithm with access to its face flux.
+class PisoProbe final : public solver::PisoAlgorithm
+{
+public:
+    using PisoAlgorithm::PisoAlgorithm;
+    const util::aligned_vector<double>& phi() const { return phiFlux_; }
+};
+
+class SimpleProbe final : public solver::SimpleAlgorithm
+{
+public:
+    using SimpleAlgorithm::SimpleAlgorithm;
     const util::aligned_vector<double>& phi() const { return phiFlux_; }
 };
 
@@ -358,6 +431,116 @@ int ref_rs_amg_pcg(int n, const int32_t* rp, const int32_t* ci, const double* v,
     }
     std::memcpy(x, X.data(), size_t(n) * sizeof(double));
     return it;
+}
+
+// LeastSquaresGradient::evaluate of phi on the reference mesh.
+int ref_ls_gradient(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                    uint64_t seed, const double* phi, double* gx, double* gy, double* gz)
+{
+    auto m = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
+    const size_t nC = m->cells().size();
+    solver::ScalarField f(phi, phi + nC);
+    solver::VectorField g;
+    solver::LeastSquaresGradient G(*m);
+    G.evaluate(f, g);
+    std::memcpy(gx, g.x.data(), nC * sizeof(double));
+    std::memcpy(gy, g.y.data(), nC * sizeof(double));
+    std::memcpy(gz, g.z.data(), nC * sizeof(double));
+    return 0;
+}
+
+// SimpleAlgorithm::correct_fields with the given aP, p', p, U and phi;
+// p, U (3 n) and phi are updated in place.
+int ref_correct_fields(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                       uint64_t seed, const int* bc_of_zone, int nz, double rho, double urfP,
+                       const double* aP, const double* pp, double* p, double* U, double* phi)
+{
+    auto c = make_case(Nx, Ny, Nz, o, L, perturb, seed, bc_of_zone, nz, nullptr, rho, 1e-3, 0.0,
+                       0.7, U);
+    const size_t nC = c->mesh->cells().size(), nF = c->mesh->faces().size();
+    auto* P = c->fields.find_scalar("p");
+    std::memcpy(P->data(), p, nC * sizeof(double));
+    c->probe->options().urfP = urfP;
+    c->probe->correct(aP, pp, phi);
+    auto* V = c->fields.find_vector("U");
+    std::memcpy(p, P->data(), nC * sizeof(double));
+    std::memcpy(U, V->x.data(), nC * sizeof(double));
+    std::memcpy(U + nC, V->y.data(), nC * sizeof(double));
+    std::memcpy(U + 2 * nC, V->z.data(), nC * sizeof(double));
+    std::memcpy(phi, c->probe->phi().data(), nF * sizeof(double));
+    return 0;
+}
+
+// n_steps calls of advance_time_step(n_inner) (n_steps = 0: one iterate())
+// of SimpleAlgorithm (algorithm 0) or PisoAlgorithm (1) with a dense direct
+// solve.  U (3 n) and p are the initial fields and receive the final ones;
+// phi the final face flux; res[4] the last SimpleResiduals.
+int ref_flow(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+             uint64_t seed, const int* bc_of_zone, int nz, const double* bc_value, int algorithm,
+             double rho, double mu, double urfU, double urfP, double dt, int bdf2,
+             int n_correctors, int n_nonorth, int n_steps, int n_inner, double* U, double* p,
+             double* phi, double* res)
+{
+    auto mesh = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
+    const size_t nC = mesh->cells().size(), nF = mesh->faces().size();
+    std::vector<solver::BoundarySpec> bcs;
+    for (int z = 1; z < nz; ++z) {
+        solver::BoundarySpec b;
+        b.zone = meshing::ZoneId(z);
+        b.type = bc_type(bc_of_zone[z]);
+        for (int d = 0; d < 3; ++d)
+            b.vectorValue[d] = bc_value ? bc_value[z * 3 + d] : 0.0;
+        bcs.push_back(b);
+    }
+    solver::FieldRegistry fields;
+    auto& Uf = fields.vector("U", nC);
+    auto& pf = fields.scalar("p", nC);
+    std::memcpy(Uf.x.data(), U, nC * sizeof(double));
+    std::memcpy(Uf.y.data(), U + nC, nC * sizeof(double));
+    std::memcpy(Uf.z.data(), U + 2 * nC, nC * sizeof(double));
+    std::memcpy(pf.data(), p, nC * sizeof(double));
+    DenseSolver sm, sp;
+    const auto scheme = bdf2 ? solver::TemporalScheme::BDF2 : solver::TemporalScheme::ImplicitEuler;
+    std::unique_ptr<solver::SimpleAlgorithm> alg;
+    const util::aligned_vector<double>* phiRef = nullptr;
+    if (algorithm == 1) {
+        solver::PisoOptions po;
+        po.nCorrectors = n_correctors;
+        po.nNonOrthCorr = n_nonorth;
+        po.rho = rho;
+        po.mu = mu;
+        po.dt = dt;
+        po.timeScheme = scheme;
+        auto a = std::make_unique<PisoProbe>(*mesh, fields, bcs, sm, sp, po);
+        phiRef = &a->phi();
+        alg = std::move(a);
+    } else {
+        solver::SimpleOptions so;
+        so.rho = rho;
+        so.mu = mu;
+        so.urfU = urfU;
+        so.urfP = urfP;
+        so.dt = dt;
+        so.timeScheme = scheme;
+        auto a = std::make_unique<SimpleProbe>(*mesh, fields, bcs, sm, sp, so);
+        phiRef = &a->phi();
+        alg = std::move(a);
+    }
+    solver::SimpleResiduals r{};
+    if (n_steps == 0)
+        r = alg->iterate();
+    for (int s = 0; s < n_steps; ++s)
+        r = alg->advance_time_step(n_inner);
+    auto& Uo = *fields.find_vector("U");
+    std::memcpy(U, Uo.x.data(), nC * sizeof(double));
+    std::memcpy(U + nC, Uo.y.data(), nC * sizeof(double));
+    std::memcpy(U + 2 * nC, Uo.z.data(), nC * sizeof(double));
+    std::memcpy(p, fields.find_scalar("p")->data(), nC * sizeof(double));
+    std::memcpy(phi, phiRef->data(), nF * sizeof(double));
+    for (int k = 0; k < 3; ++k)
+        res[k] = r.mom[k];
+    res[3] = r.cont;
+    return 0;
 }
 
 } // extern "C"

@@ -54,6 +54,33 @@ class SolverCfg(C.Structure):
     ]
 
 
+MI_FLOW_SIMPLE, MI_FLOW_PISO = 0, 1
+MI_TIME_EULER, MI_TIME_BDF2 = 0, 1
+
+
+class FlowCfg(C.Structure):
+    _fields_ = [
+        ("algorithm", C.c_int),
+        ("rho", C.c_double),
+        ("mu", C.c_double),
+        ("urfU", C.c_double),
+        ("urfP", C.c_double),
+        ("dt", C.c_double),
+        ("time_scheme", C.c_int),
+        ("n_correctors", C.c_int),
+        ("n_nonorth", C.c_int),
+        ("momentum", SolverCfg),
+        ("pressure", SolverCfg),
+    ]
+
+
+class FlowResiduals(C.Structure):
+    _fields_ = [("mom", C.c_double * 3), ("cont", C.c_double)]
+
+    def as_dict(self):
+        return {"mom": list(self.mom), "cont": self.cont}
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int),
@@ -143,6 +170,14 @@ PROTOTYPES = {
     "mi_batch_tuning": (_i, [_vp, C.POINTER(_i), C.POINTER(_d), C.POINTER(_d)]),
     "mi_batch_destroy": (_i, [_vp]),
     "mi_mesh_destroy": (_i, [_vp]),
+    "mi_ls_gradient": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "mi_correct_fields": (_i, [_vp, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "mi_flow_cfg_default": (None, [_i, C.POINTER(FlowCfg)]),
+    "mi_flow_create": (_i, [_vp, _vp, C.POINTER(FlowCfg), C.POINTER(_vp)]),
+    "mi_flow_fields": (_i, [_vp] + [C.POINTER(_vp)] * 5),
+    "mi_flow_iterate": (_i, [_vp, C.POINTER(FlowResiduals)]),
+    "mi_flow_advance_time_step": (_i, [_vp, _i, C.POINTER(FlowResiduals)]),
+    "mi_flow_destroy": (_i, [_vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
     "mi_amg_host_level_info": (_i, [_vp, _i, _i, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _pd]),
@@ -469,6 +504,15 @@ class DeviceMesh:
         bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
         check(self.lib.mi_face_flux(self.h, rho, d_Ux.ptr, d_Uy.ptr, d_Uz.ptr, ptr(bv), d_phi.ptr))
 
+    def ls_gradient(self, d_phi, d_gx, d_gy, d_gz):
+        """LeastSquaresGradient::evaluate of a cell field (device arrays)."""
+        check(self.lib.mi_ls_gradient(self.h, d_phi.ptr, d_gx.ptr, d_gy.ptr, d_gz.ptr))
+
+    def correct_fields(self, urfP, rho, d_aP, d_pp, d_p, d_U, d_phi=None):
+        """SimpleAlgorithm::correct_fields on device arrays (d_U: three arrays)."""
+        check(self.lib.mi_correct_fields(self.h, urfP, rho, d_aP.ptr, d_pp.ptr, d_p.ptr, d_U[0].ptr,
+                                         d_U[1].ptr, d_U[2].ptr, _dptr(d_phi)))
+
     def close(self):
         if getattr(self, "h", None):
             self.lib.mi_mesh_destroy(self.h)
@@ -601,3 +645,88 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def flow_cfg(algorithm=MI_FLOW_SIMPLE, momentum=None, pressure=None, **kw):
+    """mi_flow_cfg with the defaults of the algorithm; momentum / pressure are
+    dicts of SolverCfg fields."""
+    c = FlowCfg()
+    load().mi_flow_cfg_default(algorithm, C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise AttributeError(k)
+        setattr(c, k, v)
+    for sub, fields in (("momentum", momentum), ("pressure", pressure)):
+        for k, v in (fields or {}).items():
+            if not hasattr(getattr(c, sub), k):
+                raise AttributeError(f"{sub}.{k}")
+            setattr(getattr(c, sub), k, v)
+    return c
+
+
+class Flow:
+    """mi_flow: device-resident SIMPLE / PISO iteration on a DeviceMesh."""
+
+    def __init__(self, dmesh, cfg=None, bc_value=None):
+        self.lib = load()
+        self.mesh = dmesh
+        self.cfg = cfg if cfg is not None else flow_cfg()
+        self.bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
+        h = C.c_void_p()
+        check(self.lib.mi_flow_create(dmesh.h, ptr(self.bv), C.byref(self.cfg), C.byref(h)))
+        self.h = h.value
+        p = [C.c_void_p() for _ in range(5)]
+        check(self.lib.mi_flow_fields(self.h, *(C.byref(x) for x in p)))
+        self._ptrs = [x.value for x in p]
+
+    def _get(self, k, n):
+        out = np.empty(n)
+        check(self.lib.mi_memcpy_d2h(ptr(out), self._ptrs[k], out.nbytes))
+        return out
+
+    def _set(self, k, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(self.lib.mi_memcpy_h2d(self._ptrs[k], ptr(a), a.nbytes))
+
+    @property
+    def U(self):
+        return [self._get(k, self.mesh.n_cells) for k in range(3)]
+
+    @U.setter
+    def U(self, U):
+        for k in range(3):
+            self._set(k, U[k])
+
+    @property
+    def p(self):
+        return self._get(3, self.mesh.n_cells)
+
+    @p.setter
+    def p(self, v):
+        self._set(3, v)
+
+    @property
+    def phi(self):
+        return self._get(4, self.mesh.n_faces)
+
+    def iterate(self):
+        r = FlowResiduals()
+        check(self.lib.mi_flow_iterate(self.h, C.byref(r)))
+        return r.as_dict()
+
+    def advance_time_step(self, n_inner=1):
+        r = FlowResiduals()
+        check(self.lib.mi_flow_advance_time_step(self.h, n_inner, C.byref(r)))
+        return r.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_flow_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -12,72 +12,11 @@
 // there are no atomics and each row's sums are formed in the reference's
 // order.  With floating-point contraction off the coefficients are the same
 // IEEE operations as the reference's, so the assembled values are bit-exact.
-#include "csr.hpp"
+#include "mesh.hpp"
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
-
-namespace mi
-{
-
-enum : int8_t { BC_WALL = 0, BC_PRESSURE = 1, BC_SYMMETRY = 2, BC_VELOCITY_INLET = 3, BC_OUTFLOW = 4 };
-
-struct MeshDev
-{
-    int n_cells, n_faces;
-    const int* owner;
-    const int* nbr; // -1 on boundary faces
-    const double *ax, *ay, *az, *fcx, *fcy, *fcz;
-    const int8_t* fbc;  // boundary class of each face (interior: -1)
-    const int* fzone;   // zone id of each face
-    const double *ccx, *ccy, *ccz, *vol;
-    const int* cf_off;  // [n_cells+1]
-    const int* cf_face; // faces of each cell, ascending id
-    const int* cf_slot; // CSR position of (cell, other cell); -1 on boundary faces
-    const int* rp;      // CSR row pointer (for zeroing rows)
-    const int* diag;    // CSR position of the diagonal
-};
-
-} // namespace mi
-
-struct mi_mesh
-{
-    std::shared_ptr<mi::Pattern> pat;
-    int64_t n_cells = 0, n_faces = 0;
-    int n_zones = 0;
-    std::vector<int> bc_of_zone;
-    bool any_dirichlet = false;
-    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot;
-    mi::DevBuf<int8_t> fbc;
-    mi::DevBuf<double> ax, ay, az, fcx, fcy, fcz, ccx, ccy, ccz, vol;
-    mi::MeshDev dev() const
-    {
-        mi::MeshDev d{};
-        d.n_cells = int(n_cells);
-        d.n_faces = int(n_faces);
-        d.owner = owner.p;
-        d.nbr = nbr.p;
-        d.ax = ax.p;
-        d.ay = ay.p;
-        d.az = az.p;
-        d.fcx = fcx.p;
-        d.fcy = fcy.p;
-        d.fcz = fcz.p;
-        d.fbc = fbc.p;
-        d.fzone = fzone.p;
-        d.ccx = ccx.p;
-        d.ccy = ccy.p;
-        d.ccz = ccz.p;
-        d.vol = vol.p;
-        d.cf_off = cf_off.p;
-        d.cf_face = cf_face.p;
-        d.cf_slot = cf_slot.p;
-        d.rp = pat->rp.p;
-        d.diag = pat->diag.p;
-        return d;
-    }
-};
 
 namespace mi
 {
@@ -195,11 +134,14 @@ __global__ __launch_bounds__(kBlock) void k_face_gradient(MeshDev m, double rho,
 
 // Momentum row of cell c, one velocity component, first-order upwind +
 // diffusion (src/solver/SimpleAlgorithm.cpp:240-355).  pgradV (nullable) is
-// the cell's grad(p)_comp * V_c, subtracted from the rhs (:307-315).
+// the cell's grad(p)_comp * V_c, subtracted from the rhs (:307-315).  With dt
+// the time term (:317-336): implicit Euler on Uold, or BDF2 on Uold and Unm1
+// when Unm1 is given.
 __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
     MeshDev m, int comp, double mu, double rho, const double* __restrict__ phi,
     const double* __restrict__ pgradV, const double* __restrict__ dt,
-    const double* __restrict__ Uold, const double* __restrict__ Ucomp, double urf,
+    const double* __restrict__ Uold, const double* __restrict__ Unm1,
+    const double* __restrict__ Ucomp, double urf,
     const double* __restrict__ bcValue, double* __restrict__ val, double* __restrict__ rhs)
 {
     const int c = int(blockIdx.x) * kBlock + threadIdx.x;
@@ -253,8 +195,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
     double a = (kd >= 0) ? val[kd] + diag : diag;
     if (dt && kd >= 0) {
         const double Vrho_dt = rho * m.vol[c] / dt[c];
-        a += Vrho_dt;
-        b += (Uold ? Uold[c] : 0.0) * Vrho_dt;
+        const double un = Uold ? Uold[c] : 0.0;
+        if (Unm1) {
+            a += 1.5 * Vrho_dt;
+            b += (2.0 * un - 0.5 * Unm1[c]) * Vrho_dt;
+        } else {
+            a += Vrho_dt;
+            b += un * Vrho_dt;
+        }
     }
     if (kd >= 0) {
         const double scaled = a / urf;
@@ -301,6 +249,49 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshDev m, double rho,
 
 #pragma clang fp contract(on)
 
+int launch_assemble_pressure(const mi_mesh* M, double rho, const double* aP, const double* gamma,
+                             const double* phi, int pin, double* val, double* rhs, hipStream_t s)
+{
+    if (M->n_cells > 0) {
+        k_assemble_pressure<<<grid_for(M->n_cells, kBlock), kBlock, 0, s>>>(
+            M->dev(), rho, aP, gamma, phi, pin, val, rhs);
+        MI_HIP(hipGetLastError());
+    }
+    return MI_OK;
+}
+
+int launch_assemble_momentum(const mi_mesh* M, int comp, double mu, double rho, const double* phi,
+                             const double* pgradV, const double* dt, const double* Uold,
+                             const double* Unm1, const double* Ucomp, double urf,
+                             const double* bc_dev, double* val, double* rhs, hipStream_t s)
+{
+    if (M->n_cells > 0) {
+        k_assemble_momentum<<<grid_for(M->n_cells, kBlock), kBlock, 0, s>>>(
+            M->dev(), comp, mu, rho, phi, pgradV, dt, Uold, Unm1, Ucomp, urf, bc_dev, val, rhs);
+        MI_HIP(hipGetLastError());
+    }
+    return MI_OK;
+}
+
+int launch_face_flux(const mi_mesh* M, double rho, const double* Ux, const double* Uy,
+                     const double* Uz, const double* bc_dev, double* phi, hipStream_t s)
+{
+    if (M->n_faces > 0) {
+        k_face_flux<<<grid_for(M->n_faces, kBlock), kBlock, 0, s>>>(M->dev(), rho, Ux, Uy, Uz,
+                                                                    bc_dev, phi);
+        MI_HIP(hipGetLastError());
+    }
+    return MI_OK;
+}
+
+int mesh_bc_values(const mi_mesh* M, const double* h, DevBuf<double>& d, hipStream_t s)
+{
+    std::vector<double> v(size_t(std::max(M->n_zones, 1)) * 3, 0.0);
+    if (h)
+        std::memcpy(v.data(), h, size_t(M->n_zones) * 3 * sizeof(double));
+    return d.upload(v.data(), v.size(), s);
+}
+
 } // namespace mi
 
 using namespace mi;
@@ -323,19 +314,6 @@ template <class T> int up(mi::DevBuf<T>& d, const T* h, int64_t n, hipStream_t s
         return set_error(MI_ERR_BAD_ARG, "null mesh array");
     return d.upload(h, size_t(n), s);
 }
-
-// Device copy of a small host table for a kernel launch.
-struct BcValues
-{
-    DevBuf<double> d;
-    int set(const double* h, int n_zones, hipStream_t s)
-    {
-        std::vector<double> v(size_t(std::max(n_zones, 1)) * 3, 0.0);
-        if (h)
-            std::memcpy(v.data(), h, size_t(n_zones) * 3 * sizeof(double));
-        return d.upload(v.data(), v.size(), s);
-    }
-};
 
 } // namespace
 
@@ -481,11 +459,7 @@ MI_API int mi_assemble_pressure(mi_mesh* M, double rho, const double* aP, const 
     MI_TRY(ensure_device());
     const int pin = (!M->any_dirichlet && M->n_cells > 0) ? 1 : 0;
     hipStream_t s = default_stream();
-    if (M->n_cells > 0) {
-        k_assemble_pressure<<<grid_for(M->n_cells, kBlock), kBlock, 0, s>>>(
-            M->dev(), rho, aP, gamma, phi, pin, A->val.p, rhs);
-        MI_HIP(hipGetLastError());
-    }
+    MI_TRY(launch_assemble_pressure(M, rho, aP, gamma, phi, pin, A->val.p, rhs, s));
     MI_HIP(hipStreamSynchronize(s));
     A->values_changed(false);
     if (pinned)
@@ -504,13 +478,10 @@ MI_API int mi_assemble_momentum(mi_mesh* M, int comp, double mu, double rho, con
         return set_error(MI_ERR_BAD_ARG, "matrix was not created on this mesh's pattern");
     MI_TRY(ensure_device());
     hipStream_t s = default_stream();
-    BcValues bv;
-    MI_TRY(bv.set(bc_value_host, M->n_zones, s));
-    if (M->n_cells > 0) {
-        k_assemble_momentum<<<grid_for(M->n_cells, kBlock), kBlock, 0, s>>>(
-            M->dev(), comp, mu, rho, phi, pgradV, dt, Uold, Ucomp, urf, bv.d.p, A->val.p, rhs);
-        MI_HIP(hipGetLastError());
-    }
+    DevBuf<double> bv;
+    MI_TRY(mesh_bc_values(M, bc_value_host, bv, s));
+    MI_TRY(launch_assemble_momentum(M, comp, mu, rho, phi, pgradV, dt, Uold, nullptr, Ucomp, urf,
+                                    bv.p, A->val.p, rhs, s));
     MI_HIP(hipStreamSynchronize(s));
     A->values_changed(false);
     return MI_OK;
@@ -540,13 +511,9 @@ MI_API int mi_face_flux(mi_mesh* M, double rho, const double* Ux, const double* 
         return set_error(MI_ERR_BAD_ARG, "null argument");
     MI_TRY(ensure_device());
     hipStream_t s = default_stream();
-    BcValues bv;
-    MI_TRY(bv.set(bc_value_host, M->n_zones, s));
-    if (M->n_faces > 0) {
-        k_face_flux<<<grid_for(M->n_faces, kBlock), kBlock, 0, s>>>(M->dev(), rho, Ux, Uy, Uz,
-                                                                    bv.d.p, phi);
-        MI_HIP(hipGetLastError());
-    }
+    DevBuf<double> bv;
+    MI_TRY(mesh_bc_values(M, bc_value_host, bv, s));
+    MI_TRY(launch_face_flux(M, rho, Ux, Uy, Uz, bv.p, phi, s));
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }

@@ -1,0 +1,86 @@
+// mesh.hpp — the SoA mesh on the device (mi_mesh) and the view kernels take.
+#pragma once
+
+#include "csr.hpp"
+
+#include <memory>
+#include <vector>
+
+namespace mi
+{
+
+enum : int8_t { BC_WALL = 0, BC_PRESSURE = 1, BC_SYMMETRY = 2, BC_VELOCITY_INLET = 3, BC_OUTFLOW = 4 };
+
+struct MeshDev
+{
+    int n_cells, n_faces;
+    const int* owner;
+    const int* nbr; // -1 on boundary faces
+    const double *ax, *ay, *az, *fcx, *fcy, *fcz;
+    const int8_t* fbc;  // boundary class of each face (interior: -1)
+    const int* fzone;   // zone id of each face
+    const double *ccx, *ccy, *ccz, *vol;
+    const int* cf_off;  // [n_cells+1]
+    const int* cf_face; // faces of each cell, ascending id
+    const int* cf_slot; // CSR position of (cell, other cell); -1 on boundary faces
+    const int* rp;      // CSR row pointer (for zeroing rows)
+    const int* diag;    // CSR position of the diagonal
+};
+
+} // namespace mi
+
+struct mi_mesh
+{
+    std::shared_ptr<mi::Pattern> pat;
+    int64_t n_cells = 0, n_faces = 0;
+    int n_zones = 0;
+    std::vector<int> bc_of_zone;
+    bool any_dirichlet = false;
+    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot;
+    mi::DevBuf<int8_t> fbc;
+    mi::DevBuf<double> ax, ay, az, fcx, fcy, fcz, ccx, ccy, ccz, vol;
+    // LeastSquaresGradient's inverted 3x3 moment matrices, 6 per cell, built
+    // on first use (geometry only)
+    mi::DevBuf<double> ls_minv;
+    mi::MeshDev dev() const
+    {
+        mi::MeshDev d{};
+        d.n_cells = int(n_cells);
+        d.n_faces = int(n_faces);
+        d.owner = owner.p;
+        d.nbr = nbr.p;
+        d.ax = ax.p;
+        d.ay = ay.p;
+        d.az = az.p;
+        d.fcx = fcx.p;
+        d.fcy = fcy.p;
+        d.fcz = fcz.p;
+        d.fbc = fbc.p;
+        d.fzone = fzone.p;
+        d.ccx = ccx.p;
+        d.ccy = ccy.p;
+        d.ccz = ccz.p;
+        d.vol = vol.p;
+        d.cf_off = cf_off.p;
+        d.cf_face = cf_face.p;
+        d.cf_slot = cf_slot.p;
+        d.rp = pat->rp.p;
+        d.diag = pat->diag.p;
+        return d;
+    }
+};
+
+namespace mi
+{
+// Asynchronous launchers of the assembly kernels on stream s (assembly.hip).
+int launch_assemble_pressure(const mi_mesh* M, double rho, const double* aP, const double* gamma,
+                             const double* phi, int pin, double* val, double* rhs, hipStream_t s);
+int launch_assemble_momentum(const mi_mesh* M, int comp, double mu, double rho, const double* phi,
+                             const double* pgradV, const double* dt, const double* Uold,
+                             const double* Unm1, const double* Ucomp, double urf,
+                             const double* bc_dev, double* val, double* rhs, hipStream_t s);
+int launch_face_flux(const mi_mesh* M, double rho, const double* Ux, const double* Uy,
+                     const double* Uz, const double* bc_dev, double* phi, hipStream_t s);
+// Zone velocity table [n_zones * 3] on the device (h nullable: zeros).
+int mesh_bc_values(const mi_mesh* M, const double* h, DevBuf<double>& d, hipStream_t s);
+} // namespace mi

@@ -50,16 +50,18 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     # ctypes mirrors must have the C compiler's layout
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "simall_amd.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu\\n", sizeof(mi_solver_cfg), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(mi_solver_cfg), '
                    'sizeof(mi_stats), sizeof(mi_mesh_soa), offsetof(mi_solver_cfg, use_graph), '
-                   'offsetof(mi_stats, solve_seconds));return 0;}\n')
+                   'offsetof(mi_stats, solve_seconds), sizeof(mi_flow_cfg), '
+                   'offsetof(mi_flow_cfg, pressure), sizeof(mi_flow_residuals));return 0;}\n')
     exe = tmp_path / "sz"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
                                            check=True).stdout.split()]
     assert got == [ctypes.sizeof(capi.SolverCfg), ctypes.sizeof(capi.Stats),
                    ctypes.sizeof(capi.MeshSoa), capi.SolverCfg.use_graph.offset,
-                   capi.Stats.solve_seconds.offset]
+                   capi.Stats.solve_seconds.offset, ctypes.sizeof(capi.FlowCfg),
+                   capi.FlowCfg.pressure.offset, ctypes.sizeof(capi.FlowResiduals)]
 
 
 def test_no_device_is_reported_not_faked():

@@ -207,3 +207,73 @@ def test_face_flux_uniform_velocity():
     assert np.allclose(phi, expect, rtol=1e-14, atol=1e-15)
     phiw = oracle.face_flux(m, [wl.BC_WALL] * 7, 2.0, U)
     assert (phiw[m.neighbor == BND] == 0).all()
+
+
+def _moments(m):
+    """Per-cell w d d^T sums (w = 1/(|d|^2 + 1e-30)) over the faces a cell
+    owns, the faces it is the neighbour of, and its boundary faces."""
+    n = m.n_cells
+    own, nbr, bnd = (np.zeros((n, 3, 3)) for _ in range(3))
+    inner = m.neighbor != BND
+    f = np.flatnonzero(inner)
+    o, nb = m.owner[f].astype(np.int64), m.neighbor[f].astype(np.int64)
+    d = np.stack([m.cellCx[nb] - m.cellCx[o], m.cellCy[nb] - m.cellCy[o], m.cellCz[nb] - m.cellCz[o]], 1)
+    wdd = d[:, :, None] * d[:, None, :] / ((d * d).sum(1) + 1e-30)[:, None, None]
+    np.add.at(own, o, wdd)
+    np.add.at(nbr, nb, wdd)
+    fb = np.flatnonzero(~inner)
+    ob = m.owner[fb].astype(np.int64)
+    db = np.stack([m.faceCx[fb] - m.cellCx[ob], m.faceCy[fb] - m.cellCy[ob], m.faceCz[fb] - m.cellCz[ob]], 1)
+    np.add.at(bnd, ob, db[:, :, None] * db[:, None, :] / ((db * db).sum(1) + 1e-30)[:, None, None])
+    return own, nbr, bnd
+
+
+def test_ls_gradient_closed_form_for_linear_fields():
+    """For phi = a.x the restated LeastSquaresGradient gives
+    g = M^-1 (M_own - M_nbr) a, M = M_own + M_nbr + M_bnd: the reference adds
+    w d (-dphi) to the neighbour cell (Gradient.cpp:118-120) with d seen from
+    the owner, so faces a cell is the neighbour of enter with the wrong sign.
+    Interior cells of a uniform grid get zero; a boundary cell at x-min gets
+    a_x / 2."""
+    m = wl.Mesh(6, 5, 4, perturb=0.2, node_seed=3)
+    a = np.array([0.7, -1.3, 2.1])
+    phi = 0.4 + a[0] * m.cellCx + a[1] * m.cellCy + a[2] * m.cellCz
+    g = np.stack(oracle.ls_gradient(m, phi), 1)
+    own, nbr, bnd = _moments(m)
+    expect = np.linalg.solve(own + nbr + bnd, np.einsum("cij,j->ci", own - nbr, a)[..., None])[..., 0]
+    assert np.allclose(g, expect, rtol=0, atol=1e-11)
+    u = wl.Mesh(6, 5, 4)
+    gu = oracle.ls_gradient(u, a[0] * u.cellCx)[0].reshape(4, 5, 6)
+    assert np.allclose(gu[1:-1, 1:-1, 1:-1], 0.0, atol=1e-13)
+    assert np.allclose(gu[2, 2, 0], a[0] / 2, rtol=1e-13)
+    zero = oracle.ls_gradient(m, np.full(m.n_cells, 3.0))
+    assert all(np.array_equal(z, np.zeros(m.n_cells)) for z in zero)
+
+
+def test_correct_fields_closed_forms():
+    m = wl.Mesh(4, 3, 3, perturb=0.1, node_seed=2)
+    n = m.n_cells
+    aP = np.linspace(0.5, 2.0, n)
+    U = [np.linspace(-1, 1, n) * (k + 1) for k in range(3)]
+    phi = np.linspace(0, 1, m.n_faces)
+    p0 = np.linspace(2, 3, n)
+    # constant p': no gradient, no face difference: only p moves
+    p, Uc, phic = oracle.correct_fields(m, 0.3, 1.0, aP, np.full(n, 5.0), p0, U, phi)
+    assert np.allclose(p, p0 + 1.5, rtol=1e-15)
+    assert all(np.array_equal(Uc[k], U[k]) for k in range(3)) and np.array_equal(phic, phi)
+    # linear p' = c.x: U -= V/aP g with g the closed form of the gradient
+    # test above; interior faces get phi += rho |S|^2 / (aP_f |d.S|) * (-(c.d))
+    c = np.array([0.3, -0.2, 0.5])
+    pp = m.cellCx * c[0] + m.cellCy * c[1] + m.cellCz * c[2]
+    p, Uc, phic = oracle.correct_fields(m, 1.0, 2.0, aP, pp, p0, U, phi)
+    own, nbr, bnd = _moments(m)
+    g = np.linalg.solve(own + nbr + bnd, np.einsum("cij,j->ci", own - nbr, c)[..., None])[..., 0]
+    for k in range(3):
+        expect = U[k] - m.volume / aP * g[:, k]
+        assert np.allclose(Uc[k], expect, rtol=1e-12, atol=1e-12)
+    f = np.flatnonzero(m.neighbor != BND)
+    o, nb = m.owner[f].astype(np.int64), m.neighbor[f].astype(np.int64)
+    d = np.stack([m.cellCx[nb] - m.cellCx[o], m.cellCy[nb] - m.cellCy[o], m.cellCz[nb] - m.cellCz[o]])
+    S = np.stack([m.areaX[f], m.areaY[f], m.areaZ[f]])
+    coef = 2.0 * (S * S).sum(0) / (0.5 * (aP[o] + aP[nb]) * np.abs((d * S).sum(0)))
+    assert np.allclose(phic[f], phi[f] - coef * (c @ d), rtol=1e-12, atol=1e-13)

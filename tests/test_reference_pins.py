@@ -2,10 +2,12 @@
 (CPU only; skipped where oracle/_ref/libsimall_ref.so was not built, i.e.
 without the SimAll sources).
 
-oracle/ref.mk compiles the reference's mesher, SIMPLE assembly, SpMV, discrete
-adjoint and Ruge-Stueben AMG from their own source files; these tests require
-bit-exact agreement of mesh arrays, assembled coefficients and SpMV, and the
-reference adjoint known answer.
+oracle/ref.mk compiles the reference's mesher, SIMPLE assembly and field
+correction, least-squares gradient, PISO loop, SpMV, discrete adjoint and
+Ruge-Stueben AMG from their own source files; these tests require bit-exact
+agreement of mesh arrays, assembled coefficients, gradients, corrected fields
+and SpMV, the reference adjoint known answer, and SIMPLE / PISO iterations
+that agree to the rounding of two direct solvers.
 """
 import ctypes as C
 import os
@@ -35,6 +37,9 @@ def ref():
         "ref_spmv": [i, P, P, P, P, P],
         "ref_discrete_adjoint": [i, P, P, P, P, i, P, P, i, i, d, P, P, P, P],
         "ref_rs_amg_pcg": [i, P, P, P, P, d, i, P, P, P],
+        "ref_ls_gradient": [i, i, i, P, P, d, u64, P, P, P, P],
+        "ref_correct_fields": [i, i, i, P, P, d, u64, P, i, d, d, P, P, P, P, P],
+        "ref_flow": [i, i, i, P, P, d, u64, P, i, P, i, d, d, d, d, d, i, i, i, i, i, P, P, P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -192,3 +197,89 @@ def test_reference_rs_amg_pcg_runs(ref):
     assert 0 < it < 100 and last.value < 1e-8
     xo, _, _, _ = oracle.cg(oracle.Matrix(rp, ci, vals), b, oracle.PC_JACOBI, 1e-12, 10000)
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-5
+
+
+@pytest.mark.parametrize("N,o,L,perturb,seed", CASES)
+def test_ls_gradient_matches_reference(ref, N, o, L, perturb, seed):
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=perturb, node_seed=seed)
+    phi = wl.normal(8, m.n_cells)
+    g = oracle.ls_gradient(m, phi)
+    r = [np.empty(m.n_cells) for _ in range(3)]
+    ref.ref_ls_gradient(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed), _p(phi),
+                        *(_p(a) for a in r))
+    for k in range(3):
+        assert np.array_equal(g[k], r[k]), k
+
+
+@pytest.mark.parametrize("N,o,L,perturb,seed", CASES[1:])
+def test_correct_fields_matches_reference(ref, N, o, L, perturb, seed):
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=perturb, node_seed=seed)
+    bc = np.zeros(7, np.int32)
+    bc[2] = wl.BC_PRESSURE
+    n = m.n_cells
+    aP = wl.uniform(3, 0.5, 2.0, n)
+    aP[::7] = 0.0  # max(aP, 1e-30) branch
+    pp, p0 = wl.normal(5, n), wl.normal(6, n)
+    U0 = [wl.normal(10 + k, n) for k in range(3)]
+    phi0 = wl.normal(9, m.n_faces)
+    p, U, phi = oracle.correct_fields(m, 0.3, 1.3, aP, pp, p0, U0, phi0)
+    rp_, rU, rphi = p0.copy(), np.concatenate(U0), phi0.copy()
+    ref.ref_correct_fields(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed), _p(bc), 7,
+                           C.c_double(1.3), C.c_double(0.3), _p(aP), _p(pp), _p(rp_), _p(rU), _p(rphi))
+    assert np.array_equal(p, rp_)
+    for k in range(3):
+        assert np.array_equal(U[k], rU[k * n:(k + 1) * n]), k
+    assert np.array_equal(phi, rphi)
+
+
+def _channel():
+    """4x3x3 channel: velocity inlet on x-min, pressure outlet on x-max, walls."""
+    N, o, L = (5, 3, 3), (0.0, 0.0, 0.0), (2.0, 1.0, 1.0)
+    bc = np.array([0, wl.BC_VELOCITY_INLET, wl.BC_PRESSURE, wl.BC_WALL, wl.BC_WALL, wl.BC_WALL,
+                   wl.BC_SYMMETRY], np.int32)
+    bcv = np.zeros((7, 3))
+    bcv[1] = [1.0, 0.1, 0.0]
+    return N, o, L, bc, bcv
+
+
+def _ref_flow(ref, N, o, L, perturb, seed, bc, bcv, algorithm, rho, mu, urfU, urfP, dt, bdf2,
+              ncorr, nonorth, steps, inner, n, nf, U0=None, p0=None):
+    U = np.zeros(3 * n) if U0 is None else np.concatenate(U0)
+    p = np.zeros(n) if p0 is None else p0.copy()
+    phi, res = np.empty(nf), np.empty(4)
+    ref.ref_flow(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed), _p(bc), 7, _p(bcv),
+                 algorithm, C.c_double(rho), C.c_double(mu), C.c_double(urfU), C.c_double(urfP),
+                 C.c_double(dt), int(bdf2), ncorr, nonorth, steps, inner, _p(U), _p(p), _p(phi), _p(res))
+    return [U[k * n:(k + 1) * n] for k in range(3)], p, phi, res
+
+
+def _close(a, b, rtol):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.linalg.norm(a - b) <= rtol * max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("algorithm", ["simple", "piso"])
+def test_flow_iterations_match_reference(ref, algorithm):
+    """Three outer iterations (SIMPLE, steady, urf 0.7/0.3) or three BDF2 time
+    steps (PISO, 2 correctors) of the oracle's Flow against the reference's
+    own SimpleAlgorithm / PisoAlgorithm, both with direct linear solves."""
+    N, o, L, bc, bcv = _channel()
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=0.1, node_seed=4)
+    piso = algorithm == "piso"
+    F = oracle.Flow(m, bc, bcv, algorithm=algorithm, rho=1.1, mu=0.05,
+                    dt=0.02 if piso else 0.0)
+    for _ in range(3):
+        res = F.advance_time_step(1) if piso else F.iterate()
+    U, p, phi, rres = _ref_flow(ref, N, o, L, 0.1, 4, bc, bcv, 1 if piso else 0, 1.1, 0.05,
+                                F.urfU, F.urfP, F.dt, F.bdf2, F.n_correctors, F.n_nonorth,
+                                3 if piso else 0, 1, m.n_cells, m.n_faces)
+    if not piso:  # the reference driver ran one iterate(): run two more on its output
+        for _ in range(2):
+            U, p, phi, rres = _ref_flow(ref, N, o, L, 0.1, 4, bc, bcv, 0, 1.1, 0.05, F.urfU, F.urfP,
+                                        0.0, False, 1, 0, 0, 1, m.n_cells, m.n_faces, U0=U, p0=p)
+    for k in range(3):
+        assert _close(F.U[k], U[k], 1e-10), k
+    assert _close(F.p, p, 1e-10) and _close(F.phi, phi, 1e-10)
+    assert np.allclose(res["mom"] + [res["cont"]], rres, rtol=1e-8, atol=1e-12)
+    assert np.linalg.norm(F.U[0]) > 0.1  # the inlet drives a flow
+
