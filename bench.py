@@ -124,6 +124,45 @@ def bench_momentum(capi, wl, lib, steps, log, N=96):
     return out
 
 
+def bench_flow(capi, wl, lib, log, N=96, iters=3):
+    """Device-resident SIMPLE outer iterations and PISO time steps on a 96^3
+    channel (velocity inlet, pressure outlet, walls; rho 1, mu 1e-3): per
+    iteration the face fluxes, three momentum assemblies and GMRES(30)-ILU(0)
+    solves, the pressure assembly, its AMG-PCG solve and the correction.  With
+    the reference's preconditioner rebuild at every solve, and keeping the
+    preconditioners across iterations."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    m = wl.Mesh(N, N, N)
+    bc = np.array([0, wl.BC_VELOCITY_INLET, wl.BC_PRESSURE] + [wl.BC_WALL] * 4, np.int32)
+    bcv = np.zeros((7, 3))
+    bcv[1] = [1.0, 0.0, 0.0]
+    dm = capi.DeviceMesh(m, bc)
+    log(f"flow {N}^3: n={m.n_cells} ({time.perf_counter() - t0:.1f}s)")
+    out = {"workload": f"{N}^3 channel, velocity inlet / pressure outlet / walls, rho 1, mu 1e-3; "
+                       f"momentum GMRES(30)-ILU(0) 1e-8, pressure CG-AMG 1e-8",
+           "n": m.n_cells}
+    for name, alg, keep in (("simple", capi.MI_FLOW_SIMPLE, 0), ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1),
+                            ("piso", capi.MI_FLOW_PISO, 0)):
+        cfg = capi.flow_cfg(alg, momentum=dict(keep_preconditioner=keep),
+                            pressure=dict(keep_preconditioner=keep))
+        F = capi.Flow(dm, cfg, bcv)
+        times, res = [], None
+        for it in range(iters + 1):  # the first iteration builds the patterns
+            lib.mi_device_synchronize()
+            t1 = time.perf_counter()
+            res = F.advance_time_step(1) if alg == capi.MI_FLOW_PISO else F.iterate()
+            lib.mi_device_synchronize()
+            if it > 0:
+                times.append(time.perf_counter() - t1)
+        out[name] = {"ms_per_iteration": round(1e3 * float(np.median(times)), 1),
+                     "iterations_timed": iters, "last_residuals": res}
+        F.close()
+    log(f"flow: {out}")
+    return out
+
+
 def bench_batch(capi, wl, lib, steps, log, N=100, n_eq=32):
     """32 transport equations A_e = V/dt I + Gamma_e L on one 100^3 perturbed,
     permuted mesh (1,000,000 cells): one batched solve of all equations to
@@ -209,6 +248,7 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--caller-order", action="store_true",
                     help="also time SpMV on the caller's (randomly numbered) matrix")
+    ap.add_argument("--no-flow", action="store_true", help="skip the SIMPLE / PISO iteration section")
     ap.add_argument("--no-momentum", action="store_true",
                     help="skip the ILU(0)-BiCGStab momentum measurement (96^3)")
     ap.add_argument("--no-batch", action="store_true",
@@ -353,6 +393,10 @@ def main():
     if not args.no_momentum:
         momentum = bench_momentum(capi, wl, lib, args.steps, log)
 
+    flow = None
+    if not args.no_flow:
+        flow = bench_flow(capi, wl, lib, log)
+
     batch = None
     if not args.no_batch:
         batch = bench_batch(capi, wl, lib, args.steps, log)
@@ -416,6 +460,7 @@ def main():
         "adjoint": adjoint,
         "batched": batch,
         "momentum": momentum,
+        "flow": flow,
     }
     print(json.dumps(out), flush=True)
 

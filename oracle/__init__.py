@@ -19,6 +19,9 @@ How each is pinned (tests/test_oracle.py):
   ssor               identity -> r (tests/unit/solver/test_ssor_preconditioner.cpp:44-63);
                      CG+SSOR 1-D Laplacian n=50 (:65-88)
   gmres              exact solution vs numpy on random nonsymmetric systems
+  tfqmr              reference KATs: Jacobi on the 1-D Laplacian n=30 (||r||<1e-5) and ILU on
+                     upwind convection-diffusion n=40, Pe=5 (||r||<1e-4)
+                     (tests/unit/solver/test_tfqmr_solver.cpp:66-111); dense solves
   gmres_transpose    KAT psi=(0.5,1/6) (tests/unit/longtail/test_longtail_modules.cpp:32-54)
   build_sparsity     brute-force neighbour sets from the face lists
   assemble_pressure  uniform grid, unit gamma: diagonal = #interior neighbours, off-diag -1,
@@ -88,6 +91,7 @@ def load():
             "orc_cg_solve": (i, [P(OrcCsr), i, d, i, vp, vp, P(d), vp, P(i)]),
             "orc_bicgstab_solve": (i, [P(OrcCsr), i, d, i, vp, vp, P(d), vp, P(i)]),
             "orc_gmres_solve": (i, [P(OrcCsr), i, d, i, i, vp, vp, P(d)]),
+            "orc_tfqmr_solve": (i, [P(OrcCsr), i, d, i, vp, vp, P(d)]),
             "orc_gmres_transpose": (i, [P(OrcCsr), vp, vp, i, i, d, P(i), P(d)]),
             "orc_build_sparsity": (i64, [P(OrcMesh), vp, vp]),
             "orc_assemble_pressure": (i, [P(OrcMesh), vp, i, d, vp, vp, vp, P(OrcCsr), vp, vp]),
@@ -251,6 +255,15 @@ def gmres(A, b, pc=PC_ILU, tol=1e-8, max_iter=1000, restart=30):
     x = np.zeros(A.n)
     last = C.c_double()
     it = load().orc_gmres_solve(C.byref(A.s), pc, tol, max_iter, restart, _p(b), _p(x), C.byref(last))
+    return x, it, last.value
+
+
+def tfqmr(A, b, pc=PC_JACOBI, tol=1e-8, max_iter=1000):
+    """TfqmrSolver::solve (LinearSolvers.cpp:497-603) -> (x, iterations, last)."""
+    b = _vec(b)
+    x = np.zeros(A.n)
+    last = C.c_double()
+    it = load().orc_tfqmr_solve(C.byref(A.s), pc, tol, max_iter, _p(b), _p(x), C.byref(last))
     return x, it, last.value
 
 

@@ -438,6 +438,128 @@ done:
 }
 
 /* GMRESmSolver::solve, src/solver/LinearSolvers.cpp:267-342 */
+/* TfqmrSolver::solve, src/solver/LinearSolvers.cpp:497-603: x0 = 0; the
+ * double-step quasi-minimal residual loop with the residual bound
+ * tau*sqrt(2k+3) and a true-residual confirmation.  Returns the iteration
+ * count as the reference: it + 1 on convergence, it on a breakdown
+ * (|rh.v| or |rho| < 1e-30), maxIter at the limit. */
+int orc_tfqmr_solve(const orc_csr* A, int pc, double tol, int maxIter, const double* b,
+                    double* x, double* last_residual)
+{
+    orc_pc P;
+    if (pc_setup(&P, A, pc) != 0)
+        return -1;
+    const int n = A->n;
+    double* r = (double*)xcalloc(n);
+    double* rh = (double*)xcalloc(n);
+    double* w = (double*)xcalloc(n);
+    double* d = (double*)xcalloc(n);
+    double* u1 = (double*)xcalloc(n);
+    double* u2 = (double*)xcalloc(n);
+    double* v = (double*)xcalloc(n);
+    double* Au1 = (double*)xcalloc(n);
+    double* Au2 = (double*)xcalloc(n);
+    double* z = (double*)xcalloc(n);
+    int ret = maxIter;
+    double last = 0.0;
+    for (int i = 0; i < n; ++i) {
+        x[i] = 0.0;
+        r[i] = b[i];
+        rh[i] = r[i];
+        w[i] = r[i];
+    }
+    pc_apply(&P, r, u1);
+    orc_spmv(A, u1, v);
+    memcpy(Au1, v, sizeof(double) * (size_t)n);
+    double tau = orc_norm2(r, n);
+    double bnorm = orc_norm2(b, n);
+    if (bnorm < 1.0e-30)
+        bnorm = 1.0e-30;
+    if (tau / bnorm < tol) {
+        last = tau / bnorm;
+        ret = 0;
+        goto done;
+    }
+    {
+        double rho = orc_dot(rh, r, n);
+        double theta = 0.0, eta = 0.0, alpha = 0.0;
+        for (int it = 0; it < maxIter; ++it) {
+            const double sigma = orc_dot(rh, v, n);
+            if (fabs(sigma) < 1.0e-30) {
+                last = tau / bnorm;
+                ret = it;
+                goto done;
+            }
+            alpha = rho / sigma;
+            pc_apply(&P, v, z);
+            for (int i = 0; i < n; ++i)
+                u2[i] = u1[i] - alpha * z[i];
+            orc_spmv(A, u2, Au2);
+            for (int half = 0; half < 2; ++half) {
+                const double* Au = half == 0 ? Au1 : Au2;
+                const double* u = half == 0 ? u1 : u2;
+                for (int i = 0; i < n; ++i)
+                    w[i] -= alpha * Au[i];
+                const double theta_new = orc_norm2(w, n) / (tau > 1.0e-30 ? tau : 1.0e-30);
+                const double c = 1.0 / sqrt(1.0 + theta_new * theta_new);
+                const double tau_new = tau * theta_new * c;
+                const double eta_new = c * c * alpha;
+                const double coef = theta * theta * eta / (alpha > 1.0e-30 ? alpha : 1.0e-30);
+                for (int i = 0; i < n; ++i) {
+                    d[i] = u[i] + coef * d[i];
+                    x[i] += eta_new * d[i];
+                }
+                tau = tau_new;
+                theta = theta_new;
+                eta = eta_new;
+            }
+            const double resBound = tau * sqrt(2.0 * (it + 1) + 1.0);
+            last = resBound / bnorm;
+            if (last < tol) {
+                orc_spmv(A, x, z);
+                for (int i = 0; i < n; ++i)
+                    z[i] = b[i] - z[i];
+                last = orc_norm2(z, n) / bnorm;
+                if (last < tol) {
+                    ret = it + 1;
+                    goto done;
+                }
+            }
+            for (int i = 0; i < n; ++i)
+                r[i] -= alpha * (Au1[i] + Au2[i]);
+            const double rhoNew = orc_dot(rh, r, n);
+            if (fabs(rho) < 1.0e-30) {
+                last = tau / bnorm;
+                ret = it;
+                goto done;
+            }
+            const double beta = rhoNew / rho;
+            pc_apply(&P, r, z);
+            for (int i = 0; i < n; ++i)
+                u1[i] = z[i] + beta * (u2[i] + beta * u1[i]);
+            orc_spmv(A, u1, Au1);
+            for (int i = 0; i < n; ++i)
+                v[i] = Au1[i] + beta * (Au2[i] + beta * v[i]);
+            rho = rhoNew;
+        }
+    }
+done:
+    if (last_residual)
+        *last_residual = last;
+    free(r);
+    free(rh);
+    free(w);
+    free(d);
+    free(u1);
+    free(u2);
+    free(v);
+    free(Au1);
+    free(Au2);
+    free(z);
+    pc_free(&P);
+    return ret;
+}
+
 int orc_gmres_solve(const orc_csr* A, int pc, double tol, int maxIter, int restart,
                     const double* b, double* x, double* last_residual)
 {

@@ -84,6 +84,8 @@ int orc_cg_solve(const orc_csr* A, int pc, double tol, int maxIter,
 int orc_bicgstab_solve(const orc_csr* A, int pc, double tol, int maxIter,
                        const double* b, double* x, double* last_residual,
                        double* history, int* n_history);
+int orc_tfqmr_solve(const orc_csr* A, int pc, double tol, int maxIter, const double* b,
+                    double* x, double* last_residual);
 int orc_gmres_solve(const orc_csr* A, int pc, double tol, int maxIter, int restart,
                     const double* b, double* x, double* last_residual);
 

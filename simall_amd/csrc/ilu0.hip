@@ -125,6 +125,184 @@ __global__ __launch_bounds__(kBlock) void k_ilu_two_run(StreamMat<T> M, const T*
     stream_rows(M, x, e);
 }
 
+// SSOR with omega = 1 (SsorPreconditioner, src/solver/LinearSolvers.cpp:163-236,
+// as make_preconditioner builds it: :247-248): forward
+// y_i = (r_i - sum_{j<i} a_ij y_j) / a_ii, then z_i = (a_ii y_i -
+// sum_{j>i} a_ij z_j) / a_ii, each division a multiplication by the stored
+// 1/a_ii (1 for a zero diagonal).  With omega = 1 the reference's scalings by
+// omega and 2 - omega are exact and drop out.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ssor_lower_level(int cnt, const int* __restrict__ rows,
+                                                             const int* __restrict__ rp,
+                                                             const int* __restrict__ lend,
+                                                             const int* __restrict__ ci,
+                                                             const T* __restrict__ a,
+                                                             const T* __restrict__ invD,
+                                                             const T* __restrict__ r,
+                                                             T* __restrict__ y)
+{
+    const int t = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= cnt)
+        return;
+    const int i = rows[t];
+    T s = r[i];
+    for (int k = rp[i]; k < lend[i]; ++k)
+        s -= a[k] * y[ci[k]];
+    y[i] = s * invD[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ssor_upper_level(int cnt, const int* __restrict__ rows,
+                                                             const int* __restrict__ rp,
+                                                             const int* __restrict__ diag,
+                                                             const int* __restrict__ ci,
+                                                             const T* __restrict__ a,
+                                                             const T* __restrict__ D,
+                                                             const T* __restrict__ invD,
+                                                             const T* __restrict__ y,
+                                                             T* __restrict__ z)
+{
+    const int t = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= cnt)
+        return;
+    const int i = rows[t];
+    T s = D[i] * y[i];
+    for (int k = diag[i] + 1; k < rp[i + 1]; ++k)
+        s -= a[k] * z[ci[k]];
+    z[i] = s * invD[i];
+}
+
+template <class T> struct EpiSsorLower
+{
+    const T* r;
+    const T* invD;
+    T* y;
+    T ri = T(0), di = T(1);
+    __device__ __forceinline__ void pre(int i)
+    {
+        ri = r[i];
+        di = invD[i];
+    }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = (ri - T(s)) * di; }
+};
+
+template <class T> struct EpiSsorUpper
+{
+    const T* y;
+    const T* D;
+    const T* invD;
+    T* z;
+    T yi = T(0), Di = T(1), di = T(1);
+    __device__ __forceinline__ void pre(int i)
+    {
+        yi = y[i];
+        Di = D[i];
+        di = invD[i];
+    }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        z[i] = (Di * yi - T(s)) * di;
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ssor_lower_run(StreamMat<T> L, const T* __restrict__ r,
+                                                           const T* __restrict__ invD, T* y)
+{
+    EpiSsorLower<T> e{r, invD, y};
+    stream_rows(L, y, e);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ssor_upper_run(StreamMat<T> U, const T* __restrict__ y,
+                                                           const T* __restrict__ D,
+                                                           const T* __restrict__ invD, T* z)
+{
+    EpiSsorUpper<T> e{y, D, invD, z};
+    stream_rows(U, z, e);
+}
+
+// D = a_ii, invD = 1 / a_ii (1 when zero), in T
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ssor_diag(int n, const int* __restrict__ diag,
+                                                      const double* __restrict__ a,
+                                                      T* __restrict__ D, T* __restrict__ invD)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const double d = a[diag[i]];
+    D[i] = T(d);
+    invD[i] = T(d != 0.0 ? 1.0 / d : 1.0);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_to_t(int64_t n, const double* __restrict__ a,
+                                                 T* __restrict__ b)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
+        b[i] = T(a[i]);
+}
+
+// IKJ elimination of rows [r0, r1) of one run, in place on the FP64 factor:
+// the operations of ilu0_factor_host for those rows.  The rows a row
+// eliminates with lie in earlier runs, whose elimination has finished.
+__global__ __launch_bounds__(kBlock) void k_ilu_factor_run(int r0, int r1,
+                                                           const int* __restrict__ rp,
+                                                           const int* __restrict__ ci,
+                                                           const int* __restrict__ diag,
+                                                           double* __restrict__ lu)
+{
+    const int i = r0 + int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= r1)
+        return;
+    const int e = rp[i + 1];
+    for (int kp = rp[i]; kp < e; ++kp) {
+        const int k = ci[kp];
+        if (k >= i)
+            break;
+        const int dk = diag[k];
+        const double piv = lu[dk];
+        if (piv == 0.0)
+            continue;
+        const double m = lu[kp] / piv;
+        lu[kp] = m;
+        int qp = kp + 1;
+        for (int jp = dk + 1; jp < rp[k + 1]; ++jp) {
+            const int j = ci[jp];
+            while (qp < e && ci[qp] < j)
+                ++qp;
+            if (qp >= e)
+                break;
+            if (ci[qp] == j)
+                lu[qp] -= m * lu[jp];
+        }
+    }
+}
+
+// Split factors in the iteration precision: L and U values through their
+// maps into the full factor, and the diagonal.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_ilu_split(int nl, const int* __restrict__ lmap, int nu,
+                                                      const int* __restrict__ umap, int n,
+                                                      const int* __restrict__ diag,
+                                                      const double* __restrict__ lu,
+                                                      T* __restrict__ lval, T* __restrict__ uval,
+                                                      T* __restrict__ d)
+{
+    const int64_t total = int64_t(nl) + nu + n;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < total; t += stride) {
+        if (t < nl)
+            lval[t] = T(lu[lmap[t]]);
+        else if (t < int64_t(nl) + nu)
+            uval[t - nl] = T(lu[umap[t - nl]]);
+        else
+            d[t - nl - nu] = T(lu[diag[t - nl - nu]]);
+    }
+}
+
 // The rows written by a run are never read by the same launch (a run has no
 // dependency inside it), so y may be both the gathered vector and the output.
 template <class T>
@@ -301,66 +479,81 @@ void run_blocks(const std::vector<int>& bounds, bool descending, const std::vect
 }
 } // namespace
 
-template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s)
+template <class T> int Ilu0<T>::prepare(const Pattern& P, hipStream_t s)
 {
+    if (level_built && levels_for == &P)
+        return MI_OK;
     graphs.clear();
     n = int(P.n);
     const int* rp = P.h_rp.data();
     const int* ci = P.h_ci.data();
-    std::vector<double> lu(vals);
-    std::vector<int> diag, lend(static_cast<size_t>(n));
+    std::vector<int> diag(static_cast<size_t>(n), -1), lend(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i) {
         int k = rp[i];
         while (k < rp[i + 1] && ci[k] < i)
             ++k;
         lend[size_t(i)] = k;
-    }
-    MI_TRY(ilu0_factor_host(n, rp, ci, lu, diag, &zero_pivots));
-    for (int i = 0; i < n; ++i)
-        if (diag[size_t(i)] < 0)
+        if (k < rp[i + 1] && ci[k] == i)
+            diag[size_t(i)] = k;
+        else
             return set_error(MI_ERR_ZERO_PIVOT, "ILU(0): row %d has no diagonal entry", i);
-    if (!level_built || levels_for != &P) {
-        std::vector<int> lrows, loff, urows, uoff;
-        ilu0_levels(n, rp, ci, lend, lrows, loff, urows, uoff);
-        MI_TRY(d_lrows.upload(lrows.data(), lrows.size(), s));
-        MI_TRY(d_urows.upload(urows.data(), urows.size(), s));
-        MI_TRY(d_lend.upload(lend.data(), lend.size(), s));
-        MI_TRY(d_diag.upload(diag.data(), diag.size(), s));
-        h_loff = loff;
-        h_uoff = uoff;
-        // run sweeps when every run holds many rows (each launch has the
-        // work of a large SpMV); otherwise the dependency levels
-        std::vector<int> lbeg, ubeg;
-        ilu0_segments(n, rp, ci, lend, diag, lbeg, ubeg);
-        const int runs = int(std::max(lbeg.size(), ubeg.size())) - 1;
-        seg = n > 0 && int64_t(n) >= int64_t(2048) * runs;
-        two = false;
-        if (seg) {
-            std::vector<int> trp, tci, blk;
-            split_triangle(n, rp, ci, [&](int i) { return rp[i]; },
-                           [&](int i) { return lend[size_t(i)]; }, trp, tci, Lmap);
-            run_blocks(lbeg, false, trp, blk, Lsb);
-            const std::vector<int> lrp_host = trp;
-            auto lrp_at = [&](int i) { return lrp_host[size_t(i)]; };
-            MI_TRY(Lrp.upload(trp.data(), trp.size(), s));
-            MI_TRY(Lci.upload(tci.data(), tci.size(), s));
-            MI_TRY(Lblk.upload(blk.data(), blk.size(), s));
-            MI_HIP(hipStreamSynchronize(s));
-            split_triangle(n, rp, ci, [&](int i) { return diag[size_t(i)] + 1; },
-                           [&](int i) { return rp[i + 1]; }, trp, tci, Umap);
-            run_blocks(ubeg, true, trp, blk, Usb);
-            // two runs [0, b) and [b, n) where the first has no L entries and
-            // the second no U entries (a red-black ordering): one launch per run
-            two = lbeg.size() == 3 && ubeg.size() == 3 && lbeg[1] == ubeg[1] &&
-                  lrp_at(lbeg[1]) == 0 && trp[size_t(n)] == trp[size_t(ubeg[1])];
-            MI_TRY(Urp.upload(trp.data(), trp.size(), s));
-            MI_TRY(Uci.upload(tci.data(), tci.size(), s));
-            MI_TRY(Ublk.upload(blk.data(), blk.size(), s));
-        }
-        MI_HIP(hipStreamSynchronize(s));
-        level_built = true;
-        levels_for = &P;
     }
+    std::vector<int> lrows, loff, urows, uoff;
+    ilu0_levels(n, rp, ci, lend, lrows, loff, urows, uoff);
+    MI_TRY(d_lrows.upload(lrows.data(), lrows.size(), s));
+    MI_TRY(d_urows.upload(urows.data(), urows.size(), s));
+    MI_TRY(d_lend.upload(lend.data(), lend.size(), s));
+    MI_TRY(d_diag.upload(diag.data(), diag.size(), s));
+    h_loff = loff;
+    h_uoff = uoff;
+    h_diag = diag;
+    // run sweeps when every run holds many rows (each launch has the work of
+    // a large SpMV); otherwise the dependency levels
+    ilu0_segments(n, rp, ci, lend, diag, lbeg, ubeg);
+    const int runs = int(std::max(lbeg.size(), ubeg.size())) - 1;
+    seg = n > 0 && int64_t(n) >= int64_t(2048) * runs;
+    two = false;
+    if (seg) {
+        std::vector<int> trp, tci, blk;
+        split_triangle(n, rp, ci, [&](int i) { return rp[i]; },
+                       [&](int i) { return lend[size_t(i)]; }, trp, tci, Lmap);
+        run_blocks(lbeg, false, trp, blk, Lsb);
+        const bool first_without_l = lbeg.size() == 3 && trp[size_t(lbeg[1])] == 0;
+        MI_TRY(Lrp.upload(trp.data(), trp.size(), s));
+        MI_TRY(Lci.upload(tci.data(), tci.size(), s));
+        MI_TRY(Lblk.upload(blk.data(), blk.size(), s));
+        MI_TRY(d_Lmap.upload(Lmap.data(), Lmap.size(), s));
+        MI_HIP(hipStreamSynchronize(s));
+        split_triangle(n, rp, ci, [&](int i) { return diag[size_t(i)] + 1; },
+                       [&](int i) { return rp[i + 1]; }, trp, tci, Umap);
+        run_blocks(ubeg, true, trp, blk, Usb);
+        // two runs [0, b) and [b, n) where the first has no L entries and
+        // the second no U entries (a red-black ordering): one launch per run
+        two = first_without_l && ubeg.size() == 3 && lbeg[1] == ubeg[1] &&
+              trp[size_t(n)] == trp[size_t(ubeg[1])];
+        MI_TRY(Urp.upload(trp.data(), trp.size(), s));
+        MI_TRY(Uci.upload(tci.data(), tci.size(), s));
+        MI_TRY(Ublk.upload(blk.data(), blk.size(), s));
+        MI_TRY(d_Umap.upload(Umap.data(), Umap.size(), s));
+        MI_TRY(Lval.alloc(std::max<size_t>(Lmap.size(), 1)));
+        MI_TRY(Uval.alloc(std::max<size_t>(Umap.size(), 1)));
+        MI_TRY(Ud.alloc(static_cast<size_t>(std::max(n, 1))));
+    }
+    MI_TRY(d_y.alloc(static_cast<size_t>(std::max(n, 1))));
+    rp_dev = P.rp.p;
+    ci_dev = P.ci.p;
+    MI_HIP(hipStreamSynchronize(s));
+    level_built = true;
+    levels_for = &P;
+    return MI_OK;
+}
+
+template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s)
+{
+    MI_TRY(prepare(P, s));
+    std::vector<double> lu(vals);
+    std::vector<int> diag;
+    MI_TRY(ilu0_factor_host(n, P.h_rp.data(), P.h_ci.data(), lu, diag, &zero_pivots));
     if (seg) {
         std::vector<T> lv(Lmap.size()), uv(Umap.size()), dv(static_cast<size_t>(n));
         for (size_t j = 0; j < Lmap.size(); ++j)
@@ -372,14 +565,58 @@ template <class T> int Ilu0<T>::setup(const Pattern& P, const std::vector<double
         MI_TRY(Lval.upload(lv.data(), lv.size(), s));
         MI_TRY(Uval.upload(uv.data(), uv.size(), s));
         MI_TRY(Ud.upload(dv.data(), dv.size(), s));
-        MI_HIP(hipStreamSynchronize(s));
     }
     std::vector<T> tv(lu.begin(), lu.end());
     MI_TRY(d_lu.upload(tv.data(), tv.size(), s));
-    MI_TRY(d_y.alloc(static_cast<size_t>(std::max(n, 1))));
-    rp_dev = P.rp.p;
-    ci_dev = P.ci.p;
     MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+template <class T> int Ilu0<T>::setup_device(const Pattern& P, const double* vals, hipStream_t s)
+{
+    MI_TRY(prepare(P, s));
+    if (!seg)
+        return set_error(MI_ERR_UNSUPPORTED, "ILU(0) device factorization needs a run ordering");
+    const size_t nnz = size_t(P.nnz);
+    MI_TRY(d_fac.alloc(std::max<size_t>(nnz, 1)));
+    if (nnz)
+        MI_HIP(hipMemcpyAsync(d_fac.p, vals, nnz * sizeof(double), hipMemcpyDeviceToDevice, s));
+    for (size_t t = 0; t + 1 < lbeg.size(); ++t) {
+        const int r0 = lbeg[t], r1 = lbeg[t + 1];
+        if (r1 > r0)
+            k_ilu_factor_run<<<grid_for(r1 - r0, kBlock), kBlock, 0, s>>>(r0, r1, P.rp.p, P.ci.p,
+                                                                          d_diag.p, d_fac.p);
+    }
+    const int nl = int(Lmap.size()), nu = int(Umap.size());
+    k_ilu_split<T><<<reduce_grid(int64_t(nl) + nu + n, 4), kBlock, 0, s>>>(
+        nl, d_Lmap.p, nu, d_Umap.p, n, d_diag.p, d_fac.p, Lval.p, Uval.p, Ud.p);
+    MI_HIP(hipGetLastError());
+    zero_pivots = -1; // not counted on this path
+    return MI_OK;
+}
+
+template <class T> int Ilu0<T>::setup_ssor(const Pattern& P, const double* vals, hipStream_t s)
+{
+    MI_TRY(prepare(P, s));
+    ssor = true;
+    const int64_t nnz = P.nnz;
+    MI_TRY(d_invD.alloc(static_cast<size_t>(std::max(n, 1))));
+    MI_TRY(Ud.alloc(static_cast<size_t>(std::max(n, 1))));
+    if (n > 0) {
+        k_ssor_diag<T><<<grid_for(n, kBlock), kBlock, 0, s>>>(n, d_diag.p, vals, Ud.p, d_invD.p);
+        MI_HIP(hipGetLastError());
+    }
+    if (seg) {
+        const int nl = int(Lmap.size()), nu = int(Umap.size());
+        // the diagonal slot of k_ilu_split is rewritten with the same a_ii
+        k_ilu_split<T><<<reduce_grid(int64_t(nl) + nu + n, 4), kBlock, 0, s>>>(
+            nl, d_Lmap.p, nu, d_Umap.p, n, d_diag.p, vals, Lval.p, Uval.p, Ud.p);
+    } else {
+        MI_TRY(d_lu.alloc(static_cast<size_t>(std::max<int64_t>(nnz, 1))));
+        if (nnz > 0)
+            k_to_t<T><<<reduce_grid(nnz, 4), kBlock, 0, s>>>(nnz, vals, d_lu.p);
+    }
+    MI_HIP(hipGetLastError());
     return MI_OK;
 }
 
@@ -389,6 +626,37 @@ template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_
     // when asked (the level count is in the hundreds for 3-D meshes)
     if (use_graph)
         return graphs.run(r, z, s, [&]() { return apply(r, z, s, false); });
+    if (ssor) {
+        if (seg) {
+            for (size_t t = 0; t + 1 < Lsb.size(); t += 2) {
+                const int nrb = Lsb[t + 1];
+                if (nrb == 0)
+                    continue;
+                StreamMat<T> L{n, nrb, Lblk.p + Lsb[t], Lrp.p, Lci.p, Lval.p};
+                k_ssor_lower_run<T><<<stream_grid(nrb), kBlock, 0, s>>>(L, r, d_invD.p, d_y.p);
+            }
+            for (size_t t = 0; t + 1 < Usb.size(); t += 2) {
+                const int nrb = Usb[t + 1];
+                if (nrb == 0)
+                    continue;
+                StreamMat<T> U{n, nrb, Ublk.p + Usb[t], Urp.p, Uci.p, Uval.p};
+                k_ssor_upper_run<T><<<stream_grid(nrb), kBlock, 0, s>>>(U, d_y.p, Ud.p, d_invD.p, z);
+            }
+        } else {
+            for (size_t l = 0; l + 1 < h_loff.size(); ++l) {
+                const int beg = h_loff[l], cnt = h_loff[l + 1] - beg;
+                k_ssor_lower_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(
+                    cnt, d_lrows.p + beg, rp_dev, d_lend.p, ci_dev, d_lu.p, d_invD.p, r, d_y.p);
+            }
+            for (size_t l = 0; l + 1 < h_uoff.size(); ++l) {
+                const int beg = h_uoff[l], cnt = h_uoff[l + 1] - beg;
+                k_ssor_upper_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(
+                    cnt, d_urows.p + beg, rp_dev, d_diag.p, ci_dev, d_lu.p, Ud.p, d_invD.p, d_y.p, z);
+            }
+        }
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
     if (seg && two) {
         // rows [b, n): z = (r - L r) / d ; rows [0, b): z = (r - U z) / d
         const int nl = Lsb[3], nu = Usb[3];
@@ -445,18 +713,43 @@ struct IluPrecond final : Precond
     Ilu0<double> f64;
     Ilu0<float> f32;
     bool has64 = false, has32 = false;
+    bool ssor = false; // SSOR(1) sweeps on A itself instead of ILU(0) factors
     int setup(mi_csr* A, bool need64, bool need32, hipStream_t s) override
     {
-        MI_TRY(A->sync_host());
+        const Pattern& P = *A->pat;
         has64 = has32 = false;
-        if (need64) {
-            MI_TRY(f64.setup(*A->pat, A->h_val, s));
-            has64 = true;
+        if (ssor) {
+            if (need64)
+                MI_TRY(f64.setup_ssor(P, A->val.p, s));
+            if (need32)
+                MI_TRY(f32.setup_ssor(P, A->val.p, s));
+            MI_HIP(hipStreamSynchronize(s));
+            has64 = need64;
+            has32 = need32;
+            return MI_OK;
         }
-        if (need32) {
-            MI_TRY(f32.setup(*A->pat, A->h_val, s));
-            has32 = true;
+        if (need64)
+            MI_TRY(f64.prepare(P, s));
+        if (need32)
+            MI_TRY(f32.prepare(P, s));
+        // run orderings factor on the device from the device values; the
+        // caller's order is factored on the host, the reference's way
+        const bool dev = (!need64 || f64.seg) && (!need32 || f32.seg);
+        if (dev) {
+            if (need64)
+                MI_TRY(f64.setup_device(P, A->val.p, s));
+            if (need32)
+                MI_TRY(f32.setup_device(P, A->val.p, s));
+            MI_HIP(hipStreamSynchronize(s));
+        } else {
+            MI_TRY(A->sync_host());
+            if (need64)
+                MI_TRY(f64.setup(P, A->h_val, s));
+            if (need32)
+                MI_TRY(f32.setup(P, A->h_val, s));
         }
+        has64 = need64;
+        has32 = need32;
         return MI_OK;
     }
     int apply64(const double* r, double* z, hipStream_t s) override
@@ -472,12 +765,19 @@ struct IluPrecond final : Precond
         return has32 ? f32.apply(r, z, s, false)
                      : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
     }
-    const char* name() const override { return "ilu0"; }
+    const char* name() const override { return ssor ? "ssor" : "ilu0"; }
     int levels() const override { return has64 ? f64.sweep_launches() : f32.sweep_launches(); }
 };
 } // namespace
 
 std::unique_ptr<Precond> make_ilu0() { return std::make_unique<IluPrecond>(); }
+
+std::unique_ptr<Precond> make_ssor()
+{
+    auto p = std::make_unique<IluPrecond>();
+    p->ssor = true;
+    return p;
+}
 
 } // namespace mi
 

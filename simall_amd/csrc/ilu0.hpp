@@ -50,18 +50,34 @@ template <class T> struct Ilu0
     DevBuf<T> Lval, Uval, Ud;
     std::vector<int> Lsb, Usb; // per run in sweep order: offset into L/Ublk, row blocks
     std::vector<int> Lmap, Umap; // position in the LU factor of each L / U entry
+    std::vector<int> lbeg, ubeg; // run boundaries in sweep order
+    std::vector<int> h_diag;
+    DevBuf<int> d_Lmap, d_Umap;
+    DevBuf<double> d_fac; // FP64 factor of the device factorization
 
+    // Pattern analysis (levels, runs, split patterns), once per pattern.
+    int prepare(const Pattern& P, hipStream_t s);
+    // Host IKJ factorization of host values (every ordering).
     int setup(const Pattern& P, const std::vector<double>& vals, hipStream_t s);
+    // Device factorization of device values, run by run (run orderings only:
+    // seg after prepare); the same operations as the host factorization.
+    int setup_device(const Pattern& P, const double* vals_device, hipStream_t s);
+    // SSOR(1) on A's own values (device): no factorization.
+    int setup_ssor(const Pattern& P, const double* vals_device, hipStream_t s);
+    bool ssor = false;
+    DevBuf<T> d_invD; // SSOR: 1 / a_ii (Ud holds a_ii)
     int apply(const T* r, T* z, hipStream_t s, bool use_graph = false);
     // launches per sweep: runs (segment sweeps) or dependency levels
     int sweep_launches() const
     {
         if (seg)
-            return two ? 1 : int(Lsb.size() / 2);
+            return (two && !ssor) ? 1 : int(Lsb.size() / 2);
         return h_loff.empty() ? 0 : int(h_loff.size()) - 1;
     }
 };
 
 std::unique_ptr<Precond> make_ilu0();
+// SSOR with omega = 1 (make_preconditioner's SsorPreconditioner(1.0)).
+std::unique_ptr<Precond> make_ssor();
 
 } // namespace mi

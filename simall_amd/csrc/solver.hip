@@ -252,13 +252,15 @@ bool want_reorder(const mi_solver* S)
     return r > 0 || (r < 0 && S->A->n() >= 100000);
 }
 
-// Ordering of the working matrix: multicolour for ILU(0) when asked (2) or by
-// default at large sizes (its triangular sweeps then have one level per
-// colour instead of hundreds), reverse Cuthill-McKee otherwise.
+// Ordering of the working matrix: multicolour for the triangular-sweep
+// preconditioners (ILU(0), SSOR) when asked (2) or by default at large sizes
+// (their sweeps then have one level per colour instead of hundreds), reverse
+// Cuthill-McKee otherwise.
 bool want_colors(const mi_solver* S)
 {
     const int r = S->cfg.reorder;
-    return r == 2 || (r < 0 && S->cfg.preconditioner == MI_PC_ILU);
+    const int pc = S->cfg.preconditioner;
+    return r == 2 || (r < 0 && (pc == MI_PC_ILU || pc == MI_PC_SSOR));
 }
 
 // Computes the reverse Cuthill-McKee permutation of A's pattern and the
@@ -347,7 +349,8 @@ int build(mi_solver* S)
             S->pc = make_ilu0();
             break;
         case MI_PC_SSOR:
-            return set_error(MI_ERR_UNSUPPORTED, "SSOR is not available on device");
+            S->pc = make_ssor();
+            break;
         default:
             S->pc = make_jacobi(); // Jacobi, None and unknown kinds (LinearSolvers.cpp:240-254)
         }

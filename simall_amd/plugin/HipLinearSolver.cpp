@@ -10,7 +10,9 @@ bool HipLinearSolver::supported(const solver::LinearSolverConfig& cfg)
 {
     // On device: CG, BiCGStab and GMRES(m) with Jacobi / None (Jacobi, as
     // make_preconditioner's default branch: src/solver/LinearSolvers.cpp:240-254),
-    // ILU(0) or SA-AMG.  TFQMR and SSOR stay on the host.
+    // ILU(0), SSOR or SA-AMG.  TFQMR stays on the host: its direction update
+    // (LinearSolvers.cpp:592-593) diverges unless the preconditioner is exact,
+    // so a device copy would only reproduce a divergent iteration.
     if (cfg.kind != solver::LinearSolverKind::CG && cfg.kind != solver::LinearSolverKind::BiCGSTAB &&
         cfg.kind != solver::LinearSolverKind::GMRES)
         return false;
@@ -19,6 +21,7 @@ bool HipLinearSolver::supported(const solver::LinearSolverConfig& cfg)
     return cfg.preconditioner == solver::PreconditionerKind::Jacobi ||
            cfg.preconditioner == solver::PreconditionerKind::None ||
            cfg.preconditioner == solver::PreconditionerKind::ILU ||
+           cfg.preconditioner == solver::PreconditionerKind::SSOR ||
            cfg.preconditioner == solver::PreconditionerKind::AMG;
 }
 

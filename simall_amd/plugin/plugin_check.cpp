@@ -108,10 +108,24 @@ int main(int argc, char** argv)
         cfg.kind = solver::LinearSolverKind::TFQMR; // no device path: host keeps its own
         if (g_factory(&cfg) != nullptr)
             return fail("TFQMR should not be claimed");
-        cfg.kind = solver::LinearSolverKind::CG;
-        cfg.preconditioner = solver::PreconditionerKind::SSOR;
-        if (g_factory(&cfg) != nullptr)
-            return fail("SSOR should not be claimed");
+        {
+            // tests/unit/solver/test_ssor_preconditioner.cpp:65-88 through ILinearSolver
+            solver::LinearSolverConfig c2;
+            c2.kind = solver::LinearSolverKind::CG;
+            c2.preconditioner = solver::PreconditionerKind::SSOR;
+            c2.maxIterations = 200;
+            c2.tolerance = 1e-10;
+            std::unique_ptr<solver::ILinearSolver> ss(g_factory(&c2));
+            if (!ss)
+                return fail("CG/SSOR factory");
+            const auto L50 = tridiag(50, -1.0, 2.0, -1.0);
+            util::aligned_vector<double> b50(50, 1.0), x50;
+            const int its = ss->solve(L50, b50, x50);
+            const double rs = residual(L50, b50, x50);
+            std::printf("CG/SSOR n=50: %d iterations, ||r|| %.3e\n", its, rs);
+            if (!(rs < 1e-7 && its > 0))
+                return fail("CG/SSOR solve through ILinearSolver");
+        }
         {
             // GMRES(20) + ILU(0) as the turbulence models configure it
             // (src/turbulence/KOmegaSST.cpp:32-37), on a nonsymmetric system

@@ -162,3 +162,63 @@ def test_ilu0_run_sweeps_match_oracle(gpu, momentum):
         xo, ito, _, _ = oracle.bicgstab(B, b[perm], oracle.PC_ILU, 1e-8, 1000)
         assert rc == capi.MI_OK and abs(st.iterations - ito) <= 2
 
+
+
+def test_ssor_apply_bit_exact(gpu, momentum):
+    """SSOR(1) sweeps in the caller's order against the oracle's serial
+    SsorPreconditioner (LinearSolvers.cpp:163-236): exact equality."""
+    A, b = momentum
+    s = solver(A, preconditioner=capi.MI_PC_SSOR)
+    r = np.random.default_rng(4).standard_normal(A.n)
+    z = s.precond_apply(r)
+    assert np.array_equal(z, oracle.ssor_apply(A, r, 1.0))
+    # identity: SSOR(1) returns r (tests/unit/solver/test_ssor_preconditioner.cpp:44-63)
+    I = oracle.Matrix.from_dense(np.eye(5))
+    si = solver(I, preconditioner=capi.MI_PC_SSOR)
+    assert np.array_equal(si.precond_apply(np.arange(1.0, 6.0)), np.arange(1.0, 6.0))
+
+
+def test_cg_ssor_reference_case(gpu):
+    # tests/unit/solver/test_ssor_preconditioner.cpp:65-88
+    rp, ci, va = tridiag_csr(50, -1.0, 2.0, -1.0)
+    A = oracle.Matrix(rp, ci, va)
+    s = solver(A, kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_SSOR, tolerance=1e-10,
+               max_iterations=200)
+    x, st, rc = s.solve(np.ones(50))
+    xo, ito, _, _ = oracle.cg(A, np.ones(50), oracle.PC_SSOR, 1e-10, 200)
+    assert rc == capi.MI_OK and st.iterations > 0 and abs(st.iterations - ito) <= 1
+    assert np.linalg.norm(np.ones(50) - A.dense() @ x) < 1e-7
+
+
+@pytest.mark.parametrize("prec", [capi.MI_PRECISION_FP64, capi.MI_PRECISION_MIXED])
+def test_bicgstab_ssor_parity(gpu, momentum, prec):
+    A, b = momentum
+    s = solver(A, preconditioner=capi.MI_PC_SSOR, precision=prec, true_residual=1)
+    x, st, rc = s.solve(b)
+    xo, ito, _, _ = oracle.bicgstab(A, b, oracle.PC_SSOR, 1e-8, 1000)
+    rel = np.linalg.norm(b - oracle.spmv(A, x)) / np.linalg.norm(b)
+    print(f"SSOR-BiCGStab prec {prec}: GPU {st.iterations} its, oracle {ito}, rel {rel:.2e}")
+    assert rc == capi.MI_OK and rel <= 1e-8
+    if prec == capi.MI_PRECISION_FP64:
+        assert abs(st.iterations - ito) <= 2
+
+
+def test_ssor_run_sweeps_red_black(gpu, momentum):
+    """A red-black ordered matrix: SSOR swept run by run on the CSR-stream
+    engine agrees with the oracle's serial sweeps to rounding."""
+    import scipy.sparse as sp
+
+    A, b = momentum
+    N = 16
+    i = np.arange(A.n)
+    parity = (i % N + (i // N) % N + i // (N * N)) % 2
+    perm = np.concatenate([np.flatnonzero(parity == 0), np.flatnonzero(parity == 1)])
+    S = sp.csr_matrix((A.values, A.colIdx, A.rowPtr), shape=(A.n, A.n))[perm][:, perm].tocsr()
+    S.sort_indices()
+    B = oracle.Matrix(S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data.copy())
+    s = solver(B, preconditioner=capi.MI_PC_SSOR)
+    assert s.info()["levels"] == 2
+    r = np.random.default_rng(5).standard_normal(B.n)
+    z = s.precond_apply(r)
+    zo = oracle.ssor_apply(B, r, 1.0)
+    assert np.linalg.norm(z - zo) <= 1e-13 * np.linalg.norm(zo)

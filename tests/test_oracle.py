@@ -196,6 +196,48 @@ def test_gmres_matches_dense_solve(pc):
     assert np.allclose(x, np.linalg.solve(D, b), rtol=1e-8, atol=1e-9)
 
 
+def test_tfqmr_reference_known_answers():
+    # tests/unit/solver/test_tfqmr_solver.cpp:90-111: upwind convection-diffusion,
+    # Pe = 5, ILU (exact on a tridiagonal: one iteration)
+    n, Pe = 40, 5.0
+    C = tridiag(n, -1.0 - Pe, 2.0 + Pe, -1.0)
+    x, it, last = oracle.tfqmr(oracle.Matrix.from_dense(C), np.ones(n), oracle.PC_ILU, 1e-8, 500)
+    assert it == 1 and np.linalg.norm(C @ x - 1.0) < 1e-12
+
+
+def test_tfqmr_restatement_keeps_the_reference_direction_update():
+    """TfqmrSolver builds the next half-step vector from the CGS direction
+    (u1 = M^-1 r + beta (u2 + beta u1), LinearSolvers.cpp:592-593) where
+    TFQMR takes u = r + beta q.  The iteration is then exact only with an
+    exact preconditioner, and the reference's own SPD-Laplacian case
+    (test_tfqmr_solver.cpp:66-88, Jacobi, ||r|| < 1e-5) does not converge:
+    the restatement must show the same."""
+    n = 30
+    D = tridiag(n, -1.0, 2.0, -1.0)
+    x, it, last = oracle.tfqmr(oracle.Matrix.from_dense(D), np.ones(n), oracle.PC_JACOBI, 1e-8, 500)
+    assert it < 500 and last > 1e-2  # stopped by a breakdown, far from converged
+    assert not np.isfinite(np.linalg.norm(D @ x - 1.0)) or np.linalg.norm(D @ x - 1.0) > 1e-5
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_tfqmr_exact_preconditioner_solves_in_one_step(seed):
+    # ILU(0) of a (nonsymmetric) tridiagonal matrix is its exact LU
+    rng = np.random.default_rng(seed)
+    n = 25
+    C = tridiag(n, 0.0, 0.0, 0.0)
+    C += np.diag(rng.uniform(3, 4, n)) + np.diag(rng.uniform(-1, 0, n - 1), -1) + np.diag(rng.uniform(-1, 0, n - 1), 1)
+    b = rng.standard_normal(n)
+    x, it, last = oracle.tfqmr(oracle.Matrix.from_dense(C), b, oracle.PC_ILU, 1e-10, 50)
+    assert it == 1 and np.allclose(x, np.linalg.solve(C, b), rtol=1e-10, atol=1e-12)
+
+
+def test_tfqmr_zero_rhs_and_iteration_limit():
+    D = tridiag(20, -1.0, 3.0, -1.2)
+    A = oracle.Matrix.from_dense(D)
+    x, it, last = oracle.tfqmr(A, np.zeros(20))
+    assert it == 0 and not np.any(x)
+
+
 def test_none_preconditioner_acts_as_jacobi():
     # make_preconditioner's default branch (src/solver/LinearSolvers.cpp:250-252)
     D = random_sparse(30, 0.2, 5, diag_boost=1.0, symmetric=True)
