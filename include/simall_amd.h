@@ -239,12 +239,14 @@ MI_API int mi_solver_destroy(mi_solver* s);
  * with its own ILinearSolver::solve on an identical pattern
  * (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232); one
  * mi_batch_solve replaces those n_eq calls.  Per equation the iteration is
- * Jacobi-PCG from x = 0 (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452)
- * in FP32 with FP64 dot products, norms and iterate, refined on the FP64 true
- * residual until ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations
- * converge and drop out independently.  cfg (nullable: defaults) supplies
- * kind = CG, preconditioner = Jacobi or None, tolerance, max_iterations (per
- * equation, all sweeps), inner_tolerance, max_outer and check_interval.
+ * Jacobi-PCG from x = 0 (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452),
+ * or Jacobi-preconditioned BiCGStab (BiCGSTABSolver::solve, :360-404) for
+ * nonsymmetric (convective) equations, in FP32 with FP64 dot products, norms
+ * and iterate, refined on the FP64 true residual until
+ * ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations converge and drop out
+ * independently.  cfg (nullable: defaults) supplies kind = CG or BiCGStab,
+ * preconditioner = Jacobi or None, tolerance, max_iterations (per equation,
+ * all sweeps), inner_tolerance, max_outer and check_interval.
  * Vectors are equation-major: b[e * n + i].  The rows per workgroup of the
  * batched SpMV are chosen by timing candidates at creation (mi_batch_tuning). */
 typedef struct mi_batch mi_batch;

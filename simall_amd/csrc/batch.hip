@@ -17,14 +17,16 @@
 // shared pattern.
 //
 // Arithmetic: Jacobi-PCG per equation (CGSolver::solve,
-// src/solver/LinearSolvers.cpp:420-452) with FP32 values and vectors and FP64
-// dot products, norms and iterate (accumulated in FP64 as in the mixed
-// single-equation solver); an FP64 true residual with the FP64 values
-// refines each equation until ||b_e - A_e x_e|| <= tol ||b_e||.  Each
-// equation has its own scalars and convergence flag.  One inner iteration is
-// five launches (batched SpMV + p.Ap, finalize, update, finalize, direction)
-// captured once into a HIP graph of `chunk` iterations that the host replays
-// between polls.
+// src/solver/LinearSolvers.cpp:420-452), or for nonsymmetric equations
+// (convection) Jacobi-preconditioned BiCGStab (BiCGSTABSolver::solve,
+// :360-404), with FP32 values and vectors and FP64 dot products, norms and
+// iterate (accumulated in FP64 as in the mixed single-equation solver); an
+// FP64 true residual with the FP64 values refines each equation until
+// ||b_e - A_e x_e|| <= tol ||b_e||.  Each equation has its own scalars and
+// convergence flag.  One inner CG iteration is five launches (batched SpMV +
+// p.Ap, finalize, update, finalize, direction), one BiCGStab iteration nine;
+// they are captured once into a HIP graph of `chunk` iterations that the
+// host replays between polls.
 #include "csr.hpp"
 #include "krylov.hpp"
 #include "reorder.hpp"
@@ -45,10 +47,17 @@ constexpr int kWaves = kBlock / kWave;
 struct BEq
 {
     double rz, alpha, beta, r0, bnorm, tol_in, rel, last;
-    int iter, total, outer, done, fin, status, pad[2];
+    double rho, omega; // BiCGStab: rho of the previous iteration (rz holds the current), omega
+    int iter, total, outer, done, fin, status, sconv, pad;
 };
 
-enum BOp : int { B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4, B_P = 5 };
+// Stages.  B_BI_*: BiCGStab; B_BI_T skips equations whose s already met the
+// inner tolerance (sconv).
+enum BOp : int {
+    B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4, B_P = 5,
+    B_BI_INIT = 6, B_BI_P = 7, B_BI_ALPHA = 8, B_BI_S = 9, B_BI_T = 10, B_BI_OMEGA = 11,
+    B_BI_UPDATE = 12
+};
 
 struct BArgs
 {
@@ -63,9 +72,13 @@ struct BArgs
     const double* b;   // [E][n]
     double* x;
     double* rt;
-    float* r;
+    float* r;          // BiCGStab: r, then s in place
     float* p;
-    float* Ap;
+    float* Ap;         // BiCGStab: v
+    float* rh;         // BiCGStab shadow residual, y = D^-1 p, z = D^-1 s, t = A z
+    float* y;
+    float* z;
+    float* t;
     BEq* st;           // nullptr: every equation active (autotuner)
     double* partials;  // [2][E][G]
 };
@@ -126,6 +139,8 @@ __device__ __forceinline__ int b_list(const BArgs& a, int op, int* act)
             bool on = true;
             if (op == B_TRUE)
                 on = !a.st[e].fin;
+            else if (op == B_BI_T)
+                on = b_iterating(a, e) && !a.st[e].sconv;
             else if (op != B_BNORM)
                 on = b_iterating(a, e);
             if (on)
@@ -137,27 +152,36 @@ __device__ __forceinline__ int b_list(const BArgs& a, int op, int* act)
     return cnt;
 }
 
-// Ap_e = A_e p_e for the iterating equations (FP32); partials of p_e.Ap_e.
-__global__ __launch_bounds__(kBlock) void k_bspmv_dot(BArgs a)
+// Batched SpMV out_e = A_e in_e (FP32) for the equations of the stage, with
+// FP64 partials of per-row products:
+//   B_ALPHA     Ap = A p;  p.Ap                (CG)
+//   B_BI_ALPHA  v = A y;   rh.v                (BiCGStab)
+//   B_BI_T      t = A z;   t.s, t.t            (BiCGStab, s held in r)
+template <int OP>
+__global__ __launch_bounds__(kBlock) void k_bspmv(BArgs a)
 {
     __shared__ float prod[kStreamNnz];
     __shared__ int act[kMaxEq];
     __shared__ double wsum[2][kMaxEq][kWaves];
     BSums S{wsum};
     S.clear(a.E);
-    const int nact = b_list(a, B_ALPHA, act);
+    const int nact = b_list(a, OP, act);
     const int G = gridDim.x;
     const int lb = xcd_logical_block(blockIdx.x, G);
     const int64_t units = int64_t(nact) * a.nrb;
     const int64_t per = (units + G - 1) / G;
     const int64_t u0 = lb * per, u1 = min(units, u0 + per);
     const int tid = threadIdx.x;
+    const float* vin = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
+    float* vout = OP == B_BI_T ? a.t : a.Ap;
+    const float* vw = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
     for (int64_t u = u0; u < u1; ++u) {
         const int e = act[u / a.nrb];
         const int rb = int(u % a.nrb);
         const float* __restrict__ ve = a.v32 + size_t(e) * a.nnz;
-        const float* __restrict__ pe = a.p + size_t(e) * a.n;
-        float* __restrict__ Ae = a.Ap + size_t(e) * a.n;
+        const float* __restrict__ pe = vin + size_t(e) * a.n;
+        const float* __restrict__ we = vw + size_t(e) * a.n;
+        float* __restrict__ Ae = vout + size_t(e) * a.n;
         const int r0 = a.blk[rb], r1 = a.blk[rb + 1];
         const int k0 = a.rp[r0], k1 = a.rp[r1];
         if (k1 - k0 > kStreamNnz) {
@@ -167,22 +191,25 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_dot(BArgs a)
                 s += ve[k] * pe[a.ci[k]];
             prod[tid] = s;
             __syncthreads();
-            double d = 0.0;
+            double d0 = 0.0, d1 = 0.0;
             if (tid == 0) {
                 float t = 0.f;
                 for (int j = 0; j < kBlock; ++j)
                     t += prod[j];
                 Ae[r0] = t;
-                d = double(pe[r0]) * double(t);
+                d0 = double(we[r0]) * double(t);
+                d1 = double(t) * double(t);
             }
-            S.add(0, e, d);
+            S.add(0, e, d0);
+            if (OP == B_BI_T)
+                S.add(1, e, d1);
             __syncthreads();
             continue;
         }
         const int i = r0 + tid;
-        float pi = 0.f;
+        float wi = 0.f;
         if (i < r1)
-            pi = pe[i];
+            wi = we[i];
         int c[kLoadsPerThread];
         float v[kLoadsPerThread];
 #pragma unroll
@@ -196,19 +223,22 @@ __global__ __launch_bounds__(kBlock) void k_bspmv_dot(BArgs a)
             if (c[j] >= 0)
                 prod[tid + j * kBlock] = v[j] * pe[c[j]];
         __syncthreads();
-        double d = 0.0;
+        double d0 = 0.0, d1 = 0.0;
         if (i < r1) {
             float s = 0.f;
             const int end = a.rp[i + 1] - k0;
             for (int k = a.rp[i] - k0; k < end; ++k)
                 s += prod[k];
             Ae[i] = s;
-            d = double(pi) * double(s);
+            d0 = double(wi) * double(s);
+            d1 = double(s) * double(s);
         }
-        S.add(0, e, d);
+        S.add(0, e, d0);
+        if (OP == B_BI_T)
+            S.add(1, e, d1);
         __syncthreads();
     }
-    S.store(a, 1);
+    S.store(a, OP == B_BI_T ? 2 : 1);
 }
 
 // rt_e = b_e - A_e x_e in FP64 for the unfinished equations, r_e = float(rt_e);
@@ -321,18 +351,64 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
                 s0 += double(ri) * double(ri);
                 s1 += double(ri) * double(zi);
             }
-        } else { // B_P
+        } else if (op == B_P) {
             const float be = float(a.st[e].beta);
             for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
                 a.p[off + i] = a.invD[off + i] * a.r[off + i] + be * a.p[off + i];
             continue;
+        } else if (op == B_BI_INIT) {
+            // rh = r; p = v = 0; partials r.r, rh.r (BiCGSTABSolver::solve, :366-372)
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                const float ri = a.r[off + i];
+                a.rh[off + i] = ri;
+                a.p[off + i] = 0.f;
+                a.Ap[off + i] = 0.f;
+                s0 += double(ri) * double(ri);
+            }
+            s1 = s0;
+        } else if (op == B_BI_P) {
+            // p = r + beta (p - omega v); y = D^-1 p (:380-382)
+            const float be = float(a.st[e].beta), om = float(a.st[e].omega);
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                const float pi = a.r[off + i] + be * (a.p[off + i] - om * a.Ap[off + i]);
+                a.p[off + i] = pi;
+                a.y[off + i] = a.invD[off + i] * pi;
+            }
+            continue;
+        } else if (op == B_BI_S) {
+            // s = r - alpha v (in r); z = D^-1 s; partials s.s (:384-386)
+            const float al = float(a.st[e].alpha);
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                const float si = a.r[off + i] - al * a.Ap[off + i];
+                a.r[off + i] = si;
+                a.z[off + i] = a.invD[off + i] * si;
+                s0 += double(si) * double(si);
+            }
+        } else { // B_BI_UPDATE: x += alpha y + omega z; r = s - omega t (:393-397)
+            const BEq& q = a.st[e];
+            const double al = q.alpha;
+            if (q.sconv) {
+                // s met the tolerance: x += alpha y and stop (:387-390)
+                for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
+                    a.x[off + i] += al * double(a.y[off + i]);
+                continue;
+            }
+            const double om = q.omega;
+            const float omf = float(om);
+            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
+                a.x[off + i] += al * double(a.y[off + i]) + om * double(a.z[off + i]);
+                const float ri = a.r[off + i] - omf * a.t[off + i];
+                a.r[off + i] = ri;
+                s0 += double(ri) * double(ri);
+                s1 += double(a.rh[off + i]) * double(ri);
+            }
         }
         S.add(0, e, s0);
-        if (op != B_BNORM)
+        if (op != B_BNORM && op != B_BI_S)
             S.add(1, e, s1);
     }
-    if (op != B_P)
-        S.store(a, op == B_BNORM ? 1 : 2);
+    if (op != B_P && op != B_BI_P)
+        S.store(a, (op == B_BNORM || op == B_BI_S) ? 1 : 2);
 }
 
 struct BFinArgs
@@ -347,7 +423,10 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
                                                  int op, BFinArgs f)
 {
     const int e = blockIdx.x;
-    const int nv = (op == B_INIT || op == B_UPDATE) ? 2 : 1;
+    const int nv = (op == B_INIT || op == B_UPDATE || op == B_BI_INIT || op == B_BI_OMEGA ||
+                    op == B_BI_UPDATE)
+                       ? 2
+                       : 1;
     double tot[2] = {0.0, 0.0};
     for (int v = 0; v < nv; ++v)
         for (int g = threadIdx.x; g < f.G; g += kBlock)
@@ -422,6 +501,59 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
             s.rz = tot[1];
         }
         break;
+    case B_BI_INIT:
+        if (s.done)
+            break;
+        s.r0 = sqrt(tot[0]);
+        s.sconv = 0;
+        s.rho = 1.0;
+        s.alpha = 1.0;
+        s.omega = 1.0;
+        s.rz = tot[1]; // rho_new = rh.r
+        if (!(s.r0 > 0.0) || s.rz == 0.0)
+            s.done = 1;
+        else
+            s.beta = (s.rz / s.rho) * (s.alpha / s.omega);
+        break;
+    case B_BI_ALPHA:
+        if (s.done)
+            break;
+        if (tot[0] != 0.0 && isfinite(tot[0])) {
+            s.alpha = s.rz / tot[0];
+        } else {
+            s.alpha = 0.0;
+            s.done = 1; // breakdown: the outer true residual decides
+        }
+        break;
+    case B_BI_S:
+        if (s.done)
+            break;
+        if (sqrt(tot[0]) / s.r0 < s.tol_in)
+            s.sconv = 1;
+        break;
+    case B_BI_OMEGA:
+        if (s.done || s.sconv)
+            break;
+        s.omega = tot[0] / fmax(tot[1], 1e-30);
+        break;
+    case B_BI_UPDATE:
+        if (s.done)
+            break;
+        ++s.iter;
+        ++s.total;
+        if (s.sconv) {
+            s.done = 1;
+            break;
+        }
+        s.last = sqrt(tot[0]) / s.r0;
+        if (s.last < s.tol_in || s.total >= f.max_iter || tot[1] == 0.0 || s.omega == 0.0) {
+            s.done = 1;
+        } else {
+            s.rho = s.rz;
+            s.rz = tot[1];
+            s.beta = (s.rz / s.rho) * (s.alpha / s.omega);
+        }
+        break;
     default:
         break;
     }
@@ -492,6 +624,8 @@ struct mi_batch
     int E = 0, G = 8;
     int64_t n = 0, nnz = 0;
     DevBuf<float> v32, invD, r, p, Ap;
+    DevBuf<float> rh, y, z, t; // BiCGStab vectors, allocated on its first solve
+    int graph_kind = -1;
     DevBuf<double> v64, b, x, rt, partials;
     DevBuf<BEq> st;
     BEq* h_st = nullptr;
@@ -529,6 +663,10 @@ struct mi_batch
         a.r = r.p;
         a.p = p.p;
         a.Ap = Ap.p;
+        a.rh = rh.p;
+        a.y = y.p;
+        a.z = z.p;
+        a.t = t.p;
         a.st = st.p;
         a.partials = partials.p;
         return a;
@@ -549,7 +687,7 @@ int fin(mi_batch* B, int op, const BFinArgs& f)
 
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f)
 {
-    k_bspmv_dot<<<B->G, kBlock, 0, B->stream>>>(a);
+    k_bspmv<B_ALPHA><<<B->G, kBlock, 0, B->stream>>>(a);
     MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_ALPHA, f));
     k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_UPDATE);
@@ -558,6 +696,25 @@ int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f)
     k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_P);
     MI_HIP(hipGetLastError());
     return MI_OK;
+}
+
+// One BiCGStab iteration for every iterating equation (nine launches).
+int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f)
+{
+    hipStream_t s = B->stream;
+    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_P);
+    k_bspmv<B_BI_ALPHA><<<B->G, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    MI_TRY(fin(B, B_BI_ALPHA, f));
+    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_S);
+    MI_HIP(hipGetLastError());
+    MI_TRY(fin(B, B_BI_S, f));
+    k_bspmv<B_BI_T><<<B->G, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    MI_TRY(fin(B, B_BI_OMEGA, f));
+    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_UPDATE);
+    MI_HIP(hipGetLastError());
+    return fin(B, B_BI_UPDATE, f);
 }
 
 // Autotuner: workgroups of the persistent batched launches, timed on this
@@ -576,12 +733,12 @@ int autotune(mi_batch* B)
     for (int g : {512, 1024, 2048, 4096}) {
         if (g > kBatchMaxGrid)
             break;
-        k_bspmv_dot<<<g, kBlock, 0, B->stream>>>(a);
+        k_bspmv<B_ALPHA><<<g, kBlock, 0, B->stream>>>(a);
         MI_HIP(hipGetLastError());
         MI_HIP(hipEventRecord(e0, B->stream));
         const int reps = 3;
         for (int r = 0; r < reps; ++r)
-            k_bspmv_dot<<<g, kBlock, 0, B->stream>>>(a);
+            k_bspmv<B_ALPHA><<<g, kBlock, 0, B->stream>>>(a);
         MI_HIP(hipEventRecord(e1, B->stream));
         MI_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -600,8 +757,9 @@ int autotune(mi_batch* B)
 
 int check_cfg(const mi_solver_cfg* c)
 {
-    if (c->kind != MI_SOLVER_CG)
-        return set_error(MI_ERR_UNSUPPORTED, "batched solve: only CG (kind %d given)", c->kind);
+    if (c->kind != MI_SOLVER_CG && c->kind != MI_SOLVER_BICGSTAB)
+        return set_error(MI_ERR_UNSUPPORTED, "batched solve: CG or BiCGStab (kind %d given)",
+                         c->kind);
     if (c->preconditioner != MI_PC_JACOBI && c->preconditioner != MI_PC_NONE)
         return set_error(MI_ERR_UNSUPPORTED, "batched solve: Jacobi preconditioner only");
     if (!(c->tolerance > 0.0) || c->max_iterations < 0)
@@ -640,13 +798,20 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     }
     k_bfill<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(int64_t(m), B->x.p);
     MI_HIP(hipGetLastError());
+    const bool bicg = c.kind == MI_SOLVER_BICGSTAB;
+    if (bicg && !B->rh.p) {
+        const size_t nE = size_t(n) * size_t(E);
+        for (DevBuf<float>* v : {&B->rh, &B->y, &B->z, &B->t})
+            MI_TRY(v->alloc(nE));
+    }
     const BArgs a = B->args();
     k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BNORM);
     MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_BNORM, f));
     const int chunk = c.check_interval > 0 ? c.check_interval : 8;
     const bool use_graph = c.use_graph != 0;
-    if (use_graph && (!B->graph || B->graph_chunk != chunk ||
+    auto iteration = [&]() { return bicg ? one_iteration_bicg(B, a, f) : one_iteration(B, a, f); };
+    if (use_graph && (!B->graph || B->graph_chunk != chunk || B->graph_kind != c.kind ||
                       std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0)) {
         if (B->graph) {
             (void)hipGraphExecDestroy(B->graph);
@@ -657,7 +822,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         int rc = MI_OK;
         for (int it = 0; it < chunk && rc == MI_OK; ++it)
-            rc = one_iteration(B, a, f);
+            rc = iteration();
         const hipError_t ec = hipStreamEndCapture(s, &g);
         if (rc != MI_OK || ec != hipSuccess || !g) {
             if (g)
@@ -668,6 +833,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         (void)hipGraphDestroy(g);
         MI_HIP(ei);
         B->graph_chunk = chunk;
+        B->graph_kind = c.kind;
         B->graph_f = f;
     }
     for (;;) {
@@ -681,15 +847,15 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             all_fin = all_fin && B->h_st[e].fin;
         if (all_fin)
             break;
-        k_bvec<<<B->G, kBlock, 0, s>>>(a, B_INIT);
+        k_bvec<<<B->G, kBlock, 0, s>>>(a, bicg ? B_BI_INIT : B_INIT);
         MI_HIP(hipGetLastError());
-        MI_TRY(fin(B, B_INIT, f));
+        MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f));
         for (;;) {
             if (use_graph) {
                 MI_HIP(hipGraphLaunch(B->graph, s));
             } else {
                 for (int it = 0; it < chunk; ++it)
-                    MI_TRY(one_iteration(B, a, f));
+                    MI_TRY(iteration());
             }
             MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(E), hipMemcpyDeviceToHost,
                                   s));

@@ -143,3 +143,76 @@ def test_batch_reordered_path(gpu):
     for e in range(4):
         M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
         assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+
+
+def _convective_systems(N, n_eq, seed=3):
+    """Convection-diffusion transport: the momentum assembly with the
+    Taylor-Green mass flux, a pseudo-transient diagonal and a diffusivity per
+    equation: nonsymmetric systems on one pattern."""
+    L = 2 * np.pi
+    m = wl.Mesh(N, N, N, extent=(L, L, L), perturb=0.1, node_seed=seed)
+    bc = [wl.BC_WALL] * 7
+    dm = capi.DeviceMesh(m, bc)
+    A = dm.create_matrix()
+    rhs = capi.DeviceArray(m.n_cells)
+    U = wl.taylor_green(m)
+    phi = capi.DeviceArray.from_host(oracle.face_flux(m, bc, 1.0, U))
+    dt = capi.DeviceArray.from_host(wl.loguniform(seed, -1, 0, m.n_cells))
+    uold = capi.DeviceArray.from_host(np.zeros(m.n_cells))
+    rp, ci = A.pattern()
+    gam = wl.loguniform(seed + 1, -3, -1, n_eq)
+    vals = []
+    for e in range(n_eq):
+        dm.assemble_momentum(A, rhs, 0, float(gam[e]), 1.0, phi, d_dt=dt, d_Uold=uold)
+        vals.append(A.values())
+    bs = np.stack([wl.normal(seed + 10 + e, m.n_cells) for e in range(n_eq)])
+    return A, rp, ci, vals, bs
+
+
+def test_batch_bicgstab_nonsymmetric_matches_oracle(gpu):
+    A, rp, ci, vals, b = _convective_systems(16, 6)
+    M0 = sp.csr_matrix((vals[0], ci, rp), shape=(A.n,) * 2)
+    assert abs(M0 - M0.T).max() > 1e-3  # nonsymmetric
+    B = capi.Batch(A, len(vals))
+    B.set_values(np.stack(vals))
+    b[3] = 0.0
+    x, its, rel, rc = B.solve(b, kind=capi.MI_SOLVER_BICGSTAB, tolerance=1e-8, max_iterations=2000)
+    print(f"batched BiCGStab iterations {its.tolist()}")
+    assert rc == capi.MI_OK and np.all(rel <= 1e-8)
+    assert its[3] == 0 and np.all(x[3] == 0.0)
+    for e in range(len(vals)):
+        if e == 3:
+            continue
+        M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
+        assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+        xo, ito, _, _ = oracle.bicgstab(oracle.Matrix(rp, ci, vals[e]), b[e], oracle.PC_JACOBI,
+                                        1e-12, 5000)
+        assert np.linalg.norm(x[e] - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_batch_bicgstab_graph_replays_match_eager(gpu):
+    A, rp, ci, vals, b = _convective_systems(12, 4, seed=5)
+    B = capi.Batch(A, len(vals))
+    B.set_values(np.stack(vals))
+    kw = dict(kind=capi.MI_SOLVER_BICGSTAB, tolerance=1e-8, check_interval=3)
+    xe, itse, _, _ = B.solve(b, use_graph=0, **kw)
+    xg, itsg, _, _ = B.solve(b, use_graph=1, **kw)
+    xg2, itsg2, _, _ = B.solve(b, use_graph=1, **kw)
+    assert np.array_equal(itse, itsg) and np.array_equal(itsg, itsg2)
+    assert np.array_equal(xe, xg) and np.array_equal(xg, xg2)
+    # the same batch then solved with CG keeps working (graph rebuilt per kind)
+    x, its, rel, rc = B.solve(b, kind=capi.MI_SOLVER_CG, tolerance=1e-6, max_iterations=3000)
+    assert rc in (capi.MI_OK, capi.MI_ERR_NOT_CONVERGED)
+
+
+def test_batch_bicgstab_reordered_path(gpu):
+    """48^3 = 110592 rows: the batch reorders its pattern (RCM)."""
+    A, rp, ci, vals, b = _convective_systems(48, 3, seed=7)
+    B = capi.Batch(A, len(vals))
+    B.set_values(np.stack(vals))
+    x, its, rel, rc = B.solve(b, kind=capi.MI_SOLVER_BICGSTAB, tolerance=1e-8, max_iterations=3000)
+    print(f"48^3 batched BiCGStab iterations {its.tolist()}")
+    assert rc == capi.MI_OK
+    for e in range(len(vals)):
+        M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
+        assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
