@@ -432,6 +432,60 @@ MI_API int mi_flow_iterate(mi_flow* f, mi_flow_residuals* res);
 MI_API int mi_flow_advance_time_step(mi_flow* f, int n_inner, mi_flow_residuals* res);
 MI_API int mi_flow_destroy(mi_flow* f);
 
+/* -------------------------------------------------------------- checkpoint
+ * The SIMALLCP file of Checkpoint::write / read (src/io/Checkpoint.hpp:27-57,
+ * src/io/Checkpoint.cpp:67-230): a header (magic, version 1, flags,
+ * timestamp), CheckpointMeta, the mesh section and the field section; files
+ * written here are read by the reference and the reverse.  The mesh section
+ * needs the node data meshing::Mesh keeps besides mi_mesh_soa: */
+typedef struct {
+    int64_t n_nodes;
+    const double* x;
+    const double* y;
+    const double* z;
+    const int32_t* faceNodeOffsets;  /* [n_faces + 1] */
+    const uint64_t* faceNodeIndices; /* [faceNodeOffsets[n_faces]] */
+} mi_mesh_nodes;
+/* CheckpointMeta (Checkpoint.hpp:35-40). */
+typedef struct {
+    int32_t iteration;
+    double time;
+    uint64_t settings_hash;
+} mi_checkpoint_meta;
+typedef struct {
+    const char* name;
+    const double* data; /* [n_cells], host */
+} mi_named_field;
+typedef struct {
+    const char* name;
+    const double* x; /* [n_cells] each, host */
+    const double* y;
+    const double* z;
+} mi_named_vector;
+/* Writes <path>.tmp and renames it over <path> (host arrays). */
+MI_API int mi_checkpoint_write(const char* path, const mi_mesh_soa* mesh,
+                               const mi_mesh_nodes* nodes, int n_scalars,
+                               const mi_named_field* scalars, int n_vectors,
+                               const mi_named_vector* vectors, const mi_checkpoint_meta* meta);
+/* Reads a checkpoint into host memory: MI_ERR_BAD_ARG on a bad magic or a
+ * truncated file, MI_ERR_UNSUPPORTED on a newer version. */
+typedef struct mi_checkpoint mi_checkpoint;
+MI_API int mi_checkpoint_open(const char* path, mi_checkpoint** out);
+MI_API int mi_checkpoint_info(const mi_checkpoint* c, mi_checkpoint_meta* meta, int64_t* n_cells,
+                              int64_t* n_faces, int64_t* n_nodes);
+/* Copies up to capacity values of a named field; count receives its length. */
+MI_API int mi_checkpoint_scalar(const mi_checkpoint* c, const char* name, double* out,
+                                int64_t capacity, int64_t* count);
+MI_API int mi_checkpoint_vector(const mi_checkpoint* c, const char* name, double* x, double* y,
+                                double* z, int64_t capacity, int64_t* count);
+MI_API int mi_checkpoint_close(mi_checkpoint* c);
+/* The device-resident flow state as fields "p" and "U" (the FieldRegistry
+ * names of SimpleAlgorithm), with the caller's mesh arrays. */
+MI_API int mi_flow_checkpoint_write(mi_flow* f, const char* path, const mi_mesh_soa* mesh,
+                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta);
+/* Loads "p" and "U" from a checkpoint of a mesh with this flow's cell count. */
+MI_API int mi_flow_checkpoint_read(mi_flow* f, const char* path, mi_checkpoint_meta* meta);
+
 #ifdef __cplusplus
 }
 #endif

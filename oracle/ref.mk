@@ -9,7 +9,7 @@ SRCS := $(REF)/src/meshing/CartesianMesher.cpp $(REF)/src/meshing/Connectivity.c
         $(REF)/src/meshing/MeshStorage.cpp $(REF)/src/solver/SimpleAlgorithm.cpp \
         $(REF)/src/solver/Gradient.cpp $(REF)/src/solver/PisoAlgorithm.cpp \
         $(REF)/src/adjoint/DiscreteAdjoint.cpp \
-        $(REF)/src/solver/AMGPreconditioner.cpp
+        $(REF)/src/solver/AMGPreconditioner.cpp $(REF)/src/io/Checkpoint.cpp
 # -ffp-contract=off: x86-64 code without fused multiply-add, as the
 # reference's default build.
 FLAGS := -std=c++20 -O2 -ffp-contract=off -fPIC -I$(REF)/src -I$(REF) -w

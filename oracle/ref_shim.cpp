@@ -12,8 +12,10 @@
 // Ruge-Stueben AMGPreconditioner (src/solver/AMGPreconditioner.cpp),
 // LeastSquaresGradient (src/solver/Gradient.cpp) and the SIMPLE / PISO outer
 // iterations (SimpleAlgorithm.cpp, PisoAlgorithm.cpp) with a dense direct
-// solve standing in for the linear solvers.
+// solve standing in for the linear solvers, and the SIMALLCP checkpoint
+// writer and reader (src/io/Checkpoint.cpp).
 #include "adjoint/DiscreteAdjoint.hpp"
+#include "io/Checkpoint.hpp"
 #include "meshing/CartesianMesher.hpp"
 #include "solver/AMGPreconditioner.hpp"
 #include "solver/Gradient.hpp"
@@ -540,6 +542,85 @@ int ref_flow(int Nx, int Ny, int Nz, const double* o, const double* L, double pe
     for (int k = 0; k < 3; ++k)
         res[k] = r.mom[k];
     res[3] = r.cont;
+    return 0;
+}
+
+// io::Checkpoint::write of the reference mesh with scalar field "p" and
+// vector field "U" (n_cells values each; U as x, y, z blocks).
+int ref_checkpoint_write(const char* path, int Nx, int Ny, int Nz, const double* o,
+                         const double* L, double perturb, uint64_t seed, const double* p,
+                         const double* U, int iteration, double time, uint64_t hash)
+{
+    auto m = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
+    const size_t nC = m->cells().size();
+    solver::FieldRegistry F;
+    std::memcpy(F.scalar("p", nC).data(), p, nC * sizeof(double));
+    auto& V = F.vector("U", nC);
+    std::memcpy(V.x.data(), U, nC * sizeof(double));
+    std::memcpy(V.y.data(), U + nC, nC * sizeof(double));
+    std::memcpy(V.z.data(), U + 2 * nC, nC * sizeof(double));
+    io::CheckpointMeta meta;
+    meta.iteration = iteration;
+    meta.time = time;
+    meta.settingsHash = hash;
+    return io::Checkpoint::write(path, *m, F, meta) ? 0 : 1;
+}
+
+// io::Checkpoint::read: 0 on success, 1 when the reader refuses the file.
+// sizes[3] = cells, faces, nodes of the mesh read back; the mesh arrays go to
+// the same buffers as ref_mesh_arrays; scalar "p" and vector "U" (3 n blocks)
+// are copied when present (have[0], have[1]).
+int ref_checkpoint_read(const char* path, int* iteration, double* time, uint64_t* hash,
+                        int64_t* sizes, uint64_t* owner, uint64_t* neighbor, double* area,
+                        double* fc, uint32_t* zone, double* vol, double* cc, int32_t* cfo,
+                        uint64_t* cfi, double* p, double* U, int* have)
+{
+    meshing::Mesh m;
+    solver::FieldRegistry F;
+    io::CheckpointMeta meta;
+    if (!io::Checkpoint::read(path, m, F, meta))
+        return 1;
+    *iteration = meta.iteration;
+    *time = meta.time;
+    *hash = meta.settingsHash;
+    const auto& Fc = m.faces();
+    const auto& C = m.cells();
+    const size_t nF = Fc.owner.size(), nC = C.volume.size();
+    sizes[0] = int64_t(nC);
+    sizes[1] = int64_t(nF);
+    sizes[2] = int64_t(m.nodes().x.size());
+    if (owner) {
+        std::memcpy(owner, Fc.owner.data(), nF * 8);
+        std::memcpy(neighbor, Fc.neighbor.data(), nF * 8);
+        std::memcpy(area, Fc.areaX.data(), nF * 8);
+        std::memcpy(area + nF, Fc.areaY.data(), nF * 8);
+        std::memcpy(area + 2 * nF, Fc.areaZ.data(), nF * 8);
+        std::memcpy(fc, Fc.centroidX.data(), nF * 8);
+        std::memcpy(fc + nF, Fc.centroidY.data(), nF * 8);
+        std::memcpy(fc + 2 * nF, Fc.centroidZ.data(), nF * 8);
+        std::memcpy(zone, Fc.boundaryZone.data(), nF * 4);
+        std::memcpy(vol, C.volume.data(), nC * 8);
+        std::memcpy(cc, C.centroidX.data(), nC * 8);
+        std::memcpy(cc + nC, C.centroidY.data(), nC * 8);
+        std::memcpy(cc + 2 * nC, C.centroidZ.data(), nC * 8);
+        std::memcpy(cfo, C.faceOffsets.data(), (nC + 1) * 4);
+        std::memcpy(cfi, C.faceIndices.data(), size_t(C.faceOffsets[nC]) * 8);
+    }
+    have[0] = have[1] = 0;
+    if (auto* P = F.find_scalar("p")) {
+        have[0] = 1;
+        if (p)
+            std::memcpy(p, P->data(), P->size() * sizeof(double));
+    }
+    if (auto* V = F.find_vector("U")) {
+        have[1] = 1;
+        if (U) {
+            const size_t n = V->x.size();
+            std::memcpy(U, V->x.data(), n * sizeof(double));
+            std::memcpy(U + n, V->y.data(), n * sizeof(double));
+            std::memcpy(U + 2 * n, V->z.data(), n * sizeof(double));
+        }
+    }
     return 0;
 }
 

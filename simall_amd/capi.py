@@ -74,6 +74,24 @@ class FlowCfg(C.Structure):
     ]
 
 
+class MeshNodes(C.Structure):
+    _fields_ = [("n_nodes", C.c_int64)] + [
+        (k, C.c_void_p) for k in ("x", "y", "z", "faceNodeOffsets", "faceNodeIndices")
+    ]
+
+
+class CheckpointMeta(C.Structure):
+    _fields_ = [("iteration", C.c_int32), ("time", C.c_double), ("settings_hash", C.c_uint64)]
+
+
+class NamedField(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("data", C.c_void_p)]
+
+
+class NamedVector(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p)]
+
+
 class FlowResiduals(C.Structure):
     _fields_ = [("mom", C.c_double * 3), ("cont", C.c_double)]
 
@@ -178,6 +196,14 @@ PROTOTYPES = {
     "mi_flow_iterate": (_i, [_vp, C.POINTER(FlowResiduals)]),
     "mi_flow_advance_time_step": (_i, [_vp, _i, C.POINTER(FlowResiduals)]),
     "mi_flow_destroy": (_i, [_vp]),
+    "mi_checkpoint_write": (_i, [C.c_char_p, _vp, _vp, _i, _vp, _i, _vp, _vp]),
+    "mi_checkpoint_open": (_i, [C.c_char_p, C.POINTER(_vp)]),
+    "mi_checkpoint_info": (_i, [_vp, _vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "mi_checkpoint_scalar": (_i, [_vp, C.c_char_p, _vp, _i64, C.POINTER(_i64)]),
+    "mi_checkpoint_vector": (_i, [_vp, C.c_char_p, _vp, _vp, _vp, _i64, C.POINTER(_i64)]),
+    "mi_checkpoint_close": (_i, [_vp]),
+    "mi_flow_checkpoint_write": (_i, [_vp, C.c_char_p, _vp, _vp, _vp]),
+    "mi_flow_checkpoint_read": (_i, [_vp, C.c_char_p, _vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
     "mi_amg_host_level_info": (_i, [_vp, _i, _i, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _pd]),
@@ -460,15 +486,102 @@ def _dptr(d):
     return None if d is None else d.ptr
 
 
+def mesh_soa(mesh):
+    """MeshSoa over the host arrays of a simall_amd.workload.Mesh (the mesh
+    must outlive the struct)."""
+    s = MeshSoa()
+    s.n_cells, s.n_faces = mesh.n_cells, mesh.n_faces
+    for k, _ in MeshSoa._fields_[2:]:
+        setattr(s, k, getattr(mesh, k).ctypes.data)
+    return s
+
+
+def mesh_nodes(mesh):
+    """MeshNodes over the node arrays of a simall_amd.workload.Mesh."""
+    s = MeshNodes()
+    s.n_nodes = mesh.nodeX.size
+    s.x, s.y, s.z = (getattr(mesh, k).ctypes.data for k in ("nodeX", "nodeY", "nodeZ"))
+    s.faceNodeOffsets = mesh.faceNodeOffsets.ctypes.data
+    s.faceNodeIndices = mesh.faceNodeIndices.ctypes.data
+    return s
+
+
+def checkpoint_write(path, mesh, scalars=None, vectors=None, iteration=0, time=0.0,
+                     settings_hash=0):
+    """Checkpoint::write of a workload Mesh with named host fields:
+    scalars {name: array[n_cells]}, vectors {name: (x, y, z)}."""
+    lib = load()
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data
+
+    sc = list((scalars or {}).items())
+    vc = list((vectors or {}).items())
+    S = (NamedField * max(len(sc), 1))()
+    V = (NamedVector * max(len(vc), 1))()
+    for i, (name, a) in enumerate(sc):
+        S[i].name, S[i].data = name.encode(), arr(a)
+    for i, (name, (x, y, z)) in enumerate(vc):
+        V[i].name, V[i].x, V[i].y, V[i].z = name.encode(), arr(x), arr(y), arr(z)
+    meta = CheckpointMeta(iteration, time, settings_hash)
+    soa, nodes = mesh_soa(mesh), mesh_nodes(mesh)
+    check(lib.mi_checkpoint_write(os.fsencode(path), C.byref(soa), C.byref(nodes), len(sc), S,
+                                  len(vc), V, C.byref(meta)))
+
+
+class Checkpoint:
+    """A checkpoint file read into host memory (mi_checkpoint_open)."""
+
+    def __init__(self, path):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.mi_checkpoint_open(os.fsencode(path), C.byref(h)))
+        self.h = h.value
+        meta = CheckpointMeta()
+        nc, nf, nn = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.lib.mi_checkpoint_info(self.h, C.byref(meta), C.byref(nc), C.byref(nf),
+                                          C.byref(nn)))
+        self.iteration, self.time, self.settings_hash = meta.iteration, meta.time, meta.settings_hash
+        self.n_cells, self.n_faces, self.n_nodes = nc.value, nf.value, nn.value
+
+    def scalar(self, name):
+        cnt = C.c_int64()
+        check(self.lib.mi_checkpoint_scalar(self.h, name.encode(), None, 0, C.byref(cnt)))
+        out = np.empty(cnt.value)
+        check(self.lib.mi_checkpoint_scalar(self.h, name.encode(), ptr(out), out.size, C.byref(cnt)))
+        return out
+
+    def vector(self, name):
+        cnt = C.c_int64()
+        check(self.lib.mi_checkpoint_vector(self.h, name.encode(), None, None, None, 0,
+                                            C.byref(cnt)))
+        out = [np.empty(cnt.value) for _ in range(3)]
+        check(self.lib.mi_checkpoint_vector(self.h, name.encode(), *(ptr(a) for a in out),
+                                            cnt.value, C.byref(cnt)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mi_checkpoint_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceMesh:
     """mi_mesh handle built from a simall_amd.workload.Mesh (host SoA arrays)."""
 
     def __init__(self, mesh, bc_of_zone):
         self.lib = load()
-        s = MeshSoa()
-        s.n_cells, s.n_faces = mesh.n_cells, mesh.n_faces
-        for k, _ in MeshSoa._fields_[2:]:
-            setattr(s, k, getattr(mesh, k).ctypes.data)
+        s = mesh_soa(mesh)
+        self.host = mesh
         self.bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
         h = C.c_void_p()
         check(self.lib.mi_mesh_create(C.byref(s), ptr(self.bc), self.bc.size, C.byref(h)))
@@ -718,6 +831,20 @@ class Flow:
         r = FlowResiduals()
         check(self.lib.mi_flow_advance_time_step(self.h, n_inner, C.byref(r)))
         return r.as_dict()
+
+    def checkpoint_write(self, path, iteration=0, time=0.0, settings_hash=0):
+        """Writes p and U with the DeviceMesh's host mesh (mi_flow_checkpoint_write)."""
+        m = self.mesh.host
+        soa, nodes = mesh_soa(m), mesh_nodes(m)
+        meta = CheckpointMeta(iteration, time, settings_hash)
+        check(self.lib.mi_flow_checkpoint_write(self.h, os.fsencode(path), C.byref(soa),
+                                                C.byref(nodes), C.byref(meta)))
+
+    def checkpoint_read(self, path):
+        """Loads p and U; returns (iteration, time, settings_hash)."""
+        meta = CheckpointMeta()
+        check(self.lib.mi_flow_checkpoint_read(self.h, os.fsencode(path), C.byref(meta)))
+        return meta.iteration, meta.time, meta.settings_hash
 
     def close(self):
         if getattr(self, "h", None):

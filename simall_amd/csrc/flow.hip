@@ -16,9 +16,11 @@
 #include "solver.hpp"
 #include "stream.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <vector>
 
 namespace mi
 {
@@ -602,6 +604,61 @@ MI_API int mi_flow_advance_time_step(mi_flow* F, int n_inner, mi_flow_residuals*
 MI_API int mi_flow_destroy(mi_flow* F)
 {
     delete F;
+    return MI_OK;
+}
+
+MI_API int mi_flow_checkpoint_write(mi_flow* F, const char* path, const mi_mesh_soa* mesh,
+                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta)
+{
+    clear_error();
+    if (!F || !path || !mesh || !nodes)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    if (mesh->n_cells != F->n || mesh->n_faces != F->nf)
+        return set_error(MI_ERR_BAD_ARG, "mesh arrays do not match the flow's mesh");
+    MI_TRY(ensure_device());
+    const size_t n = size_t(F->n);
+    std::vector<double> h[4];
+    for (int k = 0; k < 4; ++k) {
+        h[k].resize(std::max<size_t>(n, 1));
+        if (n)
+            MI_HIP(hipMemcpyAsync(h[k].data(), k < 3 ? F->U[k].p : F->p.p, n * sizeof(double),
+                                  hipMemcpyDeviceToHost, F->s));
+    }
+    MI_HIP(hipStreamSynchronize(F->s));
+    const mi_named_field sc{"p", h[3].data()};
+    const mi_named_vector vc{"U", h[0].data(), h[1].data(), h[2].data()};
+    return mi_checkpoint_write(path, mesh, nodes, 1, &sc, 1, &vc, meta);
+}
+
+MI_API int mi_flow_checkpoint_read(mi_flow* F, const char* path, mi_checkpoint_meta* meta)
+{
+    clear_error();
+    if (!F || !path)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    MI_TRY(ensure_device());
+    mi_checkpoint* c = nullptr;
+    MI_TRY(mi_checkpoint_open(path, &c));
+    std::unique_ptr<mi_checkpoint, int (*)(mi_checkpoint*)> guard(c, mi_checkpoint_close);
+    int64_t nc = 0;
+    MI_TRY(mi_checkpoint_info(c, meta, &nc, nullptr, nullptr));
+    if (nc != F->n)
+        return set_error(MI_ERR_BAD_ARG, "checkpoint has %lld cells, the flow %d",
+                         (long long)nc, F->n);
+    const size_t n = size_t(F->n);
+    std::vector<double> h[4];
+    for (auto& v : h)
+        v.resize(std::max<size_t>(n, 1));
+    int64_t cnt = 0;
+    MI_TRY(mi_checkpoint_scalar(c, "p", h[3].data(), int64_t(n), &cnt));
+    if (cnt != int64_t(n))
+        return set_error(MI_ERR_BAD_ARG, "field p has %lld values", (long long)cnt);
+    MI_TRY(mi_checkpoint_vector(c, "U", h[0].data(), h[1].data(), h[2].data(), int64_t(n), &cnt));
+    if (cnt != int64_t(n))
+        return set_error(MI_ERR_BAD_ARG, "field U has %lld values", (long long)cnt);
+    for (int k = 0; k < 4 && n; ++k)
+        MI_HIP(hipMemcpyAsync(k < 3 ? F->U[k].p : F->p.p, h[k].data(), n * sizeof(double),
+                              hipMemcpyHostToDevice, F->s));
+    MI_HIP(hipStreamSynchronize(F->s));
     return MI_OK;
 }
 

@@ -110,3 +110,34 @@ def test_flow_fields_round_trip_and_restart(gpu):
     ref.iterate()
     F.iterate()
     assert _rel(F.p, ref.p) < 1e-8 and _rel(F.U[1], ref.U[1]) < 1e-8
+
+
+def test_flow_checkpoint_restart(gpu, tmp_path):
+    """mi_flow_checkpoint_write / _read: p and U go through a SIMALLCP file
+    (the format the reference's Checkpoint reads, pinned on the CPU), and a
+    flow restarted from the file continues bit for bit as the one that wrote
+    it (a SIMPLE iteration starts from U and p alone)."""
+    m, bc, bcv = _channel((8, 5, 5))
+    dm = capi.DeviceMesh(m, bc)
+    cfg = capi.flow_cfg(rho=1.0, mu=0.02, momentum=TIGHT, pressure=TIGHT)
+    A = capi.Flow(dm, cfg, bcv)
+    A.iterate()
+    A.iterate()
+    path = tmp_path / "flow.simcp"
+    A.checkpoint_write(path, iteration=2, time=0.0, settings_hash=77)
+    c = capi.Checkpoint(path)
+    assert (c.iteration, c.settings_hash, c.n_cells, c.n_faces) == (2, 77, m.n_cells, m.n_faces)
+    assert np.array_equal(c.scalar("p"), A.p)
+    assert all(np.array_equal(a, b) for a, b in zip(c.vector("U"), A.U))
+    B = capi.Flow(dm, cfg, bcv)
+    assert B.checkpoint_read(path) == (2, 0.0, 77)
+    assert np.array_equal(B.p, A.p)
+    ra, rb = A.iterate(), B.iterate()
+    assert ra == rb
+    assert np.array_equal(B.p, A.p) and all(np.array_equal(a, b) for a, b in zip(B.U, A.U))
+    # a checkpoint of another mesh is refused
+    m2 = wl.Mesh(4, 4, 4)
+    capi.checkpoint_write(tmp_path / "other.simcp", m2, {"p": np.zeros(m2.n_cells)},
+                          {"U": [np.zeros(m2.n_cells)] * 3})
+    with pytest.raises(capi.MiError):
+        B.checkpoint_read(tmp_path / "other.simcp")
