@@ -483,7 +483,9 @@ MI_API int mi_checkpoint_close(mi_checkpoint* c);
  * names of SimpleAlgorithm), with the caller's mesh arrays. */
 MI_API int mi_flow_checkpoint_write(mi_flow* f, const char* path, const mi_mesh_soa* mesh,
                                     const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta);
-/* Loads "p" and "U" from a checkpoint of a mesh with this flow's cell count. */
+/* Loads "p" and "U" from a checkpoint of a mesh with this flow's cell count
+ * and clears the time history (the next time step is implicit Euler, as the
+ * first step of a new flow). */
 MI_API int mi_flow_checkpoint_read(mi_flow* f, const char* path, mi_checkpoint_meta* meta);
 
 #ifdef __cplusplus

@@ -183,6 +183,17 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
     MI_TRY(v.upload(tv.data(), tv.size(), s));
     // the hierarchy's operators need no column-order sums: long rows get lanes
     MI_TRY(rb.upload(nr, h.rp.data(), s, stream_lanes_per_row(nr, nnz, true))); // synchronizes
+    if (nr > 0 && column_order_enabled()) {
+        // values kept only in the column order of the row blocks (sm() streams them)
+        MI_TRY(rb.sort_columns(rp.p, ci.p, nnz, s));
+        if (rb.sorted()) {
+            DevBuf<T> sv;
+            MI_TRY(sv.alloc(size_t(nnz)));
+            MI_TRY(launch_permute<T>(nnz, rb.sperm.p, v.p, sv.p, s));
+            MI_HIP(hipStreamSynchronize(s));
+            v = std::move(sv);
+        }
+    }
     return MI_OK;
 }
 

@@ -16,8 +16,8 @@ template <class T> struct DMat
     int64_t nnz = 0;
     int lanes = 8;
     DevBuf<int> rp, ci;
-    DevBuf<T> v;
-    DevBuf<int> dg; // diagonal positions when v holds the FP32 difference form
+    DevBuf<T> v;    // in the column order of rb when rb.sorted()
+    DevBuf<int> dg; // diagonal positions (CSR) when v holds the FP32 difference form
     RowBlocks rb;
     // diff_form: store the FP32 difference form (stream_rows_diff) when every
     // row holds its diagonal; ignored for FP64.
@@ -25,7 +25,8 @@ template <class T> struct DMat
 #if defined(__HIPCC__)
     StreamMat<T> sm() const
     {
-        return StreamMat<T>{nr, rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr, rb.lpr};
+        return StreamMat<T>{nr,     rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr,
+                            rb.lpr, rb.sorted() ? rb.cw.p : nullptr};
     }
 #endif
 };

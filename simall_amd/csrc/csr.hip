@@ -103,9 +103,57 @@ int make_pattern(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
         p->h_rp.assign(1, 0);
     }
     p->lanes = lanes_for(n, nnz);
-    if (n > 0)
+    if (n > 0) {
         MI_TRY(p->rb.upload(int(n), p->h_rp.data(), s));
+        if (column_order_enabled())
+            MI_TRY(p->rb.sort_columns(p->rp.p, p->ci.p, nnz, s));
+    }
     out = std::move(p);
+    return MI_OK;
+}
+
+template <class T>
+static int permuted_copy(const Pattern& P, const T* src, DevBuf<T>& dst, bool& ok, hipStream_t s)
+{
+    if (ok)
+        return MI_OK;
+    MI_TRY(dst.alloc(size_t(P.nnz)));
+    MI_TRY(launch_permute<T>(P.nnz, P.rb.sperm.p, src, dst.p, s));
+    ok = true;
+    return MI_OK;
+}
+
+int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out)
+{
+    const Pattern& P = *A->pat;
+    out = StreamMat<double>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, A->val.p, nullptr,
+                            P.rb.lpr};
+    if (P.rb.sorted()) {
+        MI_TRY(permuted_copy(P, A->val.p, A->s_val, A->s_val_ok, s));
+        out.val = A->s_val.p;
+        out.cw = P.rb.cw.p;
+    }
+    return MI_OK;
+}
+
+int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out)
+{
+    const Pattern& P = *A->pat;
+    if (op)
+        MI_TRY(A->ensure_fp32_op(s));
+    else
+        MI_TRY(A->ensure_fp32(s));
+    const float* v = op ? A->op32() : A->val32.p;
+    out = StreamMat<float>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, v,
+                           op ? A->diag32() : nullptr, P.rb.lpr};
+    if (P.rb.sorted()) {
+        if (op)
+            MI_TRY(permuted_copy(P, v, A->s_op32, A->s_op32_ok, s));
+        else
+            MI_TRY(permuted_copy(P, v, A->s_val32, A->s_val32_ok, s));
+        out.val = op ? A->s_op32.p : A->s_val32.p;
+        out.cw = P.rb.cw.p;
+    }
     return MI_OK;
 }
 
@@ -326,7 +374,9 @@ MI_API int mi_csr_spmv(mi_csr* A, const double* x, double* y)
     DevBuf<double> dx, dy;
     MI_TRY(dx.upload(x, size_t(n), s));
     MI_TRY(dy.alloc(size_t(n)));
-    MI_TRY(launch_spmv_stream<double>(*A->pat, A->val.p, dx.p, dy.p, s));
+    StreamMat<double> V;
+    MI_TRY(stream_view64(A, s, V));
+    MI_TRY(launch_spmv_view<double>(V, dx.p, dy.p, s));
     MI_HIP(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
@@ -337,13 +387,16 @@ static int spmv_device_impl(mi_csr* A, int precision, const void* x, void* y, hi
     const int64_t n = A->n();
     if (n == 0)
         return MI_OK;
-    if (precision == MI_PRECISION_FP64)
-        return launch_spmv_stream<double>(*A->pat, A->val.p, static_cast<const double*>(x),
-                                          static_cast<double*>(y), s);
+    if (precision == MI_PRECISION_FP64) {
+        StreamMat<double> V;
+        MI_TRY(stream_view64(A, s, V));
+        return launch_spmv_view<double>(V, static_cast<const double*>(x), static_cast<double*>(y),
+                                        s);
+    }
     if (precision == MI_PRECISION_MIXED) {
-        MI_TRY(A->ensure_fp32(s));
-        return launch_spmv_stream<float>(*A->pat, A->val32.p, static_cast<const float*>(x),
-                                         static_cast<float*>(y), s);
+        StreamMat<float> V;
+        MI_TRY(stream_view32(A, s, false, V));
+        return launch_spmv_view<float>(V, static_cast<const float*>(x), static_cast<float*>(y), s);
     }
     return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);
 }

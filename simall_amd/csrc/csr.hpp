@@ -35,6 +35,12 @@ struct mi_csr
     bool val32_valid = false;
     uint64_t version = 1;     // bumped on every value update
 
+    // Values in the pattern's column order (RowBlocks::sort_columns), built on
+    // demand by the stream views below.
+    mi::DevBuf<double> s_val;
+    mi::DevBuf<float> s_val32, s_op32;
+    bool s_val_ok = false, s_val32_ok = false, s_op32_ok = false;
+
     // Transposed copy (mi_csr_transpose): tmap[k] = position in the source
     // matrix of this matrix's k-th value.
     mi::DevBuf<int> tmap;
@@ -48,6 +54,7 @@ struct mi_csr
         host_valid = host_valid_now;
         val32_valid = false;
         val32d_valid = false;
+        s_val_ok = s_val32_ok = s_op32_ok = false;
         ++version;
     }
     int64_t nnz() const { return pat->nnz; }
@@ -77,4 +84,12 @@ int make_csr_on_pattern(const std::shared_ptr<Pattern>& pat, mi_csr** out);
 int launch_inv_diag(const Pattern& p, const double* val, double* invD, hipStream_t s);
 int launch_inv_diag32(const Pattern& p, const double* val, float* invD, hipStream_t s);
 int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s);
+#if defined(__HIPCC__)
+// Stream views of a matrix for the CSR-stream kernels: column-ordered blocks
+// with a permuted copy of the values when the pattern has them, else CSR.
+// view32: op = the inner-iteration operator (op32 / diag32), else the
+// rounded values.
+int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out);
+int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out);
+#endif
 } // namespace mi

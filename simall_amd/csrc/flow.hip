@@ -659,6 +659,8 @@ MI_API int mi_flow_checkpoint_read(mi_flow* F, const char* path, mi_checkpoint_m
         MI_HIP(hipMemcpyAsync(k < 3 ? F->U[k].p : F->p.p, h[k].data(), n * sizeof(double),
                               hipMemcpyHostToDevice, F->s));
     MI_HIP(hipStreamSynchronize(F->s));
+    // no time history survives a restart: the next step starts it again
+    F->time_step = 0;
     return MI_OK;
 }
 

@@ -98,6 +98,7 @@ template <class T> struct CgArgs
     const int* blk;  // CSR-stream row blocks of the matrix
     int nrb;
     const int* diag; // FP32 difference form: diagonal positions, else nullptr
+    const unsigned* cw; // column-ordered row blocks (val in that order), else nullptr
     // FP32 inner solves of the mixed form: when set, the iterate is
     // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
     // then unused.  The caller's FP64 solution takes the correction directly
@@ -138,8 +139,9 @@ template <class T> struct BiArgs
     unsigned grid;
     const int* blk;
     int nrb;
-    const int* diag; // as in CgArgs
-    double* xacc;    // as in CgArgs
+    const int* diag;    // as in CgArgs
+    const unsigned* cw; // as in CgArgs
+    double* xacc;       // as in CgArgs
 };
 
 template <class T> int bicg_init(const BiArgs<T>& a, bool warm, double tol, int max_iter,

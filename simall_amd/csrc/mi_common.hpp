@@ -38,6 +38,10 @@ void clear_error();
 
 // Initialises the HIP runtime once per process and checks for a device.
 int ensure_device();
+// Row blocks of stream matrices are stored in column order (the faster
+// gathers) unless the environment sets SIMALL_AMD_CSR_ORDER=1, which keeps
+// CSR order (for comparisons; results are identical either way).
+bool column_order_enabled();
 // Library stream used when a caller passes a null stream.
 hipStream_t default_stream();
 

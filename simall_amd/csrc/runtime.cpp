@@ -5,6 +5,7 @@
 #include "mi_common.hpp"
 
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -49,6 +50,15 @@ static int init_device(int dev)
     MI_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
     g_device = dev;
     return MI_OK;
+}
+
+bool column_order_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("SIMALL_AMD_CSR_ORDER");
+        return !(e && std::strcmp(e, "1") == 0);
+    }();
+    return on;
 }
 
 int ensure_device()

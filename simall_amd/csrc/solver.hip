@@ -192,9 +192,9 @@ using Clock = std::chrono::steady_clock;
 int true_residual(mi_solver* S, const double* b, const double* x, double* r64, float* r32,
                   double* out_dev)
 {
-    const Pattern& P = *S->M->pat;
-    StreamMat<double> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, S->M->val.p};
-    const unsigned G = stream_grid(P.rb.nrb);
+    StreamMat<double> A;
+    MI_TRY(stream_view64(S->M, S->stream, A));
+    const unsigned G = stream_grid(A.nrb);
     k_true_residual<<<G, kBlock, 0, S->stream>>>(A, x, b, r64, r32, S->partials.p);
     MI_HIP(hipGetLastError());
     return launch_fin_sum(S->partials.p, G, true, out_dev, S->stream);
@@ -420,14 +420,28 @@ template <class Body> int iterate(mi_solver* S, KState* st, int max_iter, int ch
 // One Krylov solve A x = b in element type T on the working matrix with the
 // vector pool w; the final KState is left in S->h_st.  With xacc set (FP32
 // inner solves) the iterate is added to that FP64 vector instead of x.
+// The working matrix as the Krylov kernels stream it: FP64 values, or the
+// FP32 inner-iteration operator.
+static int krylov_view(mi_solver* S, StreamMat<double>& V)
+{
+    return stream_view64(S->M, S->stream, V);
+}
+static int krylov_view(mi_solver* S, StreamMat<float>& V)
+{
+    return stream_view32(S->M, S->stream, true, V);
+}
+
 template <class T>
-int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevBuf<T>* w,
-               double tol, int max_iter, bool warm, double* xacc = nullptr)
+int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, double tol,
+               int max_iter, bool warm, double* xacc = nullptr)
 {
     const Pattern& P = *S->M->pat;
     hipStream_t s = S->stream;
     Precond* pc = S->pc.get();
     const int n = int(P.n);
+    StreamMat<T> V;
+    MI_TRY(krylov_view(S, V));
+    const T* val = V.val;
     const int chunk = check_interval(S, n);
     if (S->cfg.kind == MI_SOLVER_BICGSTAB) {
         BiArgs<T> a{};
@@ -452,7 +466,8 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
         a.lanes = P.lanes;
         a.grid = range_grid(n, 256);
         a.blk = P.rb.blk.p;
-        a.diag = std::is_same<T, float>::value ? S->M->diag32() : nullptr;
+        a.diag = V.diag;
+        a.cw = V.cw;
         a.nrb = P.rb.nrb;
         a.xacc = xacc;
         MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
@@ -484,7 +499,8 @@ int run_krylov(mi_solver* S, const T* val, const T* invD, const T* b, T* x, DevB
     a.grid = range_grid(n, 256);
     a.blk = P.rb.blk.p;
     a.nrb = P.rb.nrb;
-    a.diag = std::is_same<T, float>::value ? S->M->diag32() : nullptr;
+    a.diag = V.diag;
+    a.cw = V.cw;
     a.xacc = xacc;
     const bool jacobi = invD != nullptr;
     if (jacobi) {
@@ -530,8 +546,8 @@ int solve_fp64(mi_solver* S, const double* b, double* x, mi_stats* out)
     const double bnorm = std::max(S->h_scal[1], 1e-30);
     const double init_res = warm ? S->h_scal[2] : S->h_scal[1];
     for (;;) {
-        MI_TRY(run_krylov<double>(S, S->M->val.p, invD, b, x, S->w64, c.tolerance,
-                                  c.max_iterations - total, warm));
+        MI_TRY(run_krylov<double>(S, invD, b, x, S->w64, c.tolerance, c.max_iterations - total,
+                                  warm));
         const KState st = *S->h_st;
         copy_history(S, st.iter);
         total += st.iter;
@@ -601,8 +617,8 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
         // less when that already meets the outer target; x += d is done by
         // the inner update kernels in FP64 (xacc)
         const double tol_in = std::max(inner_tol, 0.5 * c.tolerance * bnorm / rn);
-        MI_TRY(run_krylov<float>(S, S->M->op32(), invD, r32, d32, S->w32, tol_in,
-                                 c.max_iterations - total, false, x));
+        MI_TRY(run_krylov<float>(S, invD, r32, d32, S->w32, tol_in, c.max_iterations - total,
+                                 false, x));
         const KState st = *S->h_st;
         copy_history(S, st.iter);
         total += st.iter;
@@ -684,7 +700,7 @@ int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
     a.n = n;
     a.m = m;
     a.ld = W.ld;
-    a.A64 = StreamMat<double>{n, P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, M->val.p, nullptr, P.rb.lpr};
+    MI_TRY(stream_view64(M, s, a.A64));
     if constexpr (sizeof(T) == 8) {
         a.V = W.V64.p;
         a.w = W.w64.p;
@@ -694,8 +710,7 @@ int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
         a.V = W.V32.p;
         a.w = W.w32.p;
         a.invD = pc->inv_diag32();
-        a.A = StreamMat<float>{n, P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, M->op32(), M->diag32(),
-                               P.rb.lpr};
+        MI_TRY(stream_view32(M, s, true, a.A));
     }
     a.b = b;
     a.x = x;

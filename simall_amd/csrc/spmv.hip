@@ -66,6 +66,107 @@ int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide)
     return l;
 }
 
+// One workgroup per row block: the block's (column, slot) keys sorted in LDS
+// by a bitonic network, then written as packed entries.  A block of one
+// long row keeps its CSR order (its columns are already ascending).
+__global__ __launch_bounds__(kBlock) void k_sort_blocks(const int* __restrict__ blk,
+                                                        const int* __restrict__ rp,
+                                                        const int* __restrict__ ci,
+                                                        unsigned* __restrict__ cw,
+                                                        int* __restrict__ sperm,
+                                                        int* __restrict__ bad)
+{
+    __shared__ unsigned key[kStreamNnz];
+    const int rb = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int r0 = blk[rb], r1 = blk[rb + 1];
+    const int q0 = rp[r0], q1 = rp[r1];
+    const int cnt = q1 - q0;
+    // column field in [0, 2^21 - 1): a packed entry is never the -1 marker
+    auto field = [&](int col) -> unsigned {
+        const int64_t d = int64_t(col) - r0 + kColBias;
+        if (d < 0 || d >= int64_t(kColMask))
+            bad[0] = 1;
+        return unsigned(d) & kColMask;
+    };
+    if (cnt > kStreamNnz) {
+        for (int k = q0 + tid; k < q1; k += kBlock) {
+            cw[k] = field(ci[k]);
+            sperm[k] = k;
+        }
+        return;
+    }
+    for (int m = tid; m < kStreamNnz; m += kBlock)
+        key[m] = m < cnt ? (field(ci[q0 + m]) << 11) | unsigned(m) : 0xFFFFFFFFu;
+    for (int size = 2; size <= kStreamNnz; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            __syncthreads();
+            for (int t = tid; t < kStreamNnz / 2; t += kBlock) {
+                const int lo = 2 * t - (t & (stride - 1));
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const unsigned a = key[lo], b = key[hi];
+                if ((a > b) == up) {
+                    key[lo] = b;
+                    key[hi] = a;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int m = tid; m < cnt; m += kBlock) {
+        const unsigned kv = key[m];
+        const unsigned slot = kv & 2047u;
+        cw[q0 + m] = (slot << kSlotShift) | (kv >> 11);
+        sperm[q0 + m] = q0 + int(slot);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_permute(int64_t nnz, const int* __restrict__ perm,
+                                                    const T* __restrict__ src, T* __restrict__ dst)
+{
+    for (int64_t m = int64_t(blockIdx.x) * kBlock + threadIdx.x; m < nnz;
+         m += int64_t(gridDim.x) * kBlock)
+        dst[m] = src[perm[m]];
+}
+
+template <class T>
+int launch_permute(int64_t nnz, const int* perm, const T* src, T* dst, hipStream_t s)
+{
+    if (nnz <= 0)
+        return MI_OK;
+    k_permute<T><<<grid_for(nnz, kBlock), kBlock, 0, s>>>(nnz, perm, src, dst);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+template int launch_permute<double>(int64_t, const int*, const double*, double*, hipStream_t);
+template int launch_permute<float>(int64_t, const int*, const float*, float*, hipStream_t);
+
+int RowBlocks::sort_columns(const int* rp_dev, const int* ci_dev, int64_t nnz, hipStream_t s)
+{
+    cw.release();
+    sperm.release();
+    if (nrb == 0 || nnz == 0)
+        return MI_OK;
+    DevBuf<unsigned> w;
+    DevBuf<int> perm, bad;
+    MI_TRY(w.alloc(size_t(nnz)));
+    MI_TRY(perm.alloc(size_t(nnz)));
+    MI_TRY(bad.alloc(1));
+    MI_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    k_sort_blocks<<<unsigned(nrb), kBlock, 0, s>>>(blk.p, rp_dev, ci_dev, w.p, perm.p, bad.p);
+    MI_HIP(hipGetLastError());
+    int h_bad = 0;
+    MI_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    if (h_bad)
+        return MI_OK; // a column too far from its block: CSR order stays
+    cw = std::move(w);
+    sperm = std::move(perm);
+    return MI_OK;
+}
+
 int RowBlocks::upload(int n, const int* rp_host, hipStream_t s, int lanes_per_row)
 {
     if (lanes_per_row < 1 || lanes_per_row > 64 || (lanes_per_row & (lanes_per_row - 1)))
@@ -98,10 +199,15 @@ __global__ __launch_bounds__(kBlock) void k_stream_spmv(StreamMat<T> A, const T*
 template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const T* x, T* y,
                                           hipStream_t s)
 {
-    if (P.n == 0)
-        return MI_OK;
     StreamMat<T> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, val};
-    k_stream_spmv<T><<<stream_grid(P.rb.nrb), kBlock, 0, s>>>(A, x, y);
+    return launch_spmv_view(A, x, y, s);
+}
+
+template <class T> int launch_spmv_view(const StreamMat<T>& A, const T* x, T* y, hipStream_t s)
+{
+    if (A.n == 0)
+        return MI_OK;
+    k_stream_spmv<T><<<stream_grid(A.nrb), kBlock, 0, s>>>(A, x, y);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -110,6 +216,9 @@ template int launch_spmv_stream<double>(const Pattern&, const double*, const dou
                                         hipStream_t);
 template int launch_spmv_stream<float>(const Pattern&, const float*, const float*, float*,
                                        hipStream_t);
+template int launch_spmv_view<double>(const StreamMat<double>&, const double*, double*,
+                                      hipStream_t);
+template int launch_spmv_view<float>(const StreamMat<float>&, const float*, float*, hipStream_t);
 
 template <class VT, class XT, class YT>
 static int spmv_dispatch(int n, int64_t nnz, const int* rp, const int* ci, const VT* val,

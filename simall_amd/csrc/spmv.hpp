@@ -78,6 +78,11 @@ int launch_spmv_f32(int n, int64_t nnz, const int* rp, const int* ci, const floa
 struct Pattern;
 template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const T* x, T* y,
                                           hipStream_t s);
+#if defined(__HIPCC__)
+// y = A x over a stream view (CSR or column-ordered blocks).
+template <class T> struct StreamMat;
+template <class T> int launch_spmv_view(const StreamMat<T>& A, const T* x, T* y, hipStream_t s);
+#endif
 // r = b - A x  (FP64)
 int launch_residual_f64(int n, int64_t nnz, const int* rp, const int* ci, const double* val,
                         const double* x, const double* b, double* r, hipStream_t s, int lanes = 0);

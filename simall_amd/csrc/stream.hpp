@@ -17,6 +17,14 @@
 // reduction (fixed order, deterministic).  FP32 matrices may instead carry
 // the difference form (stream_rows_diff), selected by StreamMat::diag.
 //
+// Column-ordered blocks (StreamMat::cw): the gathers of x, not the streams of
+// colIdx/values, bound this engine (a gather instruction whose 64 lanes hit
+// scattered cache lines costs several times a coalesced one).  The nonzeros
+// of every row block can therefore be stored in column order, each with the
+// CSR slot it came from: a wavefront's gather then walks ascending, mostly
+// adjacent columns, and the product still lands in LDS at its CSR slot, so
+// the row sums and their order are unchanged.
+//
 // Workgroups are persistent over a contiguous range of row blocks; the range
 // of hardware block b follows xcd_logical_block so each XCD walks one slice
 // of rows and keeps its part of x in its own L2.
@@ -44,13 +52,33 @@ void build_row_blocks(int n, const int* rp, std::vector<int>& blk, int max_rows 
 // best on the SA-AMG levels of a 2M-row 3D pressure system).
 int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide);
 
+// Packed entry of a column-ordered row block: slot << 21 | (col - r0 + 2^20),
+// r0 = the block's first row, slot = position in the block's CSR span (0 for
+// a block of one long row, whose column order is its CSR order).
+constexpr int kSlotShift = 21;
+constexpr unsigned kColMask = (1u << kSlotShift) - 1u;
+constexpr int kColBias = 1 << 20;
+
 struct RowBlocks
 {
     DevBuf<int> blk;
     int nrb = 0;
     int lpr = 1; // lanes per row (a power of two); rows per block <= kStreamRows / lpr
+    // Column order (sort_columns): cw[m] packed entries, sperm[m] = CSR
+    // position of the m-th stored nonzero.  Empty when not built or when a
+    // column lies 2^20 or more rows from its block's first row.
+    DevBuf<unsigned> cw;
+    DevBuf<int> sperm;
+    bool sorted() const { return cw.n > 0; }
     int upload(int n, const int* rp_host, hipStream_t s, int lanes_per_row = 1);
+    // Builds cw / sperm on the device (one workgroup per row block sorts its
+    // columns in LDS); leaves them empty for out-of-range columns.
+    int sort_columns(const int* rp_dev, const int* ci_dev, int64_t nnz, hipStream_t s);
 };
+
+// dst[m] = src[perm[m]]: values into a row-block column order.
+template <class T>
+int launch_permute(int64_t nnz, const int* perm, const T* src, T* dst, hipStream_t s);
 
 // Workgroups for a stream launch: enough to cover the row blocks, a multiple
 // of 8, at most kMaxReduceBlocks.
@@ -84,7 +112,24 @@ template <class T> struct StreamMat
     // lpr).  1 sums each row in column order on one thread; more lanes take
     // every lpr-th product and combine with xor shuffles (fixed order).
     int lpr = 1;
+    // Column-ordered blocks: packed entries (RowBlocks::cw); val is then in
+    // the same order.  nullptr: ci and val in CSR order.
+    const unsigned* cw = nullptr;
 };
+
+// Column of a stream entry: the plain column, or decoded from a packed one
+// relative to the block's first row r0.
+template <class T>
+__device__ __forceinline__ int stream_col(const StreamMat<T>& A, int e, int r0)
+{
+    return A.cw ? r0 - kColBias + int(unsigned(e) & kColMask) : e;
+}
+// LDS slot of the j-th entry of thread tid.
+template <class T>
+__device__ __forceinline__ int stream_slot(const StreamMat<T>& A, int e, int tid, int j)
+{
+    return A.cw ? int(unsigned(e) >> kSlotShift) : tid + j * kBlock;
+}
 
 // Sum over the lpr lanes of a row group (aligned within the wavefront).
 template <class Acc> __device__ __forceinline__ Acc lane_group_sum(Acc s, int lpr)
@@ -104,15 +149,17 @@ __device__ __forceinline__ double* stream_lds()
 
 // Column/value streams of one row block (nonzeros k0..k1, at most
 // kStreamNnz) into registers: kLoadsPerThread independent loads per thread,
-// c[j] = -1 past the end.
+// c[j] = -1 past the end (never a packed entry, whose column field is below
+// 2^21 - 1).
 template <class T>
 __device__ __forceinline__ void stream_load(const StreamMat<T>& A, int k0, int k1, int tid,
                                             int (&c)[kLoadsPerThread], T (&a)[kLoadsPerThread])
 {
+    const int* src = A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci;
 #pragma unroll
     for (int j = 0; j < kLoadsPerThread; ++j) {
         const int k = k0 + tid + j * kBlock;
-        c[j] = k < k1 ? __builtin_nontemporal_load(&A.ci[k]) : -1;
+        c[j] = k < k1 ? __builtin_nontemporal_load(&src[k]) : -1;
         a[j] = k < k1 ? __builtin_nontemporal_load(&A.val[k]) : T(0);
     }
 }
@@ -164,9 +211,10 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             const int kd = A.diag[r0];
             const float xi = x[r0];
             float s = 0.f;
+            const int* src = A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci;
             for (int k = k0 + tid; k < k1; k += kBlock) {
                 const float v = __builtin_nontemporal_load(&A.val[k]);
-                const float xj = x[__builtin_nontemporal_load(&A.ci[k])];
+                const float xj = x[stream_col(A, __builtin_nontemporal_load(&src[k]), r0)];
                 s += (k == kd) ? v * xi : v * (xj - xi);
             }
             av[tid] = s;
@@ -193,9 +241,10 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
         for (int j = 0; j < kLoadsPerThread; ++j) {
-            if (c[j] >= 0) {
-                av[tid + j * kBlock] = a[j];
-                xv[tid + j * kBlock] = x[c[j]];
+            if (c[j] != -1) {
+                const int q = stream_slot(A, c[j], tid, j);
+                av[q] = a[j];
+                xv[q] = x[stream_col(A, c[j], r0)];
             }
         }
         __syncthreads();
@@ -246,10 +295,11 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
             // one long row: strided partial sums, fixed-order block reduction
             if (tid == 0)
                 epi.pre(r0);
+            const int* src = A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci;
             Acc s = 0;
             for (int k = k0 + tid; k < k1; k += kBlock)
                 s += Acc(__builtin_nontemporal_load(&A.val[k])) *
-                     Acc(x[__builtin_nontemporal_load(&A.ci[k])]);
+                     Acc(x[stream_col(A, __builtin_nontemporal_load(&src[k]), r0)]);
             prod[tid] = s;
             __syncthreads();
             if (tid == 0) {
@@ -274,8 +324,8 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
         for (int j = 0; j < kLoadsPerThread; ++j)
-            if (c[j] >= 0)
-                prod[tid + j * kBlock] = Acc(a[j]) * Acc(x[c[j]]);
+            if (c[j] != -1)
+                prod[stream_slot(A, c[j], tid, j)] = Acc(a[j]) * Acc(x[stream_col(A, c[j], r0)]);
         __syncthreads();
         // FP32 blocks are short on bytes per synchronisation, so the next
         // block's streams are prefetched; for FP64 the extra live registers
