@@ -17,6 +17,24 @@ using namespace mi;
 
 extern "C" {
 
+// rowPtr starts at 0 and never decreases; every column index in [0, n).
+static int check_host_csr(int64_t n, const int32_t* rp, const int32_t* ci)
+{
+    if (n == 0)
+        return MI_OK;
+    if (rp[0] != 0)
+        return set_error(MI_ERR_BAD_ARG, "rowPtr[0] = %d, expected 0", rp[0]);
+    for (int64_t i = 0; i < n; ++i) {
+        if (rp[i + 1] < rp[i])
+            return set_error(MI_ERR_BAD_ARG, "rowPtr decreases at row %lld", (long long)i);
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] < 0 || ci[k] >= n)
+                return set_error(MI_ERR_BAD_ARG, "column %d out of range in row %lld", ci[k],
+                                 (long long)i);
+    }
+    return MI_OK;
+}
+
 MI_API int mi_amg_host_build(int64_t n, const int32_t* rp, const int32_t* ci, const double* v,
                              const mi_solver_cfg* cfg, mi_amg_host** out)
 {
@@ -24,6 +42,7 @@ MI_API int mi_amg_host_build(int64_t n, const int32_t* rp, const int32_t* ci, co
     if (!out || n < 0 || n > INT32_MAX - 1 || (n > 0 && (!rp || !ci || !v)))
         return set_error(MI_ERR_BAD_ARG, "bad argument");
     *out = nullptr;
+    MI_TRY(check_host_csr(n, rp, ci));
     HCsr A;
     A.nr = A.nc = int(n);
     A.rp.assign(rp, rp + n + 1);
@@ -126,6 +145,7 @@ MI_API int mi_rcm_permutation(int64_t n, const int32_t* rp, const int32_t* ci, i
 {
     if (n < 0 || n > INT32_MAX - 1 || (n > 0 && (!rp || !ci || !perm)))
         return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(check_host_csr(n, rp, ci));
     std::vector<int> p;
     rcm_order(int(n), rp, ci, p);
     std::memcpy(perm, p.data(), p.size() * sizeof(int32_t));

@@ -517,7 +517,7 @@ MI_API int mi_flow_create(mi_mesh* M, const double* bc_value_host, const mi_flow
     MI_TRY(F->rhs.alloc(n));
     MI_TRY(F->dt.alloc(n));
     MI_TRY(F->phi.alloc(nf));
-    MI_TRY(F->partials.alloc(size_t(kMaxReduceBlocks)));
+    MI_TRY(F->partials.alloc(size_t(std::max(kMaxReduceBlocks, kMaxStreamBlocks))));
     MI_TRY(F->scal.alloc(8));
     MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&F->h_scal), 8 * sizeof(double)));
     // zero fields and BDF2 history (build_sparsity: SimpleAlgorithm.cpp:84-96)

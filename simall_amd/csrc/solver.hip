@@ -235,7 +235,8 @@ int ensure_workspace(mi_solver* S)
         MI_TRY(S->w32[9].alloc(n));
     }
     MI_TRY(S->st.alloc(1));
-    MI_TRY(S->partials.alloc(size_t(kMaxReduceBlocks) * kMaxReduceValues));
+    MI_TRY(S->partials.alloc(size_t(std::max(kMaxReduceBlocks, kMaxStreamBlocks)) *
+                             kMaxReduceValues));
     MI_TRY(S->dscal.alloc(8));
     if (!S->h_st)
         MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&S->h_st), sizeof(KState)));

@@ -80,13 +80,18 @@ struct RowBlocks
 template <class T>
 int launch_permute(int64_t nnz, const int* perm, const T* src, T* dst, hipStream_t s);
 
-// Workgroups for a stream launch: enough to cover the row blocks, a multiple
-// of 8, at most kMaxReduceBlocks.
+// Workgroups for a stream launch: one per row block up to kMaxStreamBlocks
+// (a multiple of 8), beyond that each workgroup walks a contiguous range.
+// Persistent workgroups measured faster than one block per workgroup on the
+// 2M-row pressure solve (FP32 blocks prefetch the next block's streams).
+// Kernels that write one reduction partial per workgroup size their buffers
+// for kMaxStreamBlocks.
+constexpr int kMaxStreamBlocks = 2048;
 inline unsigned stream_grid(int nrb)
 {
     int64_t g = nrb;
-    if (g > kMaxReduceBlocks)
-        g = kMaxReduceBlocks;
+    if (g > kMaxStreamBlocks)
+        g = kMaxStreamBlocks;
     g = (g + 7) / 8 * 8;
     if (g < 8)
         g = 8;
@@ -237,6 +242,13 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
         const bool mine = i < r1 && sub == 0;
         if (mine)
             epi.pre(i);
+        // the row's extent and diagonal slot, ahead of the barrier
+        int kb = 0, ke = 0, kd = 0;
+        if (i < r1) {
+            kb = A.rp[i] - k0;
+            ke = A.rp[i + 1] - k0;
+            kd = A.diag[i] - k0;
+        }
         if (!have)
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
@@ -251,10 +263,8 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
         have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
         float s = 0.f;
         if (i < r1) {
-            const int kd = A.diag[i] - k0;
             const float xi = xv[kd];
-            const int e = A.rp[i + 1] - k0;
-            for (int k = A.rp[i] - k0 + sub; k < e; k += lpr)
+            for (int k = kb + sub; k < ke; k += lpr)
                 s += (k == kd) ? av[k] * xi : av[k] * (xv[k] - xi);
         }
         if (lpr > 1)
@@ -320,6 +330,12 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
         const bool mine = i < r1 && sub == 0;
         if (mine)
             epi.pre(i);
+        // the row's extent, ahead of the barrier
+        int kb = 0, ke = 0;
+        if (i < r1) {
+            kb = A.rp[i] - k0;
+            ke = A.rp[i + 1] - k0;
+        }
         if (!have)
             stream_load(A, k0, k1, tid, c, a);
 #pragma unroll
@@ -333,11 +349,9 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
         if constexpr (sizeof(T) == 4)
             have = stream_prefetch(A, rb, rb1, k1, tid, c, a);
         Acc s = 0;
-        if (i < r1) {
-            const int e = A.rp[i + 1] - k0;
-            for (int k = A.rp[i] - k0 + sub; k < e; k += lpr)
+        if (i < r1)
+            for (int k = kb + sub; k < ke; k += lpr)
                 s += prod[k];
-        }
         if (lpr > 1)
             s = lane_group_sum(s, lpr);
         if (mine)

@@ -114,3 +114,20 @@ def test_rcm_permutation():
     Ac = A.tocoo()
     assert np.abs(Ac.row - Ac.col).max() > 5 * bw
     assert bw <= 3 * 10 * 10
+
+
+def test_host_csr_input_is_validated():
+    """Malformed patterns are rejected before any host algorithm walks them."""
+    A, _ = system(4)
+    rp, ci = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    bad_col = ci.copy()
+    bad_col[5] = -1
+    bad_col2 = ci.copy()
+    bad_col2[7] = A.shape[0]
+    bad_rp = rp.copy()
+    bad_rp[3], bad_rp[4] = bad_rp[4], bad_rp[3] - 1
+    for r, c in ((rp, bad_col), (rp, bad_col2), (bad_rp, ci)):
+        with pytest.raises(capi.MiError):
+            capi.rcm_permutation(r, c)
+        with pytest.raises(capi.MiError):
+            capi.AmgHost(r, c, np.ones(c.size))
