@@ -26,7 +26,7 @@ template <class T> struct DMat
     StreamMat<T> sm() const
     {
         return StreamMat<T>{nr,     rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr,
-                            rb.lpr, rb.sorted() ? rb.cw.p : nullptr};
+                            rb.lpr, rb.sorted() ? rb.cw.p : nullptr, rb.hdr.p};
     }
 #endif
 };

@@ -128,6 +128,7 @@ int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out)
     const Pattern& P = *A->pat;
     out = StreamMat<double>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, A->val.p, nullptr,
                             P.rb.lpr};
+    out.hdr = P.rb.hdr.p;
     if (P.rb.sorted()) {
         MI_TRY(permuted_copy(P, A->val.p, A->s_val, A->s_val_ok, s));
         out.val = A->s_val.p;
@@ -146,6 +147,7 @@ int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out)
     const float* v = op ? A->op32() : A->val32.p;
     out = StreamMat<float>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, v,
                            op ? A->diag32() : nullptr, P.rb.lpr};
+    out.hdr = P.rb.hdr.p;
     if (P.rb.sorted()) {
         if (op)
             MI_TRY(permuted_copy(P, v, A->s_op32, A->s_op32_ok, s));

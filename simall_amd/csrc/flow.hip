@@ -341,6 +341,7 @@ int flow_resid(mi_flow* F, const mi_csr* A, const double* x, const double* b, do
 {
     const Pattern& P = *A->pat;
     StreamMat<double> S{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, A->val.p, nullptr, P.rb.lpr};
+    S.hdr = P.rb.hdr.p;
     const unsigned G = stream_grid(P.rb.nrb);
     k_resid<<<G, kBlock, 0, F->s>>>(S, x, b, F->partials.p);
     MI_HIP(hipGetLastError());

@@ -254,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr};
     EpiCgAp<T> e{a.p, a.Ap};
     stream_rows(A, a.p, e);
     double v[1] = {e.acc};
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr};
     EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
     stream_rows(A, mode == 0 ? a.y : a.z, e);
     double v[2] = {e.acc0, e.acc1};

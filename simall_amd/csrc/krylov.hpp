@@ -99,6 +99,7 @@ template <class T> struct CgArgs
     int nrb;
     const int* diag; // FP32 difference form: diagonal positions, else nullptr
     const unsigned* cw; // column-ordered row blocks (val in that order), else nullptr
+    const int* hdr;     // row block boundaries as {row, nonzero} pairs, else nullptr
     // FP32 inner solves of the mixed form: when set, the iterate is
     // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
     // then unused.  The caller's FP64 solution takes the correction directly
@@ -141,6 +142,7 @@ template <class T> struct BiArgs
     int nrb;
     const int* diag;    // as in CgArgs
     const unsigned* cw; // as in CgArgs
+    const int* hdr;     // as in CgArgs
     double* xacc;       // as in CgArgs
 };
 

@@ -469,6 +469,7 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
         a.blk = P.rb.blk.p;
         a.diag = V.diag;
         a.cw = V.cw;
+        a.hdr = V.hdr;
         a.nrb = P.rb.nrb;
         a.xacc = xacc;
         MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
@@ -502,6 +503,7 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
     a.nrb = P.rb.nrb;
     a.diag = V.diag;
     a.cw = V.cw;
+    a.hdr = V.hdr;
     a.xacc = xacc;
     const bool jacobi = invD != nullptr;
     if (jacobi) {

@@ -177,6 +177,12 @@ int RowBlocks::upload(int n, const int* rp_host, hipStream_t s, int lanes_per_ro
     build_row_blocks(n, rp_host, b, kStreamRows / lpr);
     nrb = int(b.size()) - 1;
     MI_TRY(blk.upload(b.data(), b.size(), s));
+    std::vector<int> h(2 * b.size());
+    for (size_t r = 0; r < b.size(); ++r) {
+        h[2 * r] = b[r];
+        h[2 * r + 1] = rp_host[b[r]];
+    }
+    MI_TRY(hdr.upload(h.data(), h.size(), s));
     return hipStreamSynchronize(s) == hipSuccess ? MI_OK
                                                   : set_error(MI_ERR_DEVICE, "row block upload");
 }
@@ -200,6 +206,7 @@ template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const 
                                           hipStream_t s)
 {
     StreamMat<T> A{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, val};
+    A.hdr = P.rb.hdr.p;
     return launch_spmv_view(A, x, y, s);
 }
 

@@ -62,6 +62,9 @@ constexpr int kColBias = 1 << 20;
 struct RowBlocks
 {
     DevBuf<int> blk;
+    // {first row, first nonzero} of every block boundary (nrb + 1 pairs):
+    // one load instead of blk -> rowPtr at the start of a block
+    DevBuf<int> hdr;
     int nrb = 0;
     int lpr = 1; // lanes per row (a power of two); rows per block <= kStreamRows / lpr
     // Column order (sort_columns): cw[m] packed entries, sperm[m] = CSR
@@ -120,7 +123,30 @@ template <class T> struct StreamMat
     // Column-ordered blocks: packed entries (RowBlocks::cw); val is then in
     // the same order.  nullptr: ci and val in CSR order.
     const unsigned* cw = nullptr;
+    // Block boundaries as {first row, first nonzero} pairs (RowBlocks::hdr);
+    // nullptr: read blk and rp.
+    const int* hdr = nullptr;
 };
+
+// Rows r0..r1 and nonzeros k0..k1 of row block rb.
+template <class T>
+__device__ __forceinline__ void stream_block(const StreamMat<T>& A, int rb, int& r0, int& r1,
+                                             int& k0, int& k1)
+{
+    if (A.hdr) {
+        const int2 h0 = reinterpret_cast<const int2*>(A.hdr)[rb];
+        const int2 h1 = reinterpret_cast<const int2*>(A.hdr)[rb + 1];
+        r0 = h0.x;
+        k0 = h0.y;
+        r1 = h1.x;
+        k1 = h1.y;
+    } else {
+        r0 = A.blk[rb];
+        r1 = A.blk[rb + 1];
+        k0 = A.rp[r0];
+        k1 = A.rp[r1];
+    }
+}
 
 // Column of a stream entry: the plain column, or decoded from a packed one
 // relative to the block's first row r0.
@@ -179,7 +205,7 @@ __device__ __forceinline__ bool stream_prefetch(const StreamMat<T>& A, int rb, i
 {
     if (rb + 1 >= rb1)
         return false;
-    const int q1 = A.rp[A.blk[rb + 2]];
+    const int q1 = A.hdr ? A.hdr[2 * (rb + 2) + 1] : A.rp[A.blk[rb + 2]];
     if (q1 - k1 > kStreamNnz)
         return false;
     stream_load(A, k1, q1, tid, c, a);
@@ -208,8 +234,8 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
     float a[kLoadsPerThread];
     bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
-        const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
-        const int k0 = A.rp[r0], k1 = A.rp[r1];
+        int r0, r1, k0, k1;
+        stream_block(A, rb, r0, r1, k0, k1);
         if (k1 - k0 > kStreamNnz) {
             if (tid == 0)
                 epi.pre(r0);
@@ -299,8 +325,8 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
     T a[kLoadsPerThread];
     bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
-        const int r0 = A.blk[rb], r1 = A.blk[rb + 1];
-        const int k0 = A.rp[r0], k1 = A.rp[r1];
+        int r0, r1, k0, k1;
+        stream_block(A, rb, r0, r1, k0, k1);
         if (k1 - k0 > kStreamNnz) {
             // one long row: strided partial sums, fixed-order block reduction
             if (tid == 0)
