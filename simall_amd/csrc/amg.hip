@@ -209,7 +209,8 @@ template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r
 }
 
 template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree,
-                                            int coarse_degree, double lmin_ratio, hipStream_t s)
+                                            int coarse_degree, double lmin_ratio, hipStream_t s,
+                                            const double* cinv_device)
 {
     const int nl = int(H.levels.size());
     graphs.clear();
@@ -254,7 +255,136 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
         }
     }
     nc = H.coarse_n;
-    MI_TRY(cinv.upload(H.coarse_inverse.data(), H.coarse_inverse.size(), s));
+    if (cinv_device) {
+        MI_TRY(cinv.alloc(size_t(nc) * size_t(nc)));
+        if (nc > 0)
+            MI_HIP(hipMemcpyAsync(cinv.p, cinv_device, size_t(nc) * size_t(nc) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+    } else {
+        MI_TRY(cinv.upload(H.coarse_inverse.data(), H.coarse_inverse.size(), s));
+    }
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+// ------------------------------------------------- dense coarsest inverse
+// Pivot step of column c (one workgroup): the row of largest |a_rc|, r >= c
+// (the first on ties, c itself unless another is strictly larger), swapped
+// into row c; a pivot below 1e-13 * amax replaced by +-1e-10 * amax; row c of
+// a and inv scaled by 1 / pivot; column c saved in f (the factors).
+__global__ __launch_bounds__(kBlock) void k_gj_pivot(int n, int c, double amax,
+                                                     double* __restrict__ a,
+                                                     double* __restrict__ inv,
+                                                     double* __restrict__ f)
+{
+    __shared__ double sv[kBlock];
+    __shared__ int si[kBlock];
+    const int tid = threadIdx.x;
+    const size_t N = size_t(n);
+    double best = -1.0;
+    int bi = n;
+    for (int r = c + tid; r < n; r += kBlock) {
+        const double v = fabs(a[size_t(r) * N + size_t(c)]);
+        if (v > best) {
+            best = v;
+            bi = r;
+        }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double v2 = sv[tid + w];
+            const int i2 = si[tid + w];
+            if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) {
+                sv[tid] = v2;
+                si[tid] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    int piv = si[0];
+    if (piv >= n || !(sv[0] > fabs(a[size_t(c) * N + size_t(c)])))
+        piv = c;
+    __syncthreads();
+    if (piv != c) {
+        for (int k = tid; k < n; k += kBlock) {
+            const size_t x = size_t(c) * N + size_t(k), y = size_t(piv) * N + size_t(k);
+            const double t0 = a[x], t1 = inv[x];
+            a[x] = a[y];
+            inv[x] = inv[y];
+            a[y] = t0;
+            inv[y] = t1;
+        }
+    }
+    __syncthreads();
+    double p = a[size_t(c) * N + size_t(c)];
+    if (fabs(p) <= 1e-13 * amax)
+        p = (p >= 0 ? 1.0 : -1.0) * 1e-10 * (amax > 0 ? amax : 1.0);
+    const double ip = 1.0 / p;
+    __syncthreads();
+    for (int k = tid; k < n; k += kBlock) {
+        const size_t x = size_t(c) * N + size_t(k);
+        a[x] = (k == c ? p : a[x]) * ip;
+        inv[x] *= ip;
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += kBlock)
+        f[r] = a[size_t(r) * N + size_t(c)];
+}
+
+// Rows r != c with a nonzero factor: a[r][k] -= f_r a[c][k] (k >= c; the
+// pivot row is zero left of c) and inv[r][k] -= f_r inv[c][k] (all k).
+// Grid: x over the n - c + n columns, y over the rows.
+__global__ __launch_bounds__(kBlock) void k_gj_eliminate(int n, int c,
+                                                         const double* __restrict__ f,
+                                                         double* __restrict__ a,
+                                                         double* __restrict__ inv)
+{
+#pragma clang fp contract(off)
+    const int r = int(blockIdx.y);
+    const int ca = n - c;
+    const int j = int(blockIdx.x) * kBlock + int(threadIdx.x);
+    if (r == c || j >= ca + n)
+        return;
+    const double fr = f[r];
+    if (fr == 0.0)
+        return;
+    const size_t N = size_t(n);
+    if (j < ca) {
+        const size_t k = size_t(c + j);
+        a[size_t(r) * N + k] -= fr * a[size_t(c) * N + k];
+    } else {
+        const size_t k = size_t(j - ca);
+        inv[size_t(r) * N + k] -= fr * inv[size_t(c) * N + k];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_identity(int n, double* __restrict__ m)
+{
+    const size_t N = size_t(n);
+    for (size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x; q < N * N;
+         q += size_t(gridDim.x) * kBlock)
+        m[q] = (q / N == q % N) ? 1.0 : 0.0;
+}
+
+int dense_inverse_device(int n, double amax, double* a, double* inv, hipStream_t s)
+{
+    if (n <= 0)
+        return MI_OK;
+    if (n > 65535)
+        return set_error(MI_ERR_BAD_ARG, "dense inverse of %d rows exceeds the grid", n);
+    DevBuf<double> f;
+    MI_TRY(f.alloc(size_t(n)));
+    k_identity<<<grid_for(int64_t(n) * n, kBlock), kBlock, 0, s>>>(n, inv);
+    MI_HIP(hipGetLastError());
+    for (int c = 0; c < n; ++c) {
+        k_gj_pivot<<<1, kBlock, 0, s>>>(n, c, amax, a, inv, f.p);
+        const dim3 g(grid_for(int64_t(n - c) + n, kBlock), unsigned(n));
+        k_gj_eliminate<<<g, kBlock, 0, s>>>(n, c, f.p, a, inv);
+    }
+    MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }
@@ -336,6 +466,7 @@ struct AmgPrecond final : Precond
     int coarse_degree = 0;
     double lmin_ratio = 0.3;
     AmgHierarchyHost host;
+    DevBuf<double> cinv; // inverse of the coarsest operator (copied into each hierarchy)
     DHierarchy<double> h64;
     DHierarchy<float> h32;
     bool has64 = false, has32 = false;
@@ -348,17 +479,27 @@ struct AmgPrecond final : Precond
         H.rp = A->pat->h_rp;
         H.ci = A->pat->h_ci;
         H.v = A->h_val;
+        params.host_coarse_inverse = false;
         if (amg_build(H, params, host) != 0)
             return set_error(MI_ERR_UNSUPPORTED,
                              "AMG coarsening stalled above %d rows (no dense coarsest solve)",
                              kAmgMaxDense);
+        // the coarsest inverse, once for both precisions, on the device
+        const int nc = host.coarse_n;
+        double amax = 0.0;
+        for (int i = 0; i < nc; ++i)
+            amax = std::max(amax, std::abs(host.coarse_dense[size_t(i) * size_t(nc) + size_t(i)]));
+        DevBuf<double> dense;
+        MI_TRY(dense.upload(host.coarse_dense.data(), host.coarse_dense.size(), s));
+        MI_TRY(cinv.alloc(size_t(nc) * size_t(nc)));
+        MI_TRY(dense_inverse_device(nc, amax, dense.p, cinv.p, s));
         has64 = has32 = false;
         if (need64) {
-            MI_TRY(h64.build(host, degree, coarse_degree, lmin_ratio, s));
+            MI_TRY(h64.build(host, degree, coarse_degree, lmin_ratio, s, cinv.p));
             has64 = true;
         }
         if (need32) {
-            MI_TRY(h32.build(host, degree, coarse_degree, lmin_ratio, s));
+            MI_TRY(h32.build(host, degree, coarse_degree, lmin_ratio, s, cinv.p));
             has32 = true;
         }
         return MI_OK;

@@ -49,13 +49,21 @@ template <class T> struct DHierarchy
     GraphCache graphs; // captured V-cycles
     // degree: Chebyshev degree on level 0; coarse_degree on the coarser
     // levels (< 1: the same as level 0).
+    // cinv_device: the coarsest inverse already on the device (n_c x n_c,
+    // copied), or nullptr to upload H.coarse_inverse.
     int build(const AmgHierarchyHost& H, int degree, int coarse_degree, double lmin_ratio,
-              hipStream_t s);
+              hipStream_t s, const double* cinv_device = nullptr);
     // x0 = V-cycle(b0); x0 needs no initialisation.  With use_graph the
     // cycle's launches are replayed from a graph captured for (b0, x0, s).
     int vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph = false);
 };
 
 std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c);
+
+// Inverse of a dense row-major n x n matrix on the device: Gauss-Jordan with
+// partial pivoting on [A | I], the operations of the host dense_inverse in
+// the same order (identical results), one pivot column per two launches.
+// a is overwritten; inv receives the inverse.
+int dense_inverse_device(int n, double amax, double* a, double* inv, hipStream_t s);
 
 } // namespace mi

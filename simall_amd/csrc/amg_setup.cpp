@@ -14,6 +14,7 @@
 //   A_c         R A P with R = P^T (symmetric V-cycle for CG)
 // and the coarsest operator is inverted densely in FP64.
 #include "amg_setup.hpp"
+#include "host_threads.hpp"
 #include "reorder.hpp"
 
 #include <algorithm>
@@ -30,40 +31,55 @@ namespace
 
 int thread_count(int requested)
 {
-    int t = requested > 0 ? requested : int(std::thread::hardware_concurrency());
-    if (t < 1)
-        t = 1;
-    if (t > 16)
-        t = 16;
-    return t;
+    return host_threads(requested);
 }
 
 template <class F> void parallel_for(int n, int threads, F&& f)
 {
-    if (threads <= 1 || n < 4096) {
-        f(0, n, 0);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const int chunk = (n + threads - 1) / threads;
-    for (int t = 0; t < threads; ++t) {
-        const int b = t * chunk, e = std::min(n, b + chunk);
-        if (b >= e)
-            break;
-        pool.emplace_back([&, b, e, t] { f(b, e, t); });
-    }
-    for (auto& th : pool)
-        th.join();
+    host_parallel_for(n, threads, f);
 }
 
 std::vector<double> diagonal(const HCsr& A)
 {
     std::vector<double> d(size_t(A.nr), 0.0);
-    for (int i = 0; i < A.nr; ++i)
-        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k)
-            if (A.ci[size_t(k)] == i)
-                d[size_t(i)] += A.v[size_t(k)];
+    parallel_for(A.nr, thread_count(0), [&](int b, int e, int) {
+        for (int i = b; i < e; ++i)
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k)
+                if (A.ci[size_t(k)] == i)
+                    d[size_t(i)] += A.v[size_t(k)];
+    });
     return d;
+}
+
+// Concatenates per-range CSR pieces (rows in range order) into one matrix.
+struct RowPart
+{
+    std::vector<int> len, ci;
+    std::vector<double> v;
+};
+
+void join_parts(std::vector<RowPart>& parts, int n, std::vector<int>& rp, std::vector<int>& ci,
+                std::vector<double>* v)
+{
+    rp.assign(size_t(n) + 1, 0);
+    size_t total = 0;
+    int row = 0;
+    for (const RowPart& p : parts)
+        for (int l : p.len) {
+            total += size_t(l);
+            rp[size_t(++row)] = int(total);
+        }
+    ci.resize(total);
+    if (v)
+        v->resize(total);
+    size_t off = 0;
+    for (RowPart& p : parts) {
+        std::copy(p.ci.begin(), p.ci.end(), ci.begin() + long(off));
+        if (v)
+            std::copy(p.v.begin(), p.v.end(), v->begin() + long(off));
+        off += p.ci.size();
+        p = RowPart{};
+    }
 }
 
 } // namespace
@@ -71,20 +87,24 @@ std::vector<double> diagonal(const HCsr& A)
 void amg_strength(const HCsr& A, double theta, std::vector<int>& srp, std::vector<int>& sci)
 {
     const std::vector<double> d = diagonal(A);
-    srp.assign(size_t(A.nr) + 1, 0);
-    sci.clear();
-    sci.reserve(size_t(A.nnz()));
-    for (int i = 0; i < A.nr; ++i) {
-        for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
-            const int j = A.ci[size_t(k)];
-            if (j == i)
-                continue;
-            const double a = std::abs(A.v[size_t(k)]);
-            if (a > 0.0 && a >= theta * std::sqrt(std::abs(d[size_t(i)] * d[size_t(j)])))
-                sci.push_back(j);
+    const int threads = thread_count(0);
+    std::vector<RowPart> parts(static_cast<size_t>(threads));
+    parallel_for(A.nr, threads, [&](int b, int e, int t) {
+        RowPart& P = parts[size_t(t)];
+        for (int i = b; i < e; ++i) {
+            const size_t before = P.ci.size();
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                const int j = A.ci[size_t(k)];
+                if (j == i)
+                    continue;
+                const double a = std::abs(A.v[size_t(k)]);
+                if (a > 0.0 && a >= theta * std::sqrt(std::abs(d[size_t(i)] * d[size_t(j)])))
+                    P.ci.push_back(j);
+            }
+            P.len.push_back(int(P.ci.size() - before));
         }
-        srp[size_t(i) + 1] = int(sci.size());
-    }
+    });
+    join_parts(parts, A.nr, srp, sci, nullptr);
 }
 
 int amg_aggregate(int n, const std::vector<int>& srp, const std::vector<int>& sci, const HCsr& A,
@@ -181,16 +201,36 @@ double amg_lambda_max(const HCsr& A, int iters)
                 y[size_t(i)] = s[size_t(i)] * acc;
             }
         });
+        // dot products over fixed 64K-row chunks added in chunk order: the
+        // estimate does not depend on the number of threads
+        constexpr int kChunk = 1 << 16;
+        const int nch = (n + kChunk - 1) / kChunk;
+        std::vector<double> part(size_t(nch) * 3, 0.0);
+        host_parallel_for(nch, std::min(threads, nch), [&](int b, int e, int) {
+            for (int c = b; c < e; ++c) {
+                double xy = 0.0, xx = 0.0, yy = 0.0;
+                for (int i = c * kChunk; i < std::min(n, (c + 1) * kChunk); ++i) {
+                    xy += x[size_t(i)] * y[size_t(i)];
+                    xx += x[size_t(i)] * x[size_t(i)];
+                    yy += y[size_t(i)] * y[size_t(i)];
+                }
+                part[size_t(c) * 3] = xy;
+                part[size_t(c) * 3 + 1] = xx;
+                part[size_t(c) * 3 + 2] = yy;
+            }
+        }, 2);
         double xy = 0.0, xx = 0.0, yy = 0.0;
-        for (int i = 0; i < n; ++i) {
-            xy += x[size_t(i)] * y[size_t(i)];
-            xx += x[size_t(i)] * x[size_t(i)];
-            yy += y[size_t(i)] * y[size_t(i)];
+        for (int c = 0; c < nch; ++c) {
+            xy += part[size_t(c) * 3];
+            xx += part[size_t(c) * 3 + 1];
+            yy += part[size_t(c) * 3 + 2];
         }
         lam = xx > 0 ? std::abs(xy / xx) : 1.0;
         const double inv = yy > 0 ? 1.0 / std::sqrt(yy) : 1.0;
-        for (int i = 0; i < n; ++i)
-            x[size_t(i)] = y[size_t(i)] * inv;
+        parallel_for(n, threads, [&](int b, int e, int) {
+            for (int i = b; i < e; ++i)
+                x[size_t(i)] = y[size_t(i)] * inv;
+        });
     }
     return lam;
 }
@@ -206,16 +246,15 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
     const std::vector<double> d = diagonal(A);
     P.nr = n;
     P.nc = nagg;
-    P.rp.assign(size_t(n) + 1, 0);
-    P.ci.clear();
-    P.v.clear();
-    P.ci.reserve(size_t(n) * 8);
-    P.v.reserve(size_t(n) * 8);
+    const int threads = thread_count(0);
+    std::vector<RowPart> parts(static_cast<size_t>(threads));
+    parallel_for(n, threads, [&](int rb, int re, int t) {
+    RowPart& out = parts[size_t(t)];
     std::vector<int> cols;
     std::vector<double> acc(static_cast<size_t>(nagg), 0.0);
     std::vector<char> used(static_cast<size_t>(nagg), 0);
     std::vector<char> strong(static_cast<size_t>(n), 0);
-    for (int i = 0; i < n; ++i) {
+    for (int i = rb; i < re; ++i) {
         cols.clear();
         auto add = [&](int c, double v) {
             if (!used[size_t(c)]) {
@@ -252,15 +291,18 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
             for (int s = (*srp)[size_t(i)]; s < (*srp)[size_t(i) + 1]; ++s)
                 strong[size_t((*sci)[size_t(s)])] = 0;
         std::sort(cols.begin(), cols.end());
+        const size_t before = out.ci.size();
         for (int c : cols) {
             used[size_t(c)] = 0;
             if (acc[size_t(c)] != 0.0) {
-                P.ci.push_back(c);
-                P.v.push_back(acc[size_t(c)]);
+                out.ci.push_back(c);
+                out.v.push_back(acc[size_t(c)]);
             }
         }
-        P.rp[size_t(i) + 1] = int(P.ci.size());
+        out.len.push_back(int(out.ci.size() - before));
     }
+    });
+    join_parts(parts, n, P.rp, P.ci, &P.v);
 }
 
 void csr_transpose(const HCsr& A, HCsr& T)
@@ -453,30 +495,34 @@ void amg_renumber_coarse(HCsr& Ac, HCsr& P)
     rcm_order(nc, Ac.rp.data(), Ac.ci.data(), perm);
     permute_pattern(nc, Ac.rp.data(), Ac.ci.data(), perm, brp, bci, vmap);
     std::vector<double> bv(vmap.size());
-    for (size_t k = 0; k < vmap.size(); ++k)
-        bv[k] = Ac.v[size_t(vmap[k])];
+    parallel_for(int(vmap.size()), thread_count(0), [&](int b, int e, int) {
+        for (int k = b; k < e; ++k)
+            bv[size_t(k)] = Ac.v[size_t(vmap[size_t(k)])];
+    });
     Ac.rp = std::move(brp);
     Ac.ci = std::move(bci);
     Ac.v = std::move(bv);
     std::vector<int> inv(static_cast<size_t>(nc));
     for (int i = 0; i < nc; ++i)
         inv[size_t(perm[size_t(i)])] = i;
-    std::vector<std::pair<int, double>> row;
-    for (int i = 0; i < P.nr; ++i) {
-        row.clear();
-        for (int k = P.rp[size_t(i)]; k < P.rp[size_t(i) + 1]; ++k)
-            row.emplace_back(inv[size_t(P.ci[size_t(k)])], P.v[size_t(k)]);
-        std::sort(row.begin(), row.end(),
-                  [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
-                      return a.first < b.first;
-                  });
-        int k = P.rp[size_t(i)];
-        for (const auto& e : row) {
-            P.ci[size_t(k)] = e.first;
-            P.v[size_t(k)] = e.second;
-            ++k;
+    parallel_for(P.nr, thread_count(0), [&](int b, int e, int) {
+        std::vector<std::pair<int, double>> row;
+        for (int i = b; i < e; ++i) {
+            row.clear();
+            for (int k = P.rp[size_t(i)]; k < P.rp[size_t(i) + 1]; ++k)
+                row.emplace_back(inv[size_t(P.ci[size_t(k)])], P.v[size_t(k)]);
+            std::sort(row.begin(), row.end(),
+                      [](const std::pair<int, double>& x, const std::pair<int, double>& y) {
+                          return x.first < y.first;
+                      });
+            int k = P.rp[size_t(i)];
+            for (const auto& en : row) {
+                P.ci[size_t(k)] = en.first;
+                P.v[size_t(k)] = en.second;
+                ++k;
+            }
         }
-    }
+    });
 }
 
 int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
@@ -528,11 +574,16 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
     const int nc = C.A.nr;
     if (nc > kAmgMaxDense)
         return 1; // coarsening stalled: the coarsest level is too large for a dense solve
-    std::vector<double> dense(size_t(nc) * size_t(nc), 0.0);
+    std::vector<double>& dense = out.coarse_dense;
+    dense.assign(size_t(nc) * size_t(nc), 0.0);
     for (int i = 0; i < nc; ++i)
         for (int k = C.A.rp[size_t(i)]; k < C.A.rp[size_t(i) + 1]; ++k)
             dense[size_t(i) * size_t(nc) + size_t(C.A.ci[size_t(k)])] += C.A.v[size_t(k)];
-    dense_inverse(nc, dense, out.coarse_inverse, threads);
+    out.coarse_inverse.clear();
+    if (p.host_coarse_inverse) {
+        std::vector<double> work(dense);
+        dense_inverse(nc, work, out.coarse_inverse, threads);
+    }
     out.coarse_n = nc;
     out.operator_complexity = double(nnz_total) / double(std::max<int64_t>(A0.nnz(), 1));
     out.grid_complexity = double(rows_total) / double(std::max(A0.nr, 1));

@@ -29,6 +29,10 @@ struct AmgParams
     double omega_scale = 4.0 / 3.0; // prolongator smoothing weight * lambda_max(D^-1 A)
     int threads = 0;            // 0: hardware concurrency (at most 16)
     bool filter = true;         // smooth P with the filtered operator (weak entries lumped)
+    // Invert the coarsest operator on the host (coarse_inverse); otherwise
+    // only coarse_dense is filled and the caller inverts it (the device
+    // hierarchy does, with the same elimination: dense_inverse_device).
+    bool host_coarse_inverse = true;
 };
 
 struct AmgLevelHost
@@ -42,7 +46,8 @@ struct AmgLevelHost
 struct AmgHierarchyHost
 {
     std::vector<AmgLevelHost> levels;
-    std::vector<double> coarse_inverse; // dense row-major inverse of the coarsest A
+    std::vector<double> coarse_dense;   // the coarsest A, dense row-major
+    std::vector<double> coarse_inverse; // its inverse (AmgParams::host_coarse_inverse)
     int coarse_n = 0;
     double operator_complexity = 0.0;
     double grid_complexity = 0.0;

@@ -5,6 +5,8 @@
 // one level set (N^2 cells for an N^3 grid), which stays in each XCD's L2.
 #include "reorder.hpp"
 
+#include "host_threads.hpp"
+
 #include <algorithm>
 #include <numeric>
 
@@ -39,15 +41,22 @@ void symmetrise(int n, const int* rp, const int* ci, std::vector<int>& arp, std:
                 aci[size_t(pos[size_t(j)]++)] = i;
             }
         }
-    // sort + unique each row (structurally symmetric inputs give duplicates)
+    // sort + unique each row (structurally symmetric inputs give duplicates),
+    // rows in parallel, then compaction
+    std::vector<int> ulen(static_cast<size_t>(n), 0);
+    host_parallel_for(n, host_threads(), [&](int b0, int e0, int) {
+        for (int i = b0; i < e0; ++i) {
+            auto b = aci.begin() + arp[size_t(i)], e = aci.begin() + arp[size_t(i) + 1];
+            std::sort(b, e);
+            ulen[size_t(i)] = int(std::unique(b, e) - b);
+        }
+    });
     std::vector<int> nrp(size_t(n) + 1, 0);
     int w = 0;
     for (int i = 0; i < n; ++i) {
-        auto b = aci.begin() + arp[size_t(i)], e = aci.begin() + arp[size_t(i) + 1];
-        std::sort(b, e);
-        auto u = std::unique(b, e);
-        for (auto it = b; it != u; ++it)
-            aci[size_t(w++)] = *it;
+        const int b = arp[size_t(i)];
+        for (int q = 0; q < ulen[size_t(i)]; ++q)
+            aci[size_t(w++)] = aci[size_t(b + q)];
         nrp[size_t(i) + 1] = w;
     }
     aci.resize(size_t(w));
@@ -148,20 +157,22 @@ void permute_pattern(int n, const int* rp, const int* ci, const std::vector<int>
     }
     bci.resize(size_t(brp[size_t(n)]));
     vmap.resize(size_t(brp[size_t(n)]));
-    std::vector<std::pair<int, int>> row;
-    for (int i = 0; i < n; ++i) {
-        const int o = perm[size_t(i)];
-        row.clear();
-        for (int k = rp[o]; k < rp[o + 1]; ++k)
-            row.emplace_back(inv[size_t(ci[k])], k);
-        std::sort(row.begin(), row.end());
-        int w = brp[size_t(i)];
-        for (auto& [c, k] : row) {
-            bci[size_t(w)] = c;
-            vmap[size_t(w)] = k;
-            ++w;
+    host_parallel_for(n, host_threads(), [&](int b0, int e0, int) {
+        std::vector<std::pair<int, int>> row;
+        for (int i = b0; i < e0; ++i) {
+            const int o = perm[size_t(i)];
+            row.clear();
+            for (int k = rp[o]; k < rp[o + 1]; ++k)
+                row.emplace_back(inv[size_t(ci[k])], k);
+            std::sort(row.begin(), row.end());
+            int w = brp[size_t(i)];
+            for (auto& [c, k] : row) {
+                bci[size_t(w)] = c;
+                vmap[size_t(w)] = k;
+                ++w;
+            }
         }
-    }
+    });
 }
 
 int bandwidth(int n, const int* rp, const int* ci)
