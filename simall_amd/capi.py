@@ -158,6 +158,7 @@ PROTOTYPES = {
     "mi_csr_spmv": (_i, [_vp, _vp, _vp]),
     "mi_csr_spmv_device": (_i, [_vp, _i, _vp, _vp, _vp]),
     "mi_csr_time_spmv": (_i, [_vp, _i, _vp, _vp, _i, _pd]),
+    "mi_csr_time_spmv_cold": (_i, [_vp, _i, _vp, _vp, _i, _i64, _pd]),
     "mi_csr_transpose": (_i, [_vp, C.POINTER(_vp)]),
     "mi_csr_transpose_values": (_i, [_vp, _vp]),
     "mi_csr_destroy": (_i, [_vp]),
@@ -170,6 +171,7 @@ PROTOTYPES = {
     "mi_solver_last_residual": (_d, [_vp]),
     "mi_solver_precond_apply": (_i, [_vp, _vp, _vp]),
     "mi_solver_time_spmv": (_i, [_vp, _i, _i, _pd, _pd]),
+    "mi_solver_time_spmv_cold": (_i, [_vp, _i, _i, _i64, _pd, _pd]),
     "mi_solver_info": (_i, [_vp, C.POINTER(_i), _pd, C.POINTER(_i), C.POINTER(_i)]),
     "mi_solver_history": (_i, [_vp, _pd, _i, C.POINTER(_i)]),
     "mi_solver_destroy": (_i, [_vp]),
@@ -418,6 +420,13 @@ class Solver:
     def time_spmv(self, precision=MI_PRECISION_FP64, reps=50):
         us, by = C.c_double(), C.c_double()
         check(self.lib.mi_solver_time_spmv(self.h, precision, reps, C.byref(us), C.byref(by)))
+        return us.value, by.value
+
+    def time_spmv_cold(self, precision=MI_PRECISION_FP64, reps=20, flush_bytes=1 << 30):
+        """Average of `reps` SpMV launches, each after a read sweep of flush_bytes."""
+        us, by = C.c_double(), C.c_double()
+        check(self.lib.mi_solver_time_spmv_cold(self.h, precision, reps, flush_bytes, C.byref(us),
+                                                C.byref(by)))
         return us.value, by.value
 
     def info(self):

@@ -439,6 +439,62 @@ MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x, void* y, in
     return MI_OK;
 }
 
+// Reads n doubles (one partial per workgroup, so the loads are not dead).
+__global__ __launch_bounds__(kBlock) void k_read_sweep(size_t n, const double* __restrict__ buf,
+                                                       double* __restrict__ sink)
+{
+    double s = 0.0;
+    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * kBlock)
+        s += buf[i];
+    if (threadIdx.x == 0)
+        sink[blockIdx.x] = s;
+}
+
+MI_API int mi_csr_time_spmv_cold(mi_csr* A, int precision, const void* x, void* y, int reps,
+                                 int64_t flush_bytes, double* avg_us)
+{
+    if (!A || !avg_us || reps < 1 || (A->n() && (!x || !y)))
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    if (flush_bytes < (int64_t(512) << 20))
+        return set_error(MI_ERR_BAD_ARG, "flush of %lld bytes is below 512 MiB",
+                         (long long)flush_bytes);
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    MI_TRY(spmv_device_impl(A, precision, x, y, s)); // FP32 copy and permuted values first
+    constexpr unsigned kSweepBlocks = 4096;
+    const size_t nf = size_t(flush_bytes) / sizeof(double);
+    DevBuf<double> buf, sink;
+    MI_TRY(buf.alloc(nf));
+    MI_TRY(sink.alloc(kSweepBlocks));
+    MI_HIP(hipMemsetAsync(buf.p, 0, nf * sizeof(double), s));
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    double total_ms = 0.0;
+    int rc = MI_OK;
+    for (int i = 0; i < reps && rc == MI_OK; ++i) {
+        k_read_sweep<<<kSweepBlocks, kBlock, 0, s>>>(nf, buf.p, sink.p);
+        if (hipEventRecord(e0, s) != hipSuccess) {
+            rc = set_error(MI_ERR_DEVICE, "event record");
+            break;
+        }
+        rc = spmv_device_impl(A, precision, x, y, s);
+        float ms = 0.f;
+        if (rc == MI_OK && (hipEventRecord(e1, s) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = set_error(MI_ERR_DEVICE, "event timing");
+        total_ms += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    MI_TRY(rc);
+    MI_HIP(hipGetLastError());
+    *avg_us = total_ms * 1000.0 / reps;
+    return MI_OK;
+}
+
 MI_API int mi_csr_transpose(const mi_csr* A, mi_csr** out)
 {
     if (!A || !out)

@@ -1081,6 +1081,31 @@ MI_API int mi_solver_time_spmv(mi_solver* S, int precision, int reps, double* av
     return mi_csr_time_spmv(M, precision, xb.p, yb.p, reps, avg_us);
 }
 
+MI_API int mi_solver_time_spmv_cold(mi_solver* S, int precision, int reps, int64_t flush_bytes,
+                                    double* avg_us, double* bytes)
+{
+    clear_error();
+    if (!S || !S->A || reps < 1 || !avg_us)
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(ensure_device());
+    MI_TRY(build(S));
+    mi_csr* M = S->M;
+    const int64_t n = M->n(), nnz = M->nnz();
+    const size_t vb = precision == MI_PRECISION_MIXED ? 4 : 8;
+    if (bytes)
+        *bytes = double(nnz) * double(vb + 4) + double(n + 1) * 4.0 + 2.0 * double(n) * double(vb);
+    if (n == 0) {
+        *avg_us = 0.0;
+        return MI_OK;
+    }
+    DevBuf<char> xb, yb;
+    MI_TRY(xb.alloc(size_t(n) * vb));
+    MI_TRY(yb.alloc(size_t(n) * vb));
+    MI_HIP(hipMemsetAsync(xb.p, 0, size_t(n) * vb, default_stream()));
+    MI_HIP(hipStreamSynchronize(default_stream()));
+    return mi_csr_time_spmv_cold(M, precision, xb.p, yb.p, reps, flush_bytes, avg_us);
+}
+
 MI_API int mi_adjoint_solve(mi_solver* S, const double* g, double* lambda, mi_stats* stats)
 {
     clear_error();

@@ -126,3 +126,19 @@ def test_jacobi_with_and_without_reordering(gpu):
         res.append((x, st.iterations))
     assert abs(res[0][1] - ito) <= 2 and abs(res[1][1] - ito) <= 2
     assert np.linalg.norm(res[1][0] - xo) / np.linalg.norm(xo) < 1e-6
+
+
+def test_spmv_timing_entry_points(gpu, sys24):
+    """mi_solver_time_spmv (back to back) and _cold (a read sweep of at least
+    512 MiB before every launch) time the working matrix and report its
+    algorithmic bytes; a smaller flush is refused."""
+    A, Ao, b, xt = sys24
+    s = solver(A, reorder=1)
+    n, nnz = A.shape[0], A.nnz
+    for prec, vb in ((capi.MI_PRECISION_FP64, 8), (capi.MI_PRECISION_MIXED, 4)):
+        us, by = s.time_spmv(prec, reps=5)
+        usc, byc = s.time_spmv_cold(prec, reps=3)
+        assert us > 0 and usc > 0
+        assert by == byc == nnz * (vb + 4) + (n + 1) * 4 + 2 * n * vb
+    with pytest.raises(capi.MiError):
+        s.time_spmv_cold(capi.MI_PRECISION_FP64, reps=1, flush_bytes=1 << 20)
