@@ -214,7 +214,13 @@ template <class T> int launch_spmv_view(const StreamMat<T>& A, const T* x, T* y,
 {
     if (A.n == 0)
         return MI_OK;
-    k_stream_spmv<T><<<stream_grid(A.nrb), kBlock, 0, s>>>(A, x, y);
+    // FP64: one row block per workgroup (no reduction partials here), so the
+    // dispatcher hands out blocks as workgroups retire; measured on the 128^3
+    // pressure matrix from HBM 47.9 vs 48.9 us.  FP32 keeps persistent ranges,
+    // whose next-block prefetch is worth more (39.7 vs 38.1 us).
+    const unsigned G = sizeof(T) == 8 ? unsigned((int64_t(A.nrb) + 7) / 8 * 8)
+                                      : stream_grid(A.nrb);
+    k_stream_spmv<T><<<G, kBlock, 0, s>>>(A, x, y);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
