@@ -129,10 +129,13 @@ MI_API int mi_csr_spmv_device(mi_csr* A, int precision, const void* x_device, vo
  * returns the average launch duration in microseconds. */
 MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x_device, void* y_device,
                             int reps, double* avg_us);
-/* The same with the caches emptied first: before each launch a kernel reads
- * flush_bytes (at least 512 MiB: more than the 256 MB Infinity Cache) of an
- * unrelated buffer, and each launch is timed alone.  The matrix streams then
- * come from HBM, not from a cache that a back-to-back loop leaves warm. */
+/* The same with the matrix streams out of every cache: the launches rotate
+ * over copies of the matrix (row pointers, columns, values) whose total
+ * exceeds flush_bytes (at least 512 MiB: more than the 256 MB Infinity
+ * Cache), so each launch finds its own copy evicted by the others.  A
+ * back-to-back loop on one copy of a matrix that fits in that cache would
+ * time cache bandwidth, not HBM.  Launches are queued back to back between
+ * two events, as in mi_csr_time_spmv. */
 MI_API int mi_csr_time_spmv_cold(mi_csr* A, int precision, const void* x_device, void* y_device,
                                  int reps, int64_t flush_bytes, double* avg_us);
 /* Explicit transpose A^T as a new handle: the transposed pattern is built on
@@ -225,7 +228,7 @@ MI_API int mi_solver_info(const mi_solver* s, int* levels, double* operator_comp
  * pointers + x once + y). */
 MI_API int mi_solver_time_spmv(mi_solver* s, int precision, int reps, double* avg_us,
                                double* bytes);
-/* mi_solver_time_spmv with mi_csr_time_spmv_cold's cache flush before every launch. */
+/* mi_solver_time_spmv with mi_csr_time_spmv_cold's rotation over copies. */
 MI_API int mi_solver_time_spmv_cold(mi_solver* s, int precision, int reps, int64_t flush_bytes,
                                     double* avg_us, double* bytes);
 /* Adjoint solve A^T lambda = g (device vectors) with the solver's

@@ -423,7 +423,8 @@ class Solver:
         return us.value, by.value
 
     def time_spmv_cold(self, precision=MI_PRECISION_FP64, reps=20, flush_bytes=1 << 30):
-        """Average of `reps` SpMV launches, each after a read sweep of flush_bytes."""
+        """Average of `reps` SpMV launches rotating over copies of the matrix
+        (more than flush_bytes in all), so none finds its streams cached."""
         us, by = C.c_double(), C.c_double()
         check(self.lib.mi_solver_time_spmv_cold(self.h, precision, reps, flush_bytes, C.byref(us),
                                                 C.byref(by)))

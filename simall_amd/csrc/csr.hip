@@ -3,7 +3,9 @@
 #include "krylov.hpp"
 #include "spmv.hpp"
 
+#include <cmath>
 #include <cstring>
+#include <vector>
 
 using namespace mi;
 
@@ -207,6 +209,70 @@ int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s)
         return MI_OK;
     k_convert_f32<<<reduce_grid(n, 4), kBlock, 0, s>>>(n, src, dst);
     MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+// Copies of the streams a view reads (row pointers, block boundaries,
+// columns or packed entries, values), for timing launches whose data is
+// not in any cache.
+template <class T> struct StreamCopy
+{
+    DevBuf<int> rp, blk, hdr, ci;
+    DevBuf<T> val;
+    StreamMat<T> V;
+    int make(const StreamMat<T>& A, int64_t nnz, hipStream_t s)
+    {
+        auto dup = [&](auto& dst, const auto* src, size_t count) -> int {
+            if (!src || count == 0)
+                return MI_OK;
+            MI_TRY(dst.alloc(count));
+            MI_HIP(hipMemcpyAsync(dst.p, src, count * sizeof(*src), hipMemcpyDeviceToDevice, s));
+            return MI_OK;
+        };
+        V = A;
+        MI_TRY(dup(rp, A.rp, size_t(A.n) + 1));
+        MI_TRY(dup(blk, A.blk, size_t(A.nrb) + 1));
+        MI_TRY(dup(hdr, A.hdr, 2 * (size_t(A.nrb) + 1)));
+        MI_TRY(dup(ci, A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci, size_t(nnz)));
+        MI_TRY(dup(val, A.val, size_t(nnz)));
+        V.rp = rp.p;
+        V.blk = blk.p;
+        V.hdr = A.hdr ? hdr.p : nullptr;
+        V.ci = A.cw ? A.ci : ci.p;
+        V.cw = A.cw ? reinterpret_cast<const unsigned*>(ci.p) : nullptr;
+        V.val = val.p;
+        return MI_OK;
+    }
+};
+
+template <class T>
+static int time_cold(const StreamMat<T>& A, int64_t nnz, const T* x, T* y, int reps,
+                     int64_t flush_bytes, hipStream_t s, double* avg_us)
+{
+    const double per = double(nnz) * double(4 + sizeof(T)) + double(A.n + 1) * 4.0;
+    const int nc = int(std::ceil(double(flush_bytes) / per)) + 1;
+    std::vector<StreamCopy<T>> cp(static_cast<size_t>(nc));
+    for (auto& c : cp)
+        MI_TRY(c.make(A, nnz, s));
+    for (auto& c : cp) // first touch of every copy
+        MI_TRY(launch_spmv_view<T>(c.V, x, y, s));
+    hipEvent_t e0, e1;
+    MI_HIP(hipEventCreate(&e0));
+    MI_HIP(hipEventCreate(&e1));
+    float ms = 0.f;
+    int rc = MI_OK;
+    if (hipEventRecord(e0, s) != hipSuccess)
+        rc = set_error(MI_ERR_DEVICE, "event record");
+    for (int i = 0; i < reps && rc == MI_OK; ++i)
+        rc = launch_spmv_view<T>(cp[size_t(i % nc)].V, x, y, s);
+    if (rc == MI_OK &&
+        (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = set_error(MI_ERR_DEVICE, "event timing");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    MI_TRY(rc);
+    *avg_us = double(ms) * 1000.0 / reps;
     return MI_OK;
 }
 
@@ -439,18 +505,6 @@ MI_API int mi_csr_time_spmv(mi_csr* A, int precision, const void* x, void* y, in
     return MI_OK;
 }
 
-// Reads n doubles (one partial per workgroup, so the loads are not dead).
-__global__ __launch_bounds__(kBlock) void k_read_sweep(size_t n, const double* __restrict__ buf,
-                                                       double* __restrict__ sink)
-{
-    double s = 0.0;
-    for (size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x; i < n;
-         i += size_t(gridDim.x) * kBlock)
-        s += buf[i];
-    if (threadIdx.x == 0)
-        sink[blockIdx.x] = s;
-}
-
 MI_API int mi_csr_time_spmv_cold(mi_csr* A, int precision, const void* x, void* y, int reps,
                                  int64_t flush_bytes, double* avg_us)
 {
@@ -461,38 +515,23 @@ MI_API int mi_csr_time_spmv_cold(mi_csr* A, int precision, const void* x, void* 
                          (long long)flush_bytes);
     MI_TRY(ensure_device());
     hipStream_t s = default_stream();
-    MI_TRY(spmv_device_impl(A, precision, x, y, s)); // FP32 copy and permuted values first
-    constexpr unsigned kSweepBlocks = 4096;
-    const size_t nf = size_t(flush_bytes) / sizeof(double);
-    DevBuf<double> buf, sink;
-    MI_TRY(buf.alloc(nf));
-    MI_TRY(sink.alloc(kSweepBlocks));
-    MI_HIP(hipMemsetAsync(buf.p, 0, nf * sizeof(double), s));
-    hipEvent_t e0, e1;
-    MI_HIP(hipEventCreate(&e0));
-    MI_HIP(hipEventCreate(&e1));
-    double total_ms = 0.0;
-    int rc = MI_OK;
-    for (int i = 0; i < reps && rc == MI_OK; ++i) {
-        k_read_sweep<<<kSweepBlocks, kBlock, 0, s>>>(nf, buf.p, sink.p);
-        if (hipEventRecord(e0, s) != hipSuccess) {
-            rc = set_error(MI_ERR_DEVICE, "event record");
-            break;
-        }
-        rc = spmv_device_impl(A, precision, x, y, s);
-        float ms = 0.f;
-        if (rc == MI_OK && (hipEventRecord(e1, s) != hipSuccess ||
-                            hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = set_error(MI_ERR_DEVICE, "event timing");
-        total_ms += ms;
+    if (A->n() == 0) {
+        *avg_us = 0.0;
+        return MI_OK;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    MI_TRY(rc);
-    MI_HIP(hipGetLastError());
-    *avg_us = total_ms * 1000.0 / reps;
-    return MI_OK;
+    if (precision == MI_PRECISION_FP64) {
+        StreamMat<double> V;
+        MI_TRY(stream_view64(A, s, V));
+        return time_cold<double>(V, A->nnz(), static_cast<const double*>(x),
+                                 static_cast<double*>(y), reps, flush_bytes, s, avg_us);
+    }
+    if (precision == MI_PRECISION_MIXED) {
+        StreamMat<float> V;
+        MI_TRY(stream_view32(A, s, false, V));
+        return time_cold<float>(V, A->nnz(), static_cast<const float*>(x), static_cast<float*>(y),
+                                reps, flush_bytes, s, avg_us);
+    }
+    return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);
 }
 
 MI_API int mi_csr_transpose(const mi_csr* A, mi_csr** out)

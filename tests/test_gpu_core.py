@@ -151,6 +151,44 @@ def test_spmv_long_rows(gpu):
     assert np.all(np.abs(y - yo) <= 1e-13 * scale)
 
 
+@pytest.mark.parametrize("far", [False, True])
+def test_spmv_column_ordered_blocks(gpu, far):
+    """Row blocks are stored in column order when every column lies within
+    2^20 rows of its block's first row, and in CSR order otherwise (far=True:
+    one entry of row 5 reaches the last column of a 1.05M-row matrix).
+    Either way the products are summed in CSR order: identical results."""
+    rng = np.random.default_rng(21)
+    n = (1 << 20) + 4096
+    rows = [np.arange(n)]
+    cols = [np.arange(n)]
+    for off in (-1, 1, -700, 700, -40000, 40000):
+        i = np.arange(max(0, -off), min(n, n - off))
+        rows.append(i)
+        cols.append(i + off)
+    if far:
+        rows.append(np.array([5]))
+        cols.append(np.array([n - 1]))
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    order = np.lexsort((c, r))
+    r, c = r[order], c[order]
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    ci = c.astype(np.int32)
+    va = rng.standard_normal(ci.size)
+    x = rng.standard_normal(n)
+    A = capi.Csr(rp, ci, va)
+    y = A.spmv(x)
+    yo = oracle.spmv(oracle.Matrix(rp, ci, va), x)
+    assert np.array_equal(y, yo)
+    dx = capi.DeviceArray.from_host(x.astype(np.float32))
+    dy = capi.DeviceArray(n, np.float32)
+    A.spmv_device(dx, dy, capi.MI_PRECISION_MIXED)
+    scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x)) + 1e-300
+    assert np.all(np.abs(dy.download().astype(np.float64) - yo) <= 2e-6 * scale)
+
+
 def test_spmv_empty_matrix(gpu):
     A = capi.Csr(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0))
     assert A.spmv(np.zeros(0)).size == 0
