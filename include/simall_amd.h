@@ -491,13 +491,27 @@ MI_API int mi_checkpoint_scalar(const mi_checkpoint* c, const char* name, double
 MI_API int mi_checkpoint_vector(const mi_checkpoint* c, const char* name, double* x, double* y,
                                 double* z, int64_t capacity, int64_t* count);
 MI_API int mi_checkpoint_close(mi_checkpoint* c);
+/* Optional AMG trailer (no counterpart in the reference's format; the
+ * reference's reader stops after the field section and ignores it): the SA-AMG
+ * hierarchy appended after the fields, so that a restarted run that keeps its
+ * preconditioner (mi_solver_cfg.keep_preconditioner) needs no AMG setup and
+ * continues bit for bit.  levels = 0 when the file has none. */
+MI_API int mi_checkpoint_amg_levels(const mi_checkpoint* c, int* levels);
+/* Appends a host hierarchy (mi_amg_host_build) to a checkpoint file. */
+MI_API int mi_checkpoint_append_amg(const char* path, const mi_amg_host* h);
 /* The device-resident flow state as fields "p" and "U" (the FieldRegistry
- * names of SimpleAlgorithm), with the caller's mesh arrays. */
+ * names of SimpleAlgorithm), with the caller's mesh arrays.  flags
+ * MI_CHECKPOINT_AMG also appends the pressure solver's AMG hierarchy
+ * (MI_ERR_UNSUPPORTED if it has built none). */
+enum { MI_CHECKPOINT_AMG = 1 };
 MI_API int mi_flow_checkpoint_write(mi_flow* f, const char* path, const mi_mesh_soa* mesh,
-                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta);
+                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta,
+                                    int flags);
 /* Loads "p" and "U" from a checkpoint of a mesh with this flow's cell count
  * and clears the time history (the next time step is implicit Euler, as the
- * first step of a new flow). */
+ * first step of a new flow).  An AMG trailer is adopted by the pressure
+ * solver when it uses AMG with keep_preconditioner (its next solve then
+ * skips the setup), and ignored otherwise. */
 MI_API int mi_flow_checkpoint_read(mi_flow* f, const char* path, mi_checkpoint_meta* meta);
 
 #ifdef __cplusplus

@@ -21,6 +21,7 @@ MI_ERR_BAD_ARG = 3
 MI_ERR_DEVICE = 4
 MI_ERR_NO_DEVICE = 5
 MI_ERR_UNSUPPORTED = 6
+MI_CHECKPOINT_AMG = 1
 MI_ERR_ZERO_PIVOT = 7
 
 MI_SOLVER_GMRES, MI_SOLVER_BICGSTAB, MI_SOLVER_CG, MI_SOLVER_TFQMR = 0, 1, 2, 3
@@ -204,7 +205,9 @@ PROTOTYPES = {
     "mi_checkpoint_scalar": (_i, [_vp, C.c_char_p, _vp, _i64, C.POINTER(_i64)]),
     "mi_checkpoint_vector": (_i, [_vp, C.c_char_p, _vp, _vp, _vp, _i64, C.POINTER(_i64)]),
     "mi_checkpoint_close": (_i, [_vp]),
-    "mi_flow_checkpoint_write": (_i, [_vp, C.c_char_p, _vp, _vp, _vp]),
+    "mi_checkpoint_amg_levels": (_i, [_vp, C.POINTER(_i)]),
+    "mi_checkpoint_append_amg": (_i, [C.c_char_p, _vp]),
+    "mi_flow_checkpoint_write": (_i, [_vp, C.c_char_p, _vp, _vp, _vp, _i]),
     "mi_flow_checkpoint_read": (_i, [_vp, C.c_char_p, _vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
@@ -542,6 +545,11 @@ def checkpoint_write(path, mesh, scalars=None, vectors=None, iteration=0, time=0
                                   len(vc), V, C.byref(meta)))
 
 
+def checkpoint_append_amg(path, amg_host):
+    """Appends an AmgHost hierarchy to a checkpoint file (the AMG trailer)."""
+    check(load().mi_checkpoint_append_amg(os.fsencode(path), amg_host.h))
+
+
 class Checkpoint:
     """A checkpoint file read into host memory (mi_checkpoint_open)."""
 
@@ -563,6 +571,13 @@ class Checkpoint:
         out = np.empty(cnt.value)
         check(self.lib.mi_checkpoint_scalar(self.h, name.encode(), ptr(out), out.size, C.byref(cnt)))
         return out
+
+    @property
+    def amg_levels(self):
+        """Levels of the AMG trailer (0: none)."""
+        lv = C.c_int()
+        check(self.lib.mi_checkpoint_amg_levels(self.h, C.byref(lv)))
+        return lv.value
 
     def vector(self, name):
         cnt = C.c_int64()
@@ -842,13 +857,15 @@ class Flow:
         check(self.lib.mi_flow_advance_time_step(self.h, n_inner, C.byref(r)))
         return r.as_dict()
 
-    def checkpoint_write(self, path, iteration=0, time=0.0, settings_hash=0):
-        """Writes p and U with the DeviceMesh's host mesh (mi_flow_checkpoint_write)."""
+    def checkpoint_write(self, path, iteration=0, time=0.0, settings_hash=0, amg=False):
+        """Writes p and U with the DeviceMesh's host mesh (mi_flow_checkpoint_write);
+        amg=True appends the pressure solver's AMG hierarchy."""
         m = self.mesh.host
         soa, nodes = mesh_soa(m), mesh_nodes(m)
         meta = CheckpointMeta(iteration, time, settings_hash)
         check(self.lib.mi_flow_checkpoint_write(self.h, os.fsencode(path), C.byref(soa),
-                                                C.byref(nodes), C.byref(meta)))
+                                                C.byref(nodes), C.byref(meta),
+                                                MI_CHECKPOINT_AMG if amg else 0))
 
     def checkpoint_read(self, path):
         """Loads p and U; returns (iteration, time, settings_hash)."""

@@ -484,6 +484,21 @@ struct AmgPrecond final : Precond
             return set_error(MI_ERR_UNSUPPORTED,
                              "AMG coarsening stalled above %d rows (no dense coarsest solve)",
                              kAmgMaxDense);
+        return build_device(need64, need32, s);
+    }
+    const AmgHierarchyHost* hierarchy() const override
+    {
+        return (has64 || has32) ? &host : nullptr;
+    }
+    int install(const AmgHierarchyHost& H, bool need64, bool need32, hipStream_t s) override
+    {
+        host = H;
+        host.coarse_inverse.clear();
+        return build_device(need64, need32, s);
+    }
+    // Device hierarchies from `host`.
+    int build_device(bool need64, bool need32, hipStream_t s)
+    {
         // the coarsest inverse, once for both precisions, on the device
         const int nc = host.coarse_n;
         double amax = 0.0;

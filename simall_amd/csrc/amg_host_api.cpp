@@ -2,6 +2,7 @@
 // RCM permutation the solver builds during setup, exposed for inspection
 // (no device is touched).
 #include "amg_setup.hpp"
+#include "checkpoint.hpp"
 #include "mi_common.hpp"
 #include "reorder.hpp"
 
@@ -14,6 +15,14 @@ struct mi_amg_host
 };
 
 using namespace mi;
+
+namespace mi
+{
+const AmgHierarchyHost* amg_host_hierarchy(const mi_amg_host* h)
+{
+    return h ? &h->h : nullptr;
+}
+} // namespace mi
 
 extern "C" {
 

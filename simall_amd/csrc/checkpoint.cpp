@@ -11,6 +11,7 @@
 // renamed over <path>, as the reference does.
 #include "simall_amd.h"
 
+#include "checkpoint.hpp"
 #include "mi_common.hpp"
 
 #include <algorithm>
@@ -18,6 +19,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -28,6 +30,7 @@ struct mi_checkpoint
     int64_t n_nodes = 0, n_faces = 0, n_cells = 0;
     std::map<std::string, std::vector<double>> scalars;
     std::map<std::string, std::vector<double>> vectors[3];
+    std::unique_ptr<mi::AmgHierarchyHost> amg; // optional trailer
 };
 
 namespace
@@ -108,8 +111,123 @@ struct In
 };
 
 constexpr char kMagic[8] = {'S', 'I', 'M', 'A', 'L', 'L', 'C', 'P'};
+constexpr char kAmgTag[8] = {'S', 'I', 'M', 'A', 'M', 'G', 'H', '1'};
+
+// AMG trailer: tag, u32 levels, per level {u32 has_P, CSR A, f64 lambda_max,
+// i32 aggregates, [CSR P, CSR R]}, i32 coarse n, dense coarsest operator
+// (u64 count + f64), f64 operator and grid complexity.  A CSR is i32 rows,
+// i32 columns, then rowPtr, colIdx, values as u64 count + elements.
+void put_csr(Out& o, const mi::HCsr& a)
+{
+    o.val(int32_t(a.nr));
+    o.val(int32_t(a.nc));
+    o.vec(a.rp.data(), uint64_t(a.rp.size()));
+    o.vec(a.ci.data(), uint64_t(a.ci.size()));
+    o.vec(a.v.data(), uint64_t(a.v.size()));
+}
+
+template <class T> bool get_vec(In& in, std::vector<T>& v, uint64_t expect)
+{
+    const uint64_t n = in.val<uint64_t>();
+    if (!in.ok || n != expect) {
+        in.ok = false;
+        return false;
+    }
+    v.resize(size_t(n));
+    in.bytes(v.data(), size_t(n) * sizeof(T));
+    return in.ok;
+}
+
+bool get_csr(In& in, mi::HCsr& a)
+{
+    a.nr = in.val<int32_t>();
+    a.nc = in.val<int32_t>();
+    if (!in.ok || a.nr < 0 || a.nc < 0 || !get_vec(in, a.rp, uint64_t(a.nr) + 1))
+        return false;
+    const int nnz = a.rp.back();
+    if (a.rp.front() != 0 || nnz < 0 || !get_vec(in, a.ci, uint64_t(nnz)) ||
+        !get_vec(in, a.v, uint64_t(nnz)))
+        return false;
+    for (int i = 0; i < a.nr; ++i)
+        if (a.rp[size_t(i) + 1] < a.rp[size_t(i)])
+            return false;
+    for (int c : a.ci)
+        if (c < 0 || c >= a.nc)
+            return false;
+    return true;
+}
+
+bool get_amg(In& in, mi::AmgHierarchyHost& h)
+{
+    const uint32_t nl = in.val<uint32_t>();
+    if (!in.ok || nl == 0 || nl > 64)
+        return false;
+    h.levels.assign(nl, mi::AmgLevelHost{});
+    for (uint32_t l = 0; l < nl; ++l) {
+        mi::AmgLevelHost& L = h.levels[l];
+        const uint32_t has_p = in.val<uint32_t>();
+        if (!get_csr(in, L.A))
+            return false;
+        L.lmax = in.val<double>();
+        L.n_aggregates = in.val<int32_t>();
+        if (has_p && (!get_csr(in, L.P) || !get_csr(in, L.R)))
+            return false;
+        if (!in.ok || (has_p != 0) != (l + 1 < nl))
+            return false;
+    }
+    h.coarse_n = in.val<int32_t>();
+    if (!in.ok || h.coarse_n != h.levels.back().A.nr ||
+        !get_vec(in, h.coarse_dense, uint64_t(h.coarse_n) * uint64_t(h.coarse_n)))
+        return false;
+    h.operator_complexity = in.val<double>();
+    h.grid_complexity = in.val<double>();
+    return in.ok;
+}
 
 } // namespace
+
+namespace mi
+{
+
+int checkpoint_append_amg(const char* path, const AmgHierarchyHost& h)
+{
+    if (!path || h.levels.empty() ||
+        h.coarse_dense.size() != size_t(h.coarse_n) * size_t(h.coarse_n))
+        return set_error(MI_ERR_BAD_ARG, "no hierarchy to append");
+    Out o;
+    o.f = std::fopen(path, "ab");
+    if (!o.f)
+        return set_error(MI_ERR_BAD_ARG, "cannot open %s", path);
+    o.bytes(kAmgTag, 8);
+    o.val(uint32_t(h.levels.size()));
+    for (size_t l = 0; l < h.levels.size(); ++l) {
+        const AmgLevelHost& L = h.levels[l];
+        const bool has_p = l + 1 < h.levels.size();
+        o.val(uint32_t(has_p ? 1 : 0));
+        put_csr(o, L.A);
+        o.val(L.lmax);
+        o.val(int32_t(L.n_aggregates));
+        if (has_p) {
+            put_csr(o, L.P);
+            put_csr(o, L.R);
+        }
+    }
+    o.val(int32_t(h.coarse_n));
+    o.vec(h.coarse_dense.data(), uint64_t(h.coarse_dense.size()));
+    o.val(h.operator_complexity);
+    o.val(h.grid_complexity);
+    const bool wrote = o.ok;
+    if (std::fclose(o.f) != 0 || !wrote)
+        return set_error(MI_ERR_DEVICE, "append to %s failed", path);
+    return MI_OK;
+}
+
+const AmgHierarchyHost* checkpoint_amg(const mi_checkpoint* c)
+{
+    return c ? c->amg.get() : nullptr;
+}
+
+} // namespace mi
 
 extern "C" {
 
@@ -249,6 +367,20 @@ MI_API int mi_checkpoint_open(const char* path, mi_checkpoint** out)
         }
         if (!in.ok)
             rc = set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed checkpoint", path);
+        // optional AMG trailer
+        char tag[8];
+        if (rc == MI_OK && std::fread(tag, 1, 8, in.f) == 8) {
+            if (std::memcmp(tag, kAmgTag, 8) != 0) {
+                rc = set_error(MI_ERR_BAD_ARG, "%s: unknown data after the field section", path);
+            } else {
+                c->amg = std::make_unique<mi::AmgHierarchyHost>();
+                char extra;
+                if (!get_amg(in, *c->amg))
+                    rc = set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed AMG trailer", path);
+                else if (std::fread(&extra, 1, 1, in.f) == 1)
+                    rc = set_error(MI_ERR_BAD_ARG, "%s: unknown data after the AMG trailer", path);
+            }
+        }
     }
     std::fclose(in.f);
     if (rc != MI_OK) {
@@ -309,6 +441,23 @@ MI_API int mi_checkpoint_vector(const mi_checkpoint* c, const char* name, double
                         sizeof(double) * size_t(std::min<int64_t>(capacity, int64_t(v.size()))));
     }
     return MI_OK;
+}
+
+MI_API int mi_checkpoint_amg_levels(const mi_checkpoint* c, int* levels)
+{
+    if (!c || !levels)
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    *levels = c->amg ? int(c->amg->levels.size()) : 0;
+    return MI_OK;
+}
+
+MI_API int mi_checkpoint_append_amg(const char* path, const mi_amg_host* h)
+{
+    mi::clear_error();
+    const mi::AmgHierarchyHost* H = mi::amg_host_hierarchy(h);
+    if (!H)
+        return set_error(MI_ERR_BAD_ARG, "null hierarchy");
+    return mi::checkpoint_append_amg(path, *H);
 }
 
 MI_API int mi_checkpoint_close(mi_checkpoint* c)

@@ -12,6 +12,7 @@
 // with contraction off every sum is the reference's sum bit for bit.  The
 // outer iteration keeps U, p, the face flux, the momentum diagonal and both
 // matrices in HBM; the host only sequences launches and the two solvers.
+#include "checkpoint.hpp"
 #include "mesh.hpp"
 #include "solver.hpp"
 #include "stream.hpp"
@@ -609,11 +610,18 @@ MI_API int mi_flow_destroy(mi_flow* F)
 }
 
 MI_API int mi_flow_checkpoint_write(mi_flow* F, const char* path, const mi_mesh_soa* mesh,
-                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta)
+                                    const mi_mesh_nodes* nodes, const mi_checkpoint_meta* meta,
+                                    int flags)
 {
     clear_error();
-    if (!F || !path || !mesh || !nodes)
-        return set_error(MI_ERR_BAD_ARG, "null argument");
+    if (!F || !path || !mesh || !nodes || (flags & ~MI_CHECKPOINT_AMG))
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    const AmgHierarchyHost* amg = nullptr;
+    if (flags & MI_CHECKPOINT_AMG) {
+        amg = solver_amg(F->sp.get());
+        if (!amg)
+            return set_error(MI_ERR_UNSUPPORTED, "the pressure solver has built no AMG hierarchy");
+    }
     if (mesh->n_cells != F->n || mesh->n_faces != F->nf)
         return set_error(MI_ERR_BAD_ARG, "mesh arrays do not match the flow's mesh");
     MI_TRY(ensure_device());
@@ -628,7 +636,8 @@ MI_API int mi_flow_checkpoint_write(mi_flow* F, const char* path, const mi_mesh_
     MI_HIP(hipStreamSynchronize(F->s));
     const mi_named_field sc{"p", h[3].data()};
     const mi_named_vector vc{"U", h[0].data(), h[1].data(), h[2].data()};
-    return mi_checkpoint_write(path, mesh, nodes, 1, &sc, 1, &vc, meta);
+    MI_TRY(mi_checkpoint_write(path, mesh, nodes, 1, &sc, 1, &vc, meta));
+    return amg ? checkpoint_append_amg(path, *amg) : MI_OK;
 }
 
 MI_API int mi_flow_checkpoint_read(mi_flow* F, const char* path, mi_checkpoint_meta* meta)
@@ -662,6 +671,12 @@ MI_API int mi_flow_checkpoint_read(mi_flow* F, const char* path, mi_checkpoint_m
     MI_HIP(hipStreamSynchronize(F->s));
     // no time history survives a restart: the next step starts it again
     F->time_step = 0;
+    // a cached pressure hierarchy replaces the first setup of a solver that
+    // keeps its preconditioner
+    const mi_solver_cfg& pc = F->cfg.pressure;
+    if (const AmgHierarchyHost* amg = checkpoint_amg(c))
+        if (pc.preconditioner == MI_PC_AMG && pc.keep_preconditioner)
+            MI_TRY(solver_adopt_amg(F->sp.get(), *amg));
     return MI_OK;
 }
 

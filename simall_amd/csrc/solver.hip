@@ -310,7 +310,7 @@ int build(mi_solver* S)
         S->pc.reset();
         S->built_version = 0;
     }
-    if (S->pc && S->built_version == A->version)
+    if (S->pc && S->built_version == A->version && !S->pending_amg)
         return MI_OK;
     mi_csr* M = S->M;
     if (S->ro) {
@@ -334,7 +334,7 @@ int build(mi_solver* S)
     }
     if (mixed)
         MI_TRY(M->ensure_fp32_op(S->stream));
-    if (S->pc && S->cfg.keep_preconditioner) {
+    if (S->pc && S->cfg.keep_preconditioner && !S->pending_amg) {
         // same pattern, new values: keep the preconditioner built on earlier values
         MI_HIP(hipStreamSynchronize(S->stream));
         S->built_version = A->version;
@@ -358,7 +358,17 @@ int build(mi_solver* S)
         S->pc->use_graph = S->cfg.use_graph != 0;
     }
     MI_HIP(hipStreamSynchronize(S->stream));
-    MI_TRY(S->pc->setup(M, !mixed, mixed, S->stream));
+    if (S->pending_amg) {
+        // an adopted hierarchy: it must have been built on this working pattern
+        std::unique_ptr<AmgHierarchyHost> H = std::move(S->pending_amg);
+        const HCsr& A0 = H->levels.front().A;
+        if (A0.nr != int(M->n()) || A0.rp != M->pat->h_rp || A0.ci != M->pat->h_ci)
+            return set_error(MI_ERR_BAD_ARG,
+                             "adopted AMG hierarchy was built on another matrix pattern");
+        MI_TRY(S->pc->install(*H, !mixed, mixed, S->stream));
+    } else {
+        MI_TRY(S->pc->setup(M, !mixed, mixed, S->stream));
+    }
     MI_TRY(ensure_workspace(S));
     MI_HIP(hipStreamSynchronize(S->stream));
     S->built_version = A->version;
@@ -885,6 +895,29 @@ int solve_device(mi_solver* S, const double* b, double* x, mi_stats* stats)
 }
 
 } // namespace
+
+namespace mi
+{
+
+int solver_adopt_amg(mi_solver* S, const AmgHierarchyHost& H)
+{
+    if (!S || !S->A)
+        return set_error(MI_ERR_BAD_ARG, "solver has no matrix: call mi_solver_setup first");
+    if (S->cfg.preconditioner != MI_PC_AMG || !S->cfg.keep_preconditioner)
+        return set_error(MI_ERR_UNSUPPORTED,
+                         "only an AMG solver that keeps its preconditioner adopts a hierarchy");
+    if (H.levels.empty())
+        return set_error(MI_ERR_BAD_ARG, "empty hierarchy");
+    S->pending_amg = std::make_unique<AmgHierarchyHost>(H);
+    return MI_OK;
+}
+
+const AmgHierarchyHost* solver_amg(const mi_solver* S)
+{
+    return S && S->pc ? S->pc->hierarchy() : nullptr;
+}
+
+} // namespace mi
 
 extern "C" {
 

@@ -12,6 +12,8 @@ namespace mi
 
 // A preconditioner z = M^-1 r in FP64 and/or FP32.  inv_diag*() non-null
 // means M is diagonal and the Krylov kernels fuse its application.
+struct AmgHierarchyHost;
+
 struct Precond
 {
     virtual ~Precond() = default;
@@ -29,6 +31,13 @@ struct Precond
     virtual const char* name() const = 0;
     virtual int levels() const { return 1; }
     virtual double complexity() const { return 1.0; }
+    // AMG only: the host hierarchy of the last setup, and building the device
+    // hierarchies from a given one instead of from a matrix.
+    virtual const AmgHierarchyHost* hierarchy() const { return nullptr; }
+    virtual int install(const AmgHierarchyHost&, bool, bool, hipStream_t)
+    {
+        return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no hierarchy", name());
+    }
     bool use_graph = true; // replay applies from captured HIP graphs
 };
 
@@ -49,6 +58,15 @@ struct Reordering
     int colors = 0; // multicolour ordering: colour count (0 for RCM)
 };
 
+// Makes the solver's next setup install H (an AMG hierarchy built for its
+// working matrix's pattern, e.g. from a checkpoint) instead of building one;
+// with keep_preconditioner the solves after it keep H.  Requires an AMG
+// configuration with keep_preconditioner and a bound matrix.
+int solver_adopt_amg(mi_solver* S, const AmgHierarchyHost& H);
+// The AMG hierarchy the solver's preconditioner was last set up with, or
+// nullptr.
+const AmgHierarchyHost* solver_amg(const mi_solver* S);
+
 } // namespace mi
 
 struct mi_solver
@@ -60,6 +78,9 @@ struct mi_solver
     std::unique_ptr<mi::Reordering> ro;
     uint64_t built_version = 0;
     std::unique_ptr<mi::Precond> pc;
+    // A hierarchy to install at the next setup instead of building one
+    // (solver_adopt_amg).
+    std::unique_ptr<mi::AmgHierarchyHost> pending_amg;
     hipStream_t stream = nullptr;
 
     mi::DevBuf<double> b, x, hist; // host-API staging of b and x; residual history

@@ -198,3 +198,48 @@ def test_reference_rejects_what_we_reject(lib, ref, tmp_path):
         assert rc == 1, name
         with pytest.raises(capi.MiError):
             capi.Checkpoint(path)
+
+
+def _amg_host(m):
+    """A host SA-AMG hierarchy of the mesh's pressure matrix (unit conductances)."""
+    import oracle
+    rp, ci = oracle.build_sparsity(m)
+    vals, _, _ = oracle.assemble_pressure(m, oracle.Matrix(rp, ci, np.zeros(ci.size)),
+                                          [wl.BC_WALL] * 7)
+    cfg = capi.default_cfg()
+    cfg.amg_coarse_size = 16
+    return capi.AmgHost(rp, ci, vals, cfg)
+
+
+def test_amg_trailer_round_trip(lib, tmp_path):
+    """The optional AMG trailer after the field section is read back with the
+    fields intact; a truncated trailer or foreign trailing data is refused."""
+    m = _mesh()
+    p, U = _fields(m.n_cells)
+    H = _amg_host(m)
+    assert H.levels >= 2
+    path = tmp_path / "amg.simcp"
+    capi.checkpoint_write(path, m, {"p": p}, {"U": U}, iteration=3)
+    assert capi.Checkpoint(path).amg_levels == 0
+    capi.checkpoint_append_amg(path, H)
+    c = capi.Checkpoint(path)
+    assert c.amg_levels == H.levels
+    assert np.array_equal(c.scalar("p"), p) and c.iteration == 3
+    b = open(path, "rb").read()
+    for name, data in (("trunc", b[:-5]), ("foreign", b[:-1] + b"\x00" * 9)):
+        bad = tmp_path / name
+        bad.write_bytes(data)
+        with pytest.raises(capi.MiError):
+            capi.Checkpoint(bad)
+
+
+@need_ref
+def test_reference_reads_file_with_amg_trailer(lib, ref, tmp_path):
+    m = _mesh()
+    p, U = _fields(m.n_cells)
+    path = tmp_path / "amg.simcp"
+    capi.checkpoint_write(path, m, {"p": p}, {"U": U}, iteration=9, time=1.0)
+    capi.checkpoint_append_amg(path, _amg_host(m))
+    rc, meta, sizes, out, have = _ref_read(ref, path, m.n_cells, m.n_faces)
+    assert rc == 0 and meta == (9, 1.0, 0) and have.tolist() == [1, 1]
+    assert np.array_equal(out["p"], p) and np.array_equal(out["U"], np.concatenate(U))

@@ -141,3 +141,39 @@ def test_flow_checkpoint_restart(gpu, tmp_path):
                           {"U": [np.zeros(m2.n_cells)] * 3})
     with pytest.raises(capi.MiError):
         B.checkpoint_read(tmp_path / "other.simcp")
+
+
+def test_flow_restart_with_cached_amg_hierarchy(gpu, tmp_path):
+    """With keep_preconditioner the pressure solver reuses the hierarchy of its
+    first solve.  A checkpoint carrying that hierarchy (the AMG trailer)
+    restarts bit for bit; without it the restarted solver builds a hierarchy
+    from the current matrix and the iterates differ."""
+    m, bc, bcv = _channel((8, 5, 5))
+    dm = capi.DeviceMesh(m, bc)
+    pres = dict(TIGHT, keep_preconditioner=1)
+    cfg = capi.flow_cfg(rho=1.0, mu=0.02, momentum=TIGHT, pressure=pres)
+    A = capi.Flow(dm, cfg, bcv)
+    A.iterate()
+    A.iterate()
+    with_amg, without = tmp_path / "amg.simcp", tmp_path / "plain.simcp"
+    A.checkpoint_write(with_amg, iteration=2, amg=True)
+    A.checkpoint_write(without, iteration=2)
+    assert capi.Checkpoint(with_amg).amg_levels >= 1
+    assert capi.Checkpoint(without).amg_levels == 0
+    B = capi.Flow(dm, cfg, bcv)
+    B.checkpoint_read(with_amg)
+    C = capi.Flow(dm, cfg, bcv)
+    C.checkpoint_read(without)
+    ra, rb, rc = A.iterate(), B.iterate(), C.iterate()
+    assert ra == rb
+    assert np.array_equal(B.p, A.p) and all(np.array_equal(a, b) for a, b in zip(B.U, A.U))
+    assert not np.array_equal(C.p, A.p)
+    assert _rel(C.p, A.p) < 1e-6
+    # a flow that rebuilds its preconditioner at every solve ignores the trailer
+    D = capi.Flow(dm, capi.flow_cfg(rho=1.0, mu=0.02, momentum=TIGHT, pressure=TIGHT), bcv)
+    D.checkpoint_read(with_amg)
+    D.iterate()
+    # the hierarchy needs a flow whose pressure solver has built one
+    E = capi.Flow(dm, cfg, bcv)
+    with pytest.raises(capi.MiError):
+        E.checkpoint_write(tmp_path / "none.simcp", amg=True)
