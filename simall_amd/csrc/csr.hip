@@ -232,7 +232,7 @@ template <class T> struct StreamCopy
         V = A;
         MI_TRY(dup(rp, A.rp, size_t(A.n) + 1));
         MI_TRY(dup(blk, A.blk, size_t(A.nrb) + 1));
-        MI_TRY(dup(hdr, A.hdr, 2 * (size_t(A.nrb) + 1)));
+        MI_TRY(dup(hdr, A.hdr, 4 * (size_t(A.nrb) + 1)));
         MI_TRY(dup(ci, A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci, size_t(nnz)));
         MI_TRY(dup(val, A.val, size_t(nnz)));
         V.rp = rp.p;

@@ -99,7 +99,7 @@ template <class T> struct CgArgs
     int nrb;
     const int* diag; // FP32 difference form: diagonal positions, else nullptr
     const unsigned* cw; // column-ordered row blocks (val in that order), else nullptr
-    const int* hdr;     // row block boundaries as {row, nonzero} pairs, else nullptr
+    const int* hdr;     // row block boundaries {row, nonzero, column base, 0}, else nullptr
     // FP32 inner solves of the mixed form: when set, the iterate is
     // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
     // then unused.  The caller's FP64 solution takes the correction directly

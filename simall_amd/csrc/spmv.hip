@@ -5,6 +5,8 @@
 #include "spmv.hpp"
 #include "stream.hpp"
 
+#include <climits>
+
 namespace mi
 {
 
@@ -67,25 +69,42 @@ int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide)
 }
 
 // One workgroup per row block: the block's (column, slot) keys sorted in LDS
-// by a bitonic network, then written as packed entries.  A block of one
-// long row keeps its CSR order (its columns are already ascending).
+// by a bitonic network, then written as packed entries relative to the
+// block's smallest column (written to the block header).  A block of one long row keeps its CSR order
+// (its columns are already ascending).
 __global__ __launch_bounds__(kBlock) void k_sort_blocks(const int* __restrict__ blk,
                                                         const int* __restrict__ rp,
                                                         const int* __restrict__ ci,
                                                         unsigned* __restrict__ cw,
                                                         int* __restrict__ sperm,
+                                                        int* __restrict__ hdr,
                                                         int* __restrict__ bad)
 {
     __shared__ unsigned key[kStreamNnz];
+    __shared__ int cmin[kBlock];
     const int rb = blockIdx.x;
     const int tid = threadIdx.x;
     const int r0 = blk[rb], r1 = blk[rb + 1];
     const int q0 = rp[r0], q1 = rp[r1];
     const int cnt = q1 - q0;
+    // smallest column of the block
+    int m = INT_MAX;
+    for (int k = q0 + tid; k < q1; k += kBlock)
+        m = min(m, ci[k]);
+    cmin[tid] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (tid < w)
+            cmin[tid] = min(cmin[tid], cmin[tid + w]);
+        __syncthreads();
+    }
+    const int c0 = cnt > 0 ? cmin[0] : 0;
+    if (tid == 0)
+        hdr[4 * rb + 2] = c0;
     // column field in [0, 2^21 - 1): a packed entry is never the -1 marker
     auto field = [&](int col) -> unsigned {
-        const int64_t d = int64_t(col) - r0 + kColBias;
-        if (d < 0 || d >= int64_t(kColMask))
+        const int64_t d = int64_t(col) - c0;
+        if (d >= int64_t(kColMask))
             bad[0] = 1;
         return unsigned(d) & kColMask;
     };
@@ -96,8 +115,8 @@ __global__ __launch_bounds__(kBlock) void k_sort_blocks(const int* __restrict__ 
         }
         return;
     }
-    for (int m = tid; m < kStreamNnz; m += kBlock)
-        key[m] = m < cnt ? (field(ci[q0 + m]) << 11) | unsigned(m) : 0xFFFFFFFFu;
+    for (int j = tid; j < kStreamNnz; j += kBlock)
+        key[j] = j < cnt ? (field(ci[q0 + j]) << 11) | unsigned(j) : 0xFFFFFFFFu;
     for (int size = 2; size <= kStreamNnz; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             __syncthreads();
@@ -114,11 +133,11 @@ __global__ __launch_bounds__(kBlock) void k_sort_blocks(const int* __restrict__ 
         }
     }
     __syncthreads();
-    for (int m = tid; m < cnt; m += kBlock) {
-        const unsigned kv = key[m];
+    for (int j = tid; j < cnt; j += kBlock) {
+        const unsigned kv = key[j];
         const unsigned slot = kv & 2047u;
-        cw[q0 + m] = (slot << kSlotShift) | (kv >> 11);
-        sperm[q0 + m] = q0 + int(slot);
+        cw[q0 + j] = (slot << kSlotShift) | (kv >> 11);
+        sperm[q0 + j] = q0 + int(slot);
     }
 }
 
@@ -155,13 +174,14 @@ int RowBlocks::sort_columns(const int* rp_dev, const int* ci_dev, int64_t nnz, h
     MI_TRY(perm.alloc(size_t(nnz)));
     MI_TRY(bad.alloc(1));
     MI_HIP(hipMemsetAsync(bad.p, 0, sizeof(int), s));
-    k_sort_blocks<<<unsigned(nrb), kBlock, 0, s>>>(blk.p, rp_dev, ci_dev, w.p, perm.p, bad.p);
+    k_sort_blocks<<<unsigned(nrb), kBlock, 0, s>>>(blk.p, rp_dev, ci_dev, w.p, perm.p, hdr.p,
+                                                   bad.p);
     MI_HIP(hipGetLastError());
     int h_bad = 0;
     MI_HIP(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
     if (h_bad)
-        return MI_OK; // a column too far from its block: CSR order stays
+        return MI_OK; // a block whose columns span too far: CSR order stays
     cw = std::move(w);
     sperm = std::move(perm);
     return MI_OK;
@@ -177,10 +197,10 @@ int RowBlocks::upload(int n, const int* rp_host, hipStream_t s, int lanes_per_ro
     build_row_blocks(n, rp_host, b, kStreamRows / lpr);
     nrb = int(b.size()) - 1;
     MI_TRY(blk.upload(b.data(), b.size(), s));
-    std::vector<int> h(2 * b.size());
+    std::vector<int> h(4 * b.size(), 0);
     for (size_t r = 0; r < b.size(); ++r) {
-        h[2 * r] = b[r];
-        h[2 * r + 1] = rp_host[b[r]];
+        h[4 * r] = b[r];
+        h[4 * r + 1] = rp_host[b[r]];
     }
     MI_TRY(hdr.upload(h.data(), h.size(), s));
     return hipStreamSynchronize(s) == hipSuccess ? MI_OK

@@ -52,24 +52,24 @@ void build_row_blocks(int n, const int* rp, std::vector<int>& blk, int max_rows 
 // best on the SA-AMG levels of a 2M-row 3D pressure system).
 int stream_lanes_per_row(int64_t n, int64_t nnz, bool wide);
 
-// Packed entry of a column-ordered row block: slot << 21 | (col - r0 + 2^20),
-// r0 = the block's first row, slot = position in the block's CSR span (0 for
-// a block of one long row, whose column order is its CSR order).
+// Packed entry of a column-ordered row block: slot << 21 | (col - c0), c0 =
+// the block's smallest column, slot = position in the block's CSR span (0
+// for a block of one long row, whose column order is its CSR order).
 constexpr int kSlotShift = 21;
 constexpr unsigned kColMask = (1u << kSlotShift) - 1u;
-constexpr int kColBias = 1 << 20;
 
 struct RowBlocks
 {
     DevBuf<int> blk;
-    // {first row, first nonzero} of every block boundary (nrb + 1 pairs):
-    // one load instead of blk -> rowPtr at the start of a block
+    // {first row, first nonzero, smallest column, 0} of every block boundary
+    // (nrb + 1 quadruples; the column base is set by sort_columns): one load
+    // instead of blk -> rowPtr at the start of a block
     DevBuf<int> hdr;
     int nrb = 0;
     int lpr = 1; // lanes per row (a power of two); rows per block <= kStreamRows / lpr
     // Column order (sort_columns): cw[m] packed entries, sperm[m] = CSR
-    // position of the m-th stored nonzero.  Empty when not built or when a
-    // column lies 2^20 or more rows from its block's first row.
+    // position of the m-th stored nonzero; each block's column base is in hdr.
+    // Empty when not built or when a block's columns span 2^21 - 1 or more.
     DevBuf<unsigned> cw;
     DevBuf<int> sperm;
     bool sorted() const { return cw.n > 0; }
@@ -120,24 +120,26 @@ template <class T> struct StreamMat
     // lpr).  1 sums each row in column order on one thread; more lanes take
     // every lpr-th product and combine with xor shuffles (fixed order).
     int lpr = 1;
-    // Column-ordered blocks: packed entries (RowBlocks::cw); val is then in
-    // the same order.  nullptr: ci and val in CSR order.
+    // Column-ordered blocks: packed entries (RowBlocks::cw), column bases in
+    // hdr (required with cw); val is then in the same order.  nullptr: ci and
+    // val in CSR order.
     const unsigned* cw = nullptr;
-    // Block boundaries as {first row, first nonzero} pairs (RowBlocks::hdr);
-    // nullptr: read blk and rp.
+    // Block boundaries as {first row, first nonzero, column base, 0}
+    // (RowBlocks::hdr); nullptr: read blk and rp.
     const int* hdr = nullptr;
 };
 
-// Rows r0..r1 and nonzeros k0..k1 of row block rb.
+// Rows r0..r1, nonzeros k0..k1 and column base c0 of row block rb.
 template <class T>
 __device__ __forceinline__ void stream_block(const StreamMat<T>& A, int rb, int& r0, int& r1,
-                                             int& k0, int& k1)
+                                             int& k0, int& k1, int& c0)
 {
     if (A.hdr) {
-        const int2 h0 = reinterpret_cast<const int2*>(A.hdr)[rb];
-        const int2 h1 = reinterpret_cast<const int2*>(A.hdr)[rb + 1];
+        const int4 h0 = reinterpret_cast<const int4*>(A.hdr)[rb];
+        const int4 h1 = reinterpret_cast<const int4*>(A.hdr)[rb + 1];
         r0 = h0.x;
         k0 = h0.y;
+        c0 = h0.z;
         r1 = h1.x;
         k1 = h1.y;
     } else {
@@ -145,16 +147,18 @@ __device__ __forceinline__ void stream_block(const StreamMat<T>& A, int rb, int&
         r1 = A.blk[rb + 1];
         k0 = A.rp[r0];
         k1 = A.rp[r1];
+        c0 = 0;
     }
 }
 
 // Column of a stream entry: the plain column, or decoded from a packed one
-// relative to the block's first row r0.
+// relative to the block's column base c0.
 template <class T>
-__device__ __forceinline__ int stream_col(const StreamMat<T>& A, int e, int r0)
+__device__ __forceinline__ int stream_col(const StreamMat<T>& A, int e, int c0)
 {
-    return A.cw ? r0 - kColBias + int(unsigned(e) & kColMask) : e;
+    return A.cw ? c0 + int(unsigned(e) & kColMask) : e;
 }
+
 // LDS slot of the j-th entry of thread tid.
 template <class T>
 __device__ __forceinline__ int stream_slot(const StreamMat<T>& A, int e, int tid, int j)
@@ -205,7 +209,7 @@ __device__ __forceinline__ bool stream_prefetch(const StreamMat<T>& A, int rb, i
 {
     if (rb + 1 >= rb1)
         return false;
-    const int q1 = A.hdr ? A.hdr[2 * (rb + 2) + 1] : A.rp[A.blk[rb + 2]];
+    const int q1 = A.hdr ? A.hdr[4 * (rb + 2) + 1] : A.rp[A.blk[rb + 2]];
     if (q1 - k1 > kStreamNnz)
         return false;
     stream_load(A, k1, q1, tid, c, a);
@@ -234,8 +238,8 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
     float a[kLoadsPerThread];
     bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
-        int r0, r1, k0, k1;
-        stream_block(A, rb, r0, r1, k0, k1);
+        int r0, r1, k0, k1, c0;
+        stream_block(A, rb, r0, r1, k0, k1, c0);
         if (k1 - k0 > kStreamNnz) {
             if (tid == 0)
                 epi.pre(r0);
@@ -245,7 +249,7 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             const int* src = A.cw ? reinterpret_cast<const int*>(A.cw) : A.ci;
             for (int k = k0 + tid; k < k1; k += kBlock) {
                 const float v = __builtin_nontemporal_load(&A.val[k]);
-                const float xj = x[stream_col(A, __builtin_nontemporal_load(&src[k]), r0)];
+                const float xj = x[stream_col(A, __builtin_nontemporal_load(&src[k]), c0)];
                 s += (k == kd) ? v * xi : v * (xj - xi);
             }
             av[tid] = s;
@@ -282,7 +286,7 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
             if (c[j] != -1) {
                 const int q = stream_slot(A, c[j], tid, j);
                 av[q] = a[j];
-                xv[q] = x[stream_col(A, c[j], r0)];
+                xv[q] = x[stream_col(A, c[j], c0)];
             }
         }
         __syncthreads();
@@ -325,8 +329,8 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
     T a[kLoadsPerThread];
     bool have = false;
     for (int rb = rb0; rb < rb1; ++rb) {
-        int r0, r1, k0, k1;
-        stream_block(A, rb, r0, r1, k0, k1);
+        int r0, r1, k0, k1, c0;
+        stream_block(A, rb, r0, r1, k0, k1, c0);
         if (k1 - k0 > kStreamNnz) {
             // one long row: strided partial sums, fixed-order block reduction
             if (tid == 0)
@@ -335,7 +339,7 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
             Acc s = 0;
             for (int k = k0 + tid; k < k1; k += kBlock)
                 s += Acc(__builtin_nontemporal_load(&A.val[k])) *
-                     Acc(x[stream_col(A, __builtin_nontemporal_load(&src[k]), r0)]);
+                     Acc(x[stream_col(A, __builtin_nontemporal_load(&src[k]), c0)]);
             prod[tid] = s;
             __syncthreads();
             if (tid == 0) {
@@ -367,7 +371,7 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
 #pragma unroll
         for (int j = 0; j < kLoadsPerThread; ++j)
             if (c[j] != -1)
-                prod[stream_slot(A, c[j], tid, j)] = Acc(a[j]) * Acc(x[stream_col(A, c[j], r0)]);
+                prod[stream_slot(A, c[j], tid, j)] = Acc(a[j]) * Acc(x[stream_col(A, c[j], c0)]);
         __syncthreads();
         // FP32 blocks are short on bytes per synchronisation, so the next
         // block's streams are prefetched; for FP64 the extra live registers

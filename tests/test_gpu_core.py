@@ -153,12 +153,12 @@ def test_spmv_long_rows(gpu):
 
 @pytest.mark.parametrize("far", [False, True])
 def test_spmv_column_ordered_blocks(gpu, far):
-    """Row blocks are stored in column order when every column lies within
-    2^20 rows of its block's first row, and in CSR order otherwise (far=True:
-    one entry of row 5 reaches the last column of a 1.05M-row matrix).
-    Either way the products are summed in CSR order: identical results."""
+    """Row blocks are stored in column order when each block's columns span
+    fewer than 2^21 - 1, and in CSR order otherwise (far=True: one entry of
+    row 5 reaches the last column of a 2.1M-row matrix).  Either way the
+    products are summed in CSR order: identical results."""
     rng = np.random.default_rng(21)
-    n = (1 << 20) + 4096
+    n = (1 << 21) + 4096
     rows = [np.arange(n)]
     cols = [np.arange(n)]
     for off in (-1, 1, -700, 700, -40000, 40000):
