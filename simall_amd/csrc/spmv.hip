@@ -222,6 +222,66 @@ __global__ __launch_bounds__(kBlock) void k_stream_spmv(StreamMat<T> A, const T*
     stream_rows(A, x, e);
 }
 
+// y = A x for one column-ordered row block per workgroup with one thread per
+// row: the CSR-stream engine's arithmetic (products at their CSR slots, rows
+// summed in column order, long rows by the same strided fold), without the
+// epilogue, lane-group and persistent-range machinery the solver kernels need.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmv_cols(StreamMat<T> A, const T* __restrict__ x,
+                                                      T* __restrict__ y)
+{
+#pragma clang fp contract(off)
+    using Acc = typename acc_type<T>::type;
+    Acc* prod = reinterpret_cast<Acc*>(stream_lds());
+    const int rb = xcd_logical_block(blockIdx.x, gridDim.x);
+    if (rb >= A.nrb)
+        return;
+    const int4 h0 = reinterpret_cast<const int4*>(A.hdr)[rb];
+    const int4 h1 = reinterpret_cast<const int4*>(A.hdr)[rb + 1];
+    const int r0 = h0.x, k0 = h0.y, c0 = h0.z, r1 = h1.x, k1 = h1.y;
+    const int tid = threadIdx.x;
+    if (k1 - k0 > kStreamNnz) {
+        Acc s = 0;
+        for (int k = k0 + tid; k < k1; k += kBlock)
+            s += Acc(__builtin_nontemporal_load(&A.val[k])) *
+                 Acc(x[c0 + int(__builtin_nontemporal_load(&A.cw[k]) & kColMask)]);
+        prod[tid] = s;
+        __syncthreads();
+        if (tid == 0) {
+            Acc t = 0;
+            for (int j = 0; j < kBlock; ++j)
+                t += prod[j];
+            y[r0] = T(t);
+        }
+        return;
+    }
+    const int i = r0 + tid;
+    int kb = 0, ke = 0;
+    if (i < r1) {
+        kb = A.rp[i] - k0;
+        ke = A.rp[i + 1] - k0;
+    }
+    unsigned w[kLoadsPerThread];
+    T a[kLoadsPerThread];
+#pragma unroll
+    for (int j = 0; j < kLoadsPerThread; ++j) {
+        const int k = k0 + tid + j * kBlock;
+        w[j] = k < k1 ? __builtin_nontemporal_load(&A.cw[k]) : 0xFFFFFFFFu;
+        a[j] = k < k1 ? __builtin_nontemporal_load(&A.val[k]) : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < kLoadsPerThread; ++j)
+        if (w[j] != 0xFFFFFFFFu)
+            prod[w[j] >> kSlotShift] = Acc(a[j]) * Acc(x[c0 + int(w[j] & kColMask)]);
+    __syncthreads();
+    if (i < r1) {
+        Acc s = 0;
+        for (int k = kb; k < ke; ++k)
+            s += prod[k];
+        y[i] = T(s);
+    }
+}
+
 template <class T> int launch_spmv_stream(const Pattern& P, const T* val, const T* x, T* y,
                                           hipStream_t s)
 {
@@ -238,6 +298,11 @@ template <class T> int launch_spmv_view(const StreamMat<T>& A, const T* x, T* y,
     // dispatcher hands out blocks as workgroups retire; measured on the 128^3
     // pressure matrix from HBM 47.9 vs 48.9 us.  FP32 keeps persistent ranges,
     // whose next-block prefetch is worth more (39.7 vs 38.1 us).
+    if (sizeof(T) == 8 && A.cw && A.hdr && A.lpr == 1) {
+        k_spmv_cols<T><<<unsigned((int64_t(A.nrb) + 7) / 8 * 8), kBlock, 0, s>>>(A, x, y);
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
     const unsigned G = sizeof(T) == 8 ? unsigned((int64_t(A.nrb) + 7) / 8 * 8)
                                       : stream_grid(A.nrb);
     k_stream_spmv<T><<<G, kBlock, 0, s>>>(A, x, y);
