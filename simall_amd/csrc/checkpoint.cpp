@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,14 @@ struct In
 {
     FILE* f = nullptr;
     bool ok = true;
+    uint64_t size = 0; // file size: no array may claim more than what is left
+    bool fits(uint64_t n, size_t elem)
+    {
+        const long pos = std::ftell(f);
+        if (pos < 0 || n > (uint64_t(1) << 40) || n * elem > size - uint64_t(pos))
+            ok = false;
+        return ok;
+    }
     void bytes(void* p, size_t n)
     {
         if (ok && n && std::fread(p, 1, n, f) != n)
@@ -80,6 +89,8 @@ struct In
     uint64_t skip(size_t elem)
     {
         const uint64_t n = val<uint64_t>();
+        if (ok)
+            fits(n, elem);
         if (ok && n && std::fseek(f, long(n * elem), SEEK_CUR) != 0)
             ok = false;
         return n;
@@ -88,10 +99,8 @@ struct In
     {
         const uint64_t n = val<uint64_t>();
         std::vector<double> v;
-        if (!ok || n > (uint64_t(1) << 40)) {
-            ok = false;
+        if (!ok || !fits(n, sizeof(double)))
             return v;
-        }
         v.resize(size_t(n));
         bytes(v.data(), size_t(n) * sizeof(double));
         return v;
@@ -100,10 +109,8 @@ struct In
     {
         const uint32_t n = val<uint32_t>();
         std::string s;
-        if (!ok || n > (1u << 20)) {
-            ok = false;
+        if (!ok || n > (1u << 20) || !fits(n, 1))
             return s;
-        }
         s.resize(n);
         bytes(s.data(), n);
         return s;
@@ -129,7 +136,7 @@ void put_csr(Out& o, const mi::HCsr& a)
 template <class T> bool get_vec(In& in, std::vector<T>& v, uint64_t expect)
 {
     const uint64_t n = in.val<uint64_t>();
-    if (!in.ok || n != expect) {
+    if (!in.ok || n != expect || !in.fits(n, sizeof(T))) {
         in.ok = false;
         return false;
     }
@@ -229,6 +236,72 @@ const AmgHierarchyHost* checkpoint_amg(const mi_checkpoint* c)
 
 } // namespace mi
 
+namespace
+{
+
+// Header, meta, the mesh section (counts only), the fields and the optional
+// AMG trailer of an open checkpoint file.
+int read_checkpoint(In& in, mi_checkpoint& c, const char* path)
+{
+    char magic[8];
+    in.bytes(magic, 8);
+    c.version = in.val<uint32_t>();
+    (void)in.val<uint32_t>(); // flags
+    (void)in.val<uint64_t>(); // timestamp
+    if (!in.ok || std::memcmp(magic, kMagic, 8) != 0)
+        return set_error(MI_ERR_BAD_ARG, "%s: not a SIMALLCP checkpoint", path);
+    if (c.version > 1)
+        return set_error(MI_ERR_UNSUPPORTED, "%s: checkpoint version %u not supported", path,
+                         c.version);
+    c.meta.iteration = in.val<int32_t>();
+    (void)in.val<uint32_t>();
+    c.meta.time = in.val<double>();
+    c.meta.settings_hash = in.val<uint64_t>();
+    // mesh section: counts only
+    c.n_nodes = int64_t(in.skip(8));
+    in.skip(8);
+    in.skip(8);
+    c.n_faces = int64_t(in.skip(8)); // owner
+    in.skip(8);                       // neighbour
+    for (int k = 0; k < 6; ++k)
+        in.skip(8); // areas, centroids
+    in.skip(4);     // boundary zone
+    in.skip(4);     // node offsets
+    in.skip(8);     // node indices
+    c.n_cells = int64_t(in.skip(8)); // volume
+    for (int k = 0; k < 3; ++k)
+        in.skip(8);
+    in.skip(4);
+    in.skip(8);
+    const uint32_t ns = in.val<uint32_t>(), nv = in.val<uint32_t>();
+    for (uint32_t i = 0; in.ok && i < ns; ++i) {
+        std::string name = in.str();
+        c.scalars[name] = in.doubles();
+    }
+    for (uint32_t i = 0; in.ok && i < nv; ++i) {
+        std::string name = in.str();
+        for (int d = 0; d < 3; ++d)
+            c.vectors[d][name] = in.doubles();
+    }
+    if (!in.ok)
+        return set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed checkpoint", path);
+    // optional AMG trailer
+    char tag[8];
+    if (std::fread(tag, 1, 8, in.f) != 8)
+        return MI_OK;
+    if (std::memcmp(tag, kAmgTag, 8) != 0)
+        return set_error(MI_ERR_BAD_ARG, "%s: unknown data after the field section", path);
+    c.amg = std::make_unique<mi::AmgHierarchyHost>();
+    if (!get_amg(in, *c.amg))
+        return set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed AMG trailer", path);
+    char extra;
+    if (std::fread(&extra, 1, 1, in.f) == 1)
+        return set_error(MI_ERR_BAD_ARG, "%s: unknown data after the AMG trailer", path);
+    return MI_OK;
+}
+
+} // namespace
+
 extern "C" {
 
 MI_API int mi_checkpoint_write(const char* path, const mi_mesh_soa* m, const mi_mesh_nodes* nodes,
@@ -318,68 +391,21 @@ MI_API int mi_checkpoint_open(const char* path, mi_checkpoint** out)
     in.f = std::fopen(path, "rb");
     if (!in.f)
         return set_error(MI_ERR_BAD_ARG, "cannot open %s", path);
-    auto c = new (std::nothrow) mi_checkpoint();
-    if (!c) {
-        std::fclose(in.f);
-        return set_error(MI_ERR_DEVICE, "out of host memory");
+    if (std::fseek(in.f, 0, SEEK_END) == 0) {
+        const long end = std::ftell(in.f);
+        in.size = end > 0 ? uint64_t(end) : 0;
     }
-    char magic[8];
-    in.bytes(magic, 8);
-    c->version = in.val<uint32_t>();
-    (void)in.val<uint32_t>(); // flags
-    (void)in.val<uint64_t>(); // timestamp
+    std::rewind(in.f);
+    auto c = new (std::nothrow) mi_checkpoint();
     int rc = MI_OK;
-    if (!in.ok || std::memcmp(magic, kMagic, 8) != 0) {
-        rc = set_error(MI_ERR_BAD_ARG, "%s: not a SIMALLCP checkpoint", path);
-    } else if (c->version > 1) {
-        rc = set_error(MI_ERR_UNSUPPORTED, "%s: checkpoint version %u not supported", path,
-                       c->version);
+    if (!c) {
+        rc = set_error(MI_ERR_DEVICE, "out of host memory");
     } else {
-        c->meta.iteration = in.val<int32_t>();
-        (void)in.val<uint32_t>();
-        c->meta.time = in.val<double>();
-        c->meta.settings_hash = in.val<uint64_t>();
-        // mesh section: counts only
-        c->n_nodes = int64_t(in.skip(8));
-        in.skip(8);
-        in.skip(8);
-        c->n_faces = int64_t(in.skip(8)); // owner
-        in.skip(8);                        // neighbour
-        for (int k = 0; k < 6; ++k)
-            in.skip(8); // areas, centroids
-        in.skip(4);     // boundary zone
-        in.skip(4);     // node offsets
-        in.skip(8);     // node indices
-        c->n_cells = int64_t(in.skip(8)); // volume
-        for (int k = 0; k < 3; ++k)
-            in.skip(8);
-        in.skip(4);
-        in.skip(8);
-        const uint32_t ns = in.val<uint32_t>(), nv = in.val<uint32_t>();
-        for (uint32_t i = 0; in.ok && i < ns; ++i) {
-            std::string name = in.str();
-            c->scalars[name] = in.doubles();
-        }
-        for (uint32_t i = 0; in.ok && i < nv; ++i) {
-            std::string name = in.str();
-            for (int d = 0; d < 3; ++d)
-                c->vectors[d][name] = in.doubles();
-        }
-        if (!in.ok)
-            rc = set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed checkpoint", path);
-        // optional AMG trailer
-        char tag[8];
-        if (rc == MI_OK && std::fread(tag, 1, 8, in.f) == 8) {
-            if (std::memcmp(tag, kAmgTag, 8) != 0) {
-                rc = set_error(MI_ERR_BAD_ARG, "%s: unknown data after the field section", path);
-            } else {
-                c->amg = std::make_unique<mi::AmgHierarchyHost>();
-                char extra;
-                if (!get_amg(in, *c->amg))
-                    rc = set_error(MI_ERR_BAD_ARG, "%s: truncated or malformed AMG trailer", path);
-                else if (std::fread(&extra, 1, 1, in.f) == 1)
-                    rc = set_error(MI_ERR_BAD_ARG, "%s: unknown data after the AMG trailer", path);
-            }
+        try {
+            rc = read_checkpoint(in, *c, path);
+        } catch (const std::bad_alloc&) {
+            // sizes read from a damaged file
+            rc = set_error(MI_ERR_BAD_ARG, "%s: sizes in the checkpoint exceed host memory", path);
         }
     }
     std::fclose(in.f);

@@ -3,6 +3,7 @@
 #include "krylov.hpp"
 #include "spmv.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -249,8 +250,10 @@ template <class T>
 static int time_cold(const StreamMat<T>& A, int64_t nnz, const T* x, T* y, int reps,
                      int64_t flush_bytes, hipStream_t s, double* avg_us)
 {
+    // enough copies to exceed flush_bytes, at most 64 (a matrix smaller than
+    // flush_bytes / 64 is timed with 64 copies)
     const double per = double(nnz) * double(4 + sizeof(T)) + double(A.n + 1) * 4.0;
-    const int nc = int(std::ceil(double(flush_bytes) / per)) + 1;
+    const int nc = int(std::min(64.0, std::ceil(double(flush_bytes) / per) + 1.0));
     std::vector<StreamCopy<T>> cp(static_cast<size_t>(nc));
     for (auto& c : cp)
         MI_TRY(c.make(A, nnz, s));

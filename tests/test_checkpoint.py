@@ -243,3 +243,20 @@ def test_reference_reads_file_with_amg_trailer(lib, ref, tmp_path):
     rc, meta, sizes, out, have = _ref_read(ref, path, m.n_cells, m.n_faces)
     assert rc == 0 and meta == (9, 1.0, 0) and have.tolist() == [1, 1]
     assert np.array_equal(out["p"], p) and np.array_equal(out["U"], np.concatenate(U))
+
+
+def test_rejects_counts_beyond_the_file(lib, tmp_path):
+    """An array count larger than what is left of the file is refused before
+    anything is allocated for it."""
+    m = _mesh()
+    path = tmp_path / "c.simcp"
+    capi.checkpoint_write(path, m, {"p": np.zeros(m.n_cells)})
+    b = bytearray(open(path, "rb").read())
+    # the count of the last array (field "p") sits 8 * n_cells + 8 bytes from the end
+    pos = len(b) - 8 * m.n_cells - 8
+    assert np.frombuffer(bytes(b[pos:pos + 8]), np.uint64)[0] == m.n_cells
+    b[pos:pos + 8] = np.uint64(1 << 39).tobytes()
+    bad = tmp_path / "huge.simcp"
+    bad.write_bytes(bytes(b))
+    with pytest.raises(capi.MiError):
+        capi.Checkpoint(bad)
