@@ -73,14 +73,14 @@ template <class T> struct EpiCheb
     }
 };
 
-template <class T>
+template <class T, bool kSell>
 __global__ __launch_bounds__(kBlock) void k_cheb_step(StreamMat<T> A, const T* __restrict__ din,
                                                       T* __restrict__ dout, T* __restrict__ r,
                                                       const T* __restrict__ invD,
                                                       T* __restrict__ x, T beta, T alpha, int mode)
 {
     EpiCheb<T> e{din, dout, r, invD, x, beta, alpha, mode};
-    stream_rows(A, din, e);
+    rows_of<kSell>(A, din, e);
 }
 
 // e = P x_c ; x += e
@@ -97,12 +97,12 @@ template <class T> struct EpiProlong
     }
 };
 
-template <class T>
+template <class T, bool kSell>
 __global__ __launch_bounds__(kBlock) void k_prolong(StreamMat<T> P, const T* __restrict__ xc,
                                                     T* __restrict__ e, T* __restrict__ x)
 {
     EpiProlong<T> ep{e, x};
-    stream_rows(P, xc, ep);
+    rows_of<kSell>(P, xc, ep);
 }
 
 // b_c = R r
@@ -113,12 +113,12 @@ template <class T> struct EpiStoreT
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
 };
 
-template <class T>
+template <class T, bool kSell>
 __global__ __launch_bounds__(kBlock) void k_restrict(StreamMat<T> R, const T* __restrict__ r,
                                                      T* __restrict__ bc)
 {
     EpiStoreT<T> e{bc};
-    stream_rows(R, r, e);
+    rows_of<kSell>(R, r, e);
 }
 
 // x = Ainv b, dense FP64, one workgroup per row: the n_c workgroups keep
@@ -183,7 +183,16 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
     MI_TRY(v.upload(tv.data(), tv.size(), s));
     // the hierarchy's operators need no column-order sums: long rows get lanes
     MI_TRY(rb.upload(nr, h.rp.data(), s, stream_lanes_per_row(nr, nnz, true))); // synchronizes
-    if (nr > 0 && column_order_enabled()) {
+    MI_TRY(sell.build(nr, nnz, h.rp.data(), h.ci.data(), nr == nc, s));
+    if (sell.built()) {
+        // values kept only in SELL-64 slot order (sm() streams them)
+        DevBuf<T> ev;
+        MI_TRY(ev.alloc(size_t(sell.padded)));
+        MI_TRY(launch_permute<T>(sell.padded, sell.eperm.p, v.p, ev.p, s));
+        MI_HIP(hipStreamSynchronize(s));
+        v = std::move(ev);
+        sell.eperm.release();
+    } else if (nr > 0 && column_order_enabled()) {
         // values kept only in the column order of the row blocks (sm() streams them)
         MI_TRY(rb.sort_columns(rp.p, ci.p, nnz, s));
         if (rb.sorted()) {
@@ -202,8 +211,12 @@ template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r
 {
     if (A.nr == 0)
         return MI_OK;
-    k_cheb_step<T><<<stream_grid(A.rb.nrb), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
-                                                              T(alpha), mode);
+    if (A.sell.built())
+        k_cheb_step<T, true><<<A.grid(), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
+                                                         T(alpha), mode);
+    else
+        k_cheb_step<T, false><<<A.grid(), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
+                                                          T(alpha), mode);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -414,8 +427,8 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         }
         MI_TRY(step<T>(d.A, din, nullptr, d.r.p, d.invD.p, x, 0.0, 0.0, 1, s));
         if (d.R.nr > 0)
-            k_restrict<T><<<stream_grid(d.R.rb.nrb), kBlock, 0, s>>>(d.R.sm(), d.r.p,
-                                                                      L[size_t(l) + 1].b.p);
+            (d.R.sell.built() ? k_restrict<T, true> : k_restrict<T, false>)<<<d.R.grid(), kBlock, 0, s>>>(
+                d.R.sm(), d.r.p, L[size_t(l) + 1].b.p);
         MI_HIP(hipGetLastError());
     }
     // coarsest level
@@ -434,7 +447,8 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         T* x = xptr(l);
         const T* xc = xptr(l + 1);
         if (n > 0)
-            k_prolong<T><<<stream_grid(d.P.rb.nrb), kBlock, 0, s>>>(d.P.sm(), xc, d.e.p, x);
+            (d.P.sell.built() ? k_prolong<T, true> : k_prolong<T, false>)<<<d.P.grid(), kBlock, 0, s>>>(
+                d.P.sm(), xc, d.e.p, x);
         MI_HIP(hipGetLastError());
         // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps
         MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, s));

@@ -16,18 +16,28 @@ template <class T> struct DMat
     int64_t nnz = 0;
     int lanes = 8;
     DevBuf<int> rp, ci;
-    DevBuf<T> v;    // in the column order of rb when rb.sorted()
+    DevBuf<T> v;    // in SELL-64 slot order when sell.built(), else in the column order of rb when rb.sorted()
     DevBuf<int> dg; // diagonal positions (CSR) when v holds the FP32 difference form
     RowBlocks rb;
+    SellForm sell;  // short rows only (the finest operator, the prolongators)
     // diff_form: store the FP32 difference form (stream_rows_diff) when every
     // row holds its diagonal; ignored for FP64.
     int upload(const HCsr& h, hipStream_t s, bool diff_form = false);
 #if defined(__HIPCC__)
     StreamMat<T> sm() const
     {
-        return StreamMat<T>{nr,     rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr,
-                            rb.lpr, rb.sorted() ? rb.cw.p : nullptr, rb.hdr.p};
+        StreamMat<T> m{nr,     rb.nrb, rb.blk.p, rp.p, ci.p, v.p, dg.n ? dg.p : nullptr,
+                       rb.lpr, rb.sorted() ? rb.cw.p : nullptr, rb.hdr.p};
+        if (sell.built()) {
+            m.cw = nullptr;
+            m.soff = sell.soff.p;
+            m.eci = sell.eci.p;
+            m.nsl = sell.nsl;
+        }
+        return m;
     }
+    // Workgroups of a launch over sm().
+    unsigned grid() const { return sell.built() ? sell_grid(sell.nsl) : stream_grid(rb.nrb); }
 #endif
 };
 

@@ -110,6 +110,7 @@ int make_pattern(int64_t n, int64_t nnz, const int32_t* rp, const int32_t* ci,
         MI_TRY(p->rb.upload(int(n), p->h_rp.data(), s));
         if (column_order_enabled())
             MI_TRY(p->rb.sort_columns(p->rp.p, p->ci.p, nnz, s));
+        MI_TRY(p->sell.build(int(n), nnz, p->h_rp.data(), p->h_ci.data(), true, s));
     }
     out = std::move(p);
     return MI_OK;
@@ -126,12 +127,38 @@ static int permuted_copy(const Pattern& P, const T* src, DevBuf<T>& dst, bool& o
     return MI_OK;
 }
 
-int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out)
+template <class T>
+static int sell_copy(const Pattern& P, const T* src, DevBuf<T>& dst, bool& ok, hipStream_t s)
+{
+    if (ok)
+        return MI_OK;
+    if (dst.n < size_t(P.sell.padded))
+        MI_TRY(dst.alloc(size_t(P.sell.padded)));
+    MI_TRY(launch_permute<T>(P.sell.padded, P.sell.eperm.p, src, dst.p, s));
+    ok = true;
+    return MI_OK;
+}
+
+template <class T> static void sell_fields(const Pattern& P, StreamMat<T>& out)
+{
+    out.soff = P.sell.soff.p;
+    out.eci = P.sell.eci.p;
+    out.nsl = P.sell.nsl;
+    out.cw = nullptr;
+}
+
+int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out, bool sell)
 {
     const Pattern& P = *A->pat;
     out = StreamMat<double>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, A->val.p, nullptr,
                             P.rb.lpr};
     out.hdr = P.rb.hdr.p;
+    if (sell && P.sell.built()) {
+        MI_TRY(sell_copy(P, A->val.p, A->e_val, A->e_val_ok, s));
+        out.val = A->e_val.p;
+        sell_fields(P, out);
+        return MI_OK;
+    }
     if (P.rb.sorted()) {
         MI_TRY(permuted_copy(P, A->val.p, A->s_val, A->s_val_ok, s));
         out.val = A->s_val.p;
@@ -140,7 +167,7 @@ int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out)
     return MI_OK;
 }
 
-int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out)
+int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out, bool sell)
 {
     const Pattern& P = *A->pat;
     if (op)
@@ -151,6 +178,15 @@ int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out)
     out = StreamMat<float>{int(P.n), P.rb.nrb, P.rb.blk.p, P.rp.p, P.ci.p, v,
                            op ? A->diag32() : nullptr, P.rb.lpr};
     out.hdr = P.rb.hdr.p;
+    if (sell && P.sell.built()) {
+        if (op)
+            MI_TRY(sell_copy(P, v, A->e_op32, A->e_op32_ok, s));
+        else
+            MI_TRY(sell_copy(P, v, A->e_val32, A->e_val32_ok, s));
+        out.val = op ? A->e_op32.p : A->e_val32.p;
+        sell_fields(P, out);
+        return MI_OK;
+    }
     if (P.rb.sorted()) {
         if (op)
             MI_TRY(permuted_copy(P, v, A->s_op32, A->s_op32_ok, s));
@@ -218,10 +254,11 @@ int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s)
 // not in any cache.
 template <class T> struct StreamCopy
 {
-    DevBuf<int> rp, blk, hdr, ci;
+    DevBuf<int> rp, blk, hdr, ci, soff;
     DevBuf<T> val;
     StreamMat<T> V;
-    int make(const StreamMat<T>& A, int64_t nnz, hipStream_t s)
+    // padded: slots of the SELL-64 form when A carries it
+    int make(const StreamMat<T>& A, int64_t nnz, int64_t padded, hipStream_t s)
     {
         auto dup = [&](auto& dst, const auto* src, size_t count) -> int {
             if (!src || count == 0)
@@ -231,6 +268,15 @@ template <class T> struct StreamCopy
             return MI_OK;
         };
         V = A;
+        if (A.soff) {
+            MI_TRY(dup(soff, A.soff, size_t(A.nsl) + 1));
+            MI_TRY(dup(ci, A.eci, size_t(padded)));
+            MI_TRY(dup(val, A.val, size_t(padded)));
+            V.soff = soff.p;
+            V.eci = ci.p;
+            V.val = val.p;
+            return MI_OK;
+        }
         MI_TRY(dup(rp, A.rp, size_t(A.n) + 1));
         MI_TRY(dup(blk, A.blk, size_t(A.nrb) + 1));
         MI_TRY(dup(hdr, A.hdr, 4 * (size_t(A.nrb) + 1)));
@@ -247,8 +293,8 @@ template <class T> struct StreamCopy
 };
 
 template <class T>
-static int time_cold(const StreamMat<T>& A, int64_t nnz, const T* x, T* y, int reps,
-                     int64_t flush_bytes, hipStream_t s, double* avg_us)
+static int time_cold(const StreamMat<T>& A, int64_t nnz, int64_t padded, const T* x, T* y,
+                     int reps, int64_t flush_bytes, hipStream_t s, double* avg_us)
 {
     // enough copies to exceed flush_bytes, at most 64 (a matrix smaller than
     // flush_bytes / 64 is timed with 64 copies)
@@ -256,7 +302,7 @@ static int time_cold(const StreamMat<T>& A, int64_t nnz, const T* x, T* y, int r
     const int nc = int(std::min(64.0, std::ceil(double(flush_bytes) / per) + 1.0));
     std::vector<StreamCopy<T>> cp(static_cast<size_t>(nc));
     for (auto& c : cp)
-        MI_TRY(c.make(A, nnz, s));
+        MI_TRY(c.make(A, nnz, padded, s));
     for (auto& c : cp) // first touch of every copy
         MI_TRY(launch_spmv_view<T>(c.V, x, y, s));
     hipEvent_t e0, e1;
@@ -446,7 +492,7 @@ MI_API int mi_csr_spmv(mi_csr* A, const double* x, double* y)
     MI_TRY(dx.upload(x, size_t(n), s));
     MI_TRY(dy.alloc(size_t(n)));
     StreamMat<double> V;
-    MI_TRY(stream_view64(A, s, V));
+    MI_TRY(stream_view64(A, s, V, true));
     MI_TRY(launch_spmv_view<double>(V, dx.p, dy.p, s));
     MI_HIP(hipMemcpyAsync(y, dy.p, size_t(n) * sizeof(double), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
@@ -460,13 +506,13 @@ static int spmv_device_impl(mi_csr* A, int precision, const void* x, void* y, hi
         return MI_OK;
     if (precision == MI_PRECISION_FP64) {
         StreamMat<double> V;
-        MI_TRY(stream_view64(A, s, V));
+        MI_TRY(stream_view64(A, s, V, true));
         return launch_spmv_view<double>(V, static_cast<const double*>(x), static_cast<double*>(y),
                                         s);
     }
     if (precision == MI_PRECISION_MIXED) {
         StreamMat<float> V;
-        MI_TRY(stream_view32(A, s, false, V));
+        MI_TRY(stream_view32(A, s, false, V, true));
         return launch_spmv_view<float>(V, static_cast<const float*>(x), static_cast<float*>(y), s);
     }
     return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);
@@ -524,14 +570,15 @@ MI_API int mi_csr_time_spmv_cold(mi_csr* A, int precision, const void* x, void* 
     }
     if (precision == MI_PRECISION_FP64) {
         StreamMat<double> V;
-        MI_TRY(stream_view64(A, s, V));
-        return time_cold<double>(V, A->nnz(), static_cast<const double*>(x),
+        MI_TRY(stream_view64(A, s, V, true));
+        return time_cold<double>(V, A->nnz(), A->pat->sell.padded, static_cast<const double*>(x),
                                  static_cast<double*>(y), reps, flush_bytes, s, avg_us);
     }
     if (precision == MI_PRECISION_MIXED) {
         StreamMat<float> V;
-        MI_TRY(stream_view32(A, s, false, V));
-        return time_cold<float>(V, A->nnz(), static_cast<const float*>(x), static_cast<float*>(y),
+        MI_TRY(stream_view32(A, s, false, V, true));
+        return time_cold<float>(V, A->nnz(), A->pat->sell.padded, static_cast<const float*>(x),
+                                static_cast<float*>(y),
                                 reps, flush_bytes, s, avg_us);
     }
     return set_error(MI_ERR_BAD_ARG, "unknown precision %d", precision);

@@ -19,6 +19,7 @@ struct Pattern
     DevBuf<int> diag; // position of a_ii in each row, -1 if absent
     bool all_diag = false; // every row stores its diagonal entry
     RowBlocks rb;     // CSR-stream row blocks
+    SellForm sell;    // SELL-64 form when every row is short
     int lanes = 8;
 };
 } // namespace mi
@@ -40,6 +41,10 @@ struct mi_csr
     mi::DevBuf<double> s_val;
     mi::DevBuf<float> s_val32, s_op32;
     bool s_val_ok = false, s_val32_ok = false, s_op32_ok = false;
+    // Values in the pattern's SELL-64 slot order (padding 0), on demand.
+    mi::DevBuf<double> e_val;
+    mi::DevBuf<float> e_val32, e_op32;
+    bool e_val_ok = false, e_val32_ok = false, e_op32_ok = false;
 
     // Transposed copy (mi_csr_transpose): tmap[k] = position in the source
     // matrix of this matrix's k-th value.
@@ -55,6 +60,7 @@ struct mi_csr
         val32_valid = false;
         val32d_valid = false;
         s_val_ok = s_val32_ok = s_op32_ok = false;
+        e_val_ok = e_val32_ok = e_op32_ok = false;
         ++version;
     }
     int64_t nnz() const { return pat->nnz; }
@@ -88,8 +94,9 @@ int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s);
 // Stream views of a matrix for the CSR-stream kernels: column-ordered blocks
 // with a permuted copy of the values when the pattern has them, else CSR.
 // view32: op = the inner-iteration operator (op32 / diag32), else the
-// rounded values.
-int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out);
-int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out);
+// rounded values.  sell: the SELL-64 form when the pattern has one (only for
+// kernels that run stream_rows / sell_rows, launched with launch_grid).
+int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out, bool sell = false);
+int stream_view32(mi_csr* A, hipStream_t s, bool op, StreamMat<float>& out, bool sell = false);
 #endif
 } // namespace mi

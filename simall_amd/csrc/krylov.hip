@@ -249,14 +249,14 @@ template <class T> struct EpiCgAp
 };
 
 // Ap = A p ; partials of p.Ap
-template <class T>
+template <class T, bool kSell>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr, a.soff, a.eci, a.nsl};
     EpiCgAp<T> e{a.p, a.Ap};
-    stream_rows(A, a.p, e);
+    rows_of<kSell>(A, a.p, e);
     double v[1] = {e.acc};
     block_partials<1>(v, a.partials);
 }
@@ -365,8 +365,8 @@ template <class T> int cg_init_p(const CgArgs<T>& a, double tol, int max_iter, h
 
 template <class T> static int spmv_dot(const CgArgs<T>& a, hipStream_t s)
 {
-    const unsigned G = stream_grid(a.nrb);
-    k_cg_spmv_dot<T><<<G, kBlock, 0, s>>>(a);
+    const unsigned G = a.soff ? sell_grid(a.nsl) : stream_grid(a.nrb);
+    (a.soff ? k_cg_spmv_dot<T, true> : k_cg_spmv_dot<T, false>)<<<G, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
     return fin(1, a.partials, G, FIN_CG_ALPHA, a.st, a.history, 0, 0, s);
 }
@@ -456,14 +456,14 @@ template <class T> struct EpiBi
 };
 
 // mode 0: v = A y, partials of rh.v ; mode 1: t = A z, partials of t.s, t.t
-template <class T>
+template <class T, bool kSell>
 __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
 {
     if (a.st->done)
         return;
-    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr};
+    StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr, a.soff, a.eci, a.nsl};
     EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
-    stream_rows(A, mode == 0 ? a.y : a.z, e);
+    rows_of<kSell>(A, mode == 0 ? a.y : a.z, e);
     double v[2] = {e.acc0, e.acc1};
     block_partials<2>(v, a.partials);
 }
@@ -526,8 +526,8 @@ template <class T> int bicg_stage1(const BiArgs<T>& a, hipStream_t s)
 
 template <class T> int bicg_stage2(const BiArgs<T>& a, hipStream_t s)
 {
-    const unsigned G = stream_grid(a.nrb);
-    k_bi_spmv<T><<<G, kBlock, 0, s>>>(a, 0);
+    const unsigned G = a.soff ? sell_grid(a.nsl) : stream_grid(a.nrb);
+    (a.soff ? k_bi_spmv<T, true> : k_bi_spmv<T, false>)<<<G, kBlock, 0, s>>>(a, 0);
     MI_HIP(hipGetLastError());
     MI_TRY(fin(2, a.partials, G, FIN_BI_ALPHA, a.st, a.history, 0, 0, s));
     k_bi_s<T><<<a.grid, kBlock, 0, s>>>(a);
@@ -537,8 +537,8 @@ template <class T> int bicg_stage2(const BiArgs<T>& a, hipStream_t s)
 
 template <class T> int bicg_stage3(const BiArgs<T>& a, hipStream_t s)
 {
-    const unsigned G = stream_grid(a.nrb);
-    k_bi_spmv<T><<<G, kBlock, 0, s>>>(a, 1);
+    const unsigned G = a.soff ? sell_grid(a.nsl) : stream_grid(a.nrb);
+    (a.soff ? k_bi_spmv<T, true> : k_bi_spmv<T, false>)<<<G, kBlock, 0, s>>>(a, 1);
     MI_HIP(hipGetLastError());
     MI_TRY(fin(2, a.partials, G, FIN_BI_OMEGA, a.st, a.history, 0, 0, s));
     k_bi_update<T><<<a.grid, kBlock, 0, s>>>(a);

@@ -100,6 +100,9 @@ template <class T> struct CgArgs
     const int* diag; // FP32 difference form: diagonal positions, else nullptr
     const unsigned* cw; // column-ordered row blocks (val in that order), else nullptr
     const int* hdr;     // row block boundaries {row, nonzero, column base, 0}, else nullptr
+    const int* soff;    // SELL-64 form (val in its slot order), else nullptr
+    const int* eci;
+    int nsl;
     // FP32 inner solves of the mixed form: when set, the iterate is
     // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
     // then unused.  The caller's FP64 solution takes the correction directly
@@ -143,6 +146,9 @@ template <class T> struct BiArgs
     const int* diag;    // as in CgArgs
     const unsigned* cw; // as in CgArgs
     const int* hdr;     // as in CgArgs
+    const int* soff;
+    const int* eci;
+    int nsl;
     double* xacc;       // as in CgArgs
 };
 

@@ -42,6 +42,7 @@ int ensure_device();
 // gathers) unless the environment sets SIMALL_AMD_CSR_ORDER=1, which keeps
 // CSR order (for comparisons; results are identical either way).
 bool column_order_enabled();
+bool sell_enabled();
 // Library stream used when a caller passes a null stream.
 hipStream_t default_stream();
 

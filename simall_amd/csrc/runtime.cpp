@@ -61,6 +61,15 @@ bool column_order_enabled()
     return on;
 }
 
+bool sell_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("SIMALL_AMD_NO_SELL");
+        return !(e && std::strcmp(e, "1") == 0);
+    }();
+    return on;
+}
+
 int ensure_device()
 {
     std::call_once(g_init_once, [] { g_init_rc = init_device(0); });

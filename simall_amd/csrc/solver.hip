@@ -235,7 +235,7 @@ int ensure_workspace(mi_solver* S)
         MI_TRY(S->w32[9].alloc(n));
     }
     MI_TRY(S->st.alloc(1));
-    MI_TRY(S->partials.alloc(size_t(std::max(kMaxReduceBlocks, kMaxStreamBlocks)) *
+    MI_TRY(S->partials.alloc(size_t(std::max({kMaxReduceBlocks, kMaxStreamBlocks, kMaxSellBlocks})) *
                              kMaxReduceValues));
     MI_TRY(S->dscal.alloc(8));
     if (!S->h_st)
@@ -435,11 +435,11 @@ template <class Body> int iterate(mi_solver* S, KState* st, int max_iter, int ch
 // FP32 inner-iteration operator.
 static int krylov_view(mi_solver* S, StreamMat<double>& V)
 {
-    return stream_view64(S->M, S->stream, V);
+    return stream_view64(S->M, S->stream, V, true);
 }
 static int krylov_view(mi_solver* S, StreamMat<float>& V)
 {
-    return stream_view32(S->M, S->stream, true, V);
+    return stream_view32(S->M, S->stream, true, V, true);
 }
 
 template <class T>
@@ -480,6 +480,9 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
         a.diag = V.diag;
         a.cw = V.cw;
         a.hdr = V.hdr;
+        a.soff = V.soff;
+        a.eci = V.eci;
+        a.nsl = V.nsl;
         a.nrb = P.rb.nrb;
         a.xacc = xacc;
         MI_TRY(bicg_init<T>(a, warm, tol, max_iter, s));
@@ -514,6 +517,9 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
     a.diag = V.diag;
     a.cw = V.cw;
     a.hdr = V.hdr;
+    a.soff = V.soff;
+    a.eci = V.eci;
+    a.nsl = V.nsl;
     a.xacc = xacc;
     const bool jacobi = invD != nullptr;
     if (jacobi) {

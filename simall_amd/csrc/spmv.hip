@@ -5,6 +5,8 @@
 #include "spmv.hpp"
 #include "stream.hpp"
 
+#include "host_threads.hpp"
+
 #include <climits>
 
 namespace mi
@@ -147,7 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_permute(int64_t nnz, const int* __re
 {
     for (int64_t m = int64_t(blockIdx.x) * kBlock + threadIdx.x; m < nnz;
          m += int64_t(gridDim.x) * kBlock)
-        dst[m] = src[perm[m]];
+    {
+        const int q = perm[m];
+        dst[m] = q >= 0 ? src[q] : T(0); // -1: a SELL padding slot
+    }
 }
 
 template <class T>
@@ -161,6 +166,64 @@ int launch_permute(int64_t nnz, const int* perm, const T* src, T* dst, hipStream
 }
 template int launch_permute<double>(int64_t, const int*, const double*, double*, hipStream_t);
 template int launch_permute<float>(int64_t, const int*, const float*, float*, hipStream_t);
+
+int SellForm::build(int n, int64_t nnz, const int* rp, const int* ci, bool mark_diag,
+                    hipStream_t s)
+{
+    soff.release();
+    eci.release();
+    eperm.release();
+    nsl = 0;
+    padded = 0;
+    if (n <= 0 || n >= kSellDiag || !sell_enabled())
+        return MI_OK;
+    const int ns = (n + kSellSlice - 1) / kSellSlice;
+    std::vector<int> off(size_t(ns) + 1);
+    int64_t tot = 0;
+    for (int sl = 0; sl < ns; ++sl) {
+        int w = 0;
+        const int r1 = std::min(n, (sl + 1) * kSellSlice);
+        for (int i = sl * kSellSlice; i < r1; ++i)
+            w = std::max(w, rp[i + 1] - rp[i]);
+        if (w > kSellMaxWidth)
+            return MI_OK;
+        off[size_t(sl)] = int(tot);
+        tot += int64_t(kSellSlice) * w;
+        if (tot > INT_MAX)
+            return MI_OK;
+    }
+    off[size_t(ns)] = int(tot);
+    if (tot > nnz + nnz / 4 + kSellSlice)
+        return MI_OK;
+    std::vector<int> c(size_t(tot), -1), pm(size_t(tot), -1);
+    host_parallel_for(ns, host_threads(), [&](int b, int e, int) {
+        for (int sl = b; sl < e; ++sl) {
+            const int r1 = std::min(n, (sl + 1) * kSellSlice);
+            for (int i = sl * kSellSlice; i < r1; ++i) {
+                bool seen = !mark_diag;
+                for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                    const size_t m = size_t(off[size_t(sl)]) +
+                                     size_t(kSellSlice) * size_t(k - rp[i]) +
+                                     size_t(i - sl * kSellSlice);
+                    int col = ci[k];
+                    if (!seen && col == i) {
+                        col |= kSellDiag;
+                        seen = true;
+                    }
+                    c[m] = col;
+                    pm[m] = k;
+                }
+            }
+        }
+    }, 256);
+    MI_TRY(soff.upload(off.data(), off.size(), s));
+    MI_TRY(eci.upload(c.data(), c.size(), s));
+    MI_TRY(eperm.upload(pm.data(), pm.size(), s));
+    MI_HIP(hipStreamSynchronize(s));
+    nsl = ns;
+    padded = tot;
+    return MI_OK;
+}
 
 int RowBlocks::sort_columns(const int* rp_dev, const int* ci_dev, int64_t nnz, hipStream_t s)
 {
@@ -213,6 +276,15 @@ template <class T> struct EpiStore
     __device__ __forceinline__ void pre(int) {}
     __device__ __forceinline__ void row(int i, typename acc_type<T>::type s) { y[i] = T(s); }
 };
+
+// y = A x on the SELL-64 form (sell_rows: one lane per row, CSR order).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(StreamMat<T> A, const T* __restrict__ x,
+                                                      T* __restrict__ y)
+{
+    EpiStore<T> e{y};
+    sell_rows(A, x, e);
+}
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_stream_spmv(StreamMat<T> A, const T* __restrict__ x,
@@ -294,6 +366,11 @@ template <class T> int launch_spmv_view(const StreamMat<T>& A, const T* x, T* y,
 {
     if (A.n == 0)
         return MI_OK;
+    if (A.soff) {
+        k_sell_spmv<T><<<sell_grid(A.nsl), kBlock, 0, s>>>(A, x, y);
+        MI_HIP(hipGetLastError());
+        return MI_OK;
+    }
     // FP64: one row block per workgroup (no reduction partials here), so the
     // dispatcher hands out blocks as workgroups retire; measured on the 128^3
     // pressure matrix from HBM 47.9 vs 48.9 us.  FP32 keeps persistent ranges,

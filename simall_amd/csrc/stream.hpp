@@ -83,6 +83,54 @@ struct RowBlocks
 template <class T>
 int launch_permute(int64_t nnz, const int* perm, const T* src, T* dst, hipStream_t s);
 
+// SELL-64 form of a matrix whose rows are all short (the 7-point pressure
+// operator, the prolongator): slices of kSellSlice consecutive rows, one
+// wavefront per slice and one lane per row.  Slot k of the slice's lane l
+// sits at soff[s] + 64 k + l and holds the row's k-th CSR entry (column -1:
+// padding), so every lane sums its row in CSR order with no LDS staging and
+// no barrier, and every stream load is one coalesced 64-lane access.
+// Measured on the 2M-row pressure matrix against column-ordered row blocks:
+// FP64 35.8 vs 41.3 us, FP32 21.7 vs 28.4 us (results identical).
+constexpr int kSellSlice = 64;
+constexpr int kSellMaxWidth = 16;        // longest row the form stores
+constexpr int kSellUnroll = 8;           // slots loaded ahead of the gathers
+constexpr int kSellDiag = 1 << 30;       // marks a row's diagonal slot (square operators)
+constexpr int kSellColMask = kSellDiag - 1;
+constexpr int kMaxSellBlocks = 8192;     // workgroups (4 slices each) before slices per wave grow
+
+struct SellForm
+{
+    DevBuf<int> soff;  // nsl + 1 slot offsets (multiples of 64)
+    DevBuf<int> eci;   // column per slot, kSellDiag on the diagonal slot, -1 padding
+    DevBuf<int> eperm; // CSR position per slot, -1 padding (value permutation)
+    int nsl = 0;
+    int64_t padded = 0;
+    bool built() const { return nsl > 0; }
+    // Host build from CSR arrays; leaves the form empty (and returns MI_OK)
+    // when a row holds more than kSellMaxWidth entries, padding would exceed
+    // a quarter of nnz, or n >= 2^30.  mark_diag: flag the first entry with
+    // column == row in each row (the difference form's diagonal slot).
+    int build(int n, int64_t nnz, const int* rp_host, const int* ci_host, bool mark_diag,
+              hipStream_t s);
+};
+
+// Whether patterns get a SELL-64 form (default on; SIMALL_AMD_NO_SELL=1 keeps
+// every matrix on row blocks, for comparisons).
+bool sell_enabled();
+
+// Workgroups of a SELL launch: one per 4 slices up to kMaxSellBlocks (a
+// multiple of 8); kernels with reduction partials size them for that.
+inline unsigned sell_grid(int nsl)
+{
+    int64_t g = (int64_t(nsl) + 3) / 4;
+    if (g > kMaxSellBlocks)
+        g = kMaxSellBlocks;
+    g = (g + 7) / 8 * 8;
+    if (g < 8)
+        g = 8;
+    return unsigned(g);
+}
+
 // Workgroups for a stream launch: one per row block up to kMaxStreamBlocks
 // (a multiple of 8), beyond that each workgroup walks a contiguous range.
 // Persistent workgroups measured faster than one block per workgroup on the
@@ -127,7 +175,18 @@ template <class T> struct StreamMat
     // Block boundaries as {first row, first nonzero, column base, 0}
     // (RowBlocks::hdr); nullptr: read blk and rp.
     const int* hdr = nullptr;
+    // SELL-64 form (SellForm): slot offsets and columns; val then holds the
+    // padded values in slot order and the row-block fields are unused.
+    const int* soff = nullptr;
+    const int* eci = nullptr;
+    int nsl = 0;
 };
+
+// Workgroups for a launch over A in whichever form it carries.
+template <class T> inline unsigned launch_grid(const StreamMat<T>& A)
+{
+    return A.soff ? sell_grid(A.nsl) : stream_grid(A.nrb);
+}
 
 // Rows r0..r1, nonzeros k0..k1 and column base c0 of row block rb.
 template <class T>
@@ -305,9 +364,84 @@ __device__ __forceinline__ void stream_rows_diff(const StreamMat<float>& A,
     }
 }
 
+// SELL-64 rows: wavefront w of workgroup b takes slice 4q + w for the
+// workgroup's contiguous range of slice quads q (one quad per workgroup when
+// the grid is sell_grid's).  Each lane sums its row in CSR order, in the
+// arithmetic of the row-block paths: plain products a_ij x_j, or for an FP32
+// difference-form operator (A.diag set) a_ij (x_j - x_i) off the diagonal and
+// d_i x_i on it.  No barrier: lanes past the last slice simply return.
+template <class T, class X, class Epi>
+__device__ __forceinline__ void sell_rows(const StreamMat<T>& A, const X* __restrict__ x, Epi& epi)
+{
+    using Acc = typename acc_type<T>::type;
+    constexpr bool kDiffCapable = std::is_same<T, float>::value && std::is_same<X, float>::value;
+    const bool diff = kDiffCapable && A.diag != nullptr;
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int nq = (A.nsl + 3) >> 2;
+    const int per = (nq + G - 1) / G;
+    const int q0 = lb * per;
+    const int q1 = min(nq, q0 + per);
+    const int wave = int(threadIdx.x) / kSellSlice;
+    const int lane = int(threadIdx.x) % kSellSlice;
+    for (int q = q0; q < q1; ++q) {
+        const int sl = q * 4 + wave;
+        if (sl >= A.nsl)
+            return;
+        const int i = sl * kSellSlice + lane;
+        const bool mine = i < A.n;
+        if (mine)
+            epi.pre(i);
+        const int o0 = A.soff[sl] + lane;
+        const int w = (A.soff[sl + 1] - A.soff[sl]) / kSellSlice;
+        int c[kSellUnroll];
+        T a[kSellUnroll];
+#pragma unroll
+        for (int k = 0; k < kSellUnroll; ++k) {
+            c[k] = k < w ? __builtin_nontemporal_load(&A.eci[o0 + k * kSellSlice]) : -1;
+            a[k] = k < w ? __builtin_nontemporal_load(&A.val[o0 + k * kSellSlice]) : T(0);
+        }
+        X xv[kSellUnroll];
+#pragma unroll
+        for (int k = 0; k < kSellUnroll; ++k)
+            xv[k] = c[k] >= 0 ? x[c[k] & kSellColMask] : X(0);
+        Acc s = 0;
+        if (kDiffCapable && diff) {
+            const X xi = mine ? x[i] : X(0);
+#pragma unroll
+            for (int k = 0; k < kSellUnroll; ++k)
+                if (c[k] >= 0)
+                    s += (c[k] & kSellDiag) ? Acc(a[k] * xi) : Acc(a[k] * (xv[k] - xi));
+            for (int k = kSellUnroll; k < w; ++k) {
+                const int ck = A.eci[o0 + k * kSellSlice];
+                if (ck >= 0) {
+                    const T ak = A.val[o0 + k * kSellSlice];
+                    s += (ck & kSellDiag) ? Acc(ak * xi) : Acc(ak * (x[ck & kSellColMask] - xi));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kSellUnroll; ++k)
+                if (c[k] >= 0)
+                    s += Acc(a[k]) * Acc(xv[k]);
+            for (int k = kSellUnroll; k < w; ++k) {
+                const int ck = A.eci[o0 + k * kSellSlice];
+                if (ck >= 0)
+                    s += Acc(A.val[o0 + k * kSellSlice]) * Acc(x[ck & kSellColMask]);
+            }
+        }
+        if (mine)
+            epi.row(i, s);
+    }
+}
+
 // Runs epi.row(i, sum_i) for every row of A handled by this workgroup.  The
 // epilogue's pre(i) is called first, on the same thread, at the start of the
-// row's block: it loads the per-row operands row(i, .) will use.
+// row's block: it loads the per-row operands row(i, .) will use.  Row blocks
+// only: kernels that also take SELL-64 views are instantiated once per form
+// (rows_of below), so neither path's registers limit the other's occupancy
+// (one kernel holding both ran the 30-nonzero-per-row FP32 level at 63
+// instead of 40 us).
 template <class T, class X, class Epi>
 __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __restrict__ x, Epi& epi)
 {
@@ -388,6 +522,17 @@ __device__ __forceinline__ void stream_rows(const StreamMat<T>& A, const X* __re
             epi.row(i, s);
         __syncthreads();
     }
+}
+
+// The row engine of a kernel instantiated for one form: SELL-64 (kSell) or
+// row blocks.  Launch sites pick the instance from StreamMat::soff.
+template <bool kSell, class T, class X, class Epi>
+__device__ __forceinline__ void rows_of(const StreamMat<T>& A, const X* __restrict__ x, Epi& epi)
+{
+    if constexpr (kSell)
+        sell_rows(A, x, epi);
+    else
+        stream_rows(A, x, epi);
 }
 
 #pragma clang fp contract(on)

@@ -156,7 +156,8 @@ def test_spmv_column_ordered_blocks(gpu, far):
     """Row blocks are stored in column order when each block's columns span
     fewer than 2^21 - 1, and in CSR order otherwise (far=True: one entry of
     row 5 reaches the last column of a 2.1M-row matrix).  Either way the
-    products are summed in CSR order: identical results."""
+    products are summed in CSR order: identical results.  Row 9 holds 24
+    entries, more than the SELL-64 form stores, so the row blocks run."""
     rng = np.random.default_rng(21)
     n = (1 << 21) + 4096
     rows = [np.arange(n)]
@@ -165,6 +166,9 @@ def test_spmv_column_ordered_blocks(gpu, far):
         i = np.arange(max(0, -off), min(n, n - off))
         rows.append(i)
         cols.append(i + off)
+    wide = np.setdiff1d(np.arange(100, 2500, 100), [9 - 1, 9 + 1, 9 + 700])[:17]
+    rows.append(np.full(wide.size, 9))
+    cols.append(wide)
     if far:
         rows.append(np.array([5]))
         cols.append(np.array([n - 1]))
@@ -187,6 +191,44 @@ def test_spmv_column_ordered_blocks(gpu, far):
     A.spmv_device(dx, dy, capi.MI_PRECISION_MIXED)
     scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x)) + 1e-300
     assert np.all(np.abs(dy.download().astype(np.float64) - yo) <= 2e-6 * scale)
+
+
+@pytest.mark.parametrize("n,width", [(64 * 900 + 37, 7), (64 * 700 + 5, 13), (50, 3)])
+def test_spmv_sell_form(gpu, n, width):
+    """Matrices whose rows are all short (at most 16 entries, padding within a
+    quarter of nnz) run on the SELL-64 form: one lane per row summing its CSR
+    entries in order.  Ragged rows (some empty, some shorter), n not a multiple
+    of 64, and rows longer than the 8 slots loaded ahead (width 13): FP64
+    results identical to the oracle's; the FP32 product within rounding."""
+    rng = np.random.default_rng(n)
+    lens = np.full(n, width)
+    lens[rng.choice(n, n // 10, replace=False)] = rng.integers(0, width, n // 10)
+    lens[: min(n, 3)] = 0
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    ci = np.empty(rp[-1], np.int32)
+    for i in range(n):
+        if lens[i]:
+            lo = max(0, min(i - 3 * width, n - 6 * width))
+            pool = np.arange(lo, min(n, lo + 6 * width))
+            ci[rp[i]:rp[i + 1]] = np.sort(rng.choice(pool, lens[i], replace=False))
+    rp = rp.astype(np.int32)
+    va = rng.standard_normal(ci.size)
+    va[rng.random(ci.size) < 0.01] = 0.0
+    x = rng.standard_normal(n)
+    A = capi.Csr(rp, ci, va)
+    y = A.spmv(x)
+    yo = oracle.spmv(oracle.Matrix(rp, ci, va), x)
+    assert np.array_equal(y, yo)
+    dx = capi.DeviceArray.from_host(x)
+    dy = capi.DeviceArray(n, np.float64)
+    A.spmv_device(dx, dy, capi.MI_PRECISION_FP64)
+    assert np.array_equal(dy.download(), yo)
+    dx32 = capi.DeviceArray.from_host(x.astype(np.float32))
+    dy32 = capi.DeviceArray(n, np.float32)
+    A.spmv_device(dx32, dy32, capi.MI_PRECISION_MIXED)
+    scale = oracle.spmv(oracle.Matrix(rp, ci, np.abs(va)), np.abs(x)) + 1e-300
+    assert np.all(np.abs(dy32.download().astype(np.float64) - yo) <= 2e-6 * scale)
 
 
 def test_spmv_empty_matrix(gpu):
