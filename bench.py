@@ -25,7 +25,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the FP64 SpMV from the rocprofv3 FETCH_SIZE /
 # WRITE_SIZE passes (scaled by a copy kernel of known size), written next to
 # the profile it came from.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_spmv_cols_fp64_poisson128_traffic.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sell_spmv_fp64_poisson128_traffic.md")
 COPY_DOUBLES = 1 << 26  # calibration copy: 512 MiB read + 512 MiB written
 
 
@@ -536,7 +536,7 @@ def main():
                  "bandwidth": [info["bandwidth_before"], info["bandwidth_after"]]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64c, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64c / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(), "kernel": "k_spmv_cols<double>",
+                     "traffic": read_traffic(), "kernel": "k_sell_spmv<double>",
                      "algorithmic_bytes": by64, "avg_us": round(us64c, 2),
                      "cache": "cold: 20 back-to-back launches rotating over copies of the "
                               "matrix (> 1 GiB in all), HIP events",
