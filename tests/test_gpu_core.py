@@ -322,6 +322,65 @@ def test_cavity_mixed_precision_true_residual(gpu, cavity):
     assert abs(rel - st.true_relative_residual) <= 1e-3 * rel + 1e-15
 
 
+def _ragged_spd(seed=5):
+    """A 27x29x25 7-point operator with random conductances, plus 3 long
+    symmetric couplings on each of 20 rows (up to 10 entries): short, ragged rows,
+    n = 19575 (not a multiple of 64), strictly diagonally dominant."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = 27, 29, 25
+    n = nx * ny * nz
+    idx = np.arange(n).reshape(nz, ny, nx)
+    pairs = [(idx[:, :, :-1], idx[:, :, 1:]), (idx[:, :-1, :], idx[:, 1:, :]),
+             (idx[:-1, :, :], idx[1:, :, :])]
+    r = np.concatenate([a.ravel() for a, _ in pairs])
+    c = np.concatenate([b.ravel() for _, b in pairs])
+    far_r = np.repeat(rng.choice(n, 20, replace=False), 3)
+    far_c = (far_r + rng.integers(500, 5000, far_r.size)) % n
+    keep = far_r != far_c
+    r = np.concatenate([r, far_r[keep]])
+    c = np.concatenate([c, far_c[keep]])
+    w = 10.0 ** rng.uniform(-1, 1, r.size)
+    A = sp.coo_matrix((-w, (r, c)), shape=(n, n))
+    A = (A + A.T).tocsr()
+    A.sum_duplicates()
+    d = -np.asarray(A.sum(axis=1)).ravel() + 0.05
+    A = (A + sp.diags(d)).tocsr()
+    A.sort_indices()
+    rp, ci = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    lens = np.diff(rp)
+    ns = (n + 63) // 64
+    L = np.zeros(ns * 64, np.int64)
+    L[:n] = lens
+    padded = 64 * L.reshape(ns, 64).max(1).sum()
+    assert lens.max() <= 16 and padded <= 1.25 * ci.size and lens.max() > 8
+    return oracle.Matrix(rp, ci, A.data.copy()), rng.standard_normal(n)
+
+
+def test_krylov_on_sell_form_ragged_rows(gpu):
+    """CG and BiCGStab whose operator runs on the SELL-64 form (short ragged
+    rows, some longer than the 8 slots loaded ahead): FP64 Jacobi-PCG follows
+    the oracle's iteration; the mixed solve (FP32 difference form, diagonal
+    slot flagged in the slot columns) meets the FP64 true-residual tolerance."""
+    Ao, b = _ragged_spd()
+    A = capi.Csr(Ao.rowPtr, Ao.colIdx, Ao.values)
+    xo, ito, _, _ = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-10, 5000)
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-10,
+                    max_iterations=5000)
+    s.setup(A)
+    x, st, rc = s.solve(b)
+    assert rc == capi.MI_OK and abs(st.iterations - ito) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    for kind in (capi.MI_SOLVER_CG, capi.MI_SOLVER_BICGSTAB):
+        for prec in (capi.MI_PRECISION_FP64, capi.MI_PRECISION_MIXED):
+            s = capi.Solver(kind=kind, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-9,
+                            max_iterations=5000, precision=prec)
+            s.setup(A)
+            x, st, rc = s.solve(b)
+            rel = np.linalg.norm(b - oracle.spmv(Ao, x)) / np.linalg.norm(b)
+            assert rc == capi.MI_OK and rel <= 1.5e-9, (kind, prec, rel)
+
+
 def test_fp64_true_residual_mode(gpu, cavity):
     w, Ao, b = cavity
     s = capi.Solver(kind=capi.MI_SOLVER_CG, tolerance=1e-8, max_iterations=20000, true_residual=1)
