@@ -77,21 +77,21 @@ template <class T> struct EpiGmStep
 };
 
 // w = b - A64 x (or v_0 = D^-1 (b - A64 x) with Jacobi)
-template <class T> __global__ __launch_bounds__(kBlock) void k_gm_residual(GmArgs<T> a)
+template <class T, bool kSell> __global__ __launch_bounds__(kBlock) void k_gm_residual(GmArgs<T> a)
 {
     if (gm_idle(a.g, false))
         return;
     EpiGmRes<T> e{a.b, a.invD ? a.V : a.w, a.invD};
-    stream_rows(a.A64, a.x, e);
+    rows_of<kSell>(a.A64, a.x, e);
 }
 
 // w = A v_j (or v_{j+1} = D^-1 A v_j with Jacobi)
-template <class T> __global__ __launch_bounds__(kBlock) void k_gm_spmv(GmArgs<T> a, int j)
+template <class T, bool kSell> __global__ __launch_bounds__(kBlock) void k_gm_spmv(GmArgs<T> a, int j)
 {
     if (gm_idle(a.g, true))
         return;
     EpiGmStep<T> e{a.invD ? a.V + size_t(j + 1) * a.ld : a.w, a.invD};
-    stream_rows(a.A, a.V + size_t(j) * a.ld, e);
+    rows_of<kSell>(a.A, a.V + size_t(j) * a.ld, e);
 }
 
 // Per-wave sums of nv values, then one partial per value and block:
@@ -423,7 +423,7 @@ template <class T> int gm_init(const GmArgs<T>& a, const double* bnorm_dev, doub
 
 template <class T> int gm_cycle_residual(const GmArgs<T>& a, hipStream_t s)
 {
-    k_gm_residual<T><<<a.Gs, kBlock, 0, s>>>(a);
+    (a.A64.soff ? k_gm_residual<T, true> : k_gm_residual<T, false>)<<<a.Gs, kBlock, 0, s>>>(a);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -439,7 +439,7 @@ template <class T> int gm_cycle_begin(const GmArgs<T>& a, hipStream_t s)
 
 template <class T> int gm_step_spmv(const GmArgs<T>& a, int j, hipStream_t s)
 {
-    k_gm_spmv<T><<<a.Gs, kBlock, 0, s>>>(a, j);
+    (a.A.soff ? k_gm_spmv<T, true> : k_gm_spmv<T, false>)<<<a.Gs, kBlock, 0, s>>>(a, j);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }

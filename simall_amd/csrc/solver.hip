@@ -719,7 +719,7 @@ int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
     a.n = n;
     a.m = m;
     a.ld = W.ld;
-    MI_TRY(stream_view64(M, s, a.A64));
+    MI_TRY(stream_view64(M, s, a.A64, true));
     if constexpr (sizeof(T) == 8) {
         a.V = W.V64.p;
         a.w = W.w64.p;
@@ -729,7 +729,7 @@ int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
         a.V = W.V32.p;
         a.w = W.w32.p;
         a.invD = pc->inv_diag32();
-        MI_TRY(stream_view32(M, s, true, a.A));
+        MI_TRY(stream_view32(M, s, true, a.A, true));
     }
     a.b = b;
     a.x = x;
@@ -752,7 +752,7 @@ int run_gmres(mi_solver* S, const double* b, double* x, bool mixed, int* cycles)
     a.history = S->hist.p;
     a.hist_cap = int(S->hist.n);
     a.G = int(W.G);
-    a.Gs = stream_grid(P.rb.nrb);
+    a.Gs = launch_grid(a.A64); // A64 and A share the pattern's form
     MI_TRY(gm_init<T>(a, S->dscal.p + 1, S->cfg.tolerance, S->cfg.max_iterations, mixed, s));
     const int chunk = check_interval(S, n);
     auto poll = [&]() -> int {
