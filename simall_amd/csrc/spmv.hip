@@ -195,6 +195,11 @@ int SellForm::build(int n, int64_t nnz, const int* rp, const int* ci, bool mark_
     off[size_t(ns)] = int(tot);
     if (tot > nnz + nnz / 4 + kSellSlice)
         return MI_OK;
+    // columns share their word with the diagonal flag (bit 30): a rectangular
+    // operator with 2^30 or more columns keeps row blocks
+    for (int64_t k = 0; k < nnz; ++k)
+        if (ci[k] < 0 || ci[k] >= kSellDiag)
+            return MI_OK;
     std::vector<int> c(size_t(tot), -1), pm(size_t(tot), -1);
     host_parallel_for(ns, host_threads(), [&](int b, int e, int) {
         for (int sl = b; sl < e; ++sl) {

@@ -108,7 +108,7 @@ struct SellForm
     bool built() const { return nsl > 0; }
     // Host build from CSR arrays; leaves the form empty (and returns MI_OK)
     // when a row holds more than kSellMaxWidth entries, padding would exceed
-    // a quarter of nnz, or n >= 2^30.  mark_diag: flag the first entry with
+    // a quarter of nnz, or n or a column index reaches 2^30.  mark_diag: flag the first entry with
     // column == row in each row (the difference form's diagonal slot).
     int build(int n, int64_t nnz, const int* rp_host, const int* ci_host, bool mark_diag,
               hipStream_t s);
