@@ -113,11 +113,20 @@ MI_API int mi_csr_create(int64_t n, int64_t nnz, const int32_t* rowPtr, const in
  * re-filled every outer iteration: src/solver/SimpleAlgorithm.cpp:49-98).
  * Solvers set up on this matrix rebuild their preconditioner on next solve. */
 MI_API int mi_csr_update_values(mi_csr* A, const double* values_host);
+/* Device-side variant.  values_device may be the handle's own value array
+ * (mi_csr_values_device): then nothing is copied and the call only marks the
+ * values changed, which a caller must do after writing through that pointer. */
 MI_API int mi_csr_update_values_device(mi_csr* A, const double* values_device);
 MI_API int mi_csr_get_values(const mi_csr* A, double* values_host);
 MI_API int mi_csr_get_pattern(const mi_csr* A, int32_t* rowPtr_host, int32_t* colIdx_host);
 MI_API int mi_csr_info(const mi_csr* A, int64_t* n, int64_t* nnz);
-/* Device pointer to the FP64 values (for kernels that fill them in place). */
+/* Device pointer to the FP64 values (for kernels that fill them in place).
+ * The handle's derived copies (host mirror, SELL-64 and column-ordered value
+ * streams, FP32 forms, cached preconditioners) are marked stale once, by
+ * this call.  After every later write through the pointer, call
+ * mi_csr_update_values_device(A, *values_device): with the handle's own
+ * pointer it copies nothing and only marks the values changed.  Without it,
+ * solves and SpMVs keep using the copies derived from the earlier values. */
 MI_API int mi_csr_values_device(mi_csr* A, double** values_device);
 /* y = A x with host arrays (CSRMatrix::spmv, src/solver/CSRMatrix.hpp:25-35). */
 MI_API int mi_csr_spmv(mi_csr* A, const double* x_host, double* y_host);

@@ -182,6 +182,19 @@ bool get_amg(In& in, mi::AmgHierarchyHost& h)
         if (!in.ok || (has_p != 0) != (l + 1 < nl))
             return false;
     }
+    // The V-cycle indexes level l+1's vectors with R_l's rows and P_l's
+    // columns: a trailer whose shapes do not chain is refused before any of
+    // it reaches the device.
+    for (uint32_t l = 0; l < nl; ++l) {
+        const mi::AmgLevelHost& L = h.levels[l];
+        if (L.A.nr != L.A.nc)
+            return false;
+        if (l + 1 < nl) {
+            const int nf = L.A.nr, nc = h.levels[l + 1].A.nr;
+            if (L.P.nr != nf || L.P.nc != nc || L.R.nr != nc || L.R.nc != nf)
+                return false;
+        }
+    }
     h.coarse_n = in.val<int32_t>();
     if (!in.ok || h.coarse_n != h.levels.back().A.nr ||
         !get_vec(in, h.coarse_dense, uint64_t(h.coarse_n) * uint64_t(h.coarse_n)))

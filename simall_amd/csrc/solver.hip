@@ -627,7 +627,15 @@ int solve_mixed(mi_solver* S, const double* b, double* x, mi_stats* out)
     double rn = init_res;
     int total = 0, outer = 0, status = MI_OK;
     const double inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-4;
-    while (rn / bnorm >= c.tolerance) {
+    // `!(x < tol)` so that a NaN residual never counts as converged, as the
+    // reference's `last_ < tol` test (src/solver/LinearSolvers.cpp:323)
+    while (!(rn / bnorm < c.tolerance)) {
+        if (!std::isfinite(rn)) {
+            // a NaN/Inf in b or an overflowing inner solve: nothing to refine
+            status = set_error(MI_ERR_BREAKDOWN, "mixed refinement: non-finite residual %g",
+                               rn);
+            break;
+        }
         if (outer >= std::max(c.max_outer, 1) || total >= c.max_iterations) {
             status = MI_ERR_NOT_CONVERGED;
             break;

@@ -260,3 +260,59 @@ def test_rejects_counts_beyond_the_file(lib, tmp_path):
     bad.write_bytes(bytes(b))
     with pytest.raises(capi.MiError):
         capi.Checkpoint(bad)
+
+
+def _trailer_fields(b):
+    """Byte offsets of each level's {A, P, R} row/column counts in the AMG
+    trailer of file bytes b (layout: tag, u32 levels, per level u32 has_P, CSR A,
+    f64, i32, [CSR P, CSR R]; a CSR is i32 rows, i32 cols, three u64-counted
+    arrays of 4, 4 and 8 bytes per element)."""
+    pos = b.rindex(b"SIMAMGH1") + 8
+    nl = int(np.frombuffer(b[pos:pos + 4], np.uint32)[0])
+    pos += 4
+    out = []
+
+    def csr(pos):
+        at = pos
+        pos += 8
+        for width in (4, 4, 8):
+            cnt = int(np.frombuffer(b[pos:pos + 8], np.uint64)[0])
+            pos += 8 + width * cnt
+        return at, pos
+
+    for _ in range(nl):
+        has_p = int(np.frombuffer(b[pos:pos + 4], np.uint32)[0])
+        pos += 4
+        a, pos = csr(pos)
+        pos += 12
+        lev = {"A": a}
+        if has_p:
+            lev["P"], pos = csr(pos)
+            lev["R"], pos = csr(pos)
+        out.append(lev)
+    return out
+
+
+def test_rejects_amg_trailer_whose_shapes_do_not_chain(lib, tmp_path):
+    """Each CSR of the trailer is valid on its own, but the V-cycle needs A_l
+    square, P_l of shape n_l x n_{l+1} and R_l of n_{l+1} x n_l: a trailer that
+    breaks any of these is refused (it would index level vectors out of range)."""
+    m = _mesh()
+    path = tmp_path / "amg.simcp"
+    capi.checkpoint_write(path, m, {"p": np.zeros(m.n_cells)})
+    capi.checkpoint_append_amg(path, _amg_host(m))
+    good = open(path, "rb").read()
+    capi.Checkpoint(path)  # the untouched file is accepted
+    lev = _trailer_fields(good)
+    assert len(lev) >= 2 and "P" in lev[0]
+    # widen one column count: every column index stays inside it, so only the
+    # shape check can catch it
+    for which, lvl in (("A", 0), ("P", 0), ("R", 0), ("A", len(lev) - 1)):
+        b = bytearray(good)
+        at = lev[lvl][which] + 4
+        nc = int(np.frombuffer(bytes(b[at:at + 4]), np.int32)[0])
+        b[at:at + 4] = np.int32(nc + 1).tobytes()
+        bad = tmp_path / f"bad_{which}{lvl}.simcp"
+        bad.write_bytes(bytes(b))
+        with pytest.raises(capi.MiError):
+            capi.Checkpoint(bad)

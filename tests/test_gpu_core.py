@@ -388,3 +388,52 @@ def test_fp64_true_residual_mode(gpu, cavity):
     x, st, rc = s.solve(b)
     rel = np.linalg.norm(b - oracle.spmv(Ao, x)) / np.linalg.norm(b)
     assert rc == capi.MI_OK and rel < 1e-8
+
+
+@pytest.mark.parametrize("kind", [capi.MI_SOLVER_CG, capi.MI_SOLVER_BICGSTAB])
+def test_mixed_refinement_never_converges_on_nan(gpu, kind):
+    """A NaN in b makes the FP64 true residual NaN; the mixed refinement loop
+    must report failure, as the reference's `last_ < tol` never holds for NaN
+    (src/solver/LinearSolvers.cpp:323, :440-443)."""
+    rp, ci, va = tridiag_csr(300, -1.0, 4.0, -1.0)
+    A = capi.Csr(rp, ci, va)
+    b = np.ones(300)
+    b[17] = np.nan
+    s = capi.Solver(kind=kind, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                    max_iterations=100, precision=capi.MI_PRECISION_MIXED)
+    s.setup(A)
+    x, st, rc = s.solve(b)
+    assert rc == capi.MI_ERR_BREAKDOWN and not st.converged
+    assert not np.isfinite(s.last_residual()) or s.last_residual() >= 1e-8
+    # the same solver still solves a finite system afterwards
+    b[17] = 1.0
+    x, st, rc = s.solve(b)
+    assert rc == capi.MI_OK and st.converged
+    assert np.linalg.norm(b - oracle.spmv(oracle.Matrix(rp, ci, va), x)) <= 1e-8 * np.linalg.norm(b)
+
+
+def test_values_written_through_the_device_pointer(gpu):
+    """Writes through mi_csr_values_device take effect once the caller marks
+    them with mi_csr_update_values_device(A, ptr) (no copy: same pointer);
+    the SELL-64 / column-ordered value copies and the solver follow."""
+    import ctypes as C
+    rp, ci, va = tridiag_csr(5000, -1.0, 4.0, -1.0)
+    A = capi.Csr(rp, ci, va)
+    x = np.random.default_rng(2).standard_normal(5000)
+    y0 = A.spmv(x)
+    p = C.c_void_p()
+    capi.check(gpu.mi_csr_values_device(A.h, C.byref(p)))
+    new = np.ascontiguousarray(va * 2.0)
+    capi.check(gpu.mi_memcpy_h2d(p, new.ctypes.data, new.nbytes))
+    capi.check(gpu.mi_csr_update_values_device(A.h, p))
+    assert np.array_equal(A.values(), new)
+    assert np.array_equal(A.spmv(x), 2.0 * y0)  # scaling by 2 is exact
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, tolerance=1e-10, max_iterations=500)
+    s.setup(A)
+    xs, st, rc = s.solve(y0)
+    assert rc == capi.MI_OK and np.allclose(xs, 0.5 * x, rtol=0, atol=1e-8)
+    # a second in-place write, again marked, is seen by the same solver
+    capi.check(gpu.mi_memcpy_h2d(p, va.ctypes.data, va.nbytes))
+    capi.check(gpu.mi_csr_update_values_device(A.h, p))
+    xs, st, rc = s.solve(y0)
+    assert rc == capi.MI_OK and np.allclose(xs, x, rtol=0, atol=1e-8)

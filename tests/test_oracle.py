@@ -188,6 +188,68 @@ def test_ssor_identity_and_laplacian_cg():
     assert np.linalg.norm(b - D @ x) < 1e-7
 
 
+def _ssor_closed_form(Dm, r, omega, d_mid=None):
+    """z = w(2-w) (D+wU)^-1 D_mid (D+wL)^-1 r: the inverse of the reference's
+    M = (D+wL) D^-1 (D+wU) / (w(2-w)) (src/solver/LinearSolvers.cpp:160-161),
+    by dense triangular solves.  D is the diagonal used by the two sweeps and
+    d_mid the one the middle scaling multiplies by (they differ only on rows
+    whose stored diagonal is zero)."""
+    D = np.diag(np.diag(Dm))
+    Dmid = D if d_mid is None else np.diag(d_mid)
+    L, U = np.tril(Dm, -1), np.triu(Dm, 1)
+    y = np.linalg.solve(D + omega * L, r)
+    return omega * (2.0 - omega) * np.linalg.solve(D + omega * U, Dmid @ y)
+
+
+@pytest.mark.parametrize("omega", [1.0, 0.6, 1.45])
+def test_ssor_matches_closed_form(omega):
+    """orc_ssor_apply against the closed form of the SSOR inverse on a
+    nonsymmetric matrix with a strongly non-uniform diagonal (1..50): a dropped
+    D scaling, a swapped sweep or a wrong (2-w) / w factor all change z."""
+    rng = np.random.default_rng(int(omega * 100))
+    n = 60
+    Dm = np.where(rng.random((n, n)) < 0.12, rng.uniform(-1, 1, (n, n)), 0.0)
+    np.fill_diagonal(Dm, 10.0 ** rng.uniform(0, 1.7, n))
+    r = rng.standard_normal(n)
+    z = oracle.ssor_apply(oracle.Matrix.from_dense(Dm), r, omega)
+    ze = _ssor_closed_form(Dm, r, omega)
+    assert np.linalg.norm(z - ze) <= 1e-13 * np.linalg.norm(ze)
+    # the same holds for the defining relation M z = r
+    D = np.diag(np.diag(Dm))
+    M = (D + omega * np.tril(Dm, -1)) @ np.linalg.inv(D) @ (D + omega * np.triu(Dm, 1))
+    assert np.linalg.norm(M @ z / (omega * (2 - omega)) - r) <= 1e-12 * np.linalg.norm(r)
+
+
+def test_ssor_missing_and_zero_diagonal():
+    """Rows without a stored diagonal sweep with 1 in its place and scale by 1;
+    a stored zero diagonal sweeps with 1 but scales by 0 (diag_ keeps the
+    stored value, invDiag_ falls back to 1: src/solver/LinearSolvers.cpp:174-184)."""
+    rng = np.random.default_rng(7)
+    n, omega = 30, 1.3
+    Dm = np.where(rng.random((n, n)) < 0.15, rng.uniform(-0.3, 0.3, (n, n)), 0.0)
+    np.fill_diagonal(Dm, rng.uniform(2, 9, n))
+    rp, ci, va = [0], [], []
+    missing, zero = 4, 11
+    for i in range(n):
+        for j in range(n):
+            if i == j and i == missing:
+                continue
+            if i == j or Dm[i, j] != 0.0:
+                ci.append(j)
+                va.append(0.0 if i == j == zero else Dm[i, j])
+        rp.append(len(ci))
+    A = oracle.Matrix(np.array(rp, np.int32), np.array(ci, np.int32), np.array(va))
+    r = rng.standard_normal(n)
+    z = oracle.ssor_apply(A, r, omega)
+    Ds = Dm.copy()
+    Ds[missing, missing] = 1.0
+    Ds[zero, zero] = 1.0
+    d_mid = np.diag(Ds).copy()
+    d_mid[zero] = 0.0
+    ze = _ssor_closed_form(Ds, r, omega, d_mid)
+    assert np.linalg.norm(z - ze) <= 1e-13 * np.linalg.norm(ze)
+
+
 @pytest.mark.parametrize("pc", [oracle.PC_JACOBI, oracle.PC_ILU])
 def test_gmres_matches_dense_solve(pc):
     D = random_sparse(50, 0.1, 3, diag_boost=0.5)
