@@ -94,11 +94,14 @@ MI_API int mi_spmv_csr(size_t n, const void* rowPtr, const void* colIdx, const v
 /* z[i] = r[i] / diag[i] (GpuKernels.hpp:85) */
 MI_API int mi_jacobi_apply(const void* diag, const void* r, void* z, size_t n, void* stream);
 /* In-place ILU(0) of device CSR values (GpuKernels.hpp:87-93; factorized on the
- * host like the reference, then written back).  *ok = 0 when a zero pivot was
- * met, as the reference's false return. */
+ * host like the reference, then written back).  *ok = 0 as the reference's
+ * false return (GpuKernels.cpp:345-388): a row without a stored diagonal
+ * (values left unchanged), or a zero pivot (the factorization stops there,
+ * rows before it factored). */
 MI_API int mi_ilu0_factor_inplace(size_t n, const void* rowPtr, const void* colIdx, void* values,
                                   int* ok);
-/* z = U^-1 L^-1 r with level-scheduled device sweeps (GpuKernels.hpp:95-107). */
+/* z = U^-1 L^-1 r with level-scheduled device sweeps (GpuKernels.hpp:95-107).
+ * A row without a stored diagonal, or with a zero one, is divided by 1. */
 MI_API int mi_ilu0_apply(size_t n, const void* rowPtr, const void* colIdx, const void* values,
                          const void* r, void* z, void* stream);
 

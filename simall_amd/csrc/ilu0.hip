@@ -45,14 +45,21 @@ __global__ __launch_bounds__(kBlock) void k_lower_level(int cnt, const int* __re
     y[i] = s;
 }
 
+// A row without a stored diagonal is not divided.  With `prefix` its sum
+// runs, as ILU0Preconditioner::apply's loop from diagPtr_[i] + 1 = 0 does
+// (src/solver/LinearSolvers.cpp:146-150), over every position before the end
+// of the row -- rows 0..i of the factor -- where the columns <= i meet the
+// still-zero entries of the reference's z; without it (the GpuKernels facade,
+// src/gpu/GpuKernels.cpp:466-477) over the row's own upper entries.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_upper_level(int cnt, const int* __restrict__ rows,
                                                         const int* __restrict__ rp,
                                                         const int* __restrict__ diag,
+                                                        const int* __restrict__ lend,
                                                         const int* __restrict__ ci,
                                                         const T* __restrict__ lu,
                                                         const T* __restrict__ y,
-                                                        T* __restrict__ z)
+                                                        T* __restrict__ z, int prefix)
 {
     const int t = int(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= cnt)
@@ -60,10 +67,18 @@ __global__ __launch_bounds__(kBlock) void k_upper_level(int cnt, const int* __re
     const int i = rows[t];
     const int kd = diag[i];
     T s = y[i];
-    for (int k = kd + 1; k < rp[i + 1]; ++k)
-        s -= lu[k] * z[ci[k]];
-    const T d = lu[kd];
-    z[i] = (d != T(0)) ? s / d : s;
+    if (kd >= 0) {
+        for (int k = kd + 1; k < rp[i + 1]; ++k)
+            s -= lu[k] * z[ci[k]];
+        const T d = lu[kd];
+        z[i] = (d != T(0)) ? s / d : s;
+        return;
+    }
+    for (int k = prefix ? 0 : lend[i]; k < rp[i + 1]; ++k) {
+        const int j = ci[k];
+        s -= lu[k] * (j > i ? z[j] : T(0));
+    }
+    z[i] = s;
 }
 
 // y_i = r_i - sum_{j < i} l_ij y_j over the rows of one run
@@ -155,6 +170,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_ssor_upper_level(int cnt, const int* __restrict__ rows,
                                                              const int* __restrict__ rp,
                                                              const int* __restrict__ diag,
+                                                             const int* __restrict__ lend,
                                                              const int* __restrict__ ci,
                                                              const T* __restrict__ a,
                                                              const T* __restrict__ D,
@@ -167,7 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_ssor_upper_level(int cnt, const int*
         return;
     const int i = rows[t];
     T s = D[i] * y[i];
-    for (int k = diag[i] + 1; k < rp[i + 1]; ++k)
+    // the row's entries right of the diagonal (lend: first column >= i, the
+    // start when no diagonal is stored)
+    for (int k = diag[i] >= 0 ? diag[i] + 1 : lend[i]; k < rp[i + 1]; ++k)
         s -= a[k] * z[ci[k]];
     z[i] = s * invD[i];
 }
@@ -222,7 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_ssor_upper_run(StreamMat<T> U, const
     stream_rows(U, z, e);
 }
 
-// D = a_ii, invD = 1 / a_ii (1 when zero), in T
+// D = a_ii, invD = 1 / a_ii (1 when zero); a row without a stored diagonal
+// has D = invD = 1 (diag_ and invDiag_ keep their initial 1:
+// src/solver/LinearSolvers.cpp:174-184)
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_ssor_diag(int n, const int* __restrict__ diag,
                                                       const double* __restrict__ a,
@@ -231,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_ssor_diag(int n, const int* __restri
     const int i = int(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n)
         return;
-    const double d = a[diag[i]];
+    const double d = diag[i] >= 0 ? a[diag[i]] : 1.0;
     D[i] = T(d);
     invD[i] = T(d != 0.0 ? 1.0 / d : 1.0);
 }
@@ -371,9 +391,48 @@ int ilu0_factor_host(int n, const int* rp, const int* ci, std::vector<double>& l
     return MI_OK;
 }
 
+bool ilu0_factor_facade(int n, const int* rp, const int* ci, std::vector<double>& lu)
+{
+    std::vector<int> diag(static_cast<size_t>(n), -1);
+    for (int i = 0; i < n; ++i) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i) {
+                diag[size_t(i)] = k;
+                break;
+            }
+        if (diag[size_t(i)] < 0)
+            return false; // nothing has been modified yet
+    }
+    for (int i = 1; i < n; ++i) {
+        for (int kp = rp[i]; kp < rp[i + 1]; ++kp) {
+            const int k = ci[kp];
+            if (k >= i)
+                break;
+            const double piv = lu[size_t(diag[size_t(k)])];
+            if (piv == 0.0)
+                return false; // rows before i stay factored, as the facade leaves them
+            const double m = lu[size_t(kp)] / piv;
+            lu[size_t(kp)] = m;
+            int qp = kp + 1;
+            for (int jp = diag[size_t(k)] + 1; jp < rp[k + 1]; ++jp) {
+                const int j = ci[jp];
+                while (qp < rp[i + 1] && ci[qp] < j)
+                    ++qp;
+                if (qp >= rp[i + 1])
+                    break;
+                if (ci[qp] == j)
+                    lu[size_t(qp)] -= m * lu[size_t(jp)];
+            }
+        }
+        if (lu[size_t(diag[size_t(i)])] == 0.0)
+            return false;
+    }
+    return true;
+}
+
 void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& lend,
                  std::vector<int>& lrows, std::vector<int>& loff, std::vector<int>& urows,
-                 std::vector<int>& uoff)
+                 std::vector<int>& uoff, const std::vector<int>* prefix_diag)
 {
     std::vector<int> lev(static_cast<size_t>(n), 0);
     int nl = 0;
@@ -399,7 +458,10 @@ void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& le
     int nu = 0;
     for (int i = n - 1; i >= 0; --i) {
         int l = 0;
-        for (int k = lend[size_t(i)]; k < rp[i + 1]; ++k)
+        // a row without a diagonal under the prefix rule reads z_j of every
+        // column j > i in rows 0..i
+        const bool pre = prefix_diag && (*prefix_diag)[size_t(i)] < 0;
+        for (int k = pre ? 0 : lend[size_t(i)]; k < rp[i + 1]; ++k)
             if (ci[k] > i)
                 l = std::max(l, lev[size_t(ci[k])] + 1);
         // lev[] of rows > i already hold their U levels (processed first)
@@ -488,6 +550,7 @@ template <class T> int Ilu0<T>::prepare(const Pattern& P, hipStream_t s)
     const int* rp = P.h_rp.data();
     const int* ci = P.h_ci.data();
     std::vector<int> diag(static_cast<size_t>(n), -1), lend(static_cast<size_t>(n));
+    int missing = 0;
     for (int i = 0; i < n; ++i) {
         int k = rp[i];
         while (k < rp[i + 1] && ci[k] < i)
@@ -496,10 +559,10 @@ template <class T> int Ilu0<T>::prepare(const Pattern& P, hipStream_t s)
         if (k < rp[i + 1] && ci[k] == i)
             diag[size_t(i)] = k;
         else
-            return set_error(MI_ERR_ZERO_PIVOT, "ILU(0): row %d has no diagonal entry", i);
+            ++missing;
     }
     std::vector<int> lrows, loff, urows, uoff;
-    ilu0_levels(n, rp, ci, lend, lrows, loff, urows, uoff);
+    ilu0_levels(n, rp, ci, lend, lrows, loff, urows, uoff, &diag);
     MI_TRY(d_lrows.upload(lrows.data(), lrows.size(), s));
     MI_TRY(d_urows.upload(urows.data(), urows.size(), s));
     MI_TRY(d_lend.upload(lend.data(), lend.size(), s));
@@ -511,7 +574,9 @@ template <class T> int Ilu0<T>::prepare(const Pattern& P, hipStream_t s)
     // a large SpMV); otherwise the dependency levels
     ilu0_segments(n, rp, ci, lend, diag, lbeg, ubeg);
     const int runs = int(std::max(lbeg.size(), ubeg.size())) - 1;
-    seg = n > 0 && int64_t(n) >= int64_t(2048) * runs;
+    // rows without a diagonal keep the level sweeps, which follow the
+    // reference's treatment of them exactly
+    seg = missing == 0 && n > 0 && int64_t(n) >= int64_t(2048) * runs;
     two = false;
     if (seg) {
         std::vector<int> trp, tci, blk;
@@ -651,7 +716,8 @@ template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_
             for (size_t l = 0; l + 1 < h_uoff.size(); ++l) {
                 const int beg = h_uoff[l], cnt = h_uoff[l + 1] - beg;
                 k_ssor_upper_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(
-                    cnt, d_urows.p + beg, rp_dev, d_diag.p, ci_dev, d_lu.p, Ud.p, d_invD.p, d_y.p, z);
+                    cnt, d_urows.p + beg, rp_dev, d_diag.p, d_lend.p, ci_dev, d_lu.p, Ud.p,
+                    d_invD.p, d_y.p, z);
             }
         }
         MI_HIP(hipGetLastError());
@@ -696,8 +762,8 @@ template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_
     }
     for (size_t l = 0; l + 1 < h_uoff.size(); ++l) {
         const int beg = h_uoff[l], cnt = h_uoff[l + 1] - beg;
-        k_upper_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(cnt, d_urows.p + beg, rp_dev,
-                                                                  d_diag.p, ci_dev, d_lu.p, d_y.p, z);
+        k_upper_level<T><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(
+            cnt, d_urows.p + beg, rp_dev, d_diag.p, d_lend.p, ci_dev, d_lu.p, d_y.p, z, 1);
     }
     MI_HIP(hipGetLastError());
     return MI_OK;
@@ -805,14 +871,11 @@ MI_API int mi_ilu0_factor_inplace(size_t n, const void* rowPtr, const void* colI
     MI_HIP(hipMemcpyAsync(ci.data(), colIdx, nnz * sizeof(int), hipMemcpyDeviceToHost, s));
     MI_HIP(hipMemcpyAsync(v.data(), values, nnz * sizeof(double), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
-    std::vector<int> diag;
-    int zeros = 0;
-    MI_TRY(ilu0_factor_host(int(n), rp.data(), ci.data(), v, diag, &zeros));
+    const bool done = ilu0_factor_facade(int(n), rp.data(), ci.data(), v);
     MI_HIP(hipMemcpyAsync(values, v.data(), nnz * sizeof(double), hipMemcpyHostToDevice, s));
     MI_HIP(hipStreamSynchronize(s));
-    // GpuKernels::ilu0_factor_inplace returns false on a zero pivot
     if (ok)
-        *ok = zeros == 0;
+        *ok = done ? 1 : 0;
     return MI_OK;
 }
 
@@ -842,11 +905,9 @@ MI_API int mi_ilu0_apply(size_t n, const void* rowPtr, const void* colIdx, const
             ++k;
         lend[i] = k;
         if (k < rp[i + 1] && ci[size_t(k)] == int(i))
-            diag[i] = k;
-        else
-            return set_error(MI_ERR_ZERO_PIVOT, "ILU(0): row %zu has no diagonal entry", i);
+            diag[i] = k; // absent: divide by 1 (src/gpu/GpuKernels.cpp:466-477)
     }
-    ilu0_levels(int(n), rp.data(), ci.data(), lend, lrows, loff, urows, uoff);
+    ilu0_levels(int(n), rp.data(), ci.data(), lend, lrows, loff, urows, uoff, nullptr);
     DevBuf<int> dl, du, dlend, ddiag;
     DevBuf<double> y;
     MI_TRY(dl.upload(lrows.data(), lrows.size(), s));
@@ -865,7 +926,7 @@ MI_API int mi_ilu0_apply(size_t n, const void* rowPtr, const void* colIdx, const
     for (size_t l = 0; l + 1 < uoff.size(); ++l) {
         const int beg = uoff[l], cnt = uoff[l + 1] - beg;
         k_upper_level<double><<<grid_for(cnt, kBlock), kBlock, 0, s>>>(
-            cnt, du.p + beg, drp, ddiag.p, dci, lu, y.p, static_cast<double*>(z));
+            cnt, du.p + beg, drp, ddiag.p, dlend.p, dci, lu, y.p, static_cast<double*>(z), 0);
     }
     MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(s));

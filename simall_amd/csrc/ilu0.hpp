@@ -14,6 +14,12 @@ namespace mi
 int ilu0_factor_host(int n, const int* rp, const int* ci, std::vector<double>& lu,
                      std::vector<int>& diag, int* zero_pivots);
 
+// ILU(0) of the GPU facade (GpuKernels::ilu0_factor_inplace,
+// src/gpu/GpuKernels.cpp:345-388): false, with nothing modified, when a row
+// has no diagonal; false at the first zero pivot met, with the rows before it
+// already factored; true otherwise.  Same elimination as ilu0_factor_host.
+bool ilu0_factor_facade(int n, const int* rp, const int* ci, std::vector<double>& lu);
+
 // Runs of consecutive rows with no dependency inside a run: for the L sweep,
 // row i starts a new run when one of its lower entries lies in the current
 // run; for the U sweep the same scanning from the last row.  lbeg / ubeg
@@ -23,9 +29,12 @@ void ilu0_segments(int n, const int* rp, const int* ci, const std::vector<int>& 
 
 // Dependency levels of the L and U sweeps: rows of level l are
 // rows[off[l] .. off[l+1]).  lend[i] = first position of row i with col >= i.
+// With prefix_diag, a row whose (*prefix_diag)[i] < 0 (no stored diagonal)
+// depends on every column > i of rows 0..i, as ILU0Preconditioner::apply
+// sweeps such a row (src/solver/LinearSolvers.cpp:146-150).
 void ilu0_levels(int n, const int* rp, const int* ci, const std::vector<int>& lend,
                  std::vector<int>& lrows, std::vector<int>& loff, std::vector<int>& urows,
-                 std::vector<int>& uoff);
+                 std::vector<int>& uoff, const std::vector<int>* prefix_diag);
 
 template <class T> struct Ilu0
 {

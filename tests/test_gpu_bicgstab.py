@@ -222,3 +222,92 @@ def test_ssor_run_sweeps_red_black(gpu, momentum):
     z = s.precond_apply(r)
     zo = oracle.ssor_apply(B, r, 1.0)
     assert np.linalg.norm(z - zo) <= 1e-13 * np.linalg.norm(zo)
+
+
+# A 3x3 matrix whose row 1 stores no diagonal (columns 0 and 2 only).
+KAT_RP = np.array([0, 2, 4, 5], np.int32)
+KAT_CI = np.array([0, 2, 0, 2, 2], np.int32)
+KAT_VA = np.array([2.0, 1.0, 1.0, 3.0, 4.0])
+
+
+def test_ilu0_missing_diagonal_known_answer(gpu):
+    """ILU0Preconditioner (src/solver/LinearSolvers.cpp:81-151) on KAT_*, by
+    hand: setup eliminates row 1 with row 0 (m = 1/2, a_12 = 3 - 1/2 = 2.5), so
+    lu = [2, 1, 0.5, 2.5, 4].  For r = (1, 2, 8): y = (1, 1.5, 8); z_2 = 8/4 = 2;
+    row 1 has diagPtr_ = -1, so its loop runs from position 0 through the end
+    of row 1, subtracting 1*z_2 (row 0's entry) and 2.5*z_2 with no division:
+    z_1 = 1.5 - 2 - 5 = -5.5; z_0 = (1 - 1*2)/2 = -0.5.  The device level
+    sweeps (caller's order) and the oracle both give exactly that."""
+    A = oracle.Matrix(KAT_RP, KAT_CI, KAT_VA)
+    lu, dp = oracle.ilu0(A)
+    assert np.array_equal(lu, [2.0, 1.0, 0.5, 2.5, 4.0]) and dp[1] == -1
+    r = np.array([1.0, 2.0, 8.0])
+    expect = np.array([-0.5, -5.5, 2.0])
+    assert np.array_equal(oracle.ilu0_apply(A, lu, dp, r), expect)
+    s = solver(A)
+    assert np.array_equal(s.precond_apply(r), expect)
+
+
+def test_ilu0_facade_missing_diagonal_and_zero_pivot(gpu):
+    """The GpuKernels facade (src/gpu/GpuKernels.cpp:345-477): the factorization
+    refuses a matrix with a row lacking its diagonal and leaves the values as
+    they were; it stops at the first zero pivot with the rows before it
+    factored; its apply sweeps a row without a diagonal over the row's own
+    upper entries and divides by 1."""
+    drp, dci = capi.DeviceArray.from_host(KAT_RP), capi.DeviceArray.from_host(KAT_CI)
+    dv = capi.DeviceArray.from_host(KAT_VA)
+    ok = capi.C.c_int(7)
+    capi.check(gpu.mi_ilu0_factor_inplace(3, drp.ptr, dci.ptr, dv.ptr, capi.C.byref(ok)))
+    assert ok.value == 0 and np.array_equal(dv.download(), KAT_VA)
+    # apply on the factor [2, 1, 0.5, 2.5, 4]: y = (1, 1.5, 8), z_2 = 2,
+    # z_1 = 1.5 - 2.5*2 = -3.5 (no division), z_0 = (1 - 2)/2 = -0.5
+    dlu = capi.DeviceArray.from_host(np.array([2.0, 1.0, 0.5, 2.5, 4.0]))
+    dr, dz = capi.DeviceArray.from_host(np.array([1.0, 2.0, 8.0])), capi.DeviceArray(3)
+    capi.check(gpu.mi_ilu0_apply(3, drp.ptr, dci.ptr, dlu.ptr, dr.ptr, dz.ptr, None))
+    assert np.array_equal(dz.download(), [-0.5, -3.5, 2.0])
+    # [[1,1,0],[1,1,0],[0,0,1]]: row 1 becomes (m = 1, a_11 = 1 - 1 = 0) and the
+    # facade returns false there, row 2 untouched
+    rp = np.array([0, 2, 4, 5], np.int32)
+    ci = np.array([0, 1, 0, 1, 2], np.int32)
+    dv2 = capi.DeviceArray.from_host(np.array([1.0, 1.0, 1.0, 1.0, 1.0]))
+    d2rp, d2ci = capi.DeviceArray.from_host(rp), capi.DeviceArray.from_host(ci)
+    capi.check(gpu.mi_ilu0_factor_inplace(3, d2rp.ptr, d2ci.ptr, dv2.ptr, capi.C.byref(ok)))
+    assert ok.value == 0 and np.array_equal(dv2.download(), [1.0, 1.0, 1.0, 0.0, 1.0])
+    # a stored zero on row 0's diagonal is met as the pivot of row 1: nothing changes
+    dv3 = capi.DeviceArray.from_host(np.array([0.0, 1.0, 1.0, 1.0, 1.0]))
+    capi.check(gpu.mi_ilu0_factor_inplace(3, d2rp.ptr, d2ci.ptr, dv3.ptr, capi.C.byref(ok)))
+    assert ok.value == 0 and np.array_equal(dv3.download(), [0.0, 1.0, 1.0, 1.0, 1.0])
+
+
+def _missing_diagonal_matrix(n=3000, seed=4):
+    """Random nonsymmetric, diagonally dominant rows with sorted columns; every
+    37th row stores no diagonal and every 53rd a zero one."""
+    rng = np.random.default_rng(seed)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        cols = set((i + rng.integers(-40, 41, 6)).clip(0, n - 1).tolist())
+        cols.add(i)
+        if i % 37 == 5:
+            cols.discard(i)
+        for j in sorted(cols):
+            ci.append(j)
+            if j == i:
+                va.append(0.0 if i % 53 == 7 else 8.0 + rng.random())
+            else:
+                va.append(rng.uniform(-1, 1))
+        rp.append(len(ci))
+    return oracle.Matrix(np.array(rp, np.int32), np.array(ci, np.int32), np.array(va))
+
+
+def test_ilu0_and_ssor_rows_without_diagonal_bit_exact(gpu):
+    """Rows without a stored diagonal and rows with a zero one: the device
+    ILU(0) (host factorization, level sweeps with the reference's loop for such
+    rows) and SSOR(1) sweeps equal the oracle bit for bit."""
+    A = _missing_diagonal_matrix()
+    r = np.random.default_rng(9).standard_normal(A.n)
+    lu, dp = oracle.ilu0(A)
+    assert (dp < 0).sum() > 50
+    s = solver(A)
+    assert np.array_equal(s.precond_apply(r), oracle.ilu0_apply(A, lu, dp, r))
+    s = solver(A, preconditioner=capi.MI_PC_SSOR)
+    assert np.array_equal(s.precond_apply(r), oracle.ssor_apply(A, r, 1.0))
