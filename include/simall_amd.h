@@ -393,6 +393,16 @@ MI_API int mi_face_gradient(mi_mesh* m, double rho, const double* aP_device, int
 /* SimpleAlgorithm::compute_face_fluxes (src/solver/SimpleAlgorithm.cpp:201-237). */
 MI_API int mi_face_flux(mi_mesh* m, double rho, const double* Ux_device, const double* Uy_device,
                         const double* Uz_device, const double* bc_value_host, double* phi_device);
+/* mi_face_flux that allocates nothing, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight outer-iteration loops).
+ * No counterpart in the reference.  The zone velocity table lives in a
+ * library-owned device array of 1024 zones (24 KiB per GPU, held until the
+ * process ends); a mesh with more zones is refused with MI_ERR_BAD_ARG.
+ * Every call on a GPU shares that GPU's table: calls that overlap on one GPU
+ * (several host threads) race. */
+MI_API int mi_face_flux_noalloc(mi_mesh* m, double rho, const double* Ux_device,
+                                const double* Uy_device, const double* Uz_device,
+                                const double* bc_value_host, double* phi_device);
 MI_API int mi_mesh_destroy(mi_mesh* m);
 
 /* ----------------------------------------------------- gradient, correction

@@ -182,6 +182,7 @@ PROTOTYPES = {
     "mi_assemble_pressure": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i)]),
     "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "mi_face_flux_noalloc": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
     "mi_face_gradient": (_i, [_vp, _d, _vp, _i, _vp, _vp, _vp]),
     "mi_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "mi_batch_set_equation": (_i, [_vp, _i, _vp]),

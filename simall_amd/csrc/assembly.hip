@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 namespace mi
@@ -500,6 +501,97 @@ MI_API int mi_face_gradient(mi_mesh* M, double rho, const double* aP, int pinned
                                                                         grad);
         MI_HIP(hipGetLastError());
     }
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
+}
+
+} // extern "C"
+
+// ------------------------------------------------- allocation-free face flux
+// mi_face_flux_noalloc keeps the zone velocity table, which mi_face_flux
+// uploads into a buffer of its own on every call, in this file-scope device
+// array instead.  Each GPU holds its own copy at its own address for the
+// life of the process; host code never passes the array's name to a kernel,
+// only the address hipGetSymbolAddress returns for the current device.
+namespace
+{
+constexpr int kNoAllocMaxZones = 1024; // 24 KiB of zone velocities
+constexpr int kZoneChunk = 128;        // zones per upload launch (3 KiB of arguments)
+constexpr int kMaxDevices = 64;
+} // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocMaxZones * 3];
+
+namespace
+{
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_table(double** out)
+{
+    int dev = 0;
+    MI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return set_error(MI_ERR_DEVICE, "device %d beyond the scratch table of %d devices", dev,
+                         kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(MI_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                             hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return MI_OK;
+}
+
+// Up to kZoneChunk zone velocities passed by value, so that filling the
+// table needs neither a host staging copy nor a memset.
+struct ZoneChunk
+{
+    int first;
+    int count;
+    double v[kZoneChunk * 3];
+};
+
+__global__ __launch_bounds__(kBlock) void k_store_zones(ZoneChunk c, double* __restrict__ table)
+{
+    for (int t = threadIdx.x; t < c.count * 3; t += kBlock)
+        table[size_t(c.first) * 3 + size_t(t)] = c.v[t];
+}
+} // namespace
+
+extern "C" {
+
+MI_API int mi_face_flux_noalloc(mi_mesh* M, double rho, const double* Ux, const double* Uy,
+                                const double* Uz, const double* bc_value_host, double* phi)
+{
+    if (!M || !Ux || !Uy || !Uz || !phi)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    if (M->n_zones > kNoAllocMaxZones)
+        return set_error(MI_ERR_BAD_ARG,
+                         "mi_face_flux_noalloc: %d boundary zones exceed its table of %d "
+                         "(use mi_face_flux)",
+                         M->n_zones, kNoAllocMaxZones);
+    MI_TRY(ensure_device());
+    double* table = nullptr;
+    MI_TRY(scratch_table(&table));
+    hipStream_t s = default_stream();
+    // the zones this mesh can read (an inlet face's zone is < n_zones); rows
+    // a previous call left beyond them are never read
+    for (int z0 = 0; z0 < M->n_zones; z0 += kZoneChunk) {
+        ZoneChunk c;
+        c.first = z0;
+        c.count = std::min(kZoneChunk, M->n_zones - z0);
+        for (int t = 0; t < c.count * 3; ++t)
+            c.v[t] = bc_value_host ? bc_value_host[size_t(z0) * 3 + size_t(t)] : 0.0;
+        k_store_zones<<<1, kBlock, 0, s>>>(c, table);
+        MI_HIP(hipGetLastError());
+    }
+    MI_TRY(launch_face_flux(M, rho, Ux, Uy, Uz, table, phi, s));
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }
