@@ -22,10 +22,6 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the FP64 SpMV from the rocprofv3 FETCH_SIZE /
-# WRITE_SIZE passes (scaled by a copy kernel of known size), written next to
-# the profile it came from.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sell_spmv_fp64_poisson128_traffic.md")
 COPY_DOUBLES = 1 << 26  # calibration copy: 512 MiB read + 512 MiB written
 
 
@@ -38,29 +34,366 @@ def spmv_bytes(n, nnz, vbytes, ibytes=4):
     return nnz * (vbytes + ibytes) + (n + 1) * 4 + 2 * n * vbytes
 
 
-def read_traffic():
-    """traffic_bytes_per_launch recorded from the counter passes, or None."""
+def host_info():
+    """nproc, the CPUs this process may run on, and the CPU model."""
+    aff = sorted(os.sched_getaffinity(0))
+    model = None
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
     except OSError:
         pass
-    return None
+    return {"nproc": os.cpu_count(), "cpus_allowed": len(aff), "cpu": model}, aff
 
 
-def cpu_baseline(A_rp, A_ci, A_vals, b, iterations_full, sample_iters):
-    """Times the oracle's serial Jacobi-PCG (the reference algorithm, 1 core)
-    for `sample_iters` iterations and projects to `iterations_full`."""
+class Pinned:
+    """Runs the enclosed CPU work on one host core (the calling thread's
+    affinity; the HIP runtime's own threads keep theirs): the reference's
+    solver path is single-threaded."""
+
+    def __init__(self, core):
+        self.core = core
+
+    def __enter__(self):
+        self.old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.old)
+
+
+def _rel(a, b):
+    import numpy as np
+
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _pressure_oracle(oracle, w):
+    import numpy as np
+
+    rp, ci = oracle.build_sparsity(w.mesh)
+    P = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    vals, _, pinned = oracle.assemble_pressure(w.mesh, P, w.bc_of_zone, gamma=w.gamma)
+    b = w.b.copy()
+    if pinned:
+        b[0] = 0.0
+    return oracle.Matrix(rp, ci, vals), b, pinned
+
+
+def _gpu_solve(capi, Ao, b, **kw):
+    cfg = dict(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-12,
+               max_iterations=50000, true_residual=1, max_outer=30)
+    cfg.update(kw)
+    s = capi.Solver(**cfg)
+    s.setup(capi.Csr(Ao.rowPtr, Ao.colIdx, Ao.values))
+    x, st, rc = s.solve(b)
+    return x, st.as_dict(), rc
+
+
+def cpu_leg(capi, wl, lib, A, b, x_mixed, d_b, d_x, sample_iters, core, host):
+    """The reference's CPU path timed on one pinned host core, and each
+    config's GPU solution against the FP64 oracle (the checker).
+
+    Returns (cpu_baseline, error_vs_oracle).  The headline value is the
+    reference algorithm for config 2 -- FP64 Jacobi-PCG (CGSolver::solve,
+    src/solver/LinearSolvers.cpp:415-459) to 1e-8 on the 128^3 system: the
+    iteration count comes from the GPU FP64 Jacobi-PCG, which runs the same
+    recurrence (within +-2 iterations of the oracle), and the time per
+    iteration from a bounded sample of the oracle's serial loop."""
+    import numpy as np
+
     import oracle
 
-    Ao = oracle.Matrix(A_rp, A_ci, A_vals)
-    t0 = time.perf_counter()
-    _, it, _, _ = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-30, sample_iters)
-    dt = time.perf_counter() - t0
-    per_it = dt / max(it, 1)
-    return per_it * iterations_full * 1e3, dt, it
+    t_leg = time.perf_counter()
+    rp, ci = A.pattern()
+    Ao = oracle.Matrix(rp, ci, A.values())
+    n = A.n
+    # --- config 2: Jacobi-PCG to 1e-8 at 128^3 (projected)
+    sj = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                     max_iterations=100000)
+    sj.setup(A)
+    st, rc = sj.solve_device(d_b, d_x)
+    lib.mi_device_synchronize()
+    t1 = time.perf_counter()
+    st, rc = sj.solve_device(d_b, d_x)
+    lib.mi_device_synchronize()
+    gpu_j_ms = 1e3 * (time.perf_counter() - t1)
+    its_j = st.as_dict()["iterations"]
+    with Pinned(core):
+        t1 = time.perf_counter()
+        _, done, _, _ = oracle.cg(Ao, b, oracle.PC_JACOBI, 1e-30, sample_iters)
+        dt = time.perf_counter() - t1
+    per_it = dt / max(done, 1)
+    value_ms = per_it * its_j * 1e3
+    log(f"cpu: 128^3 Jacobi-PCG {per_it * 1e3:.2f} ms/iteration x {its_j} iterations "
+        f"= {value_ms / 1e3:.1f} s (GPU FP64 Jacobi-PCG {gpu_j_ms:.1f} ms)")
+    configs = {"2": {"jacobi_pcg_fp64": {
+        "cpu_ms_projected": round(value_ms, 1), "ms_per_iteration": round(1e3 * per_it, 3),
+        "iterations_to_1e-8": its_j, "sample_iterations": done, "sample_s": round(dt, 2),
+        "gpu_fp64_jacobi_pcg_ms": round(gpu_j_ms, 2)}}}
+    errors = {}
+
+    # --- config 1: 128x128 cavity, Jacobi-PCG FP64 to 1e-8, full size
+    wc = wl.cavity2d(128)
+    Ac, bc, _ = _pressure_oracle(oracle, wc)
+    tc = []
+    with Pinned(core):
+        for _ in range(5):
+            t1 = time.perf_counter()
+            xo8, ito8, _, _ = oracle.cg(Ac, bc, oracle.PC_JACOBI, 1e-8, 10000)
+            tc.append(time.perf_counter() - t1)
+    xot, _, _, _ = oracle.cg(Ac, bc, oracle.PC_JACOBI, 1e-13, 20000)
+    xg8, st8, _ = _gpu_solve(capi, Ac, bc, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                             true_residual=0)
+    xgt, stt, _ = _gpu_solve(capi, Ac, bc, preconditioner=capi.MI_PC_JACOBI)
+    xgm, stm, _ = _gpu_solve(capi, Ac, bc, preconditioner=capi.MI_PC_JACOBI,
+                             precision=capi.MI_PRECISION_MIXED)
+    configs["1"] = {"workload": "128x128 cavity slab, Jacobi-PCG FP64 to 1e-8 (full size)",
+                    "cpu_ms": round(1e3 * float(np.median(tc)), 2), "iterations": ito8}
+    errors["1"] = {"size": "128x128x1",
+                   "fp64_jacobi_1e-8_vs_oracle_1e-8": _rel(xg8, xo8),
+                   "iterations_gpu_oracle": [st8["iterations"], ito8],
+                   "fp64_1e-12_vs_oracle_1e-13": _rel(xgt, xot),
+                   "mixed_1e-12_vs_oracle_1e-13": _rel(xgm, xot)}
+
+    # --- config 2 at 48^3 (oracle-sized) and the 128^3 mixed solve
+    w48 = wl.poisson3d(48)
+    A48, b48, _ = _pressure_oracle(oracle, w48)
+    xo48, _, _, _ = oracle.cg(A48, b48, oracle.PC_JACOBI, 1e-13, 50000)
+    x64t, _, _ = _gpu_solve(capi, A48, b48)
+    xmt, _, _ = _gpu_solve(capi, A48, b48, precision=capi.MI_PRECISION_MIXED)
+    st_t = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-12,
+                       max_iterations=5000, true_residual=1, max_outer=30)
+    st_t.setup(A)
+    st_t.solve_device(d_b, d_x)
+    x128t = d_x.download()
+    errors["2"] = {"size": "48^3 (config-2 generator)",
+                   "fp64_amg_1e-12_vs_oracle_1e-13": _rel(x64t, xo48),
+                   "mixed_amg_1e-12_vs_oracle_1e-13": _rel(xmt, xo48),
+                   "128^3_mixed_1e-8_vs_gpu_fp64_1e-12": _rel(x_mixed, x128t),
+                   "128^3_true_relative_residual_mixed_host_checked":
+                       float(np.linalg.norm(b - oracle.spmv(Ao, x_mixed)) / np.linalg.norm(b))}
+
+    # --- config 2: the reference's RS-AMG-PCG (compiled reference sources)
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libsimall_ref.so")
+    if os.path.exists(ref_path):
+        import ctypes as C
+
+        ref = C.CDLL(ref_path)
+        P_, i_, d_ = C.c_void_p, C.c_int, C.c_double
+        ref.ref_rs_amg_pcg.argtypes = [i_, P_, P_, P_, P_, d_, i_, P_, P_, P_]
+        ref.ref_rs_amg_pcg.restype = i_
+        rs = {}
+        for N in (24, 32):
+            Ar, br, _ = _pressure_oracle(oracle, wl.poisson3d(N))
+            xr = np.zeros(Ar.n)
+            last, setup = C.c_double(), C.c_double()
+            with Pinned(core):
+                t1 = time.perf_counter()
+                it = ref.ref_rs_amg_pcg(Ar.n, Ar.rowPtr.ctypes.data, Ar.colIdx.ctypes.data,
+                                        Ar.values.ctypes.data, br.ctypes.data, 1e-8, 10000,
+                                        xr.ctypes.data, C.byref(last), C.byref(setup))
+                tot = time.perf_counter() - t1
+            rs[f"{N}^3"] = {"n": Ar.n, "setup_s": round(setup.value, 3),
+                            "solve_s": round(tot - setup.value, 3), "iterations": it,
+                            "last_residual": last.value}
+        n1, n2 = rs["24^3"]["n"], rs["32^3"]["n"]
+        p = np.log(rs["32^3"]["setup_s"] / rs["24^3"]["setup_s"]) / np.log(n2 / n1)
+        rs["setup_growth_exponent"] = round(float(p), 2)
+        rs["projected_128^3_setup_s"] = round(float(rs["32^3"]["setup_s"] * (n / n2) ** p), 0)
+        rs["kind"] = "reference"
+        configs["2"]["rs_amg_pcg"] = rs
+        log(f"cpu: reference RS-AMG-PCG {rs}")
+
+    # --- config 3: ILU(0)-BiCGStab FP64 to 1e-8, 96^3 full size
+    w3 = wl.momentum3d(96)
+    m3 = w3.mesh
+    phi = oracle.face_flux(m3, w3.bc_of_zone, 1.0, w3.U)
+    rp3, ci3 = oracle.build_sparsity(m3)
+    v3, _ = oracle.assemble_momentum(m3, oracle.Matrix(rp3, ci3, np.zeros(ci3.size)),
+                                     w3.bc_of_zone, 0, w3.mu, 1.0, phi, dt=w3.dt)
+    A3 = oracle.Matrix(rp3, ci3, v3)
+    with Pinned(core):
+        t1 = time.perf_counter()
+        _, it3, _, _ = oracle.bicgstab(A3, w3.b, oracle.PC_ILU, 1e-8, 2000)
+        t3 = time.perf_counter() - t1
+    xo3, _, _, _ = oracle.bicgstab(A3, w3.b, oracle.PC_ILU, 1e-13, 5000)
+    bi = dict(kind=capi.MI_SOLVER_BICGSTAB, preconditioner=capi.MI_PC_ILU)
+    x3, _, _ = _gpu_solve(capi, A3, w3.b, **bi)
+    x3m, _, _ = _gpu_solve(capi, A3, w3.b, precision=capi.MI_PRECISION_MIXED, **bi)
+    configs["3"] = {"workload": "96^3 momentum, ILU(0)-BiCGStab FP64 to 1e-8 (full size, "
+                                "factorization included)",
+                    "cpu_ms": round(1e3 * t3, 1), "iterations": it3}
+    errors["3"] = {"size": "96^3 (full)", "fp64_ilu_bicgstab_1e-12_vs_oracle_1e-13": _rel(x3, xo3),
+                   "mixed_1e-12_vs_oracle_1e-13": _rel(x3m, xo3)}
+
+    # --- config 4: 32 sequential Jacobi-PCG solves at 100^3 (projected), and
+    # the batched solve against the oracle at 32^3
+    w4 = wl.transport(100, 32)
+    dm4 = capi.DeviceMesh(w4.mesh, w4.bc_of_zone)
+    A4 = dm4.create_matrix()
+    rhs4 = capi.DeviceArray(A4.n)
+    phi4 = capi.DeviceArray.from_host(np.zeros(w4.mesh.n_faces))
+    dt4 = capi.DeviceArray.from_host(np.full(A4.n, w4.dt))
+    u4 = capi.DeviceArray.from_host(np.zeros(A4.n))
+    s4 = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-8,
+                     max_iterations=20000)
+    its4 = []
+    for e in range(32):
+        dm4.assemble_momentum(A4, rhs4, 0, float(w4.gammas[e]), 1.0, phi4, d_dt=dt4, d_Uold=u4)
+        s4.setup(A4)
+        st, rc = s4.solve_device(capi.DeviceArray.from_host(w4.bs[e]), capi.DeviceArray(A4.n))
+        its4.append(st.as_dict()["iterations"])
+        if e == 0:
+            rp4, ci4 = A4.pattern()
+            A40 = oracle.Matrix(rp4, ci4, A4.values())
+    with Pinned(core):
+        t1 = time.perf_counter()
+        _, d4, _, _ = oracle.cg(A40, w4.bs[0], oracle.PC_JACOBI, 1e-30, 150)
+        t4 = time.perf_counter() - t1
+    configs["4"] = {"workload": "32 transport equations on 100^3, one Jacobi-PCG FP64 solve each "
+                                "to 1e-8 (iteration counts from the GPU FP64 Jacobi-PCG)",
+                    "cpu_ms_projected": round(1e3 * t4 / d4 * sum(its4), 1),
+                    "ms_per_iteration": round(1e3 * t4 / d4, 3),
+                    "iterations_total": int(sum(its4)),
+                    "iterations_min_max": [int(min(its4)), int(max(its4))]}
+    w4s = wl.transport(32, 4)
+    dm4s = capi.DeviceMesh(w4s.mesh, w4s.bc_of_zone)
+    A4s = dm4s.create_matrix()
+    rhs4s = capi.DeviceArray(A4s.n)
+    phi4s = capi.DeviceArray.from_host(np.zeros(w4s.mesh.n_faces))
+    dt4s = capi.DeviceArray.from_host(np.full(A4s.n, w4s.dt))
+    u4s = capi.DeviceArray.from_host(np.zeros(A4s.n))
+    rp4s, ci4s = A4s.pattern()
+    vals4 = []
+    for e in range(4):
+        dm4s.assemble_momentum(A4s, rhs4s, 0, float(w4s.gammas[e]), 1.0, phi4s, d_dt=dt4s,
+                               d_Uold=u4s)
+        vals4.append(A4s.values())
+    B4 = capi.Batch(A4s, 4)
+    B4.set_values(np.stack(vals4))
+    xb, _, _, _ = B4.solve(np.stack(w4s.bs[:4]), tolerance=1e-12, max_iterations=5000,
+                           max_outer=40)
+    e4 = [_rel(xb[e], oracle.cg(oracle.Matrix(rp4s, ci4s, vals4[e]), w4s.bs[e],
+                                oracle.PC_JACOBI, 1e-13, 20000)[0]) for e in range(4)]
+    errors["4"] = {"size": "32^3, 4 equations (config-4 generator)",
+                   "batched_1e-12_vs_oracle_1e-13_max": max(e4)}
+
+    # --- config 5: the reference adjoint's GMRES(30)-Jacobi on A^T
+    # (src/adjoint/DiscreteAdjoint.cpp:54-146) on the 128^3 system, a bounded
+    # sample; and the GPU adjoint against the oracle at 24^3
+    wts = wl.uniform(wl.seed(5, wl.Q_WEIGHTS), 0.0, 1.0, n)
+    g = 2.0 * wts * x128t
+    with Pinned(core):
+        t1 = time.perf_counter()
+        _, ok5, it5, res5 = oracle.gmres_transpose(Ao, g, max_iter=60, restart=30, tol=1e-10)
+        t5 = time.perf_counter() - t1
+    configs["5"] = {"workload": "GMRES(30)-Jacobi on A^T, 128^3 (the reference adjoint), "
+                                "a 60-iteration sample",
+                    "ms_per_iteration": round(1e3 * t5 / max(it5, 1), 2),
+                    "iterations_sampled": it5, "converged": ok5,
+                    "residual_after_sample": res5,
+                    "projected_s_for_its_500_iteration_cap": round(t5 / max(it5, 1) * 500, 1)}
+    w5 = wl.poisson3d(24)
+    m5 = w5.mesh
+    A5, b5, pinned5 = _pressure_oracle(oracle, w5)
+    w5t = wl.uniform(wl.seed(5, wl.Q_WEIGHTS), 0.0, 1.0, m5.n_cells)
+    dm5 = capi.DeviceMesh(m5, w5.bc_of_zone)
+    A5d = dm5.create_matrix()
+    r5 = capi.DeviceArray(m5.n_cells)
+    dm5.assemble_pressure(A5d, r5, d_gamma=capi.DeviceArray.from_host(w5.gamma))
+    sf = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-12,
+                     max_iterations=5000, true_residual=1, max_outer=30)
+    sf.setup(A5d)
+    dx5 = capi.DeviceArray(m5.n_cells)
+    sf.solve_device(capi.DeviceArray.from_host(b5), dx5)
+    x5 = dx5.download()
+    sa = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-10,
+                     max_iterations=5000, true_residual=1, max_outer=30)
+    sa.setup(A5d)
+    dl5 = capi.DeviceArray(m5.n_cells)
+    sa.adjoint_solve_device(capi.DeviceArray.from_host(2.0 * w5t * x5), dl5)
+    dg5 = capi.DeviceArray(m5.n_faces)
+    dm5.face_gradient(dx5, dl5, dg5, pinned5)
+    xo5, _, _, _ = oracle.cg(A5, b5, oracle.PC_JACOBI, 1e-13, 50000)
+    # the pinned pressure matrix is symmetric: A^T = A
+    lo5, _, _, _ = oracle.cg(A5, 2.0 * w5t * xo5, oracle.PC_JACOBI, 1e-13, 50000)
+    go5 = oracle.face_gradient(m5, w5.bc_of_zone, xo5, lo5, pinned5)
+    gg5 = dg5.download()
+    errors["5"] = {"size": "24^3 (config-5 system)", "x": _rel(x5, xo5),
+                   "lambda": _rel(dl5.download(), lo5),
+                   "gradient_max_err_over_max": float(np.abs(gg5 - go5).max() / np.abs(go5).max())}
+
+    cpu = {"value": round(value_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
+           "sample": f"the oracle's serial FP64 Jacobi-PCG (the reference's CG arithmetic) on the "
+                     f"128^3 system, {done} iterations in {dt:.1f} s on one pinned core, times "
+                     f"the {its_j} iterations its recurrence needs to reach 1e-8 (counted by the "
+                     f"GPU FP64 Jacobi-PCG, which tracks it within +-2)",
+           "core": core, "host": host, "configs": configs,
+           "leg_s": round(time.perf_counter() - t_leg, 1)}
+    log(f"cpu leg: {time.perf_counter() - t_leg:.1f} s")
+    return cpu, errors
+
+
+def measure_traffic():
+    """HBM bytes per launch of k_sell_spmv<double> from two rocprofv3 counter
+    passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --spmv-only`, each
+    scaled by what the same counter reads for the plain copy kernel of known
+    size that run also launches.  Runs the passes as child processes before
+    this process touches the GPU; returns (bytes or None, details)."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, {"source": "rocprofv3 not found"}
+    out = {}
+    base = tempfile.mkdtemp(prefix="simall_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = os.path.join(base, counter.lower())
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
+               "-d", d, "-o", counter.lower(), "--", sys.executable, os.path.abspath(__file__),
+               "--spmv-only"]
+        t1 = time.perf_counter()
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                           stderr=subprocess.PIPE, text=True)
+        log(f"traffic pass {counter}: rc {r.returncode} ({time.perf_counter() - t1:.1f}s)")
+        if r.returncode != 0:
+            return None, {"source": f"rocprofv3 --pmc {counter} exited {r.returncode}"}
+        sums = {"spmv": [], "copy": []}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for row in csv.DictReader(f):
+                    name = row.get("Kernel_Name", "")
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = ("spmv" if "k_sell_spmv<double>" in name else
+                           "copy" if "k_axpby" in name else None)
+                    if key:
+                        sums[key].append(float(row["Counter_Value"]))
+        if not sums["spmv"] or not sums["copy"]:
+            return None, {"source": f"no {counter} rows for the SpMV or the copy kernel"}
+        out[counter] = {k: (sum(v) / len(v), len(v)) for k, v in sums.items()}
+    shutil.rmtree(base, ignore_errors=True)
+    copy_kib = 8 * COPY_DOUBLES / 1024.0
+    sf = copy_kib / out["FETCH_SIZE"]["copy"][0]
+    sw = copy_kib / out["WRITE_SIZE"]["copy"][0]
+    fetch = out["FETCH_SIZE"]["spmv"][0] * sf * 1024.0
+    write = out["WRITE_SIZE"]["spmv"][0] * sw * 1024.0
+    det = {"source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over bench.py --spmv-only, "
+                     "measured in this run",
+           "fetch_bytes": round(fetch), "write_bytes": round(write),
+           "fetch_scale": round(sf, 4), "write_scale": round(sw, 4),
+           "spmv_dispatches": out["FETCH_SIZE"]["spmv"][1]}
+    return fetch + write, det
 
 
 def bench_momentum(capi, wl, lib, steps, log, N=96):
@@ -307,8 +640,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--pc", default="amg", choices=["jacobi", "amg"])
-    ap.add_argument("--cpu-iters", type=int, default=900)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=400,
+                    help="oracle Jacobi-PCG iterations timed for the 128^3 CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline leg and the comparisons with the oracle")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure HBM traffic")
     ap.add_argument("--modes", default="fp64,mixed", help="solver precisions to time")
     ap.add_argument("--degree", type=int, default=0, help="AMG smoother degree (0: default)")
     ap.add_argument("--coarse-degree", type=int, default=0,
@@ -338,6 +675,12 @@ def main():
 
     from simall_amd import capi
     from simall_amd import workload as wl
+
+    traffic, traffic_detail = None, {"source": "not measured (--no-traffic)"}
+    if not args.spmv_only and not args.no_traffic:
+        # child processes, before this process initialises the GPU
+        traffic, traffic_detail = measure_traffic()
+        log(f"traffic: {traffic} {traffic_detail}")
 
     lib = capi.load()
     capi.check(lib.mi_init(0))
@@ -440,7 +783,7 @@ def main():
     if "fp64" in solvers and not args.no_adjoint:
         # discrete adjoint of the FP64 solve: A^T lambda = 2 w x, then dJ/dgamma_f per face
         s64 = solvers["fp64"]
-        wts = wl.uniform(wl.seed(4, 5), 0.0, 1.0, n)
+        wts = wl.uniform(wl.seed(5, wl.Q_WEIGHTS), 0.0, 1.0, n)
         d_g = capi.DeviceArray.from_host(2.0 * wts * results["fp64"]["x"])
         d_l = capi.DeviceArray(n)
         d_xs = capi.DeviceArray.from_host(results["fp64"]["x"])
@@ -486,17 +829,13 @@ def main():
         x64, xm = results["fp64"]["x"], results["mixed"]["x"]
         err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
 
-    cpu = None
-    if not args.no_cpu and "fp64" in results:
-        rp, ci = A.pattern()
-        vals = A.values()
-        its = results["fp64"]["stats"]["iterations"]
-        proj_ms, dt, done = cpu_baseline(rp, ci, vals, b, its, args.cpu_iters)
-        cpu = {"value": round(proj_ms, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"{done} iterations of the oracle's serial Jacobi-PCG on the same "
-                         f"{args.n}^3 system ({dt:.1f} s), projected to the {its} iterations "
-                         f"of the FP64 GPU solve"}
+    cpu, errors = None, None
+    if not args.no_cpu and "mixed" in results:
+        host, aff = host_info()
+        cpu, errors = cpu_leg(capi, wl, lib, A, b, results["mixed"]["x"], d_b, d_x,
+                              args.cpu_iters, aff[0], host)
         log(f"cpu baseline: {cpu}")
+        log(f"error vs oracle: {errors}")
 
     mixed_ms = results["mixed"]["ms"] if "mixed" in results else float("nan")
     fp64_ms = results["fp64"]["ms"] if "fp64" in results else float("nan")
@@ -536,12 +875,14 @@ def main():
                  "bandwidth": [info["bandwidth_before"], info["bandwidth_after"]]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64c, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs64c / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(), "kernel": "k_sell_spmv<double>",
+                     "traffic": round(traffic) if traffic else None,
+                     "traffic_detail": traffic_detail, "kernel": "k_sell_spmv<double>",
                      "algorithmic_bytes": by64, "avg_us": round(us64c, 2),
                      "cache": "cold: 20 back-to-back launches rotating over copies of the "
                               "matrix (> 1 GiB in all), HIP events",
                      "warm_achieved": round(gbs64, 1), "warm_avg_us": round(us64, 2)},
         "cpu_baseline": cpu,
+        "error_vs_oracle": errors,
         "adjoint": adjoint,
         "batched": batch,
         "momentum": momentum,
