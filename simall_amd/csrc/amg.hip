@@ -207,18 +207,12 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
 }
 
 template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r, const T* invD,
-                                   T* x, double beta, double alpha, int mode, hipStream_t s)
+                                   T* x, double beta, double alpha, int mode, Launcher& L)
 {
     if (A.nr == 0)
         return MI_OK;
-    if (A.sell.built())
-        k_cheb_step<T, true><<<A.grid(), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
-                                                         T(alpha), mode);
-    else
-        k_cheb_step<T, false><<<A.grid(), kBlock, 0, s>>>(A.sm(), din, dout, r, invD, x, T(beta),
-                                                          T(alpha), mode);
-    MI_HIP(hipGetLastError());
-    return MI_OK;
+    return L(A.sell.built() ? k_cheb_step<T, true> : k_cheb_step<T, false>, A.grid(), kBlock,
+             A.sm(), din, dout, r, invD, x, T(beta), T(alpha), mode);
 }
 
 template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree,
@@ -405,7 +399,13 @@ int dense_inverse_device(int n, double amax, double* a, double* inv, hipStream_t
 template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph)
 {
     if (use_graph)
-        return graphs.run(b0, x0, s, [&]() { return vcycle(b0, x0, s, false); });
+        return graphs.run(b0, x0, s, [&](Launcher& L) -> int { return cycle(b0, x0, L); });
+    Launcher L{s};
+    return cycle(b0, x0, L);
+}
+
+template <class T> int DHierarchy<T>::cycle(const T* b0, T* x0, Launcher& Lc)
+{
     const int nl = int(L.size());
     auto bptr = [&](int l) -> const T* { return l == 0 ? b0 : L[size_t(l)].b.p; };
     auto xptr = [&](int l) -> T* { return l == 0 ? x0 : L[size_t(l)].x.p; };
@@ -415,30 +415,27 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         const int n = d.A.nr;
         const T* b = bptr(l);
         T* x = xptr(l);
-        k_cheb_init<T><<<rows_grid(n, 1), kBlock, 0, s>>>(n, b, d.invD.p, T(d.alpha[0]), x, d.r.p,
-                                                         d.d.p);
-        MI_HIP(hipGetLastError());
+        MI_TRY(Lc(k_cheb_init<T>, rows_grid(n, 1), kBlock, n, b, d.invD.p, T(d.alpha[0]), x,
+                  d.r.p, d.d.p));
         T* din = d.d.p;
         T* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
             MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
-                           0, s));
+                           0, Lc));
             std::swap(din, dout);
         }
-        MI_TRY(step<T>(d.A, din, nullptr, d.r.p, d.invD.p, x, 0.0, 0.0, 1, s));
+        MI_TRY(step<T>(d.A, din, nullptr, d.r.p, d.invD.p, x, 0.0, 0.0, 1, Lc));
         if (d.R.nr > 0)
-            (d.R.sell.built() ? k_restrict<T, true> : k_restrict<T, false>)<<<d.R.grid(), kBlock, 0, s>>>(
-                d.R.sm(), d.r.p, L[size_t(l) + 1].b.p);
-        MI_HIP(hipGetLastError());
+            MI_TRY(Lc(d.R.sell.built() ? k_restrict<T, true> : k_restrict<T, false>, d.R.grid(),
+                      kBlock, d.R.sm(), static_cast<const T*>(d.r.p), L[size_t(l) + 1].b.p));
     }
     // coarsest level
     {
         const T* b = bptr(nl - 1);
         T* x = xptr(nl - 1);
-        if (nc > 0) {
-            k_coarse_solve<T><<<unsigned(nc), kBlock, 0, s>>>(nc, cinv.p, b, x);
-            MI_HIP(hipGetLastError());
-        }
+        if (nc > 0)
+            MI_TRY(Lc(k_coarse_solve<T>, unsigned(nc), kBlock, nc,
+                      static_cast<const double*>(cinv.p), b, x));
     }
     // upward leg: prolongate, post-smooth
     for (int l = nl - 2; l >= 0; --l) {
@@ -447,16 +444,15 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
         T* x = xptr(l);
         const T* xc = xptr(l + 1);
         if (n > 0)
-            (d.P.sell.built() ? k_prolong<T, true> : k_prolong<T, false>)<<<d.P.grid(), kBlock, 0, s>>>(
-                d.P.sm(), xc, d.e.p, x);
-        MI_HIP(hipGetLastError());
+            MI_TRY(Lc(d.P.sell.built() ? k_prolong<T, true> : k_prolong<T, false>, d.P.grid(),
+                      kBlock, d.P.sm(), xc, d.e.p, x));
         // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps
-        MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, s));
+        MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, Lc));
         T* din = d.d.p;
         T* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
             MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
-                           0, s));
+                           0, Lc));
             std::swap(din, dout);
         }
     }

@@ -66,6 +66,8 @@ template <class T> struct DHierarchy
     // x0 = V-cycle(b0); x0 needs no initialisation.  With use_graph the
     // cycle's launches are replayed from a graph captured for (b0, x0, s).
     int vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph = false);
+    // the launches of one V-cycle, issued through L (at once or into a graph)
+    int cycle(const T* b0, T* x0, Launcher& L);
 };
 
 std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c);

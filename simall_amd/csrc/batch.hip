@@ -678,43 +678,40 @@ namespace
 
 constexpr int kBatchMaxGrid = 4096;
 
-int fin(mi_batch* B, int op, const BFinArgs& f)
+int fin(mi_batch* B, int op, const BFinArgs& f, Launcher& L)
 {
-    k_bfin<<<B->E, kBlock, 0, B->stream>>>(B->partials.p, B->st.p, op, f);
-    MI_HIP(hipGetLastError());
-    return MI_OK;
+    return L(k_bfin, unsigned(B->E), kBlock, static_cast<const double*>(B->partials.p), B->st.p,
+             op, f);
 }
 
-int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f)
+int fin(mi_batch* B, int op, const BFinArgs& f)
 {
-    k_bspmv<B_ALPHA><<<B->G, kBlock, 0, B->stream>>>(a);
-    MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_ALPHA, f));
-    k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_UPDATE);
-    MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_UPDATE, f));
-    k_bvec<<<B->G, kBlock, 0, B->stream>>>(a, B_P);
-    MI_HIP(hipGetLastError());
-    return MI_OK;
+    Launcher L{B->stream};
+    return fin(B, op, f, L);
+}
+
+// One CG iteration for every iterating equation (five launches).
+int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
+{
+    MI_TRY(L(k_bspmv<B_ALPHA>, B->G, kBlock, a));
+    MI_TRY(fin(B, B_ALPHA, f, L));
+    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_UPDATE)));
+    MI_TRY(fin(B, B_UPDATE, f, L));
+    return L(k_bvec, B->G, kBlock, a, int(B_P));
 }
 
 // One BiCGStab iteration for every iterating equation (nine launches).
-int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f)
+int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    hipStream_t s = B->stream;
-    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_P);
-    k_bspmv<B_BI_ALPHA><<<B->G, kBlock, 0, s>>>(a);
-    MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_BI_ALPHA, f));
-    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_S);
-    MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_BI_S, f));
-    k_bspmv<B_BI_T><<<B->G, kBlock, 0, s>>>(a);
-    MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_BI_OMEGA, f));
-    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BI_UPDATE);
-    MI_HIP(hipGetLastError());
-    return fin(B, B_BI_UPDATE, f);
+    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_P)));
+    MI_TRY(L(k_bspmv<B_BI_ALPHA>, B->G, kBlock, a));
+    MI_TRY(fin(B, B_BI_ALPHA, f, L));
+    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_S)));
+    MI_TRY(fin(B, B_BI_S, f, L));
+    MI_TRY(L(k_bspmv<B_BI_T>, B->G, kBlock, a));
+    MI_TRY(fin(B, B_BI_OMEGA, f, L));
+    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_UPDATE)));
+    return fin(B, B_BI_UPDATE, f, L);
 }
 
 // Autotuner: workgroups of the persistent batched launches, timed on this
@@ -809,29 +806,21 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     MI_HIP(hipGetLastError());
     MI_TRY(fin(B, B_BNORM, f));
     const int chunk = c.check_interval > 0 ? c.check_interval : 8;
-    const bool use_graph = c.use_graph != 0;
-    auto iteration = [&]() { return bicg ? one_iteration_bicg(B, a, f) : one_iteration(B, a, f); };
+    const bool use_graph = c.use_graph != 0 && graphs_enabled();
+    auto iteration = [&](Launcher& L) {
+        return bicg ? one_iteration_bicg(B, a, f, L) : one_iteration(B, a, f, L);
+    };
     if (use_graph && (!B->graph || B->graph_chunk != chunk || B->graph_kind != c.kind ||
                       std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0)) {
         if (B->graph) {
             (void)hipGraphExecDestroy(B->graph);
             B->graph = nullptr;
         }
-        MI_HIP(hipStreamSynchronize(s));
-        hipGraph_t g = nullptr;
-        MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        int rc = MI_OK;
-        for (int it = 0; it < chunk && rc == MI_OK; ++it)
-            rc = iteration();
-        const hipError_t ec = hipStreamEndCapture(s, &g);
-        if (rc != MI_OK || ec != hipSuccess || !g) {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return rc != MI_OK ? rc : set_error(MI_ERR_DEVICE, "graph capture failed");
-        }
-        const hipError_t ei = hipGraphInstantiate(&B->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        MI_HIP(ei);
+        MI_TRY(build_graph(s, [&](Launcher& L) -> int {
+            for (int it = 0; it < chunk; ++it)
+                MI_TRY(iteration(L));
+            return MI_OK;
+        }, &B->graph));
         B->graph_chunk = chunk;
         B->graph_kind = c.kind;
         B->graph_f = f;
@@ -854,8 +843,9 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             if (use_graph) {
                 MI_HIP(hipGraphLaunch(B->graph, s));
             } else {
+                Launcher L{s};
                 for (int it = 0; it < chunk; ++it)
-                    MI_TRY(iteration());
+                    MI_TRY(iteration(L));
             }
             MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(E), hipMemcpyDeviceToHost,
                                   s));

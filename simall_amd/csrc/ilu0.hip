@@ -545,7 +545,6 @@ template <class T> int Ilu0<T>::prepare(const Pattern& P, hipStream_t s)
 {
     if (level_built && levels_for == &P)
         return MI_OK;
-    graphs.clear();
     n = int(P.n);
     const int* rp = P.h_rp.data();
     const int* ci = P.h_ci.data();
@@ -685,12 +684,11 @@ template <class T> int Ilu0<T>::setup_ssor(const Pattern& P, const double* vals,
     return MI_OK;
 }
 
-template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s, bool use_graph)
+template <class T> int Ilu0<T>::apply(const T* r, T* z, hipStream_t s)
 {
-    // two sweeps of one launch per level: replayed from a captured graph
-    // when asked (the level count is in the hundreds for 3-D meshes)
-    if (use_graph)
-        return graphs.run(r, z, s, [&]() { return apply(r, z, s, false); });
+    // Launched directly: a graph of the ~600 level launches of a 3-D mesh in
+    // natural order measured no faster (the cost of each tiny level launch is
+    // on the device), and the run orderings need two to four launches.
     if (ssor) {
         if (seg) {
             for (size_t t = 0; t + 1 < Lsb.size(); t += 2) {
@@ -820,16 +818,11 @@ struct IluPrecond final : Precond
     }
     int apply64(const double* r, double* z, hipStream_t s) override
     {
-        // The level sweeps are launched directly: a captured graph of the
-        // ~600 level launches of a 3-D mesh measured no faster (the cost of
-        // each tiny level launch is on the device, not the host).
-        return has64 ? f64.apply(r, z, s, false)
-                     : set_error(MI_ERR_UNSUPPORTED, "no FP64 ILU(0)");
+        return has64 ? f64.apply(r, z, s) : set_error(MI_ERR_UNSUPPORTED, "no FP64 ILU(0)");
     }
     int apply32(const float* r, float* z, hipStream_t s) override
     {
-        return has32 ? f32.apply(r, z, s, false)
-                     : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
+        return has32 ? f32.apply(r, z, s) : set_error(MI_ERR_UNSUPPORTED, "no FP32 ILU(0)");
     }
     const char* name() const override { return ssor ? "ssor" : "ilu0"; }
     int levels() const override { return has64 ? f64.sweep_launches() : f32.sweep_launches(); }

@@ -47,7 +47,6 @@ template <class T> struct Ilu0
     std::vector<int> h_loff, h_uoff;
     bool level_built = false;
     const Pattern* levels_for = nullptr;
-    GraphCache graphs; // captured applies (one level kernel per launch)
 
     // Segment sweeps: when the rows fall into a few long runs of consecutive
     // rows that do not depend on each other (a multicolour ordering: one run
@@ -75,7 +74,7 @@ template <class T> struct Ilu0
     int setup_ssor(const Pattern& P, const double* vals_device, hipStream_t s);
     bool ssor = false;
     DevBuf<T> d_invD; // SSOR: 1 / a_ii (Ud holds a_ii)
-    int apply(const T* r, T* z, hipStream_t s, bool use_graph = false);
+    int apply(const T* r, T* z, hipStream_t s);
     // launches per sweep: runs (segment sweeps) or dependency levels
     int sweep_launches() const
     {

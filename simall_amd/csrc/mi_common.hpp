@@ -8,6 +8,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <tuple>
+#include <type_traits>
 #include <utility>
 
 #include "simall_amd.h"
@@ -43,6 +45,11 @@ int ensure_device();
 // CSR order (for comparisons; results are identical either way).
 bool column_order_enabled();
 bool sell_enabled();
+// Graph replays (V-cycles, chunks of batched iterations) are on unless the
+// environment sets SIMALL_AMD_NO_GRAPH=1, which launches the same kernels
+// directly: rocprofv3's kernel trace has ended the host process with SIGSEGV
+// inside the runtime below hipGraph calls, so traced runs launch directly.
+bool graphs_enabled();
 // Library stream used when a caller passes a null stream.
 hipStream_t default_stream();
 
@@ -206,10 +213,73 @@ __device__ __forceinline__ void fold_partials(const double* partials, int G, dou
 
 #endif
 
+// Kernel launches that either run at once on `s` or, while `g` is set, are
+// appended to that graph as kernel nodes, each depending on the previous one.
+// Graphs are built node by node rather than by stream capture: nothing is
+// enqueued while a graph is built, and a graph can only ever hold kernels.
+struct Launcher
+{
+    hipStream_t s = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphNode_t last = nullptr;
+
+    // k(args...) on a grid x block launch without dynamic LDS.
+    template <class... P, class... A>
+    int operator()(void (*k)(P...), dim3 grid, dim3 block, A&&... args)
+    {
+        static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+        std::tuple<std::decay_t<P>...> t(static_cast<std::decay_t<P>>(std::forward<A>(args))...);
+        return go(reinterpret_cast<const void*>(k), grid, block, t,
+                  std::index_sequence_for<P...>{});
+    }
+
+  private:
+    template <class Tup, size_t... I>
+    int go(const void* f, dim3 grid, dim3 block, Tup& t, std::index_sequence<I...>)
+    {
+        void* params[sizeof...(I) > 0 ? sizeof...(I) : 1] = {static_cast<void*>(&std::get<I>(t))...};
+        if (!g) {
+            MI_HIP(hipLaunchKernel(f, grid, block, params, 0, s));
+            return MI_OK;
+        }
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(f);
+        kp.gridDim = grid;
+        kp.blockDim = block;
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = params;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        MI_HIP(hipGraphAddKernelNode(&node, g, last ? &last : nullptr, last ? 1 : 0, &kp));
+        last = node;
+        return MI_OK;
+    }
+};
+
+// Builds an executable graph from `launches(Launcher&)` (kernel nodes only).
+template <class F> int build_graph(hipStream_t s, F&& launches, hipGraphExec_t* out)
+{
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    MI_HIP(hipGraphCreate(&g, 0));
+    Launcher L{s, g, nullptr};
+    const int rc = launches(L);
+    if (rc != MI_OK) {
+        (void)hipGraphDestroy(g);
+        return rc;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    MI_HIP(ei);
+    *out = exec;
+    return MI_OK;
+}
+
 // Replays of a fixed launch sequence (a preconditioner apply): one
-// executable HIP graph per (input, output, stream), captured on first use.
+// executable HIP graph per (input, output, stream), built on first use.
 // The owner clears the cache whenever the buffers or constants the launches
-// use change.  The sequences captured here contain kernel launches only.
+// use change.
 struct GraphCache
 {
     struct Entry
@@ -230,26 +300,20 @@ struct GraphCache
             (void)hipGraphExecDestroy(e.exec);
         entries.clear();
     }
+    // launches(Launcher&) issues the sequence through the launcher it is given
     template <class F> int run(const void* in, const void* out, hipStream_t s, F&& launches)
     {
+        if (!graphs_enabled()) {
+            Launcher L{s};
+            return launches(L);
+        }
         for (const Entry& e : entries)
             if (e.in == in && e.out == out && e.s == s) {
                 MI_HIP(hipGraphLaunch(e.exec, s));
                 return MI_OK;
             }
-        hipGraph_t g = nullptr;
-        MI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc = launches();
-        const hipError_t ec = hipStreamEndCapture(s, &g);
-        if (rc != MI_OK || ec != hipSuccess || !g) {
-            if (g)
-                (void)hipGraphDestroy(g);
-            return rc != MI_OK ? rc : set_error(MI_ERR_DEVICE, "graph capture failed");
-        }
         hipGraphExec_t exec = nullptr;
-        const hipError_t ei = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        MI_HIP(ei);
+        MI_TRY(build_graph(s, launches, &exec));
         entries.push_back(Entry{in, out, s, exec});
         MI_HIP(hipGraphLaunch(exec, s));
         return MI_OK;

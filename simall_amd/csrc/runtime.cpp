@@ -61,6 +61,15 @@ bool column_order_enabled()
     return on;
 }
 
+bool graphs_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("SIMALL_AMD_NO_GRAPH");
+        return !(e && std::strcmp(e, "1") == 0);
+    }();
+    return on;
+}
+
 bool sell_enabled()
 {
     static const bool on = [] {
