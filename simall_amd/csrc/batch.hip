@@ -3,30 +3,39 @@
 // SimAll solves every scalar transport equation (energy, k/omega, species)
 // with its own lin_.solve on an identical pattern
 // (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232).  Here the
-// E equations of one pattern are solved together: each stage of the
-// iteration is one launch for all equations.
+// E equations of one pattern are solved together, each stage of the
+// iteration one launch for all of them.
 //
-// Layout: equation-major.  Values v[e * nnz + k] (FP32 for the iteration,
-// FP64 for the true residual), vectors x[e * n + i].  A persistent workgroup
-// walks its range of the pattern's row blocks (<= 256 rows, <= 2048
-// nonzeros); for each row block it stages the shared column indices in LDS
-// once, then for every equation still iterating streams that equation's
-// values, gathers its x, and sums each row in column order (thread per row).
-// An equation that has converged is skipped for the whole launch, so the
-// traffic of a launch is that of the active equations plus one pass over the
-// shared pattern.
+// Layout.  The Krylov working set is slot-interleaved: the equations being
+// iterated occupy W slots (1 <= W <= 64), and every per-equation
+// quantity of a nonzero k or a row i is a line of W consecutive values:
+// v32[k * W + s], r[i * W + s], ...  A row of the batched SpMV is handled by
+// W threads (one per slot): one load of colIdx[k] serves every equation,
+// the values of nonzero k are one contiguous W-wide line, and the gather of
+// x at column c is one contiguous line too.  The shared pattern is streamed
+// once per launch and no gather is scattered across equations.  When the
+// pattern has a SELL-64 form (short rows: 7-point operators) the nonzeros
+// are stored in its slot order, m = soff[slice] + 64 k + row-in-slice, so the
+// loads of colIdx and of the value lines of the rows of a workgroup step are
+// contiguous for any W.
+//
+// As equations converge their slots go idle; at the next poll the live ones
+// are repacked into a layout exactly as wide as their number (the other of
+// two slot buffer sets), so launches stream the bytes of the equations still
+// iterating and little more.  Between refinement sweeps everything per
+// equation lives equation-major ("home"): FP64 and FP32 values
+// v[e * nnz + k], b, x and the FP64 true residual [e * n + i].
 //
 // Arithmetic: Jacobi-PCG per equation (CGSolver::solve,
 // src/solver/LinearSolvers.cpp:420-452), or for nonsymmetric equations
 // (convection) Jacobi-preconditioned BiCGStab (BiCGSTABSolver::solve,
-// :360-404), with FP32 values and vectors and FP64 dot products, norms and
-// iterate (accumulated in FP64 as in the mixed single-equation solver); an
-// FP64 true residual with the FP64 values refines each equation until
-// ||b_e - A_e x_e|| <= tol ||b_e||.  Each equation has its own scalars and
-// convergence flag.  One inner CG iteration is five launches (batched SpMV +
-// p.Ap, finalize, update, finalize, direction), one BiCGStab iteration nine;
-// they are captured once into a HIP graph of `chunk` iterations that the
-// host replays between polls.
+// :360-404), with FP32 values, vectors and correction and FP64 dot products
+// and norms; the FP64 true residual with the FP64 values refines each
+// equation until ||b_e - A_e x_e|| <= tol ||b_e|| (x_e accumulates each
+// sweep's correction in FP64).  Each equation has its own scalars and
+// convergence flag.  One inner CG iteration is five launches (batched SpMV
+// + p.Ap, finalize, update, finalize, direction), one BiCGStab iteration
+// nine; `chunk` iterations are replayed from a HIP graph between polls.
 #include "csr.hpp"
 #include "krylov.hpp"
 #include "reorder.hpp"
@@ -43,6 +52,8 @@ namespace mi
 
 constexpr int kMaxEq = 64;
 constexpr int kWaves = kBlock / kWave;
+constexpr int kBatchMaxGrid = 4096;
+constexpr int kSpmmUnroll = 8; // nonzeros of a row loaded ahead of their gathers
 
 struct BEq
 {
@@ -59,19 +70,19 @@ enum BOp : int {
     B_BI_UPDATE = 12
 };
 
+// Slot-layout stages.
 struct BArgs
 {
-    int n, E, nrb;
+    int n, W;
     int64_t nnz;
-    const int* blk; // pattern row blocks
     const int* rp;
     const int* ci;
-    const float* v32;  // [E][nnz]
-    const double* v64; // [E][nnz]
-    const float* invD; // [E][n]
-    const double* b;   // [E][n]
-    double* x;
-    double* rt;
+    const int* soff;   // SELL-64 slice offsets (sell layout)
+    const int* eci;    // SELL-64 slot columns (diagonal flag, -1 padding)
+    const float* v32;  // [nnz or padded SELL slots][W]
+    const float* invD; // [n][W]
+    const int* eq;     // [W] equation of each slot, -1: empty
+    float* xs;         // [n][W] correction of the current sweep
     float* r;          // BiCGStab: r, then s in place
     float* p;
     float* Ap;         // BiCGStab: v
@@ -79,178 +90,322 @@ struct BArgs
     float* y;
     float* z;
     float* t;
-    BEq* st;           // nullptr: every equation active (autotuner)
-    double* partials;  // [2][E][G]
+    BEq* st;           // nullptr: every occupied slot active (autotuner)
+    double* partials;  // [2][W][G]
 };
 
-__device__ __forceinline__ bool b_iterating(const BArgs& a, int e)
+// Equation-major stages (norm of b, FP64 true residual).
+struct HArgs
 {
-    return a.st == nullptr || !a.st[e].done;
+    int n, E, nrb;
+    int64_t nnz;
+    const int* blk; // pattern row blocks
+    const int* rp;
+    const int* ci;
+    const double* v64; // [E][nnz]
+    const double* b;   // [E][n]
+    const double* x;   // [E][n]
+    double* rt;        // [E][n]
+    BEq* st;
+    double* partials;  // [E][G]
+};
+
+__device__ __forceinline__ bool slot_on(const BArgs& a, int op, int e)
+{
+    if (e < 0)
+        return false;
+    if (a.st == nullptr)
+        return true;
+    const BEq& q = a.st[e];
+    if (op == B_BI_T)
+        return !q.done && !q.sconv;
+    return !q.done;
 }
 
-// Per-workgroup FP64 sums of up to two values per equation: each wavefront
-// adds its share into its own slot (fixed order), the slots are folded at
-// the end and stored as partials[v][e][block].
-struct BSums
+// Threads of a workgroup as P = kBlock / W rows of W slots: thread t takes
+// slot t % W of row t / W (threads from P W on are idle).
+struct SlotMap
 {
-    double (*w)[kMaxEq][kWaves];
-    __device__ __forceinline__ void clear(int E)
+    int W, P, sl, rr;
+    bool live;
+    __device__ __forceinline__ explicit SlotMap(int w)
+        : W(w), P(kBlock / w), sl(int(threadIdx.x) % w), rr(int(threadIdx.x) / w),
+          live(int(threadIdx.x) < (kBlock / w) * w)
     {
-        for (int t = threadIdx.x; t < 2 * kMaxEq * kWaves; t += kBlock)
-            (&w[0][0][0])[t] = 0.0;
-        __syncthreads();
-        (void)E;
-    }
-    __device__ __forceinline__ void add(int v, int e, double s)
-    {
-        s = wave_sum(s);
-        if ((threadIdx.x & (kWave - 1)) == 0)
-            w[v][e][threadIdx.x / kWave] += s;
-    }
-    __device__ __forceinline__ void store(const BArgs& a, int nv)
-    {
-        __syncthreads();
-        const int G = gridDim.x;
-        for (int t = threadIdx.x; t < nv * a.E; t += kBlock) {
-            const int v = t / a.E, e = t % a.E;
-            double s = 0.0;
-            for (int q = 0; q < kWaves; ++q)
-                s += w[v][e][q];
-            a.partials[(size_t(v) * a.E + e) * G + blockIdx.x] = s;
-        }
     }
 };
+
+// FP64 sums of up to two values per slot over a workgroup, in a fixed
+// order (rows 0 .. P-1 of each slot); partials[v][s][block].
+__device__ __forceinline__ void slot_partials(const BArgs& a, const SlotMap& m, int nv, double d0,
+                                              double d1)
+{
+    __shared__ double lds[2][kBlock];
+    lds[0][threadIdx.x] = d0;
+    lds[1][threadIdx.x] = d1;
+    __syncthreads();
+    const int G = gridDim.x;
+    if (int(threadIdx.x) < nv * m.W) {
+        const int v = threadIdx.x / m.W, sl = threadIdx.x % m.W;
+        double s = 0.0;
+        for (int q = 0; q < m.P; ++q)
+            s += lds[v][q * m.W + sl];
+        a.partials[(size_t(v) * m.W + sl) * G + blockIdx.x] = s;
+    }
+}
+
+// Contiguous range [beg, end) of `count` items for this workgroup, the
+// ranges of one XCD adjacent (xcd_logical_block), `align` dividing every
+// boundary.
+__device__ __forceinline__ void block_range(int64_t count, int64_t align, int64_t& beg,
+                                            int64_t& end)
+{
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int64_t per = ((count + G - 1) / G + align - 1) / align * align;
+    beg = min(count, int64_t(lb) * per);
+    end = min(count, beg + per);
+}
 
 // ------------------------------------------------------------------ kernels
-// Work of a launch: units (equation, row block) for SpMV-shaped stages and
-// (equation, row chunk) for vector stages, over the equations the stage
-// applies to.  Every workgroup builds the same list of those equations, then
-// walks its contiguous range of units, so units of one equation are
-// consecutive and each is processed exactly like one block of the
-// single-equation CSR-stream engine.
-constexpr int kChunkRows = kBlock * 8;
-
-__device__ __forceinline__ int b_list(const BArgs& a, int op, int* act)
-{
-    __shared__ int cnt;
-    if (threadIdx.x == 0) {
-        int c = 0;
-        for (int e = 0; e < a.E; ++e) {
-            bool on = true;
-            if (op == B_TRUE)
-                on = !a.st[e].fin;
-            else if (op == B_BI_T)
-                on = b_iterating(a, e) && !a.st[e].sconv;
-            else if (op != B_BNORM)
-                on = b_iterating(a, e);
-            if (on)
-                act[c++] = e;
-        }
-        cnt = c;
-    }
-    __syncthreads();
-    return cnt;
-}
-
-// Batched SpMV out_e = A_e in_e (FP32) for the equations of the stage, with
-// FP64 partials of per-row products:
+// Batched SpMV out = A in over the slot layout, with FP64 partials per slot:
 //   B_ALPHA     Ap = A p;  p.Ap                (CG)
 //   B_BI_ALPHA  v = A y;   rh.v                (BiCGStab)
 //   B_BI_T      t = A z;   t.s, t.t            (BiCGStab, s held in r)
-template <int OP>
-__global__ __launch_bounds__(kBlock) void k_bspmv(BArgs a)
+// W threads per row (one per slot), kBlock / W rows per workgroup at a time;
+// each thread sums its equation's row in CSR order.
+template <int OP, bool kSell>
+__global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 {
-    __shared__ float prod[kStreamNnz];
-    __shared__ int act[kMaxEq];
-    __shared__ double wsum[2][kMaxEq][kWaves];
-    BSums S{wsum};
-    S.clear(a.E);
-    const int nact = b_list(a, OP, act);
-    const int G = gridDim.x;
-    const int lb = xcd_logical_block(blockIdx.x, G);
-    const int64_t units = int64_t(nact) * a.nrb;
-    const int64_t per = (units + G - 1) / G;
-    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
-    const int tid = threadIdx.x;
+    const SlotMap m(a.W);
+    const int W = m.W, sl = m.sl, grp = m.rr, R = m.P;
+    const int e = m.live ? a.eq[sl] : -1;
+    const bool on = slot_on(a, OP, e);
     const float* vin = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
     float* vout = OP == B_BI_T ? a.t : a.Ap;
     const float* vw = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
-    for (int64_t u = u0; u < u1; ++u) {
-        const int e = act[u / a.nrb];
-        const int rb = int(u % a.nrb);
-        const float* __restrict__ ve = a.v32 + size_t(e) * a.nnz;
-        const float* __restrict__ pe = vin + size_t(e) * a.n;
-        const float* __restrict__ we = vw + size_t(e) * a.n;
-        float* __restrict__ Ae = vout + size_t(e) * a.n;
-        const int r0 = a.blk[rb], r1 = a.blk[rb + 1];
-        const int k0 = a.rp[r0], k1 = a.rp[r1];
-        if (k1 - k0 > kStreamNnz) {
-            // one long row: strided sums, fixed-order fold over the block
-            float s = 0.f;
-            for (int k = k0 + tid; k < k1; k += kBlock)
-                s += ve[k] * pe[a.ci[k]];
-            prod[tid] = s;
-            __syncthreads();
-            double d0 = 0.0, d1 = 0.0;
-            if (tid == 0) {
-                float t = 0.f;
-                for (int j = 0; j < kBlock; ++j)
-                    t += prod[j];
-                Ae[r0] = t;
-                d0 = double(we[r0]) * double(t);
-                d1 = double(t) * double(t);
+    int64_t r0, r1;
+    block_range(a.n, R, r0, r1);
+    double d0 = 0.0, d1 = 0.0;
+    if (on) {
+        for (int64_t i = r0 + grp; i < r1; i += R) {
+            float acc = 0.f;
+            if constexpr (kSell) {
+                // slot k of row i: m0 + 64 k, in CSR order
+                const int slc = int(i / kSellSlice);
+                const int m0 = a.soff[slc] + int(i % kSellSlice);
+                const int w = (a.soff[slc + 1] - a.soff[slc]) / kSellSlice;
+                for (int k0 = 0; k0 < w; k0 += kSpmmUnroll) {
+                    int c[kSpmmUnroll];
+                    float v[kSpmmUnroll];
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j) {
+                        const size_t mk = size_t(m0) + size_t(k0 + j) * kSellSlice;
+                        const bool in = k0 + j < w;
+                        c[j] = in ? a.eci[mk] : -1;
+                        v[j] = in ? __builtin_nontemporal_load(&a.v32[mk * W + sl]) : 0.f;
+                    }
+                    float xv[kSpmmUnroll];
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j)
+                        xv[j] = c[j] >= 0 ? vin[size_t(c[j] & kSellColMask) * W + sl] : 0.f;
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j)
+                        if (c[j] >= 0)
+                            acc += v[j] * xv[j];
+                }
+            } else {
+                const int kb = a.rp[i], ke = a.rp[i + 1];
+                for (int k0 = kb; k0 < ke; k0 += kSpmmUnroll) {
+                    int c[kSpmmUnroll];
+                    float v[kSpmmUnroll];
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j) {
+                        const int k = k0 + j;
+                        c[j] = k < ke ? a.ci[k] : -1;
+                        v[j] = k < ke ? __builtin_nontemporal_load(&a.v32[size_t(k) * W + sl])
+                                      : 0.f;
+                    }
+                    float xv[kSpmmUnroll];
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j)
+                        xv[j] = c[j] >= 0 ? vin[size_t(c[j]) * W + sl] : 0.f;
+#pragma unroll
+                    for (int j = 0; j < kSpmmUnroll; ++j)
+                        if (c[j] >= 0)
+                            acc += v[j] * xv[j];
+                }
             }
-            S.add(0, e, d0);
+            const size_t q = size_t(i) * W + sl;
+            vout[q] = acc;
+            d0 += double(vw[q]) * double(acc);
             if (OP == B_BI_T)
-                S.add(1, e, d1);
-            __syncthreads();
-            continue;
+                d1 += double(acc) * double(acc);
         }
-        const int i = r0 + tid;
-        float wi = 0.f;
-        if (i < r1)
-            wi = we[i];
-        int c[kLoadsPerThread];
-        float v[kLoadsPerThread];
-#pragma unroll
-        for (int j = 0; j < kLoadsPerThread; ++j) {
-            const int k = k0 + tid + j * kBlock;
-            c[j] = k < k1 ? __builtin_nontemporal_load(&a.ci[k]) : -1;
-            v[j] = k < k1 ? __builtin_nontemporal_load(&ve[k]) : 0.f;
-        }
-#pragma unroll
-        for (int j = 0; j < kLoadsPerThread; ++j)
-            if (c[j] >= 0)
-                prod[tid + j * kBlock] = v[j] * pe[c[j]];
-        __syncthreads();
-        double d0 = 0.0, d1 = 0.0;
-        if (i < r1) {
-            float s = 0.f;
-            const int end = a.rp[i + 1] - k0;
-            for (int k = a.rp[i] - k0; k < end; ++k)
-                s += prod[k];
-            Ae[i] = s;
-            d0 = double(wi) * double(s);
-            d1 = double(s) * double(s);
-        }
-        S.add(0, e, d0);
-        if (OP == B_BI_T)
-            S.add(1, e, d1);
-        __syncthreads();
     }
-    S.store(a, OP == B_BI_T ? 2 : 1);
+    slot_partials(a, m, OP == B_BI_T ? 2 : 1, d0, d1);
 }
 
-// rt_e = b_e - A_e x_e in FP64 for the unfinished equations, r_e = float(rt_e);
-// partials of ||rt_e||^2.
-__global__ __launch_bounds__(kBlock) void k_btrue(BArgs a)
+// Vector stages over the slot layout (element q = i W + s; thread t of a
+// workgroup takes slot t % W of rows t / W, t / W + P, ...):
+//   B_INIT      p = D^-1 r; r.z, r.r
+//   B_UPDATE    xs += alpha p, r -= alpha Ap; r.r, r.z
+//   B_P         p = D^-1 r + beta p
+//   B_BI_INIT   rh = r; p = v = 0; r.r, rh.r
+//   B_BI_P      p = r + beta (p - omega v); y = D^-1 p
+//   B_BI_S      s = r - alpha v (in r); z = D^-1 s; s.s
+//   B_BI_UPDATE xs += alpha y + omega z; r = s - omega t; r.r, rh.r
+__global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
+{
+    const SlotMap m(a.W);
+    const int W = m.W;
+    const int e = m.live ? a.eq[m.sl] : -1;
+    const bool on = slot_on(a, op, e);
+    int64_t i0, i1;
+    block_range(a.n, m.P, i0, i1);
+    // this thread's elements: rows i0 + rr, i0 + rr + P, ... of its slot
+    const int64_t q0 = (i0 + m.rr) * W + m.sl, q1 = i1 * W;
+    const int64_t step = int64_t(m.P) * W;
+    double s0 = 0.0, s1 = 0.0;
+    if (on) {
+        const BEq& st = a.st[e];
+        if (op == B_INIT) {
+            for (int64_t q = q0; q < q1; q += step) {
+                const float ri = a.r[q], zi = a.invD[q] * ri;
+                a.p[q] = zi;
+                s0 += double(ri) * double(zi);
+                s1 += double(ri) * double(ri);
+            }
+        } else if (op == B_UPDATE) {
+            const float al = float(st.alpha);
+            for (int64_t q = q0; q < q1; q += step) {
+                a.xs[q] += al * a.p[q];
+                const float ri = a.r[q] - al * a.Ap[q];
+                a.r[q] = ri;
+                const float zi = a.invD[q] * ri;
+                s0 += double(ri) * double(ri);
+                s1 += double(ri) * double(zi);
+            }
+        } else if (op == B_P) {
+            const float be = float(st.beta);
+            for (int64_t q = q0; q < q1; q += step)
+                a.p[q] = a.invD[q] * a.r[q] + be * a.p[q];
+        } else if (op == B_BI_INIT) {
+            // rh = r; p = v = 0 (BiCGSTABSolver::solve, :366-372)
+            for (int64_t q = q0; q < q1; q += step) {
+                const float ri = a.r[q];
+                a.rh[q] = ri;
+                a.p[q] = 0.f;
+                a.Ap[q] = 0.f;
+                s0 += double(ri) * double(ri);
+            }
+            s1 = s0;
+        } else if (op == B_BI_P) {
+            // (:380-382)
+            const float be = float(st.beta), om = float(st.omega);
+            for (int64_t q = q0; q < q1; q += step) {
+                const float pi = a.r[q] + be * (a.p[q] - om * a.Ap[q]);
+                a.p[q] = pi;
+                a.y[q] = a.invD[q] * pi;
+            }
+        } else if (op == B_BI_S) {
+            // (:384-386)
+            const float al = float(st.alpha);
+            for (int64_t q = q0; q < q1; q += step) {
+                const float si = a.r[q] - al * a.Ap[q];
+                a.r[q] = si;
+                a.z[q] = a.invD[q] * si;
+                s0 += double(si) * double(si);
+            }
+        } else { // B_BI_UPDATE (:387-397)
+            const float al = float(st.alpha);
+            if (st.sconv) {
+                // s met the tolerance: x += alpha y and stop (:387-390)
+                for (int64_t q = q0; q < q1; q += step)
+                    a.xs[q] += al * a.y[q];
+            } else {
+                const float om = float(st.omega);
+                for (int64_t q = q0; q < q1; q += step) {
+                    a.xs[q] += al * a.y[q] + om * a.z[q];
+                    const float ri = a.r[q] - om * a.t[q];
+                    a.r[q] = ri;
+                    s0 += double(ri) * double(ri);
+                    s1 += double(a.rh[q]) * double(ri);
+                }
+            }
+        }
+    }
+    if (op == B_P || op == B_BI_P)
+        return;
+    slot_partials(a, m, (op == B_BI_S) ? 1 : 2, s0, s1);
+}
+
+// Equation-major stages.  Per-workgroup sums of one value per equation,
+// stored as partials[e][block].
+__device__ __forceinline__ void eq_partial(int e, double v, double* lds)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        lds[e * kWaves + threadIdx.x / kWave] += v;
+}
+
+__device__ __forceinline__ void eq_store(const HArgs& a, const double* lds)
+{
+    __syncthreads();
+    const int G = gridDim.x;
+    for (int e = threadIdx.x; e < a.E; e += kBlock) {
+        double s = 0.0;
+        for (int q = 0; q < kWaves; ++q)
+            s += lds[e * kWaves + q];
+        a.partials[size_t(e) * G + blockIdx.x] = s;
+    }
+}
+
+// ||b_e||^2 of every equation, over units (equation, chunk of rows).
+constexpr int kChunkRows = kBlock * 8;
+__global__ __launch_bounds__(kBlock) void k_bnorm(HArgs a)
+{
+    __shared__ double lds[kMaxEq * kWaves];
+    for (int t = threadIdx.x; t < a.E * kWaves; t += kBlock)
+        lds[t] = 0.0;
+    __syncthreads();
+    const int G = gridDim.x;
+    const int lb = xcd_logical_block(blockIdx.x, G);
+    const int nch = (a.n + kChunkRows - 1) / kChunkRows;
+    const int64_t units = int64_t(a.E) * nch;
+    const int64_t per = (units + G - 1) / G;
+    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
+    for (int64_t u = u0; u < u1; ++u) {
+        const int e = int(u / nch);
+        const int c0 = int(u % nch) * kChunkRows, c1 = min(a.n, c0 + kChunkRows);
+        const double* be = a.b + size_t(e) * a.n;
+        double s = 0.0;
+        for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
+            s += be[i] * be[i];
+        eq_partial(e, s, lds);
+    }
+    eq_store(a, lds);
+}
+
+// rt_e = b_e - A_e x_e in FP64 for the unfinished equations, over units
+// (equation, row block) of the pattern's CSR-stream row blocks; partials of
+// ||rt_e||^2.
+__global__ __launch_bounds__(kBlock) void k_btrue(HArgs a)
 {
     __shared__ double prod[kStreamNnz];
+    __shared__ double lds[kMaxEq * kWaves];
     __shared__ int act[kMaxEq];
-    __shared__ double wsum[2][kMaxEq][kWaves];
-    BSums S{wsum};
-    S.clear(a.E);
-    const int nact = b_list(a, B_TRUE, act);
+    __shared__ int nact;
+    for (int t = threadIdx.x; t < a.E * kWaves; t += kBlock)
+        lds[t] = 0.0;
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int e = 0; e < a.E; ++e)
+            if (!a.st[e].fin)
+                act[c++] = e;
+        nact = c;
+    }
+    __syncthreads();
     const int G = gridDim.x;
     const int lb = xcd_logical_block(blockIdx.x, G);
     const int64_t units = int64_t(nact) * a.nrb;
@@ -278,10 +433,9 @@ __global__ __launch_bounds__(kBlock) void k_btrue(BArgs a)
                     t += prod[j];
                 const double ri = a.b[off + r0] - t;
                 a.rt[off + r0] = ri;
-                a.r[off + r0] = float(ri);
                 rr = ri * ri;
             }
-            S.add(0, e, rr);
+            eq_partial(e, rr, lds);
             __syncthreads();
             continue;
         }
@@ -297,145 +451,24 @@ __global__ __launch_bounds__(kBlock) void k_btrue(BArgs a)
                 s += prod[k];
             const double ri = a.b[off + i] - s;
             a.rt[off + i] = ri;
-            a.r[off + i] = float(ri);
             rr = ri * ri;
         }
-        S.add(0, e, rr);
+        eq_partial(e, rr, lds);
         __syncthreads();
     }
-    S.store(a, 1);
-}
-
-// Vector stages over units (equation, chunk of kChunkRows rows):
-//   B_BNORM  partials ||b_e||^2 (all equations)
-//   B_INIT   p = D^-1 r; partials r.z, r.r
-//   B_UPDATE x += alpha p (FP64), r -= alpha Ap; partials r.r, r.z
-//   B_P      p = D^-1 r + beta p
-__global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
-{
-    __shared__ int act[kMaxEq];
-    __shared__ double wsum[2][kMaxEq][kWaves];
-    BSums S{wsum};
-    S.clear(a.E);
-    const int nact = b_list(a, op, act);
-    const int G = gridDim.x;
-    const int lb = xcd_logical_block(blockIdx.x, G);
-    const int nch = (a.n + kChunkRows - 1) / kChunkRows;
-    const int64_t units = int64_t(nact) * nch;
-    const int64_t per = (units + G - 1) / G;
-    const int64_t u0 = lb * per, u1 = min(units, u0 + per);
-    for (int64_t u = u0; u < u1; ++u) {
-        const int e = act[u / nch];
-        const int c0 = int(u % nch) * kChunkRows;
-        const int c1 = min(a.n, c0 + kChunkRows);
-        const size_t off = size_t(e) * a.n;
-        double s0 = 0.0, s1 = 0.0;
-        if (op == B_BNORM) {
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
-                s0 += a.b[off + i] * a.b[off + i];
-        } else if (op == B_INIT) {
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                const float ri = a.r[off + i], zi = a.invD[off + i] * ri;
-                a.p[off + i] = zi;
-                s0 += double(ri) * double(zi);
-                s1 += double(ri) * double(ri);
-            }
-        } else if (op == B_UPDATE) {
-            const double al = a.st[e].alpha;
-            const float alf = float(al);
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                a.x[off + i] += al * double(a.p[off + i]);
-                const float ri = a.r[off + i] - alf * a.Ap[off + i];
-                a.r[off + i] = ri;
-                const float zi = a.invD[off + i] * ri;
-                s0 += double(ri) * double(ri);
-                s1 += double(ri) * double(zi);
-            }
-        } else if (op == B_P) {
-            const float be = float(a.st[e].beta);
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
-                a.p[off + i] = a.invD[off + i] * a.r[off + i] + be * a.p[off + i];
-            continue;
-        } else if (op == B_BI_INIT) {
-            // rh = r; p = v = 0; partials r.r, rh.r (BiCGSTABSolver::solve, :366-372)
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                const float ri = a.r[off + i];
-                a.rh[off + i] = ri;
-                a.p[off + i] = 0.f;
-                a.Ap[off + i] = 0.f;
-                s0 += double(ri) * double(ri);
-            }
-            s1 = s0;
-        } else if (op == B_BI_P) {
-            // p = r + beta (p - omega v); y = D^-1 p (:380-382)
-            const float be = float(a.st[e].beta), om = float(a.st[e].omega);
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                const float pi = a.r[off + i] + be * (a.p[off + i] - om * a.Ap[off + i]);
-                a.p[off + i] = pi;
-                a.y[off + i] = a.invD[off + i] * pi;
-            }
-            continue;
-        } else if (op == B_BI_S) {
-            // s = r - alpha v (in r); z = D^-1 s; partials s.s (:384-386)
-            const float al = float(a.st[e].alpha);
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                const float si = a.r[off + i] - al * a.Ap[off + i];
-                a.r[off + i] = si;
-                a.z[off + i] = a.invD[off + i] * si;
-                s0 += double(si) * double(si);
-            }
-        } else { // B_BI_UPDATE: x += alpha y + omega z; r = s - omega t (:393-397)
-            const BEq& q = a.st[e];
-            const double al = q.alpha;
-            if (q.sconv) {
-                // s met the tolerance: x += alpha y and stop (:387-390)
-                for (int i = c0 + threadIdx.x; i < c1; i += kBlock)
-                    a.x[off + i] += al * double(a.y[off + i]);
-                continue;
-            }
-            const double om = q.omega;
-            const float omf = float(om);
-            for (int i = c0 + threadIdx.x; i < c1; i += kBlock) {
-                a.x[off + i] += al * double(a.y[off + i]) + om * double(a.z[off + i]);
-                const float ri = a.r[off + i] - omf * a.t[off + i];
-                a.r[off + i] = ri;
-                s0 += double(ri) * double(ri);
-                s1 += double(a.rh[off + i]) * double(ri);
-            }
-        }
-        S.add(0, e, s0);
-        if (op != B_BNORM && op != B_BI_S)
-            S.add(1, e, s1);
-    }
-    if (op != B_P && op != B_BI_P)
-        S.store(a, (op == B_BNORM || op == B_BI_S) ? 1 : 2);
+    eq_store(a, lds);
 }
 
 struct BFinArgs
 {
-    int E, G, max_iter, max_outer;
+    int E, W, G, max_iter, max_outer;
     double tol, inner_tol;
 };
 
-// One workgroup per equation: fold the G partials (fixed order) and update
-// the equation's scalars.
-__global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ partials, BEq* st,
-                                                 int op, BFinArgs f)
+// Updates one equation's scalars from the totals of a stage (the
+// finalizations of CGSolver::solve and BiCGSTABSolver::solve).
+__device__ void b_finalize(BEq& s, int op, const double (&tot)[2], const BFinArgs& f)
 {
-    const int e = blockIdx.x;
-    const int nv = (op == B_INIT || op == B_UPDATE || op == B_BI_INIT || op == B_BI_OMEGA ||
-                    op == B_BI_UPDATE)
-                       ? 2
-                       : 1;
-    double tot[2] = {0.0, 0.0};
-    for (int v = 0; v < nv; ++v)
-        for (int g = threadIdx.x; g < f.G; g += kBlock)
-            tot[v] += partials[(size_t(v) * f.E + e) * f.G + g];
-    __shared__ double lds[2 * kWaves];
-    block_sum<2>(tot, lds);
-    if (threadIdx.x != 0)
-        return;
-    BEq& s = st[e];
     switch (op) {
     case B_BNORM:
         s.bnorm = fmax(sqrt(tot[0]), 1e-30);
@@ -445,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
         s.outer = 0;
         s.status = 0;
         s.rel = 1.0;
-        if (tot[0] == 0.0) { // zero rhs: x = 0 (the caller zeroed x)
+        if (tot[0] == 0.0) { // zero rhs: x = 0 (x is zeroed before the solve)
             s.fin = 1;
             s.rel = 0.0;
         }
@@ -458,7 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
         if (s.rel < f.tol) {
             s.fin = 1;
             s.done = 1;
-        } else if (s.outer >= f.max_outer || s.total >= f.max_iter) {
+        } else if (s.outer >= f.max_outer || s.total >= f.max_iter || !isfinite(rn)) {
             s.fin = 1; // not converged: status marks it
             s.done = 1;
             s.status = 1;
@@ -559,7 +592,105 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
     }
 }
 
-// d[k] = src[map[k]] (map null: identity) in FP64 and FP32
+// One workgroup per slot: fold the slot's G partials (fixed order) and
+// update its equation's scalars.
+__global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ partials,
+                                                 const int* __restrict__ eq, BEq* st, int op,
+                                                 BFinArgs f)
+{
+    const int sl = blockIdx.x;
+    const int e = eq[sl];
+    if (e < 0)
+        return;
+    const int nv = (op == B_INIT || op == B_UPDATE || op == B_BI_INIT || op == B_BI_OMEGA ||
+                    op == B_BI_UPDATE)
+                       ? 2
+                       : 1;
+    double tot[2] = {0.0, 0.0};
+    for (int v = 0; v < nv; ++v)
+        for (int g = threadIdx.x; g < f.G; g += kBlock)
+            tot[v] += partials[(size_t(v) * f.W + sl) * f.G + g];
+    __shared__ double lds[2 * kWaves];
+    block_sum<2>(tot, lds);
+    if (threadIdx.x == 0)
+        b_finalize(st[e], op, tot, f);
+}
+
+// One workgroup per equation for the equation-major stages.
+__global__ __launch_bounds__(kBlock) void k_bfin_eq(const double* __restrict__ partials, BEq* st,
+                                                    int op, BFinArgs f)
+{
+    const int e = blockIdx.x;
+    double tot[2] = {0.0, 0.0};
+    for (int g = threadIdx.x; g < f.G; g += kBlock)
+        tot[0] += partials[size_t(e) * f.G + g];
+    __shared__ double lds[2 * kWaves];
+    block_sum<2>(tot, lds);
+    if (threadIdx.x == 0)
+        b_finalize(st[e], op, tot, f);
+}
+
+// Home (equation-major, rows of length `rows`) <-> slot layout, through a
+// tile of 64 rows x W slots in LDS so both sides are accessed in lines:
+//   mode 0: slot[i W + s] = float(home[eq[s] * rows + i]), or `fill` for an
+//           empty slot (for every slot when use_fill)
+//   mode 1: home[eq[s] * rows + i] += H(slot[i W + s]) for the slots set in
+//           `mask`
+constexpr int kTileRows = 64;
+template <class H>
+__global__ __launch_bounds__(kBlock) void k_btile(int64_t rows, int W, const int* __restrict__ eq,
+                                                  uint64_t mask, int mode, int use_fill,
+                                                  float fill, H* home, float* slot)
+{
+    __shared__ float tile[kTileRows * kMaxEq];
+    const int64_t ntile = (rows + kTileRows - 1) / kTileRows;
+    for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
+        const int64_t i0 = tb * kTileRows;
+        const int nr = int(min(int64_t(kTileRows), rows - i0));
+        if (mode == 0) {
+            // read each slot's run of rows, write whole lines
+            for (int t = threadIdx.x; t < kTileRows * W; t += kBlock) {
+                const int sl = t / kTileRows, r = t % kTileRows;
+                const int e = eq[sl];
+                float v = fill;
+                if (r < nr && e >= 0 && !use_fill)
+                    v = float(home[size_t(e) * rows + i0 + r]);
+                tile[r * W + sl] = v;
+            }
+            __syncthreads();
+            for (int t = threadIdx.x; t < nr * W; t += kBlock)
+                slot[size_t(i0) * W + t] = tile[t];
+        } else {
+            for (int t = threadIdx.x; t < nr * W; t += kBlock)
+                tile[t] = slot[size_t(i0) * W + t];
+            __syncthreads();
+            for (int t = threadIdx.x; t < kTileRows * W; t += kBlock) {
+                const int sl = t / kTileRows, r = t % kTileRows;
+                const int e = eq[sl];
+                if (r < nr && e >= 0 && ((mask >> sl) & 1ull))
+                    home[size_t(e) * rows + i0 + r] += H(tile[r * W + sl]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Narrower slot layout from a wider one: dst[i W2 + s] = src[i W1 + map[s]].
+__global__ __launch_bounds__(kBlock) void k_brepack(int64_t rows, int W1, int W2,
+                                                    const int* __restrict__ map,
+                                                    const float* __restrict__ src,
+                                                    float* __restrict__ dst)
+{
+    const int64_t m = rows * W2;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
+        const int64_t i = q / W2;
+        const int sl = int(q - i * W2);
+        dst[q] = src[size_t(i) * W1 + map[sl]];
+    }
+}
+
+// d[k] = src[map[k]] (map null: identity) in FP64 and, unless d32 is null, FP32
 __global__ __launch_bounds__(kBlock) void k_bconvert(int64_t m, const int* __restrict__ map,
                                                      const double* __restrict__ src,
                                                      double* __restrict__ d64,
@@ -569,7 +700,8 @@ __global__ __launch_bounds__(kBlock) void k_bconvert(int64_t m, const int* __res
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
         const double v = src[map ? map[i] : i];
         d64[i] = v;
-        d32[i] = float(v);
+        if (d32)
+            d32[i] = float(v);
     }
 }
 
@@ -588,6 +720,18 @@ __global__ __launch_bounds__(kBlock) void k_bpermute(int64_t n, int E, const int
             dst[t] = src[j];
         else
             dst[j] = src[t];
+    }
+}
+
+// FP32 values of one equation in SELL-64 slot order (0 in padding slots).
+__global__ __launch_bounds__(kBlock) void k_bsell32(int64_t m, const int* __restrict__ eperm,
+                                                    const double* __restrict__ v64,
+                                                    float* __restrict__ v32)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
+        const int k = eperm[i];
+        v32[i] = k >= 0 ? float(v64[k]) : 0.f;
     }
 }
 
@@ -611,9 +755,25 @@ __global__ __launch_bounds__(kBlock) void k_bfill(int64_t m, double* __restrict_
         x[i] = 0.0;
 }
 
+__global__ __launch_bounds__(kBlock) void k_bzero32(int64_t m, float* __restrict__ x)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride)
+        x[i] = 0.f;
+}
+
 } // namespace mi
 
 using namespace mi;
+
+// One slot layout: its width, slot table and the arrays of the working set.
+struct BSlots
+{
+    int W = 0;
+    std::vector<int> eqs; // equation of each slot, -1 empty (size W)
+    DevBuf<int> d_eq;
+    DevBuf<float> v32, invD, xs, r, p, Ap, rh, y, z, t;
+};
 
 struct mi_batch
 {
@@ -621,18 +781,26 @@ struct mi_batch
     std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM)
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
-    int E = 0, G = 8;
+    int E = 0, G = 8, Wmax = 1;
     int64_t n = 0, nnz = 0;
-    DevBuf<float> v32, invD, r, p, Ap;
-    DevBuf<float> rh, y, z, t; // BiCGStab vectors, allocated on its first solve
-    int graph_kind = -1;
-    DevBuf<double> v64, b, x, rt, partials;
+    bool sell = false;   // values in the pattern's SELL-64 slot order
+    int64_t vrows = 0;   // value lines: nnz, or the SELL-64 slots
+    // equation-major ("home") arrays
+    DevBuf<double> v64, b, x, rt;
+    DevBuf<float> v32h, invDh;
+    DevBuf<double> partials;
     DevBuf<BEq> st;
     BEq* h_st = nullptr;
+    // two slot layouts (packing and repacking write the other one)
+    BSlots set[2];
+    int cur = 0;
+    bool slots_valid = false; // set[cur] holds the home values of its equations
+    bool pc_none = false;     // invD of the packed slots is 1
+    bool bicg_alloc = false;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
-    int graph_chunk = 0;
-    BFinArgs graph_f{}; // finalize arguments baked into the captured graph
+    int graph_chunk = 0, graph_kind = -1, graph_set = -1, graph_W = 0;
+    BFinArgs graph_f{}; // finalize arguments baked into the graph
     double tune_us = 0.0;
 
     ~mi_batch()
@@ -644,9 +812,35 @@ struct mi_batch
         if (stream)
             (void)hipStreamDestroy(stream);
     }
-    BArgs args() const
+    BArgs args(int which) const
     {
+        const BSlots& S = set[which];
         BArgs a{};
+        a.n = int(n);
+        a.W = S.W;
+        a.nnz = nnz;
+        a.rp = pat->rp.p;
+        a.ci = pat->ci.p;
+        a.soff = sell ? pat->sell.soff.p : nullptr;
+        a.eci = sell ? pat->sell.eci.p : nullptr;
+        a.v32 = S.v32.p;
+        a.invD = S.invD.p;
+        a.eq = S.d_eq.p;
+        a.xs = S.xs.p;
+        a.r = S.r.p;
+        a.p = S.p.p;
+        a.Ap = S.Ap.p;
+        a.rh = S.rh.p;
+        a.y = S.y.p;
+        a.z = S.z.p;
+        a.t = S.t.p;
+        a.st = st.p;
+        a.partials = partials.p;
+        return a;
+    }
+    HArgs hargs() const
+    {
+        HArgs a{};
         a.n = int(n);
         a.E = E;
         a.nrb = pat->rb.nrb;
@@ -654,19 +848,10 @@ struct mi_batch
         a.blk = pat->rb.blk.p;
         a.rp = pat->rp.p;
         a.ci = pat->ci.p;
-        a.v32 = v32.p;
         a.v64 = v64.p;
-        a.invD = invD.p;
         a.b = b.p;
         a.x = x.p;
         a.rt = rt.p;
-        a.r = r.p;
-        a.p = p.p;
-        a.Ap = Ap.p;
-        a.rh = rh.p;
-        a.y = y.p;
-        a.z = z.p;
-        a.t = t.p;
         a.st = st.p;
         a.partials = partials.p;
         return a;
@@ -676,24 +861,126 @@ struct mi_batch
 namespace
 {
 
-constexpr int kBatchMaxGrid = 4096;
+// Working-set arrays of a slot layout (sized for the widest layout;
+// BiCGStab's extra vectors when first needed).
+int alloc_slots(mi_batch* B, BSlots& S, bool bicg)
+{
+    const size_t nW = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Wmax);
+    const size_t zW = size_t(std::max<int64_t>(B->vrows, 1)) * size_t(B->Wmax);
+    if (!S.v32.p) {
+        MI_TRY(S.v32.alloc(zW));
+        for (DevBuf<float>* v : {&S.invD, &S.xs, &S.r, &S.p, &S.Ap})
+            MI_TRY(v->alloc(nW));
+        MI_TRY(S.d_eq.alloc(size_t(kMaxEq)));
+    }
+    if (bicg && !S.rh.p)
+        for (DevBuf<float>* v : {&S.rh, &S.y, &S.z, &S.t})
+            MI_TRY(v->alloc(nW));
+    return MI_OK;
+}
+
+int upload_eqs(mi_batch* B, BSlots& S)
+{
+    MI_HIP(hipMemcpyAsync(S.d_eq.p, S.eqs.data(), sizeof(int) * size_t(S.W),
+                          hipMemcpyHostToDevice, B->stream));
+    MI_HIP(hipStreamSynchronize(B->stream)); // S.eqs is host memory that may change
+    return MI_OK;
+}
+
+unsigned tile_grid(int64_t rows)
+{
+    const int64_t t = (rows + kTileRows - 1) / kTileRows;
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>(t, 8192)));
+}
+
+// Slots for `eqs` in the other layout, values and inverse diagonals from the
+// home arrays; it becomes the current layout.
+int pack_from_home(mi_batch* B, const std::vector<int>& eqs, bool bicg)
+{
+    const int nxt = B->cur ^ 1;
+    BSlots& S = B->set[nxt];
+    MI_TRY(alloc_slots(B, S, bicg));
+    S.W = int(eqs.size());
+    S.eqs = eqs;
+    MI_TRY(upload_eqs(B, S));
+    hipStream_t s = B->stream;
+    if (B->vrows)
+        k_btile<float><<<tile_grid(B->vrows), kBlock, 0, s>>>(B->vrows, S.W, S.d_eq.p, 0ull, 0, 0,
+                                                              0.f, B->v32h.p, S.v32.p);
+    if (B->n)
+        k_btile<float><<<tile_grid(B->n), kBlock, 0, s>>>(B->n, S.W, S.d_eq.p, 0ull, 0,
+                                                          B->pc_none ? 1 : 0, 1.f, B->invDh.p,
+                                                          S.invD.p);
+    MI_HIP(hipGetLastError());
+    B->cur = nxt;
+    B->slots_valid = true;
+    return MI_OK;
+}
+
+// x_e += the current sweep's correction for the slots in `mask`.
+int retire(mi_batch* B, uint64_t mask)
+{
+    const BSlots& S = B->set[B->cur];
+    if (!mask || B->n == 0)
+        return MI_OK;
+    k_btile<double><<<tile_grid(B->n), kBlock, 0, B->stream>>>(B->n, S.W, S.d_eq.p, mask, 1, 0,
+                                                               0.f, B->x.p, S.xs.p);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+// The slots `keep` (positions in the current layout) of every working array
+// into the other layout, keep.size() wide.
+int repack(mi_batch* B, const std::vector<int>& keep, bool bicg)
+{
+    const BSlots& A = B->set[B->cur];
+    BSlots& S = B->set[B->cur ^ 1];
+    MI_TRY(alloc_slots(B, S, bicg));
+    S.W = int(keep.size());
+    S.eqs.assign(size_t(S.W), -1);
+    std::vector<int> map(size_t(S.W), 0);
+    for (size_t q = 0; q < keep.size(); ++q) {
+        S.eqs[q] = A.eqs[size_t(keep[q])];
+        map[q] = keep[q];
+    }
+    MI_TRY(upload_eqs(B, S));
+    DevBuf<int> d_map;
+    MI_TRY(d_map.upload(map.data(), map.size(), B->stream));
+    hipStream_t s = B->stream;
+    auto cp = [&](int64_t rows, const DevBuf<float>& from, DevBuf<float>& to) {
+        if (rows > 0)
+            k_brepack<<<reduce_grid(rows * S.W, 4), kBlock, 0, s>>>(rows, A.W, S.W, d_map.p,
+                                                                    from.p, to.p);
+    };
+    cp(B->vrows, A.v32, S.v32);
+    cp(B->n, A.invD, S.invD);
+    cp(B->n, A.xs, S.xs);
+    cp(B->n, A.r, S.r);
+    cp(B->n, A.p, S.p);
+    cp(B->n, A.Ap, S.Ap);
+    if (bicg) {
+        cp(B->n, A.rh, S.rh);
+        cp(B->n, A.y, S.y);
+        cp(B->n, A.z, S.z);
+        cp(B->n, A.t, S.t);
+    }
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(s)); // d_map is released on return
+    B->cur ^= 1;
+    return MI_OK;
+}
 
 int fin(mi_batch* B, int op, const BFinArgs& f, Launcher& L)
 {
-    return L(k_bfin, unsigned(B->E), kBlock, static_cast<const double*>(B->partials.p), B->st.p,
-             op, f);
-}
-
-int fin(mi_batch* B, int op, const BFinArgs& f)
-{
-    Launcher L{B->stream};
-    return fin(B, op, f, L);
+    const BSlots& S = B->set[B->cur];
+    return L(k_bfin, unsigned(S.W), kBlock, static_cast<const double*>(B->partials.p),
+             static_cast<const int*>(S.d_eq.p), B->st.p, op, f);
 }
 
 // One CG iteration for every iterating equation (five launches).
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(k_bspmv<B_ALPHA>, B->G, kBlock, a));
+    MI_TRY(L(B->sell ? k_bspmm<B_ALPHA, true> : k_bspmm<B_ALPHA, false>, B->G, kBlock, a));
     MI_TRY(fin(B, B_ALPHA, f, L));
     MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_UPDATE)));
     MI_TRY(fin(B, B_UPDATE, f, L));
@@ -704,38 +991,43 @@ int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
     MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_P)));
-    MI_TRY(L(k_bspmv<B_BI_ALPHA>, B->G, kBlock, a));
+    MI_TRY(L(B->sell ? k_bspmm<B_BI_ALPHA, true> : k_bspmm<B_BI_ALPHA, false>, B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_ALPHA, f, L));
     MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_S)));
     MI_TRY(fin(B, B_BI_S, f, L));
-    MI_TRY(L(k_bspmv<B_BI_T>, B->G, kBlock, a));
+    MI_TRY(L(B->sell ? k_bspmm<B_BI_T, true> : k_bspmm<B_BI_T, false>, B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_OMEGA, f, L));
     MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_UPDATE)));
     return fin(B, B_BI_UPDATE, f, L);
 }
 
-// Autotuner: workgroups of the persistent batched launches, timed on this
-// pattern with every equation active.
+// Autotuner: workgroups of the batched launches, timed on this pattern in
+// the widest layout with every equation's slot active.
 int autotune(mi_batch* B)
 {
     B->G = 8;
     if (B->n == 0)
         return MI_OK;
+    std::vector<int> all(size_t(B->E));
+    for (int e = 0; e < B->E; ++e)
+        all[size_t(e)] = e;
+    MI_TRY(pack_from_home(B, all, false));
     hipEvent_t e0, e1;
     MI_HIP(hipEventCreate(&e0));
     MI_HIP(hipEventCreate(&e1));
     double best = 1e300;
-    BArgs a = B->args();
-    a.st = nullptr; // all equations active
-    for (int g : {512, 1024, 2048, 4096}) {
+    BArgs a = B->args(B->cur);
+    a.st = nullptr;
+    for (int g : {256, 512, 1024, 2048, 4096}) {
         if (g > kBatchMaxGrid)
             break;
-        k_bspmv<B_ALPHA><<<g, kBlock, 0, B->stream>>>(a);
-        MI_HIP(hipGetLastError());
+        Launcher L{B->stream};
+        auto k = B->sell ? k_bspmm<B_ALPHA, true> : k_bspmm<B_ALPHA, false>;
+        MI_TRY(L(k, unsigned(g), kBlock, a));
         MI_HIP(hipEventRecord(e0, B->stream));
         const int reps = 3;
         for (int r = 0; r < reps; ++r)
-            k_bspmv<B_ALPHA><<<g, kBlock, 0, B->stream>>>(a);
+            MI_TRY(L(k, unsigned(g), kBlock, a));
         MI_HIP(hipEventRecord(e1, B->stream));
         MI_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -749,6 +1041,7 @@ int autotune(mi_batch* B)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     B->tune_us = best;
+    B->slots_valid = false; // packed from values not yet set
     return MI_OK;
 }
 
@@ -761,6 +1054,14 @@ int check_cfg(const mi_solver_cfg* c)
         return set_error(MI_ERR_UNSUPPORTED, "batched solve: Jacobi preconditioner only");
     if (!(c->tolerance > 0.0) || c->max_iterations < 0)
         return set_error(MI_ERR_BAD_ARG, "bad tolerance or iteration limit");
+    return MI_OK;
+}
+
+int read_state(mi_batch* B)
+{
+    MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(B->E), hipMemcpyDeviceToHost,
+                          B->stream));
+    MI_HIP(hipStreamSynchronize(B->stream));
     return MI_OK;
 }
 
@@ -779,6 +1080,17 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         }
         return MI_OK;
     }
+    const bool bicg = c.kind == MI_SOLVER_BICGSTAB;
+    const bool pc_none = c.preconditioner == MI_PC_NONE;
+    if (pc_none != B->pc_none) {
+        B->pc_none = pc_none;
+        B->slots_valid = false;
+    }
+    if (bicg && !B->bicg_alloc) {
+        MI_TRY(alloc_slots(B, B->set[0], true));
+        MI_TRY(alloc_slots(B, B->set[1], true));
+        B->bicg_alloc = true;
+    }
     BFinArgs f{};
     f.E = E;
     f.G = B->G;
@@ -795,66 +1107,97 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     }
     k_bfill<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(int64_t(m), B->x.p);
     MI_HIP(hipGetLastError());
-    const bool bicg = c.kind == MI_SOLVER_BICGSTAB;
-    if (bicg && !B->rh.p) {
-        const size_t nE = size_t(n) * size_t(E);
-        for (DevBuf<float>* v : {&B->rh, &B->y, &B->z, &B->t})
-            MI_TRY(v->alloc(nE));
-    }
-    const BArgs a = B->args();
-    k_bvec<<<B->G, kBlock, 0, s>>>(a, B_BNORM);
+    const HArgs h = B->hargs();
+    k_bnorm<<<B->G, kBlock, 0, s>>>(h);
+    k_bfin_eq<<<E, kBlock, 0, s>>>(B->partials.p, B->st.p, B_BNORM, f);
     MI_HIP(hipGetLastError());
-    MI_TRY(fin(B, B_BNORM, f));
     const int chunk = c.check_interval > 0 ? c.check_interval : 8;
     const bool use_graph = c.use_graph != 0 && graphs_enabled();
-    auto iteration = [&](Launcher& L) {
-        return bicg ? one_iteration_bicg(B, a, f, L) : one_iteration(B, a, f, L);
-    };
-    if (use_graph && (!B->graph || B->graph_chunk != chunk || B->graph_kind != c.kind ||
-                      std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0)) {
-        if (B->graph) {
-            (void)hipGraphExecDestroy(B->graph);
-            B->graph = nullptr;
-        }
-        MI_TRY(build_graph(s, [&](Launcher& L) -> int {
-            for (int it = 0; it < chunk; ++it)
-                MI_TRY(iteration(L));
-            return MI_OK;
-        }, &B->graph));
-        B->graph_chunk = chunk;
-        B->graph_kind = c.kind;
-        B->graph_f = f;
-    }
     for (;;) {
-        k_btrue<<<B->G, kBlock, 0, s>>>(a);
+        // FP64 true residual of the unfinished equations; which need a sweep
+        k_btrue<<<B->G, kBlock, 0, s>>>(h);
+        k_bfin_eq<<<E, kBlock, 0, s>>>(B->partials.p, B->st.p, B_TRUE, f);
         MI_HIP(hipGetLastError());
-        MI_TRY(fin(B, B_TRUE, f));
-        MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(E), hipMemcpyDeviceToHost, s));
-        MI_HIP(hipStreamSynchronize(s));
-        bool all_fin = true;
+        MI_TRY(read_state(B));
+        std::vector<int> want;
         for (int e = 0; e < E; ++e)
-            all_fin = all_fin && B->h_st[e].fin;
-        if (all_fin)
+            if (!B->h_st[e].fin)
+                want.push_back(e);
+        if (want.empty())
             break;
-        k_bvec<<<B->G, kBlock, 0, s>>>(a, bicg ? B_BI_INIT : B_INIT);
-        MI_HIP(hipGetLastError());
-        MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f));
+        {
+            std::vector<int> have(B->set[B->cur].eqs);
+            have.erase(std::remove(have.begin(), have.end(), -1), have.end());
+            if (!B->slots_valid || have != want)
+                MI_TRY(pack_from_home(B, want, bicg));
+        }
+        // r = float(rt) and xs = 0 for this sweep, then the first stage
+        {
+            BSlots& S = B->set[B->cur];
+            k_btile<double><<<tile_grid(n), kBlock, 0, s>>>(n, S.W, S.d_eq.p, 0ull, 0, 0, 0.f,
+                                                            B->rt.p, S.r.p);
+            k_bzero32<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(n * S.W, S.xs.p);
+            MI_HIP(hipGetLastError());
+            Launcher L{s};
+            f.W = S.W;
+            MI_TRY(L(k_bvec, B->G, kBlock, B->args(B->cur), int(bicg ? B_BI_INIT : B_INIT)));
+            MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f, L));
+        }
         for (;;) {
+            const BArgs a = B->args(B->cur);
+            f.W = a.W;
+            auto iteration = [&](Launcher& L) {
+                return bicg ? one_iteration_bicg(B, a, f, L) : one_iteration(B, a, f, L);
+            };
             if (use_graph) {
+                if (!B->graph || B->graph_chunk != chunk || B->graph_kind != c.kind ||
+                    B->graph_set != B->cur || B->graph_W != a.W ||
+                    std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0) {
+                    if (B->graph) {
+                        (void)hipGraphExecDestroy(B->graph);
+                        B->graph = nullptr;
+                    }
+                    MI_TRY(build_graph(s, [&](Launcher& L) -> int {
+                        for (int it = 0; it < chunk; ++it)
+                            MI_TRY(iteration(L));
+                        return MI_OK;
+                    }, &B->graph));
+                    B->graph_chunk = chunk;
+                    B->graph_kind = c.kind;
+                    B->graph_set = B->cur;
+                    B->graph_W = a.W;
+                    B->graph_f = f;
+                }
                 MI_HIP(hipGraphLaunch(B->graph, s));
             } else {
                 Launcher L{s};
                 for (int it = 0; it < chunk; ++it)
                     MI_TRY(iteration(L));
             }
-            MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(E), hipMemcpyDeviceToHost,
-                                  s));
-            MI_HIP(hipStreamSynchronize(s));
-            bool all_done = true;
-            for (int e = 0; e < E; ++e)
-                all_done = all_done && B->h_st[e].done;
-            if (all_done)
+            MI_TRY(read_state(B));
+            const BSlots& S = B->set[B->cur];
+            std::vector<int> live;
+            uint64_t idle = 0;
+            for (int q = 0; q < S.W; ++q) {
+                const int e = S.eqs[size_t(q)];
+                if (e < 0)
+                    continue;
+                if (B->h_st[e].done)
+                    idle |= 1ull << q;
+                else
+                    live.push_back(q);
+            }
+            if (live.empty()) {
+                MI_TRY(retire(B, idle));
                 break;
+            }
+            if (int(live.size()) < S.W) {
+                // idle slots: their corrections go home and the live slots
+                // move to a layout as wide as their number
+                MI_TRY(retire(B, idle));
+                MI_TRY(repack(B, live, bicg));
+                B->slots_valid = false; // no longer the home values of `want`
+            }
         }
     }
     int status = MI_OK;
@@ -891,6 +1234,7 @@ MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
     B->src = A->pat;
     B->pat = A->pat;
     B->E = n_eq;
+    B->Wmax = n_eq;
     B->n = A->n();
     B->nnz = A->nnz();
     MI_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
@@ -907,19 +1251,29 @@ MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
         B->reordered = true;
     }
     MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_st), sizeof(BEq) * kMaxEq, 0));
+    B->sell = B->pat->sell.built() && B->pat->sell.eperm.p != nullptr;
+    B->vrows = B->sell ? B->pat->sell.padded : B->nnz;
     const size_t nE = size_t(std::max<int64_t>(B->n, 1)) * size_t(n_eq);
     const size_t zE = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(n_eq);
-    MI_TRY(B->v32.alloc(zE));
+    const size_t vE = size_t(std::max<int64_t>(B->vrows, 1)) * size_t(n_eq);
     MI_TRY(B->v64.alloc(zE));
-    for (DevBuf<float>* v : {&B->invD, &B->r, &B->p, &B->Ap})
-        MI_TRY(v->alloc(nE));
+    MI_TRY(B->v32h.alloc(vE));
+    MI_TRY(B->invDh.alloc(nE));
     for (DevBuf<double>* v : {&B->b, &B->x, &B->rt})
         MI_TRY(v->alloc(nE));
-    MI_TRY(B->partials.alloc(size_t(2) * kBatchMaxGrid * size_t(n_eq)));
+    MI_TRY(B->partials.alloc(size_t(2) * kBatchMaxGrid * size_t(kMaxEq)));
     MI_TRY(B->st.alloc(size_t(n_eq)));
-    MI_HIP(hipMemsetAsync(B->v32.p, 0, zE * sizeof(float), B->stream));
-    MI_HIP(hipMemsetAsync(B->p.p, 0, nE * sizeof(float), B->stream));
+    MI_HIP(hipMemsetAsync(B->v32h.p, 0, vE * sizeof(float), B->stream));
+    MI_HIP(hipMemsetAsync(B->v64.p, 0, zE * sizeof(double), B->stream));
+    MI_HIP(hipMemsetAsync(B->invDh.p, 0, nE * sizeof(float), B->stream));
     MI_HIP(hipMemsetAsync(B->st.p, 0, sizeof(BEq) * size_t(n_eq), B->stream));
+    MI_HIP(hipStreamSynchronize(B->stream));
+    MI_TRY(alloc_slots(B.get(), B->set[0], false));
+    MI_TRY(alloc_slots(B.get(), B->set[1], false));
+    for (BSlots& S : B->set) {
+        MI_HIP(hipMemsetAsync(S.p.p, 0, S.p.n * sizeof(float), B->stream));
+        MI_HIP(hipMemsetAsync(S.Ap.p, 0, S.Ap.n * sizeof(float), B->stream));
+    }
     MI_HIP(hipStreamSynchronize(B->stream));
     MI_TRY(autotune(B.get()));
     *out = B.release();
@@ -936,15 +1290,19 @@ MI_API int mi_batch_set_equation(mi_batch* B, int eq, const mi_csr* A)
     MI_TRY(ensure_device());
     hipStream_t s = B->stream;
     double* v64 = B->v64.p + size_t(eq) * size_t(B->nnz);
+    float* v32 = B->v32h.p + size_t(eq) * size_t(B->vrows);
     if (B->nnz)
         k_bconvert<<<reduce_grid(B->nnz, 4), kBlock, 0, s>>>(
-            B->nnz, B->reordered ? B->d_vmap.p : nullptr, A->val.p, v64,
-            B->v32.p + size_t(eq) * size_t(B->nnz));
+            B->nnz, B->reordered ? B->d_vmap.p : nullptr, A->val.p, v64, B->sell ? nullptr : v32);
+    if (B->sell && B->vrows)
+        k_bsell32<<<reduce_grid(B->vrows, 4), kBlock, 0, s>>>(B->vrows, B->pat->sell.eperm.p, v64,
+                                                              v32);
     if (B->n)
         k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
-            int(B->n), B->pat->diag.p, v64, B->invD.p + size_t(eq) * size_t(B->n));
+            int(B->n), B->pat->diag.p, v64, B->invDh.p + size_t(eq) * size_t(B->n));
     MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(s));
+    B->slots_valid = false;
     return MI_OK;
 }
 
@@ -1002,7 +1360,7 @@ MI_API int mi_batch_tuning(const mi_batch* B, int* rows_per_block, double* spmm_
 {
     if (!B)
         return set_error(MI_ERR_BAD_ARG, "null handle");
-    if (rows_per_block) // rows of one persistent workgroup
+    if (rows_per_block) // rows of one workgroup
         *rows_per_block = int((B->n + B->G - 1) / B->G);
     if (spmm_us)
         *spmm_us = B->tune_us;
