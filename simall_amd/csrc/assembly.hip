@@ -41,36 +41,29 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
     double diag = 0.0, b = 0.0;
     for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
         const int f = m.cf_face[k];
-        const int o = m.owner[f];
-        const int nb = m.nbr[f];
-        const double Ax = m.ax[f], Ay = m.ay[f], Az = m.az[f];
-        const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
+        const int oth = m.sl_oth[k];
+        const int tag = m.sl_tag[k];
+        const bool own = (tag & 16) != 0;
+        const double Amag2 = m.sl_s2[k];
+        const double ddotA = m.sl_ds[k];
         const double g = gamma ? gamma[f] : 1.0;
-        if (nb >= 0) {
-            const double dx = m.ccx[nb] - m.ccx[o];
-            const double dy = m.ccy[nb] - m.ccy[o];
-            const double dz = m.ccz[nb] - m.ccz[o];
-            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        if (oth >= 0) {
             double aPf = 1.0;
             if (aP) {
-                const double dO = fmax(aP[o], 1e-30);
-                const double dN = fmax(aP[nb], 1e-30);
+                const double dO = fmax(aP[own ? c : oth], 1e-30);
+                const double dN = fmax(aP[own ? oth : c], 1e-30);
                 aPf = 0.5 * (dO + dN);
             }
             const double coef = rho * g * Amag2 / fmax(aPf * ddotA, 1e-30);
             diag += coef;
             val[m.cf_slot[k]] -= coef;
-        } else if (m.fbc[f] == BC_PRESSURE) {
-            const double dx = m.fcx[f] - m.ccx[o];
-            const double dy = m.fcy[f] - m.ccy[o];
-            const double dz = m.fcz[f] - m.ccz[o];
-            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
-            const double dO = aP ? fmax(aP[o], 1e-30) : 1.0;
+        } else if ((tag & 15) - 1 == BC_PRESSURE) {
+            const double dO = aP ? fmax(aP[c], 1e-30) : 1.0;
             const double coef = rho * g * Amag2 / fmax(dO * ddotA, 1e-30);
             diag += coef;
         }
         if (phi) {
-            if (c == o)
+            if (own)
                 b -= phi[f];
             else
                 b += phi[f];
@@ -87,17 +80,52 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
     rhs[c] = b;
 }
 
+// Geometry of every cell-face slot (MeshDev::sl_s2 / sl_ds), once per mesh:
+// |S|^2 and |d . S| with the operations the assembly kernels would apply to
+// the face and cell arrays, so the kernels' arithmetic is unchanged.
+__global__ __launch_bounds__(kBlock) void k_slot_geom(MeshDev m, double* __restrict__ s2,
+                                                      double* __restrict__ ds)
+{
+    const int c = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= m.n_cells)
+        return;
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        const int f = m.cf_face[k];
+        const int o = m.owner[f];
+        const int nb = m.nbr[f];
+        const double Ax = m.ax[f], Ay = m.ay[f], Az = m.az[f];
+        double dx, dy, dz;
+        if (nb >= 0) {
+            dx = m.ccx[nb] - m.ccx[o];
+            dy = m.ccy[nb] - m.ccy[o];
+            dz = m.ccz[nb] - m.ccz[o];
+        } else {
+            dx = m.fcx[f] - m.ccx[o];
+            dy = m.fcy[f] - m.ccy[o];
+            dz = m.fcz[f] - m.ccz[o];
+        }
+        s2[k] = Ax * Ax + Ay * Ay + Az * Az;
+        ds[k] = fabs(dx * Ax + dy * Ay + dz * Az);
+    }
+}
+
 // dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for the pressure matrix of
 // k_assemble_pressure: an interior face adds k_f (e_o - e_n)(e_o - e_n)^T, a
 // pressure-boundary face k_f e_o e_o^T, with k_f = gamma_f * G_f; the pinned
 // cell's diagonal carries the factor 1e6.  The rhs does not depend on gamma.
 // (Gradient = dJ/dalpha - (dR/dalpha)^T psi: src/adjoint/DiscreteAdjoint.cpp:170-182,
 // with R = A(gamma) x - b.)
-__global__ __launch_bounds__(kBlock) void k_face_gradient(MeshDev m, double rho,
-                                                          const double* __restrict__ aP, int pin,
-                                                          const double* __restrict__ x,
-                                                          const double* __restrict__ lam,
-                                                          double* __restrict__ grad)
+//
+// The face loop streams per-face data only: owner and neighbour in a cell
+// numbering that follows the face list (fg_o, fg_n), and |S|^2, |d . S|
+// precomputed once per mesh (k_fg_geom) with the operations the coefficient
+// of k_assemble_pressure uses; x, lambda and aP are gathered into that
+// numbering once per call (k_fg_gather), so the cell reads of neighbouring
+// faces share cache lines even on a randomly numbered mesh.
+__global__ __launch_bounds__(kBlock) void k_fg_geom(MeshDev m, const int* __restrict__ inv,
+                                                    int* __restrict__ fo, int* __restrict__ fn,
+                                                    double* __restrict__ s2,
+                                                    double* __restrict__ ds)
 {
     const int f = int(blockIdx.x) * kBlock + threadIdx.x;
     if (f >= m.n_faces)
@@ -106,26 +134,75 @@ __global__ __launch_bounds__(kBlock) void k_face_gradient(MeshDev m, double rho,
     const int nb = m.nbr[f];
     const double Ax = m.ax[f], Ay = m.ay[f], Az = m.az[f];
     const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
-    const double so = (pin && o == 0) ? 1.0e6 : 1.0;
-    double g = 0.0;
+    fo[f] = inv[o];
+    double d = 0.0;
+    int tag = -2;
     if (nb >= 0) {
         const double dx = m.ccx[nb] - m.ccx[o];
         const double dy = m.ccy[nb] - m.ccy[o];
         const double dz = m.ccz[nb] - m.ccz[o];
-        const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
-        double aPf = 1.0;
-        if (aP)
-            aPf = 0.5 * (fmax(aP[o], 1e-30) + fmax(aP[nb], 1e-30));
-        const double G = rho * Amag2 / fmax(aPf * ddotA, 1e-30);
-        const double sn = (pin && nb == 0) ? 1.0e6 : 1.0;
-        const double lo = lam[o], ln = lam[nb], xo = x[o], xn = x[nb];
-        const double t = (lo - ln) * (xo - xn) + (so - 1.0) * lo * xo + (sn - 1.0) * ln * xn;
-        g = -G * t;
+        d = fabs(dx * Ax + dy * Ay + dz * Az);
+        tag = inv[nb];
     } else if (m.fbc[f] == BC_PRESSURE) {
         const double dx = m.fcx[f] - m.ccx[o];
         const double dy = m.fcy[f] - m.ccy[o];
         const double dz = m.fcz[f] - m.ccz[o];
-        const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        d = fabs(dx * Ax + dy * Ay + dz * Az);
+        tag = -1;
+    }
+    fn[f] = tag;
+    s2[f] = Amag2;
+    ds[f] = d;
+}
+
+// xr[i] = x[perm[i]], lr[i] = lam[perm[i]], and aP when given
+__global__ __launch_bounds__(kBlock) void k_fg_gather(int n, const int* __restrict__ perm,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ lam,
+                                                      const double* __restrict__ aP,
+                                                      double* __restrict__ xr,
+                                                      double* __restrict__ lr,
+                                                      double* __restrict__ ar)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const int c = perm[i];
+    xr[i] = x[c];
+    lr[i] = lam[c];
+    if (aP)
+        ar[i] = aP[c];
+}
+
+__global__ __launch_bounds__(kBlock) void k_face_gradient(int n_faces, const int* __restrict__ fo,
+                                                          const int* __restrict__ fn,
+                                                          const double* __restrict__ s2,
+                                                          const double* __restrict__ ds,
+                                                          double rho, const double* __restrict__ aP,
+                                                          int pin_cell,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ lam,
+                                                          double* __restrict__ grad)
+{
+    const int f = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (f >= n_faces)
+        return;
+    const int o = fo[f];
+    const int nb = fn[f];
+    const double Amag2 = s2[f];
+    const double ddotA = ds[f];
+    const double so = (o == pin_cell) ? 1.0e6 : 1.0;
+    double g = 0.0;
+    if (nb >= 0) {
+        double aPf = 1.0;
+        if (aP)
+            aPf = 0.5 * (fmax(aP[o], 1e-30) + fmax(aP[nb], 1e-30));
+        const double G = rho * Amag2 / fmax(aPf * ddotA, 1e-30);
+        const double sn = (nb == pin_cell) ? 1.0e6 : 1.0;
+        const double lo = lam[o], ln = lam[nb], xo = x[o], xn = x[nb];
+        const double t = (lo - ln) * (xo - xn) + (so - 1.0) * lo * xo + (sn - 1.0) * ln * xn;
+        g = -G * t;
+    } else if (nb == -1) {
         const double dO = aP ? fmax(aP[o], 1e-30) : 1.0;
         const double G = rho * Amag2 / fmax(dO * ddotA, 1e-30);
         g = -G * so * lam[o] * x[o];
@@ -153,20 +230,16 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
     double diag = 0.0, b = 0.0;
     for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
         const int f = m.cf_face[k];
-        const int o = m.owner[f];
-        const int nb = m.nbr[f];
-        const double Ax = m.ax[f], Ay = m.ay[f], Az = m.az[f];
-        const double Amag2 = Ax * Ax + Ay * Ay + Az * Az;
+        const int oth = m.sl_oth[k];
+        const int tag = m.sl_tag[k];
+        const double Amag2 = m.sl_s2[k];
+        const double ddotA = m.sl_ds[k];
         const double Ff = phi[f];
         const double ConvO = fmax(Ff, 0.0);
         const double ConvN = fmax(-Ff, 0.0);
-        if (nb >= 0) {
-            const double dx = m.ccx[nb] - m.ccx[o];
-            const double dy = m.ccy[nb] - m.ccy[o];
-            const double dz = m.ccz[nb] - m.ccz[o];
-            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+        if (oth >= 0) {
             const double D = mu * Amag2 / fmax(ddotA, 1e-30);
-            if (c == o) {
+            if (tag & 16) {
                 diag += ConvO + D;
                 val[m.cf_slot[k]] += -(ConvN + D);
             } else {
@@ -174,11 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
                 val[m.cf_slot[k]] += -(ConvO + D);
             }
         } else {
-            const int bc = m.fbc[f];
-            const double dx = m.fcx[f] - m.ccx[o];
-            const double dy = m.fcy[f] - m.ccy[o];
-            const double dz = m.fcz[f] - m.ccz[o];
-            const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
+            const int bc = (tag & 15) - 1;
             const double D = mu * Amag2 / fmax(ddotA, 1e-30);
             if (bc == BC_WALL || bc == BC_VELOCITY_INLET) {
                 const double Ub = (bc == BC_WALL) ? 0.0 : bcValue[m.fzone[f] * 3 + comp];
@@ -300,6 +369,46 @@ using namespace mi;
 namespace
 {
 
+// Face-gradient state of a mesh (see mi_mesh::fg_*), built once.
+int face_gradient_setup(mi_mesh* M, hipStream_t s)
+{
+    if (M->fg_o.p)
+        return MI_OK;
+    const int n = int(M->n_cells), F = int(M->n_faces);
+    std::vector<int> own(static_cast<size_t>(F)), nbr(static_cast<size_t>(F));
+    MI_HIP(hipMemcpyAsync(own.data(), M->owner.p, sizeof(int) * size_t(F), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipMemcpyAsync(nbr.data(), M->nbr.p, sizeof(int) * size_t(F), hipMemcpyDeviceToHost, s));
+    MI_HIP(hipStreamSynchronize(s));
+    // cells in the order the face list first meets them, then any it never meets
+    std::vector<int> inv(size_t(n), -1), perm;
+    perm.reserve(size_t(n));
+    auto see = [&](int c) {
+        if (c >= 0 && inv[size_t(c)] < 0) {
+            inv[size_t(c)] = int(perm.size());
+            perm.push_back(c);
+        }
+    };
+    for (int f = 0; f < F; ++f) {
+        see(own[size_t(f)]);
+        see(nbr[size_t(f)]);
+    }
+    for (int c = 0; c < n; ++c)
+        see(c);
+    DevBuf<int> d_inv;
+    MI_TRY(d_inv.upload(inv.data(), inv.size(), s));
+    MI_TRY(M->fg_perm.upload(perm.data(), perm.size(), s));
+    MI_TRY(M->fg_o.alloc(size_t(F)));
+    MI_TRY(M->fg_n.alloc(size_t(F)));
+    MI_TRY(M->fg_s2.alloc(size_t(F)));
+    MI_TRY(M->fg_ds.alloc(size_t(F)));
+    k_fg_geom<<<grid_for(F, kBlock), kBlock, 0, s>>>(M->dev(), d_inv.p, M->fg_o.p, M->fg_n.p,
+                                                     M->fg_s2.p, M->fg_ds.p);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(s));
+    M->fg_pin = n > 0 ? inv[0] : -1;
+    return MI_OK;
+}
+
 int bc_of(const mi_mesh* M, uint32_t zone)
 {
     // A zone with no entry behaves as the reference's find_bc == nullptr: the
@@ -367,7 +476,8 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
     // (src/solver/SimpleAlgorithm.cpp:63-83); per-cell faces sorted by id and
     // the CSR slot of the cell across each interior face.
     std::vector<int> rp(size_t(nC) + 1, 0), ci;
-    std::vector<int> cf_off(size_t(nC) + 1, 0), cf_face, cf_slot;
+    std::vector<int> cf_off(size_t(nC) + 1, 0), cf_face, cf_slot, sl_oth;
+    std::vector<int8_t> sl_tag;
     ci.reserve(size_t(nC) * 7);
     cf_face.reserve(size_t(nC) * 6);
     cf_slot.reserve(size_t(nC) * 6);
@@ -398,6 +508,8 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
             cf_face.push_back(f);
             const int o = own[size_t(f)], nb = nbr[size_t(f)];
             const int oth = (o == int(c)) ? nb : o;
+            sl_oth.push_back(oth);
+            sl_tag.push_back(int8_t(((o == int(c)) ? 16 : 0) + (int(fbc[size_t(f)]) + 1)));
             int slot = -1;
             if (oth >= 0)
                 slot = row0 + int(std::lower_bound(nb_list.begin(), nb_list.end(), oth) -
@@ -424,6 +536,14 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
     MI_TRY(up(M->cf_off, cf_off.data(), nC + 1, s));
     MI_TRY(up(M->cf_face, cf_face.data(), int64_t(cf_face.size()), s));
     MI_TRY(up(M->cf_slot, cf_slot.data(), int64_t(cf_slot.size()), s));
+    MI_TRY(up(M->sl_oth, sl_oth.data(), int64_t(sl_oth.size()), s));
+    MI_TRY(up(M->sl_tag, sl_tag.data(), int64_t(sl_tag.size()), s));
+    MI_TRY(M->sl_s2.alloc(std::max<size_t>(cf_face.size(), 1)));
+    MI_TRY(M->sl_ds.alloc(std::max<size_t>(cf_face.size(), 1)));
+    if (nC > 0) {
+        k_slot_geom<<<grid_for(nC, kBlock), kBlock, 0, s>>>(M->dev(), M->sl_s2.p, M->sl_ds.p);
+        MI_HIP(hipGetLastError());
+    }
     MI_HIP(hipStreamSynchronize(s));
     *out = M.release();
     return MI_OK;
@@ -495,12 +615,20 @@ MI_API int mi_face_gradient(mi_mesh* M, double rho, const double* aP, int pinned
         return set_error(MI_ERR_BAD_ARG, "null argument");
     MI_TRY(ensure_device());
     hipStream_t s = default_stream();
-    if (M->n_faces > 0) {
-        k_face_gradient<<<grid_for(M->n_faces, kBlock), kBlock, 0, s>>>(M->dev(), rho, aP,
-                                                                        pinned ? 1 : 0, x, lambda,
-                                                                        grad);
-        MI_HIP(hipGetLastError());
-    }
+    if (M->n_faces == 0)
+        return MI_OK;
+    MI_TRY(face_gradient_setup(M, s));
+    const int n = int(M->n_cells);
+    MI_TRY(M->fg_x.alloc(size_t(n)));
+    MI_TRY(M->fg_l.alloc(size_t(n)));
+    if (aP)
+        MI_TRY(M->fg_a.alloc(size_t(n)));
+    k_fg_gather<<<grid_for(n, kBlock), kBlock, 0, s>>>(n, M->fg_perm.p, x, lambda, aP, M->fg_x.p,
+                                                       M->fg_l.p, aP ? M->fg_a.p : nullptr);
+    k_face_gradient<<<grid_for(M->n_faces, kBlock), kBlock, 0, s>>>(
+        int(M->n_faces), M->fg_o.p, M->fg_n.p, M->fg_s2.p, M->fg_ds.p, rho,
+        aP ? M->fg_a.p : nullptr, pinned ? M->fg_pin : -1, M->fg_x.p, M->fg_l.p, grad);
+    MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }

@@ -25,6 +25,14 @@ struct MeshDev
     const int* cf_slot; // CSR position of (cell, other cell); -1 on boundary faces
     const int* rp;      // CSR row pointer (for zeroing rows)
     const int* diag;    // CSR position of the diagonal
+    // per cell-face slot (the cf_* order): |S_f|^2, |d . S_f| (d from the
+    // owner's centroid to the neighbour's, or to the face centroid on a
+    // boundary), the cell across the face (-1 on the boundary), and a tag:
+    // 16 when the slot's cell owns the face, plus the face's boundary class + 1
+    const double* sl_s2;
+    const double* sl_ds;
+    const int* sl_oth;
+    const int8_t* sl_tag;
 };
 
 } // namespace mi
@@ -36,12 +44,21 @@ struct mi_mesh
     int n_zones = 0;
     std::vector<int> bc_of_zone;
     bool any_dirichlet = false;
-    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot;
-    mi::DevBuf<int8_t> fbc;
+    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot, sl_oth;
+    mi::DevBuf<int8_t> fbc, sl_tag;
+    mi::DevBuf<double> sl_s2, sl_ds;
     mi::DevBuf<double> ax, ay, az, fcx, fcy, fcz, ccx, ccy, ccz, vol;
     // LeastSquaresGradient's inverted 3x3 moment matrices, 6 per cell, built
     // on first use (geometry only)
     mi::DevBuf<double> ls_minv;
+    // Face-gradient state, built on first use: cells renumbered in the order
+    // the face list first meets them (fg_perm[new] = old), every face's owner
+    // and neighbour in that numbering (neighbour -1: pressure boundary, -2:
+    // a boundary that contributes nothing), |S_f|^2 and |d_f . S_f|; and the
+    // renumbered copies of x, lambda and aP a call gathers.
+    mi::DevBuf<int> fg_perm, fg_o, fg_n;
+    mi::DevBuf<double> fg_s2, fg_ds, fg_x, fg_l, fg_a;
+    int fg_pin = -1; // cell 0 in the renumbering
     mi::MeshDev dev() const
     {
         mi::MeshDev d{};
@@ -66,6 +83,10 @@ struct mi_mesh
         d.cf_slot = cf_slot.p;
         d.rp = pat->rp.p;
         d.diag = pat->diag.p;
+        d.sl_s2 = sl_s2.p;
+        d.sl_ds = sl_ds.p;
+        d.sl_oth = sl_oth.p;
+        d.sl_tag = sl_tag.p;
         return d;
     }
 };
