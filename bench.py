@@ -463,7 +463,11 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3):
     iteration the face fluxes, three momentum assemblies and GMRES(30)-ILU(0)
     solves, the pressure assembly, its AMG-PCG solve and the correction.  With
     the reference's preconditioner rebuild at every solve, and keeping the
-    preconditioners across iterations."""
+    preconditioners across iterations.  `simple` and `piso` set up the AMG
+    hierarchy for every pressure solve by the device numeric re-setup
+    (amg_resetup = 1, the default); `simple_full_setup` by a full host setup
+    each time (amg_resetup = 0, the reference's semantics);
+    `simple_keep_pc` keeps the first hierarchy."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -476,10 +480,12 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3):
     out = {"workload": f"{N}^3 channel, velocity inlet / pressure outlet / walls, rho 1, mu 1e-3; "
                        f"momentum GMRES(30)-ILU(0) 1e-8, pressure CG-AMG 1e-8",
            "n": m.n_cells}
-    for name, alg, keep in (("simple", capi.MI_FLOW_SIMPLE, 0), ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1),
-                            ("piso", capi.MI_FLOW_PISO, 0)):
+    for name, alg, keep, resetup in (("simple", capi.MI_FLOW_SIMPLE, 0, 1),
+                                     ("simple_full_setup", capi.MI_FLOW_SIMPLE, 0, 0),
+                                     ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1, 1),
+                                     ("piso", capi.MI_FLOW_PISO, 0, 1)):
         cfg = capi.flow_cfg(alg, momentum=dict(keep_preconditioner=keep),
-                            pressure=dict(keep_preconditioner=keep))
+                            pressure=dict(keep_preconditioner=keep, amg_resetup=resetup))
         F = capi.Flow(dm, cfg, bcv)
         times, res = [], None
         for it in range(iters + 1):  # the first iteration builds the patterns

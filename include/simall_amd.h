@@ -196,6 +196,14 @@ typedef struct {
                                 built on earlier values (the operator is always current) */
     int use_graph;           /* 1: replay captured HIP graphs (AMG V-cycles; the batched
                                 solve's check_interval iterations), 0: launch each kernel */
+    int amg_resetup;         /* 1 (default): when the bound matrix keeps the pattern of the
+                                last full AMG setup, a setup for new values recomputes the
+                                hierarchy's values on the device with that setup's aggregates,
+                                strong couplings and sparsity patterns (lambda_max, smoothed P,
+                                R = P^T, Galerkin R A P, coarsest inverse); 0: a full host
+                                setup for every new set of values, as
+                                AMGPreconditioner::setup (src/solver/AMGPreconditioner.cpp:248-326)
+                                at every solve (src/solver/LinearSolvers.cpp:422) */
 } mi_solver_cfg;
 
 /* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */
@@ -303,12 +311,24 @@ MI_API int mi_amg_host_build(int64_t n, const int32_t* rowPtr, const int32_t* co
 MI_API int mi_amg_host_levels(const mi_amg_host* h, int* levels, double* operator_complexity,
                               double* grid_complexity);
 /* which: 0 = A_l, 1 = P_l (prolongation from level l+1), 2 = R_l = P_l^T,
- * 3 = dense inverse of the coarsest operator (row-major, values only). */
+ * 3 = dense inverse of the coarsest operator (row-major, values only),
+ * 4 = the aggregate of every row of A_l in level l+1's numbering (colIdx
+ *     receives nrows entries, -1 for a row in no aggregate; ncols = the
+ *     aggregate count),
+ * 5 = per entry of A_l, 1 where it is a strong coupling kept by the filtered
+ *     prolongator smoothing (colIdx receives nnz entries).
+ * 4 and 5 exist on every level but the coarsest of a hierarchy built here
+ * (not of one read from a checkpoint). */
 MI_API int mi_amg_host_level_info(const mi_amg_host* h, int level, int which, int64_t* nrows,
                                   int64_t* ncols, int64_t* nnz, double* lambda_max);
 MI_API int mi_amg_host_level_get(const mi_amg_host* h, int level, int which, int32_t* rowPtr,
                                  int32_t* colIdx, double* values);
 MI_API int mi_amg_host_destroy(mi_amg_host* h);
+/* A host copy of the hierarchy an AMG solver applies now: after a device
+ * re-setup (mi_solver_cfg.amg_resetup) the values recomputed on the device,
+ * and the dense coarsest inverse the device computed.  No reference
+ * counterpart (the reference's levels are private). */
+MI_API int mi_solver_amg_host(mi_solver* s, mi_amg_host** out);
 /* perm[new] = old, reverse Cuthill-McKee of the symmetrised pattern. */
 MI_API int mi_rcm_permutation(int64_t n, const int32_t* rowPtr, const int32_t* colIdx,
                               int32_t* perm, int32_t* bandwidth_after);

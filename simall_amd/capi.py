@@ -52,6 +52,7 @@ class SolverCfg(C.Structure):
         ("amg_coarse_sweeps", C.c_int),
         ("keep_preconditioner", C.c_int),
         ("use_graph", C.c_int),
+        ("amg_resetup", C.c_int),
     ]
 
 
@@ -211,6 +212,7 @@ PROTOTYPES = {
     "mi_flow_checkpoint_write": (_i, [_vp, C.c_char_p, _vp, _vp, _vp, _i]),
     "mi_flow_checkpoint_read": (_i, [_vp, C.c_char_p, _vp]),
     "mi_amg_host_build": (_i, [_i64, _vp, _vp, _vp, C.POINTER(SolverCfg), C.POINTER(_vp)]),
+    "mi_solver_amg_host": (_i, [_vp, C.POINTER(_vp)]),
     "mi_amg_host_levels": (_i, [_vp, C.POINTER(_i), _pd, _pd]),
     "mi_amg_host_level_info": (_i, [_vp, _i, _i, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), _pd]),
     "mi_amg_host_level_get": (_i, [_vp, _i, _i, _vp, _vp, _vp]),
@@ -665,19 +667,22 @@ class DeviceMesh:
 
 
 class AmgHost:
-    """The SA-AMG hierarchy built on the host (inspection; no device needed)."""
+    """The SA-AMG hierarchy built on the host (inspection; no device needed),
+    or (from_solver) a copy of the one an AMG solver applies."""
 
-    A, P, R, COARSE_INV = 0, 1, 2, 3
+    A, P, R, COARSE_INV, AGG, STRONG = 0, 1, 2, 3, 4, 5
 
-    def __init__(self, rowPtr, colIdx, values, cfg=None):
+    def __init__(self, rowPtr=None, colIdx=None, values=None, cfg=None, handle=None):
         self.lib = load()
-        rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
-        ci = np.ascontiguousarray(colIdx, dtype=np.int32)
-        va = np.ascontiguousarray(values, dtype=np.float64)
-        h = C.c_void_p()
-        cfg = cfg if cfg is not None else default_cfg()
-        check(self.lib.mi_amg_host_build(rp.size - 1, ptr(rp), ptr(ci), ptr(va), C.byref(cfg), C.byref(h)))
-        self.h = h.value
+        if handle is None:
+            rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
+            ci = np.ascontiguousarray(colIdx, dtype=np.int32)
+            va = np.ascontiguousarray(values, dtype=np.float64)
+            h = C.c_void_p()
+            cfg = cfg if cfg is not None else default_cfg()
+            check(self.lib.mi_amg_host_build(rp.size - 1, ptr(rp), ptr(ci), ptr(va), C.byref(cfg), C.byref(h)))
+            handle = h.value
+        self.h = handle
         nl, oc, gc = C.c_int(), C.c_double(), C.c_double()
         check(self.lib.mi_amg_host_levels(self.h, C.byref(nl), C.byref(oc), C.byref(gc)))
         self.levels, self.operator_complexity, self.grid_complexity = nl.value, oc.value, gc.value
@@ -686,6 +691,26 @@ class AmgHost:
         nr, nc, nnz, lam = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
         check(self.lib.mi_amg_host_level_info(self.h, level, which, C.byref(nr), C.byref(nc), C.byref(nnz), C.byref(lam)))
         return nr.value, nc.value, nnz.value, lam.value
+
+    @classmethod
+    def from_solver(cls, solver):
+        h = C.c_void_p()
+        check(load().mi_solver_amg_host(solver.h, C.byref(h)))
+        return cls(handle=h.value)
+
+    def aggregates(self, level):
+        """(aggregate of every row of A_l, aggregate count)."""
+        nr, nc, nnz, _ = self.info(level, self.AGG)
+        out = np.empty(nnz, np.int32)
+        check(self.lib.mi_amg_host_level_get(self.h, level, self.AGG, None, ptr(out), None))
+        return out, nc
+
+    def strong(self, level):
+        """1 per entry of A_l that is a strong coupling of the filtered smoothing."""
+        nr, nc, nnz, _ = self.info(level, self.STRONG)
+        out = np.empty(nnz, np.int32)
+        check(self.lib.mi_amg_host_level_get(self.h, level, self.STRONG, None, ptr(out), None))
+        return out
 
     def matrix(self, level, which):
         """(rowPtr, colIdx, values, ncols) of A_l / P_l / R_l, or the dense coarse inverse."""

@@ -16,6 +16,7 @@
 #include "krylov.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -190,8 +191,7 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
         MI_TRY(ev.alloc(size_t(sell.padded)));
         MI_TRY(launch_permute<T>(sell.padded, sell.eperm.p, v.p, ev.p, s));
         MI_HIP(hipStreamSynchronize(s));
-        v = std::move(ev);
-        sell.eperm.release();
+        v = std::move(ev); // eperm stays for refresh
     } else if (nr > 0 && column_order_enabled()) {
         // values kept only in the column order of the row blocks (sm() streams them)
         MI_TRY(rb.sort_columns(rp.p, ci.p, nnz, s));
@@ -204,6 +204,97 @@ template <class T> int DMat<T>::upload(const HCsr& h, hipStream_t s, bool diff_f
         }
     }
     return MI_OK;
+}
+
+// tmp = v with each row's diagonal slot (dg) holding the FP64 row sum: the
+// difference form upload computes on the host.
+__global__ __launch_bounds__(kBlock) void k_diff_rowsum(int n, const int* __restrict__ rp,
+                                                        const double* __restrict__ v,
+                                                        const int* __restrict__ dg,
+                                                        double* __restrict__ tmp)
+{
+#pragma clang fp contract(off)
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    double rs = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        rs += v[k];
+        tmp[k] = v[k];
+    }
+    tmp[dg[i]] = rs;
+}
+
+// dst[m] = T(src[perm[m]]) (perm[m] = -1: a SELL padding slot, 0), or
+// T(src[m]) without perm.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_permute_cvt(int64_t m, const int* __restrict__ perm,
+                                                        const double* __restrict__ src,
+                                                        T* __restrict__ dst)
+{
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < m;
+         t += int64_t(gridDim.x) * kBlock) {
+        const int q = perm ? perm[t] : int(t);
+        dst[t] = q >= 0 ? T(src[q]) : T(0);
+    }
+}
+
+// invD_i = 1 / a_ii (the row's last nonzero diagonal entry), 1 without one.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_level_invd(int n, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci,
+                                                       const double* __restrict__ v,
+                                                       T* __restrict__ invD)
+{
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    T inv = T(1);
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        if (ci[k] == i && v[k] != 0.0)
+            inv = T(1.0 / v[k]);
+    invD[i] = inv;
+}
+
+template <class T> int DMat<T>::refresh(const double* src, double* tmp, hipStream_t s)
+{
+    if (nnz == 0)
+        return MI_OK;
+    if (dg.n) {
+        k_diff_rowsum<<<grid_for(nr, kBlock), kBlock, 0, s>>>(nr, rp.p, src, dg.p, tmp);
+        src = tmp;
+    }
+    if (sell.built())
+        k_permute_cvt<T><<<reduce_grid(sell.padded, 4), kBlock, 0, s>>>(sell.padded, sell.eperm.p,
+                                                                        src, v.p);
+    else if (rb.sorted())
+        k_permute_cvt<T><<<reduce_grid(nnz, 4), kBlock, 0, s>>>(nnz, rb.sperm.p, src, v.p);
+    else
+        k_permute_cvt<T><<<reduce_grid(nnz, 4), kBlock, 0, s>>>(nnz, nullptr, src, v.p);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+// Chebyshev step coefficients on [lmin, lmax] of D^-1 A, lmin = lmax * ratio
+// (ratio 1/2 for degree 1: damped Jacobi).
+static void chebyshev(double lmax_est, int ldeg, double lmin_ratio, std::vector<double>& alpha,
+                      std::vector<double>& beta)
+{
+    const double lmax = lmax_est > 0 ? lmax_est : 2.0;
+    const double ratio = ldeg == 1 ? 0.5 : lmin_ratio;
+    const double lmin = lmax * ratio;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    alpha.assign(size_t(ldeg), 0.0);
+    beta.assign(size_t(ldeg), 0.0);
+    alpha[0] = 1.0 / theta;
+    double rho = 1.0 / sigma;
+    for (int k = 1; k < ldeg; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        beta[size_t(k)] = rn * rho;
+        alpha[size_t(k)] = 2.0 * rn / delta;
+        rho = rn;
+    }
 }
 
 template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r, const T* invD,
@@ -224,6 +315,7 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
     L.clear();
     L.resize(size_t(nl));
     deg = std::max(degree, 1);
+    this->lmin_ratio = lmin_ratio;
     for (int l = 0; l < nl; ++l) {
         const AmgLevelHost& h = H.levels[size_t(l)];
         DLevel<T>& d = L[size_t(l)];
@@ -244,22 +336,7 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
         MI_HIP(hipStreamSynchronize(s));
         for (DevBuf<T>* b : {&d.x, &d.b, &d.r, &d.d, &d.d2, &d.e})
             MI_TRY(b->alloc(size_t(std::max(n, 1))));
-        // Chebyshev coefficients on [lmin, lmax] of D^-1 A
-        const double lmax = h.lmax > 0 ? h.lmax : 2.0;
-        const double ratio = ldeg == 1 ? 0.5 : lmin_ratio;
-        const double lmin = lmax * ratio;
-        const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-        const double sigma = theta / delta;
-        d.alpha.assign(size_t(ldeg), 0.0);
-        d.beta.assign(size_t(ldeg), 0.0);
-        d.alpha[0] = 1.0 / theta;
-        double rho = 1.0 / sigma;
-        for (int k = 1; k < ldeg; ++k) {
-            const double rn = 1.0 / (2.0 * sigma - rho);
-            d.beta[size_t(k)] = rn * rho;
-            d.alpha[size_t(k)] = 2.0 * rn / delta;
-            rho = rn;
-        }
+        chebyshev(h.lmax, ldeg, lmin_ratio, d.alpha, d.beta);
     }
     nc = H.coarse_n;
     if (cinv_device) {
@@ -274,124 +351,379 @@ template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degre
     return MI_OK;
 }
 
+// ------------------------------------------------------------- refresh
+template <class T>
+int DHierarchy<T>::refresh(const std::vector<LevelValues>& v, const double* cinv_device,
+                           double* tmp, hipStream_t s)
+{
+    const int nl = int(L.size());
+    if (int(v.size()) != nl)
+        return set_error(MI_ERR_BAD_ARG, "refresh of %d levels for a hierarchy of %d",
+                         int(v.size()), nl);
+    graphs.clear(); // the captured cycles hold the old Chebyshev coefficients
+    for (int l = 0; l < nl; ++l) {
+        DLevel<T>& d = L[size_t(l)];
+        const LevelValues& lv = v[size_t(l)];
+        MI_TRY(d.A.refresh(lv.A, tmp, s));
+        if (d.A.nr > 0)
+            k_level_invd<T><<<grid_for(d.A.nr, kBlock), kBlock, 0, s>>>(d.A.nr, d.A.rp.p,
+                                                                        d.A.ci.p, lv.A, d.invD.p);
+        MI_HIP(hipGetLastError());
+        if (l + 1 < nl) {
+            MI_TRY(d.P.refresh(lv.P, tmp, s));
+            MI_TRY(d.R.refresh(lv.R, tmp, s));
+        }
+        chebyshev(lv.lmax, d.deg, lmin_ratio, d.alpha, d.beta);
+    }
+    if (nc > 0)
+        MI_HIP(hipMemcpyAsync(cinv.p, cinv_device, size_t(nc) * size_t(nc) * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+    return MI_OK;
+}
+
 // ------------------------------------------------- dense coarsest inverse
-// Pivot step of column c (one workgroup): the row of largest |a_rc|, r >= c
-// (the first on ties, c itself unless another is strictly larger), swapped
-// into row c; a pivot below 1e-13 * amax replaced by +-1e-10 * amax; row c of
-// a and inv scaled by 1 / pivot; column c saved in f (the factors).
-__global__ __launch_bounds__(kBlock) void k_gj_pivot(int n, int c, double amax,
-                                                     double* __restrict__ a,
+// Blocked Gauss-Jordan on [A | I] with the host dense_inverse's per-entry
+// operations.  Pivot step c of the host: the row piv >= c of largest |a_rc|
+// (the first on ties, c unless another is strictly larger) is exchanged with
+// row c; a pivot below 1e-13 * amax is replaced by +-1e-10 * amax; row c is
+// scaled by ip = 1 / pivot; every row r != c with f_r = a_rc != 0 takes
+// a_rk -= f_r a_ck (k >= c) and inv_rk -= f_r inv_ck (all k).
+//
+// Panel c0 .. c0 + nb - 1 (nb <= NB):
+//   k_gj_panel   one workgroup holds columns c0.. of every row in registers
+//                and performs the nb pivot steps on them, recording ip_j, the
+//                exchanged row and f_r,j of every row (F, n x NB).  The rows
+//                the panel touches (its own rows c0.. and the exchanged ones,
+//                at most 2 NB: the set S) are listed in info.
+//   k_gj_rows    per column right of the panel (and every inverse column):
+//                the rows of S replay the nb steps (exchange, scale, update)
+//                and the scaled pivot-row values u_j,k are stored (U).
+//   k_gj_update  every other row r: a_rk -= f_r,j u_j,k for j = 0 .. nb-1,
+//                skipping f_r,j = 0, as the host's nb steps do.
+// Columns left of the panel are no longer read (the host leaves them at the
+// values they reached); so are panel columns after their step.
+constexpr int kGjThreads = 1024;
+
+__global__ __launch_bounds__(kBlock) void k_gj_setup(int n, const double* __restrict__ a,
                                                      double* __restrict__ inv,
-                                                     double* __restrict__ f)
+                                                     double* __restrict__ amax)
 {
-    __shared__ double sv[kBlock];
-    __shared__ int si[kBlock];
-    const int tid = threadIdx.x;
-    const size_t N = size_t(n);
-    double best = -1.0;
-    int bi = n;
-    for (int r = c + tid; r < n; r += kBlock) {
-        const double v = fabs(a[size_t(r) * N + size_t(c)]);
-        if (v > best) {
-            best = v;
-            bi = r;
-        }
-    }
-    sv[tid] = best;
-    si[tid] = bi;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            const double v2 = sv[tid + w];
-            const int i2 = si[tid + w];
-            if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) {
-                sv[tid] = v2;
-                si[tid] = i2;
-            }
-        }
-        __syncthreads();
-    }
-    int piv = si[0];
-    if (piv >= n || !(sv[0] > fabs(a[size_t(c) * N + size_t(c)])))
-        piv = c;
-    __syncthreads();
-    if (piv != c) {
-        for (int k = tid; k < n; k += kBlock) {
-            const size_t x = size_t(c) * N + size_t(k), y = size_t(piv) * N + size_t(k);
-            const double t0 = a[x], t1 = inv[x];
-            a[x] = a[y];
-            inv[x] = inv[y];
-            a[y] = t0;
-            inv[y] = t1;
-        }
-    }
-    __syncthreads();
-    double p = a[size_t(c) * N + size_t(c)];
-    if (fabs(p) <= 1e-13 * amax)
-        p = (p >= 0 ? 1.0 : -1.0) * 1e-10 * (amax > 0 ? amax : 1.0);
-    const double ip = 1.0 / p;
-    __syncthreads();
-    for (int k = tid; k < n; k += kBlock) {
-        const size_t x = size_t(c) * N + size_t(k);
-        a[x] = (k == c ? p : a[x]) * ip;
-        inv[x] *= ip;
-    }
-    __syncthreads();
-    for (int r = tid; r < n; r += kBlock)
-        f[r] = a[size_t(r) * N + size_t(c)];
-}
-
-// Rows r != c with a nonzero factor: a[r][k] -= f_r a[c][k] (k >= c; the
-// pivot row is zero left of c) and inv[r][k] -= f_r inv[c][k] (all k).
-// Grid: x over the n - c + n columns, y over the rows.
-__global__ __launch_bounds__(kBlock) void k_gj_eliminate(int n, int c,
-                                                         const double* __restrict__ f,
-                                                         double* __restrict__ a,
-                                                         double* __restrict__ inv)
-{
-#pragma clang fp contract(off)
-    const int r = int(blockIdx.y);
-    const int ca = n - c;
-    const int j = int(blockIdx.x) * kBlock + int(threadIdx.x);
-    if (r == c || j >= ca + n)
-        return;
-    const double fr = f[r];
-    if (fr == 0.0)
-        return;
-    const size_t N = size_t(n);
-    if (j < ca) {
-        const size_t k = size_t(c + j);
-        a[size_t(r) * N + k] -= fr * a[size_t(c) * N + k];
-    } else {
-        const size_t k = size_t(j - ca);
-        inv[size_t(r) * N + k] -= fr * inv[size_t(c) * N + k];
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_identity(int n, double* __restrict__ m)
-{
+    // inv = I; amax = max |a_ii| (NaN diagonals ignored, as std::max does)
     const size_t N = size_t(n);
     for (size_t q = size_t(blockIdx.x) * kBlock + threadIdx.x; q < N * N;
          q += size_t(gridDim.x) * kBlock)
-        m[q] = (q / N == q % N) ? 1.0 : 0.0;
+        inv[q] = (q / N == q % N) ? 1.0 : 0.0;
+    if (blockIdx.x != 0)
+        return;
+    __shared__ double lds[kBlock];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        const double v = fabs(a[size_t(i) * N + size_t(i)]);
+        m = (m < v) ? v : m;
+    }
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double o = lds[threadIdx.x + w];
+            lds[threadIdx.x] = (lds[threadIdx.x] < o) ? o : lds[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        amax[0] = lds[0];
 }
 
-int dense_inverse_device(int n, double amax, double* a, double* inv, hipStream_t s)
+template <int NB, int RPT>
+__global__ __launch_bounds__(kGjThreads) void k_gj_panel(int n, int c0,
+                                                         const double* __restrict__ amax_p,
+                                                         const double* __restrict__ a,
+                                                         double* __restrict__ F,
+                                                         int* __restrict__ info,
+                                                         double* __restrict__ ipv)
+{
+#pragma clang fp contract(off)
+    constexpr int kWaves = kGjThreads / kWave;
+    __shared__ double wbest[kWaves];
+    __shared__ int wrow[kWaves];
+    __shared__ double rowc[NB], rowp[NB];
+    __shared__ double dabs;
+    __shared__ int spiv[NB];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int nb = min(NB, n - c0);
+    const size_t N = size_t(n);
+    const double amax = amax_p[0];
+    double pv[RPT][NB];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = tid + q * kGjThreads;
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            pv[q][j] = (r < n && j < nb) ? a[size_t(r) * N + size_t(c0 + j)] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j >= nb)
+            break;
+        const int c = c0 + j;
+        // largest |a_rc| over r >= c, the first row on ties
+        double best = -1.0;
+        int bi = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * kGjThreads;
+            if (r >= c && r < n) {
+                const double v = fabs(pv[q][j]);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+        }
+#pragma unroll
+        for (int off = kWave / 2; off >= 1; off >>= 1) {
+            const double ob = __shfl_xor(best, off, kWave);
+            const int oi = __shfl_xor(bi, off, kWave);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            wbest[wid] = best;
+            wrow[wid] = bi;
+        }
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+            if (tid + q * kGjThreads == c)
+                dabs = fabs(pv[q][j]);
+        __syncthreads();
+        double bb = wbest[0];
+        int br = wrow[0];
+        for (int w = 1; w < kWaves; ++w) {
+            const double ob = wbest[w];
+            const int oi = wrow[w];
+            if (ob > bb || (ob == bb && oi < br)) {
+                bb = ob;
+                br = oi;
+            }
+        }
+        const int piv = (br < n && bb > dabs) ? br : c;
+        // exchange rows c and piv (panel columns >= j)
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * kGjThreads;
+            if (r == c)
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (k >= j)
+                        rowc[k] = pv[q][k];
+            if (r == piv)
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (k >= j)
+                        rowp[k] = pv[q][k];
+        }
+        __syncthreads();
+        double p = rowp[j];
+        if (fabs(p) <= 1e-13 * amax)
+            p = (p >= 0 ? 1.0 : -1.0) * 1e-10 * (amax > 0 ? amax : 1.0);
+        const double ip = 1.0 / p;
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * kGjThreads;
+            if (r == piv && piv != c)
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (k >= j)
+                        pv[q][k] = rowc[k];
+            if (r == c)
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (k >= j)
+                        pv[q][k] = (k == j ? p : rowp[k]) * ip;
+        }
+        // factors of column c and the updates of the panel's later columns
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = tid + q * kGjThreads;
+            if (r < n) {
+                const double f = r == c ? 0.0 : pv[q][j];
+                F[size_t(r) * NB + size_t(j)] = f;
+                if (f != 0.0)
+#pragma unroll
+                    for (int k = 0; k < NB; ++k)
+                        if (k > j && k < nb)
+                            pv[q][k] -= f * (rowp[k] * ip);
+            }
+        }
+        if (tid == 0) {
+            ipv[j] = ip;
+            spiv[j] = piv;
+        }
+    }
+    if (tid == 0) {
+        // S: slots 0 .. nb-1 are rows c0 .. c0+nb-1, then the other exchanged rows
+        int* pslot = info + 1;
+        int* slist = info + 1 + NB;
+        int cnt = nb;
+        for (int j = 0; j < nb; ++j)
+            slist[j] = c0 + j;
+        for (int j = 0; j < nb; ++j) {
+            const int pr = spiv[j];
+            int sl = -1;
+            if (pr < c0 + nb) {
+                sl = pr - c0;
+            } else {
+                for (int t = nb; t < cnt; ++t)
+                    if (slist[t] == pr)
+                        sl = t;
+                if (sl < 0) {
+                    sl = cnt;
+                    slist[cnt++] = pr;
+                }
+            }
+            pslot[j] = sl;
+        }
+        info[0] = cnt;
+    }
+}
+
+// Column q of the panel's right part: q < ca is column c0 + nb + q of a,
+// q >= ca column q - ca of inv.
+template <int NB>
+__global__ __launch_bounds__(kBlock) void k_gj_rows(int n, int c0, const int* __restrict__ info,
+                                                    const double* __restrict__ ipv,
+                                                    const double* __restrict__ F,
+                                                    double* __restrict__ a,
+                                                    double* __restrict__ inv,
+                                                    double* __restrict__ U)
+{
+#pragma clang fp contract(off)
+    __shared__ double vals[2 * NB][kBlock];
+    const int nb = min(NB, n - c0);
+    const int ca = n - c0 - nb;
+    const int ncol = ca + n;
+    const int q = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= ncol)
+        return;
+    double* M = q < ca ? a : inv;
+    const size_t k = q < ca ? size_t(c0 + nb + q) : size_t(q - ca);
+    const size_t N = size_t(n);
+    const int cnt = info[0];
+    const int* pslot = info + 1;
+    const int* slist = info + 1 + NB;
+    double* v = &vals[0][threadIdx.x];
+    for (int t = 0; t < cnt; ++t)
+        v[t * kBlock] = M[size_t(slist[t]) * N + k];
+    for (int j = 0; j < nb; ++j) {
+        const int ps = pslot[j];
+        if (ps != j) {
+            const double x = v[j * kBlock];
+            v[j * kBlock] = v[ps * kBlock];
+            v[ps * kBlock] = x;
+        }
+        const double u = v[j * kBlock] * ipv[j];
+        v[j * kBlock] = u;
+        U[size_t(j) * size_t(ncol) + size_t(q)] = u;
+        for (int t = 0; t < cnt; ++t) {
+            if (t == j)
+                continue;
+            const double f = F[size_t(slist[t]) * NB + size_t(j)];
+            if (f != 0.0)
+                v[t * kBlock] -= f * u;
+        }
+    }
+    for (int t = 0; t < cnt; ++t)
+        M[size_t(slist[t]) * N + k] = v[t * kBlock];
+}
+
+// Rows outside S: blocks of 64 columns (one per lane) by kGjUpdateRows rows
+// (the block's 4 wavefronts take every 4th row).
+constexpr int kGjUpdateRows = 64;
+
+template <int NB>
+__global__ __launch_bounds__(kBlock) void k_gj_update(int n, int c0, const int* __restrict__ info,
+                                                      const double* __restrict__ F,
+                                                      const double* __restrict__ U,
+                                                      double* __restrict__ a,
+                                                      double* __restrict__ inv)
+{
+#pragma clang fp contract(off)
+    const int nb = min(NB, n - c0);
+    const int ca = n - c0 - nb;
+    const int ncol = ca + n;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    const int q = int(blockIdx.x) * kWave + lane;
+    if (q >= ncol)
+        return;
+    double* M = q < ca ? a : inv;
+    const size_t k = q < ca ? size_t(c0 + nb + q) : size_t(q - ca);
+    const size_t N = size_t(n);
+    const int cnt = info[0];
+    const int* slist = info + 1 + NB;
+    double u[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        u[j] = j < nb ? U[size_t(j) * size_t(ncol) + size_t(q)] : 0.0;
+    const int r0 = int(blockIdx.y) * kGjUpdateRows;
+    const int r1 = min(n, r0 + kGjUpdateRows);
+    for (int r = r0 + wid; r < r1; r += kBlock / kWave) {
+        bool in_s = r >= c0 && r < c0 + nb;
+        for (int t = nb; t < cnt && !in_s; ++t)
+            in_s = slist[t] == r;
+        if (in_s)
+            continue;
+        const double* fr = F + size_t(r) * NB;
+        double x = M[size_t(r) * N + k];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < nb) {
+                const double f = fr[j];
+                if (f != 0.0)
+                    x -= f * u[j];
+            }
+        }
+        M[size_t(r) * N + k] = x;
+    }
+}
+
+template <int NB, int RPT>
+static int gj_run(int n, double* a, double* inv, DenseInverse& w, hipStream_t s)
+{
+    MI_TRY(w.F.alloc(size_t(n) * NB));
+    MI_TRY(w.U.alloc(size_t(NB) * size_t(2 * n)));
+    MI_TRY(w.ipv.alloc(NB));
+    MI_TRY(w.amax.alloc(1));
+    MI_TRY(w.info.alloc(1 + 3 * NB));
+    k_gj_setup<<<grid_for(int64_t(n) * n, kBlock), kBlock, 0, s>>>(n, a, inv, w.amax.p);
+    for (int c0 = 0; c0 < n; c0 += NB) {
+        const int nb = std::min(NB, n - c0);
+        const int ncol = (n - c0 - nb) + n;
+        k_gj_panel<NB, RPT><<<1, kGjThreads, 0, s>>>(n, c0, w.amax.p, a, w.F.p, w.info.p, w.ipv.p);
+        k_gj_rows<NB><<<grid_for(ncol, kBlock), kBlock, 0, s>>>(n, c0, w.info.p, w.ipv.p, w.F.p, a,
+                                                                inv, w.U.p);
+        const dim3 g(grid_for(ncol, kWave), grid_for(n, kGjUpdateRows));
+        k_gj_update<NB><<<g, kBlock, 0, s>>>(n, c0, w.info.p, w.F.p, w.U.p, a, inv);
+    }
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
+int DenseInverse::run(int n, double* a, double* inv, hipStream_t s)
 {
     if (n <= 0)
         return MI_OK;
-    if (n > 65535)
-        return set_error(MI_ERR_BAD_ARG, "dense inverse of %d rows exceeds the grid", n);
-    DevBuf<double> f;
-    MI_TRY(f.alloc(size_t(n)));
-    k_identity<<<grid_for(int64_t(n) * n, kBlock), kBlock, 0, s>>>(n, inv);
-    MI_HIP(hipGetLastError());
-    for (int c = 0; c < n; ++c) {
-        k_gj_pivot<<<1, kBlock, 0, s>>>(n, c, amax, a, inv, f.p);
-        const dim3 g(grid_for(int64_t(n - c) + n, kBlock), unsigned(n));
-        k_gj_eliminate<<<g, kBlock, 0, s>>>(n, c, f.p, a, inv);
-    }
-    MI_HIP(hipGetLastError());
+    if (n > kAmgMaxDense)
+        return set_error(MI_ERR_BAD_ARG, "dense inverse of %d rows (at most %d)", n, kAmgMaxDense);
+    static_assert(kAmgMaxDense <= 4 * kGjThreads, "panel rows per thread");
+    if (n <= kGjThreads)
+        MI_TRY((gj_run<16, 1>(n, a, inv, *this, s)));
+    else if (n <= 2 * kGjThreads)
+        MI_TRY((gj_run<8, 2>(n, a, inv, *this, s)));
+    else
+        MI_TRY((gj_run<8, 4>(n, a, inv, *this, s)));
     MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }
@@ -474,11 +806,20 @@ struct AmgPrecond final : Precond
     int degree = 2;
     int coarse_degree = 0;
     double lmin_ratio = 0.3;
-    AmgHierarchyHost host;
+    // the hierarchy of the last full setup; after a numeric re-setup its
+    // values are those of that setup until sync_host downloads the new ones
+    mutable AmgHierarchyHost host;
+    mutable bool host_stale = false;
+    hipStream_t last_stream = nullptr;
     DevBuf<double> cinv; // inverse of the coarsest operator (copied into each hierarchy)
+    DenseInverse gj;
     DHierarchy<double> h64;
     DHierarchy<float> h32;
     bool has64 = false, has32 = false;
+    std::shared_ptr<Pattern> built_pat;   // pattern of the last full setup
+    std::unique_ptr<AmgNumeric> num;      // re-setup data, recorded on first use
+    DevBuf<double> tmp;                   // scratch of the refreshes
+    bool num_failed = false;
 
     int setup(mi_csr* A, bool need64, bool need32, hipStream_t s) override
     {
@@ -489,20 +830,97 @@ struct AmgPrecond final : Precond
         H.ci = A->pat->h_ci;
         H.v = A->h_val;
         params.host_coarse_inverse = false;
+        num.reset();
+        num_failed = false;
+        host_stale = false;
+        built_pat.reset();
+        last_stream = s;
         if (amg_build(H, params, host) != 0)
             return set_error(MI_ERR_UNSUPPORTED,
                              "AMG coarsening stalled above %d rows (no dense coarsest solve)",
                              kAmgMaxDense);
-        return build_device(need64, need32, s);
+        MI_TRY(build_device(need64, need32, s));
+        built_pat = A->pat;
+        return MI_OK;
+    }
+    bool can_resetup(const mi_csr* A, bool need64, bool need32) const override
+    {
+        if (num_failed || !built_pat || A->pat != built_pat || need64 != has64 || need32 != has32)
+            return false;
+        for (size_t l = 0; l + 1 < host.levels.size(); ++l)
+            if (host.levels[l].agg.empty() || host.levels[l].strong.empty())
+                return false;
+        return true;
+    }
+    int resetup(mi_csr* A, bool need64, bool need32, hipStream_t s) override
+    {
+        (void)need64;
+        (void)need32;
+        last_stream = s;
+        if (!num) {
+            auto nm = std::make_unique<AmgNumeric>();
+            const int rc = nm->record(host, A->pat->rp.p, A->pat->ci.p, s);
+            if (rc != MI_OK) {
+                // a hierarchy the numeric phase cannot follow: full setups from now on
+                clear_error();
+                const int rc2 = setup(A, need64, need32, s);
+                num_failed = true;
+                return rc2;
+            }
+            MI_TRY(tmp.alloc(nm->max_nnz()));
+            num = std::move(nm);
+        }
+        MI_TRY(num->run(A->val.p, params.omega_scale, s));
+        const int nc = host.coarse_n;
+        MI_TRY(gj.run(nc, num->dense.p, cinv.p, s));
+        std::vector<LevelValues> v(host.levels.size());
+        for (size_t l = 0; l < v.size(); ++l)
+            v[l] = num->values(int(l));
+        if (has64)
+            MI_TRY(h64.refresh(v, cinv.p, tmp.p, s));
+        if (has32)
+            MI_TRY(h32.refresh(v, cinv.p, tmp.p, s));
+        MI_HIP(hipStreamSynchronize(s));
+        host_stale = true;
+        return MI_OK;
+    }
+    // Brings `host` up to the device's values after numeric re-setups.
+    int sync_host() const
+    {
+        if (!host_stale)
+            return MI_OK;
+        MI_TRY(num->download(host, last_stream));
+        host_stale = false;
+        return MI_OK;
     }
     const AmgHierarchyHost* hierarchy() const override
     {
-        return (has64 || has32) ? &host : nullptr;
+        if (!(has64 || has32) || sync_host() != MI_OK)
+            return nullptr;
+        return &host;
+    }
+    int host_copy(AmgHierarchyHost& out, hipStream_t s) override
+    {
+        if (!(has64 || has32))
+            return set_error(MI_ERR_UNSUPPORTED, "no AMG hierarchy has been built");
+        MI_TRY(sync_host());
+        out = host;
+        const size_t nn = size_t(host.coarse_n) * size_t(host.coarse_n);
+        out.coarse_inverse.resize(nn);
+        if (nn)
+            MI_HIP(hipMemcpyAsync(out.coarse_inverse.data(), cinv.p, nn * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+        MI_HIP(hipStreamSynchronize(s));
+        return MI_OK;
     }
     int install(const AmgHierarchyHost& H, bool need64, bool need32, hipStream_t s) override
     {
         host = H;
         host.coarse_inverse.clear();
+        num.reset();
+        host_stale = false;
+        built_pat.reset();
+        last_stream = s;
         return build_device(need64, need32, s);
     }
     // Device hierarchies from `host`.
@@ -510,13 +928,10 @@ struct AmgPrecond final : Precond
     {
         // the coarsest inverse, once for both precisions, on the device
         const int nc = host.coarse_n;
-        double amax = 0.0;
-        for (int i = 0; i < nc; ++i)
-            amax = std::max(amax, std::abs(host.coarse_dense[size_t(i) * size_t(nc) + size_t(i)]));
         DevBuf<double> dense;
         MI_TRY(dense.upload(host.coarse_dense.data(), host.coarse_dense.size(), s));
         MI_TRY(cinv.alloc(size_t(nc) * size_t(nc)));
-        MI_TRY(dense_inverse_device(nc, amax, dense.p, cinv.p, s));
+        MI_TRY(gj.run(nc, dense.p, cinv.p, s));
         has64 = has32 = false;
         if (need64) {
             MI_TRY(h64.build(host, degree, coarse_degree, lmin_ratio, s, cinv.p));

@@ -22,6 +22,14 @@ const AmgHierarchyHost* amg_host_hierarchy(const mi_amg_host* h)
 {
     return h ? &h->h : nullptr;
 }
+
+int amg_host_wrap(AmgHierarchyHost&& H, mi_amg_host** out)
+{
+    auto w = std::make_unique<mi_amg_host>();
+    w->h = std::move(H);
+    *out = w.release();
+    return MI_OK;
+}
 } // namespace mi
 
 extern "C" {
@@ -93,14 +101,24 @@ MI_API int mi_amg_host_levels(const mi_amg_host* H, int* levels, double* operato
     return MI_OK;
 }
 
-// which: 0 = A_l, 1 = P_l (level l+1 -> l), 2 = R_l = P_l^T, 3 = coarsest dense inverse
+// which: 0 = A_l, 1 = P_l (level l+1 -> l), 2 = R_l = P_l^T, 3 = coarsest dense inverse,
+// 4 = aggregates of the rows of A_l, 5 = strong-coupling mask of A_l's entries
 MI_API int mi_amg_host_level_info(const mi_amg_host* H, int level, int which, int64_t* nrows,
                                   int64_t* ncols, int64_t* nnz, double* lambda_max)
 {
-    if (!H || level < 0 || level >= int(H->h.levels.size()) || which < 0 || which > 3)
+    if (!H || level < 0 || level >= int(H->h.levels.size()) || which < 0 || which > 5)
         return set_error(MI_ERR_BAD_ARG, "bad level or matrix selector");
     const AmgLevelHost& L = H->h.levels[size_t(level)];
-    if (which == 3) {
+    if (which >= 4) {
+        if (L.agg.empty() || L.strong.empty())
+            return set_error(MI_ERR_UNSUPPORTED, "level %d records no aggregates", level);
+        if (nrows)
+            *nrows = L.A.nr;
+        if (ncols)
+            *ncols = which == 4 ? L.n_aggregates : L.A.nc;
+        if (nnz)
+            *nnz = which == 4 ? int64_t(L.agg.size()) : int64_t(L.strong.size());
+    } else if (which == 3) {
         const int64_t c = H->h.coarse_n;
         if (nrows)
             *nrows = c;
@@ -125,8 +143,19 @@ MI_API int mi_amg_host_level_info(const mi_amg_host* H, int level, int which, in
 MI_API int mi_amg_host_level_get(const mi_amg_host* H, int level, int which, int32_t* rp,
                                  int32_t* ci, double* v)
 {
-    if (!H || level < 0 || level >= int(H->h.levels.size()) || which < 0 || which > 3)
+    if (!H || level < 0 || level >= int(H->h.levels.size()) || which < 0 || which > 5)
         return set_error(MI_ERR_BAD_ARG, "bad level or matrix selector");
+    if (which >= 4) {
+        const AmgLevelHost& L = H->h.levels[size_t(level)];
+        if (L.agg.empty() || L.strong.empty())
+            return set_error(MI_ERR_UNSUPPORTED, "level %d records no aggregates", level);
+        if (ci && which == 4)
+            std::memcpy(ci, L.agg.data(), L.agg.size() * sizeof(int32_t));
+        if (ci && which == 5)
+            for (size_t k = 0; k < L.strong.size(); ++k)
+                ci[k] = L.strong[k];
+        return MI_OK;
+    }
     if (which == 3) {
         if (v)
             std::memcpy(v, H->h.coarse_inverse.data(), H->h.coarse_inverse.size() * sizeof(double));

@@ -170,6 +170,18 @@ int amg_aggregate(int n, const std::vector<int>& srp, const std::vector<int>& sc
     return nagg;
 }
 
+void amg_lambda_start(int n, std::vector<double>& x)
+{
+    x.resize(size_t(n));
+    uint64_t st = 0x9E3779B97F4A7C15ULL;
+    for (int i = 0; i < n; ++i) {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        x[size_t(i)] = double(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+}
+
 double amg_lambda_max(const HCsr& A, int iters)
 {
     // Power iteration on the symmetric scaling D^-1/2 A D^-1/2 (same spectrum
@@ -181,13 +193,7 @@ double amg_lambda_max(const HCsr& A, int iters)
     std::vector<double> s(static_cast<size_t>(n)), x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i)
         s[size_t(i)] = d[size_t(i)] != 0.0 ? 1.0 / std::sqrt(std::abs(d[size_t(i)])) : 1.0;
-    uint64_t st = 0x9E3779B97F4A7C15ULL;
-    for (int i = 0; i < n; ++i) {
-        st ^= st << 13;
-        st ^= st >> 7;
-        st ^= st << 17;
-        x[size_t(i)] = double(st >> 11) * (1.0 / 9007199254740992.0) - 0.5;
-    }
+    amg_lambda_start(n, x);
     const int threads = thread_count(0);
     double lam = 1.0;
     for (int it = 0; it < iters; ++it) {
@@ -486,7 +492,7 @@ int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv, int t
 // A_c <- Q A_c Q^T and P <- P Q^T (R is rebuilt as P^T).  An equivalent
 // hierarchy whose coarse operators have a narrow band, so the device gathers
 // of x on those levels touch few cache lines.
-void amg_renumber_coarse(HCsr& Ac, HCsr& P)
+void amg_renumber_coarse(HCsr& Ac, HCsr& P, std::vector<int>& agg)
 {
     const int nc = Ac.nr;
     if (nc < 2)
@@ -505,6 +511,9 @@ void amg_renumber_coarse(HCsr& Ac, HCsr& P)
     std::vector<int> inv(static_cast<size_t>(nc));
     for (int i = 0; i < nc; ++i)
         inv[size_t(perm[size_t(i)])] = i;
+    for (int& a : agg)
+        if (a >= 0)
+            a = inv[size_t(a)];
     parallel_for(P.nr, thread_count(0), [&](int b, int e, int) {
         std::vector<std::pair<int, double>> row;
         for (int i = b; i < e; ++i) {
@@ -545,6 +554,22 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
             amg_strength(A, 0.0, srp, sci);
             nagg = amg_aggregate(A.nr, srp, sci, A, agg);
         }
+        // the strong entries of A that the filtered smoothing keeps (every
+        // off-diagonal entry when unfiltered), for numeric re-setups
+        L.strong.assign(size_t(A.nnz()), 0);
+        parallel_for(A.nr, thread_count(0), [&](int b, int e, int) {
+            std::vector<char> mark(size_t(A.nc), 0);
+            for (int i = b; i < e; ++i) {
+                for (int s = srp[size_t(i)]; s < srp[size_t(i) + 1]; ++s)
+                    mark[size_t(sci[size_t(s)])] = 1;
+                for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                    const int j = A.ci[size_t(k)];
+                    L.strong[size_t(k)] = j != i && (!p.filter || mark[size_t(j)]);
+                }
+                for (int s = srp[size_t(i)]; s < srp[size_t(i) + 1]; ++s)
+                    mark[size_t(sci[size_t(s)])] = 0;
+            }
+        });
         if (nagg <= 0 || nagg >= A.nr * 9 / 10)
             break; // no useful coarsening
         L.lmax = amg_lambda_max(A, 20) * 1.05;
@@ -562,7 +587,8 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
             csr_multiply(R0, AP, Ac, threads);
         }
         if (p.renumber)
-            amg_renumber_coarse(Ac, L.P);
+            amg_renumber_coarse(Ac, L.P, agg);
+        L.agg = std::move(agg);
         csr_transpose(L.P, L.R);
         out.levels.emplace_back();
         out.levels.back().A = std::move(Ac);

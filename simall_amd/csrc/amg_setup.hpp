@@ -41,6 +41,12 @@ struct AmgLevelHost
     HCsr P, R;       // prolongator to this level from the next, R = P^T (empty on the coarsest)
     double lmax = 0; // estimate of lambda_max(D^-1 A)
     int n_aggregates = 0;
+    // What a numeric re-setup on new values of the same pattern reuses (empty
+    // in hierarchies read from a checkpoint): the aggregate of every row in
+    // the next level's numbering, and per entry of A whether it is a strong
+    // coupling kept by the filtered smoothing operator.
+    std::vector<int> agg;
+    std::vector<uint8_t> strong;
 };
 
 struct AmgHierarchyHost
@@ -65,6 +71,8 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
 void csr_transpose(const HCsr& A, HCsr& T);
 void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads);
 double amg_lambda_max(const HCsr& A, int iters);
+// The start vector of amg_lambda_max's power iteration (n entries).
+void amg_lambda_start(int n, std::vector<double>& x);
 int dense_inverse(int n, std::vector<double>& a, std::vector<double>& inv, int threads = 1);
 
 } // namespace mi

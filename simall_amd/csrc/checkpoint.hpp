@@ -20,5 +20,7 @@ const AmgHierarchyHost* checkpoint_amg(const mi_checkpoint* c);
 
 // The hierarchy behind an mi_amg_host handle (amg_host_api.cpp).
 const AmgHierarchyHost* amg_host_hierarchy(const mi_amg_host* h);
+// A new mi_amg_host handle holding H.
+int amg_host_wrap(AmgHierarchyHost&& H, mi_amg_host** out);
 
 } // namespace mi

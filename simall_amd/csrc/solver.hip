@@ -11,6 +11,7 @@
 #include "solver.hpp"
 #include "stream.hpp"
 #include "amg.hpp"
+#include "checkpoint.hpp"
 #include "gmres.hpp"
 #include "ilu0.hpp"
 #include "reorder.hpp"
@@ -366,6 +367,9 @@ int build(mi_solver* S)
             return set_error(MI_ERR_BAD_ARG,
                              "adopted AMG hierarchy was built on another matrix pattern");
         MI_TRY(S->pc->install(*H, !mixed, mixed, S->stream));
+    } else if (S->cfg.amg_resetup && S->pc->can_resetup(M, !mixed, mixed)) {
+        // same pattern, new values: the hierarchy's values recomputed on the device
+        MI_TRY(S->pc->resetup(M, !mixed, mixed, S->stream));
     } else {
         MI_TRY(S->pc->setup(M, !mixed, mixed, S->stream));
     }
@@ -961,6 +965,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->amg_coarse_sweeps = 1;
     c->keep_preconditioner = 0;
     c->use_graph = 1;
+    c->amg_resetup = 1;
 }
 
 MI_API int mi_solver_create(const mi_solver_cfg* cfg, mi_solver** out)
@@ -1101,6 +1106,20 @@ MI_API int mi_solver_info(const mi_solver* S, int* levels, double* operator_comp
     if (bandwidth_after)
         *bandwidth_after = S->ro ? S->ro->bandwidth_after : -1;
     return MI_OK;
+}
+
+MI_API int mi_solver_amg_host(mi_solver* S, mi_amg_host** out)
+{
+    clear_error();
+    if (!S || !out)
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    *out = nullptr;
+    if (!S->pc)
+        return set_error(MI_ERR_UNSUPPORTED, "the solver has built no preconditioner");
+    MI_TRY(ensure_device());
+    AmgHierarchyHost H;
+    MI_TRY(S->pc->host_copy(H, S->stream));
+    return amg_host_wrap(std::move(H), out);
 }
 
 MI_API int mi_solver_time_spmv(mi_solver* S, int precision, int reps, double* avg_us,

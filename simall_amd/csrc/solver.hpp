@@ -38,6 +38,19 @@ struct Precond
     {
         return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no hierarchy", name());
     }
+    // Whether new values of A can be taken by resetup (same pattern object as
+    // the last full setup, the same precisions), and that numeric setup.
+    virtual bool can_resetup(const mi_csr*, bool, bool) const { return false; }
+    virtual int resetup(mi_csr*, bool, bool, hipStream_t)
+    {
+        return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no numeric re-setup", name());
+    }
+    // AMG only: a host copy of the hierarchy applied now, with the device's
+    // coarsest inverse.
+    virtual int host_copy(AmgHierarchyHost&, hipStream_t)
+    {
+        return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no hierarchy", name());
+    }
     bool use_graph = true; // replay applies from captured HIP graphs
 };
 

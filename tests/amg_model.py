@@ -83,3 +83,65 @@ def pcg(A, b, M, tol=1e-8, maxit=1000):
         p = z + (rzn / rz) * p
         rz = rzn
     return x, maxit
+
+
+def smoothed_prolongator(A, agg, nagg, strong, lmax, omega_scale=4.0 / 3.0):
+    """P = (I - omega D_F^-1 A_F) T for fixed aggregates and strong couplings
+    (every entry, no pattern restriction): T[i, agg[i]] = 1 for aggregated
+    rows; A_F keeps A's strong off-diagonal entries and has the filtered
+    diagonal D_F = diag(A) + the row's weak off-diagonal entries;
+    omega = omega_scale / lmax.  `strong` is per entry of A in CSR order."""
+    import scipy.sparse as sp
+    A = A.tocsr()
+    n = A.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    cols = A.indices
+    diag = rows == cols
+    d = np.bincount(rows[diag], weights=A.data[diag], minlength=n)
+    weak = (~diag) & (np.asarray(strong) == 0)
+    dF = d + np.bincount(rows[weak], weights=A.data[weak], minlength=n)
+    keep = (~diag) & (np.asarray(strong) != 0)
+    AF = sp.csr_matrix((A.data[keep], (rows[keep], cols[keep])), shape=(n, n)) + sp.diags(dF)
+    has = agg >= 0
+    T = sp.csr_matrix((np.ones(has.sum()), (np.arange(n)[has], agg[has])), shape=(n, nagg))
+    omega = omega_scale / lmax
+    w = np.where(dF != 0, omega / np.where(dF != 0, dF, 1.0), 0.0)
+    return (T - sp.diags(w) @ (AF @ T)).tocsr()
+
+
+def restrict_to_pattern(M, pattern):
+    """M's values at the (row, column) positions of the sparse matrix
+    `pattern`, in its CSR order (positions M does not hold give 0)."""
+    M = M.tocsr()
+    n = pattern.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(pattern.indptr))
+    return np.asarray(M[rows, pattern.indices]).ravel()
+
+
+def lambda_start(n):
+    """The power iteration's start vector (amg_lambda_start): xorshift64."""
+    out = np.empty(n)
+    st = 0x9E3779B97F4A7C15
+    mask = (1 << 64) - 1
+    for i in range(n):
+        st ^= (st << 13) & mask
+        st ^= st >> 7
+        st ^= (st << 17) & mask
+        out[i] = float(st >> 11) * (1.0 / 9007199254740992.0) - 0.5
+    return out
+
+
+def lambda_max_estimate(A, iters=20):
+    """amg_lambda_max: power iteration on D^-1/2 A D^-1/2 from lambda_start,
+    the last Rayleigh quotient (without the 1.05 margin)."""
+    A = A.tocsr()
+    d = A.diagonal()
+    s = np.where(d != 0, 1.0 / np.sqrt(np.abs(np.where(d != 0, d, 1.0))), 1.0)
+    x = lambda_start(A.shape[0])
+    lam = 1.0
+    for _ in range(iters):
+        y = s * (A @ (s * x))
+        xy, xx, yy = x @ y, x @ x, y @ y
+        lam = abs(xy / xx) if xx > 0 else 1.0
+        x = y * (1.0 / np.sqrt(yy) if yy > 0 else 1.0)
+    return lam

@@ -131,3 +131,50 @@ def test_host_csr_input_is_validated():
             capi.rcm_permutation(r, c)
         with pytest.raises(capi.MiError):
             capi.AmgHost(r, c, np.ones(c.size))
+
+
+def test_numeric_phase_model_reproduces_the_host_setup():
+    """The numpy statement of the numeric phase (the smoothed prolongator from
+    the recorded aggregates and strong couplings, lambda_max by the same power
+    iteration, A_c = R A P) reproduces every level of a host build; the device
+    re-setup is checked against this model on new values."""
+    from amg_model import smoothed_prolongator, restrict_to_pattern, lambda_max_estimate
+    A, b = system(16)
+    cfg = capi.default_cfg()
+    cfg.amg_coarse_size = 64
+    H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=cfg)
+    assert H.levels >= 3
+    for l in range(H.levels - 1):
+        Al, P = H.scipy(l, H.A), H.scipy(l, H.P)
+        agg, nagg = H.aggregates(l)
+        strong = H.strong(l)
+        assert agg.size == Al.shape[0] and strong.size == Al.nnz
+        assert nagg == P.shape[1] and agg.max() == nagg - 1 and agg.min() >= -1
+        lmax = H.info(l, H.A)[3]
+        lam = lambda_max_estimate(Al)
+        assert abs(lam * 1.05 - lmax) <= 1e-10 * lmax
+        Pm = smoothed_prolongator(Al, agg, nagg, strong, lmax)
+        vals = restrict_to_pattern(Pm, P)
+        assert np.allclose(vals, P.data, rtol=1e-13, atol=1e-15)
+        # the host dropped only exact zeros
+        outside = Pm - sp.csr_matrix((vals, P.indices, P.indptr), shape=P.shape)
+        assert abs(outside).max() <= 1e-15 * abs(P).max()
+        Ac = H.scipy(l + 1, H.A)
+        ref = restrict_to_pattern(P.T @ Al @ P, Ac)
+        assert np.allclose(ref, Ac.data, rtol=1e-12, atol=1e-13 * abs(Ac.data).max())
+
+
+def test_strong_mask_marks_the_strength_test():
+    """Entry a_ij (i != j) is strong iff |a_ij| >= theta sqrt(|a_ii a_jj|) and
+    a_ij != 0 (theta 0.02), which the prolongator's filtering keeps."""
+    A, b = system(12)
+    cfg = capi.default_cfg()
+    cfg.amg_coarse_size = 64
+    H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=cfg)
+    strong = H.strong(0)
+    A = A.tocsr()
+    rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    d = A.diagonal()
+    a = np.abs(A.data)
+    want = (rows != A.indices) & (a > 0) & (a >= 0.02 * np.sqrt(np.abs(d[rows] * d[A.indices])))
+    assert np.array_equal(strong.astype(bool), want)
