@@ -457,7 +457,7 @@ def bench_momentum(capi, wl, lib, steps, log, N=96):
     return out
 
 
-def bench_flow(capi, wl, lib, log, N=96, iters=3):
+def bench_flow(capi, wl, lib, log, N=96, iters=3, variants=None):
     """Device-resident SIMPLE outer iterations and PISO time steps on a 96^3
     channel (velocity inlet, pressure outlet, walls; rho 1, mu 1e-3): per
     iteration the face fluxes, three momentum assemblies and GMRES(30)-ILU(0)
@@ -484,6 +484,8 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3):
                                      ("simple_full_setup", capi.MI_FLOW_SIMPLE, 0, 0),
                                      ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1, 1),
                                      ("piso", capi.MI_FLOW_PISO, 0, 1)):
+        if variants and name not in variants:
+            continue
         cfg = capi.flow_cfg(alg, momentum=dict(keep_preconditioner=keep),
                             pressure=dict(keep_preconditioner=keep, amg_resetup=resetup))
         F = capi.Flow(dm, cfg, bcv)
@@ -668,6 +670,12 @@ def main():
                     help="skip the batched transport measurement (32 equations on a 100^3 mesh)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the discrete-adjoint timing (J = sum w x^2 on the same system)")
+    ap.add_argument("--flow-only", action="store_true",
+                    help="run the SIMPLE / PISO iteration section alone (short run for profiles)")
+    ap.add_argument("--flow-n", type=int, default=96, help="cells per axis of the flow section")
+    ap.add_argument("--flow-iters", type=int, default=3, help="timed iterations per flow variant")
+    ap.add_argument("--flow-variants", default="",
+                    help="comma list of simple, simple_full_setup, simple_keep_pc, piso (default all)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the working-matrix SpMV and a calibration copy only "
                          "(short run for hardware-counter passes)")
@@ -681,6 +689,14 @@ def main():
 
     from simall_amd import capi
     from simall_amd import workload as wl
+
+    if args.flow_only:
+        lib = capi.load()
+        capi.check(lib.mi_init(0))
+        out = bench_flow(capi, wl, lib, log, N=args.flow_n, iters=args.flow_iters,
+                         variants=[v for v in args.flow_variants.split(",") if v])
+        print(json.dumps({"flow": out}), flush=True)
+        return
 
     traffic, traffic_detail = None, {"source": "not measured (--no-traffic)"}
     if not args.spmv_only and not args.no_traffic:

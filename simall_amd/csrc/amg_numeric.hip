@@ -47,7 +47,22 @@ struct NumLevel
     DevBuf<int> aprp, apci;       // A_l P_l
     DevBuf<double> apv;
     int64_t p_nnz = 0, r_nnz = 0;
+    int lanes = 1;                // lanes per row of the power iteration
+    bool wave_rows = false;       // P and A P a wavefront per row (few rows), else a thread
+    bool ap_wave = false;         // A P and R (A P) by k_num_product_wave
+    bool rap_wave = false;
 };
+
+namespace
+{
+int max_row(const std::vector<int>& rp)
+{
+    int m = 0;
+    for (size_t i = 0; i + 1 < rp.size(); ++i)
+        m = std::max(m, rp[i + 1] - rp[i]);
+    return m;
+}
+} // namespace
 
 AmgNumeric::AmgNumeric() = default;
 AmgNumeric::~AmgNumeric() = default;
@@ -80,7 +95,9 @@ __global__ __launch_bounds__(kBlock) void k_num_diag(int n, const int* __restric
 }
 
 // One power-iteration step y = S A S x, x = x0 (first step) or the previous
-// y times 1/||y|| (scal[1]); block partials of x.y, x.x and y.y.
+// y times 1/||y|| (scal[1]); block partials of x.y, x.x and y.y.  G lanes
+// per row (a power of two) sum the row's entries and fold them by shuffles.
+template <int G>
 __global__ __launch_bounds__(kBlock) void k_num_power(int n, const int* __restrict__ rp,
                                                       const int* __restrict__ ci,
                                                       const double* __restrict__ v,
@@ -91,22 +108,31 @@ __global__ __launch_bounds__(kBlock) void k_num_power(int n, const int* __restri
                                                       double* __restrict__ part)
 {
 #pragma clang fp contract(off)
-    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    const int sub = int(threadIdx.x) % G;
     const double inv = scal ? scal[1] : 1.0;
     double acc3[3] = {0.0, 0.0, 0.0};
-    if (i < n) {
+    // rows in a grid-stride loop (a bounded grid keeps the partials few)
+    const int64_t stride = int64_t(gridDim.x) * (kBlock / G);
+    for (int64_t i0 = int64_t(blockIdx.x) * (kBlock / G); i0 < n; i0 += stride) {
+        const int i = int(i0) + int(threadIdx.x) / G;
         double acc = 0.0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const int j = ci[k];
-            const double xj = scal ? xin[j] * inv : xin[j];
-            acc += v[k] * s[j] * xj;
+        if (i < n)
+            for (int k = rp[i] + sub; k < rp[i + 1]; k += G) {
+                const int j = ci[k];
+                const double xj = scal ? xin[j] * inv : xin[j];
+                acc += v[k] * s[j] * xj;
+            }
+#pragma unroll
+        for (int off = G / 2; off >= 1; off >>= 1)
+            acc += __shfl_xor(acc, off, G);
+        if (i < n && sub == 0) {
+            const double yi = s[i] * acc;
+            const double xi = scal ? xin[i] * inv : xin[i];
+            y[i] = yi;
+            acc3[0] += xi * yi;
+            acc3[1] += xi * xi;
+            acc3[2] += yi * yi;
         }
-        const double yi = s[i] * acc;
-        const double xi = scal ? xin[i] * inv : xin[i];
-        y[i] = yi;
-        acc3[0] = xi * yi;
-        acc3[1] = xi * xi;
-        acc3[2] = yi * yi;
     }
     block_partials<3>(acc3, part);
 }
@@ -126,10 +152,12 @@ __global__ __launch_bounds__(kBlock) void k_num_power_fin(const double* __restri
     }
 }
 
-// Row i of P = (I - omega D_F^-1 A_F) T on P's fixed (sorted) pattern: the
-// row's own aggregate takes 1, then every entry of A's row whose column is
-// aggregated adds -w dF (diagonal) or -w a_ij (strong coupling), in A's row
-// order, w = omega / dF.
+// Row i of P = (I - omega D_F^-1 A_F) T on P's fixed pattern, W lanes per row
+// (1 or a wavefront), one lane per entry of P's row (column c) at a time:
+// the entry starts at 1 when c is the row's own aggregate, then takes -w dF
+// (diagonal) or -w a_ij (strong coupling) for every entry of A's row whose
+// column lies in aggregate c, in A's row order, w = omega / dF.
+template <int W>
 __global__ __launch_bounds__(kBlock) void k_num_prolongator(
     int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
     const unsigned char* __restrict__ strong, const int* __restrict__ agg,
@@ -137,33 +165,29 @@ __global__ __launch_bounds__(kBlock) void k_num_prolongator(
     const int* __restrict__ prp, const int* __restrict__ pci, double* __restrict__ pv)
 {
 #pragma clang fp contract(off)
-    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    const int lane = int(threadIdx.x) % W;
+    const int i = int(blockIdx.x) * (kBlock / W) + int(threadIdx.x) / W;
     if (i >= n)
         return;
     const double omega = omega_scale / lmax[0];
     const double df = dF[i];
     const double w = df != 0.0 ? omega / df : 0.0;
-    const int p0 = prp[i], p1 = prp[i + 1];
-    for (int q = p0; q < p1; ++q)
-        pv[q] = 0.0;
-    auto add = [&](int c, double x) {
-        for (int q = p0; q < p1; ++q)
-            if (pci[q] == c) {
-                pv[q] += x;
-                return;
-            }
-    };
-    if (agg[i] >= 0)
-        add(agg[i], 1.0);
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-        const int j = ci[k];
-        const int a = agg[j];
-        if (a < 0)
-            continue;
-        if (j == i)
-            add(a, -w * df);
-        else if (strong[k])
-            add(a, -w * v[k]);
+    const int own = agg[i];
+    for (int q = prp[i] + lane; q < prp[i + 1]; q += W) {
+        const int c = pci[q];
+        double acc = 0.0;
+        if (own == c)
+            acc += 1.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (agg[j] != c)
+                continue;
+            if (j == i)
+                acc += -w * df;
+            else if (strong[k])
+                acc += -w * v[k];
+        }
+        pv[q] = acc;
     }
 }
 
@@ -190,7 +214,9 @@ __device__ __forceinline__ int find_col(const int* __restrict__ ci, int b, int e
 }
 
 // Row i of C = A B into C's fixed pattern: for every entry a_ik of A's row in
-// order, for every entry b_kj of B's row k in order, c_ij += a_ik b_kj.
+// order, for every entry b_kj of B's row k, c_ij += a_ik b_kj -- for each
+// c_ij the terms in A's row order, as the host's row-wise product adds them.
+// One thread per row.
 __global__ __launch_bounds__(kBlock) void k_num_product(
     int n, const int* __restrict__ arp, const int* __restrict__ aci, const double* __restrict__ av,
     const int* __restrict__ brp, const int* __restrict__ bci, const double* __restrict__ bv,
@@ -213,6 +239,65 @@ __global__ __launch_bounds__(kBlock) void k_num_product(
                 cv[pos] += a * bv[q];
         }
     }
+}
+
+// Orders the LDS accesses of one wavefront's lanes (no workgroup barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The same product with one wavefront per row of C (rows of at most
+// kProductRow entries): the row's columns and sums sit in LDS; for each
+// entry a_ik in A's row order the lanes take the entries of B's row k (all
+// in different columns of C) at once, so every c_ij receives its terms in
+// the same order as above.
+constexpr int kProductRow = 512;
+
+__global__ __launch_bounds__(kBlock) void k_num_product_wave(
+    int n, const int* __restrict__ arp, const int* __restrict__ aci, const double* __restrict__ av,
+    const int* __restrict__ brp, const int* __restrict__ bci, const double* __restrict__ bv,
+    const int* __restrict__ crp, const int* __restrict__ cci, double* __restrict__ cv)
+{
+#pragma clang fp contract(off)
+    __shared__ int cols[kBlock / kWave][kProductRow];
+    __shared__ double sums[kBlock / kWave][kProductRow];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = int(threadIdx.x) / kWave;
+    const int i = int(blockIdx.x) * (kBlock / kWave) + w;
+    if (i >= n)
+        return;
+    const int c0 = crp[i];
+    const int L = crp[i + 1] - c0;
+    int* cw = cols[w];
+    double* sw = sums[w];
+    for (int t = lane; t < L; t += kWave) {
+        cw[t] = cci[c0 + t];
+        sw[t] = 0.0;
+    }
+    wave_sync();
+    for (int k = arp[i]; k < arp[i + 1]; ++k) {
+        const int r = aci[k];
+        const double a = av[k];
+        for (int q = brp[r] + lane; q < brp[r + 1]; q += kWave) {
+            const int c = bci[q];
+            int lo = 0, hi = L;
+            while (lo < hi) {
+                const int m = (lo + hi) >> 1;
+                if (cw[m] < c)
+                    lo = m + 1;
+                else
+                    hi = m;
+            }
+            if (lo < L && cw[lo] == c)
+                sw[lo] += a * bv[q];
+        }
+        wave_sync();
+    }
+    for (int t = lane; t < L; t += kWave)
+        cv[c0 + t] = sw[t];
 }
 
 __global__ __launch_bounds__(kBlock) void k_num_zero(int64_t m, double* __restrict__ x)
@@ -281,7 +366,13 @@ int AmgNumeric::record(const AmgHierarchyHost& H, const int* rp0, const int* ci0
         MI_HIP(hipStreamSynchronize(s)); // x0 goes out of scope
         for (DevBuf<double>* b : {&d.dF, &d.s, &d.y0, &d.y1})
             MI_TRY(b->alloc(n1));
-        d.pgrid = grid_for(d.n, kBlock);
+        {
+            const int64_t avg = d.n > 0 ? (d.nnz + d.n - 1) / d.n : 1;
+            d.lanes = 1;
+            while (d.lanes < 64 && d.lanes < avg)
+                d.lanes *= 2;
+        }
+        d.pgrid = std::min<unsigned>(grid_for(int64_t(d.n) * d.lanes, kBlock), kMaxReduceBlocks);
         MI_TRY(d.part.alloc(size_t(d.pgrid) * 3));
         if (l + 1 == nl)
             break;
@@ -326,6 +417,15 @@ int AmgNumeric::record(const AmgHierarchyHost& H, const int* rp0, const int* ci0
         {
             HCsr AP;
             csr_multiply(h.A, h.P, AP, 0);
+            // Below 64K rows a thread per row leaves the device mostly idle
+            // (measured at 96^3: level 2, 3584 rows, P 128 -> 33 us and A P
+            // 722 -> 70 us a wavefront per row; level 0, 884,736 rows, P
+            // 156 vs 911 us and A P 858 vs 1531 us a thread per row).  R (A P)
+            // has long rows at every level: a wavefront per row (level 0
+            // 2115 -> 962 us, level 1 7529 -> 307 us).
+            d.wave_rows = d.n < 65536;
+            d.ap_wave = d.wave_rows && max_row(AP.rp) <= kProductRow;
+            d.rap_wave = max_row(next.rp) <= kProductRow;
             MI_TRY(d.aprp.upload(AP.rp.data(), AP.rp.size(), s));
             MI_TRY(d.apci.upload(AP.ci.data(), AP.ci.size(), s));
             MI_TRY(d.apv.alloc(size_t(std::max<int64_t>(AP.nnz(), 1))));
@@ -362,8 +462,15 @@ int AmgNumeric::run(const double* a0, double omega_scale, hipStream_t s)
             const double* xin = d.x0.p;
             for (int it = 0; it < kIters; ++it) {
                 double* y = (it & 1) ? d.y1.p : d.y0.p;
-                k_num_power<<<d.pgrid, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.s.p, xin,
-                                                       it == 0 ? nullptr : sc, y, d.part.p);
+                auto kern = d.lanes >= 64 ? k_num_power<64>
+                          : d.lanes >= 32 ? k_num_power<32>
+                          : d.lanes >= 16 ? k_num_power<16>
+                          : d.lanes >= 8  ? k_num_power<8>
+                          : d.lanes >= 4  ? k_num_power<4>
+                          : d.lanes >= 2  ? k_num_power<2>
+                                          : k_num_power<1>;
+                kern<<<d.pgrid, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.s.p, xin,
+                                                it == 0 ? nullptr : sc, y, d.part.p);
                 k_num_power_fin<<<1, kBlock, 0, s>>>(d.part.p, int(d.pgrid), it + 1 == kIters, sc);
                 xin = y;
             }
@@ -372,17 +479,32 @@ int AmgNumeric::run(const double* a0, double omega_scale, hipStream_t s)
         if (last)
             break;
         NumLevel& nx = L[size_t(l) + 1];
-        k_num_prolongator<<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p, d.agg.p,
-                                               d.dF.p, sc + 2, omega_scale, d.prp.p, d.pci.p,
-                                               d.pv.p);
+        const unsigned gw = grid_for(d.n, kBlock / kWave); // a wavefront per row
+        if (d.wave_rows)
+            k_num_prolongator<kWave><<<gw, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
+                                                           d.agg.p, d.dF.p, sc + 2, omega_scale,
+                                                           d.prp.p, d.pci.p, d.pv.p);
+        else
+            k_num_prolongator<1><<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
+                                                      d.agg.p, d.dF.p, sc + 2, omega_scale,
+                                                      d.prp.p, d.pci.p, d.pv.p);
         if (d.r_nnz > 0)
             k_num_gather<<<reduce_grid(d.r_nnz, 4), kBlock, 0, s>>>(d.r_nnz, d.rmap.p, d.pv.p,
                                                                    d.rv.p);
-        k_num_product<<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.prp.p, d.pci.p, d.pv.p,
-                                           d.aprp.p, d.apci.p, d.apv.p);
-        k_num_product<<<grid_for(nx.n, kBlock), kBlock, 0, s>>>(nx.n, d.rrp.p, d.rci.p, d.rv.p,
-                                                                d.aprp.p, d.apci.p, d.apv.p,
-                                                                nx.arp, nx.aci, nx.av.p);
+        if (d.ap_wave)
+            k_num_product_wave<<<gw, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.prp.p, d.pci.p,
+                                                     d.pv.p, d.aprp.p, d.apci.p, d.apv.p);
+        else
+            k_num_product<<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.prp.p, d.pci.p,
+                                               d.pv.p, d.aprp.p, d.apci.p, d.apv.p);
+        if (d.rap_wave)
+            k_num_product_wave<<<grid_for(nx.n, kBlock / kWave), kBlock, 0, s>>>(
+                nx.n, d.rrp.p, d.rci.p, d.rv.p, d.aprp.p, d.apci.p, d.apv.p, nx.arp, nx.aci,
+                nx.av.p);
+        else
+            k_num_product<<<grid_for(nx.n, kBlock), kBlock, 0, s>>>(
+                nx.n, d.rrp.p, d.rci.p, d.rv.p, d.aprp.p, d.apci.p, d.apv.p, nx.arp, nx.aci,
+                nx.av.p);
         MI_HIP(hipGetLastError());
     }
     // the coarsest operator, dense
