@@ -364,21 +364,41 @@ typedef struct {
  *   PRESSURE        PressureInlet/Outlet: Dirichlet p' = 0; outflow for U
  *   SYMMETRY        Symmetry: no contribution
  *   VELOCITY_INLET  VelocityInlet, MovingWall: U = zone value
- *   OUTFLOW         every other type and zones without a BoundarySpec */
+ *   OUTFLOW         every other type and zones without a BoundarySpec
+ *   PERIODIC        Periodic: faces paired with a face of another periodic
+ *                   zone couple their cells (an unpaired face acts as OUTFLOW) */
 enum { MI_BC_WALL = 0, MI_BC_PRESSURE = 1, MI_BC_SYMMETRY = 2, MI_BC_VELOCITY_INLET = 3,
-       MI_BC_OUTFLOW = 4 };
+       MI_BC_OUTFLOW = 4, MI_BC_PERIODIC = 5 };
 
 typedef struct mi_mesh mi_mesh;
 /* Uploads the mesh and builds the CSR pattern of SimpleAlgorithm::build_sparsity
- * (src/solver/SimpleAlgorithm.cpp:49-98; periodic twins not supported).
- * bc_of_zone[z] is the MI_BC_* class of zone z (zones >= n_zones: OUTFLOW). */
+ * (src/solver/SimpleAlgorithm.cpp:49-98) with the twin columns of periodic
+ * faces.  bc_of_zone[z] is the MI_BC_* class of zone z (zones >= n_zones:
+ * OUTFLOW).  Periodic zones are paired as build_periodic_pairs
+ * (:101-186) pairs them: in ascending zone id two by two (the order of a
+ * BoundarySpec list given in zone order), each face of the first zone with
+ * the nearest face of the second after the mean translation. */
 MI_API int mi_mesh_create(const mi_mesh_soa* mesh, const int32_t* bc_of_zone, int n_zones,
                           mi_mesh** out);
+/* mi_mesh_create with the caller's twin table instead of the pairing:
+ * twin_faces[f] = the face paired with boundary face f, or -1
+ * (SimpleAlgorithm::set_periodic_pairs, SimpleAlgorithm.hpp:84-92; e.g. the
+ * rotational pairs of PeriodicPairing.hpp).  Entries of interior faces are
+ * ignored. */
+MI_API int mi_mesh_create_twins(const mi_mesh_soa* mesh, const int32_t* bc_of_zone, int n_zones,
+                                const int32_t* twin_faces, mi_mesh** out);
+/* SIMPLEC denominators den[c] = max(aP[c] - sum_{k != c} |A_mom[c][k]|, 1e-30)
+ * (src/solver/SimpleAlgorithm.cpp:378-387, 400-409) from a momentum matrix on
+ * the mesh's pattern; mi_assemble_pressure with aP = den is the SIMPLEC
+ * assembly.  Device arrays of n_cells. */
+MI_API int mi_simplec_denominators(mi_mesh* m, const double* aP_device, const mi_csr* A_mom,
+                                   double* den_device);
 MI_API int mi_mesh_info(const mi_mesh* m, int64_t* n_cells, int64_t* n_faces, int64_t* nnz);
 /* A zero-valued matrix on the mesh's pattern (A_p_ / A_mom_ storage). */
 MI_API int mi_mesh_create_matrix(mi_mesh* m, mi_csr** out);
-/* SimpleAlgorithm::assemble_pressure_correction (src/solver/SimpleAlgorithm.cpp:358-447),
- * SIMPLE denominators.  Device arrays: aP[n_cells] (nullable: aP_f = 1),
+/* SimpleAlgorithm::assemble_pressure_correction (src/solver/SimpleAlgorithm.cpp:358-447)
+ * with its periodic branch (:411-419); SIMPLE denominators, or SIMPLEC with
+ * aP = mi_simplec_denominators.  Device arrays: aP[n_cells] (nullable: aP_f = 1),
  * gamma[n_faces] (nullable: 1) scaling each face coefficient
  * rho*gamma_f*|S|^2/max(aP_f*|d.S|,1e-30), phi[n_faces] (nullable: rhs 0).
  * Writes A's values and rhs[n_cells]; with no pressure-zone face the system
@@ -401,16 +421,18 @@ MI_API int mi_assemble_momentum(mi_mesh* m, int comp, double mu, double rho,
  * mi_assemble_pressure with face scales gamma: grad[f] = dJ/dgamma_f =
  * -lambda^T (dA/dgamma_f) x, where A^T lambda = dJ/dx (mi_adjoint_solve).
  * An interior face adds gamma_f G_f (e_o - e_n)(e_o - e_n)^T to A, a
- * pressure-boundary face gamma_f G_f e_o e_o^T, G_f = rho |S_f|^2 /
+ * pressure-boundary face gamma_f G_f e_o e_o^T, a periodic face
+ * gamma_f G_f e_o (e_o - e_t)^T (t: the twin's cell), G_f = rho |S_f|^2 /
  * max(aP_f |d_f . S_f|, 1e-30) (src/solver/SimpleAlgorithm.cpp:373-397,
- * 420-431); with `pinned` the pinned cell's diagonal carries the factor 1e6
+ * 411-431); with `pinned` the pinned cell's diagonal carries the factor 1e6
  * (:441-446).  Other faces get 0.  The gradient step
  * dJ/dalpha - (dR/dalpha)^T psi of src/adjoint/DiscreteAdjoint.cpp:170-182.
  * Device arrays: aP[n_cells] (nullable), x, lambda [n_cells], grad[n_faces]. */
 MI_API int mi_face_gradient(mi_mesh* m, double rho, const double* aP_device, int pinned,
                             const double* x_device, const double* lambda_device,
                             double* grad_device);
-/* SimpleAlgorithm::compute_face_fluxes (src/solver/SimpleAlgorithm.cpp:201-237). */
+/* SimpleAlgorithm::compute_face_fluxes (src/solver/SimpleAlgorithm.cpp:201-237),
+ * periodic faces averaging with the twin's cell. */
 MI_API int mi_face_flux(mi_mesh* m, double rho, const double* Ux_device, const double* Uy_device,
                         const double* Uz_device, const double* bc_value_host, double* phi_device);
 /* mi_face_flux that allocates nothing, for callers that cannot allocate
@@ -452,12 +474,14 @@ MI_API int mi_correct_fields(mi_mesh* m, double urfP, double rho, const double* 
  * times (n_nonorth + 1) pressure passes after the predictor).  Solves start
  * from x = 0 as the reference's CPU solvers do.  Fields start at zero; write
  * initial values through the pointers of mi_flow_fields. */
-enum { MI_FLOW_SIMPLE = 0, MI_FLOW_PISO = 1 };
+enum { MI_FLOW_SIMPLE = 0, MI_FLOW_PISO = 1,
+       MI_FLOW_SIMPLEC = 2 /* SimplecAlgorithm (src/solver/SimplecAlgorithm.hpp): SIMPLE with
+                              the denominators aP - sum |a_nb| in the pressure assembly */ };
 enum { MI_TIME_EULER = 0, MI_TIME_BDF2 = 1 }; /* TemporalScheme ImplicitEuler, BDF2 */
 typedef struct {
-    int algorithm;       /* MI_FLOW_SIMPLE or MI_FLOW_PISO */
+    int algorithm;       /* MI_FLOW_SIMPLE, MI_FLOW_PISO or MI_FLOW_SIMPLEC */
     double rho, mu;      /* density, dynamic viscosity (1, 1e-3) */
-    double urfU, urfP;   /* under-relaxation: SIMPLE 0.7, 0.3; PISO 1, 1 */
+    double urfU, urfP;   /* under-relaxation: SIMPLE 0.7, 0.3; PISO 1, 1; SIMPLEC 0.9, 1 */
     double dt;           /* > 0: transient term; SIMPLE 0 (steady), PISO 1e-3 */
     int time_scheme;     /* MI_TIME_EULER (SIMPLE) or MI_TIME_BDF2 (PISO); BDF2 from
                             the second time step on, implicit Euler before */

@@ -101,6 +101,15 @@ def load():
             "orc_correct_fields": (None, [P(OrcMesh), d, d, vp, vp, vp, vp, vp, vp, vp]),
             "orc_face_flux": (None, [P(OrcMesh), vp, i, vp, d, vp, vp, vp, vp]),
             "orc_face_gradient": (None, [P(OrcMesh), vp, i, d, vp, i, vp, vp, vp]),
+            "orc_periodic_twins": (None, [P(OrcMesh), vp, i, vp]),
+            "orc_simplec_denominators": (None, [P(OrcCsr), vp, vp]),
+            "orc_build_sparsity_p": (i64, [P(OrcMesh), vp, vp, vp]),
+            "orc_assemble_pressure_p": (i, [P(OrcMesh), vp, i, vp, d, vp, vp, vp, P(OrcCsr), vp,
+                                            vp]),
+            "orc_assemble_momentum_p": (None, [P(OrcMesh), vp, i, vp, vp, i, d, d, vp, vp, vp, vp,
+                                               vp, vp, d, P(OrcCsr), vp, vp]),
+            "orc_face_flux_p": (None, [P(OrcMesh), vp, i, vp, vp, d, vp, vp, vp, vp]),
+            "orc_face_gradient_p": (None, [P(OrcMesh), vp, i, vp, d, vp, i, vp, vp, vp]),
             "orc_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
             "orc_uniform": (d, [P(C.c_uint64)]),
             "orc_normal": (d, [P(C.c_uint64)]),
@@ -289,49 +298,81 @@ def mesh_struct(mesh):
     return s
 
 
-def build_sparsity(mesh):
+BC_PERIODIC = 5
+
+
+def _twin(twin):
+    return None if twin is None else np.ascontiguousarray(twin, dtype=np.int32)
+
+
+def periodic_twins(mesh, bc_of_zone):
+    """SimpleAlgorithm::build_periodic_pairs (SimpleAlgorithm.cpp:101-186):
+    twin[f] = the paired face of a periodic boundary face, else -1."""
     ms = mesh_struct(mesh)
+    bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    twin = np.empty(mesh.n_faces, np.int32)
+    load().orc_periodic_twins(C.byref(ms), _p(bc), bc.size, _p(twin))
+    return twin
+
+
+def simplec_denominators(pattern, amom_values, aP):
+    """max(aP_c - sum_{k != c} |A_mom[c, k]|, 1e-30) (SimpleAlgorithm.cpp:378-387)."""
+    A = Matrix(pattern.rowPtr, pattern.colIdx, amom_values)
+    den = np.empty(pattern.n)
+    load().orc_simplec_denominators(C.byref(A.s), _p(_vec(aP)), _p(den))
+    return den
+
+
+def build_sparsity(mesh, twin=None):
+    ms = mesh_struct(mesh)
+    tw = _twin(twin)
     rp = np.empty(mesh.n_cells + 1, np.int32)
-    nnz = load().orc_build_sparsity(C.byref(ms), _p(rp), None)
+    nnz = load().orc_build_sparsity_p(C.byref(ms), _p(tw), _p(rp), None)
     ci = np.empty(nnz, np.int32)
-    load().orc_build_sparsity(C.byref(ms), _p(rp), _p(ci))
+    load().orc_build_sparsity_p(C.byref(ms), _p(tw), _p(rp), _p(ci))
     return rp, ci
 
 
-def assemble_pressure(mesh, pattern, bc_of_zone, rho=1.0, aP=None, gamma=None, phi=None):
+def assemble_pressure(mesh, pattern, bc_of_zone, rho=1.0, aP=None, gamma=None, phi=None,
+                      twin=None):
     """-> (values, rhs, pinned)."""
     ms = mesh_struct(mesh)
     bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    tw = _twin(twin)
     vals = np.empty(pattern.colIdx.size)
     rhs = np.empty(mesh.n_cells)
-    pinned = load().orc_assemble_pressure(C.byref(ms), _p(bc), bc.size, rho, _p(aP), _p(gamma), _p(phi),
-                                          C.byref(pattern.s), _p(vals), _p(rhs))
+    pinned = load().orc_assemble_pressure_p(C.byref(ms), _p(bc), bc.size, _p(tw), rho, _p(aP),
+                                            _p(gamma), _p(phi), C.byref(pattern.s), _p(vals),
+                                            _p(rhs))
     return vals, rhs, bool(pinned)
 
 
-def face_gradient(mesh, bc_of_zone, x, lam, pinned, rho=1.0, aP=None):
+def face_gradient(mesh, bc_of_zone, x, lam, pinned, rho=1.0, aP=None, twin=None):
     """dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for every face (pressure matrix)."""
     ms = mesh_struct(mesh)
     bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    tw = _twin(twin)
     x = _vec(x)
     lam = _vec(lam)
     aP = None if aP is None else _vec(aP)
     grad = np.empty(mesh.n_faces)
-    load().orc_face_gradient(C.byref(ms), _p(bc), bc.size, rho, _p(aP), int(pinned), _p(x),
-                             _p(lam), _p(grad))
+    load().orc_face_gradient_p(C.byref(ms), _p(bc), bc.size, _p(tw), rho, _p(aP), int(pinned),
+                               _p(x), _p(lam), _p(grad))
     return grad
 
 
 def assemble_momentum(mesh, pattern, bc_of_zone, comp, mu, rho, phi, dt=None, Uold=None,
-                      Ucomp=None, urf=1.0, bc_value=None, pgradV=None, Unm1=None):
+                      Ucomp=None, urf=1.0, bc_value=None, pgradV=None, Unm1=None, twin=None):
     ms = mesh_struct(mesh)
     bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    tw = _twin(twin)
     vals = np.empty(pattern.colIdx.size)
     rhs = np.empty(mesh.n_cells)
     bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
     opt = [None if a is None else _vec(a) for a in (pgradV, dt, Uold, Unm1, Ucomp)]
-    load().orc_assemble_momentum(C.byref(ms), _p(bc), bc.size, _p(bv), comp, mu, rho, _p(_vec(phi)),
-                                 *(_p(a) for a in opt), urf, C.byref(pattern.s), _p(vals), _p(rhs))
+    load().orc_assemble_momentum_p(C.byref(ms), _p(bc), bc.size, _p(tw), _p(bv), comp, mu, rho,
+                                   _p(_vec(phi)), *(_p(a) for a in opt), urf, C.byref(pattern.s),
+                                   _p(vals), _p(rhs))
     return vals, rhs
 
 
@@ -364,15 +405,18 @@ class Flow:
     def __init__(self, mesh, bc_of_zone, bc_value=None, algorithm="simple", rho=1.0, mu=1e-3,
                  urfU=None, urfP=None, dt=None, bdf2=None, n_correctors=2, n_nonorth=0):
         piso = algorithm == "piso"
+        self.simplec = algorithm == "simplec"
         self.mesh, self.bc, self.bc_value = mesh, list(bc_of_zone), bc_value
         self.piso, self.rho, self.mu = piso, rho, mu
-        self.urfU = (1.0 if piso else 0.7) if urfU is None else urfU
-        self.urfP = (1.0 if piso else 0.3) if urfP is None else urfP
+        # SimpleOptions / PisoOptions / SimplecOptions defaults
+        self.urfU = (1.0 if piso else 0.9 if self.simplec else 0.7) if urfU is None else urfU
+        self.urfP = (1.0 if piso or self.simplec else 0.3) if urfP is None else urfP
         self.dt = (1e-3 if piso else 0.0) if dt is None else dt
         self.bdf2 = piso if bdf2 is None else bdf2
         self.n_correctors, self.n_nonorth = max(1, n_correctors), n_nonorth
         n = mesh.n_cells
-        rp, ci = build_sparsity(mesh)
+        self.twin = periodic_twins(mesh, self.bc) if BC_PERIODIC in self.bc else None
+        rp, ci = build_sparsity(mesh, self.twin)
         self.pattern = Matrix(rp, ci, np.zeros(ci.size))
         self.U = [np.zeros(n) for _ in range(3)]
         self.Un = [np.zeros(n) for _ in range(3)]
@@ -391,7 +435,7 @@ class Flow:
 
     def _momentum(self, res):
         m = self.mesh
-        self.phi = face_flux(m, self.bc, self.rho, self.U, self.bc_value)
+        self.phi = face_flux(m, self.bc, self.rho, self.U, self.bc_value, self.twin)
         g = ls_gradient(m, self.p)
         dtc = np.full(m.n_cells, self.dt) if self.dt > 0 else None
         use_bdf2 = self.dt > 0 and self.bdf2 and self.step >= 1
@@ -400,16 +444,19 @@ class Flow:
                                           dt=dtc, Uold=self.Un[k] if dtc is not None else None,
                                           Ucomp=self.U[k], urf=self.urfU, bc_value=self.bc_value,
                                           pgradV=g[k] * m.volume,
-                                          Unm1=self.Unm1[k] if use_bdf2 else None)
+                                          Unm1=self.Unm1[k] if use_bdf2 else None, twin=self.twin)
             if k == 0:
                 self.aP = vals[self.pattern.diag_positions()].copy()
+            self.amom = vals
             x, A = self._solve(vals, rhs)
             self.U[k] = x
             res["mom"][k] = float(np.linalg.norm(A @ x - rhs))
 
     def _pressure(self, res):
         m = self.mesh
-        vals, rhs, _ = assemble_pressure(m, self.pattern, self.bc, self.rho, aP=self.aP, phi=self.phi)
+        den = simplec_denominators(self.pattern, self.amom, self.aP) if self.simplec else self.aP
+        vals, rhs, _ = assemble_pressure(m, self.pattern, self.bc, self.rho, aP=den, phi=self.phi,
+                                         twin=self.twin)
         pp, _ = self._solve(vals, rhs)
         res["cont"] = float(np.linalg.norm(rhs))
         self.p, U, self.phi = correct_fields(m, self.urfP, self.rho, self.aP, pp, self.p, self.U,
@@ -421,11 +468,12 @@ class Flow:
         self._momentum(res)
         if self.piso:
             for _ in range(self.n_correctors):
-                self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value)
+                self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value,
+                                     self.twin)
                 for _ in range(self.n_nonorth + 1):
                     self._pressure(res)
         else:
-            self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value)
+            self.phi = face_flux(self.mesh, self.bc, self.rho, self.U, self.bc_value, self.twin)
             self._pressure(res)
         return res
 
@@ -440,11 +488,13 @@ class Flow:
         return res
 
 
-def face_flux(mesh, bc_of_zone, rho, U, bc_value=None):
+def face_flux(mesh, bc_of_zone, rho, U, bc_value=None, twin=None):
     ms = mesh_struct(mesh)
     bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
+    tw = _twin(twin)
     phi = np.empty(mesh.n_faces)
     bv = None if bc_value is None else np.ascontiguousarray(bc_value, dtype=np.float64)
     Ux, Uy, Uz = (_vec(u) for u in U)
-    load().orc_face_flux(C.byref(ms), _p(bc), bc.size, _p(bv), rho, _p(Ux), _p(Uy), _p(Uz), _p(phi))
+    load().orc_face_flux_p(C.byref(ms), _p(bc), bc.size, _p(tw), _p(bv), rho, _p(Ux), _p(Uy),
+                           _p(Uz), _p(phi))
     return phi

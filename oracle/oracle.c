@@ -8,6 +8,7 @@
  */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -804,8 +805,91 @@ static int cmp_int(const void* a, const void* b)
     return (x > y) - (x < y);
 }
 
+/* SimpleAlgorithm::build_periodic_pairs, src/solver/SimpleAlgorithm.cpp:101-186:
+ * periodic zones (ascending zone id, the order of a BoundarySpec list in zone
+ * order) paired two by two; translation t = mean centroid of the second
+ * zone's faces - mean of the first's; every face fa of the first zone is
+ * paired with the first face fb of the second zone (face order) of least
+ * |c_fb - (c_fa + t)|^2, and twin[fb] = fa (a later fa overwrites). */
+void orc_periodic_twins(const orc_mesh* m, const int* bcOfZone, int nZones, int* twin)
+{
+    for (int64_t f = 0; f < m->n_faces; ++f)
+        twin[f] = -1;
+    int zones[64];
+    int nz = 0;
+    for (int z = 0; z < nZones && nz < 64; ++z)
+        if (bcOfZone[z] == ORC_BC_PERIODIC)
+            zones[nz++] = z;
+    if (nz < 2)
+        return;
+    int64_t* list = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m->n_faces > 0 ? m->n_faces : 1));
+    int64_t* listB = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m->n_faces > 0 ? m->n_faces : 1));
+    for (int zi = 0; zi + 1 < nz; zi += 2) {
+        int64_t na = 0, nb = 0;
+        for (int64_t f = 0; f < m->n_faces; ++f) {
+            if (m->neighbor[f] != UINT64_MAX)
+                continue;
+            if ((int)m->boundaryZone[f] == zones[zi])
+                list[na++] = f;
+            else if ((int)m->boundaryZone[f] == zones[zi + 1])
+                listB[nb++] = f;
+        }
+        if (na == 0 || nb == 0)
+            continue;
+        double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+        for (int64_t i = 0; i < na; ++i) {
+            ax += m->faceCx[list[i]];
+            ay += m->faceCy[list[i]];
+            az += m->faceCz[list[i]];
+        }
+        for (int64_t i = 0; i < nb; ++i) {
+            bx += m->faceCx[listB[i]];
+            by += m->faceCy[listB[i]];
+            bz += m->faceCz[listB[i]];
+        }
+        const double ia = 1.0 / (double)na, ib = 1.0 / (double)nb;
+        const double tx = bx * ib - ax * ia, ty = by * ib - ay * ia, tz = bz * ib - az * ia;
+        for (int64_t i = 0; i < na; ++i) {
+            const int64_t fa = list[i];
+            int64_t best = -1;
+            double bestd2 = DBL_MAX;
+            const double px = m->faceCx[fa] + tx, py = m->faceCy[fa] + ty,
+                         pz = m->faceCz[fa] + tz;
+            for (int64_t j = 0; j < nb; ++j) {
+                const int64_t fb = listB[j];
+                const double d2 = (m->faceCx[fb] - px) * (m->faceCx[fb] - px)
+                                  + (m->faceCy[fb] - py) * (m->faceCy[fb] - py)
+                                  + (m->faceCz[fb] - pz) * (m->faceCz[fb] - pz);
+                if (d2 < bestd2) {
+                    bestd2 = d2;
+                    best = fb;
+                }
+            }
+            if (best >= 0) {
+                twin[fa] = (int)best;
+                twin[best] = (int)fa;
+            }
+        }
+    }
+    free(list);
+    free(listB);
+}
+
+/* The cell across boundary face f's periodic twin, or UINT64_MAX. */
+static uint64_t twin_cell(const orc_mesh* m, const int* twin, int64_t f)
+{
+    if (!twin || twin[f] < 0)
+        return UINT64_MAX;
+    return m->owner[twin[f]];
+}
+
 /* SimpleAlgorithm::build_sparsity, src/solver/SimpleAlgorithm.cpp:49-98 */
 int64_t orc_build_sparsity(const orc_mesh* m, int* rowPtr, int* colIdx)
+{
+    return orc_build_sparsity_p(m, NULL, rowPtr, colIdx);
+}
+
+int64_t orc_build_sparsity_p(const orc_mesh* m, const int* twin, int* rowPtr, int* colIdx)
 {
     const int64_t nC = m->n_cells;
     int buf[64];
@@ -820,6 +904,8 @@ int64_t orc_build_sparsity(const orc_mesh* m, int* rowPtr, int* colIdx)
             const uint64_t oth = (m->owner[f] == (uint64_t)c) ? m->neighbor[f] : m->owner[f];
             if (oth != UINT64_MAX && cnt < 64)
                 buf[cnt++] = (int)oth;
+            else if (oth == UINT64_MAX && cnt < 64 && twin_cell(m, twin, (int64_t)f) != UINT64_MAX)
+                buf[cnt++] = (int)twin_cell(m, twin, (int64_t)f);
         }
         qsort(buf, (size_t)cnt, sizeof(int), cmp_int);
         int u = 0;
@@ -859,6 +945,30 @@ int orc_assemble_pressure(const orc_mesh* m, const int* bcOfZone, int nZones,
                           const double* phi, const orc_csr* pat,
                           double* values, double* rhs)
 {
+    return orc_assemble_pressure_p(m, bcOfZone, nZones, NULL, rho, aP, gamma, phi, pat, values,
+                                   rhs);
+}
+
+/* The SIMPLEC denominator of every cell (src/solver/SimpleAlgorithm.cpp:378-387,
+ * 400-409): max(aP_c - sum over row c's off-diagonal entries of |a|, 1e-30),
+ * from the momentum matrix on the pattern. */
+void orc_simplec_denominators(const orc_csr* amom, const double* aP, double* den)
+{
+    for (int64_t c = 0; c < amom->n; ++c) {
+        double sumOff = 0.0;
+        for (int k = amom->rowPtr[c]; k < amom->rowPtr[c + 1]; ++k)
+            if (amom->colIdx[k] != (int)c)
+                sumOff += fabs(amom->values[k]);
+        const double d = aP[c] - sumOff;
+        den[c] = d > 1e-30 ? d : 1e-30;
+    }
+}
+
+int orc_assemble_pressure_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                            double rho, const double* aP, const double* gamma,
+                            const double* phi, const orc_csr* pat,
+                            double* values, double* rhs)
+{
     const int64_t nC = m->n_cells;
     const int64_t nnz = pat->rowPtr[nC];
     int anyDirichlet = 0;
@@ -891,7 +1001,9 @@ int orc_assemble_pressure(const orc_mesh* m, const int* bcOfZone, int nZones,
             add_at(pat, values, (int)nb, (int)o, -coef);
         } else {
             const int bc = zone_bc(bcOfZone, nZones, m->boundaryZone[f]);
-            if (bc == ORC_BC_PRESSURE) {
+            const uint64_t tw = twin_cell(m, twin, f);
+            if (tw != UINT64_MAX || bc == ORC_BC_PRESSURE) {
+                /* periodic (:411-419) or Dirichlet p' = 0 (:420-431) */
                 const double dx = m->faceCx[f] - m->cellCx[o];
                 const double dy = m->faceCy[f] - m->cellCy[o];
                 const double dz = m->faceCz[f] - m->cellCz[o];
@@ -900,7 +1012,10 @@ int orc_assemble_pressure(const orc_mesh* m, const int* bcOfZone, int nZones,
                 const double den = dO * ddotA;
                 const double coef = rho * g * Amag2 / (den > 1e-30 ? den : 1e-30);
                 add_at(pat, values, (int)o, (int)o, coef);
-                anyDirichlet = 1;
+                if (tw != UINT64_MAX)
+                    add_at(pat, values, (int)o, (int)tw, -coef);
+                else
+                    anyDirichlet = 1;
             }
         }
         if (phi) {
@@ -930,6 +1045,13 @@ void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, doubl
                        const double* aP, int pinned, const double* x, const double* lam,
                        double* grad)
 {
+    orc_face_gradient_p(m, bcOfZone, nZones, NULL, rho, aP, pinned, x, lam, grad);
+}
+
+void orc_face_gradient_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                         double rho, const double* aP, int pinned, const double* x,
+                         const double* lam, double* grad)
+{
     for (int64_t f = 0; f < m->n_faces; ++f) {
         const uint64_t o = m->owner[f];
         const uint64_t nb = m->neighbor[f];
@@ -957,7 +1079,8 @@ void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, doubl
             rows[2] = (int)nb; cols[2] = (int)nb; dv[2] = dcoef;
             rows[3] = (int)nb; cols[3] = (int)o;  dv[3] = -dcoef;
             ne = 4;
-        } else if (zone_bc(bcOfZone, nZones, m->boundaryZone[f]) == ORC_BC_PRESSURE) {
+        } else if (twin_cell(m, twin, f) != UINT64_MAX
+                   || zone_bc(bcOfZone, nZones, m->boundaryZone[f]) == ORC_BC_PRESSURE) {
             const double dx = m->faceCx[f] - m->cellCx[o];
             const double dy = m->faceCy[f] - m->cellCy[o];
             const double dz = m->faceCz[f] - m->cellCz[o];
@@ -967,6 +1090,11 @@ void orc_face_gradient(const orc_mesh* m, const int* bcOfZone, int nZones, doubl
             rows[0] = (int)o; cols[0] = (int)o;
             dv[0] = rho * Amag2 / (den > 1e-30 ? den : 1e-30);
             ne = 1;
+            const uint64_t tw = twin_cell(m, twin, f);
+            if (tw != UINT64_MAX) {
+                rows[1] = (int)o; cols[1] = (int)tw; dv[1] = -dv[0];
+                ne = 2;
+            }
         }
         double t = 0.0;
         for (int e = 0; e < ne; ++e) {
@@ -986,6 +1114,16 @@ void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,
                            const double* phi, const double* pgradV, const double* dtCell,
                            const double* Uold, const double* Unm1, const double* Ucomp,
                            double urfU, const orc_csr* pat, double* values, double* rhs)
+{
+    orc_assemble_momentum_p(m, bcOfZone, nZones, NULL, bcValue, comp, mu, rho, phi, pgradV,
+                            dtCell, Uold, Unm1, Ucomp, urfU, pat, values, rhs);
+}
+
+void orc_assemble_momentum_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                             const double* bcValue, int comp, double mu, double rho,
+                             const double* phi, const double* pgradV, const double* dtCell,
+                             const double* Uold, const double* Unm1, const double* Ucomp,
+                             double urfU, const orc_csr* pat, double* values, double* rhs)
 {
     const int64_t nC = m->n_cells;
     const int64_t nnz = pat->rowPtr[nC];
@@ -1018,7 +1156,12 @@ void orc_assemble_momentum(const orc_mesh* m, const int* bcOfZone, int nZones,
             const double dz = m->faceCz[f] - m->cellCz[o];
             const double ddotA = fabs(dx * Ax + dy * Ay + dz * Az);
             const double D = mu * Amag2 / (ddotA > 1e-30 ? ddotA : 1e-30);
-            if (bc == ORC_BC_WALL || bc == ORC_BC_VELOCITY_INLET) {
+            const uint64_t tw = twin_cell(m, twin, f);
+            if (tw != UINT64_MAX) {
+                /* periodic: the owner's half of an interior coupling (:279-285) */
+                add_at(pat, values, (int)o, (int)o, ConvO + D);
+                add_at(pat, values, (int)o, (int)tw, -(ConvN + D));
+            } else if (bc == ORC_BC_WALL || bc == ORC_BC_VELOCITY_INLET) {
                 const double Ub = (bc == ORC_BC_WALL)
                                       ? 0.0
                                       : (bcValue ? bcValue[(size_t)m->boundaryZone[f] * 3 + (size_t)comp] : 0.0);
@@ -1068,6 +1211,13 @@ void orc_face_flux(const orc_mesh* m, const int* bcOfZone, int nZones,
                    const double* bcValue, double rho, const double* Ux,
                    const double* Uy, const double* Uz, double* phi)
 {
+    orc_face_flux_p(m, bcOfZone, nZones, NULL, bcValue, rho, Ux, Uy, Uz, phi);
+}
+
+void orc_face_flux_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                     const double* bcValue, double rho, const double* Ux, const double* Uy,
+                     const double* Uz, double* phi)
+{
     for (int64_t f = 0; f < m->n_faces; ++f) {
         const uint64_t o = m->owner[f];
         const uint64_t nb = m->neighbor[f];
@@ -1076,6 +1226,11 @@ void orc_face_flux(const orc_mesh* m, const int* bcOfZone, int nZones,
             ux = 0.5 * (Ux[o] + Ux[nb]);
             uy = 0.5 * (Uy[o] + Uy[nb]);
             uz = 0.5 * (Uz[o] + Uz[nb]);
+        } else if (twin_cell(m, twin, f) != UINT64_MAX) {
+            const uint64_t tw = twin_cell(m, twin, f); /* :214-218 */
+            ux = 0.5 * (Ux[o] + Ux[tw]);
+            uy = 0.5 * (Uy[o] + Uy[tw]);
+            uz = 0.5 * (Uz[o] + Uz[tw]);
         } else {
             const uint32_t z = m->boundaryZone[f];
             const int bc = zone_bc(bcOfZone, nZones, z);

@@ -125,8 +125,35 @@ enum {
     ORC_BC_PRESSURE = 1,         /* PressureInlet/Outlet: Dirichlet p'=0; outflow for U */
     ORC_BC_SYMMETRY = 2,
     ORC_BC_VELOCITY_INLET = 3,   /* fixed U = value */
-    ORC_BC_OUTFLOW = 4
+    ORC_BC_OUTFLOW = 4,
+    ORC_BC_PERIODIC = 5          /* Periodic: paired with a face of another periodic zone */
 };
+
+/* Periodic face pairs of SimpleAlgorithm::build_periodic_pairs
+ * (src/solver/SimpleAlgorithm.cpp:101-186): twin[f] = paired face or -1. */
+void orc_periodic_twins(const orc_mesh* m, const int* bcOfZone, int nZones, int* twin);
+/* SIMPLEC denominators max(aP_c - sum_{k != c} |a_ck|, 1e-30) of the momentum
+ * matrix (src/solver/SimpleAlgorithm.cpp:378-387); the pressure assembly with
+ * aP = den is the SIMPLEC assembly. */
+void orc_simplec_denominators(const orc_csr* amom, const double* aP, double* den);
+/* The functions below with a periodic twin table (NULL: none); the names
+ * without _p pass NULL. */
+int64_t orc_build_sparsity_p(const orc_mesh* m, const int* twin, int* rowPtr, int* colIdx);
+int orc_assemble_pressure_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                            double rho, const double* aP, const double* gamma,
+                            const double* phi, const orc_csr* pattern,
+                            double* values, double* rhs);
+void orc_face_gradient_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                         double rho, const double* aP, int pinned, const double* x,
+                         const double* lam, double* grad);
+void orc_assemble_momentum_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                             const double* bcValue, int comp, double mu, double rho,
+                             const double* phi, const double* pgradV, const double* dtCell,
+                             const double* Uold, const double* Unm1, const double* Ucomp,
+                             double urfU, const orc_csr* pattern, double* values, double* rhs);
+void orc_face_flux_p(const orc_mesh* m, const int* bcOfZone, int nZones, const int* twin,
+                     const double* bcValue, double rho, const double* Ux, const double* Uy,
+                     const double* Uz, double* phi);
 
 /* CSR pattern: row c = {c} U face neighbours, sorted, unique
  * (src/solver/SimpleAlgorithm.cpp:49-98, no periodic twins).

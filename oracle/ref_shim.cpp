@@ -94,9 +94,11 @@ class Probe final : public solver::SimpleAlgorithm
 public:
     using SimpleAlgorithm::SimpleAlgorithm;
     void sparsity() { build_sparsity(); }
-    void pressure(const double* aP, const double* phi)
+    void pressure(const double* aP, const double* phi, const double* amom = nullptr)
     {
         std::memcpy(aP_.data(), aP, aP_.size() * sizeof(double));
+        if (amom)
+            std::memcpy(A_mom_.values.data(), amom, A_mom_.values.size() * sizeof(double));
         phiFlux_.assign(phi, phi + mesh_.faces().size());
         assemble_pressure_correction();
     }
@@ -118,6 +120,7 @@ public:
     }
     const solver::CSRMatrix& Ap() const { return A_p_; }
     const solver::CSRMatrix& Am() const { return A_mom_; }
+    const std::vector<int>& twins() const { return periodicTwin_; }
     const util::aligned_vector<double>& rhs() const { return rhs_; }
     const util::aligned_vector<double>& phi() const { return phiFlux_; }
 };
@@ -190,6 +193,7 @@ solver::BCType bc_type(int cls)
     case 1: return solver::BCType::PressureOutlet;
     case 2: return solver::BCType::Symmetry;
     case 3: return solver::BCType::VelocityInlet;
+    case 5: return solver::BCType::Periodic;
     default: return solver::BCType::OutFlow;
     }
 }
@@ -206,7 +210,7 @@ struct Case
 std::unique_ptr<Case> make_case(int Nx, int Ny, int Nz, const double* o, const double* L,
                                 double perturb, uint64_t seed, const int* bc_of_zone, int nz,
                                 const double* bc_value, double rho, double mu, double dt, double urf,
-                                const double* U)
+                                const double* U, bool simplec = false)
 {
     auto c = std::make_unique<Case>();
     c->mesh = make_mesh(Nx, Ny, Nz, o, L, perturb, seed);
@@ -232,6 +236,8 @@ std::unique_ptr<Case> make_case(int Nx, int Ny, int Nz, const double* o, const d
     opt.dt = dt;
     opt.urfU = urf;
     opt.timeScheme = solver::TemporalScheme::ImplicitEuler;
+    if (simplec)
+        opt.algorithm = solver::PvCouplingVariant::SIMPLEC;
     c->probe = std::make_unique<Probe>(*c->mesh, c->fields, c->bcs, c->s1, c->s2, opt);
     c->probe->sparsity();
     return c;
@@ -301,6 +307,37 @@ int ref_assemble_pressure(int Nx, int Ny, int Nz, const double* o, const double*
                        1.0, nullptr);
     c->probe->pressure(aP, phi);
     copy_csr(c->probe->Ap(), rp, ci, vals);
+    std::memcpy(rhs, c->probe->rhs().data(), c->probe->rhs().size() * sizeof(double));
+    return 0;
+}
+
+// The periodic face pairs (twin[n_faces]), the SIMPLE pattern with the twin
+// columns (rp, ci; nnz returned when ci is null) of build_sparsity.
+int64_t ref_periodic(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
+                     uint64_t seed, const int* bc_of_zone, int nz, int32_t* twin, int32_t* rp,
+                     int32_t* ci)
+{
+    auto c = make_case(Nx, Ny, Nz, o, L, perturb, seed, bc_of_zone, nz, nullptr, 1.0, 1e-3, 0.0,
+                       1.0, nullptr);
+    const auto& tw = c->probe->twins();
+    if (twin)
+        for (size_t f = 0; f < c->mesh->faces().size(); ++f)
+            twin[f] = tw.empty() ? -1 : tw[f];
+    copy_csr(c->probe->Ap(), rp, ci, nullptr);
+    return int64_t(c->probe->Ap().colIdx.size());
+}
+
+// assemble_pressure_correction with the SIMPLEC denominators: the momentum
+// matrix values amom (on the pattern) and aP.
+int ref_assemble_pressure_simplec(int Nx, int Ny, int Nz, const double* o, const double* L,
+                                  double perturb, uint64_t seed, const int* bc_of_zone, int nz,
+                                  double rho, const double* aP, const double* amom,
+                                  const double* phi, double* vals, double* rhs)
+{
+    auto c = make_case(Nx, Ny, Nz, o, L, perturb, seed, bc_of_zone, nz, nullptr, rho, 1e-3, 0.0,
+                       1.0, nullptr, true);
+    c->probe->pressure(aP, phi, amom);
+    copy_csr(c->probe->Ap(), nullptr, nullptr, vals);
     std::memcpy(rhs, c->probe->rhs().data(), c->probe->rhs().size() * sizeof(double));
     return 0;
 }
@@ -474,7 +511,8 @@ int ref_correct_fields(int Nx, int Ny, int Nz, const double* o, const double* L,
 }
 
 // n_steps calls of advance_time_step(n_inner) (n_steps = 0: one iterate())
-// of SimpleAlgorithm (algorithm 0) or PisoAlgorithm (1) with a dense direct
+// of SimpleAlgorithm (algorithm 0; 2: with the SIMPLEC denominators, as
+// SimplecAlgorithm) or PisoAlgorithm (1) with a dense direct
 // solve.  U (3 n) and p are the initial fields and receive the final ones;
 // phi the final face flux; res[4] the last SimpleResiduals.
 int ref_flow(int Nx, int Ny, int Nz, const double* o, const double* L, double perturb,
@@ -524,6 +562,8 @@ int ref_flow(int Nx, int Ny, int Nz, const double* o, const double* L, double pe
         so.urfP = urfP;
         so.dt = dt;
         so.timeScheme = scheme;
+        if (algorithm == 2)
+            so.algorithm = solver::PvCouplingVariant::SIMPLEC;
         auto a = std::make_unique<SimpleProbe>(*mesh, fields, bcs, sm, sp, so);
         phiRef = &a->phi();
         alg = std::move(a);

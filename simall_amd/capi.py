@@ -56,7 +56,7 @@ class SolverCfg(C.Structure):
     ]
 
 
-MI_FLOW_SIMPLE, MI_FLOW_PISO = 0, 1
+MI_FLOW_SIMPLE, MI_FLOW_PISO, MI_FLOW_SIMPLEC = 0, 1, 2
 MI_TIME_EULER, MI_TIME_BDF2 = 0, 1
 
 
@@ -184,6 +184,8 @@ PROTOTYPES = {
     "mi_assemble_momentum": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp]),
     "mi_face_flux": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
     "mi_face_flux_noalloc": (_i, [_vp, _d, _vp, _vp, _vp, _vp, _vp]),
+    "mi_mesh_create_twins": (_i, [_vp, _vp, _i, _vp, C.POINTER(_vp)]),
+    "mi_simplec_denominators": (_i, [_vp, _vp, _vp, _vp]),
     "mi_face_gradient": (_i, [_vp, _d, _vp, _i, _vp, _vp, _vp]),
     "mi_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
     "mi_batch_set_equation": (_i, [_vp, _i, _vp]),
@@ -606,13 +608,18 @@ class Checkpoint:
 class DeviceMesh:
     """mi_mesh handle built from a simall_amd.workload.Mesh (host SoA arrays)."""
 
-    def __init__(self, mesh, bc_of_zone):
+    def __init__(self, mesh, bc_of_zone, twin=None):
         self.lib = load()
         s = mesh_soa(mesh)
         self.host = mesh
         self.bc = np.ascontiguousarray(bc_of_zone, dtype=np.int32)
         h = C.c_void_p()
-        check(self.lib.mi_mesh_create(C.byref(s), ptr(self.bc), self.bc.size, C.byref(h)))
+        if twin is None:
+            check(self.lib.mi_mesh_create(C.byref(s), ptr(self.bc), self.bc.size, C.byref(h)))
+        else:
+            tw = np.ascontiguousarray(twin, dtype=np.int32)
+            check(self.lib.mi_mesh_create_twins(C.byref(s), ptr(self.bc), self.bc.size, ptr(tw),
+                                                C.byref(h)))
         self.h = h.value
         nc, nf, nnz = C.c_int64(), C.c_int64(), C.c_int64()
         check(self.lib.mi_mesh_info(self.h, C.byref(nc), C.byref(nf), C.byref(nnz)))
@@ -635,6 +642,9 @@ class DeviceMesh:
         check(self.lib.mi_assemble_momentum(self.h, comp, mu, rho, d_phi.ptr, _dptr(d_pgradV),
                                             _dptr(d_dt), _dptr(d_Uold), _dptr(d_Ucomp), urf,
                                             ptr(bv), A.h, d_rhs.ptr))
+
+    def simplec_denominators(self, d_aP, A_mom, d_den):
+        check(self.lib.mi_simplec_denominators(self.h, d_aP.ptr, A_mom.h, d_den.ptr))
 
     def face_gradient(self, d_x, d_lam, d_grad, pinned, rho=1.0, d_aP=None):
         """d_grad[f] = dJ/dgamma_f = -lambda^T dA/dgamma_f x (pressure matrix)."""

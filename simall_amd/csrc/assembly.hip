@@ -13,9 +13,11 @@
 // order.  With floating-point contraction off the coefficients are the same
 // IEEE operations as the reference's, so the assembled values are bit-exact.
 #include "mesh.hpp"
+#include "host_threads.hpp"
 
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <vector>
 
@@ -43,11 +45,20 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
         const int f = m.cf_face[k];
         const int oth = m.sl_oth[k];
         const int tag = m.sl_tag[k];
-        const bool own = (tag & 16) != 0;
+        const bool own = (tag & kSlotOwner) != 0;
         const double Amag2 = m.sl_s2[k];
         const double ddotA = m.sl_ds[k];
         const double g = gamma ? gamma[f] : 1.0;
-        if (oth >= 0) {
+        if (tag & kSlotPeriodic) {
+            // the owner's half of the coupling with the twin's cell (:411-419)
+            const double dO = aP ? fmax(aP[c], 1e-30) : 1.0;
+            const double coef = rho * g * Amag2 / fmax(dO * ddotA, 1e-30);
+            diag += coef;
+            if (oth == c)
+                diag += -coef; // a cell that is its own twin: the same entry
+            else
+                val[m.cf_slot[k]] -= coef;
+        } else if (oth >= 0) {
             double aPf = 1.0;
             if (aP) {
                 const double dO = fmax(aP[own ? c : oth], 1e-30);
@@ -111,7 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_slot_geom(MeshDev m, double* __restr
 
 // dJ/dgamma_f = -lambda^T (dA/dgamma_f) x for the pressure matrix of
 // k_assemble_pressure: an interior face adds k_f (e_o - e_n)(e_o - e_n)^T, a
-// pressure-boundary face k_f e_o e_o^T, with k_f = gamma_f * G_f; the pinned
+// pressure-boundary face k_f e_o e_o^T, a periodic face k_f e_o (e_o - e_t)^T
+// (t the twin's cell), with k_f = gamma_f * G_f; the pinned
 // cell's diagonal carries the factor 1e6.  The rhs does not depend on gamma.
 // (Gradient = dJ/dalpha - (dR/dalpha)^T psi: src/adjoint/DiscreteAdjoint.cpp:170-182,
 // with R = A(gamma) x - b.)
@@ -143,12 +155,12 @@ __global__ __launch_bounds__(kBlock) void k_fg_geom(MeshDev m, const int* __rest
         const double dz = m.ccz[nb] - m.ccz[o];
         d = fabs(dx * Ax + dy * Ay + dz * Az);
         tag = inv[nb];
-    } else if (m.fbc[f] == BC_PRESSURE) {
+    } else if (m.ftwin[f] >= 0 || m.fbc[f] == BC_PRESSURE) {
         const double dx = m.fcx[f] - m.ccx[o];
         const double dy = m.fcy[f] - m.ccy[o];
         const double dz = m.fcz[f] - m.ccz[o];
         d = fabs(dx * Ax + dy * Ay + dz * Az);
-        tag = -1;
+        tag = m.ftwin[f] >= 0 ? -3 - inv[m.ftwin[f]] : -1;
     }
     fn[f] = tag;
     s2[f] = Amag2;
@@ -206,6 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_face_gradient(int n_faces, const int
         const double dO = aP ? fmax(aP[o], 1e-30) : 1.0;
         const double G = rho * Amag2 / fmax(dO * ddotA, 1e-30);
         g = -G * so * lam[o] * x[o];
+    } else if (nb <= -3) {
+        // periodic: G (e_o e_o^T - e_o e_t^T), t the twin's cell
+        const int t = -3 - nb;
+        const double dO = aP ? fmax(aP[o], 1e-30) : 1.0;
+        const double G = rho * Amag2 / fmax(dO * ddotA, 1e-30);
+        g = -G * lam[o] * (so * x[o] - x[t]);
     }
     grad[f] = g;
 }
@@ -238,8 +256,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
         const double ConvO = fmax(Ff, 0.0);
         const double ConvN = fmax(-Ff, 0.0);
         if (oth >= 0) {
+            // interior faces and periodic faces (the owner's half of the
+            // coupling with the twin's cell, :279-285)
             const double D = mu * Amag2 / fmax(ddotA, 1e-30);
-            if (tag & 16) {
+            if ((tag & kSlotPeriodic) && oth == c) {
+                diag += ConvO + D;
+                diag += -(ConvN + D);
+            } else if (tag & kSlotOwner) {
                 diag += ConvO + D;
                 val[m.cf_slot[k]] += -(ConvN + D);
             } else {
@@ -295,10 +318,15 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshDev m, double rho,
         return;
     const int o = m.owner[f], nb = m.nbr[f];
     double ux, uy, uz;
+    const int tw = nb < 0 ? m.ftwin[f] : -1;
     if (nb >= 0) {
         ux = 0.5 * (Ux[o] + Ux[nb]);
         uy = 0.5 * (Uy[o] + Uy[nb]);
         uz = 0.5 * (Uz[o] + Uz[nb]);
+    } else if (tw >= 0) {
+        ux = 0.5 * (Ux[o] + Ux[tw]); // periodic (:214-218)
+        uy = 0.5 * (Uy[o] + Uy[tw]);
+        uz = 0.5 * (Uz[o] + Uz[tw]);
     } else {
         const int bc = m.fbc[f];
         if (bc == BC_VELOCITY_INLET) {
@@ -317,7 +345,37 @@ __global__ __launch_bounds__(kBlock) void k_face_flux(MeshDev m, double rho,
     phi[f] = rho * (ux * m.ax[f] + uy * m.ay[f] + uz * m.az[f]);
 }
 
+// SIMPLEC denominator of cell c (src/solver/SimpleAlgorithm.cpp:378-387,
+// 400-409): max(aP_c - sum of |a| over row c's off-diagonal entries of the
+// momentum matrix, in row order, 1e-30).
+__global__ __launch_bounds__(kBlock) void k_simplec_den(int n, const int* __restrict__ rp,
+                                                       const int* __restrict__ ci,
+                                                       const double* __restrict__ amom,
+                                                       const double* __restrict__ aP,
+                                                       double* __restrict__ den)
+{
+    const int c = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= n)
+        return;
+    double sumOff = 0.0;
+    for (int k = rp[c]; k < rp[c + 1]; ++k)
+        if (ci[k] != c)
+            sumOff += fabs(amom[k]);
+    den[c] = fmax(aP[c] - sumOff, 1e-30);
+}
+
 #pragma clang fp contract(on)
+
+int launch_simplec_denominators(const mi_mesh* M, const double* amom, const double* aP,
+                                double* den, hipStream_t s)
+{
+    if (M->n_cells > 0) {
+        k_simplec_den<<<grid_for(M->n_cells, kBlock), kBlock, 0, s>>>(
+            int(M->n_cells), M->pat->rp.p, M->pat->ci.p, amom, aP, den);
+        MI_HIP(hipGetLastError());
+    }
+    return MI_OK;
+}
 
 int launch_assemble_pressure(const mi_mesh* M, double rho, const double* aP, const double* gamma,
                              const double* phi, int pin, double* val, double* rhs, hipStream_t s)
@@ -425,12 +483,78 @@ template <class T> int up(mi::DevBuf<T>& d, const T* h, int64_t n, hipStream_t s
     return d.upload(h, size_t(n), s);
 }
 
-} // namespace
+// SimpleAlgorithm::build_periodic_pairs (src/solver/SimpleAlgorithm.cpp:101-186)
+// on the host: the periodic zones in ascending zone id are paired two by two;
+// t = mean centroid of the second zone's boundary faces - mean of the
+// first's; every face fa of the first zone (face order) takes the first face
+// fb of the second zone with the least |c_fb - (c_fa + t)|^2, and
+// twin[fa] = fb, twin[fb] = fa (a later fa overwrites twin[fb]).  The
+// nearest-face searches run on host threads; the pairs are written in face
+// order afterwards.
+void periodic_pairs(const mi_mesh_soa* m, const std::vector<int>& bc_of_zone,
+                    std::vector<int>& twin)
+{
+    const int64_t nF = m->n_faces;
+    twin.assign(size_t(nF), -1);
+    std::vector<int> zones;
+    for (size_t z = 0; z < bc_of_zone.size(); ++z)
+        if (bc_of_zone[z] == BC_PERIODIC)
+            zones.push_back(int(z));
+    if (zones.size() < 2 || !m->boundaryZone || !m->faceCx)
+        return;
+    for (size_t zi = 0; zi + 1 < zones.size(); zi += 2) {
+        std::vector<int> A, B;
+        for (int64_t f = 0; f < nF; ++f) {
+            if (m->neighbor[f] != UINT64_MAX)
+                continue;
+            if (int64_t(m->boundaryZone[f]) == zones[zi])
+                A.push_back(int(f));
+            else if (int64_t(m->boundaryZone[f]) == zones[zi + 1])
+                B.push_back(int(f));
+        }
+        if (A.empty() || B.empty())
+            continue;
+        double ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+        for (int f : A) {
+            ax += m->faceCx[f];
+            ay += m->faceCy[f];
+            az += m->faceCz[f];
+        }
+        for (int f : B) {
+            bx += m->faceCx[f];
+            by += m->faceCy[f];
+            bz += m->faceCz[f];
+        }
+        const double ia = 1.0 / double(A.size()), ib = 1.0 / double(B.size());
+        const double tx = bx * ib - ax * ia, ty = by * ib - ay * ia, tz = bz * ib - az * ia;
+        std::vector<int> best(A.size(), -1);
+        host_parallel_for(int(A.size()), host_threads(), [&](int b0, int e0, int) {
+            for (int i = b0; i < e0; ++i) {
+                const int fa = A[size_t(i)];
+                const double px = m->faceCx[fa] + tx, py = m->faceCy[fa] + ty,
+                             pz = m->faceCz[fa] + tz;
+                double bestd2 = std::numeric_limits<double>::max();
+                for (int fb : B) {
+                    const double d2 = (m->faceCx[fb] - px) * (m->faceCx[fb] - px) +
+                                      (m->faceCy[fb] - py) * (m->faceCy[fb] - py) +
+                                      (m->faceCz[fb] - pz) * (m->faceCz[fb] - pz);
+                    if (d2 < bestd2) {
+                        bestd2 = d2;
+                        best[size_t(i)] = fb;
+                    }
+                }
+            }
+        });
+        for (size_t i = 0; i < A.size(); ++i)
+            if (best[i] >= 0) {
+                twin[size_t(A[i])] = best[i];
+                twin[size_t(best[i])] = A[i];
+            }
+    }
+}
 
-extern "C" {
-
-MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n_zones,
-                          mi_mesh** out)
+int mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n_zones,
+                const int32_t* twin_faces, mi_mesh** out)
 {
     clear_error();
     if (!m || !out || n_zones < 0 || (n_zones > 0 && !bc_of_zone))
@@ -449,9 +573,19 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
     M->bc_of_zone.assign(bc_of_zone, bc_of_zone + n_zones);
     hipStream_t s = default_stream();
 
+    // periodic twins: the caller's table (SimpleAlgorithm::set_periodic_pairs)
+    // or the pairing of the periodic zones; only boundary faces keep one
+    if (twin_faces) {
+        M->twin_face.assign(twin_faces, twin_faces + nF);
+        for (int64_t f = 0; f < nF; ++f)
+            if (M->twin_face[size_t(f)] < -1 || M->twin_face[size_t(f)] >= nF)
+                return set_error(MI_ERR_BAD_ARG, "face %lld: twin out of range", (long long)f);
+    } else {
+        periodic_pairs(m, M->bc_of_zone, M->twin_face);
+    }
     // face arrays, ids narrowed to int32 (kBoundaryCell -> -1)
     std::vector<int> own(static_cast<size_t>(nF)), nbr(static_cast<size_t>(nF)),
-        zone(static_cast<size_t>(nF));
+        zone(static_cast<size_t>(nF)), ftwin(static_cast<size_t>(nF), -1);
     std::vector<int8_t> fbc(static_cast<size_t>(nF));
     for (int64_t f = 0; f < nF; ++f) {
         if (m->owner[f] >= uint64_t(nC))
@@ -466,11 +600,16 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
         if (nb == UINT64_MAX) {
             const int bc = bc_of(M.get(), z);
             fbc[size_t(f)] = int8_t(bc);
-            if (bc == BC_PRESSURE)
+            if (bc == BC_PRESSURE && M->twin_face[size_t(f)] < 0)
                 M->any_dirichlet = true;
         } else {
             fbc[size_t(f)] = -1;
         }
+    }
+    for (int64_t f = 0; f < nF; ++f) {
+        const int t = M->twin_face[size_t(f)];
+        if (t >= 0 && nbr[size_t(f)] < 0)
+            ftwin[size_t(f)] = own[size_t(t)];
     }
     // CSR pattern: row c = {c} U face neighbours, sorted, unique
     // (src/solver/SimpleAlgorithm.cpp:63-83); per-cell faces sorted by id and
@@ -497,6 +636,8 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
             const int oth = (o == int(c)) ? nb : o;
             if (oth >= 0)
                 nb_list.push_back(oth);
+            else if (ftwin[size_t(f)] >= 0)
+                nb_list.push_back(ftwin[size_t(f)]);
         }
         std::sort(nb_list.begin(), nb_list.end());
         nb_list.erase(std::unique(nb_list.begin(), nb_list.end()), nb_list.end());
@@ -507,9 +648,11 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
         for (int f : faces) {
             cf_face.push_back(f);
             const int o = own[size_t(f)], nb = nbr[size_t(f)];
-            const int oth = (o == int(c)) ? nb : o;
+            const bool periodic = nb < 0 && ftwin[size_t(f)] >= 0;
+            const int oth = periodic ? ftwin[size_t(f)] : (o == int(c)) ? nb : o;
             sl_oth.push_back(oth);
-            sl_tag.push_back(int8_t(((o == int(c)) ? 16 : 0) + (int(fbc[size_t(f)]) + 1)));
+            sl_tag.push_back(int8_t(((o == int(c)) ? kSlotOwner : 0) +
+                                    (periodic ? kSlotPeriodic : 0) + (int(fbc[size_t(f)]) + 1)));
             int slot = -1;
             if (oth >= 0)
                 slot = row0 + int(std::lower_bound(nb_list.begin(), nb_list.end(), oth) -
@@ -523,6 +666,7 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
     MI_TRY(up(M->nbr, nbr.data(), nF, s));
     MI_TRY(up(M->fzone, zone.data(), nF, s));
     MI_TRY(up(M->fbc, fbc.data(), nF, s));
+    MI_TRY(up(M->ftwin, ftwin.data(), nF, s));
     MI_TRY(up(M->ax, m->areaX, nF, s));
     MI_TRY(up(M->ay, m->areaY, nF, s));
     MI_TRY(up(M->az, m->areaZ, nF, s));
@@ -546,6 +690,39 @@ MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n
     }
     MI_HIP(hipStreamSynchronize(s));
     *out = M.release();
+    return MI_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+MI_API int mi_mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n_zones,
+                          mi_mesh** out)
+{
+    return mesh_create(m, bc_of_zone, n_zones, nullptr, out);
+}
+
+MI_API int mi_mesh_create_twins(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n_zones,
+                                const int32_t* twin_faces, mi_mesh** out)
+{
+    if (!twin_faces)
+        return set_error(MI_ERR_BAD_ARG, "null twin table");
+    return mesh_create(m, bc_of_zone, n_zones, twin_faces, out);
+}
+
+MI_API int mi_simplec_denominators(mi_mesh* M, const double* aP, const mi_csr* A_mom,
+                                   double* den)
+{
+    clear_error();
+    if (!M || !A_mom || (M->n_cells > 0 && (!aP || !den)))
+        return set_error(MI_ERR_BAD_ARG, "null argument");
+    if (A_mom->pat != M->pat)
+        return set_error(MI_ERR_BAD_ARG, "the momentum matrix is not on the mesh's pattern");
+    MI_TRY(ensure_device());
+    hipStream_t s = default_stream();
+    MI_TRY(launch_simplec_denominators(M, A_mom->val.p, aP, den, s));
+    MI_HIP(hipStreamSynchronize(s));
     return MI_OK;
 }
 

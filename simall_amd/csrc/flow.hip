@@ -311,6 +311,7 @@ struct mi_flow
     int n = 0, nf = 0;
     int time_step = 0;
     DevBuf<double> U[3], Un[3], Unm1[3], p, phi, aP, pp, rhs, src[3], dt, bc, partials, scal;
+    DevBuf<double> den; // SIMPLEC denominators
     double* h_scal = nullptr; // pinned
     std::unique_ptr<mi_csr> Am, Ap;
     std::unique_ptr<mi_solver> sm, sp;
@@ -380,7 +381,11 @@ int momentum_predictor(mi_flow* F, mi_flow_residuals* res)
                                         transient ? F->dt.p : nullptr, F->Un[k].p,
                                         bdf2 ? F->Unm1[k].p : nullptr, F->U[k].p, c.urfU, F->bc.p,
                                         F->Am->val.p, F->rhs.p, F->s));
-        F->Am->values_changed(false);
+        // The matrix values do not depend on the component (only the rhs
+        // does): components 1 and 2 rewrite the same values, so the momentum
+        // solver keeps the preconditioner it set up for component 0.
+        if (k == 0)
+            F->Am->values_changed(false);
         if (k == 0) {
             k_take_diag<<<grid_for(F->n, kBlock), kBlock, 0, F->s>>>(F->n, M->pat->diag.p,
                                                                      F->Am->val.p, F->aP.p);
@@ -403,7 +408,13 @@ int pressure_corrector(mi_flow* F, double* cont)
     mi_flow_cfg& c = F->cfg;
     mi_mesh* M = F->M;
     const int pin = (!M->any_dirichlet && F->n > 0) ? 1 : 0;
-    MI_TRY(launch_assemble_pressure(M, c.rho, F->aP.p, nullptr, F->phi.p, pin, F->Ap->val.p,
+    const double* den = F->aP.p;
+    if (c.algorithm == MI_FLOW_SIMPLEC) {
+        // 1 / (aP - sum |a_nb|) in place of 1 / aP (SimpleAlgorithm.cpp:378-387)
+        MI_TRY(launch_simplec_denominators(M, F->Am->val.p, F->aP.p, F->den.p, F->s));
+        den = F->den.p;
+    }
+    MI_TRY(launch_assemble_pressure(M, c.rho, den, nullptr, F->phi.p, pin, F->Ap->val.p,
                                     F->rhs.p, F->s));
     F->Ap->values_changed(false);
     MI_TRY(flow_solve(F, F->sp.get(), F->rhs.p, F->pp.p));
@@ -452,7 +463,15 @@ MI_API void mi_flow_cfg_default(int algorithm, mi_flow_cfg* c)
     c->algorithm = algorithm;
     c->rho = 1.0;
     c->mu = 1.0e-3;
-    if (algorithm == MI_FLOW_PISO) {
+    if (algorithm == MI_FLOW_SIMPLEC) {
+        // SimplecOptions (SimplecAlgorithm.hpp:27-35)
+        c->urfU = 0.9;
+        c->urfP = 1.0;
+        c->dt = 0.0;
+        c->time_scheme = MI_TIME_EULER;
+        c->n_correctors = 1;
+        c->n_nonorth = 0;
+    } else if (algorithm == MI_FLOW_PISO) {
         // PisoOptions (PisoAlgorithm.hpp:33-41) and to_simple_opts (:63-74)
         c->urfU = 1.0;
         c->urfP = 1.0;
@@ -493,7 +512,8 @@ MI_API int mi_flow_create(mi_mesh* M, const double* bc_value_host, const mi_flow
     else
         mi_flow_cfg_default(MI_FLOW_SIMPLE, &F->cfg);
     mi_flow_cfg& c = F->cfg;
-    if (c.algorithm != MI_FLOW_SIMPLE && c.algorithm != MI_FLOW_PISO)
+    if (c.algorithm != MI_FLOW_SIMPLE && c.algorithm != MI_FLOW_PISO &&
+        c.algorithm != MI_FLOW_SIMPLEC)
         return set_error(MI_ERR_BAD_ARG, "unknown algorithm %d", c.algorithm);
     if (!(c.urfU > 0.0) || !(c.dt >= 0.0) || c.n_nonorth < 0)
         return set_error(MI_ERR_BAD_ARG, "bad flow options");
@@ -515,6 +535,7 @@ MI_API int mi_flow_create(mi_mesh* M, const double* bc_value_host, const mi_flow
     }
     MI_TRY(F->p.alloc(n));
     MI_TRY(F->aP.alloc(n));
+    MI_TRY(F->den.alloc(n));
     MI_TRY(F->pp.alloc(n));
     MI_TRY(F->rhs.alloc(n));
     MI_TRY(F->dt.alloc(n));

@@ -9,7 +9,12 @@
 namespace mi
 {
 
-enum : int8_t { BC_WALL = 0, BC_PRESSURE = 1, BC_SYMMETRY = 2, BC_VELOCITY_INLET = 3, BC_OUTFLOW = 4 };
+enum : int8_t { BC_WALL = 0, BC_PRESSURE = 1, BC_SYMMETRY = 2, BC_VELOCITY_INLET = 3, BC_OUTFLOW = 4,
+                BC_PERIODIC = 5 };
+
+// sl_tag bits
+constexpr int kSlotOwner = 16;    // the slot's cell owns the face
+constexpr int kSlotPeriodic = 32; // a boundary face with a periodic twin (sl_oth = twin's cell)
 
 struct MeshDev
 {
@@ -18,6 +23,7 @@ struct MeshDev
     const int* nbr; // -1 on boundary faces
     const double *ax, *ay, *az, *fcx, *fcy, *fcz;
     const int8_t* fbc;  // boundary class of each face (interior: -1)
+    const int* ftwin;   // the owner of a boundary face's periodic twin, else -1
     const int* fzone;   // zone id of each face
     const double *ccx, *ccy, *ccz, *vol;
     const int* cf_off;  // [n_cells+1]
@@ -27,8 +33,10 @@ struct MeshDev
     const int* diag;    // CSR position of the diagonal
     // per cell-face slot (the cf_* order): |S_f|^2, |d . S_f| (d from the
     // owner's centroid to the neighbour's, or to the face centroid on a
-    // boundary), the cell across the face (-1 on the boundary), and a tag:
-    // 16 when the slot's cell owns the face, plus the face's boundary class + 1
+    // boundary), the cell across the face (-1 on the boundary; the twin's
+    // cell on a periodic face), and a tag: kSlotOwner when the slot's cell
+    // owns the face, kSlotPeriodic on a periodic face, plus the face's
+    // boundary class + 1
     const double* sl_s2;
     const double* sl_ds;
     const int* sl_oth;
@@ -44,7 +52,8 @@ struct mi_mesh
     int n_zones = 0;
     std::vector<int> bc_of_zone;
     bool any_dirichlet = false;
-    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot, sl_oth;
+    mi::DevBuf<int> owner, nbr, fzone, cf_off, cf_face, cf_slot, sl_oth, ftwin;
+    std::vector<int> twin_face; // periodic twin face of every face, -1 (host)
     mi::DevBuf<int8_t> fbc, sl_tag;
     mi::DevBuf<double> sl_s2, sl_ds;
     mi::DevBuf<double> ax, ay, az, fcx, fcy, fcz, ccx, ccy, ccz, vol;
@@ -54,7 +63,8 @@ struct mi_mesh
     // Face-gradient state, built on first use: cells renumbered in the order
     // the face list first meets them (fg_perm[new] = old), every face's owner
     // and neighbour in that numbering (neighbour -1: pressure boundary, -2:
-    // a boundary that contributes nothing), |S_f|^2 and |d_f . S_f|; and the
+    // a boundary that contributes nothing, -3 - t: a periodic face whose
+    // twin's cell is t), |S_f|^2 and |d_f . S_f|; and the
     // renumbered copies of x, lambda and aP a call gathers.
     mi::DevBuf<int> fg_perm, fg_o, fg_n;
     mi::DevBuf<double> fg_s2, fg_ds, fg_x, fg_l, fg_a;
@@ -73,6 +83,7 @@ struct mi_mesh
         d.fcy = fcy.p;
         d.fcz = fcz.p;
         d.fbc = fbc.p;
+        d.ftwin = ftwin.p;
         d.fzone = fzone.p;
         d.ccx = ccx.p;
         d.ccy = ccy.p;
@@ -94,6 +105,8 @@ struct mi_mesh
 namespace mi
 {
 // Asynchronous launchers of the assembly kernels on stream s (assembly.hip).
+int launch_simplec_denominators(const mi_mesh* M, const double* amom, const double* aP,
+                                double* den, hipStream_t s);
 int launch_assemble_pressure(const mi_mesh* M, double rho, const double* aP, const double* gamma,
                              const double* phi, int pin, double* val, double* rhs, hipStream_t s);
 int launch_assemble_momentum(const mi_mesh* M, int comp, double mu, double rho, const double* phi,

@@ -29,7 +29,7 @@ SEED_BASE = 0x51A11000
 Q_NODES, Q_PERM, Q_GAMMA, Q_RHS, Q_WEIGHTS, Q_DT, Q_FACES, Q_EQ = 1, 2, 3, 4, 5, 6, 7, 8
 
 # boundary treatment per zone (oracle.h / mesh entry points share these values)
-BC_WALL, BC_PRESSURE, BC_SYMMETRY, BC_VELOCITY_INLET, BC_OUTFLOW = 0, 1, 2, 3, 4
+BC_WALL, BC_PRESSURE, BC_SYMMETRY, BC_VELOCITY_INLET, BC_OUTFLOW, BC_PERIODIC = 0, 1, 2, 3, 4, 5
 
 
 def seed(workload_index, quantity):

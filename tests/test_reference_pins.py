@@ -283,3 +283,133 @@ def test_flow_iterations_match_reference(ref, algorithm):
     assert np.allclose(res["mom"] + [res["cont"]], rres, rtol=1e-8, atol=1e-12)
     assert np.linalg.norm(F.U[0]) > 0.1  # the inlet drives a flow
 
+
+
+def _periodic_bc():
+    """x-periodic (zones 1, 2) and z-periodic (zones 5, 6), walls on y."""
+    return np.array([0, oracle.BC_PERIODIC, oracle.BC_PERIODIC, wl.BC_WALL, wl.BC_WALL,
+                     oracle.BC_PERIODIC, oracle.BC_PERIODIC], np.int32)
+
+
+def _ref_sig(ref):
+    i, d, u64 = C.c_int, C.c_double, C.c_uint64
+    ref.ref_periodic.argtypes = [i, i, i, P, P, d, u64, P, i, P, P, P]
+    ref.ref_periodic.restype = C.c_int64
+    ref.ref_assemble_pressure_simplec.argtypes = [i, i, i, P, P, d, u64, P, i, d, P, P, P, P, P]
+    ref.ref_assemble_pressure_simplec.restype = i
+
+
+@pytest.mark.parametrize("N,perturb", [((6, 4, 5), 0.2), ((4, 3, 2), 0.0)])
+def test_periodic_pairs_and_pattern_match_reference(ref, N, perturb):
+    """SimpleAlgorithm::build_periodic_pairs and the twin columns of
+    build_sparsity (SimpleAlgorithm.cpp:49-98, 101-186)."""
+    _ref_sig(ref)
+    o, L = (0.0, 0.0, 0.0), (1.5, 1.0, 2.0)
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=perturb, node_seed=21)
+    bc = _periodic_bc()
+    twin = oracle.periodic_twins(m, bc)
+    rp, ci = oracle.build_sparsity(m, twin)
+    rtw = np.empty(m.n_faces, np.int32)
+    nnz = ref.ref_periodic(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(21), _p(bc), 7,
+                           _p(rtw), None, None)
+    rrp, rci = np.empty(rp.size, np.int32), np.empty(nnz, np.int32)
+    ref.ref_periodic(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(21), _p(bc), 7, None,
+                     _p(rrp), _p(rci))
+    assert np.array_equal(twin, rtw)
+    assert np.array_equal(rp, rrp) and np.array_equal(ci, rci)
+    # every x- and z-boundary face is paired, both ways, with a face of the opposite zone
+    per = np.isin(m.boundaryZone, [1, 2, 5, 6]) & (m.neighbor == np.uint64(2**64 - 1))
+    assert (twin[per] >= 0).all() and (twin[~per] == -1).all()
+    assert np.array_equal(twin[twin[per]], np.nonzero(per)[0])
+    assert (ci.size - m.n_cells) == 2 * int((m.neighbor != np.uint64(2**64 - 1)).sum()) + int(per.sum()) \
+        or N[0] <= 2 or N[2] <= 2
+
+
+def test_periodic_assembly_matches_reference(ref):
+    """Pressure and momentum assembly and face fluxes with periodic twins
+    (SimpleAlgorithm.cpp:214-218, 279-285, 411-419), bit-exact."""
+    N, o, L, perturb, seed = (6, 4, 5), (0.0, 0.0, 0.0), (1.5, 1.0, 2.0), 0.2, 21
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=perturb, node_seed=seed)
+    bc = _periodic_bc()
+    twin = oracle.periodic_twins(m, bc)
+    rp, ci = oracle.build_sparsity(m, twin)
+    pat = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    aP = wl.uniform(3, 0.5, 2.0, m.n_cells)
+    phi = wl.normal(4, m.n_faces)
+    vals, rhs, pinned = oracle.assemble_pressure(m, pat, bc, rho=1.3, aP=aP, phi=phi, twin=twin)
+    rrp, rci, rv, rrhs = (np.empty(rp.size, np.int32), np.empty(ci.size, np.int32),
+                          np.empty(ci.size), np.empty(m.n_cells))
+    ref.ref_assemble_pressure(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed), _p(bc), 7,
+                              C.c_double(1.3), _p(aP), _p(phi), _p(rrp), _p(rci), _p(rv), _p(rrhs))
+    assert np.array_equal(rp, rrp) and np.array_equal(ci, rci)
+    assert np.array_equal(vals, rv) and np.array_equal(rhs, rrhs)
+    assert pinned  # no pressure zone: the reference pins cell 0
+    U = wl.taylor_green(m)
+    Uall = np.concatenate(U)
+    rho, mu, dt, urf = 1.2, 0.01, 0.05, 0.7
+    fphi = oracle.face_flux(m, bc, rho, U, twin=twin)
+    for comp in range(3):
+        Uold = U[comp] * 0.9
+        v, b = oracle.assemble_momentum(m, pat, bc, comp, mu, rho, fphi, dt=np.full(m.n_cells, dt),
+                                        Uold=Uold, Ucomp=U[comp], urf=urf, twin=twin)
+        rphi, rv2, rb = np.empty(m.n_faces), np.empty(ci.size), np.empty(m.n_cells)
+        ref.ref_assemble_momentum(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed),
+                                  _p(bc), 7, None, C.c_double(rho), C.c_double(mu), C.c_double(dt),
+                                  C.c_double(urf), _p(Uall), comp, _p(Uold), _p(rphi), _p(rv2), _p(rb))
+        assert np.array_equal(fphi, rphi)
+        assert np.array_equal(v, rv2) and np.array_equal(b, rb), comp
+
+
+def test_simplec_pressure_assembly_matches_reference(ref):
+    """SIMPLEC denominators max(aP - sum |a_nb|, 1e-30) of the momentum row
+    (SimpleAlgorithm.cpp:378-387, 400-409), bit-exact."""
+    _ref_sig(ref)
+    N, o, L, perturb, seed = (6, 5, 4), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 0.2, 12345
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=perturb, node_seed=seed)
+    bc = np.array([0, wl.BC_VELOCITY_INLET, wl.BC_PRESSURE, wl.BC_WALL, wl.BC_WALL, wl.BC_WALL,
+                   wl.BC_SYMMETRY], np.int32)
+    rp, ci = oracle.build_sparsity(m)
+    pat = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    U = wl.taylor_green(m)
+    phi = oracle.face_flux(m, bc, 1.0, U)
+    amom, _ = oracle.assemble_momentum(m, pat, bc, 0, 0.01, 1.0, phi, Ucomp=U[0], urf=0.9)
+    aP = amom[pat.diag_positions()].copy()
+    aP[::5] *= 0.5  # some denominators clipped at 1e-30
+    den = oracle.simplec_denominators(pat, amom, aP)
+    assert (den == 1e-30).any() and (den > 1e-30).any()
+    vals, rhs, _ = oracle.assemble_pressure(m, pat, bc, rho=1.3, aP=den, phi=phi)
+    rv, rrhs = np.empty(ci.size), np.empty(m.n_cells)
+    ref.ref_assemble_pressure_simplec(*N, _p(d3(o)), _p(d3(L)), C.c_double(perturb), C.c_uint64(seed),
+                                      _p(bc), 7, C.c_double(1.3), _p(aP), _p(amom), _p(phi), _p(rv),
+                                      _p(rrhs))
+    assert np.array_equal(vals, rv) and np.array_equal(rhs, rrhs)
+
+
+@pytest.mark.parametrize("case", ["simplec", "periodic"])
+def test_simplec_and_periodic_iterations_match_reference(ref, case):
+    """Three SIMPLEC outer iterations on the channel (SimplecAlgorithm
+    defaults urf 0.9 / 1.0), and three SIMPLE iterations on a periodic box
+    from a Taylor-Green start, against the reference with direct solves."""
+    if case == "simplec":
+        N, o, L, bc, bcv = _channel()
+        U0 = None
+        alg = 2
+    else:
+        N, o, L = (5, 3, 4), (0.0, 0.0, 0.0), (2.0, 1.0, 1.0)
+        bc, bcv = _periodic_bc(), np.zeros((7, 3))
+        alg = 0
+    m = wl.Mesh(*N, origin=o, extent=L, perturb=0.1, node_seed=4)
+    F = oracle.Flow(m, bc, bcv, algorithm="simplec" if alg == 2 else "simple", rho=1.1, mu=0.05)
+    if case == "periodic":
+        U0 = list(wl.taylor_green(m))
+        F.U = [u.copy() for u in U0]
+    U, p = U0, None
+    for _ in range(3):
+        res = F.iterate()
+        U, p, phi, rres = _ref_flow(ref, N, o, L, 0.1, 4, bc, bcv, alg, 1.1, 0.05, F.urfU, F.urfP,
+                                    0.0, False, 1, 0, 0, 1, m.n_cells, m.n_faces, U0=U, p0=p)
+    for k in range(3):
+        assert _close(F.U[k], U[k], 1e-10), k
+    assert _close(F.p, p, 1e-10) and _close(F.phi, phi, 1e-10)
+    assert np.allclose(res["mom"] + [res["cont"]], rres, rtol=1e-8, atol=1e-12)
+    assert np.linalg.norm(F.U[0]) > 0.01
