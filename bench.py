@@ -467,7 +467,9 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3, variants=None):
     hierarchy for every pressure solve by the device numeric re-setup
     (amg_resetup = 1, the default); `simple_full_setup` by a full host setup
     each time (amg_resetup = 0, the reference's semantics);
-    `simple_keep_pc` keeps the first hierarchy."""
+    `simple_keep_pc` keeps the first hierarchy; `simple_batched_momentum` solves
+    the three momentum components in one batched BiCGStab-Jacobi solve
+    (momentum_batch = 1)."""
     import numpy as np
 
     t0 = time.perf_counter()
@@ -480,14 +482,17 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3, variants=None):
     out = {"workload": f"{N}^3 channel, velocity inlet / pressure outlet / walls, rho 1, mu 1e-3; "
                        f"momentum GMRES(30)-ILU(0) 1e-8, pressure CG-AMG 1e-8",
            "n": m.n_cells}
-    for name, alg, keep, resetup in (("simple", capi.MI_FLOW_SIMPLE, 0, 1),
-                                     ("simple_full_setup", capi.MI_FLOW_SIMPLE, 0, 0),
-                                     ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1, 1),
-                                     ("piso", capi.MI_FLOW_PISO, 0, 1)):
+    for name, alg, keep, resetup, mbatch in (("simple", capi.MI_FLOW_SIMPLE, 0, 1, 0),
+                                             ("simple_full_setup", capi.MI_FLOW_SIMPLE, 0, 0, 0),
+                                             ("simple_keep_pc", capi.MI_FLOW_SIMPLE, 1, 1, 0),
+                                             ("simple_batched_momentum", capi.MI_FLOW_SIMPLE, 0, 1,
+                                              1),
+                                             ("piso", capi.MI_FLOW_PISO, 0, 1, 0)):
         if variants and name not in variants:
             continue
         cfg = capi.flow_cfg(alg, momentum=dict(keep_preconditioner=keep),
-                            pressure=dict(keep_preconditioner=keep, amg_resetup=resetup))
+                            pressure=dict(keep_preconditioner=keep, amg_resetup=resetup),
+                            momentum_batch=mbatch)
         F = capi.Flow(dm, cfg, bcv)
         times, res = [], None
         for it in range(iters + 1):  # the first iteration builds the patterns
@@ -675,7 +680,8 @@ def main():
     ap.add_argument("--flow-n", type=int, default=96, help="cells per axis of the flow section")
     ap.add_argument("--flow-iters", type=int, default=3, help="timed iterations per flow variant")
     ap.add_argument("--flow-variants", default="",
-                    help="comma list of simple, simple_full_setup, simple_keep_pc, piso (default all)")
+                    help="comma list of simple, simple_full_setup, simple_keep_pc, "
+                         "simple_batched_momentum, piso (default all)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the working-matrix SpMV and a calibration copy only "
                          "(short run for hardware-counter passes)")

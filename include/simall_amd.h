@@ -489,6 +489,10 @@ typedef struct {
     int n_nonorth;       /* PISO extra pressure passes per corrector (0) */
     mi_solver_cfg momentum; /* momentum solves: GMRES(30) + ILU(0), 1e-8 (LinearSolverConfig) */
     mi_solver_cfg pressure; /* pressure-correction solves: CG + SA-AMG, 1e-8 */
+    int momentum_batch;  /* 1: the three momentum components (one matrix, three right-hand
+                            sides: SimpleAlgorithm.cpp:529-532) in one mi_batch solve with
+                            momentum.tolerance / max_iterations / inner_tolerance / max_outer,
+                            BiCGStab with Jacobi; 0 (default): three solves with `momentum` */
 } mi_flow_cfg;
 /* SimpleResiduals (src/solver/SimpleAlgorithm.hpp:59-63). */
 typedef struct {

@@ -73,6 +73,7 @@ class FlowCfg(C.Structure):
         ("n_nonorth", C.c_int),
         ("momentum", SolverCfg),
         ("pressure", SolverCfg),
+        ("momentum_batch", C.c_int),
     ]
 
 

@@ -315,10 +315,14 @@ struct mi_flow
     double* h_scal = nullptr; // pinned
     std::unique_ptr<mi_csr> Am, Ap;
     std::unique_ptr<mi_solver> sm, sp;
+    mi_batch* mb = nullptr;   // momentum_batch: the three components' batched solve
+    DevBuf<double> rhs3, x3;  // their right-hand sides and solutions, component-major
     ~mi_flow()
     {
         if (s)
             (void)hipStreamSynchronize(s);
+        if (mb)
+            (void)mi_batch_destroy(mb);
         sm.reset();
         sp.reset();
         if (h_scal)
@@ -362,6 +366,50 @@ int flow_solve(mi_flow* F, mi_solver* S, const double* b, double* x)
     return MI_OK; // the reference's outer loop goes on whatever the solver returned
 }
 
+// The momentum predictor's three solves as one batched solve: the components
+// share the matrix (only the rhs depends on the component), so the three
+// assemblies write the same values; each rhs goes to its third of rhs3, the
+// batch solves the three systems together from x = 0, and each residual
+// ||A U_k - b_k|| is formed as in the one-at-a-time path.
+int momentum_batched(mi_flow* F, mi_flow_residuals* res, bool transient, bool bdf2)
+{
+    mi_flow_cfg& c = F->cfg;
+    mi_mesh* M = F->M;
+    const size_t n = size_t(F->n);
+    for (int k = 0; k < 3; ++k)
+        MI_TRY(launch_assemble_momentum(M, k, c.mu, c.rho, F->phi.p, F->src[k].p,
+                                        transient ? F->dt.p : nullptr, F->Un[k].p,
+                                        bdf2 ? F->Unm1[k].p : nullptr, F->U[k].p, c.urfU, F->bc.p,
+                                        F->Am->val.p, F->rhs3.p + size_t(k) * n, F->s));
+    F->Am->values_changed(false);
+    k_take_diag<<<grid_for(F->n, kBlock), kBlock, 0, F->s>>>(F->n, M->pat->diag.p, F->Am->val.p,
+                                                             F->aP.p);
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(F->s));
+    if (!F->mb)
+        MI_TRY(mi_batch_create(F->Am.get(), 3, &F->mb));
+    for (int k = 0; k < 3; ++k)
+        MI_TRY(mi_batch_set_equation(F->mb, k, F->Am.get()));
+    mi_solver_cfg bc = c.momentum;
+    bc.kind = MI_SOLVER_BICGSTAB;
+    bc.preconditioner = MI_PC_JACOBI;
+    int its[3];
+    double rel[3];
+    const int rc = mi_batch_solve_device(F->mb, &bc, F->rhs3.p, F->x3.p, its, rel);
+    if (rc != MI_OK && rc != MI_ERR_NOT_CONVERGED)
+        return rc; // the reference's outer loop goes on whatever the solver returned
+    for (int k = 0; k < 3; ++k) {
+        MI_HIP(hipMemcpyAsync(F->U[k].p, F->x3.p + size_t(k) * n, n * sizeof(double),
+                              hipMemcpyDeviceToDevice, F->s));
+        MI_TRY(flow_resid(F, F->Am.get(), F->U[k].p, F->rhs3.p + size_t(k) * n, F->scal.p + k));
+    }
+    MI_HIP(hipMemcpyAsync(F->h_scal, F->scal.p, 3 * sizeof(double), hipMemcpyDeviceToHost, F->s));
+    MI_HIP(hipStreamSynchronize(F->s));
+    for (int k = 0; k < 3; ++k)
+        res->mom[k] = F->h_scal[k];
+    return MI_OK;
+}
+
 // Momentum predictor: face fluxes, then per component assemble, solve and the
 // residual ||A U_k - b|| (SimpleAlgorithm.cpp:527-531 and 491-507).
 int momentum_predictor(mi_flow* F, mi_flow_residuals* res)
@@ -376,6 +424,8 @@ int momentum_predictor(mi_flow* F, mi_flow_residuals* res)
     MI_HIP(hipGetLastError());
     const bool transient = c.dt > 0.0;
     const bool bdf2 = transient && c.time_scheme == MI_TIME_BDF2 && F->time_step >= 1;
+    if (c.momentum_batch)
+        return momentum_batched(F, res, transient, bdf2);
     for (int k = 0; k < 3; ++k) {
         MI_TRY(launch_assemble_momentum(M, k, c.mu, c.rho, F->phi.p, F->src[k].p,
                                         transient ? F->dt.p : nullptr, F->Un[k].p,
@@ -536,6 +586,10 @@ MI_API int mi_flow_create(mi_mesh* M, const double* bc_value_host, const mi_flow
     MI_TRY(F->p.alloc(n));
     MI_TRY(F->aP.alloc(n));
     MI_TRY(F->den.alloc(n));
+    if (c.momentum_batch) {
+        MI_TRY(F->rhs3.alloc(3 * n));
+        MI_TRY(F->x3.alloc(3 * n));
+    }
     MI_TRY(F->pp.alloc(n));
     MI_TRY(F->rhs.alloc(n));
     MI_TRY(F->dt.alloc(n));
