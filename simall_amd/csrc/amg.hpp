@@ -37,6 +37,8 @@ template <class T> struct DMat
             m.soff = sell.soff.p;
             m.eci = sell.eci.p;
             m.nsl = sell.nsl;
+            if constexpr (std::is_same<T, float>::value)
+                m.ecd = sell.ecd.n ? sell.ecd.p : nullptr;
         }
         return m;
     }

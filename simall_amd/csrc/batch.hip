@@ -608,8 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_bfin(const double* __restrict__ part
                        : 1;
     double tot[2] = {0.0, 0.0};
     for (int v = 0; v < nv; ++v)
-        for (int g = threadIdx.x; g < f.G; g += kBlock)
-            tot[v] += partials[(size_t(v) * f.W + sl) * f.G + g];
+        tot[v] = strided_sum(partials + (size_t(v) * f.W + sl) * f.G, f.G);
     __shared__ double lds[2 * kWaves];
     block_sum<2>(tot, lds);
     if (threadIdx.x == 0)
@@ -622,8 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_bfin_eq(const double* __restrict__ p
 {
     const int e = blockIdx.x;
     double tot[2] = {0.0, 0.0};
-    for (int g = threadIdx.x; g < f.G; g += kBlock)
-        tot[0] += partials[size_t(e) * f.G + g];
+    tot[0] = strided_sum(partials + size_t(e) * f.G, f.G);
     __shared__ double lds[2 * kWaves];
     block_sum<2>(tot, lds);
     if (threadIdx.x == 0)

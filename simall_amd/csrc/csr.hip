@@ -145,6 +145,8 @@ template <class T> static void sell_fields(const Pattern& P, StreamMat<T>& out)
     out.eci = P.sell.eci.p;
     out.nsl = P.sell.nsl;
     out.cw = nullptr;
+    if constexpr (std::is_same<T, float>::value)
+        out.ecd = P.sell.ecd.n ? P.sell.ecd.p : nullptr;
 }
 
 int stream_view64(mi_csr* A, hipStream_t s, StreamMat<double>& out, bool sell)
@@ -255,6 +257,7 @@ int launch_convert_f32(int64_t n, const double* src, float* dst, hipStream_t s)
 template <class T> struct StreamCopy
 {
     DevBuf<int> rp, blk, hdr, ci, soff;
+    DevBuf<short> cd;
     DevBuf<T> val;
     StreamMat<T> V;
     // padded: slots of the SELL-64 form when A carries it
@@ -271,9 +274,11 @@ template <class T> struct StreamCopy
         if (A.soff) {
             MI_TRY(dup(soff, A.soff, size_t(A.nsl) + 1));
             MI_TRY(dup(ci, A.eci, size_t(padded)));
+            MI_TRY(dup(cd, A.ecd, size_t(padded)));
             MI_TRY(dup(val, A.val, size_t(padded)));
             V.soff = soff.p;
             V.eci = ci.p;
+            V.ecd = A.ecd ? cd.p : nullptr;
             V.val = val.p;
             return MI_OK;
         }

@@ -255,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_dot(CgArgs<T> a)
     if (a.st->done)
         return;
     StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr, a.soff, a.eci, a.nsl};
+    A.ecd = a.ecd;
     EpiCgAp<T> e{a.p, a.Ap};
     rows_of<kSell>(A, a.p, e);
     double v[1] = {e.acc};
@@ -462,6 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_bi_spmv(BiArgs<T> a, int mode)
     if (a.st->done)
         return;
     StreamMat<T> A{a.n, a.nrb, a.blk, a.rp, a.ci, a.val, a.diag, 1, a.cw, a.hdr, a.soff, a.eci, a.nsl};
+    A.ecd = a.ecd;
     EpiBi<T> e{mode == 0 ? a.rh : a.s, mode == 0 ? a.v : a.t, mode};
     rows_of<kSell>(A, mode == 0 ? a.y : a.z, e);
     double v[2] = {e.acc0, e.acc1};

@@ -102,6 +102,7 @@ template <class T> struct CgArgs
     const int* hdr;     // row block boundaries {row, nonzero, column base, 0}, else nullptr
     const int* soff;    // SELL-64 form (val in its slot order), else nullptr
     const int* eci;
+    const short* ecd; // FP32 SELL-64: 16-bit column offsets, or nullptr
     int nsl;
     // FP32 inner solves of the mixed form: when set, the iterate is
     // accumulated here in FP64 (xacc += alpha p) instead of in x, which is
@@ -148,6 +149,7 @@ template <class T> struct BiArgs
     const int* hdr;     // as in CgArgs
     const int* soff;
     const int* eci;
+    const short* ecd; // FP32 SELL-64: 16-bit column offsets, or nullptr
     int nsl;
     double* xacc;       // as in CgArgs
 };

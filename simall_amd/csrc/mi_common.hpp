@@ -45,6 +45,9 @@ int ensure_device();
 // CSR order (for comparisons; results are identical either way).
 bool column_order_enabled();
 bool sell_enabled();
+// Whether square SELL-64 forms get 16-bit column offsets (default on;
+// SIMALL_AMD_NO_SELL16=1 keeps int32 columns, for comparisons).
+bool sell_delta_enabled();
 // Graph replays (V-cycles, chunks of batched iterations) are on unless the
 // environment sets SIMALL_AMD_NO_GRAPH=1, which launches the same kernels
 // directly: rocprofv3's kernel trace has ended the host process with SIGSEGV
@@ -196,18 +199,37 @@ template <int NV> __device__ __forceinline__ void block_partials(double (&v)[NV]
             partials[size_t(j) * gridDim.x + blockIdx.x] = v[j];
 }
 
+// This thread's share of G partials: p[t] + p[t + kBlock] + ... (ascending,
+// t = threadIdx.x).  The loads are issued kFoldBatch at a time before any is
+// added, so a one-block fold waits for G / (kFoldBatch kBlock) memory round
+// trips instead of G / kBlock; slots past G add +0.0, which leaves the sum
+// (never -0.0: it starts at +0.0) bit for bit unchanged.
+constexpr int kFoldBatch = 8;
+__device__ __forceinline__ double strided_sum(const double* __restrict__ p, int G)
+{
+    double s = 0.0;
+    for (int b0 = threadIdx.x; b0 < G; b0 += kFoldBatch * kBlock) {
+        double v[kFoldBatch];
+#pragma unroll
+        for (int q = 0; q < kFoldBatch; ++q) {
+            const int b = b0 + q * kBlock;
+            v[q] = b < G ? p[b] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < kFoldBatch; ++q)
+            s += v[q];
+    }
+    return s;
+}
+
 // Totals over G partial blocks, valid in thread 0 of a one-block launch.
 template <int NV>
 __device__ __forceinline__ void fold_partials(const double* partials, int G, double (&tot)[NV])
 {
     __shared__ double lds[NV * (kBlock / kWave)];
 #pragma unroll
-    for (int j = 0; j < NV; ++j) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < G; b += kBlock)
-            s += partials[size_t(j) * G + b];
-        tot[j] = s;
-    }
+    for (int j = 0; j < NV; ++j)
+        tot[j] = strided_sum(partials + size_t(j) * G, G);
     block_sum<NV>(tot, lds);
 }
 

@@ -70,6 +70,15 @@ bool graphs_enabled()
     return on;
 }
 
+bool sell_delta_enabled()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("SIMALL_AMD_NO_SELL16");
+        return !(e && std::strcmp(e, "1") == 0);
+    }();
+    return on;
+}
+
 bool sell_enabled()
 {
     static const bool on = [] {

@@ -486,6 +486,7 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
         a.hdr = V.hdr;
         a.soff = V.soff;
         a.eci = V.eci;
+        a.ecd = V.ecd;
         a.nsl = V.nsl;
         a.nrb = P.rb.nrb;
         a.xacc = xacc;
@@ -523,6 +524,7 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
     a.hdr = V.hdr;
     a.soff = V.soff;
     a.eci = V.eci;
+    a.ecd = V.ecd;
     a.nsl = V.nsl;
     a.xacc = xacc;
     const bool jacobi = invD != nullptr;

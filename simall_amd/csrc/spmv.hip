@@ -173,6 +173,7 @@ int SellForm::build(int n, int64_t nnz, const int* rp, const int* ci, bool mark_
     soff.release();
     eci.release();
     eperm.release();
+    ecd.release();
     nsl = 0;
     padded = 0;
     if (n <= 0 || n >= kSellDiag || !sell_enabled())
@@ -224,6 +225,28 @@ int SellForm::build(int n, int64_t nnz, const int* rp, const int* ci, bool mark_
     MI_TRY(soff.upload(off.data(), off.size(), s));
     MI_TRY(eci.upload(c.data(), c.size(), s));
     MI_TRY(eperm.upload(pm.data(), pm.size(), s));
+    // 16-bit column offsets from the row, for square operators within reach
+    std::vector<short> d;
+    if (mark_diag && sell_delta_enabled()) {
+        bool ok = true;
+        d.assign(size_t(tot), kSellDelta);
+        std::vector<int> ndiag(size_t(n), 0);
+        for (int sl = 0; sl < ns && ok; ++sl) {
+            const int r1 = std::min(n, (sl + 1) * kSellSlice);
+            for (int i = sl * kSellSlice; i < r1 && ok; ++i)
+                for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                    const int64_t off_c = int64_t(ci[k]) - i;
+                    if (off_c < -32767 || off_c > 32767 || (off_c == 0 && ++ndiag[size_t(i)] > 1)) {
+                        ok = false;
+                        break;
+                    }
+                    d[size_t(off[size_t(sl)]) + size_t(kSellSlice) * size_t(k - rp[i]) +
+                      size_t(i - sl * kSellSlice)] = short(off_c);
+                }
+        }
+        if (ok)
+            MI_TRY(ecd.upload(d.data(), d.size(), s));
+    }
     MI_HIP(hipStreamSynchronize(s));
     nsl = ns;
     padded = tot;

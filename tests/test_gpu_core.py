@@ -437,3 +437,48 @@ def test_values_written_through_the_device_pointer(gpu):
     capi.check(gpu.mi_csr_update_values_device(A.h, p))
     xs, st, rc = s.solve(y0)
     assert rc == capi.MI_OK and np.allclose(xs, x, rtol=0, atol=1e-8)
+
+
+def _fp32_rowsum(rp, ci, va, x):
+    """y_i = sum over the row's entries in CSR order of fl32(a * x), each
+    addition rounded to FP32: the FP32 SELL-64 kernels' arithmetic."""
+    n = rp.size - 1
+    lens = np.diff(rp)
+    a32, x32 = va.astype(np.float32), x.astype(np.float32)
+    s = np.zeros(n, np.float32)
+    for k in range(int(lens.max(initial=0))):
+        rows = np.nonzero(lens > k)[0]
+        idx = rp[rows] + k
+        s[rows] = (s[rows] + a32[idx] * x32[ci[idx]]).astype(np.float32)
+    return s
+
+
+@pytest.mark.parametrize("band", ["narrow", "wide"])
+def test_spmv_sell_fp32_column_forms(gpu, band):
+    """A square SELL-64 operator whose entries lie within 32767 of the
+    diagonal streams 16-bit column offsets in its FP32 kernels (6 instead of
+    8 bytes per slot), a wider one int32 columns; both sum the same FP32
+    products in CSR order, equal to an FP32 emulation bit for bit."""
+    rng = np.random.default_rng(11)
+    n = 70000
+    reach = 2000 if band == "narrow" else 60000
+    offs = rng.integers(-reach, reach + 1, size=(n, 7))
+    lens0 = rng.integers(0, 8, n)
+    rows = []
+    for i in range(n):
+        c = np.clip(i + offs[i, :lens0[i]], 0, n - 1)
+        rows.append(np.unique(np.concatenate([[i], c])))  # one diagonal entry per row
+    lens = np.array([r.size for r in rows])
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    ci = np.concatenate(rows).astype(np.int32)
+    rp = rp.astype(np.int32)
+    far = np.abs(ci.astype(np.int64) - np.repeat(np.arange(n), lens)).max()
+    assert (far < 32767) == (band == "narrow")
+    va = rng.standard_normal(ci.size)
+    x = rng.standard_normal(n)
+    A = capi.Csr(rp, ci, va)
+    dx32 = capi.DeviceArray.from_host(x.astype(np.float32))
+    dy32 = capi.DeviceArray(n, np.float32)
+    A.spmv_device(dx32, dy32, capi.MI_PRECISION_MIXED)
+    assert np.array_equal(dy32.download(), _fp32_rowsum(rp, ci, va, x))
