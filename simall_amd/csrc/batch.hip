@@ -168,82 +168,172 @@ __device__ __forceinline__ void block_range(int64_t count, int64_t align, int64_
 }
 
 // ------------------------------------------------------------------ kernels
+// V consecutive FP32 slots of one row as one load (V = 1, 2 or 4).
+template <int V> struct SlotVec
+{
+    using type = float __attribute__((ext_vector_type(V)));
+    static __device__ __forceinline__ float at(const type& x, int u) { return x[u]; }
+};
+template <> struct SlotVec<1>
+{
+    using type = float;
+    static __device__ __forceinline__ float at(float x, int) { return x; }
+};
+
+// FP64 sums of up to two values per slot over a workgroup whose threads
+// each hold V consecutive slots of one row group (TW = W / V threads per
+// row, R = kBlock / TW row groups), in a fixed order (row groups 0 .. R-1
+// of each slot); partials[v][s][block].  V = 1 is the SlotMap arrangement.
+template <int V>
+__device__ __forceinline__ void slot_partials_v(const BArgs& a, int R, int grp, int s0, bool live,
+                                                int nv, const double (&d0)[V],
+                                                const double (&d1)[V])
+{
+    __shared__ double lds[2][kBlock * V];
+    if (live)
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+            lds[0][grp * a.W + s0 + u] = d0[u];
+            lds[1][grp * a.W + s0 + u] = d1[u];
+        }
+    __syncthreads();
+    const int G = gridDim.x;
+    for (int t = threadIdx.x; t < nv * a.W; t += kBlock) {
+        const int v = t / a.W, sl = t % a.W;
+        double sum = 0.0;
+        for (int q = 0; q < R; ++q)
+            sum += lds[v][q * a.W + sl];
+        a.partials[(size_t(v) * a.W + sl) * G + blockIdx.x] = sum;
+    }
+}
+
 // Batched SpMV out = A in over the slot layout, with FP64 partials per slot:
 //   B_ALPHA     Ap = A p;  p.Ap                (CG)
 //   B_BI_ALPHA  v = A y;   rh.v                (BiCGStab)
 //   B_BI_T      t = A z;   t.s, t.t            (BiCGStab, s held in r)
-// W threads per row (one per slot), kBlock / W rows per workgroup at a time;
-// each thread sums its equation's row in CSR order.
-template <int OP, bool kSell>
+// W / V threads per row, each taking V consecutive slots (equations) with
+// V-wide loads of the values and of the gathered rows of `in`: kBlock V / W
+// rows per workgroup at a time; every slot's row is summed in CSR order.
+template <int OP, bool kSell, int V>
 __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 {
-    const SlotMap m(a.W);
-    const int W = m.W, sl = m.sl, grp = m.rr, R = m.P;
-    const int e = m.live ? a.eq[sl] : -1;
-    const bool on = slot_on(a, OP, e);
+    using VT = typename SlotVec<V>::type;
+    constexpr int U = V == 1 ? kSpmmUnroll : kSpmmUnroll / 2; // row entries per pass
+    const int W = a.W;
+    const int TW = W / V;          // threads per row
+    const int R = kBlock / TW;     // rows per workgroup pass
+    const int t = int(threadIdx.x);
+    const bool live = t < R * TW;
+    const int grp = t / TW;
+    const int s0 = (t % TW) * V;   // first slot of this thread
+    bool on[V];
+    bool any = false, all = true;
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        on[u] = live && slot_on(a, OP, a.eq[s0 + u]);
+        any = any || on[u];
+        all = all && on[u];
+    }
     const float* vin = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
     float* vout = OP == B_BI_T ? a.t : a.Ap;
     const float* vw = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
     int64_t r0, r1;
     block_range(a.n, R, r0, r1);
-    double d0 = 0.0, d1 = 0.0;
-    if (on) {
+    double d0[V], d1[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+        d0[u] = d1[u] = 0.0;
+    if (any) {
         for (int64_t i = r0 + grp; i < r1; i += R) {
-            float acc = 0.f;
+            const size_t q = size_t(i) * W + s0;
+            // the dot product's operand, loaded ahead of the row's streams
+            const VT wq = *reinterpret_cast<const VT*>(vw + q);
+            VT acc = VT(0.f);
             if constexpr (kSell) {
                 // slot k of row i: m0 + 64 k, in CSR order
                 const int slc = int(i / kSellSlice);
                 const int m0 = a.soff[slc] + int(i % kSellSlice);
                 const int w = (a.soff[slc + 1] - a.soff[slc]) / kSellSlice;
-                for (int k0 = 0; k0 < w; k0 += kSpmmUnroll) {
-                    int c[kSpmmUnroll];
-                    float v[kSpmmUnroll];
+                for (int k0 = 0; k0 < w; k0 += U) {
+                    int c[U];
+                    VT v[U];
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j) {
+                    for (int j = 0; j < U; ++j) {
                         const size_t mk = size_t(m0) + size_t(k0 + j) * kSellSlice;
                         const bool in = k0 + j < w;
                         c[j] = in ? a.eci[mk] : -1;
-                        v[j] = in ? __builtin_nontemporal_load(&a.v32[mk * W + sl]) : 0.f;
+                        v[j] = in ? __builtin_nontemporal_load(
+                                        reinterpret_cast<const VT*>(a.v32 + mk * W + s0))
+                                  : VT(0.f);
                     }
-                    float xv[kSpmmUnroll];
+                    VT xv[U];
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j)
-                        xv[j] = c[j] >= 0 ? vin[size_t(c[j] & kSellColMask) * W + sl] : 0.f;
+                    for (int j = 0; j < U; ++j)
+                        xv[j] = c[j] >= 0 ? *reinterpret_cast<const VT*>(
+                                                vin + size_t(c[j] & kSellColMask) * W + s0)
+                                          : VT(0.f);
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j)
+                    for (int j = 0; j < U; ++j)
                         if (c[j] >= 0)
                             acc += v[j] * xv[j];
                 }
             } else {
                 const int kb = a.rp[i], ke = a.rp[i + 1];
-                for (int k0 = kb; k0 < ke; k0 += kSpmmUnroll) {
-                    int c[kSpmmUnroll];
-                    float v[kSpmmUnroll];
+                for (int k0 = kb; k0 < ke; k0 += U) {
+                    int c[U];
+                    VT v[U];
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j) {
+                    for (int j = 0; j < U; ++j) {
                         const int k = k0 + j;
                         c[j] = k < ke ? a.ci[k] : -1;
-                        v[j] = k < ke ? __builtin_nontemporal_load(&a.v32[size_t(k) * W + sl])
-                                      : 0.f;
+                        v[j] = k < ke ? __builtin_nontemporal_load(
+                                            reinterpret_cast<const VT*>(a.v32 + size_t(k) * W + s0))
+                                      : VT(0.f);
                     }
-                    float xv[kSpmmUnroll];
+                    VT xv[U];
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j)
-                        xv[j] = c[j] >= 0 ? vin[size_t(c[j]) * W + sl] : 0.f;
+                    for (int j = 0; j < U; ++j)
+                        xv[j] = c[j] >= 0 ? *reinterpret_cast<const VT*>(vin + size_t(c[j]) * W + s0)
+                                          : VT(0.f);
 #pragma unroll
-                    for (int j = 0; j < kSpmmUnroll; ++j)
+                    for (int j = 0; j < U; ++j)
                         if (c[j] >= 0)
                             acc += v[j] * xv[j];
                 }
             }
-            const size_t q = size_t(i) * W + sl;
-            vout[q] = acc;
-            d0 += double(vw[q]) * double(acc);
-            if (OP == B_BI_T)
-                d1 += double(acc) * double(acc);
+            if (all) {
+                *reinterpret_cast<VT*>(vout + q) = acc;
+            } else {
+#pragma unroll
+                for (int u = 0; u < V; ++u)
+                    if (on[u])
+                        vout[q + u] = SlotVec<V>::at(acc, u);
+            }
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                if (!on[u])
+                    continue;
+                const double au = double(SlotVec<V>::at(acc, u));
+                d0[u] += double(SlotVec<V>::at(wq, u)) * au;
+                if (OP == B_BI_T)
+                    d1[u] += au * au;
+            }
         }
     }
-    slot_partials(a, m, OP == B_BI_T ? 2 : 1, d0, d1);
+    slot_partials_v<V>(a, R, grp, s0, live, OP == B_BI_T ? 2 : 1, d0, d1);
+}
+
+// The SpMV kernel for the slot width W of a launch: V = 4, 2 or 1 slots per
+// thread, the widest that divides W (measured on a 1M-row 7-point batch:
+// V = 4 is the fastest at every width from 4 to 32 equations).
+template <int OP> using BSpmmKernel = void (*)(BArgs);
+template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, int W)
+{
+    if (W % 4 == 0)
+        return sell ? k_bspmm<OP, true, 4> : k_bspmm<OP, false, 4>;
+    if (W % 2 == 0)
+        return sell ? k_bspmm<OP, true, 2> : k_bspmm<OP, false, 2>;
+    return sell ? k_bspmm<OP, true, 1> : k_bspmm<OP, false, 1>;
 }
 
 // Vector stages over the slot layout (element q = i W + s; thread t of a
@@ -779,7 +869,9 @@ struct mi_batch
     std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM)
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
-    int E = 0, G = 8, Wmax = 1;
+    int E = 0, Wmax = 1;
+    int G = 8;  // workgroups of the batched SpMM (autotuned)
+    int Gv = 8; // workgroups of the vector, norm and true-residual stages (from n)
     int64_t n = 0, nnz = 0;
     bool sell = false;   // values in the pattern's SELL-64 slot order
     int64_t vrows = 0;   // value lines: nnz, or the SELL-64 slots
@@ -968,35 +1060,38 @@ int repack(mi_batch* B, const std::vector<int>& keep, bool bicg)
     return MI_OK;
 }
 
-int fin(mi_batch* B, int op, const BFinArgs& f, Launcher& L)
+// Folds the partials of a producing launch of G workgroups.
+int fin(mi_batch* B, int op, const BFinArgs& f, int G, Launcher& L)
 {
     const BSlots& S = B->set[B->cur];
+    BFinArgs g = f;
+    g.G = G;
     return L(k_bfin, unsigned(S.W), kBlock, static_cast<const double*>(B->partials.p),
-             static_cast<const int*>(S.d_eq.p), B->st.p, op, f);
+             static_cast<const int*>(S.d_eq.p), B->st.p, op, g);
 }
 
 // One CG iteration for every iterating equation (five launches).
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(B->sell ? k_bspmm<B_ALPHA, true> : k_bspmm<B_ALPHA, false>, B->G, kBlock, a));
-    MI_TRY(fin(B, B_ALPHA, f, L));
-    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_UPDATE)));
-    MI_TRY(fin(B, B_UPDATE, f, L));
-    return L(k_bvec, B->G, kBlock, a, int(B_P));
+    MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, a.W), B->G, kBlock, a));
+    MI_TRY(fin(B, B_ALPHA, f, B->G, L));
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPDATE)));
+    MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
+    return L(k_bvec, B->Gv, kBlock, a, int(B_P));
 }
 
 // One BiCGStab iteration for every iterating equation (nine launches).
 int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_P)));
-    MI_TRY(L(B->sell ? k_bspmm<B_BI_ALPHA, true> : k_bspmm<B_BI_ALPHA, false>, B->G, kBlock, a));
-    MI_TRY(fin(B, B_BI_ALPHA, f, L));
-    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_S)));
-    MI_TRY(fin(B, B_BI_S, f, L));
-    MI_TRY(L(B->sell ? k_bspmm<B_BI_T, true> : k_bspmm<B_BI_T, false>, B->G, kBlock, a));
-    MI_TRY(fin(B, B_BI_OMEGA, f, L));
-    MI_TRY(L(k_bvec, B->G, kBlock, a, int(B_BI_UPDATE)));
-    return fin(B, B_BI_UPDATE, f, L);
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_P)));
+    MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, a.W), B->G, kBlock, a));
+    MI_TRY(fin(B, B_BI_ALPHA, f, B->G, L));
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_S)));
+    MI_TRY(fin(B, B_BI_S, f, B->Gv, L));
+    MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, a.W), B->G, kBlock, a));
+    MI_TRY(fin(B, B_BI_OMEGA, f, B->G, L));
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_UPDATE)));
+    return fin(B, B_BI_UPDATE, f, B->Gv, L);
 }
 
 // Autotuner: workgroups of the batched launches, timed on this pattern in
@@ -1004,6 +1099,9 @@ int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher&
 int autotune(mi_batch* B)
 {
     B->G = 8;
+    // vector stages: one workgroup per 256 rows, up to kBatchMaxGrid (the
+    // SpMM's best grid can be far smaller and would leave them short of waves)
+    B->Gv = int(std::min<int64_t>(kBatchMaxGrid, std::max<int64_t>(8, (B->n + 2047) / 2048 * 8)));
     if (B->n == 0)
         return MI_OK;
     std::vector<int> all(size_t(B->E));
@@ -1020,7 +1118,7 @@ int autotune(mi_batch* B)
         if (g > kBatchMaxGrid)
             break;
         Launcher L{B->stream};
-        auto k = B->sell ? k_bspmm<B_ALPHA, true> : k_bspmm<B_ALPHA, false>;
+        auto k = bspmm_kernel<B_ALPHA>(B->sell, a.W);
         MI_TRY(L(k, unsigned(g), kBlock, a));
         MI_HIP(hipEventRecord(e0, B->stream));
         const int reps = 3;
@@ -1091,7 +1189,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     }
     BFinArgs f{};
     f.E = E;
-    f.G = B->G;
+    f.G = B->Gv;
     f.max_iter = c.max_iterations;
     f.max_outer = c.max_outer > 0 ? c.max_outer : 20;
     f.tol = c.tolerance;
@@ -1106,14 +1204,14 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     k_bfill<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(int64_t(m), B->x.p);
     MI_HIP(hipGetLastError());
     const HArgs h = B->hargs();
-    k_bnorm<<<B->G, kBlock, 0, s>>>(h);
+    k_bnorm<<<B->Gv, kBlock, 0, s>>>(h);
     k_bfin_eq<<<E, kBlock, 0, s>>>(B->partials.p, B->st.p, B_BNORM, f);
     MI_HIP(hipGetLastError());
     const int chunk = c.check_interval > 0 ? c.check_interval : 8;
     const bool use_graph = c.use_graph != 0 && graphs_enabled();
     for (;;) {
         // FP64 true residual of the unfinished equations; which need a sweep
-        k_btrue<<<B->G, kBlock, 0, s>>>(h);
+        k_btrue<<<B->Gv, kBlock, 0, s>>>(h);
         k_bfin_eq<<<E, kBlock, 0, s>>>(B->partials.p, B->st.p, B_TRUE, f);
         MI_HIP(hipGetLastError());
         MI_TRY(read_state(B));
@@ -1138,8 +1236,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             MI_HIP(hipGetLastError());
             Launcher L{s};
             f.W = S.W;
-            MI_TRY(L(k_bvec, B->G, kBlock, B->args(B->cur), int(bicg ? B_BI_INIT : B_INIT)));
-            MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f, L));
+            MI_TRY(L(k_bvec, B->Gv, kBlock, B->args(B->cur), int(bicg ? B_BI_INIT : B_INIT)));
+            MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f, B->Gv, L));
         }
         for (;;) {
             const BArgs a = B->args(B->cur);
