@@ -84,6 +84,87 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(StreamMat<T> A, const T* _
     rows_of<kSell>(A, din, e);
 }
 
+// The first level-0 launch of a V-cycle fused with a CG iteration: the CG
+// update of CGSolver::solve (src/solver/LinearSolvers.cpp:430-450; x +=
+// alpha p, r -= alpha Ap, partials of r.r) and then, from the updated r,
+// k_cheb_init's start (x0 = d = a0 D^-1 r, r0 = r).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cheb_init_cg(int n, CgFuse<T> f,
+                                                         const T* __restrict__ invD, T a0,
+                                                         T* __restrict__ x0, T* __restrict__ r0,
+                                                         T* __restrict__ d)
+{
+    if (f.st->done)
+        return;
+    const T al = T(f.st->alpha);
+    const double al64 = f.st->alpha;
+    const RowRange rr = block_rows(n, gridDim.x, 1);
+    double v[1] = {0.0};
+    for (int i = rr.beg + threadIdx.x; i < rr.end; i += kBlock) {
+        if (f.xacc)
+            f.xacc[i] += al64 * double(f.p[i]);
+        else
+            f.x[i] = f.x[i] + al * f.p[i];
+        const T ri = f.r[i] - al * f.Ap[i];
+        f.r[i] = ri;
+        v[0] += double(ri) * double(ri);
+        const T di = a0 * invD[i] * ri;
+        r0[i] = ri;
+        d[i] = di;
+        x0[i] = di;
+    }
+    block_partials<1>(v, f.prr);
+}
+
+// The last level-0 launch of a fused V-cycle: a mode-0 Chebyshev step that
+// also accumulates b0 . x (the CG's r.z, x being the cycle's result z).
+template <class T> struct EpiChebDot : EpiCheb<T>
+{
+    const T* b0;
+    T bi = 0;
+    double acc = 0.0;
+    __device__ __forceinline__ void pre(int i)
+    {
+        EpiCheb<T>::pre(i);
+        bi = b0[i];
+    }
+    __device__ __forceinline__ void row(int i, typename acc_type<T>::type s)
+    {
+        const T ri = this->ri0 - T(s);
+        this->r[i] = ri;
+        const T dn = this->beta * this->di + this->alpha * this->id * ri;
+        this->dout[i] = dn;
+        const T xn = this->xi + dn;
+        this->x[i] = xn;
+        acc += double(bi) * double(xn);
+    }
+};
+
+template <class T, bool kSell>
+__global__ __launch_bounds__(kBlock) void k_cheb_step_dot(StreamMat<T> A, const T* __restrict__ din,
+                                                          T* __restrict__ dout, T* __restrict__ r,
+                                                          const T* __restrict__ invD,
+                                                          T* __restrict__ x, T beta, T alpha,
+                                                          const T* __restrict__ b0,
+                                                          const KState* st, double* prz)
+{
+    if (st->done)
+        return;
+    EpiChebDot<T> e;
+    e.din = din;
+    e.dout = dout;
+    e.r = r;
+    e.invD = invD;
+    e.x = x;
+    e.beta = beta;
+    e.alpha = alpha;
+    e.mode = 0;
+    e.b0 = b0;
+    rows_of<kSell>(A, din, e);
+    double v[1] = {e.acc};
+    block_partials<1>(v, prz);
+}
+
 // e = P x_c ; x += e
 template <class T> struct EpiProlong
 {
@@ -306,12 +387,22 @@ template <class T> static int step(const DMat<T>& A, const T* din, T* dout, T* r
              A.sm(), din, dout, r, invD, x, T(beta), T(alpha), mode);
 }
 
+template <class T>
+static int step_dot(const DMat<T>& A, const T* din, T* dout, T* r, const T* invD, T* x,
+                    double beta, double alpha, const CgFuse<T>& f, Launcher& L)
+{
+    return L(A.sell.built() ? k_cheb_step_dot<T, true> : k_cheb_step_dot<T, false>, A.grid(),
+             kBlock, A.sm(), din, dout, r, invD, x, T(beta), T(alpha),
+             static_cast<const T*>(f.r), static_cast<const KState*>(f.st), f.prz);
+}
+
 template <class T> int DHierarchy<T>::build(const AmgHierarchyHost& H, int degree,
                                             int coarse_degree, double lmin_ratio, hipStream_t s,
                                             const double* cinv_device)
 {
     const int nl = int(H.levels.size());
     graphs.clear();
+    cg_graphs.clear();
     L.clear();
     L.resize(size_t(nl));
     deg = std::max(degree, 1);
@@ -361,6 +452,7 @@ int DHierarchy<T>::refresh(const std::vector<LevelValues>& v, const double* cinv
         return set_error(MI_ERR_BAD_ARG, "refresh of %d levels for a hierarchy of %d",
                          int(v.size()), nl);
     graphs.clear(); // the captured cycles hold the old Chebyshev coefficients
+    cg_graphs.clear();
     for (int l = 0; l < nl; ++l) {
         DLevel<T>& d = L[size_t(l)];
         const LevelValues& lv = v[size_t(l)];
@@ -736,7 +828,22 @@ template <class T> int DHierarchy<T>::vcycle(const T* b0, T* x0, hipStream_t s, 
     return cycle(b0, x0, L);
 }
 
-template <class T> int DHierarchy<T>::cycle(const T* b0, T* x0, Launcher& Lc)
+template <class T> int DHierarchy<T>::vcycle_cg(const CgFuse<T>& f, T* x0, hipStream_t s,
+                                                bool use_graph, unsigned* grr, unsigned* grz)
+{
+    if (L.size() < 2 || L[0].A.nr == 0)
+        return set_error(MI_ERR_UNSUPPORTED, "fused CG V-cycle needs two levels or more");
+    *grr = range_grid(L[0].A.nr, 256);
+    *grz = L[0].A.grid();
+    if (use_graph) {
+        const void* key[GraphCache::kKey] = {f.r, x0, f.p, f.Ap, f.x, f.xacc, f.st, f.prr, f.prz};
+        return cg_graphs.run_keyed(key, s, [&](Launcher& Lc) -> int { return cycle(f.r, x0, Lc, &f); });
+    }
+    Launcher Lc{s};
+    return cycle(f.r, x0, Lc, &f);
+}
+
+template <class T> int DHierarchy<T>::cycle(const T* b0, T* x0, Launcher& Lc, const CgFuse<T>* cf)
 {
     const int nl = int(L.size());
     auto bptr = [&](int l) -> const T* { return l == 0 ? b0 : L[size_t(l)].b.p; };
@@ -747,8 +854,12 @@ template <class T> int DHierarchy<T>::cycle(const T* b0, T* x0, Launcher& Lc)
         const int n = d.A.nr;
         const T* b = bptr(l);
         T* x = xptr(l);
-        MI_TRY(Lc(k_cheb_init<T>, rows_grid(n, 1), kBlock, n, b, d.invD.p, T(d.alpha[0]), x,
-                  d.r.p, d.d.p));
+        if (cf && l == 0)
+            MI_TRY(Lc(k_cheb_init_cg<T>, range_grid(n, 256), kBlock, n, *cf,
+                      static_cast<const T*>(d.invD.p), T(d.alpha[0]), x, d.r.p, d.d.p));
+        else
+            MI_TRY(Lc(k_cheb_init<T>, rows_grid(n, 1), kBlock, n, b, d.invD.p, T(d.alpha[0]), x,
+                      d.r.p, d.d.p));
         T* din = d.d.p;
         T* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
@@ -778,13 +889,23 @@ template <class T> int DHierarchy<T>::cycle(const T* b0, T* x0, Launcher& Lc)
         if (n > 0)
             MI_TRY(Lc(d.P.sell.built() ? k_prolong<T, true> : k_prolong<T, false>, d.P.grid(),
                       kBlock, d.P.sm(), xc, d.e.p, x));
-        // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps
-        MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, Lc));
+        // r -= A e ; d = alpha0 D^-1 r ; x += d, then the remaining steps; the
+        // cycle's last launch (level 0, last step) of a fused CG iteration
+        // also stores the r.z partials
+        const bool dot = cf && l == 0;
+        if (dot && d.deg == 1)
+            MI_TRY(step_dot<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], *cf, Lc));
+        else
+            MI_TRY(step<T>(d.A, d.e.p, d.d.p, d.r.p, d.invD.p, x, 0.0, d.alpha[0], 0, Lc));
         T* din = d.d.p;
         T* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
-            MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)], d.alpha[size_t(k)],
-                           0, Lc));
+            if (dot && k + 1 == d.deg)
+                MI_TRY(step_dot<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)],
+                                   d.alpha[size_t(k)], *cf, Lc));
+            else
+                MI_TRY(step<T>(d.A, din, dout, d.r.p, d.invD.p, x, d.beta[size_t(k)],
+                               d.alpha[size_t(k)], 0, Lc));
             std::swap(din, dout);
         }
     }
@@ -954,6 +1075,25 @@ struct AmgPrecond final : Precond
         if (!has32)
             return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP32 copy");
         return h32.vcycle(r, z, s, use_graph);
+    }
+    bool cg_fusable(bool fp64) const override
+    {
+        const size_t nl = fp64 ? (has64 ? h64.L.size() : 0) : (has32 ? h32.L.size() : 0);
+        return nl >= 2;
+    }
+    int apply_cg64(const CgFuse<double>& f, double* z, unsigned* grr, unsigned* grz,
+                   hipStream_t s) override
+    {
+        if (!has64)
+            return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP64 copy");
+        return h64.vcycle_cg(f, z, s, use_graph, grr, grz);
+    }
+    int apply_cg32(const CgFuse<float>& f, float* z, unsigned* grr, unsigned* grz,
+                   hipStream_t s) override
+    {
+        if (!has32)
+            return set_error(MI_ERR_UNSUPPORTED, "AMG hierarchy has no FP32 copy");
+        return h32.vcycle_cg(f, z, s, use_graph, grr, grz);
     }
     const char* name() const override { return "sa-amg"; }
     int levels() const override { return int(host.levels.size()); }

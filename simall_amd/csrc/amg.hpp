@@ -73,7 +73,8 @@ template <class T> struct DHierarchy
     int nc = 0;
     int deg = 2;
     double lmin_ratio = 0.3;
-    GraphCache graphs; // captured V-cycles
+    GraphCache graphs;    // captured V-cycles
+    GraphCache cg_graphs; // captured V-cycles fused with a CG iteration
     // degree: Chebyshev degree on level 0; coarse_degree on the coarser
     // levels (< 1: the same as level 0).
     // cinv_device: the coarsest inverse already on the device (n_c x n_c,
@@ -88,8 +89,15 @@ template <class T> struct DHierarchy
     // x0 = V-cycle(b0); x0 needs no initialisation.  With use_graph the
     // cycle's launches are replayed from a graph captured for (b0, x0, s).
     int vcycle(const T* b0, T* x0, hipStream_t s, bool use_graph = false);
-    // the launches of one V-cycle, issued through L (at once or into a graph)
-    int cycle(const T* b0, T* x0, Launcher& L);
+    // x0 = V-cycle(f.r) inside a CG iteration (CgFuse): the first level-0
+    // launch applies the CG update to f.r before the cycle reads it and
+    // stores r.r partials (*grr workgroups), the last stores r.z partials
+    // (*grz workgroups).  Needs two levels or more.
+    int vcycle_cg(const CgFuse<T>& f, T* x0, hipStream_t s, bool use_graph, unsigned* grr,
+                  unsigned* grz);
+    // the launches of one V-cycle, issued through L (at once or into a graph);
+    // cf: the fused CG iteration, or nullptr
+    int cycle(const T* b0, T* x0, Launcher& L, const CgFuse<T>* cf = nullptr);
 };
 
 std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c);

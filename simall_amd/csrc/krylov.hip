@@ -391,6 +391,43 @@ template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s)
     return fin(1, a.partials, a.grid, FIN_CG_UPDATE, a.st, a.history, 0, 0, s);
 }
 
+template <class T> int cg_iteration_spmv(const CgArgs<T>& a, hipStream_t s)
+{
+    return spmv_dot(a, s);
+}
+
+// The fused iteration's finalize: FIN_CG_UPDATE on r.r followed by FIN_CG_RZ
+// on r.z, each folded from its own producing launch.
+__global__ __launch_bounds__(kBlock) void k_fin_cg_fused(const double* __restrict__ prr, int grr,
+                                                         const double* __restrict__ prz, int grz,
+                                                         FinArgs f)
+{
+    KState* st = f.st;
+    if (st->done)
+        return;
+    double t[1], u[1];
+    fold_partials<1>(prr, grr, t);
+    fold_partials<1>(prz, grz, u);
+    if (threadIdx.x != 0)
+        return;
+    cg_stop_test(st, t[0], f.history);
+    if (!st->done) {
+        st->beta = u[0] / st->rz;
+        st->rz = u[0];
+    }
+}
+
+template <class T> int cg_iteration_fused_tail(const CgArgs<T>& a, const double* prr, unsigned grr,
+                                               const double* prz, unsigned grz, hipStream_t s)
+{
+    FinArgs f{a.st, a.history, 0, 0};
+    k_fin_cg_fused<<<1, kBlock, 0, s>>>(prr, int(grr), prz, int(grz), f);
+    MI_HIP(hipGetLastError());
+    k_cg_p<T><<<a.grid, kBlock, 0, s>>>(a);
+    MI_HIP(hipGetLastError());
+    return MI_OK;
+}
+
 template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s)
 {
     k_cg_rz<T><<<a.grid, kBlock, 0, s>>>(a);
@@ -555,6 +592,9 @@ template <class T> int bicg_stage3(const BiArgs<T>& a, hipStream_t s)
     template int cg_iteration_jacobi<T>(const CgArgs<T>&, hipStream_t);                          \
     template int cg_iteration_part1<T>(const CgArgs<T>&, hipStream_t);                           \
     template int cg_iteration_part2<T>(const CgArgs<T>&, hipStream_t);                           \
+    template int cg_iteration_spmv<T>(const CgArgs<T>&, hipStream_t);                            \
+    template int cg_iteration_fused_tail<T>(const CgArgs<T>&, const double*, unsigned,           \
+                                            const double*, unsigned, hipStream_t);               \
     template int bicg_init<T>(const BiArgs<T>&, bool, double, int, hipStream_t);                 \
     template int bicg_stage1<T>(const BiArgs<T>&, hipStream_t);                                  \
     template int bicg_stage2<T>(const BiArgs<T>&, hipStream_t);                                  \

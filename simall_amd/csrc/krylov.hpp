@@ -124,6 +124,27 @@ template <class T> int cg_iteration_jacobi(const CgArgs<T>& a, hipStream_t s);
 template <class T> int cg_iteration_part1(const CgArgs<T>& a, hipStream_t s);  // Ap, alpha, x, r, ||r||
 template <class T> int cg_iteration_part2(const CgArgs<T>& a, hipStream_t s);  // rz, beta, p
 
+// A CG iteration whose update and r.z reduction run inside the
+// preconditioner's own launches (the AMG V-cycle): the preconditioner's
+// first kernel applies x += alpha p, r -= alpha Ap and stores partials of
+// r.r, its last stores partials of r.z (r = the updated residual, z = M r).
+template <class T> struct CgFuse
+{
+    T* r;          // the CG residual (the apply's input), updated in place
+    const T* p;
+    const T* Ap;
+    T* x;          // the iterate, or nullptr when xacc is set
+    double* xacc;  // FP64 accumulation of the iterate (CgArgs::xacc)
+    KState* st;    // alpha; done: the fused kernels do nothing
+    double* prr;   // r.r partials, one per workgroup of the first kernel
+    double* prz;   // r.z partials, one per workgroup of the last kernel
+};
+template <class T> int cg_iteration_spmv(const CgArgs<T>& a, hipStream_t s); // Ap, alpha
+// After a fused apply: the stop test on r.r (grr partials), then beta and
+// rz from r.z (grz partials), then p = z + beta p.
+template <class T> int cg_iteration_fused_tail(const CgArgs<T>& a, const double* prr, unsigned grr,
+                                               const double* prz, unsigned grz, hipStream_t s);
+
 // --------------------------------------------------------- BiCGStab
 // Right-preconditioned BiCGStab of BiCGSTABSolver::solve
 // (src/solver/LinearSolvers.cpp:360-404).  KState: rz = rho_new (rh.r),

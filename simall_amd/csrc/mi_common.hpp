@@ -1,6 +1,7 @@
 // mi_common.hpp — shared host/device helpers for the gfx950 solve backend.
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -304,10 +305,12 @@ template <class F> int build_graph(hipStream_t s, F&& launches, hipGraphExec_t* 
 // use change.
 struct GraphCache
 {
+    // A captured sequence is replayed only for the stream and the pointers it
+    // was captured with (every buffer its launches read or write).
+    static constexpr int kKey = 10;
     struct Entry
     {
-        const void* in;
-        const void* out;
+        const void* key[kKey];
         hipStream_t s;
         hipGraphExec_t exec;
     };
@@ -325,18 +328,27 @@ struct GraphCache
     // launches(Launcher&) issues the sequence through the launcher it is given
     template <class F> int run(const void* in, const void* out, hipStream_t s, F&& launches)
     {
+        const void* key[kKey] = {in, out};
+        return run_keyed(key, s, std::forward<F>(launches));
+    }
+    template <class F> int run_keyed(const void* const (&key)[kKey], hipStream_t s, F&& launches)
+    {
         if (!graphs_enabled()) {
             Launcher L{s};
             return launches(L);
         }
         for (const Entry& e : entries)
-            if (e.in == in && e.out == out && e.s == s) {
+            if (e.s == s && std::equal(key, key + kKey, e.key)) {
                 MI_HIP(hipGraphLaunch(e.exec, s));
                 return MI_OK;
             }
         hipGraphExec_t exec = nullptr;
         MI_TRY(build_graph(s, launches, &exec));
-        entries.push_back(Entry{in, out, s, exec});
+        Entry e{};
+        std::copy(key, key + kKey, e.key);
+        e.s = s;
+        e.exec = exec;
+        entries.push_back(e);
         MI_HIP(hipGraphLaunch(exec, s));
         return MI_OK;
     }

@@ -395,6 +395,15 @@ template <class T> int precond(Precond* pc, const T* r, T* z, hipStream_t s)
         return pc->apply32(r, z, s);
 }
 
+template <class T>
+int precond_cg(Precond* pc, const CgFuse<T>& f, T* z, unsigned* grr, unsigned* grz, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8)
+        return pc->apply_cg64(f, z, grr, grz, s);
+    else
+        return pc->apply_cg32(f, z, grr, grz, s);
+}
+
 // Polls KState every `chunk` issued iterations until the device says done.
 // Issues iterations in chunks between polls of the device state.  Kernels of
 // an iteration after convergence return at once, but a preconditioner apply
@@ -535,9 +544,24 @@ int run_krylov(mi_solver* S, const T* invD, const T* b, T* x, DevBuf<T>* w, doub
         MI_TRY(precond<T>(pc, a.r, a.z, s));
         MI_TRY(cg_init_p<T>(a, tol, max_iter, s));
     }
+    // With an AMG V-cycle the CG update and the r.z reduction run inside the
+    // cycle's first and last launches (two launches and two vector passes
+    // fewer per iteration); the r.r and r.z partials go to their own
+    // sections of the partials buffer.
+    const bool fused = !jacobi && pc->cg_fusable(sizeof(T) == 8);
+    constexpr size_t kSection = size_t(std::max({kMaxReduceBlocks, kMaxStreamBlocks, kMaxSellBlocks}));
+    static_assert(3 * kSection <= kSection * kMaxReduceValues, "partials sections");
+    const CgFuse<T> f{a.r, a.p, a.Ap, a.xacc ? nullptr : a.x, a.xacc, a.st,
+                      a.partials + kSection, a.partials + 2 * kSection};
     return iterate(S, a.st, max_iter, chunk, [&]() -> int {
         if (jacobi)
             return cg_iteration_jacobi<T>(a, s);
+        if (fused) {
+            unsigned grr = 0, grz = 0;
+            MI_TRY(cg_iteration_spmv<T>(a, s));
+            MI_TRY(precond_cg<T>(pc, f, a.z, &grr, &grz, s));
+            return cg_iteration_fused_tail<T>(a, f.prr, grr, f.prz, grz, s);
+        }
         MI_TRY(cg_iteration_part1<T>(a, s));
         MI_TRY(precond<T>(pc, a.r, a.z, s));
         return cg_iteration_part2<T>(a, s);

@@ -51,6 +51,19 @@ struct Precond
     {
         return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no hierarchy", name());
     }
+    // CG fused with the apply (CgFuse): whether this preconditioner can run
+    // the CG update and the r.z reduction inside its own launches for the
+    // given precision, and that apply; *grr / *grz receive the workgroup
+    // counts of the launches that wrote the r.r / r.z partials.
+    virtual bool cg_fusable(bool /*fp64*/) const { return false; }
+    virtual int apply_cg64(const CgFuse<double>&, double*, unsigned*, unsigned*, hipStream_t)
+    {
+        return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no fused CG apply", name());
+    }
+    virtual int apply_cg32(const CgFuse<float>&, float*, unsigned*, unsigned*, hipStream_t)
+    {
+        return set_error(MI_ERR_UNSUPPORTED, "preconditioner %s has no fused CG apply", name());
+    }
     bool use_graph = true; // replay applies from captured HIP graphs
 };
 
