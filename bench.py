@@ -339,12 +339,13 @@ def cpu_leg(capi, wl, lib, A, b, x_mixed, d_b, d_x, sample_iters, core, host):
     return cpu, errors
 
 
-def measure_traffic():
-    """HBM bytes per launch of k_sell_spmv<double> from two rocprofv3 counter
-    passes (FETCH_SIZE, then WRITE_SIZE) over `bench.py --spmv-only`, each
-    scaled by what the same counter reads for the plain copy kernel of known
-    size that run also launches.  Runs the passes as child processes before
-    this process touches the GPU; returns (bytes or None, details)."""
+def measure_traffic(mode="--spmv-only", kernel="k_sell_spmv<double>", grid=None):
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE) over `bench.py <mode>`, each scaled by what
+    the same counter reads for the plain copy kernel of known size that run
+    also launches; `grid` (work-items) keeps only launches of that size.
+    Runs the passes as child processes before this process touches the GPU;
+    returns (bytes or None, details)."""
     import csv
     import glob
     import shutil
@@ -361,7 +362,7 @@ def measure_traffic():
         d = os.path.join(base, counter.lower())
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv",
                "-d", d, "-o", counter.lower(), "--", sys.executable, os.path.abspath(__file__),
-               "--spmv-only"]
+               mode]
         t1 = time.perf_counter()
         r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
                            stderr=subprocess.PIPE, text=True)
@@ -375,7 +376,8 @@ def measure_traffic():
                     name = row.get("Kernel_Name", "")
                     if row.get("Counter_Name") != counter:
                         continue
-                    key = ("spmv" if "k_sell_spmv<double>" in name else
+                    key = ("spmv" if kernel in name and (grid is None or
+                                                         int(row["Grid_Size"]) == grid) else
                            "copy" if "k_axpby" in name else None)
                     if key:
                         sums[key].append(float(row["Counter_Value"]))
@@ -388,7 +390,7 @@ def measure_traffic():
     sw = copy_kib / out["WRITE_SIZE"]["copy"][0]
     fetch = out["FETCH_SIZE"]["spmv"][0] * sf * 1024.0
     write = out["WRITE_SIZE"]["spmv"][0] * sw * 1024.0
-    det = {"source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over bench.py --spmv-only, "
+    det = {"source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over bench.py {mode}, "
                      "measured in this run",
            "fetch_bytes": round(fetch), "write_bytes": round(write),
            "fetch_scale": round(sf, 4), "write_scale": round(sw, 4),
@@ -509,14 +511,18 @@ def bench_flow(capi, wl, lib, log, N=96, iters=3, variants=None):
     return out
 
 
-def bench_batch(capi, wl, lib, steps, log, N=100, n_eq=32):
-    """32 transport equations A_e = V/dt I + Gamma_e L on one 100^3 perturbed,
-    permuted mesh (1,000,000 cells): one batched solve of all equations to
-    1e-8 true residual, against 32 one-at-a-time solves of the same arithmetic
-    (FP32 Jacobi-PCG inner, FP64 refinement) with the single-equation solver."""
+# The batched SpMM at 32 equations (4 per thread) and the workgroups its
+# autotuner picks on the 100^3 transport batch (1024 x 256 work-items); the
+# counter passes keep only launches of that size.
+BSPMM_KERNEL = "k_bspmm<3, true, 4>"
+BSPMM_GRID = 1024 * 256
+
+
+def transport_batch(capi, wl, N=100, n_eq=32):
+    """The 100^3 perturbed, permuted transport mesh and a batch of n_eq
+    equations A_e = V/dt I + Gamma_e L assembled on the GPU."""
     import numpy as np
 
-    t0 = time.perf_counter()
     w = wl.transport(N, n_eq)
     m = w.mesh
     dm = capi.DeviceMesh(m, w.bc_of_zone)
@@ -529,8 +535,39 @@ def bench_batch(capi, wl, lib, steps, log, N=100, n_eq=32):
     for e in range(n_eq):
         dm.assemble_momentum(A, rhs, 0, float(w.gammas[e]), 1.0, phi, d_dt=dt, d_Uold=uold)
         B.set_equation(e, A)
-    n = m.n_cells
     db = capi.DeviceArray.from_host(np.concatenate(w.bs))
+    return w, dm, A, B, db, (rhs, phi, dt, uold)
+
+
+def batch_spmm_only(capi, wl, lib):
+    """Short run for the batched SpMM's counter passes: the autotuner's
+    launches, three batched CG iterations on the real values, then the
+    calibration copy."""
+    w, dm, A, B, db, keep = transport_batch(capi, wl)
+    n = w.mesh.n_cells
+    dx = capi.DeviceArray(32 * n)
+    B.solve_device(db, dx, tolerance=1e-8, max_iterations=3, inner_tolerance=1e-5)
+    t = B.tuning()
+    src = capi.DeviceArray(COPY_DOUBLES)
+    dst = capi.DeviceArray(COPY_DOUBLES)
+    for _ in range(3):
+        capi.check(lib.mi_copy(src.ptr, dst.ptr, COPY_DOUBLES, None))
+    lib.mi_device_synchronize()
+    return {"bspmm_us": t["spmm_us"], "bspmm_algorithmic_bytes": t["spmm_bytes"],
+            "rows_per_block": t["rows_per_block"], "copy_bytes_read": 8 * COPY_DOUBLES,
+            "copy_bytes_written": 8 * COPY_DOUBLES}
+
+
+def bench_batch(capi, wl, lib, steps, log, N=100, n_eq=32, traffic=None):
+    """32 transport equations A_e = V/dt I + Gamma_e L on one 100^3 perturbed,
+    permuted mesh (1,000,000 cells): one batched solve of all equations to
+    1e-8 true residual, against 32 one-at-a-time solves of the same arithmetic
+    (FP32 Jacobi-PCG inner, FP64 refinement) with the single-equation solver."""
+    import numpy as np
+
+    t0 = time.perf_counter()
+    w, dm, A, B, db, (rhs, phi, dt, uold) = transport_batch(capi, wl, N, n_eq)
+    n = w.mesh.n_cells
     dx = capi.DeviceArray(n_eq * n)
     log(f"batched transport: {n_eq} equations, n={n} nnz={A.nnz} ({time.perf_counter() - t0:.1f}s)")
     # Jacobi-PCG on these mass-dominated systems prefers a tighter inner
@@ -573,7 +610,12 @@ def bench_batch(capi, wl, lib, steps, log, N=100, n_eq=32):
            "max_true_relative_residual": float(rel.max()), "status": rc,
            "rows_per_block": tune["rows_per_block"],
            "spmm_us": round(tune["spmm_us"], 2),
-           "spmm_GBps": round(tune["spmm_bytes"] / tune["spmm_us"] / 1e3, 1)}
+           "spmm_GBps": round(tune["spmm_bytes"] / tune["spmm_us"] / 1e3, 1),
+           "spmm_frac_of_peak": round(tune["spmm_bytes"] / tune["spmm_us"] / 1e3 / HBM_PEAK_GBS, 4),
+           "spmm_algorithmic_bytes": round(tune["spmm_bytes"])}
+    if traffic is not None:
+        out["spmm_traffic"] = round(traffic[0]) if traffic[0] else None
+        out["spmm_traffic_detail"] = traffic[1]
     log(f"batched: {out}")
     return out
 
@@ -685,6 +727,9 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the working-matrix SpMV and a calibration copy only "
                          "(short run for hardware-counter passes)")
+    ap.add_argument("--batch-spmm-only", action="store_true",
+                    help="build the 32-equation transport batch, run three batched CG "
+                         "iterations and a calibration copy only (hardware-counter passes)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the solver has no multi-GPU path; "
@@ -704,7 +749,17 @@ def main():
         print(json.dumps({"flow": out}), flush=True)
         return
 
+    if args.batch_spmm_only:
+        lib = capi.load()
+        capi.check(lib.mi_init(0))
+        print(json.dumps(batch_spmm_only(capi, wl, lib)), flush=True)
+        return
+
     traffic, traffic_detail = None, {"source": "not measured (--no-traffic)"}
+    batch_traffic = None
+    if not args.spmv_only and not args.no_traffic and not args.no_batch:
+        batch_traffic = measure_traffic("--batch-spmm-only", BSPMM_KERNEL, BSPMM_GRID)
+        log(f"batched SpMM traffic: {batch_traffic}")
     if not args.spmv_only and not args.no_traffic:
         # child processes, before this process initialises the GPU
         traffic, traffic_detail = measure_traffic()
@@ -849,7 +904,7 @@ def main():
 
     batch = None
     if not args.no_batch:
-        batch = bench_batch(capi, wl, lib, args.steps, log)
+        batch = bench_batch(capi, wl, lib, args.steps, log, traffic=batch_traffic)
         batch["convective"] = bench_batch_convective(capi, wl, lib, args.steps, log)
 
     err = None
