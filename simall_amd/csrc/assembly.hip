@@ -16,6 +16,7 @@
 #include "host_threads.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -38,26 +39,36 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
     const int c = int(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= m.n_cells)
         return;
-    for (int k = m.rp[c]; k < m.rp[c + 1]; ++k)
-        val[k] = 0.0;
+    // direct: the slots write distinct off-diagonal entries, each stored as
+    // 0 - coef (the zeroed entry minus coef) without reading it back
+    const int cs = m.ss_cnt[c];
+    const bool direct = cs >= 0;
+    const int cnt = direct ? cs : -cs;
+    if (!direct)
+        for (int k = m.rp[c]; k < m.rp[c + 1]; ++k)
+            val[k] = 0.0;
     double diag = 0.0, b = 0.0;
-    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
-        const int f = m.cf_face[k];
-        const int oth = m.sl_oth[k];
-        const int tag = m.sl_tag[k];
+    const int q0 = m.ss_off[c / kSlotSlice] + c % kSlotSlice;
+    for (int j = 0; j < cnt; ++j) {
+        const int q = q0 + j * kSlotSlice;
+        const int f = m.ss_face[q];
+        const int oth = m.ss_oth[q];
+        const int tag = m.ss_tag[q];
         const bool own = (tag & kSlotOwner) != 0;
-        const double Amag2 = m.sl_s2[k];
-        const double ddotA = m.sl_ds[k];
+        const double Amag2 = m.ss_s2[q];
+        const double ddotA = m.ss_ds[q];
         const double g = gamma ? gamma[f] : 1.0;
         if (tag & kSlotPeriodic) {
             // the owner's half of the coupling with the twin's cell (:411-419)
             const double dO = aP ? fmax(aP[c], 1e-30) : 1.0;
             const double coef = rho * g * Amag2 / fmax(dO * ddotA, 1e-30);
             diag += coef;
-            if (oth == c)
+            if (oth == c) {
                 diag += -coef; // a cell that is its own twin: the same entry
-            else
-                val[m.cf_slot[k]] -= coef;
+            } else {
+                const int e = m.ss_slot[q];
+                val[e] = (direct ? 0.0 : val[e]) - coef;
+            }
         } else if (oth >= 0) {
             double aPf = 1.0;
             if (aP) {
@@ -67,7 +78,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
             }
             const double coef = rho * g * Amag2 / fmax(aPf * ddotA, 1e-30);
             diag += coef;
-            val[m.cf_slot[k]] -= coef;
+            const int e = m.ss_slot[q];
+            val[e] = (direct ? 0.0 : val[e]) - coef;
         } else if ((tag & 15) - 1 == BC_PRESSURE) {
             const double dO = aP ? fmax(aP[c], 1e-30) : 1.0;
             const double coef = rho * g * Amag2 / fmax(dO * ddotA, 1e-30);
@@ -81,14 +93,36 @@ __global__ __launch_bounds__(kBlock) void k_assemble_pressure(MeshDev m, double 
         }
     }
     const int kd = m.diag[c];
-    if (kd >= 0)
-        val[kd] += diag;
-    if (pin && c == 0) {
-        if (kd >= 0)
-            val[kd] *= 1.0e6;
-        b = 0.0;
+    if (kd >= 0) {
+        double v = (direct ? 0.0 : val[kd]) + diag;
+        if (pin && c == 0)
+            v *= 1.0e6;
+        val[kd] = v;
     }
+    if (pin && c == 0)
+        b = 0.0;
     rhs[c] = b;
+}
+
+// The sliced copies of the slot arrays (MeshDev::ss_*), once per mesh:
+// element q takes slot smap[q] of the cell-order arrays (-1: padding).
+__global__ __launch_bounds__(kBlock) void k_slice_slots(MeshDev m, const int* __restrict__ smap,
+                                                        int64_t total, int* __restrict__ face,
+                                                        int* __restrict__ slot, int* __restrict__ oth,
+                                                        int8_t* __restrict__ tag,
+                                                        double* __restrict__ s2,
+                                                        double* __restrict__ ds)
+{
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < total;
+         q += int64_t(gridDim.x) * kBlock) {
+        const int k = smap[q];
+        face[q] = k >= 0 ? m.cf_face[k] : -1;
+        slot[q] = k >= 0 ? m.cf_slot[k] : -1;
+        oth[q] = k >= 0 ? m.sl_oth[k] : -1;
+        tag[q] = k >= 0 ? m.sl_tag[k] : int8_t(0);
+        s2[q] = k >= 0 ? m.sl_s2[k] : 0.0;
+        ds[q] = k >= 0 ? m.sl_ds[k] : 0.0;
+    }
 }
 
 // Geometry of every cell-face slot (MeshDev::sl_s2 / sl_ds), once per mesh:
@@ -243,15 +277,22 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
     const int c = int(blockIdx.x) * kBlock + threadIdx.x;
     if (c >= m.n_cells)
         return;
-    for (int k = m.rp[c]; k < m.rp[c + 1]; ++k)
-        val[k] = 0.0;
+    // direct: as in k_assemble_pressure
+    const int cs = m.ss_cnt[c];
+    const bool direct = cs >= 0;
+    const int cnt = direct ? cs : -cs;
+    if (!direct)
+        for (int k = m.rp[c]; k < m.rp[c + 1]; ++k)
+            val[k] = 0.0;
     double diag = 0.0, b = 0.0;
-    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
-        const int f = m.cf_face[k];
-        const int oth = m.sl_oth[k];
-        const int tag = m.sl_tag[k];
-        const double Amag2 = m.sl_s2[k];
-        const double ddotA = m.sl_ds[k];
+    const int q0 = m.ss_off[c / kSlotSlice] + c % kSlotSlice;
+    for (int j = 0; j < cnt; ++j) {
+        const int q = q0 + j * kSlotSlice;
+        const int f = m.ss_face[q];
+        const int oth = m.ss_oth[q];
+        const int tag = m.ss_tag[q];
+        const double Amag2 = m.ss_s2[q];
+        const double ddotA = m.ss_ds[q];
         const double Ff = phi[f];
         const double ConvO = fmax(Ff, 0.0);
         const double ConvN = fmax(-Ff, 0.0);
@@ -264,10 +305,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
                 diag += -(ConvN + D);
             } else if (tag & kSlotOwner) {
                 diag += ConvO + D;
-                val[m.cf_slot[k]] += -(ConvN + D);
+                const int e = m.ss_slot[q];
+                val[e] = (direct ? 0.0 : val[e]) + -(ConvN + D);
             } else {
                 diag += ConvN + D;
-                val[m.cf_slot[k]] += -(ConvO + D);
+                const int e = m.ss_slot[q];
+                val[e] = (direct ? 0.0 : val[e]) + -(ConvO + D);
             }
         } else {
             const int bc = (tag & 15) - 1;
@@ -285,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_momentum(
     if (pgradV)
         b -= pgradV[c];
     const int kd = m.diag[c];
-    double a = (kd >= 0) ? val[kd] + diag : diag;
+    double a = (kd >= 0) ? (direct ? 0.0 : val[kd]) + diag : diag;
     if (dt && kd >= 0) {
         const double Vrho_dt = rho * m.vol[c] / dt[c];
         const double un = Uold ? Uold[c] : 0.0;
@@ -687,6 +730,73 @@ int mesh_create(const mi_mesh_soa* m, const int32_t* bc_of_zone, int n_zones,
     if (nC > 0) {
         k_slot_geom<<<grid_for(nC, kBlock), kBlock, 0, s>>>(M->dev(), M->sl_s2.p, M->sl_ds.p);
         MI_HIP(hipGetLastError());
+    }
+    // sliced slot arrays: slices of kSlotSlice cells padded to the slice's
+    // longest cell
+    {
+        const int64_t nsl = (nC + kSlotSlice - 1) / kSlotSlice;
+        std::vector<int> soff(size_t(nsl) + 1, 0);
+        std::vector<int> cnt(static_cast<size_t>(nC), 0);
+        int64_t tot = 0;
+        for (int64_t sl = 0; sl < nsl; ++sl) {
+            int w = 0;
+            const int64_t c1 = std::min<int64_t>(nC, (sl + 1) * kSlotSlice);
+            for (int64_t c = sl * kSlotSlice; c < c1; ++c) {
+                cnt[size_t(c)] = cf_off[size_t(c) + 1] - cf_off[size_t(c)];
+                w = std::max(w, cnt[size_t(c)]);
+            }
+            // a cell whose slots write distinct off-diagonal entries of its row
+            // (every entry but the diagonal, by construction of the pattern)
+            // stores them directly; any other keeps zero-and-accumulate: its
+            // count is stored negated
+            for (int64_t c = sl * kSlotSlice; c < c1; ++c) {
+                const int k0 = cf_off[size_t(c)], k1 = cf_off[size_t(c) + 1];
+                int covered = 0;
+                bool uniq = true;
+                for (int k = k0; k < k1 && uniq; ++k) {
+                    const int e = cf_slot[size_t(k)];
+                    if (e < 0)
+                        continue;
+                    if (ci[size_t(e)] == int(c))
+                        uniq = false;
+                    for (int k2 = k0; k2 < k && uniq; ++k2)
+                        if (cf_slot[size_t(k2)] == e)
+                            uniq = false;
+                    ++covered;
+                }
+                if (!uniq || covered != rp[size_t(c) + 1] - rp[size_t(c)] - 1)
+                    cnt[size_t(c)] = -cnt[size_t(c)];
+            }
+            soff[size_t(sl)] = int(tot);
+            tot += int64_t(kSlotSlice) * w;
+            if (tot > INT_MAX)
+                return set_error(MI_ERR_BAD_ARG, "mesh too large for the sliced slot arrays");
+        }
+        soff[size_t(nsl)] = int(tot);
+        std::vector<int> smap(size_t(std::max<int64_t>(tot, 1)), -1);
+        for (int64_t c = 0; c < nC; ++c) {
+            const int64_t q0 = soff[size_t(c / kSlotSlice)] + c % kSlotSlice;
+            for (int j = 0; j < std::abs(cnt[size_t(c)]); ++j)
+                smap[size_t(q0 + int64_t(kSlotSlice) * j)] = cf_off[size_t(c)] + j;
+        }
+        MI_TRY(up(M->ss_off, soff.data(), nsl + 1, s));
+        MI_TRY(up(M->ss_cnt, cnt.data(), nC, s));
+        DevBuf<int> d_smap;
+        MI_TRY(d_smap.upload(smap.data(), smap.size(), s));
+        const size_t m_ = smap.size();
+        MI_TRY(M->ss_face.alloc(m_));
+        MI_TRY(M->ss_slot.alloc(m_));
+        MI_TRY(M->ss_oth.alloc(m_));
+        MI_TRY(M->ss_tag.alloc(m_));
+        MI_TRY(M->ss_s2.alloc(m_));
+        MI_TRY(M->ss_ds.alloc(m_));
+        if (tot > 0) {
+            k_slice_slots<<<reduce_grid(tot, 4), kBlock, 0, s>>>(
+                M->dev(), d_smap.p, tot, M->ss_face.p, M->ss_slot.p, M->ss_oth.p, M->ss_tag.p,
+                M->ss_s2.p, M->ss_ds.p);
+            MI_HIP(hipGetLastError());
+        }
+        MI_HIP(hipStreamSynchronize(s)); // d_smap is freed on leaving the scope
     }
     MI_HIP(hipStreamSynchronize(s));
     *out = M.release();

@@ -41,7 +41,23 @@ struct MeshDev
     const double* sl_ds;
     const int* sl_oth;
     const int8_t* sl_tag;
+    // The same slot arrays sliced for the cell-per-thread assembly kernels:
+    // slot j of cell c at ss_off[c / 64] + 64 j + c % 64 (each slice of 64
+    // cells padded to its longest cell), so slot j of a wavefront's cells is
+    // one coalesced access; ss_cnt[c] = the cell's slot count, negated when
+    // its slots do not write distinct off-diagonal entries of its row (the
+    // assembly kernels then zero the row and accumulate into it).
+    const int* ss_off;
+    const int* ss_cnt;
+    const int* ss_face;
+    const int* ss_slot;
+    const int* ss_oth;
+    const int8_t* ss_tag;
+    const double* ss_s2;
+    const double* ss_ds;
 };
+
+constexpr int kSlotSlice = 64;
 
 } // namespace mi
 
@@ -56,6 +72,10 @@ struct mi_mesh
     std::vector<int> twin_face; // periodic twin face of every face, -1 (host)
     mi::DevBuf<int8_t> fbc, sl_tag;
     mi::DevBuf<double> sl_s2, sl_ds;
+    // sliced copies (MeshDev::ss_*)
+    mi::DevBuf<int> ss_off, ss_cnt, ss_face, ss_slot, ss_oth;
+    mi::DevBuf<int8_t> ss_tag;
+    mi::DevBuf<double> ss_s2, ss_ds;
     mi::DevBuf<double> ax, ay, az, fcx, fcy, fcz, ccx, ccy, ccz, vol;
     // LeastSquaresGradient's inverted 3x3 moment matrices, 6 per cell, built
     // on first use (geometry only)
@@ -98,6 +118,14 @@ struct mi_mesh
         d.sl_ds = sl_ds.p;
         d.sl_oth = sl_oth.p;
         d.sl_tag = sl_tag.p;
+        d.ss_off = ss_off.p;
+        d.ss_cnt = ss_cnt.p;
+        d.ss_face = ss_face.p;
+        d.ss_slot = ss_slot.p;
+        d.ss_oth = ss_oth.p;
+        d.ss_tag = ss_tag.p;
+        d.ss_s2 = ss_s2.p;
+        d.ss_ds = ss_ds.p;
         return d;
     }
 };
