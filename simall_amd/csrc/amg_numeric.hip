@@ -13,7 +13,8 @@
 //               (amg_smoothed_prolongator)
 //   R           P^T, a gather through a fixed transpose map (csr_transpose)
 //   A_c         R (A P): two row-wise products into the fixed patterns of A P
-//               and A_c (csr_multiply)
+//               and A_c (csr_multiply), each as a list of term pairs per
+//               target entry recorded with the patterns (k_num_pairs)
 // Every value is formed with the host setup's operations in the host's order,
 // so the hierarchy equals a host rebuild with the same aggregates up to the
 // rounding of the power iteration's dot products (the device sums them in
@@ -21,8 +22,10 @@
 // patterns.  All work is row-parallel (one thread per row, no atomics), so
 // the results do not depend on the launch.
 #include "amg.hpp"
+#include "host_threads.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 namespace mi
@@ -46,6 +49,11 @@ struct NumLevel
     DevBuf<double> rv;
     DevBuf<int> aprp, apci;       // A_l P_l
     DevBuf<double> apv;
+    // Term lists of the products A P and R (A P) (product_pairs): offsets
+    // per target entry and (A position, B position) pairs; empty: the
+    // row-wise kernels instead
+    DevBuf<int> ap_off, rap_off;
+    DevBuf<int2> ap_pab, rap_pab;
     int64_t p_nnz = 0, r_nnz = 0;
     int lanes = 1;                // lanes per row of the power iteration
     bool wave_rows = false;       // P and A P a wavefront per row (few rows), else a thread
@@ -61,6 +69,70 @@ int max_row(const std::vector<int>& rp)
     for (size_t i = 0; i + 1 < rp.size(); ++i)
         m = std::max(m, rp[i + 1] - rp[i]);
     return m;
+}
+
+// Term lists of C = A B on C's fixed pattern (C's rows sorted): for every
+// entry of C in CSR order, the A and B positions of its terms a_ik b_kj in
+// the order the row-wise product adds them (k in A's row order); terms whose
+// column is not in C's pattern are dropped, as the row-wise product drops
+// them.  Returns false when the lists would hold 2^31 pairs or more.
+bool product_pairs(const HCsr& A, const HCsr& B, const HCsr& C, std::vector<int>& off,
+                   std::vector<int2>& pab)
+{
+    const int n = A.nr;
+    const int64_t m = C.nnz();
+    std::vector<int64_t> cnt(size_t(m) + 1, 0);
+    const int T = host_threads();
+    // pass 1: terms per entry of C
+    host_parallel_for(n, T, [&](int b, int e, int) {
+        std::vector<int> mark(size_t(std::max(C.nc, 1)), -1);
+        for (int i = b; i < e; ++i) {
+            for (int t = C.rp[size_t(i)]; t < C.rp[size_t(i) + 1]; ++t)
+                mark[size_t(C.ci[size_t(t)])] = t;
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                const int r = A.ci[size_t(k)];
+                for (int q = B.rp[size_t(r)]; q < B.rp[size_t(r) + 1]; ++q) {
+                    const int t = mark[size_t(B.ci[size_t(q)])];
+                    if (t >= 0)
+                        ++cnt[size_t(t) + 1];
+                }
+            }
+            for (int t = C.rp[size_t(i)]; t < C.rp[size_t(i) + 1]; ++t)
+                mark[size_t(C.ci[size_t(t)])] = -1;
+        }
+    });
+    for (int64_t t = 0; t < m; ++t)
+        cnt[size_t(t) + 1] += cnt[size_t(t)];
+    if (cnt[size_t(m)] >= INT_MAX)
+        return false;
+    off.assign(size_t(m) + 1, 0);
+    for (int64_t t = 0; t <= m; ++t)
+        off[size_t(t)] = int(cnt[size_t(t)]);
+    pab.assign(size_t(std::max<int64_t>(cnt[size_t(m)], 1)), int2{0, 0});
+    // pass 2: the pairs, each entry's in term order
+    host_parallel_for(n, T, [&](int b, int e, int) {
+        std::vector<int> mark(size_t(std::max(C.nc, 1)), -1);
+        std::vector<int> cur;
+        for (int i = b; i < e; ++i) {
+            const int t0 = C.rp[size_t(i)], t1 = C.rp[size_t(i) + 1];
+            cur.assign(off.begin() + t0, off.begin() + t1);
+            for (int t = t0; t < t1; ++t)
+                mark[size_t(C.ci[size_t(t)])] = t;
+            for (int k = A.rp[size_t(i)]; k < A.rp[size_t(i) + 1]; ++k) {
+                const int r = A.ci[size_t(k)];
+                for (int q = B.rp[size_t(r)]; q < B.rp[size_t(r) + 1]; ++q) {
+                    const int t = mark[size_t(B.ci[size_t(q)])];
+                    if (t >= 0) {
+                        const int w = cur[size_t(t - t0)]++;
+                        pab[size_t(w)] = int2{k, q};
+                    }
+                }
+            }
+            for (int t = t0; t < t1; ++t)
+                mark[size_t(C.ci[size_t(t)])] = -1;
+        }
+    });
+    return true;
 }
 } // namespace
 
@@ -300,6 +372,39 @@ __global__ __launch_bounds__(kBlock) void k_num_product_wave(
         cv[c0 + t] = sw[t];
 }
 
+// C = A B from the term lists of product_pairs: entry t of C sums its terms
+// a[pab.x] b[pab.y] in list order from 0 -- the additions of k_num_product
+// in the same order, one thread per entry of C instead of a search per term.
+// Four terms' loads are issued before their four additions.
+__global__ __launch_bounds__(kBlock) void k_num_pairs(int64_t m, const int* __restrict__ off,
+                                                      const int2* __restrict__ pab,
+                                                      const double* __restrict__ av,
+                                                      const double* __restrict__ bv,
+                                                      double* __restrict__ cv)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < m;
+         t += int64_t(gridDim.x) * kBlock) {
+        double sum = 0.0;
+        int u = off[t];
+        const int ue = off[t + 1];
+        for (; u + 4 <= ue; u += 4) {
+            const int2 p0 = pab[u], p1 = pab[u + 1], p2 = pab[u + 2], p3 = pab[u + 3];
+            const double a0 = av[p0.x], a1 = av[p1.x], a2 = av[p2.x], a3 = av[p3.x];
+            const double b0 = bv[p0.y], b1 = bv[p1.y], b2 = bv[p2.y], b3 = bv[p3.y];
+            sum += a0 * b0;
+            sum += a1 * b1;
+            sum += a2 * b2;
+            sum += a3 * b3;
+        }
+        for (; u < ue; ++u) {
+            const int2 p = pab[u];
+            sum += av[p.x] * bv[p.y];
+        }
+        cv[t] = sum;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_num_zero(int64_t m, double* __restrict__ x)
 {
     for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < m;
@@ -429,6 +534,18 @@ int AmgNumeric::record(const AmgHierarchyHost& H, const int* rp0, const int* ci0
             MI_TRY(d.aprp.upload(AP.rp.data(), AP.rp.size(), s));
             MI_TRY(d.apci.upload(AP.ci.data(), AP.ci.size(), s));
             MI_TRY(d.apv.alloc(size_t(std::max<int64_t>(AP.nnz(), 1))));
+            // the two products as term lists
+            std::vector<int> off;
+            std::vector<int2> pab;
+            if (product_pairs(h.A, h.P, AP, off, pab)) {
+                MI_TRY(d.ap_off.upload(off.data(), off.size(), s));
+                MI_TRY(d.ap_pab.upload(pab.data(), pab.size(), s));
+                MI_HIP(hipStreamSynchronize(s));
+            }
+            if (product_pairs(h.R, AP, next, off, pab)) {
+                MI_TRY(d.rap_off.upload(off.data(), off.size(), s));
+                MI_TRY(d.rap_pab.upload(pab.data(), pab.size(), s));
+            }
             MI_HIP(hipStreamSynchronize(s)); // AP's host arrays go out of scope
         }
     }
@@ -491,13 +608,21 @@ int AmgNumeric::run(const double* a0, double omega_scale, hipStream_t s)
         if (d.r_nnz > 0)
             k_num_gather<<<reduce_grid(d.r_nnz, 4), kBlock, 0, s>>>(d.r_nnz, d.rmap.p, d.pv.p,
                                                                    d.rv.p);
-        if (d.ap_wave)
+        const int64_t ap_nnz = int64_t(d.ap_off.n) - 1;
+        const int64_t c_nnz = int64_t(d.rap_off.n) - 1;
+        if (d.ap_off.n > 0 && ap_nnz > 0)
+            k_num_pairs<<<reduce_grid(ap_nnz, 2), kBlock, 0, s>>>(ap_nnz, d.ap_off.p, d.ap_pab.p,
+                                                                 d.av.p, d.pv.p, d.apv.p);
+        else if (d.ap_wave)
             k_num_product_wave<<<gw, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.prp.p, d.pci.p,
                                                      d.pv.p, d.aprp.p, d.apci.p, d.apv.p);
         else
             k_num_product<<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.prp.p, d.pci.p,
                                                d.pv.p, d.aprp.p, d.apci.p, d.apv.p);
-        if (d.rap_wave)
+        if (d.rap_off.n > 0 && c_nnz > 0)
+            k_num_pairs<<<reduce_grid(c_nnz, 2), kBlock, 0, s>>>(c_nnz, d.rap_off.p, d.rap_pab.p,
+                                                                d.rv.p, d.apv.p, nx.av.p);
+        else if (d.rap_wave)
             k_num_product_wave<<<grid_for(nx.n, kBlock / kWave), kBlock, 0, s>>>(
                 nx.n, d.rrp.p, d.rci.p, d.rv.p, d.aprp.p, d.apci.p, d.apv.p, nx.arp, nx.aci,
                 nx.av.p);
