@@ -10,10 +10,13 @@
 // iterated occupy W slots (1 <= W <= 64), and every per-equation
 // quantity of a nonzero k or a row i is a line of W consecutive values:
 // v32[k * W + s], r[i * W + s], ...  A row of the batched SpMV is handled by
-// W threads (one per slot): one load of colIdx[k] serves every equation,
-// the values of nonzero k are one contiguous W-wide line, and the gather of
-// x at column c is one contiguous line too.  The shared pattern is streamed
-// once per launch and no gather is scattered across equations.  When the
+// W / V threads, each taking V consecutive slots (V = 4, 2 or 1, the widest
+// dividing W) with V-wide loads: one load of colIdx[k] serves every
+// equation, the values of nonzero k are one contiguous W-wide line, and the
+// gather of x at column c is one contiguous line too.  The shared pattern is
+// streamed once per launch and no gather is scattered across equations.  The
+// SpMV runs on an autotuned grid; the vector stages, norms and true
+// residuals on one workgroup per 256 rows.  When the
 // pattern has a SELL-64 form (short rows: 7-point operators) the nonzeros
 // are stored in its slot order, m = soff[slice] + 64 k + row-in-slice, so the
 // loads of colIdx and of the value lines of the rows of a workgroup step are
