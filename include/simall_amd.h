@@ -269,24 +269,55 @@ MI_API int mi_solver_destroy(mi_solver* s);
  * n_eq (1..64) systems A_e x_e = b_e on one CSR pattern, solved together.
  * SimAll solves each scalar transport equation (energy, k/omega, species)
  * with its own ILinearSolver::solve on an identical pattern
- * (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232); one
- * mi_batch_solve replaces those n_eq calls.  Per equation the iteration is
- * Jacobi-PCG from x = 0 (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452),
- * or Jacobi-preconditioned BiCGStab (BiCGSTABSolver::solve, :360-404) for
- * nonsymmetric (convective) equations, in FP32 with FP64 dot products, norms
- * and iterate, refined on the FP64 true residual until
- * ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations converge and drop out
- * independently.  cfg (nullable: defaults) supplies kind = CG or BiCGStab,
- * preconditioner = Jacobi or None, tolerance, max_iterations (per equation,
- * all sweeps), inner_tolerance, max_outer and check_interval.
+ * (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232), and the
+ * three momentum components one after another on one matrix
+ * (src/solver/SimpleAlgorithm.cpp:529-532, solve_momentum_component
+ * :492-507); one mi_batch_solve replaces those n_eq calls.  Per equation the
+ * iteration is PCG from x = 0 (CGSolver::solve,
+ * src/solver/LinearSolvers.cpp:420-452), or preconditioned BiCGStab
+ * (BiCGSTABSolver::solve, :360-404) for nonsymmetric (convective) equations,
+ * with Jacobi, ILU(0) (ILU0Preconditioner, :78-151) or no preconditioner, in
+ * FP32 with FP64 dot products, norms and iterate, refined on the FP64 true
+ * residual until ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations converge
+ * and drop out independently.  cfg (nullable: defaults) supplies kind = CG or
+ * BiCGStab, preconditioner = Jacobi, ILU or None, tolerance, max_iterations
+ * (per equation, all sweeps), inner_tolerance, max_outer and check_interval.
  * Vectors are equation-major: b[e * n + i].  The rows per workgroup of the
- * batched SpMV are chosen by timing candidates at creation (mi_batch_tuning). */
+ * batched SpMV are chosen by timing candidates at creation (mi_batch_tuning).
+ *
+ * flags of mi_batch_create_ex:
+ *   MI_BATCH_SHARED_VALUES  one matrix for every equation (set by
+ *                           mi_batch_set_matrix), streamed once per launch
+ *                           for all right-hand sides;
+ *   MI_BATCH_ILU0           order the rows by two colours so that the ILU(0)
+ *                           factorization and its two triangular sweeps run
+ *                           on the device, row-parallel; the pattern must be
+ *                           structurally symmetric, hold every diagonal and
+ *                           be two-colourable (hexahedral and quadrilateral
+ *                           meshes), else MI_ERR_UNSUPPORTED.  The factors are
+ *                           those of the reference's IKJ loop on that
+ *                           ordering, computed in FP64 from the values at
+ *                           the first ILU(0) solve after they change. */
 typedef struct mi_batch mi_batch;
+enum { MI_BATCH_SHARED_VALUES = 1, MI_BATCH_ILU0 = 2 };
 MI_API int mi_batch_create(const mi_csr* pattern, int n_eq, mi_batch** out);
-/* Values of equation eq from a matrix with the batch's pattern. */
+MI_API int mi_batch_create_ex(const mi_csr* pattern, int n_eq, int flags, mi_batch** out);
+/* Values of equation eq from a matrix with the batch's pattern (not for a
+ * batch with shared values). */
 MI_API int mi_batch_set_equation(mi_batch* b, int eq, const mi_csr* A);
-/* All values, equation-major: values_host[e * nnz + k]. */
+/* Values of every equation from one matrix with the batch's pattern: the
+ * shared matrix, or each equation's copy. */
+MI_API int mi_batch_set_matrix(mi_batch* b, const mi_csr* A);
+/* All values, equation-major: values_host[e * nnz + k] (not for a batch with
+ * shared values). */
 MI_API int mi_batch_set_values(mi_batch* b, const double* values_host);
+/* z_e = M_e^-1 r_e for every equation with the batch's preconditioner
+ * (MI_PC_JACOBI, MI_PC_ILU or MI_PC_NONE), in the FP32 arithmetic of the
+ * iterations: IPreconditioner::apply (src/solver/LinearSolvers.cpp:55-63,
+ * 126-151) for the batch's matrices.  Equation-major device vectors in the
+ * caller's numbering. */
+MI_API int mi_batch_apply_preconditioner(mi_batch* b, int preconditioner, const double* r_device,
+                                         double* z_device);
 /* Returns MI_OK when every equation converged, else MI_ERR_NOT_CONVERGED;
  * iterations[e] and rel_residual[e] (FP64 true) are written in both cases. */
 MI_API int mi_batch_solve(mi_batch* b, const mi_solver_cfg* cfg, const double* b_host,
@@ -332,6 +363,12 @@ MI_API int mi_solver_amg_host(mi_solver* s, mi_amg_host** out);
 /* perm[new] = old, reverse Cuthill-McKee of the symmetrised pattern. */
 MI_API int mi_rcm_permutation(int64_t n, const int32_t* rowPtr, const int32_t* colIdx,
                               int32_t* perm, int32_t* bandwidth_after);
+/* perm[new] = old, the multicolour ordering of the ILU(0) solvers (greedy
+ * colouring in reverse Cuthill-McKee order, rows grouped by colour); *colors
+ * receives the colour count.  No reference counterpart (the reference
+ * factorizes in the caller's order). */
+MI_API int mi_color_permutation(int64_t n, const int32_t* rowPtr, const int32_t* colIdx,
+                                int32_t* perm, int32_t* colors);
 
 /* ------------------------------------------------------ mesh and assembly
  * The SoA mesh of meshing::Mesh (src/meshing/MeshStorage.hpp:37-74): uint64
@@ -492,7 +529,8 @@ typedef struct {
     int momentum_batch;  /* 1: the three momentum components (one matrix, three right-hand
                             sides: SimpleAlgorithm.cpp:529-532) in one mi_batch solve with
                             momentum.tolerance / max_iterations / inner_tolerance / max_outer,
-                            BiCGStab with Jacobi; 0 (default): three solves with `momentum` */
+                            BiCGStab with ILU(0) on one shared matrix; 0 (default): three
+                            solves with `momentum` */
 } mi_flow_cfg;
 /* SimpleResiduals (src/solver/SimpleAlgorithm.hpp:59-63). */
 typedef struct {

@@ -189,7 +189,11 @@ PROTOTYPES = {
     "mi_simplec_denominators": (_i, [_vp, _vp, _vp, _vp]),
     "mi_face_gradient": (_i, [_vp, _d, _vp, _i, _vp, _vp, _vp]),
     "mi_batch_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "mi_batch_create_ex": (_i, [_vp, _i, _i, C.POINTER(_vp)]),
     "mi_batch_set_equation": (_i, [_vp, _i, _vp]),
+    "mi_batch_set_matrix": (_i, [_vp, _vp]),
+    "mi_batch_apply_preconditioner": (_i, [_vp, _i, _vp, _vp]),
+    "mi_color_permutation": (_i, [_i64, _vp, _vp, _vp, C.POINTER(C.c_int32)]),
     "mi_batch_set_values": (_i, [_vp, _vp]),
     "mi_batch_solve": (_i, [_vp, C.POINTER(SolverCfg), _vp, _vp, _vp, _vp]),
     "mi_batch_solve_device": (_i, [_vp, C.POINTER(SolverCfg), _vp, _vp, _vp, _vp]),
@@ -762,19 +766,39 @@ def rcm_permutation(rowPtr, colIdx):
     return perm, bw.value
 
 
-class Batch:
-    """mi_batch: n_eq equations sharing the pattern of `A` (Csr or CsrView)."""
+MI_BATCH_SHARED_VALUES = 1
+MI_BATCH_ILU0 = 2
 
-    def __init__(self, A, n_eq):
+
+def color_permutation(rowPtr, colIdx):
+    """perm[new] = old of the multicolour ordering, and the colour count."""
+    rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
+    ci = np.ascontiguousarray(colIdx, dtype=np.int32)
+    perm = np.empty(rp.size - 1, np.int32)
+    nc = C.c_int32()
+    check(load().mi_color_permutation(rp.size - 1, ptr(rp), ptr(ci), ptr(perm), C.byref(nc)))
+    return perm, nc.value
+
+
+class Batch:
+    """mi_batch: n_eq equations sharing the pattern of `A` (Csr or CsrView);
+    flags: MI_BATCH_SHARED_VALUES (one matrix for every equation) and / or
+    MI_BATCH_ILU0 (two-colour ordering, ILU(0) available)."""
+
+    def __init__(self, A, n_eq, flags=0):
         self.lib = load()
         h = C.c_void_p()
-        check(self.lib.mi_batch_create(A.h, int(n_eq), C.byref(h)))
+        check(self.lib.mi_batch_create_ex(A.h, int(n_eq), int(flags), C.byref(h)))
         self.h = h.value
         self.A = A
+        self.flags = int(flags)
         self.n_eq, self.n, self.nnz = int(n_eq), A.n, A.nnz
 
     def set_equation(self, e, A):
         check(self.lib.mi_batch_set_equation(self.h, int(e), A.h))
+
+    def set_matrix(self, A):
+        check(self.lib.mi_batch_set_matrix(self.h, A.h))
 
     def set_values(self, values):
         v = np.ascontiguousarray(values, dtype=np.float64).reshape(self.n_eq, self.nnz)
@@ -804,6 +828,14 @@ class Batch:
         rc = self.lib.mi_batch_solve_device(self.h, C.byref(c), db.ptr, dx.ptr, ptr(its), ptr(rel))
         check(rc, allowed=(MI_OK, MI_ERR_NOT_CONVERGED))
         return its, rel, rc
+
+    def apply_preconditioner(self, r, preconditioner):
+        """z = M^-1 r per equation; r: (n_eq, n) host array."""
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(self.n_eq, self.n)
+        dr = DeviceArray.from_host(r.ravel())
+        dz = DeviceArray(r.size)
+        check(self.lib.mi_batch_apply_preconditioner(self.h, int(preconditioner), dr.ptr, dz.ptr))
+        return dz.download().reshape(self.n_eq, self.n)
 
     def tuning(self):
         rb, us, by = C.c_int(), C.c_double(), C.c_double()

@@ -195,4 +195,19 @@ MI_API int mi_rcm_permutation(int64_t n, const int32_t* rp, const int32_t* ci, i
     return MI_OK;
 }
 
+MI_API int mi_color_permutation(int64_t n, const int32_t* rp, const int32_t* ci, int32_t* perm,
+                                int32_t* colors)
+{
+    if (n < 0 || n > INT32_MAX - 1 || (n > 0 && (!rp || !ci || !perm)))
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(check_host_csr(n, rp, ci));
+    std::vector<int> p;
+    int nc = 0;
+    color_order(int(n), rp, ci, p, &nc);
+    std::memcpy(perm, p.data(), p.size() * sizeof(int32_t));
+    if (colors)
+        *colors = nc;
+    return MI_OK;
+}
+
 } // extern "C"

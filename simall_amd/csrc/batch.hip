@@ -67,10 +67,15 @@ struct BEq
 
 // Stages.  B_BI_*: BiCGStab; B_BI_T skips equations whose s already met the
 // inner tolerance (sconv).
+// With ILU(0) the preconditioner apply is two launches of its own
+// (k_bilu), so the vector stages that would fold D^-1 into them come in
+// variants without it: B_UPD_X, B_DOT_RZ, B_INIT_Z and B_PZ for CG,
+// B_BI_PN and B_BI_SN for BiCGStab.
 enum BOp : int {
     B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4, B_P = 5,
     B_BI_INIT = 6, B_BI_P = 7, B_BI_ALPHA = 8, B_BI_S = 9, B_BI_T = 10, B_BI_OMEGA = 11,
-    B_BI_UPDATE = 12
+    B_BI_UPDATE = 12, B_UPD_X = 13, B_DOT_RZ = 14, B_INIT_Z = 15, B_PZ = 16, B_BI_PN = 17,
+    B_BI_SN = 18
 };
 
 // Slot-layout stages.
@@ -82,8 +87,10 @@ struct BArgs
     const int* ci;
     const int* soff;   // SELL-64 slice offsets (sell layout)
     const int* eci;    // SELL-64 slot columns (diagonal flag, -1 padding)
-    const float* v32;  // [nnz or padded SELL slots][W]
-    const float* invD; // [n][W]
+    int shared;        // one matrix for every equation: v32[k], invD[i], lv[k]
+    const float* v32;  // [nnz or padded SELL slots][W] ([..] when shared)
+    const float* invD; // [n][W] ([n] when shared): 1/a_ii, or the ILU(0) 1/pivot
+    const float* lv;   // ILU(0) lower factor l_ik at the entries of the second colour's rows
     const int* eq;     // [W] equation of each slot, -1: empty
     float* xs;         // [n][W] correction of the current sweep
     float* r;          // BiCGStab: r, then s in place
@@ -102,6 +109,7 @@ struct HArgs
 {
     int n, E, nrb;
     int64_t nnz;
+    int64_t vstride; // distance between equations' FP64 values: nnz, or 0 when shared
     const int* blk; // pattern row blocks
     const int* rp;
     const int* ci;
@@ -217,7 +225,9 @@ __device__ __forceinline__ void slot_partials_v(const BArgs& a, int R, int grp, 
 // W / V threads per row, each taking V consecutive slots (equations) with
 // V-wide loads of the values and of the gathered rows of `in`: kBlock V / W
 // rows per workgroup at a time; every slot's row is summed in CSR order.
-template <int OP, bool kSell, int V>
+// kShared: one matrix for every equation, its value of a nonzero loaded once
+// and applied to the thread's V slots.
+template <int OP, bool kSell, int V, bool kShared>
 __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 {
     using VT = typename SlotVec<V>::type;
@@ -265,9 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
                         const size_t mk = size_t(m0) + size_t(k0 + j) * kSellSlice;
                         const bool in = k0 + j < w;
                         c[j] = in ? a.eci[mk] : -1;
-                        v[j] = in ? __builtin_nontemporal_load(
-                                        reinterpret_cast<const VT*>(a.v32 + mk * W + s0))
-                                  : VT(0.f);
+                        if constexpr (kShared)
+                            v[j] = in ? VT(a.v32[mk]) : VT(0.f);
+                        else
+                            v[j] = in ? __builtin_nontemporal_load(
+                                            reinterpret_cast<const VT*>(a.v32 + mk * W + s0))
+                                      : VT(0.f);
                     }
                     VT xv[U];
 #pragma unroll
@@ -289,9 +302,12 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
                     for (int j = 0; j < U; ++j) {
                         const int k = k0 + j;
                         c[j] = k < ke ? a.ci[k] : -1;
-                        v[j] = k < ke ? __builtin_nontemporal_load(
-                                            reinterpret_cast<const VT*>(a.v32 + size_t(k) * W + s0))
-                                      : VT(0.f);
+                        if constexpr (kShared)
+                            v[j] = k < ke ? VT(a.v32[k]) : VT(0.f);
+                        else
+                            v[j] = k < ke ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(
+                                                a.v32 + size_t(k) * W + s0))
+                                          : VT(0.f);
                     }
                     VT xv[U];
 #pragma unroll
@@ -330,17 +346,22 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 // thread, the widest that divides W (measured on a 1M-row 7-point batch:
 // V = 4 is the fastest at every width from 4 to 32 equations).
 template <int OP> using BSpmmKernel = void (*)(BArgs);
-template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, int W)
+template <int OP, bool kShared> BSpmmKernel<OP> bspmm_kernel_v(bool sell, int W)
 {
     if (W % 4 == 0)
-        return sell ? k_bspmm<OP, true, 4> : k_bspmm<OP, false, 4>;
+        return sell ? k_bspmm<OP, true, 4, kShared> : k_bspmm<OP, false, 4, kShared>;
     if (W % 2 == 0)
-        return sell ? k_bspmm<OP, true, 2> : k_bspmm<OP, false, 2>;
-    return sell ? k_bspmm<OP, true, 1> : k_bspmm<OP, false, 1>;
+        return sell ? k_bspmm<OP, true, 2, kShared> : k_bspmm<OP, false, 2, kShared>;
+    return sell ? k_bspmm<OP, true, 1, kShared> : k_bspmm<OP, false, 1, kShared>;
+}
+template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, bool shared, int W)
+{
+    return shared ? bspmm_kernel_v<OP, true>(sell, W) : bspmm_kernel_v<OP, false>(sell, W);
 }
 
 // Vector stages over the slot layout (element q = i W + s; thread t of a
-// workgroup takes slot t % W of rows t / W, t / W + P, ...):
+// workgroup takes slot t % W of rows t / W, t / W + P, ...).  D^-1 is
+// invD[q], or invD[i] for a shared matrix:
 //   B_INIT      p = D^-1 r; r.z, r.r
 //   B_UPDATE    xs += alpha p, r -= alpha Ap; r.r, r.z
 //   B_P         p = D^-1 r + beta p
@@ -348,6 +369,13 @@ template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, int W)
 //   B_BI_P      p = r + beta (p - omega v); y = D^-1 p
 //   B_BI_S      s = r - alpha v (in r); z = D^-1 s; s.s
 //   B_BI_UPDATE xs += alpha y + omega z; r = s - omega t; r.r, rh.r
+// and with the preconditioner applied by its own launches (z = M^-1 r in z):
+//   B_UPD_X     xs += alpha p, r -= alpha Ap
+//   B_DOT_RZ    r.r, r.z
+//   B_INIT_Z    p = z; r.z, r.r
+//   B_PZ        p = z + beta p
+//   B_BI_PN     p = r + beta (p - omega v)
+//   B_BI_SN     s = r - alpha v (in r); s.s
 __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
 {
     const SlotMap m(a.W);
@@ -359,30 +387,56 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
     // this thread's elements: rows i0 + rr, i0 + rr + P, ... of its slot
     const int64_t q0 = (i0 + m.rr) * W + m.sl, q1 = i1 * W;
     const int64_t step = int64_t(m.P) * W;
+    const int64_t ib = i0 + m.rr;
+    const bool sh = a.shared != 0;
+    auto dinv = [&](int64_t q, int64_t i) { return sh ? a.invD[i] : a.invD[q]; };
     double s0 = 0.0, s1 = 0.0;
     if (on) {
         const BEq& st = a.st[e];
         if (op == B_INIT) {
-            for (int64_t q = q0; q < q1; q += step) {
-                const float ri = a.r[q], zi = a.invD[q] * ri;
+            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
+                const float ri = a.r[q], zi = dinv(q, i) * ri;
                 a.p[q] = zi;
                 s0 += double(ri) * double(zi);
                 s1 += double(ri) * double(ri);
             }
         } else if (op == B_UPDATE) {
             const float al = float(st.alpha);
-            for (int64_t q = q0; q < q1; q += step) {
+            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
                 a.xs[q] += al * a.p[q];
                 const float ri = a.r[q] - al * a.Ap[q];
                 a.r[q] = ri;
-                const float zi = a.invD[q] * ri;
+                const float zi = dinv(q, i) * ri;
                 s0 += double(ri) * double(ri);
                 s1 += double(ri) * double(zi);
             }
         } else if (op == B_P) {
             const float be = float(st.beta);
+            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P)
+                a.p[q] = dinv(q, i) * a.r[q] + be * a.p[q];
+        } else if (op == B_UPD_X) {
+            const float al = float(st.alpha);
+            for (int64_t q = q0; q < q1; q += step) {
+                a.xs[q] += al * a.p[q];
+                a.r[q] = a.r[q] - al * a.Ap[q];
+            }
+        } else if (op == B_DOT_RZ) {
+            for (int64_t q = q0; q < q1; q += step) {
+                const float ri = a.r[q], zi = a.z[q];
+                s0 += double(ri) * double(ri);
+                s1 += double(ri) * double(zi);
+            }
+        } else if (op == B_INIT_Z) {
+            for (int64_t q = q0; q < q1; q += step) {
+                const float ri = a.r[q], zi = a.z[q];
+                a.p[q] = zi;
+                s0 += double(ri) * double(zi);
+                s1 += double(ri) * double(ri);
+            }
+        } else if (op == B_PZ) {
+            const float be = float(st.beta);
             for (int64_t q = q0; q < q1; q += step)
-                a.p[q] = a.invD[q] * a.r[q] + be * a.p[q];
+                a.p[q] = a.z[q] + be * a.p[q];
         } else if (op == B_BI_INIT) {
             // rh = r; p = v = 0 (BiCGSTABSolver::solve, :366-372)
             for (int64_t q = q0; q < q1; q += step) {
@@ -393,21 +447,23 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
                 s0 += double(ri) * double(ri);
             }
             s1 = s0;
-        } else if (op == B_BI_P) {
+        } else if (op == B_BI_P || op == B_BI_PN) {
             // (:380-382)
             const float be = float(st.beta), om = float(st.omega);
-            for (int64_t q = q0; q < q1; q += step) {
+            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
                 const float pi = a.r[q] + be * (a.p[q] - om * a.Ap[q]);
                 a.p[q] = pi;
-                a.y[q] = a.invD[q] * pi;
+                if (op == B_BI_P)
+                    a.y[q] = dinv(q, i) * pi;
             }
-        } else if (op == B_BI_S) {
+        } else if (op == B_BI_S || op == B_BI_SN) {
             // (:384-386)
             const float al = float(st.alpha);
-            for (int64_t q = q0; q < q1; q += step) {
+            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
                 const float si = a.r[q] - al * a.Ap[q];
                 a.r[q] = si;
-                a.z[q] = a.invD[q] * si;
+                if (op == B_BI_S)
+                    a.z[q] = dinv(q, i) * si;
                 s0 += double(si) * double(si);
             }
         } else { // B_BI_UPDATE (:387-397)
@@ -428,9 +484,176 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
             }
         }
     }
-    if (op == B_P || op == B_BI_P)
+    if (op == B_P || op == B_BI_P || op == B_UPD_X || op == B_PZ || op == B_BI_PN)
         return;
-    slot_partials(a, m, (op == B_BI_S) ? 1 : 2, s0, s1);
+    slot_partials(a, m, (op == B_BI_S || op == B_BI_SN) ? 1 : 2, s0, s1);
+}
+
+// ILU(0) apply out = (LU)^-1 in under the two-colour ordering: rows [0, n0)
+// have no lower entries (their neighbours all come later), rows [n0, n) no
+// upper entries, and the factorization changes only the second colour's
+// pivots (ILU0Preconditioner::setup, src/solver/LinearSolvers.cpp:81-124,
+// on this ordering).  The forward and backward sweeps of apply (:126-150)
+// are then one launch each:
+//   phase 0, rows [n0, n):  out_i = (in_i - sum_k l_ik in_k) / d_i
+//            (the forward sweep leaves the first colour's y_k = in_k)
+//   phase 1, rows [0, n0):  out_i = (in_i - sum_j a_ij out_j) / a_ii
+// with 1/d_i in invD and l_ik in lv at the positions of a_ik.  Slots of
+// equations not iterating are left alone.
+template <bool kSell, int V, bool kShared>
+__global__ __launch_bounds__(kBlock) void k_bilu(BArgs a, const float* __restrict__ in,
+                                                 float* __restrict__ out, int phase, int n0)
+{
+    using VT = typename SlotVec<V>::type;
+    constexpr int U = V == 1 ? kSpmmUnroll : kSpmmUnroll / 2;
+    const int W = a.W;
+    const int TW = W / V;
+    const int R = kBlock / TW;
+    const int t = int(threadIdx.x);
+    const bool live = t < R * TW;
+    const int grp = t / TW;
+    const int s0 = (t % TW) * V;
+    bool on[V];
+    bool any = false, all = true;
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        on[u] = live && slot_on(a, B_ALPHA, a.eq[s0 + u]);
+        any = any || on[u];
+        all = all && on[u];
+    }
+    if (!any)
+        return;
+    const int64_t lo = phase == 0 ? n0 : 0, hi = phase == 0 ? a.n : n0;
+    const float* fac = phase == 0 ? a.lv : a.v32;
+    const float* src = phase == 0 ? in : out;
+    int64_t r0, r1;
+    block_range(hi - lo, R, r0, r1);
+    for (int64_t i = lo + r0 + grp; i < lo + r1; i += R) {
+        const size_t q = size_t(i) * W + s0;
+        VT acc = VT(0.f);
+        auto fetch = [&](int64_t mk, int c, VT& f, VT& xv) {
+            if constexpr (kShared)
+                f = VT(fac[mk]);
+            else
+                f = __builtin_nontemporal_load(reinterpret_cast<const VT*>(fac + mk * W + s0));
+            xv = *reinterpret_cast<const VT*>(src + size_t(c) * W + s0);
+        };
+        if constexpr (kSell) {
+            const int slc = int(i / kSellSlice);
+            const int m0 = a.soff[slc] + int(i % kSellSlice);
+            const int w = (a.soff[slc + 1] - a.soff[slc]) / kSellSlice;
+            for (int k0 = 0; k0 < w; k0 += U) {
+                int c[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    const size_t mk = size_t(m0) + size_t(k0 + j) * kSellSlice;
+                    c[j] = k0 + j < w ? a.eci[mk] : -1;
+                }
+                VT f[U], xv[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    const bool off = c[j] >= 0 && !(c[j] & kSellDiag);
+                    if (off)
+                        fetch(int64_t(m0) + int64_t(k0 + j) * kSellSlice, c[j] & kSellColMask, f[j],
+                              xv[j]);
+                    else
+                        f[j] = xv[j] = VT(0.f);
+                }
+#pragma unroll
+                for (int j = 0; j < U; ++j)
+                    acc += f[j] * xv[j];
+            }
+        } else {
+            const int kb = a.rp[i], ke = a.rp[i + 1];
+            for (int k0 = kb; k0 < ke; k0 += U) {
+                int c[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j)
+                    c[j] = k0 + j < ke ? a.ci[k0 + j] : -1;
+                VT f[U], xv[U];
+#pragma unroll
+                for (int j = 0; j < U; ++j) {
+                    if (c[j] >= 0 && c[j] != int(i))
+                        fetch(k0 + j, c[j], f[j], xv[j]);
+                    else
+                        f[j] = xv[j] = VT(0.f);
+                }
+#pragma unroll
+                for (int j = 0; j < U; ++j)
+                    acc += f[j] * xv[j];
+            }
+        }
+        const VT xi = *reinterpret_cast<const VT*>(in + q);
+        VT dv;
+        if constexpr (kShared)
+            dv = VT(a.invD[i]);
+        else
+            dv = *reinterpret_cast<const VT*>(a.invD + q);
+        const VT res = (xi - acc) * dv;
+        if (all) {
+            *reinterpret_cast<VT*>(out + q) = res;
+        } else {
+#pragma unroll
+            for (int u = 0; u < V; ++u)
+                if (on[u])
+                    out[q + u] = SlotVec<V>::at(res, u);
+        }
+    }
+}
+
+using BIluKernel = void (*)(BArgs, const float*, float*, int, int);
+template <bool kShared> BIluKernel bilu_kernel_v(bool sell, int W)
+{
+    if (W % 4 == 0)
+        return sell ? k_bilu<true, 4, kShared> : k_bilu<false, 4, kShared>;
+    if (W % 2 == 0)
+        return sell ? k_bilu<true, 2, kShared> : k_bilu<false, 2, kShared>;
+    return sell ? k_bilu<true, 1, kShared> : k_bilu<false, 1, kShared>;
+}
+BIluKernel bilu_kernel(bool sell, bool shared, int W)
+{
+    return shared ? bilu_kernel_v<true>(sell, W) : bilu_kernel_v<false>(sell, W);
+}
+
+// ILU(0) factors of the equations e0 .. e0 + ne - 1 (FP64 from the FP64
+// values, stored in FP32), one thread per (equation, row), under the
+// two-colour ordering: a row i >= n0 takes, in its CSR order (every entry
+// but the diagonal precedes it), l_ik = a_ik / a_kk and d_i -= l_ik a_ki, the
+// IKJ loop of ILU0Preconditioner::setup (LinearSolvers.cpp:98-123) on this
+// ordering, where row k's only upper entry found in row i is a_ki; a zero
+// pivot a_kk leaves l_ik = a_ik and d_i alone (:104-106).  A row i < n0 keeps
+// d_i = a_ii.  ipiv = 1/d_i (1 for d_i = 0: the apply then leaves s, :148).
+// tpos[k]: position of the transposed entry; vpos[k]: the value line of
+// entry k in the working layout (the SELL-64 slot, or k).
+__global__ __launch_bounds__(kBlock) void k_bilu_factor(
+    int n, int n0, int ne, int64_t vstride, int64_t lstride, const int* __restrict__ rp,
+    const int* __restrict__ ci, const int* __restrict__ diag, const int* __restrict__ tpos,
+    const int* __restrict__ vpos, const double* __restrict__ v64, float* __restrict__ lv,
+    float* __restrict__ ipiv)
+{
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= int64_t(n) * ne)
+        return;
+    const int e = int(t / n), i = int(t - int64_t(e) * n);
+    const double* va = v64 + size_t(e) * vstride;
+    float* le = lv + size_t(e) * lstride;
+    const int kd = diag[i];
+    double d = va[kd];
+    if (i >= n0) {
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int c = ci[k];
+            if (c >= i)
+                break;
+            const double piv = va[diag[c]];
+            double l = va[k];
+            if (piv != 0.0) {
+                l = l / piv;
+                d -= l * va[tpos[k]];
+            }
+            le[vpos[k]] = float(l);
+        }
+    }
+    ipiv[size_t(e) * n + i] = float(d != 0.0 ? 1.0 / d : 1.0);
 }
 
 // Equation-major stages.  Per-workgroup sums of one value per equation,
@@ -508,7 +731,7 @@ __global__ __launch_bounds__(kBlock) void k_btrue(HArgs a)
     for (int64_t u = u0; u < u1; ++u) {
         const int e = act[u / a.nrb];
         const int rb = int(u % a.nrb);
-        const double* __restrict__ ve = a.v64 + size_t(e) * a.nnz;
+        const double* __restrict__ ve = a.v64 + size_t(e) * a.vstride;
         const double* __restrict__ xe = a.x + size_t(e) * a.n;
         const size_t off = size_t(e) * a.n;
         const int r0 = a.blk[rb], r1 = a.blk[rb + 1];
@@ -839,6 +1062,16 @@ __global__ __launch_bounds__(kBlock) void k_binvdiag(int n, const int* __restric
     invD[i] = float(v != 0.0 ? 1.0 / v : 1.0);
 }
 
+// out = D^-1 in over the whole slot layout (every slot).
+__global__ __launch_bounds__(kBlock) void k_bscale(BArgs a, const float* __restrict__ in,
+                                                   float* __restrict__ out)
+{
+    const int64_t m = int64_t(a.n) * a.W;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride)
+        out[q] = (a.shared ? a.invD[q / a.W] : a.invD[q]) * in[q];
+}
+
 __global__ __launch_bounds__(kBlock) void k_bfill(int64_t m, double* __restrict__ x)
 {
     const int64_t stride = int64_t(gridDim.x) * kBlock;
@@ -858,18 +1091,20 @@ __global__ __launch_bounds__(kBlock) void k_bzero32(int64_t m, float* __restrict
 using namespace mi;
 
 // One slot layout: its width, slot table and the arrays of the working set.
+// v32, invD (and lv with ILU(0)) are per-slot copies of the equations'
+// values; a batch with one shared matrix reads its home arrays instead.
 struct BSlots
 {
     int W = 0;
     std::vector<int> eqs; // equation of each slot, -1 empty (size W)
     DevBuf<int> d_eq;
-    DevBuf<float> v32, invD, xs, r, p, Ap, rh, y, z, t;
+    DevBuf<float> v32, invD, lv, xs, r, p, Ap, rh, y, z, t;
 };
 
 struct mi_batch
 {
     std::shared_ptr<Pattern> src; // the caller's pattern
-    std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM)
+    std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM / two colours)
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
     int E = 0, Wmax = 1;
@@ -878,9 +1113,16 @@ struct mi_batch
     int64_t n = 0, nnz = 0;
     bool sell = false;   // values in the pattern's SELL-64 slot order
     int64_t vrows = 0;   // value lines: nnz, or the SELL-64 slots
+    bool shared = false; // one matrix for every equation (multi-RHS)
+    int Em = 1;          // matrices held: E, or 1 when shared
+    // two-colour ordering for ILU(0): rows [0, n0) and [n0, n)
+    bool ilu_order = false;
+    int n0 = 0;
+    DevBuf<int> d_tpos, d_vpos; // transposed entry; value line of each entry
+    bool ilu_valid = false;     // lvh / ipivh hold the factors of the current values
     // equation-major ("home") arrays
     DevBuf<double> v64, b, x, rt;
-    DevBuf<float> v32h, invDh;
+    DevBuf<float> v32h, invDh, lvh, ipivh, ones;
     DevBuf<double> partials;
     DevBuf<BEq> st;
     BEq* h_st = nullptr;
@@ -888,11 +1130,11 @@ struct mi_batch
     BSlots set[2];
     int cur = 0;
     bool slots_valid = false; // set[cur] holds the home values of its equations
-    bool pc_none = false;     // invD of the packed slots is 1
-    bool bicg_alloc = false;
+    int pc = MI_PC_JACOBI;    // preconditioner the slots were packed for
+    bool ext_alloc = false;   // rh, y, z, t allocated
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
-    int graph_chunk = 0, graph_kind = -1, graph_set = -1, graph_W = 0;
+    int graph_chunk = 0, graph_kind = -1, graph_pc = -1, graph_set = -1, graph_W = 0;
     BFinArgs graph_f{}; // finalize arguments baked into the graph
     double tune_us = 0.0;
 
@@ -916,8 +1158,16 @@ struct mi_batch
         a.ci = pat->ci.p;
         a.soff = sell ? pat->sell.soff.p : nullptr;
         a.eci = sell ? pat->sell.eci.p : nullptr;
-        a.v32 = S.v32.p;
-        a.invD = S.invD.p;
+        a.shared = shared ? 1 : 0;
+        if (shared) {
+            a.v32 = v32h.p;
+            a.invD = pc == MI_PC_ILU ? ipivh.p : pc == MI_PC_NONE ? ones.p : invDh.p;
+            a.lv = lvh.p;
+        } else {
+            a.v32 = S.v32.p;
+            a.invD = S.invD.p;
+            a.lv = S.lv.p;
+        }
         a.eq = S.d_eq.p;
         a.xs = S.xs.p;
         a.r = S.r.p;
@@ -938,6 +1188,7 @@ struct mi_batch
         a.E = E;
         a.nrb = pat->rb.nrb;
         a.nnz = nnz;
+        a.vstride = shared ? 0 : nnz;
         a.blk = pat->rb.blk.p;
         a.rp = pat->rp.p;
         a.ci = pat->ci.p;
@@ -954,19 +1205,25 @@ struct mi_batch
 namespace
 {
 
-// Working-set arrays of a slot layout (sized for the widest layout;
-// BiCGStab's extra vectors when first needed).
-int alloc_slots(mi_batch* B, BSlots& S, bool bicg)
+// Working-set arrays of a slot layout (sized for the widest layout; the
+// vectors of BiCGStab and of a separate preconditioner apply when first
+// needed).
+int alloc_slots(mi_batch* B, BSlots& S, bool ext)
 {
     const size_t nW = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Wmax);
     const size_t zW = size_t(std::max<int64_t>(B->vrows, 1)) * size_t(B->Wmax);
-    if (!S.v32.p) {
-        MI_TRY(S.v32.alloc(zW));
-        for (DevBuf<float>* v : {&S.invD, &S.xs, &S.r, &S.p, &S.Ap})
+    if (!S.xs.p) {
+        if (!B->shared) {
+            MI_TRY(S.v32.alloc(zW));
+            MI_TRY(S.invD.alloc(nW));
+            if (B->ilu_order)
+                MI_TRY(S.lv.alloc(zW));
+        }
+        for (DevBuf<float>* v : {&S.xs, &S.r, &S.p, &S.Ap})
             MI_TRY(v->alloc(nW));
         MI_TRY(S.d_eq.alloc(size_t(kMaxEq)));
     }
-    if (bicg && !S.rh.p)
+    if (ext && !S.rh.p)
         for (DevBuf<float>* v : {&S.rh, &S.y, &S.z, &S.t})
             MI_TRY(v->alloc(nW));
     return MI_OK;
@@ -986,25 +1243,31 @@ unsigned tile_grid(int64_t rows)
     return unsigned(std::max<int64_t>(1, std::min<int64_t>(t, 8192)));
 }
 
-// Slots for `eqs` in the other layout, values and inverse diagonals from the
-// home arrays; it becomes the current layout.
-int pack_from_home(mi_batch* B, const std::vector<int>& eqs, bool bicg)
+// Slots for `eqs` in the other layout, values and preconditioner data from
+// the home arrays (per-equation matrices); it becomes the current layout.
+int pack_from_home(mi_batch* B, const std::vector<int>& eqs, bool ext)
 {
     const int nxt = B->cur ^ 1;
     BSlots& S = B->set[nxt];
-    MI_TRY(alloc_slots(B, S, bicg));
+    MI_TRY(alloc_slots(B, S, ext));
     S.W = int(eqs.size());
     S.eqs = eqs;
     MI_TRY(upload_eqs(B, S));
     hipStream_t s = B->stream;
-    if (B->vrows)
-        k_btile<float><<<tile_grid(B->vrows), kBlock, 0, s>>>(B->vrows, S.W, S.d_eq.p, 0ull, 0, 0,
-                                                              0.f, B->v32h.p, S.v32.p);
-    if (B->n)
-        k_btile<float><<<tile_grid(B->n), kBlock, 0, s>>>(B->n, S.W, S.d_eq.p, 0ull, 0,
-                                                          B->pc_none ? 1 : 0, 1.f, B->invDh.p,
-                                                          S.invD.p);
-    MI_HIP(hipGetLastError());
+    if (!B->shared) {
+        if (B->vrows)
+            k_btile<float><<<tile_grid(B->vrows), kBlock, 0, s>>>(
+                B->vrows, S.W, S.d_eq.p, 0ull, 0, 0, 0.f, B->v32h.p, S.v32.p);
+        const bool ilu = B->pc == MI_PC_ILU;
+        if (B->n)
+            k_btile<float><<<tile_grid(B->n), kBlock, 0, s>>>(
+                B->n, S.W, S.d_eq.p, 0ull, 0, B->pc == MI_PC_NONE ? 1 : 0, 1.f,
+                ilu ? B->ipivh.p : B->invDh.p, S.invD.p);
+        if (ilu && B->vrows)
+            k_btile<float><<<tile_grid(B->vrows), kBlock, 0, s>>>(
+                B->vrows, S.W, S.d_eq.p, 0ull, 0, 0, 0.f, B->lvh.p, S.lv.p);
+        MI_HIP(hipGetLastError());
+    }
     B->cur = nxt;
     B->slots_valid = true;
     return MI_OK;
@@ -1024,11 +1287,11 @@ int retire(mi_batch* B, uint64_t mask)
 
 // The slots `keep` (positions in the current layout) of every working array
 // into the other layout, keep.size() wide.
-int repack(mi_batch* B, const std::vector<int>& keep, bool bicg)
+int repack(mi_batch* B, const std::vector<int>& keep, bool ext)
 {
     const BSlots& A = B->set[B->cur];
     BSlots& S = B->set[B->cur ^ 1];
-    MI_TRY(alloc_slots(B, S, bicg));
+    MI_TRY(alloc_slots(B, S, ext));
     S.W = int(keep.size());
     S.eqs.assign(size_t(S.W), -1);
     std::vector<int> map(size_t(S.W), 0);
@@ -1045,13 +1308,17 @@ int repack(mi_batch* B, const std::vector<int>& keep, bool bicg)
             k_brepack<<<reduce_grid(rows * S.W, 4), kBlock, 0, s>>>(rows, A.W, S.W, d_map.p,
                                                                     from.p, to.p);
     };
-    cp(B->vrows, A.v32, S.v32);
-    cp(B->n, A.invD, S.invD);
+    if (!B->shared) {
+        cp(B->vrows, A.v32, S.v32);
+        cp(B->n, A.invD, S.invD);
+        if (B->pc == MI_PC_ILU)
+            cp(B->vrows, A.lv, S.lv);
+    }
     cp(B->n, A.xs, S.xs);
     cp(B->n, A.r, S.r);
     cp(B->n, A.p, S.p);
     cp(B->n, A.Ap, S.Ap);
-    if (bicg) {
+    if (ext) {
         cp(B->n, A.rh, S.rh);
         cp(B->n, A.y, S.y);
         cp(B->n, A.z, S.z);
@@ -1073,25 +1340,47 @@ int fin(mi_batch* B, int op, const BFinArgs& f, int G, Launcher& L)
              static_cast<const int*>(S.d_eq.p), B->st.p, op, g);
 }
 
-// One CG iteration for every iterating equation (five launches).
+// z = M^-1 in for the iterating slots: the two sweeps of the ILU(0) apply.
+int ilu_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
+{
+    const BIluKernel k = bilu_kernel(B->sell, B->shared, a.W);
+    MI_TRY(L(k, B->G, kBlock, a, in, out, 0, B->n0));
+    return L(k, B->G, kBlock, a, in, out, 1, B->n0);
+}
+
+// One CG iteration for every iterating equation: five launches with Jacobi,
+// eight with ILU(0).
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, a.W), B->G, kBlock, a));
+    MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W), B->G, kBlock, a));
     MI_TRY(fin(B, B_ALPHA, f, B->G, L));
+    if (B->pc == MI_PC_ILU) {
+        MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPD_X)));
+        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+        MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_DOT_RZ)));
+        MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
+        return L(k_bvec, B->Gv, kBlock, a, int(B_PZ));
+    }
     MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPDATE)));
     MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
     return L(k_bvec, B->Gv, kBlock, a, int(B_P));
 }
 
-// One BiCGStab iteration for every iterating equation (nine launches).
+// One BiCGStab iteration for every iterating equation: nine launches with
+// Jacobi, twelve with ILU(0).
 int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_P)));
-    MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, a.W), B->G, kBlock, a));
+    const bool ilu = B->pc == MI_PC_ILU;
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_PN : B_BI_P)));
+    if (ilu)
+        MI_TRY(ilu_apply(B, a, a.p, a.y, L));
+    MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, B->shared, a.W), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_ALPHA, f, B->G, L));
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_S)));
+    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_SN : B_BI_S)));
     MI_TRY(fin(B, B_BI_S, f, B->Gv, L));
-    MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, a.W), B->G, kBlock, a));
+    if (ilu)
+        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+    MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, B->shared, a.W), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_OMEGA, f, B->G, L));
     MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_UPDATE)));
     return fin(B, B_BI_UPDATE, f, B->Gv, L);
@@ -1121,7 +1410,7 @@ int autotune(mi_batch* B)
         if (g > kBatchMaxGrid)
             break;
         Launcher L{B->stream};
-        auto k = bspmm_kernel<B_ALPHA>(B->sell, a.W);
+        auto k = bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W);
         MI_TRY(L(k, unsigned(g), kBlock, a));
         MI_HIP(hipEventRecord(e0, B->stream));
         const int reps = 3;
@@ -1144,13 +1433,20 @@ int autotune(mi_batch* B)
     return MI_OK;
 }
 
-int check_cfg(const mi_solver_cfg* c)
+int check_cfg(const mi_batch* B, const mi_solver_cfg* c)
 {
     if (c->kind != MI_SOLVER_CG && c->kind != MI_SOLVER_BICGSTAB)
         return set_error(MI_ERR_UNSUPPORTED, "batched solve: CG or BiCGStab (kind %d given)",
                          c->kind);
-    if (c->preconditioner != MI_PC_JACOBI && c->preconditioner != MI_PC_NONE)
-        return set_error(MI_ERR_UNSUPPORTED, "batched solve: Jacobi preconditioner only");
+    if (c->preconditioner == MI_PC_ILU) {
+        if (!B->ilu_order)
+            return set_error(MI_ERR_UNSUPPORTED,
+                             "batched solve: ILU(0) needs a batch created with MI_BATCH_ILU0");
+    } else if (c->preconditioner != MI_PC_JACOBI && c->preconditioner != MI_PC_NONE) {
+        return set_error(MI_ERR_UNSUPPORTED,
+                         "batched solve: Jacobi, ILU(0) or no preconditioner (%d given)",
+                         c->preconditioner);
+    }
     if (!(c->tolerance > 0.0) || c->max_iterations < 0)
         return set_error(MI_ERR_BAD_ARG, "bad tolerance or iteration limit");
     return MI_OK;
@@ -1161,6 +1457,23 @@ int read_state(mi_batch* B)
     MI_HIP(hipMemcpyAsync(B->h_st, B->st.p, sizeof(BEq) * size_t(B->E), hipMemcpyDeviceToHost,
                           B->stream));
     MI_HIP(hipStreamSynchronize(B->stream));
+    return MI_OK;
+}
+
+// ILU(0) factors of every held matrix from its FP64 values.
+int factor_ilu(mi_batch* B)
+{
+    if (B->n == 0) {
+        B->ilu_valid = true;
+        return MI_OK;
+    }
+    const Pattern& P = *B->pat;
+    const int64_t work = B->n * int64_t(B->Em);
+    k_bilu_factor<<<grid_for(work, kBlock), kBlock, 0, B->stream>>>(
+        int(B->n), B->n0, B->Em, B->shared ? 0 : B->nnz, B->shared ? 0 : B->vrows, P.rp.p, P.ci.p,
+        P.diag.p, B->d_tpos.p, B->d_vpos.p, B->v64.p, B->lvh.p, B->ipivh.p);
+    MI_HIP(hipGetLastError());
+    B->ilu_valid = true;
     return MI_OK;
 }
 
@@ -1180,15 +1493,20 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         return MI_OK;
     }
     const bool bicg = c.kind == MI_SOLVER_BICGSTAB;
-    const bool pc_none = c.preconditioner == MI_PC_NONE;
-    if (pc_none != B->pc_none) {
-        B->pc_none = pc_none;
+    const int pc = c.preconditioner;
+    if (pc != B->pc) {
+        B->pc = pc;
         B->slots_valid = false;
     }
-    if (bicg && !B->bicg_alloc) {
+    if (pc == MI_PC_ILU && !B->ilu_valid) {
+        MI_TRY(factor_ilu(B));
+        B->slots_valid = false;
+    }
+    const bool ext = bicg || pc == MI_PC_ILU;
+    if (ext && !B->ext_alloc) {
         MI_TRY(alloc_slots(B, B->set[0], true));
         MI_TRY(alloc_slots(B, B->set[1], true));
-        B->bicg_alloc = true;
+        B->ext_alloc = true;
     }
     BFinArgs f{};
     f.E = E;
@@ -1228,7 +1546,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             std::vector<int> have(B->set[B->cur].eqs);
             have.erase(std::remove(have.begin(), have.end(), -1), have.end());
             if (!B->slots_valid || have != want)
-                MI_TRY(pack_from_home(B, want, bicg));
+                MI_TRY(pack_from_home(B, want, ext));
         }
         // r = float(rt) and xs = 0 for this sweep, then the first stage
         {
@@ -1239,8 +1557,18 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             MI_HIP(hipGetLastError());
             Launcher L{s};
             f.W = S.W;
-            MI_TRY(L(k_bvec, B->Gv, kBlock, B->args(B->cur), int(bicg ? B_BI_INIT : B_INIT)));
-            MI_TRY(fin(B, bicg ? B_BI_INIT : B_INIT, f, B->Gv, L));
+            const BArgs a0 = B->args(B->cur);
+            if (bicg) {
+                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_BI_INIT)));
+                MI_TRY(fin(B, B_BI_INIT, f, B->Gv, L));
+            } else if (pc == MI_PC_ILU) {
+                MI_TRY(ilu_apply(B, a0, a0.r, a0.z, L));
+                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_INIT_Z)));
+                MI_TRY(fin(B, B_INIT, f, B->Gv, L));
+            } else {
+                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_INIT)));
+                MI_TRY(fin(B, B_INIT, f, B->Gv, L));
+            }
         }
         for (;;) {
             const BArgs a = B->args(B->cur);
@@ -1250,7 +1578,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             };
             if (use_graph) {
                 if (!B->graph || B->graph_chunk != chunk || B->graph_kind != c.kind ||
-                    B->graph_set != B->cur || B->graph_W != a.W ||
+                    B->graph_pc != pc || B->graph_set != B->cur || B->graph_W != a.W ||
                     std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0) {
                     if (B->graph) {
                         (void)hipGraphExecDestroy(B->graph);
@@ -1263,6 +1591,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
                     }, &B->graph));
                     B->graph_chunk = chunk;
                     B->graph_kind = c.kind;
+                    B->graph_pc = pc;
                     B->graph_set = B->cur;
                     B->graph_W = a.W;
                     B->graph_f = f;
@@ -1294,7 +1623,7 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
                 // idle slots: their corrections go home and the live slots
                 // move to a layout as wide as their number
                 MI_TRY(retire(B, idle));
-                MI_TRY(repack(B, live, bicg));
+                MI_TRY(repack(B, live, ext));
                 B->slots_valid = false; // no longer the home values of `want`
             }
         }
@@ -1320,29 +1649,129 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     return status;
 }
 
+// The two-colour ordering of a pattern (rows of the first colour first, RCM
+// order within each colour), the first row of the second colour, and for
+// every entry of the reordered pattern the position of its transposed entry.
+// Refused unless the pattern is structurally symmetric, holds every
+// diagonal and has two colours.
+int two_colour_order(const Pattern& src, std::vector<int>& perm, std::vector<int>& brp,
+                     std::vector<int>& bci, std::vector<int>& vmap, std::vector<int>& tpos,
+                     int& n0)
+{
+    const int n = int(src.n);
+    int colors = 0;
+    color_order(n, src.h_rp.data(), src.h_ci.data(), perm, &colors);
+    if (colors > 2)
+        return set_error(MI_ERR_UNSUPPORTED,
+                         "batched ILU(0): the pattern needs %d colours (two supported)", colors);
+    permute_pattern(n, src.h_rp.data(), src.h_ci.data(), perm, brp, bci, vmap);
+    n0 = n;
+    for (int i = 0; i < n && n0 == n; ++i)
+        for (int k = brp[size_t(i)]; k < brp[size_t(i) + 1]; ++k)
+            if (bci[size_t(k)] < i) {
+                n0 = i;
+                break;
+            }
+    tpos.assign(bci.size(), -1);
+    for (int i = 0; i < n; ++i) {
+        bool has_diag = false;
+        for (int k = brp[size_t(i)]; k < brp[size_t(i) + 1]; ++k) {
+            const int c = bci[size_t(k)];
+            if (c == i) {
+                has_diag = true;
+                tpos[size_t(k)] = k;
+                continue;
+            }
+            if ((i < n0) == (c < n0))
+                return set_error(MI_ERR_UNSUPPORTED,
+                                 "batched ILU(0): rows %d and %d of one colour are coupled", i, c);
+            const int* b0 = bci.data() + brp[size_t(c)];
+            const int* b1 = bci.data() + brp[size_t(c) + 1];
+            const int* f = std::lower_bound(b0, b1, i);
+            if (f == b1 || *f != i)
+                return set_error(MI_ERR_UNSUPPORTED,
+                                 "batched ILU(0): the pattern is not symmetric (%d, %d)", i, c);
+            tpos[size_t(k)] = int(f - bci.data());
+        }
+        if (!has_diag)
+            return set_error(MI_ERR_UNSUPPORTED, "batched ILU(0): row %d has no diagonal", i);
+    }
+    return MI_OK;
+}
+
+// Value line of each CSR entry in the working layout: its SELL-64 slot.
+__global__ __launch_bounds__(kBlock) void k_bslot_of(int64_t m, const int* __restrict__ eperm,
+                                                     int* __restrict__ vpos)
+{
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < m; i += stride) {
+        const int k = eperm[i];
+        if (k >= 0)
+            vpos[k] = int(i);
+    }
+}
+
+// Values of matrix slot e (0 when shared) from the caller's FP64 values.
+int write_matrix(mi_batch* B, int e, const mi_csr* A)
+{
+    hipStream_t s = B->stream;
+    double* v64 = B->v64.p + size_t(e) * size_t(B->nnz);
+    float* v32 = B->v32h.p + size_t(e) * size_t(B->vrows);
+    if (B->nnz)
+        k_bconvert<<<reduce_grid(B->nnz, 4), kBlock, 0, s>>>(
+            B->nnz, B->reordered ? B->d_vmap.p : nullptr, A->val.p, v64, B->sell ? nullptr : v32);
+    if (B->sell && B->vrows)
+        k_bsell32<<<reduce_grid(B->vrows, 4), kBlock, 0, s>>>(B->vrows, B->pat->sell.eperm.p, v64,
+                                                              v32);
+    if (B->n)
+        k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
+            int(B->n), B->pat->diag.p, v64, B->invDh.p + size_t(e) * size_t(B->n));
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(s));
+    B->slots_valid = false;
+    B->ilu_valid = false;
+    return MI_OK;
+}
+
+bool same_pattern(const mi_batch* B, const mi_csr* A)
+{
+    return A->pat == B->src || (A->n() == B->n && A->nnz() == B->nnz &&
+                                A->pat->h_rp == B->src->h_rp && A->pat->h_ci == B->src->h_ci);
+}
+
 } // namespace
 
 extern "C" {
 
-MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
+MI_API int mi_batch_create_ex(const mi_csr* A, int n_eq, int flags, mi_batch** out)
 {
     if (!A || !out || n_eq < 1 || n_eq > kMaxEq)
         return set_error(MI_ERR_BAD_ARG, "need a pattern matrix and 1..%d equations", kMaxEq);
+    if (flags & ~(MI_BATCH_SHARED_VALUES | MI_BATCH_ILU0))
+        return set_error(MI_ERR_BAD_ARG, "unknown batch flags 0x%x", flags);
     MI_TRY(ensure_device());
     auto B = std::make_unique<mi_batch>();
     B->src = A->pat;
     B->pat = A->pat;
     B->E = n_eq;
     B->Wmax = n_eq;
+    B->shared = (flags & MI_BATCH_SHARED_VALUES) != 0;
+    B->Em = B->shared ? 1 : n_eq;
+    B->ilu_order = (flags & MI_BATCH_ILU0) != 0;
     B->n = A->n();
     B->nnz = A->nnz();
     MI_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    if (B->n >= 100000) {
-        // the same bandwidth reduction as the single-equation solver's setup
-        const Pattern& src = *A->pat;
+    const Pattern& src = *A->pat;
+    std::vector<int> tpos;
+    if (B->ilu_order || B->n >= 100000) {
+        // two colours for ILU(0), else the single-equation solver's bandwidth reduction
         std::vector<int> perm, brp, bci, vmap;
-        rcm_order(int(src.n), src.h_rp.data(), src.h_ci.data(), perm);
-        permute_pattern(int(src.n), src.h_rp.data(), src.h_ci.data(), perm, brp, bci, vmap);
+        if (B->ilu_order) {
+            MI_TRY(two_colour_order(src, perm, brp, bci, vmap, tpos, B->n0));
+        } else {
+            rcm_order(int(src.n), src.h_rp.data(), src.h_ci.data(), perm);
+            permute_pattern(int(src.n), src.h_rp.data(), src.h_ci.data(), perm, brp, bci, vmap);
+        }
         MI_TRY(make_pattern(src.n, src.nnz, brp.data(), bci.data(), B->pat, B->stream));
         MI_TRY(B->d_perm.upload(perm.data(), perm.size(), B->stream));
         MI_TRY(B->d_vmap.upload(vmap.data(), vmap.size(), B->stream));
@@ -1353,19 +1782,42 @@ MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
     B->sell = B->pat->sell.built() && B->pat->sell.eperm.p != nullptr;
     B->vrows = B->sell ? B->pat->sell.padded : B->nnz;
     const size_t nE = size_t(std::max<int64_t>(B->n, 1)) * size_t(n_eq);
-    const size_t zE = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(n_eq);
-    const size_t vE = size_t(std::max<int64_t>(B->vrows, 1)) * size_t(n_eq);
-    MI_TRY(B->v64.alloc(zE));
-    MI_TRY(B->v32h.alloc(vE));
-    MI_TRY(B->invDh.alloc(nE));
+    const size_t nM = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Em);
+    const size_t zM = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(B->Em);
+    const size_t vM = size_t(std::max<int64_t>(B->vrows, 1)) * size_t(B->Em);
+    MI_TRY(B->v64.alloc(zM));
+    MI_TRY(B->v32h.alloc(vM));
+    MI_TRY(B->invDh.alloc(nM));
     for (DevBuf<double>* v : {&B->b, &B->x, &B->rt})
         MI_TRY(v->alloc(nE));
     MI_TRY(B->partials.alloc(size_t(2) * kBatchMaxGrid * size_t(kMaxEq)));
     MI_TRY(B->st.alloc(size_t(n_eq)));
-    MI_HIP(hipMemsetAsync(B->v32h.p, 0, vE * sizeof(float), B->stream));
-    MI_HIP(hipMemsetAsync(B->v64.p, 0, zE * sizeof(double), B->stream));
-    MI_HIP(hipMemsetAsync(B->invDh.p, 0, nE * sizeof(float), B->stream));
+    MI_HIP(hipMemsetAsync(B->v32h.p, 0, vM * sizeof(float), B->stream));
+    MI_HIP(hipMemsetAsync(B->v64.p, 0, zM * sizeof(double), B->stream));
+    MI_HIP(hipMemsetAsync(B->invDh.p, 0, nM * sizeof(float), B->stream));
     MI_HIP(hipMemsetAsync(B->st.p, 0, sizeof(BEq) * size_t(n_eq), B->stream));
+    if (B->shared) {
+        std::vector<float> one(nM / size_t(B->Em), 1.f);
+        MI_TRY(B->ones.upload(one.data(), one.size(), B->stream));
+    }
+    if (B->ilu_order) {
+        MI_TRY(B->lvh.alloc(vM));
+        MI_TRY(B->ipivh.alloc(nM));
+        MI_HIP(hipMemsetAsync(B->lvh.p, 0, vM * sizeof(float), B->stream));
+        MI_HIP(hipMemsetAsync(B->ipivh.p, 0, nM * sizeof(float), B->stream));
+        MI_TRY(B->d_tpos.upload(tpos.data(), std::max<size_t>(tpos.size(), 1), B->stream));
+        if (B->sell) {
+            MI_TRY(B->d_vpos.alloc(size_t(std::max<int64_t>(B->nnz, 1))));
+            k_bslot_of<<<reduce_grid(B->vrows, 4), kBlock, 0, B->stream>>>(
+                B->vrows, B->pat->sell.eperm.p, B->d_vpos.p);
+            MI_HIP(hipGetLastError());
+        } else {
+            std::vector<int> id(size_t(std::max<int64_t>(B->nnz, 1)));
+            for (size_t k = 0; k < id.size(); ++k)
+                id[k] = int(k);
+            MI_TRY(B->d_vpos.upload(id.data(), id.size(), B->stream));
+        }
+    }
     MI_HIP(hipStreamSynchronize(B->stream));
     MI_TRY(alloc_slots(B.get(), B->set[0], false));
     MI_TRY(alloc_slots(B.get(), B->set[1], false));
@@ -1379,29 +1831,33 @@ MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
     return MI_OK;
 }
 
+MI_API int mi_batch_create(const mi_csr* A, int n_eq, mi_batch** out)
+{
+    return mi_batch_create_ex(A, n_eq, 0, out);
+}
+
 MI_API int mi_batch_set_equation(mi_batch* B, int eq, const mi_csr* A)
 {
     if (!B || !A || eq < 0 || eq >= B->E)
         return set_error(MI_ERR_BAD_ARG, "bad argument");
-    if (A->pat != B->src && (A->n() != B->n || A->nnz() != B->nnz ||
-                             A->pat->h_rp != B->src->h_rp || A->pat->h_ci != B->src->h_ci))
+    if (B->shared)
+        return set_error(MI_ERR_BAD_ARG,
+                         "a batch with shared values holds one matrix: use mi_batch_set_matrix");
+    if (!same_pattern(B, A))
         return set_error(MI_ERR_BAD_ARG, "matrix does not have the batch's pattern");
     MI_TRY(ensure_device());
-    hipStream_t s = B->stream;
-    double* v64 = B->v64.p + size_t(eq) * size_t(B->nnz);
-    float* v32 = B->v32h.p + size_t(eq) * size_t(B->vrows);
-    if (B->nnz)
-        k_bconvert<<<reduce_grid(B->nnz, 4), kBlock, 0, s>>>(
-            B->nnz, B->reordered ? B->d_vmap.p : nullptr, A->val.p, v64, B->sell ? nullptr : v32);
-    if (B->sell && B->vrows)
-        k_bsell32<<<reduce_grid(B->vrows, 4), kBlock, 0, s>>>(B->vrows, B->pat->sell.eperm.p, v64,
-                                                              v32);
-    if (B->n)
-        k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
-            int(B->n), B->pat->diag.p, v64, B->invDh.p + size_t(eq) * size_t(B->n));
-    MI_HIP(hipGetLastError());
-    MI_HIP(hipStreamSynchronize(s));
-    B->slots_valid = false;
+    return write_matrix(B, eq, A);
+}
+
+MI_API int mi_batch_set_matrix(mi_batch* B, const mi_csr* A)
+{
+    if (!B || !A)
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    if (!same_pattern(B, A))
+        return set_error(MI_ERR_BAD_ARG, "matrix does not have the batch's pattern");
+    MI_TRY(ensure_device());
+    for (int e = 0; e < B->Em; ++e)
+        MI_TRY(write_matrix(B, e, A));
     return MI_OK;
 }
 
@@ -1409,6 +1865,9 @@ MI_API int mi_batch_set_values(mi_batch* B, const double* values_host)
 {
     if (!B || (B->nnz && !values_host))
         return set_error(MI_ERR_BAD_ARG, "bad argument");
+    if (B->shared)
+        return set_error(MI_ERR_BAD_ARG,
+                         "a batch with shared values holds one matrix: use mi_batch_set_matrix");
     MI_TRY(ensure_device());
     mi_csr tmp;
     tmp.pat = B->src;
@@ -1430,9 +1889,75 @@ MI_API int mi_batch_solve_device(mi_batch* B, const mi_solver_cfg* cfg, const do
     mi_solver_cfg_default(&c);
     if (cfg)
         c = *cfg;
-    MI_TRY(check_cfg(&c));
+    MI_TRY(check_cfg(B, &c));
     MI_TRY(ensure_device());
     return solve(B, c, b_device, x_device, iterations, rel_residual);
+}
+
+MI_API int mi_batch_apply_preconditioner(mi_batch* B, int preconditioner, const double* r_device,
+                                         double* z_device)
+{
+    clear_error();
+    if (!B || (B->n && (!r_device || !z_device)))
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    mi_solver_cfg c;
+    mi_solver_cfg_default(&c);
+    c.kind = MI_SOLVER_BICGSTAB;
+    c.preconditioner = preconditioner;
+    MI_TRY(check_cfg(B, &c));
+    MI_TRY(ensure_device());
+    const int64_t n = B->n;
+    if (n == 0)
+        return MI_OK;
+    hipStream_t s = B->stream;
+    if (preconditioner != B->pc) {
+        B->pc = preconditioner;
+        B->slots_valid = false;
+    }
+    if (preconditioner == MI_PC_ILU && !B->ilu_valid)
+        MI_TRY(factor_ilu(B));
+    if (!B->ext_alloc) {
+        MI_TRY(alloc_slots(B, B->set[0], true));
+        MI_TRY(alloc_slots(B, B->set[1], true));
+        B->ext_alloc = true;
+    }
+    std::vector<int> all(size_t(B->E));
+    for (int e = 0; e < B->E; ++e)
+        all[size_t(e)] = e;
+    MI_TRY(pack_from_home(B, all, true));
+    BSlots& S = B->set[B->cur];
+    const size_t m = size_t(n) * size_t(B->E);
+    if (B->reordered)
+        k_bpermute<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(n, B->E, B->d_perm.p, r_device,
+                                                                 B->rt.p, 0);
+    else
+        MI_HIP(hipMemcpyAsync(B->rt.p, r_device, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    k_btile<double><<<tile_grid(n), kBlock, 0, s>>>(n, S.W, S.d_eq.p, 0ull, 0, 0, 0.f, B->rt.p,
+                                                    S.r.p);
+    MI_HIP(hipGetLastError());
+    BArgs a = B->args(B->cur);
+    a.st = nullptr; // every slot
+    Launcher L{s};
+    if (preconditioner == MI_PC_ILU) {
+        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+    } else {
+        k_bscale<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(a, a.r, a.z);
+        MI_HIP(hipGetLastError());
+    }
+    k_bfill<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(int64_t(m), B->x.p);
+    MI_HIP(hipGetLastError());
+    const uint64_t every = S.W >= 64 ? ~0ull : ((1ull << S.W) - 1);
+    k_btile<double><<<tile_grid(n), kBlock, 0, s>>>(n, S.W, S.d_eq.p, every, 1, 0, 0.f, B->x.p,
+                                                    a.z);
+    MI_HIP(hipGetLastError());
+    if (B->reordered)
+        k_bpermute<<<reduce_grid(int64_t(m), 4), kBlock, 0, s>>>(n, B->E, B->d_perm.p, B->x.p,
+                                                                 z_device, 1);
+    else
+        MI_HIP(hipMemcpyAsync(z_device, B->x.p, m * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MI_HIP(hipGetLastError());
+    MI_HIP(hipStreamSynchronize(s));
+    return MI_OK;
 }
 
 MI_API int mi_batch_solve(mi_batch* B, const mi_solver_cfg* cfg, const double* b_host,
@@ -1463,8 +1988,8 @@ MI_API int mi_batch_tuning(const mi_batch* B, int* rows_per_block, double* spmm_
         *rows_per_block = int((B->n + B->G - 1) / B->G);
     if (spmm_us)
         *spmm_us = B->tune_us;
-    if (spmm_bytes) // FP32 values per equation, shared columns and row pointers, p once, Ap once
-        *spmm_bytes = double(B->nnz) * (4.0 * B->E + 4.0) + 4.0 * double(B->n + 1) +
+    if (spmm_bytes) // FP32 values per matrix, shared columns and row pointers, p once, Ap once
+        *spmm_bytes = double(B->nnz) * (4.0 * B->Em + 4.0) + 4.0 * double(B->n + 1) +
                       2.0 * 4.0 * double(B->n) * B->E;
     return MI_OK;
 }

@@ -369,8 +369,10 @@ int flow_solve(mi_flow* F, mi_solver* S, const double* b, double* x)
 // The momentum predictor's three solves as one batched solve: the components
 // share the matrix (only the rhs depends on the component), so the three
 // assemblies write the same values; each rhs goes to its third of rhs3, the
-// batch solves the three systems together from x = 0, and each residual
-// ||A U_k - b_k|| is formed as in the one-at-a-time path.
+// batch solves the three systems together from x = 0 with BiCGStab and one
+// ILU(0) factorization of the shared matrix (the matrix and its factors are
+// streamed once per launch for the three right-hand sides), and each
+// residual ||A U_k - b_k|| is formed as in the one-at-a-time path.
 int momentum_batched(mi_flow* F, mi_flow_residuals* res, bool transient, bool bdf2)
 {
     mi_flow_cfg& c = F->cfg;
@@ -387,12 +389,11 @@ int momentum_batched(mi_flow* F, mi_flow_residuals* res, bool transient, bool bd
     MI_HIP(hipGetLastError());
     MI_HIP(hipStreamSynchronize(F->s));
     if (!F->mb)
-        MI_TRY(mi_batch_create(F->Am.get(), 3, &F->mb));
-    for (int k = 0; k < 3; ++k)
-        MI_TRY(mi_batch_set_equation(F->mb, k, F->Am.get()));
+        MI_TRY(mi_batch_create_ex(F->Am.get(), 3, MI_BATCH_SHARED_VALUES | MI_BATCH_ILU0, &F->mb));
+    MI_TRY(mi_batch_set_matrix(F->mb, F->Am.get()));
     mi_solver_cfg bc = c.momentum;
     bc.kind = MI_SOLVER_BICGSTAB;
-    bc.preconditioner = MI_PC_JACOBI;
+    bc.preconditioner = MI_PC_ILU;
     int its[3];
     double rel[3];
     const int rc = mi_batch_solve_device(F->mb, &bc, F->rhs3.p, F->x3.p, its, rel);

@@ -116,6 +116,26 @@ def test_rcm_permutation():
     assert bw <= 3 * 10 * 10
 
 
+def test_color_permutation():
+    """Multicolour ordering of the ILU(0) solvers: a 7-point grid takes two
+    colours, rows of one colour are never coupled, and each colour keeps the
+    reverse Cuthill-McKee order; a 3-cycle needs three colours."""
+    A, _ = system(10)
+    perm, nc = capi.color_permutation(A.indptr, A.indices)
+    assert nc == 2
+    assert np.array_equal(np.sort(perm), np.arange(A.shape[0]))
+    B = A[perm][:, perm].tocoo()
+    off = B.row != B.col
+    n0 = int(B.col[off & (B.col < B.row)].max()) + 1  # last first-colour row + 1
+    assert np.all((B.row[off] < n0) != (B.col[off] < n0))
+    rcm, _ = capi.rcm_permutation(A.indptr, A.indices)
+    pos = np.empty_like(rcm)
+    pos[rcm] = np.arange(rcm.size)
+    assert np.all(np.diff(pos[perm[:n0]]) > 0) and np.all(np.diff(pos[perm[n0:]]) > 0)
+    tri = np.array([0, 3, 6, 9], np.int32), np.array([0, 1, 2] * 3, np.int32)
+    assert capi.color_permutation(*tri)[1] == 3
+
+
 def test_host_csr_input_is_validated():
     """Malformed patterns are rejected before any host algorithm walks them."""
     A, _ = system(4)
