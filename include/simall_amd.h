@@ -328,6 +328,24 @@ MI_API int mi_batch_solve_device(mi_batch* b, const mi_solver_cfg* cfg, const do
  * equations active) and its algorithmic bytes. */
 MI_API int mi_batch_tuning(const mi_batch* b, int* rows_per_block, double* spmm_us,
                            double* spmm_bytes);
+/* The shapes the autotuner picked: the cap on equations per thread of the
+ * SpMM and ILU(0) sweeps (4, 2 or 1; a launch takes the widest of those that
+ * divides its width), their workgroup count and row walk (banded: 1, the
+ * workgroups of each XCD sweeping its eighth of the rows together; 0, one
+ * contiguous range per workgroup), the vector stages' workgroup count and the
+ * time of one vector stage (the CG update, all equations). */
+MI_API int mi_batch_tuning_shape(const mi_batch* b, int* slots_per_thread_max,
+                                 int* spmm_workgroups, int* banded, int* vector_workgroups,
+                                 double* vector_us);
+/* Cost of a grid-wide barrier on this device: microseconds per cooperative
+ * grid.sync() in a kernel of `workgroups` x 256 threads (each thread updates
+ * one element between barriers), and per kernel boundary for the same work
+ * as back-to-back launches.  The batched solver separates the dependent
+ * stages of an iteration by kernel boundaries (replayed from a HIP graph)
+ * rather than by grid barriers inside one persistent kernel; this measures
+ * that choice.  No reference counterpart. */
+MI_API int mi_time_grid_barrier(int workgroups, int iterations, double* sync_us,
+                                double* launch_us);
 MI_API int mi_batch_destroy(mi_batch* b);
 
 /* ------------------------------------------------- host-side inspection

@@ -193,6 +193,9 @@ PROTOTYPES = {
     "mi_batch_set_equation": (_i, [_vp, _i, _vp]),
     "mi_batch_set_matrix": (_i, [_vp, _vp]),
     "mi_batch_apply_preconditioner": (_i, [_vp, _i, _vp, _vp]),
+    "mi_batch_tuning_shape": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
+                                   C.POINTER(_i), C.POINTER(_d)]),
+    "mi_time_grid_barrier": (_i, [_i, _i, C.POINTER(_d), C.POINTER(_d)]),
     "mi_color_permutation": (_i, [_i64, _vp, _vp, _vp, C.POINTER(C.c_int32)]),
     "mi_batch_set_values": (_i, [_vp, _vp]),
     "mi_batch_solve": (_i, [_vp, C.POINTER(SolverCfg), _vp, _vp, _vp, _vp]),
@@ -770,6 +773,13 @@ MI_BATCH_SHARED_VALUES = 1
 MI_BATCH_ILU0 = 2
 
 
+def time_grid_barrier(workgroups, iterations=200):
+    """(us per cooperative grid.sync(), us per kernel boundary) on this device."""
+    a, b = C.c_double(), C.c_double()
+    check(load().mi_time_grid_barrier(int(workgroups), int(iterations), C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
 def color_permutation(rowPtr, colIdx):
     """perm[new] = old of the multicolour ordering, and the colour count."""
     rp = np.ascontiguousarray(rowPtr, dtype=np.int32)
@@ -840,7 +850,12 @@ class Batch:
     def tuning(self):
         rb, us, by = C.c_int(), C.c_double(), C.c_double()
         check(self.lib.mi_batch_tuning(self.h, C.byref(rb), C.byref(us), C.byref(by)))
-        return {"rows_per_block": rb.value, "spmm_us": us.value, "spmm_bytes": by.value}
+        v, g, bd, gv, vus = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        check(self.lib.mi_batch_tuning_shape(self.h, C.byref(v), C.byref(g), C.byref(bd),
+                                             C.byref(gv), C.byref(vus)))
+        return {"rows_per_block": rb.value, "spmm_us": us.value, "spmm_bytes": by.value,
+                "slots_per_thread_max": v.value, "spmm_workgroups": g.value,
+                "banded": bd.value, "vector_workgroups": gv.value, "vector_us": vus.value}
 
     def close(self):
         if getattr(self, "h", None):

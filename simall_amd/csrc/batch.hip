@@ -44,6 +44,8 @@
 #include "reorder.hpp"
 #include "stream.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -88,6 +90,7 @@ struct BArgs
     const int* soff;   // SELL-64 slice offsets (sell layout)
     const int* eci;    // SELL-64 slot columns (diagonal flag, -1 padding)
     int shared;        // one matrix for every equation: v32[k], invD[i], lv[k]
+    int band;          // row walk of the SpMM and ILU sweeps (row_walk)
     const float* v32;  // [nnz or padded SELL slots][W] ([..] when shared)
     const float* invD; // [n][W] ([n] when shared): 1/a_ii, or the ILU(0) 1/pivot
     const float* lv;   // ILU(0) lower factor l_ik at the entries of the second colour's rows
@@ -178,6 +181,40 @@ __device__ __forceinline__ void block_range(int64_t count, int64_t align, int64_
     end = min(count, beg + per);
 }
 
+// Rows [lo, lo + count) of a launch whose workgroups take passes of R rows.
+// Contiguous: each workgroup one range (block_range).  Banded: the rows are
+// cut in eight ranges, one per XCD (a workgroup runs on XCD blockIdx % 8),
+// and the workgroups of an XCD take its passes in turn, so that together
+// they sweep a band of G/8 passes that moves through the range: the rows a
+// band gathers (its own +- the bandwidth) stay few enough to be reused from
+// the XCD's L2, where contiguous ranges spread the XCD's workgroups over its
+// whole range at once.  Walk: passes start at base, base + stride, ... < lim.
+struct RowWalk
+{
+    int64_t base, lim, stride;
+};
+__device__ __forceinline__ RowWalk row_walk(int64_t lo, int64_t count, int R, int band)
+{
+    RowWalk w;
+    const int G = gridDim.x;
+    if (band && G % 8 == 0) {
+        const int Gx = G / 8;
+        const int x = int(blockIdx.x % 8), j = int(blockIdx.x / 8);
+        const int64_t per = ((count + 7) / 8 + R - 1) / R * R;
+        const int64_t xb = min(count, int64_t(x) * per), xe = min(count, xb + per);
+        w.base = lo + xb + int64_t(j) * R;
+        w.lim = lo + xe;
+        w.stride = int64_t(Gx) * R;
+    } else {
+        int64_t r0, r1;
+        block_range(count, R, r0, r1);
+        w.base = lo + r0;
+        w.lim = lo + r1;
+        w.stride = R;
+    }
+    return w;
+}
+
 // ------------------------------------------------------------------ kernels
 // V consecutive FP32 slots of one row as one load (V = 1, 2 or 4).
 template <int V> struct SlotVec
@@ -250,14 +287,16 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
     const float* vin = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
     float* vout = OP == B_BI_T ? a.t : a.Ap;
     const float* vw = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
-    int64_t r0, r1;
-    block_range(a.n, R, r0, r1);
+    const RowWalk rw = row_walk(0, a.n, R, a.band);
     double d0[V], d1[V];
 #pragma unroll
     for (int u = 0; u < V; ++u)
         d0[u] = d1[u] = 0.0;
     if (any) {
-        for (int64_t i = r0 + grp; i < r1; i += R) {
+        for (int64_t pass = rw.base; pass < rw.lim; pass += rw.stride) {
+            const int64_t i = pass + grp;
+            if (i >= rw.lim)
+                break;
             const size_t q = size_t(i) * W + s0;
             // the dot product's operand, loaded ahead of the row's streams
             const VT wq = *reinterpret_cast<const VT*>(vw + q);
@@ -343,20 +382,30 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 }
 
 // The SpMV kernel for the slot width W of a launch: V = 4, 2 or 1 slots per
-// thread, the widest that divides W (measured on a 1M-row 7-point batch:
-// V = 4 is the fastest at every width from 4 to 32 equations).
+// thread (slots_per_thread; the cap is autotuned per batch).
 template <int OP> using BSpmmKernel = void (*)(BArgs);
-template <int OP, bool kShared> BSpmmKernel<OP> bspmm_kernel_v(bool sell, int W)
+// Slots per thread of a launch of width W: the widest of 4, 2, 1 that divides
+// W and does not exceed the autotuned cap vmax.
+inline int slots_per_thread(int W, int vmax)
 {
-    if (W % 4 == 0)
+    if (W % 4 == 0 && vmax >= 4)
+        return 4;
+    if (W % 2 == 0 && vmax >= 2)
+        return 2;
+    return 1;
+}
+template <int OP, bool kShared> BSpmmKernel<OP> bspmm_kernel_v(bool sell, int V)
+{
+    if (V == 4)
         return sell ? k_bspmm<OP, true, 4, kShared> : k_bspmm<OP, false, 4, kShared>;
-    if (W % 2 == 0)
+    if (V == 2)
         return sell ? k_bspmm<OP, true, 2, kShared> : k_bspmm<OP, false, 2, kShared>;
     return sell ? k_bspmm<OP, true, 1, kShared> : k_bspmm<OP, false, 1, kShared>;
 }
-template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, bool shared, int W)
+template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, bool shared, int W, int vmax)
 {
-    return shared ? bspmm_kernel_v<OP, true>(sell, W) : bspmm_kernel_v<OP, false>(sell, W);
+    const int V = slots_per_thread(W, vmax);
+    return shared ? bspmm_kernel_v<OP, true>(sell, V) : bspmm_kernel_v<OP, false>(sell, V);
 }
 
 // Vector stages over the slot layout (element q = i W + s; thread t of a
@@ -526,9 +575,11 @@ __global__ __launch_bounds__(kBlock) void k_bilu(BArgs a, const float* __restric
     const int64_t lo = phase == 0 ? n0 : 0, hi = phase == 0 ? a.n : n0;
     const float* fac = phase == 0 ? a.lv : a.v32;
     const float* src = phase == 0 ? in : out;
-    int64_t r0, r1;
-    block_range(hi - lo, R, r0, r1);
-    for (int64_t i = lo + r0 + grp; i < lo + r1; i += R) {
+    const RowWalk rw = row_walk(lo, hi - lo, R, a.band);
+    for (int64_t pass = rw.base; pass < rw.lim; pass += rw.stride) {
+        const int64_t i = pass + grp;
+        if (i >= rw.lim)
+            break;
         const size_t q = size_t(i) * W + s0;
         VT acc = VT(0.f);
         auto fetch = [&](int64_t mk, int c, VT& f, VT& xv) {
@@ -602,17 +653,18 @@ __global__ __launch_bounds__(kBlock) void k_bilu(BArgs a, const float* __restric
 }
 
 using BIluKernel = void (*)(BArgs, const float*, float*, int, int);
-template <bool kShared> BIluKernel bilu_kernel_v(bool sell, int W)
+template <bool kShared> BIluKernel bilu_kernel_v(bool sell, int V)
 {
-    if (W % 4 == 0)
+    if (V == 4)
         return sell ? k_bilu<true, 4, kShared> : k_bilu<false, 4, kShared>;
-    if (W % 2 == 0)
+    if (V == 2)
         return sell ? k_bilu<true, 2, kShared> : k_bilu<false, 2, kShared>;
     return sell ? k_bilu<true, 1, kShared> : k_bilu<false, 1, kShared>;
 }
-BIluKernel bilu_kernel(bool sell, bool shared, int W)
+BIluKernel bilu_kernel(bool sell, bool shared, int W, int vmax)
 {
-    return shared ? bilu_kernel_v<true>(sell, W) : bilu_kernel_v<false>(sell, W);
+    const int V = slots_per_thread(W, vmax);
+    return shared ? bilu_kernel_v<true>(sell, V) : bilu_kernel_v<false>(sell, V);
 }
 
 // ILU(0) factors of the equations e0 .. e0 + ne - 1 (FP64 from the FP64
@@ -1086,6 +1138,23 @@ __global__ __launch_bounds__(kBlock) void k_bzero32(int64_t m, float* __restrict
         x[i] = 0.f;
 }
 
+// Grid-barrier timing (mi_time_grid_barrier): one element per thread updated
+// between barriers, cooperative grid.sync() against kernel boundaries.
+__global__ __launch_bounds__(kBlock) void k_gbar_sync(int iterations, float* __restrict__ buf)
+{
+    cooperative_groups::grid_group g = cooperative_groups::this_grid();
+    const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+    for (int it = 0; it < iterations; ++it) {
+        buf[i] += 1.f;
+        g.sync();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_gbar_one(float* __restrict__ buf)
+{
+    buf[size_t(blockIdx.x) * kBlock + threadIdx.x] += 1.f;
+}
+
 } // namespace mi
 
 using namespace mi;
@@ -1108,8 +1177,10 @@ struct mi_batch
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
     int E = 0, Wmax = 1;
-    int G = 8;  // workgroups of the batched SpMM (autotuned)
-    int Gv = 8; // workgroups of the vector, norm and true-residual stages (from n)
+    int G = 8;    // workgroups of the batched SpMM and ILU sweeps (autotuned)
+    int Vmax = 4; // cap on slots per thread of those launches (autotuned)
+    int band = 0; // banded row walk for those launches (autotuned)
+    int Gv = 8;   // workgroups of the vector, norm and true-residual stages (autotuned)
     int64_t n = 0, nnz = 0;
     bool sell = false;   // values in the pattern's SELL-64 slot order
     int64_t vrows = 0;   // value lines: nnz, or the SELL-64 slots
@@ -1136,7 +1207,7 @@ struct mi_batch
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0, graph_kind = -1, graph_pc = -1, graph_set = -1, graph_W = 0;
     BFinArgs graph_f{}; // finalize arguments baked into the graph
-    double tune_us = 0.0;
+    double tune_us = 0.0, tune_vec_us = 0.0;
 
     ~mi_batch()
     {
@@ -1159,6 +1230,7 @@ struct mi_batch
         a.soff = sell ? pat->sell.soff.p : nullptr;
         a.eci = sell ? pat->sell.eci.p : nullptr;
         a.shared = shared ? 1 : 0;
+        a.band = band;
         if (shared) {
             a.v32 = v32h.p;
             a.invD = pc == MI_PC_ILU ? ipivh.p : pc == MI_PC_NONE ? ones.p : invDh.p;
@@ -1343,7 +1415,7 @@ int fin(mi_batch* B, int op, const BFinArgs& f, int G, Launcher& L)
 // z = M^-1 in for the iterating slots: the two sweeps of the ILU(0) apply.
 int ilu_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
 {
-    const BIluKernel k = bilu_kernel(B->sell, B->shared, a.W);
+    const BIluKernel k = bilu_kernel(B->sell, B->shared, a.W, B->Vmax);
     MI_TRY(L(k, B->G, kBlock, a, in, out, 0, B->n0));
     return L(k, B->G, kBlock, a, in, out, 1, B->n0);
 }
@@ -1352,7 +1424,7 @@ int ilu_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher
 // eight with ILU(0).
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W), B->G, kBlock, a));
+    MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_ALPHA, f, B->G, L));
     if (B->pc == MI_PC_ILU) {
         MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPD_X)));
@@ -1374,25 +1446,27 @@ int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher&
     MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_PN : B_BI_P)));
     if (ilu)
         MI_TRY(ilu_apply(B, a, a.p, a.y, L));
-    MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, B->shared, a.W), B->G, kBlock, a));
+    MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_ALPHA, f, B->G, L));
     MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_SN : B_BI_S)));
     MI_TRY(fin(B, B_BI_S, f, B->Gv, L));
     if (ilu)
         MI_TRY(ilu_apply(B, a, a.r, a.z, L));
-    MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, B->shared, a.W), B->G, kBlock, a));
+    MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_OMEGA, f, B->G, L));
     MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_UPDATE)));
     return fin(B, B_BI_UPDATE, f, B->Gv, L);
 }
 
-// Autotuner: workgroups of the batched launches, timed on this pattern in
-// the widest layout with every equation's slot active.
+// Autotuner, timed on this pattern in the widest layout with every
+// equation's slot active: the SpMM's tile shape (slots per thread V, which
+// sets the rows of a workgroup pass, 256 V / W) and its workgroup count
+// (rows per workgroup), then the vector stages' workgroup count.
 int autotune(mi_batch* B)
 {
     B->G = 8;
-    // vector stages: one workgroup per 256 rows, up to kBatchMaxGrid (the
-    // SpMM's best grid can be far smaller and would leave them short of waves)
+    B->Vmax = 4;
+    // vector stages: one workgroup per 256 rows, up to kBatchMaxGrid, before tuning
     B->Gv = int(std::min<int64_t>(kBatchMaxGrid, std::max<int64_t>(8, (B->n + 2047) / 2048 * 8)));
     if (B->n == 0)
         return MI_OK;
@@ -1403,32 +1477,70 @@ int autotune(mi_batch* B)
     hipEvent_t e0, e1;
     MI_HIP(hipEventCreate(&e0));
     MI_HIP(hipEventCreate(&e1));
+    auto time_us = [&](auto&& launch) -> double {
+        if (launch() != MI_OK)
+            return -1.0;
+        (void)hipEventRecord(e0, B->stream);
+        const int reps = 3;
+        for (int r = 0; r < reps; ++r)
+            if (launch() != MI_OK)
+                return -1.0;
+        (void)hipEventRecord(e1, B->stream);
+        if (hipEventSynchronize(e1) != hipSuccess)
+            return -1.0;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return 1000.0 * ms / reps;
+    };
     double best = 1e300;
     BArgs a = B->args(B->cur);
     a.st = nullptr;
-    for (int g : {256, 512, 1024, 2048, 4096}) {
-        if (g > kBatchMaxGrid)
-            break;
+    const int W = a.W;
+    int last_v = 0;
+    for (int vmax : {4, 2, 1}) {
+        const int v = slots_per_thread(W, vmax);
+        if (v == last_v)
+            continue; // the same kernel as the previous cap
+        last_v = v;
+        auto k = bspmm_kernel<B_ALPHA>(B->sell, B->shared, W, vmax);
+        for (int band : {0, 1})
+            for (int g : {256, 512, 1024, 2048, 4096}) {
+                if (g > kBatchMaxGrid)
+                    break;
+                a.band = band;
+                Launcher L{B->stream};
+                const double us = time_us([&] { return L(k, unsigned(g), kBlock, a); });
+                if (us < 0.0)
+                    return set_error(MI_ERR_DEVICE, "batch autotuner: SpMM launch failed");
+                if (us < best) {
+                    best = us;
+                    B->G = g;
+                    B->Vmax = vmax;
+                    B->band = band;
+                }
+            }
+    }
+    B->tune_us = best;
+    // vector stages (the CG update, the largest): state zeroed at creation, so
+    // alpha = 0 and every slot is live
+    BArgs av = B->args(B->cur);
+    double bestv = 1e300;
+    const int gdef = B->Gv;
+    for (int g : {gdef, 1024, 2048, 4096}) {
+        if (g > kBatchMaxGrid || (g != gdef && int64_t(g) * 256 > B->n * 8))
+            continue; // at least 32 rows per workgroup
         Launcher L{B->stream};
-        auto k = bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W);
-        MI_TRY(L(k, unsigned(g), kBlock, a));
-        MI_HIP(hipEventRecord(e0, B->stream));
-        const int reps = 3;
-        for (int r = 0; r < reps; ++r)
-            MI_TRY(L(k, unsigned(g), kBlock, a));
-        MI_HIP(hipEventRecord(e1, B->stream));
-        MI_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        MI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const double us = 1000.0 * ms / reps;
-        if (us < best) {
-            best = us;
-            B->G = g;
+        const double us = time_us([&] { return L(k_bvec, unsigned(g), kBlock, av, int(B_UPDATE)); });
+        if (us < 0.0)
+            return set_error(MI_ERR_DEVICE, "batch autotuner: vector launch failed");
+        if (us < bestv) {
+            bestv = us;
+            B->Gv = g;
         }
     }
+    B->tune_vec_us = bestv;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    B->tune_us = best;
     B->slots_valid = false; // packed from values not yet set
     return MI_OK;
 }
@@ -1821,10 +1933,9 @@ MI_API int mi_batch_create_ex(const mi_csr* A, int n_eq, int flags, mi_batch** o
     MI_HIP(hipStreamSynchronize(B->stream));
     MI_TRY(alloc_slots(B.get(), B->set[0], false));
     MI_TRY(alloc_slots(B.get(), B->set[1], false));
-    for (BSlots& S : B->set) {
-        MI_HIP(hipMemsetAsync(S.p.p, 0, S.p.n * sizeof(float), B->stream));
-        MI_HIP(hipMemsetAsync(S.Ap.p, 0, S.Ap.n * sizeof(float), B->stream));
-    }
+    for (BSlots& S : B->set)
+        for (DevBuf<float>* v : {&S.p, &S.Ap, &S.xs, &S.r})
+            MI_HIP(hipMemsetAsync(v->p, 0, v->n * sizeof(float), B->stream));
     MI_HIP(hipStreamSynchronize(B->stream));
     MI_TRY(autotune(B.get()));
     *out = B.release();
@@ -1991,6 +2102,89 @@ MI_API int mi_batch_tuning(const mi_batch* B, int* rows_per_block, double* spmm_
     if (spmm_bytes) // FP32 values per matrix, shared columns and row pointers, p once, Ap once
         *spmm_bytes = double(B->nnz) * (4.0 * B->Em + 4.0) + 4.0 * double(B->n + 1) +
                       2.0 * 4.0 * double(B->n) * B->E;
+    return MI_OK;
+}
+
+MI_API int mi_batch_tuning_shape(const mi_batch* B, int* slots_per_thread_max,
+                                 int* spmm_workgroups, int* banded, int* vector_workgroups,
+                                 double* vector_us)
+{
+    if (!B)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    if (banded)
+        *banded = B->band;
+    if (slots_per_thread_max)
+        *slots_per_thread_max = B->Vmax;
+    if (spmm_workgroups)
+        *spmm_workgroups = B->G;
+    if (vector_workgroups)
+        *vector_workgroups = B->Gv;
+    if (vector_us)
+        *vector_us = B->tune_vec_us;
+    return MI_OK;
+}
+
+MI_API int mi_time_grid_barrier(int workgroups, int iterations, double* sync_us,
+                                double* launch_us)
+{
+    clear_error();
+    if (workgroups < 1 || iterations < 1 || iterations > 100000)
+        return set_error(MI_ERR_BAD_ARG, "bad argument");
+    MI_TRY(ensure_device());
+    int dev = 0, ncu = 0, per = 0, coop = 0;
+    MI_HIP(hipGetDevice(&dev));
+    MI_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    MI_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    MI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_gbar_sync, kBlock, 0));
+    if (!coop)
+        return set_error(MI_ERR_UNSUPPORTED, "device has no cooperative launch");
+    if (int64_t(workgroups) > int64_t(per) * ncu)
+        return set_error(MI_ERR_BAD_ARG, "%d workgroups exceed the %d x %d that can be resident",
+                         workgroups, per, ncu);
+    hipStream_t s = nullptr;
+    MI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    DevBuf<float> buf;
+    int rc = buf.alloc(size_t(workgroups) * kBlock);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float ms_sync = 0.f, ms_launch = 0.f;
+    auto run = [&]() -> int {
+        MI_TRY(rc);
+        MI_HIP(hipMemsetAsync(buf.p, 0, buf.n * sizeof(float), s));
+        MI_HIP(hipEventCreate(&e0));
+        MI_HIP(hipEventCreate(&e1));
+        int it = iterations;
+        float* p = buf.p;
+        void* args[] = {&it, &p};
+        MI_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_gbar_sync),
+                                          dim3(workgroups), dim3(kBlock), args, 0, s));
+        MI_HIP(hipEventRecord(e0, s));
+        MI_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_gbar_sync),
+                                          dim3(workgroups), dim3(kBlock), args, 0, s));
+        MI_HIP(hipEventRecord(e1, s));
+        MI_HIP(hipEventSynchronize(e1));
+        MI_HIP(hipEventElapsedTime(&ms_sync, e0, e1));
+        k_gbar_one<<<workgroups, kBlock, 0, s>>>(p);
+        MI_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < iterations; ++i)
+            k_gbar_one<<<workgroups, kBlock, 0, s>>>(p);
+        MI_HIP(hipEventRecord(e1, s));
+        MI_HIP(hipEventSynchronize(e1));
+        MI_HIP(hipGetLastError());
+        MI_HIP(hipEventElapsedTime(&ms_launch, e0, e1));
+        return MI_OK;
+    };
+    rc = run();
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    MI_TRY(rc);
+    if (sync_us)
+        *sync_us = 1000.0 * ms_sync / iterations;
+    if (launch_us)
+        *launch_us = 1000.0 * ms_launch / iterations;
     return MI_OK;
 }
 

@@ -105,7 +105,28 @@ def test_batch_tuning_is_reported(sys8):
     w, dm, A, rp, ci, vals = sys8
     B = capi.Batch(A, 32)
     t = B.tuning()
+    print(f"tuning: {t}")
     assert t["rows_per_block"] > 0 and t["spmm_us"] > 0 and t["spmm_bytes"] > 0
+    assert t["slots_per_thread_max"] in (1, 2, 4)
+    assert t["spmm_workgroups"] in (256, 512, 1024, 2048, 4096)
+    assert 8 <= t["vector_workgroups"] <= 4096 and t["vector_us"] > 0
+    # whatever shapes were picked, an odd width (one slot per thread) solves too
+    B3 = capi.Batch(A, 3)
+    B3.set_values(np.stack(vals[:3]))
+    x, its, rel, rc = B3.solve(np.stack(w.bs[:3]), tolerance=1e-8, max_iterations=2000)
+    assert rc == capi.MI_OK and np.all(rel <= 1e-8)
+
+
+def test_grid_barrier_timing(gpu):
+    """mi_time_grid_barrier reports a cooperative grid.sync() and a kernel
+    boundary; a grid larger than can be resident is refused."""
+    sync_us, launch_us = capi.time_grid_barrier(256, 50)
+    print(f"256 workgroups: grid.sync {sync_us:.2f} us, kernel boundary {launch_us:.2f} us")
+    assert sync_us > 0 and launch_us > 0
+    lib = capi.load()
+    a, b = capi.C.c_double(), capi.C.c_double()
+    assert lib.mi_time_grid_barrier(1 << 24, 10, capi.C.byref(a), capi.C.byref(b)) == capi.MI_ERR_BAD_ARG
+    assert lib.mi_time_grid_barrier(0, 10, capi.C.byref(a), capi.C.byref(b)) == capi.MI_ERR_BAD_ARG
 
 
 def test_batch_matches_single_equation_solver(sys8):
