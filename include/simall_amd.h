@@ -332,8 +332,9 @@ MI_API int mi_batch_tuning(const mi_batch* b, int* rows_per_block, double* spmm_
  * SpMM and ILU(0) sweeps (4, 2 or 1; a launch takes the widest of those that
  * divides its width), their workgroup count and row walk (banded: 1, the
  * workgroups of each XCD sweeping its eighth of the rows together; 0, one
- * contiguous range per workgroup), the vector stages' workgroup count and the
- * time of one vector stage (the CG update, all equations). */
+ * contiguous range per workgroup); and the vector stages' workgroup count
+ * (one per 256 rows, at most 2048) with the time of one vector stage (the CG
+ * update, all equations). */
 MI_API int mi_batch_tuning_shape(const mi_batch* b, int* slots_per_thread_max,
                                  int* spmm_workgroups, int* banded, int* vector_workgroups,
                                  double* vector_us);

@@ -136,38 +136,6 @@ __device__ __forceinline__ bool slot_on(const BArgs& a, int op, int e)
     return !q.done;
 }
 
-// Threads of a workgroup as P = kBlock / W rows of W slots: thread t takes
-// slot t % W of row t / W (threads from P W on are idle).
-struct SlotMap
-{
-    int W, P, sl, rr;
-    bool live;
-    __device__ __forceinline__ explicit SlotMap(int w)
-        : W(w), P(kBlock / w), sl(int(threadIdx.x) % w), rr(int(threadIdx.x) / w),
-          live(int(threadIdx.x) < (kBlock / w) * w)
-    {
-    }
-};
-
-// FP64 sums of up to two values per slot over a workgroup, in a fixed
-// order (rows 0 .. P-1 of each slot); partials[v][s][block].
-__device__ __forceinline__ void slot_partials(const BArgs& a, const SlotMap& m, int nv, double d0,
-                                              double d1)
-{
-    __shared__ double lds[2][kBlock];
-    lds[0][threadIdx.x] = d0;
-    lds[1][threadIdx.x] = d1;
-    __syncthreads();
-    const int G = gridDim.x;
-    if (int(threadIdx.x) < nv * m.W) {
-        const int v = threadIdx.x / m.W, sl = threadIdx.x % m.W;
-        double s = 0.0;
-        for (int q = 0; q < m.P; ++q)
-            s += lds[v][q * m.W + sl];
-        a.partials[(size_t(v) * m.W + sl) * G + blockIdx.x] = s;
-    }
-}
-
 // Contiguous range [beg, end) of `count` items for this workgroup, the
 // ranges of one XCD adjacent (xcd_logical_block), `align` dividing every
 // boundary.
@@ -221,17 +189,19 @@ template <int V> struct SlotVec
 {
     using type = float __attribute__((ext_vector_type(V)));
     static __device__ __forceinline__ float at(const type& x, int u) { return x[u]; }
+    static __device__ __forceinline__ void set(type& x, int u, float v) { x[u] = v; }
 };
 template <> struct SlotVec<1>
 {
     using type = float;
     static __device__ __forceinline__ float at(float x, int) { return x; }
+    static __device__ __forceinline__ void set(float& x, int, float v) { x = v; }
 };
 
 // FP64 sums of up to two values per slot over a workgroup whose threads
 // each hold V consecutive slots of one row group (TW = W / V threads per
 // row, R = kBlock / TW row groups), in a fixed order (row groups 0 .. R-1
-// of each slot); partials[v][s][block].  V = 1 is the SlotMap arrangement.
+// of each slot); partials[v][s][block].
 template <int V>
 __device__ __forceinline__ void slot_partials_v(const BArgs& a, int R, int grp, int s0, bool live,
                                                 int nv, const double (&d0)[V],
@@ -425,118 +395,155 @@ template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, bool shared, int W, in
 //   B_PZ        p = z + beta p
 //   B_BI_PN     p = r + beta (p - omega v)
 //   B_BI_SN     s = r - alpha v (in r); s.s
+// Threads take V consecutive slots of a row (V-wide loads and stores, like
+// the SpMM): W / V threads per row, R = kBlock V / W rows per pass; a slot of
+// an equation not iterating keeps its values.
+template <int V>
 __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
 {
-    const SlotMap m(a.W);
-    const int W = m.W;
-    const int e = m.live ? a.eq[m.sl] : -1;
-    const bool on = slot_on(a, op, e);
-    int64_t i0, i1;
-    block_range(a.n, m.P, i0, i1);
-    // this thread's elements: rows i0 + rr, i0 + rr + P, ... of its slot
-    const int64_t q0 = (i0 + m.rr) * W + m.sl, q1 = i1 * W;
-    const int64_t step = int64_t(m.P) * W;
-    const int64_t ib = i0 + m.rr;
+    using VT = typename SlotVec<V>::type;
+    const int W = a.W;
+    const int TW = W / V;
+    const int R = kBlock / TW;
+    const int t = int(threadIdx.x);
+    const bool live = t < R * TW;
+    const int grp = t / TW;
+    const int s0 = (t % TW) * V;
+    bool on[V], keep[V];
+    bool any = false, all = true, allk = true;
+    VT al = VT(0.f), be = VT(0.f), om = VT(0.f);
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+        const int e = live ? a.eq[s0 + u] : -1;
+        on[u] = slot_on(a, op, e);
+        keep[u] = on[u];
+        if (on[u]) {
+            const BEq& st = a.st[e];
+            SlotVec<V>::set(al, u, float(st.alpha));
+            SlotVec<V>::set(be, u, float(st.beta));
+            SlotVec<V>::set(om, u, float(st.omega));
+            if (op == B_BI_UPDATE && st.sconv)
+                keep[u] = false; // s met the tolerance: only x moves (:387-390)
+        }
+        any = any || on[u];
+        all = all && on[u];
+        allk = allk && keep[u];
+    }
+    int64_t r0, r1;
+    block_range(a.n, R, r0, r1);
+    double d0[V], d1[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+        d0[u] = d1[u] = 0.0;
     const bool sh = a.shared != 0;
-    auto dinv = [&](int64_t q, int64_t i) { return sh ? a.invD[i] : a.invD[q]; };
-    double s0 = 0.0, s1 = 0.0;
-    if (on) {
-        const BEq& st = a.st[e];
-        if (op == B_INIT) {
-            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
-                const float ri = a.r[q], zi = dinv(q, i) * ri;
-                a.p[q] = zi;
-                s0 += double(ri) * double(zi);
-                s1 += double(ri) * double(ri);
-            }
-        } else if (op == B_UPDATE) {
-            const float al = float(st.alpha);
-            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
-                a.xs[q] += al * a.p[q];
-                const float ri = a.r[q] - al * a.Ap[q];
-                a.r[q] = ri;
-                const float zi = dinv(q, i) * ri;
-                s0 += double(ri) * double(ri);
-                s1 += double(ri) * double(zi);
-            }
-        } else if (op == B_P) {
-            const float be = float(st.beta);
-            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P)
-                a.p[q] = dinv(q, i) * a.r[q] + be * a.p[q];
-        } else if (op == B_UPD_X) {
-            const float al = float(st.alpha);
-            for (int64_t q = q0; q < q1; q += step) {
-                a.xs[q] += al * a.p[q];
-                a.r[q] = a.r[q] - al * a.Ap[q];
-            }
-        } else if (op == B_DOT_RZ) {
-            for (int64_t q = q0; q < q1; q += step) {
-                const float ri = a.r[q], zi = a.z[q];
-                s0 += double(ri) * double(ri);
-                s1 += double(ri) * double(zi);
-            }
-        } else if (op == B_INIT_Z) {
-            for (int64_t q = q0; q < q1; q += step) {
-                const float ri = a.r[q], zi = a.z[q];
-                a.p[q] = zi;
-                s0 += double(ri) * double(zi);
-                s1 += double(ri) * double(ri);
-            }
-        } else if (op == B_PZ) {
-            const float be = float(st.beta);
-            for (int64_t q = q0; q < q1; q += step)
-                a.p[q] = a.z[q] + be * a.p[q];
-        } else if (op == B_BI_INIT) {
-            // rh = r; p = v = 0 (BiCGSTABSolver::solve, :366-372)
-            for (int64_t q = q0; q < q1; q += step) {
-                const float ri = a.r[q];
-                a.rh[q] = ri;
-                a.p[q] = 0.f;
-                a.Ap[q] = 0.f;
-                s0 += double(ri) * double(ri);
-            }
-            s1 = s0;
-        } else if (op == B_BI_P || op == B_BI_PN) {
-            // (:380-382)
-            const float be = float(st.beta), om = float(st.omega);
-            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
-                const float pi = a.r[q] + be * (a.p[q] - om * a.Ap[q]);
-                a.p[q] = pi;
+    if (any) {
+        for (int64_t i = r0 + grp; i < r1; i += R) {
+            const size_t q = size_t(i) * W + s0;
+            auto ld = [&](const float* p) { return *reinterpret_cast<const VT*>(p + q); };
+            auto put = [&](float* p, const VT& v, const bool (&m)[V], bool every) {
+                if (every) {
+                    *reinterpret_cast<VT*>(p + q) = v;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < V; ++u)
+                        if (m[u])
+                            p[q + u] = SlotVec<V>::at(v, u);
+                }
+            };
+            auto dinv = [&]() { return sh ? VT(a.invD[i]) : ld(a.invD); };
+            auto acc = [&](double (&d)[V], const VT& x, const VT& y, const bool (&m)[V]) {
+#pragma unroll
+                for (int u = 0; u < V; ++u)
+                    if (m[u])
+                        d[u] += double(SlotVec<V>::at(x, u)) * double(SlotVec<V>::at(y, u));
+            };
+            if (op == B_INIT) {
+                const VT r = ld(a.r), z = dinv() * r;
+                put(a.p, z, on, all);
+                acc(d0, r, z, on);
+                acc(d1, r, r, on);
+            } else if (op == B_UPDATE) {
+                put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
+                const VT r = ld(a.r) - al * ld(a.Ap);
+                put(a.r, r, on, all);
+                const VT z = dinv() * r;
+                acc(d0, r, r, on);
+                acc(d1, r, z, on);
+            } else if (op == B_P) {
+                put(a.p, dinv() * ld(a.r) + be * ld(a.p), on, all);
+            } else if (op == B_UPD_X) {
+                put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
+                put(a.r, ld(a.r) - al * ld(a.Ap), on, all);
+            } else if (op == B_DOT_RZ) {
+                const VT r = ld(a.r), z = ld(a.z);
+                acc(d0, r, r, on);
+                acc(d1, r, z, on);
+            } else if (op == B_INIT_Z) {
+                const VT r = ld(a.r), z = ld(a.z);
+                put(a.p, z, on, all);
+                acc(d0, r, z, on);
+                acc(d1, r, r, on);
+            } else if (op == B_PZ) {
+                put(a.p, ld(a.z) + be * ld(a.p), on, all);
+            } else if (op == B_BI_INIT) {
+                // rh = r; p = v = 0 (BiCGSTABSolver::solve, :366-372)
+                const VT r = ld(a.r);
+                put(a.rh, r, on, all);
+                put(a.p, VT(0.f), on, all);
+                put(a.Ap, VT(0.f), on, all);
+                acc(d0, r, r, on);
+                acc(d1, r, r, on);
+            } else if (op == B_BI_P || op == B_BI_PN) {
+                // (:380-382)
+                const VT pv = ld(a.r) + be * (ld(a.p) - om * ld(a.Ap));
+                put(a.p, pv, on, all);
                 if (op == B_BI_P)
-                    a.y[q] = dinv(q, i) * pi;
-            }
-        } else if (op == B_BI_S || op == B_BI_SN) {
-            // (:384-386)
-            const float al = float(st.alpha);
-            for (int64_t q = q0, i = ib; q < q1; q += step, i += m.P) {
-                const float si = a.r[q] - al * a.Ap[q];
-                a.r[q] = si;
+                    put(a.y, dinv() * pv, on, all);
+            } else if (op == B_BI_S || op == B_BI_SN) {
+                // (:384-386)
+                const VT sv = ld(a.r) - al * ld(a.Ap);
+                put(a.r, sv, on, all);
                 if (op == B_BI_S)
-                    a.z[q] = dinv(q, i) * si;
-                s0 += double(si) * double(si);
-            }
-        } else { // B_BI_UPDATE (:387-397)
-            const float al = float(st.alpha);
-            if (st.sconv) {
-                // s met the tolerance: x += alpha y and stop (:387-390)
-                for (int64_t q = q0; q < q1; q += step)
-                    a.xs[q] += al * a.y[q];
-            } else {
-                const float om = float(st.omega);
-                for (int64_t q = q0; q < q1; q += step) {
-                    a.xs[q] += al * a.y[q] + om * a.z[q];
-                    const float ri = a.r[q] - om * a.t[q];
-                    a.r[q] = ri;
-                    s0 += double(ri) * double(ri);
-                    s1 += double(a.rh[q]) * double(ri);
+                    put(a.z, dinv() * sv, on, all);
+                acc(d0, sv, sv, on);
+            } else { // B_BI_UPDATE (:387-397)
+                const VT y = ld(a.y);
+                VT xs = ld(a.xs) + al * y;
+                if (!allk) {
+                    // slots whose s met the tolerance take x += alpha y alone
+                    const VT z = ld(a.z);
+#pragma unroll
+                    for (int u = 0; u < V; ++u)
+                        if (keep[u])
+                            SlotVec<V>::set(xs, u, SlotVec<V>::at(xs, u) +
+                                                       SlotVec<V>::at(om, u) * SlotVec<V>::at(z, u));
+                    put(a.xs, xs, on, all);
+                } else {
+                    put(a.xs, xs + om * ld(a.z), on, all);
+                }
+                bool anyk = false;
+#pragma unroll
+                for (int u = 0; u < V; ++u)
+                    anyk = anyk || keep[u];
+                if (anyk) {
+                    const VT r = ld(a.r) - om * ld(a.t);
+                    put(a.r, r, keep, allk);
+                    acc(d0, r, r, keep);
+                    acc(d1, ld(a.rh), r, keep);
                 }
             }
         }
     }
     if (op == B_P || op == B_BI_P || op == B_UPD_X || op == B_PZ || op == B_BI_PN)
         return;
-    slot_partials(a, m, (op == B_BI_S || op == B_BI_SN) ? 1 : 2, s0, s1);
+    slot_partials_v<V>(a, R, grp, s0, live, (op == B_BI_S || op == B_BI_SN) ? 1 : 2, d0, d1);
 }
+
+// One slot per thread: on the 32-equation 1M-row batch V = 4 ran the CG
+// update in the same time (188.8 vs 187.6 us at full width) and the whole
+// solve 3-4 % slower (519-525 vs 506 ms).
+using BVecKernel = void (*)(BArgs, int);
+BVecKernel bvec_kernel(int) { return k_bvec<1>; }
 
 // ILU(0) apply out = (LU)^-1 in under the two-colour ordering: rows [0, n0)
 // have no lower entries (their neighbours all come later), rows [n0, n) no
@@ -1180,7 +1187,7 @@ struct mi_batch
     int G = 8;    // workgroups of the batched SpMM and ILU sweeps (autotuned)
     int Vmax = 4; // cap on slots per thread of those launches (autotuned)
     int band = 0; // banded row walk for those launches (autotuned)
-    int Gv = 8;   // workgroups of the vector, norm and true-residual stages (autotuned)
+    int Gv = 8;   // workgroups of the vector, norm and true-residual stages (from n)
     int64_t n = 0, nnz = 0;
     bool sell = false;   // values in the pattern's SELL-64 slot order
     int64_t vrows = 0;   // value lines: nnz, or the SELL-64 slots
@@ -1427,15 +1434,15 @@ int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
     MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_ALPHA, f, B->G, L));
     if (B->pc == MI_PC_ILU) {
-        MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPD_X)));
+        MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_UPD_X)));
         MI_TRY(ilu_apply(B, a, a.r, a.z, L));
-        MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_DOT_RZ)));
+        MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_DOT_RZ)));
         MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
-        return L(k_bvec, B->Gv, kBlock, a, int(B_PZ));
+        return L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_PZ));
     }
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_UPDATE)));
+    MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_UPDATE)));
     MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
-    return L(k_bvec, B->Gv, kBlock, a, int(B_P));
+    return L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_P));
 }
 
 // One BiCGStab iteration for every iterating equation: nine launches with
@@ -1443,25 +1450,26 @@ int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
     const bool ilu = B->pc == MI_PC_ILU;
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_PN : B_BI_P)));
+    MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(ilu ? B_BI_PN : B_BI_P)));
     if (ilu)
         MI_TRY(ilu_apply(B, a, a.p, a.y, L));
     MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_ALPHA, f, B->G, L));
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(ilu ? B_BI_SN : B_BI_S)));
+    MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(ilu ? B_BI_SN : B_BI_S)));
     MI_TRY(fin(B, B_BI_S, f, B->Gv, L));
     if (ilu)
         MI_TRY(ilu_apply(B, a, a.r, a.z, L));
     MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_OMEGA, f, B->G, L));
-    MI_TRY(L(k_bvec, B->Gv, kBlock, a, int(B_BI_UPDATE)));
+    MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_BI_UPDATE)));
     return fin(B, B_BI_UPDATE, f, B->Gv, L);
 }
 
 // Autotuner, timed on this pattern in the widest layout with every
 // equation's slot active: the SpMM's tile shape (slots per thread V, which
-// sets the rows of a workgroup pass, 256 V / W) and its workgroup count
-// (rows per workgroup), then the vector stages' workgroup count.
+// sets the rows of a workgroup pass, 256 V / W), its row walk (banded or
+// contiguous) and its workgroup count (rows per workgroup); the ILU(0)
+// sweeps use the same shape.
 int autotune(mi_batch* B)
 {
     B->G = 8;
@@ -1521,24 +1529,18 @@ int autotune(mi_batch* B)
             }
     }
     B->tune_us = best;
-    // vector stages (the CG update, the largest): state zeroed at creation, so
-    // alpha = 0 and every slot is live
-    BArgs av = B->args(B->cur);
-    double bestv = 1e300;
-    const int gdef = B->Gv;
-    for (int g : {gdef, 1024, 2048, 4096}) {
-        if (g > kBatchMaxGrid || (g != gdef && int64_t(g) * 256 > B->n * 8))
-            continue; // at least 32 rows per workgroup
+    // vector stages: one workgroup per 256 rows up to 2048 (whole 32-equation
+    // solves on 1M rows: 2048 workgroups 506 ms, 3912 518 ms; single launches
+    // of the CG update time alike on either grid, so they are not tuned)
+    B->Gv = std::min(B->Gv, 2048);
+    {
+        BArgs av = B->args(B->cur);
         Launcher L{B->stream};
-        const double us = time_us([&] { return L(k_bvec, unsigned(g), kBlock, av, int(B_UPDATE)); });
-        if (us < 0.0)
+        B->tune_vec_us = time_us([&] { return L(bvec_kernel(av.W), unsigned(B->Gv), kBlock, av,
+                                                int(B_UPDATE)); });
+        if (B->tune_vec_us < 0.0)
             return set_error(MI_ERR_DEVICE, "batch autotuner: vector launch failed");
-        if (us < bestv) {
-            bestv = us;
-            B->Gv = g;
-        }
     }
-    B->tune_vec_us = bestv;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     B->slots_valid = false; // packed from values not yet set
@@ -1671,14 +1673,14 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             f.W = S.W;
             const BArgs a0 = B->args(B->cur);
             if (bicg) {
-                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_BI_INIT)));
+                MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_BI_INIT)));
                 MI_TRY(fin(B, B_BI_INIT, f, B->Gv, L));
             } else if (pc == MI_PC_ILU) {
                 MI_TRY(ilu_apply(B, a0, a0.r, a0.z, L));
-                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_INIT_Z)));
+                MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_INIT_Z)));
                 MI_TRY(fin(B, B_INIT, f, B->Gv, L));
             } else {
-                MI_TRY(L(k_bvec, B->Gv, kBlock, a0, int(B_INIT)));
+                MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_INIT)));
                 MI_TRY(fin(B, B_INIT, f, B->Gv, L));
             }
         }
