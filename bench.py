@@ -813,6 +813,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--strength", type=float, default=-1.0,
                     help="AMG aggregation strength threshold (negative: the solver default)")
+    ap.add_argument("--truncation", type=float, default=-1.0,
+                    help="AMG prolongator truncation (negative: the solver default)")
     ap.add_argument("--caller-order", action="store_true",
                     help="also time SpMV on the caller's (randomly numbered) matrix")
     ap.add_argument("--no-flow", action="store_true", help="skip the SIMPLE / PISO iteration section")
@@ -912,6 +914,8 @@ def main():
             kw["amg_coarse_sweeps"] = args.coarse_degree
         if args.strength >= 0:
             kw["amg_strength"] = args.strength
+        if args.truncation >= 0:
+            kw["amg_truncation"] = args.truncation
         if args.inner_tol:
             kw["inner_tolerance"] = args.inner_tol
         s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=pc, tolerance=1e-8,

@@ -204,6 +204,10 @@ typedef struct {
                                 setup for every new set of values, as
                                 AMGPreconditioner::setup (src/solver/AMGPreconditioner.cpp:248-326)
                                 at every solve (src/solver/LinearSolvers.cpp:422) */
+    double amg_truncation;   /* >= 0: each row of a smoothed prolongator drops the entries
+                                below amg_truncation x its largest |entry| and scales the rest
+                                to keep the row sum (default 0.1); 0 keeps every entry.  A
+                                numeric re-setup keeps the set of entries of the full setup. */
 } mi_solver_cfg;
 
 /* GpuSolverStats (src/gpu/CgGpu.hpp:66-73) plus refinement counters. */

@@ -53,6 +53,7 @@ class SolverCfg(C.Structure):
         ("keep_preconditioner", C.c_int),
         ("use_graph", C.c_int),
         ("amg_resetup", C.c_int),
+        ("amg_truncation", C.c_double),
     ]
 
 

@@ -991,7 +991,7 @@ struct AmgPrecond final : Precond
             MI_TRY(tmp.alloc(nm->max_nnz()));
             num = std::move(nm);
         }
-        MI_TRY(num->run(A->val.p, params.omega_scale, s));
+        MI_TRY(num->run(A->val.p, params.omega_scale, params.truncation > 0.0, s));
         const int nc = host.coarse_n;
         MI_TRY(gj.run(nc, num->dense.p, cinv.p, s));
         std::vector<LevelValues> v(host.levels.size());
@@ -1113,6 +1113,8 @@ std::unique_ptr<Precond> make_amg(const mi_solver_cfg& c)
         p->params.theta = c.amg_strength;
     if (c.amg_jacobi_omega > 0)
         p->params.omega_scale = c.amg_jacobi_omega;
+    if (c.amg_truncation >= 0)
+        p->params.truncation = c.amg_truncation;
     p->degree = c.amg_sweeps > 0 ? c.amg_sweeps : 2;
     p->coarse_degree = c.amg_coarse_sweeps;
     return p;

@@ -121,7 +121,9 @@ struct AmgNumeric
     // rp0/ci0: the finest operator's pattern on the device (H's level 0).
     int record(const AmgHierarchyHost& H, const int* rp0, const int* ci0, hipStream_t s);
     // a0: the finest operator's new FP64 values (CSR order of H's level 0).
-    int run(const double* a0, double omega_scale, hipStream_t s);
+    // truncated: P's pattern came from a truncated full setup, whose rows
+    // are scaled to the row sums of the complete smoothed rows
+    int run(const double* a0, double omega_scale, bool truncated, hipStream_t s);
     // Values of level l for DHierarchy::refresh.
     LevelValues values(int l) const;
     // Copies the recomputed values into H (same patterns).

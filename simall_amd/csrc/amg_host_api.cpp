@@ -79,6 +79,8 @@ MI_API int mi_amg_host_build(int64_t n, const int32_t* rp, const int32_t* ci, co
             p.theta = cfg->amg_strength;
         if (cfg->amg_jacobi_omega > 0)
             p.omega_scale = cfg->amg_jacobi_omega;
+        if (cfg->amg_truncation >= 0)
+            p.truncation = cfg->amg_truncation;
     }
     auto H = std::make_unique<mi_amg_host>();
     if (amg_build(A, p, H->h) != 0)

@@ -158,10 +158,13 @@ __global__ __launch_bounds__(kBlock) void k_num_diag(int n, const int* __restric
         if (ci[k] == i)
             d += v[k];
     double f = d;
-    if (strong)
+    if (strong) {
         for (int k = rp[i]; k < rp[i + 1]; ++k)
             if (ci[k] != i && !strong[k])
                 f += v[k];
+        if (fabs(f) <= kAmgFilterCancel * fabs(d))
+            f = 0.0; // the couplings, all weak, cancel the diagonal
+    }
     dF[i] = f;
     s[i] = d != 0.0 ? 1.0 / sqrt(fabs(d)) : 1.0;
 }
@@ -228,13 +231,17 @@ __global__ __launch_bounds__(kBlock) void k_num_power_fin(const double* __restri
 // (1 or a wavefront), one lane per entry of P's row (column c) at a time:
 // the entry starts at 1 when c is the row's own aggregate, then takes -w dF
 // (diagonal) or -w a_ij (strong coupling) for every entry of A's row whose
-// column lies in aggregate c, in A's row order, w = omega / dF.
+// column lies in aggregate c, in A's row order, w = omega / dF.  truncated
+// (W = 1): the pattern holds the entries a truncated setup kept, and the row
+// is scaled by its complete sum (the same additions, in the same order, as
+// the host setup's) over the sum of the kept entries (in column order).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_num_prolongator(
     int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
     const unsigned char* __restrict__ strong, const int* __restrict__ agg,
     const double* __restrict__ dF, const double* __restrict__ lmax, double omega_scale,
-    const int* __restrict__ prp, const int* __restrict__ pci, double* __restrict__ pv)
+    const int* __restrict__ prp, const int* __restrict__ pci, double* __restrict__ pv,
+    int truncated)
 {
 #pragma clang fp contract(off)
     const int lane = int(threadIdx.x) % W;
@@ -260,6 +267,24 @@ __global__ __launch_bounds__(kBlock) void k_num_prolongator(
                 acc += -w * v[k];
         }
         pv[q] = acc;
+    }
+    if (W == 1 && truncated) {
+        double full = own >= 0 ? 1.0 : 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            const int j = ci[k];
+            if (agg[j] < 0)
+                continue;
+            if (j == i)
+                full += -w * df;
+            else if (strong[k])
+                full += -w * v[k];
+        }
+        double kept = 0.0;
+        for (int q = prp[i]; q < prp[i + 1]; ++q)
+            kept += pv[q];
+        const double scale = kept != 0.0 ? full / kept : 1.0;
+        for (int q = prp[i]; q < prp[i + 1]; ++q)
+            pv[q] = pv[q] * scale;
     }
 }
 
@@ -558,7 +583,7 @@ int AmgNumeric::record(const AmgHierarchyHost& H, const int* rp0, const int* ci0
 }
 
 // ------------------------------------------------------------------ run
-int AmgNumeric::run(const double* a0, double omega_scale, hipStream_t s)
+int AmgNumeric::run(const double* a0, double omega_scale, bool truncated, hipStream_t s)
 {
     const int nl = int(L.size());
     if (nl == 0)
@@ -597,14 +622,14 @@ int AmgNumeric::run(const double* a0, double omega_scale, hipStream_t s)
             break;
         NumLevel& nx = L[size_t(l) + 1];
         const unsigned gw = grid_for(d.n, kBlock / kWave); // a wavefront per row
-        if (d.wave_rows)
+        if (d.wave_rows && !truncated)
             k_num_prolongator<kWave><<<gw, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
                                                            d.agg.p, d.dF.p, sc + 2, omega_scale,
-                                                           d.prp.p, d.pci.p, d.pv.p);
+                                                           d.prp.p, d.pci.p, d.pv.p, 0);
         else
             k_num_prolongator<1><<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
                                                       d.agg.p, d.dF.p, sc + 2, omega_scale,
-                                                      d.prp.p, d.pci.p, d.pv.p);
+                                                      d.prp.p, d.pci.p, d.pv.p, truncated ? 1 : 0);
         if (d.r_nnz > 0)
             k_num_gather<<<reduce_grid(d.r_nnz, 4), kBlock, 0, s>>>(d.r_nnz, d.rmap.p, d.pv.p,
                                                                    d.rv.p);

@@ -242,7 +242,8 @@ double amg_lambda_max(const HCsr& A, int iters)
 }
 
 void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int nagg, double omega,
-                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci)
+                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci,
+                              double truncation)
 {
     // With a strength pattern, the smoothing uses the filtered operator A_F:
     // weak off-diagonal entries are dropped and added to the diagonal (row
@@ -262,6 +263,7 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
     std::vector<char> strong(static_cast<size_t>(n), 0);
     for (int i = rb; i < re; ++i) {
         cols.clear();
+        double full = 0.0; // the complete row's sum: its additions in order
         auto add = [&](int c, double v) {
             if (!used[size_t(c)]) {
                 used[size_t(c)] = 1;
@@ -269,6 +271,7 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
                 cols.push_back(c);
             }
             acc[size_t(c)] += v;
+            full += v;
         };
         double dF = d[size_t(i)];
         if (srp) {
@@ -279,6 +282,8 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
                 if (j != i && !strong[size_t(j)])
                     dF += A.v[size_t(k)];
             }
+            if (std::fabs(dF) <= kAmgFilterCancel * std::fabs(d[size_t(i)]))
+                dF = 0.0;
         }
         if (agg[size_t(i)] >= 0)
             add(agg[size_t(i)], 1.0);
@@ -297,12 +302,27 @@ void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int na
             for (int s = (*srp)[size_t(i)]; s < (*srp)[size_t(i) + 1]; ++s)
                 strong[size_t((*sci)[size_t(s)])] = 0;
         std::sort(cols.begin(), cols.end());
+        // truncation: entries below truncation x the row's largest magnitude
+        // are dropped and the kept ones scaled to the complete row's sum
+        double cut = 0.0, scale = 1.0;
+        if (truncation > 0.0) {
+            double rmax = 0.0;
+            for (int c : cols)
+                rmax = std::max(rmax, std::fabs(acc[size_t(c)]));
+            cut = truncation * rmax;
+            double kept = 0.0;
+            for (int c : cols)
+                if (acc[size_t(c)] != 0.0 && std::fabs(acc[size_t(c)]) >= cut)
+                    kept += acc[size_t(c)];
+            scale = kept != 0.0 ? full / kept : 1.0;
+        }
         const size_t before = out.ci.size();
         for (int c : cols) {
             used[size_t(c)] = 0;
-            if (acc[size_t(c)] != 0.0) {
+            const double v = acc[size_t(c)];
+            if (v != 0.0 && (truncation <= 0.0 || std::fabs(v) >= cut)) {
                 out.ci.push_back(c);
-                out.v.push_back(acc[size_t(c)]);
+                out.v.push_back(truncation > 0.0 ? v * scale : v);
             }
         }
         out.len.push_back(int(out.ci.size() - before));
@@ -576,9 +596,9 @@ int amg_build(const HCsr& A0, const AmgParams& p, AmgHierarchyHost& out)
         L.n_aggregates = nagg;
         const double omega = p.omega_scale / L.lmax;
         if (p.filter)
-            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, &srp, &sci);
+            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, &srp, &sci, p.truncation);
         else
-            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, nullptr, nullptr);
+            amg_smoothed_prolongator(A, agg, nagg, omega, L.P, nullptr, nullptr, p.truncation);
         HCsr AP, Ac;
         {
             HCsr R0;

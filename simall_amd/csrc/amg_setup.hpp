@@ -29,6 +29,7 @@ struct AmgParams
     double omega_scale = 4.0 / 3.0; // prolongator smoothing weight * lambda_max(D^-1 A)
     int threads = 0;            // 0: hardware concurrency (at most 16)
     bool filter = true;         // smooth P with the filtered operator (weak entries lumped)
+    double truncation = 0.0;    // drop P entries below this x the row's largest |entry|
     // Invert the coarsest operator on the host (coarse_inverse); otherwise
     // only coarse_dense is filled and the caller inverts it (the device
     // hierarchy does, with the same elimination: dense_inverse_device).
@@ -67,7 +68,12 @@ void amg_strength(const HCsr& A, double theta, std::vector<int>& srp, std::vecto
 int amg_aggregate(int n, const std::vector<int>& srp, const std::vector<int>& sci,
                   const HCsr& A, std::vector<int>& agg);
 void amg_smoothed_prolongator(const HCsr& A, const std::vector<int>& agg, int nagg, double omega,
-                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci);
+                              HCsr& P, const std::vector<int>* srp, const std::vector<int>* sci,
+                              double truncation = 0.0);
+// A filtered diagonal within this fraction of |a_ii| of zero is zero: the
+// row's couplings are all weak and cancel its diagonal (a zero row sum), and
+// the smoothing leaves the row's tentative entry alone.
+constexpr double kAmgFilterCancel = 1e-10;
 void csr_transpose(const HCsr& A, HCsr& T);
 void csr_multiply(const HCsr& A, const HCsr& B, HCsr& C, int threads);
 double amg_lambda_max(const HCsr& A, int iters);

@@ -986,6 +986,7 @@ MI_API void mi_solver_cfg_default(mi_solver_cfg* c)
     c->amg_coarse_size = 2048;
     c->amg_strength = 0.02;
     c->amg_jacobi_omega = 4.0 / 3.0;
+    c->amg_truncation = 0.1;
     c->amg_smoother = 0;
     c->amg_sweeps = 2;
     c->amg_coarse_sweeps = 1;

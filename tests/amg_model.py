@@ -85,12 +85,16 @@ def pcg(A, b, M, tol=1e-8, maxit=1000):
     return x, maxit
 
 
-def smoothed_prolongator(A, agg, nagg, strong, lmax, omega_scale=4.0 / 3.0):
+def smoothed_prolongator(A, agg, nagg, strong, lmax, omega_scale=4.0 / 3.0, pattern=None):
     """P = (I - omega D_F^-1 A_F) T for fixed aggregates and strong couplings
     (every entry, no pattern restriction): T[i, agg[i]] = 1 for aggregated
     rows; A_F keeps A's strong off-diagonal entries and has the filtered
-    diagonal D_F = diag(A) + the row's weak off-diagonal entries;
-    omega = omega_scale / lmax.  `strong` is per entry of A in CSR order."""
+    diagonal D_F = diag(A) + the row's weak off-diagonal entries, taken as 0
+    within 1e-10 |a_ii| of 0 (its couplings, all weak, cancel the diagonal:
+    the row is not smoothed); omega = omega_scale / lmax.  `strong` is per
+    entry of A in CSR order.  With `pattern` (the prolongator of a truncated
+    setup), the rows are restricted to its entries and scaled to the
+    complete rows' sums."""
     import scipy.sparse as sp
     A = A.tocsr()
     n = A.shape[0]
@@ -100,13 +104,23 @@ def smoothed_prolongator(A, agg, nagg, strong, lmax, omega_scale=4.0 / 3.0):
     d = np.bincount(rows[diag], weights=A.data[diag], minlength=n)
     weak = (~diag) & (np.asarray(strong) == 0)
     dF = d + np.bincount(rows[weak], weights=A.data[weak], minlength=n)
+    dF = np.where(np.abs(dF) <= 1e-10 * np.abs(d), 0.0, dF)
     keep = (~diag) & (np.asarray(strong) != 0)
     AF = sp.csr_matrix((A.data[keep], (rows[keep], cols[keep])), shape=(n, n)) + sp.diags(dF)
     has = agg >= 0
     T = sp.csr_matrix((np.ones(has.sum()), (np.arange(n)[has], agg[has])), shape=(n, nagg))
     omega = omega_scale / lmax
     w = np.where(dF != 0, omega / np.where(dF != 0, dF, 1.0), 0.0)
-    return (T - sp.diags(w) @ (AF @ T)).tocsr()
+    P = (T - sp.diags(w) @ (AF @ T)).tocsr()
+    if pattern is None:
+        return P
+    full = np.asarray(P.sum(axis=1)).ravel()
+    pattern = pattern.tocsr()
+    kept = restrict_to_pattern(P, pattern)
+    Pk = sp.csr_matrix((kept, pattern.indices, pattern.indptr), shape=P.shape)
+    ks = np.asarray(Pk.sum(axis=1)).ravel()
+    scale = np.where(ks != 0, full / np.where(ks != 0, ks, 1.0), 1.0)
+    return (sp.diags(scale) @ Pk).tocsr()
 
 
 def restrict_to_pattern(M, pattern):

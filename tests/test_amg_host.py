@@ -162,6 +162,7 @@ def test_numeric_phase_model_reproduces_the_host_setup():
     A, b = system(16)
     cfg = capi.default_cfg()
     cfg.amg_coarse_size = 64
+    cfg.amg_truncation = 0.0  # complete rows (test_truncated_prolongator: truncated)
     H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=cfg)
     assert H.levels >= 3
     for l in range(H.levels - 1):
@@ -179,6 +180,65 @@ def test_numeric_phase_model_reproduces_the_host_setup():
         # the host dropped only exact zeros
         outside = Pm - sp.csr_matrix((vals, P.indices, P.indptr), shape=P.shape)
         assert abs(outside).max() <= 1e-15 * abs(P).max()
+        Ac = H.scipy(l + 1, H.A)
+        ref = restrict_to_pattern(P.T @ Al @ P, Ac)
+        assert np.allclose(ref, Ac.data, rtol=1e-12, atol=1e-13 * abs(Ac.data).max())
+
+
+def test_rows_with_only_weak_couplings_are_not_smoothed():
+    """A row whose off-diagonal couplings are all weak has a filtered diagonal
+    d_F = a_ii + sum of them = its row sum, zero on the Neumann rows of the
+    pressure operator: D_F^-1 A_F is undefined there and the row keeps its
+    tentative entry 1 (not 1 - omega from a rounding residue of d_F)."""
+    A, b = system(32)
+    H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=capi.default_cfg())
+    Al, P = H.scipy(0, H.A), H.scipy(0, H.P)
+    agg, _ = H.aggregates(0)
+    strong = H.strong(0)
+    rows = np.repeat(np.arange(Al.shape[0]), np.diff(Al.indptr))
+    n_strong = np.bincount(rows, weights=strong.astype(float), minlength=Al.shape[0])
+    rsum = np.asarray(Al.sum(axis=1)).ravel()
+    isolated = np.flatnonzero((n_strong == 0) & (np.abs(rsum) <= 1e-12 * Al.diagonal()))
+    assert isolated.size > 0
+    for i in isolated:
+        cols = P.indices[P.indptr[i]:P.indptr[i + 1]]
+        assert list(cols) == [agg[i]] and P.data[P.indptr[i]] == 1.0, i
+
+
+def test_truncated_prolongator():
+    """amg_truncation = 0.1: every row of every prolongator keeps exactly the
+    entries of the complete smoothed row whose magnitude reaches 0.1 x the
+    row's largest, scaled so that the row sum is the complete row's; the
+    Galerkin operators are R A P of the truncated prolongators; the numpy
+    statement restricted to the kept entries reproduces them."""
+    from amg_model import smoothed_prolongator, restrict_to_pattern
+    A, b = system(16)
+    cfg = capi.default_cfg()
+    cfg.amg_coarse_size = 64
+    cfg.amg_truncation = 0.0
+    full = capi.AmgHost(A.indptr, A.indices, A.data, cfg=cfg)
+    cfg.amg_truncation = 0.1
+    H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=cfg)
+    assert H.levels >= 3
+    assert H.scipy(0, H.P).nnz < full.scipy(0, full.P).nnz
+    assert H.scipy(1, H.A).nnz < full.scipy(1, full.A).nnz
+    for l in range(H.levels - 1):
+        Al, P = H.scipy(l, H.A), H.scipy(l, H.P)
+        agg, nagg = H.aggregates(l)
+        lmax = H.info(l, H.A)[3]
+        Pc = smoothed_prolongator(Al, agg, nagg, H.strong(l), lmax)  # complete rows
+        Pc.eliminate_zeros()
+        rmax = np.maximum.reduceat(np.abs(Pc.data), Pc.indptr[:-1])
+        rows = np.repeat(np.arange(Pc.shape[0]), np.diff(Pc.indptr))
+        keep = np.abs(Pc.data) >= 0.1 * rmax[rows]
+        K = sp.csr_matrix((Pc.data * keep, Pc.indices.copy(), Pc.indptr.copy()), shape=Pc.shape)
+        K.eliminate_zeros()
+        K.sort_indices()
+        assert np.array_equal(K.indptr, P.indptr) and np.array_equal(K.indices, P.indices), l
+        assert np.allclose(np.asarray(P.sum(axis=1)).ravel(), np.asarray(Pc.sum(axis=1)).ravel(),
+                           rtol=1e-12, atol=1e-14)
+        Pm = smoothed_prolongator(Al, agg, nagg, H.strong(l), lmax, pattern=P)
+        assert np.allclose(restrict_to_pattern(Pm, P), P.data, rtol=1e-12, atol=1e-15)
         Ac = H.scipy(l + 1, H.A)
         ref = restrict_to_pattern(P.T @ Al @ P, Ac)
         assert np.allclose(ref, Ac.data, rtol=1e-12, atol=1e-13 * abs(Ac.data).max())

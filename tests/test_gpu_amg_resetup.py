@@ -8,7 +8,9 @@ against the numpy model of that numeric phase (tests/amg_model.py, itself
 pinned to the host build in test_amg_host.py):
   - patterns unchanged, level-0 values = the new matrix;
   - P_l = (I - omega D_F^-1 A_F) T on P's pattern, rtol 1e-12 (the model's
-    sparse products sum in another order);
+    sparse products sum in another order); with the default truncation the
+    rows of that pattern scaled to the complete rows' sums (the model with
+    pattern=), without it (amg_truncation = 0) the complete rows;
   - R_l = P_l^T exactly; A_{l+1} = R_l A_l P_l, rtol 1e-12;
   - lambda_max by the same power iteration, rtol 1e-10 (dot products summed
     in another order);
@@ -83,7 +85,7 @@ def test_full_setup_coarse_inverse_is_the_host_inverse(gpu, case):
     x, st, _ = s.solve(b)
     assert st.converged
     Hd = capi.AmgHost.from_solver(s)
-    cfg = capi.default_cfg()
+    cfg = capi.default_cfg()  # the solver's truncation default included
     cfg.amg_coarse_size = 200
     Hh = capi.AmgHost(A1.indptr, A1.indices, A1.data, cfg=cfg)
     assert Hd.levels == Hh.levels >= 3
@@ -94,10 +96,12 @@ def test_full_setup_coarse_inverse_is_the_host_inverse(gpu, case):
     assert np.array_equal(inv_d, inv_h)
 
 
-@pytest.mark.parametrize("precision", [capi.MI_PRECISION_FP64, capi.MI_PRECISION_MIXED])
-def test_resetup_matches_the_numeric_model(gpu, case, precision):
+@pytest.mark.parametrize("precision,truncation", [(capi.MI_PRECISION_FP64, 0.1),
+                                                  (capi.MI_PRECISION_MIXED, 0.1),
+                                                  (capi.MI_PRECISION_FP64, 0.0)])
+def test_resetup_matches_the_numeric_model(gpu, case, precision, truncation):
     A1, A2, b = case
-    s = capi.Solver(**dict(CFG, precision=precision))
+    s = capi.Solver(**dict(CFG, precision=precision, amg_truncation=truncation))
     M = capi.Csr(A1.indptr, A1.indices, A1.data)
     s.setup(M)
     x1, st1, _ = s.solve(b)
@@ -118,7 +122,8 @@ def test_resetup_matches_the_numeric_model(gpu, case, precision):
         lmax = H2.info(l, H2.A)[3]
         lam = lambda_max_estimate(A_l)
         assert abs(lam * 1.05 - lmax) <= 1e-10 * lmax, (l, lam * 1.05, lmax)
-        Pm = restrict_to_pattern(smoothed_prolongator(A_l, agg, nagg, strong, lmax), P)
+        Pm = restrict_to_pattern(smoothed_prolongator(A_l, agg, nagg, strong, lmax,
+                                                      pattern=P if truncation > 0 else None), P)
         err_p = abs(Pm - P.data).max() / abs(P.data).max()
         assert err_p <= 1e-12, (l, err_p)
         assert abs(P.T - R).max() == 0.0
