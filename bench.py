@@ -339,13 +339,13 @@ def cpu_leg(capi, wl, lib, A, b, x_mixed, d_b, d_x, sample_iters, core, host):
     return cpu, errors
 
 
-def measure_traffic(mode="--spmv-only", kernel="k_sell_spmv<double>", grid=None):
+def measure_traffic(mode="--spmv-only", kernel="k_sell_spmv<double>", grid=None, last=None):
     """HBM bytes per launch of `kernel` from two rocprofv3 counter passes
     (FETCH_SIZE, then WRITE_SIZE) over `bench.py <mode>`, each scaled by what
     the same counter reads for the plain copy kernel of known size that run
-    also launches; `grid` (work-items) keeps only launches of that size.
-    Runs the passes as child processes before this process touches the GPU;
-    returns (bytes or None, details)."""
+    also launches; `grid` (work-items) keeps only launches of that size,
+    `last` only the last `last` of them.  Runs the passes as child processes
+    before this process touches the GPU; returns (bytes or None, details)."""
     import csv
     import glob
     import shutil
@@ -378,18 +378,22 @@ def measure_traffic(mode="--spmv-only", kernel="k_sell_spmv<double>", grid=None)
                     if row.get("Counter_Name") != counter:
                         continue
                     if kernel in name and (grid is None or int(row["Grid_Size"]) == grid):
+                        order = row.get("Dispatch_Id") or row.get("Correlation_Id") or 0
                         launches.setdefault((name, row["Grid_Size"]), []).append(
-                            float(row["Counter_Value"]))
+                            (int(order), float(row["Counter_Value"])))
                     elif "k_axpby" in name:
                         sums["copy"].append(float(row["Counter_Value"]))
         if launches:
             # the shape launched most often: with grid=None, the autotuner's
             # pick (timed like every candidate, then used by the solve)
             key = max(launches, key=lambda k: len(launches[k]))
+            vals = [v for _, v in sorted(launches[key])]
+            if last:
+                vals = vals[-last:]
             # launches that found every equation converged read almost
             # nothing: keep those within half of the largest reading
-            top = max(launches[key])
-            sums["spmv"] = [v for v in launches[key] if v >= 0.5 * top]
+            top = max(vals)
+            sums["spmv"] = [v for v in vals if v >= 0.5 * top]
             out.setdefault("picked", {})[counter] = {"kernel": key[0], "grid": int(key[1])}
         if not sums["spmv"] or not sums["copy"]:
             return None, {"source": f"no {counter} rows for the SpMV or the copy kernel"}
@@ -865,7 +869,9 @@ def main():
     traffic, traffic_detail = None, {"source": "not measured (--no-traffic)"}
     batch_traffic = None
     if not args.spmv_only and not args.no_traffic and not args.no_batch:
-        batch_traffic = measure_traffic("--batch-spmm-only", BSPMM_KERNEL, BSPMM_GRID)
+        # the solve's three iterations (the autotuner times the same kernel
+        # at the same size with the other row walk before them)
+        batch_traffic = measure_traffic("--batch-spmm-only", BSPMM_KERNEL, BSPMM_GRID, last=3)
         log(f"batched SpMM traffic: {batch_traffic}")
     if not args.spmv_only and not args.no_traffic:
         # child processes, before this process initialises the GPU
