@@ -231,10 +231,10 @@ __global__ __launch_bounds__(kBlock) void k_num_power_fin(const double* __restri
 // (1 or a wavefront), one lane per entry of P's row (column c) at a time:
 // the entry starts at 1 when c is the row's own aggregate, then takes -w dF
 // (diagonal) or -w a_ij (strong coupling) for every entry of A's row whose
-// column lies in aggregate c, in A's row order, w = omega / dF.  truncated
-// (W = 1): the pattern holds the entries a truncated setup kept, and the row
-// is scaled by its complete sum (the same additions, in the same order, as
-// the host setup's) over the sum of the kept entries (in column order).
+// column lies in aggregate c, in A's row order, w = omega / dF.  truncated:
+// the pattern holds the entries a truncated setup kept, and the row is
+// scaled by its complete sum (the same additions, in the same order, as the
+// host setup's) over the sum of the kept entries (in column order).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_num_prolongator(
     int n, const int* __restrict__ rp, const int* __restrict__ ci, const double* __restrict__ v,
@@ -268,22 +268,32 @@ __global__ __launch_bounds__(kBlock) void k_num_prolongator(
         }
         pv[q] = acc;
     }
-    if (W == 1 && truncated) {
-        double full = own >= 0 ? 1.0 : 0.0;
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            const int j = ci[k];
-            if (agg[j] < 0)
-                continue;
-            if (j == i)
-                full += -w * df;
-            else if (strong[k])
-                full += -w * v[k];
+    if (truncated) {
+        // one lane sums the row (W > 1: the row's lanes are one wavefront,
+        // whose stores the fence makes visible to that lane) and every lane
+        // scales the entries it computed
+        if (W > 1)
+            __threadfence_block();
+        double scale = 1.0;
+        if (lane == 0) {
+            double full = own >= 0 ? 1.0 : 0.0;
+            for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                const int j = ci[k];
+                if (agg[j] < 0)
+                    continue;
+                if (j == i)
+                    full += -w * df;
+                else if (strong[k])
+                    full += -w * v[k];
+            }
+            double kept = 0.0;
+            for (int q = prp[i]; q < prp[i + 1]; ++q)
+                kept += pv[q];
+            scale = kept != 0.0 ? full / kept : 1.0;
         }
-        double kept = 0.0;
-        for (int q = prp[i]; q < prp[i + 1]; ++q)
-            kept += pv[q];
-        const double scale = kept != 0.0 ? full / kept : 1.0;
-        for (int q = prp[i]; q < prp[i + 1]; ++q)
+        if (W > 1)
+            scale = __shfl(scale, 0, W);
+        for (int q = prp[i] + lane; q < prp[i + 1]; q += W)
             pv[q] = pv[q] * scale;
     }
 }
@@ -622,10 +632,11 @@ int AmgNumeric::run(const double* a0, double omega_scale, bool truncated, hipStr
             break;
         NumLevel& nx = L[size_t(l) + 1];
         const unsigned gw = grid_for(d.n, kBlock / kWave); // a wavefront per row
-        if (d.wave_rows && !truncated)
+        if (d.wave_rows)
             k_num_prolongator<kWave><<<gw, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
                                                            d.agg.p, d.dF.p, sc + 2, omega_scale,
-                                                           d.prp.p, d.pci.p, d.pv.p, 0);
+                                                           d.prp.p, d.pci.p, d.pv.p,
+                                                           truncated ? 1 : 0);
         else
             k_num_prolongator<1><<<g, kBlock, 0, s>>>(d.n, d.arp, d.aci, d.av.p, d.strong.p,
                                                       d.agg.p, d.dF.p, sc + 2, omega_scale,
