@@ -278,14 +278,18 @@ MI_API int mi_solver_destroy(mi_solver* s);
  * (src/solver/SimpleAlgorithm.cpp:529-532, solve_momentum_component
  * :492-507); one mi_batch_solve replaces those n_eq calls.  Per equation the
  * iteration is PCG from x = 0 (CGSolver::solve,
- * src/solver/LinearSolvers.cpp:420-452), or preconditioned BiCGStab
- * (BiCGSTABSolver::solve, :360-404) for nonsymmetric (convective) equations,
- * with Jacobi, ILU(0) (ILU0Preconditioner, :78-151) or no preconditioner, in
- * FP32 with FP64 dot products, norms and iterate, refined on the FP64 true
- * residual until ||b_e - A_e x_e|| <= tolerance * ||b_e||; equations converge
- * and drop out independently.  cfg (nullable: defaults) supplies kind = CG or
- * BiCGStab, preconditioner = Jacobi, ILU or None, tolerance, max_iterations
- * (per equation, all sweeps), inner_tolerance, max_outer and check_interval.
+ * src/solver/LinearSolvers.cpp:420-452), preconditioned BiCGStab
+ * (BiCGSTABSolver::solve, :360-404) or left-preconditioned GMRES(m)
+ * (GMRESmSolver::solve, :266-340; one cycle of at most m <= 32 Arnoldi steps
+ * per refinement sweep, classical Gram-Schmidt twice) for nonsymmetric
+ * (convective) equations, with Jacobi, ILU(0) (ILU0Preconditioner, :78-151) or
+ * no preconditioner, in FP32 with FP64 dot products, norms and iterate,
+ * refined on the FP64 true residual until ||b_e - A_e x_e|| <= tolerance *
+ * ||b_e|| (for GMRES too, where the reference tests the preconditioned
+ * residual); equations converge and drop out independently.  cfg (nullable:
+ * defaults) supplies kind = CG, BiCGStab or GMRES (restart), preconditioner =
+ * Jacobi, ILU or None, tolerance, max_iterations (per equation, all sweeps;
+ * Arnoldi steps for GMRES), inner_tolerance, max_outer and check_interval.
  * Vectors are equation-major: b[e * n + i].  The rows per workgroup of the
  * batched SpMV are chosen by timing candidates at creation (mi_batch_tuning).
  *
@@ -550,10 +554,11 @@ typedef struct {
     mi_solver_cfg momentum; /* momentum solves: GMRES(30) + ILU(0), 1e-8 (LinearSolverConfig) */
     mi_solver_cfg pressure; /* pressure-correction solves: CG + SA-AMG, 1e-8 */
     int momentum_batch;  /* 1: the three momentum components (one matrix, three right-hand
-                            sides: SimpleAlgorithm.cpp:529-532) in one mi_batch solve with
-                            momentum.tolerance / max_iterations / inner_tolerance / max_outer,
-                            BiCGStab with ILU(0) on one shared matrix; 0 (default): three
-                            solves with `momentum` */
+                            sides: SimpleAlgorithm.cpp:529-532) in one mi_batch solve on one
+                            shared matrix, BiCGStab with ILU(0) (whatever momentum.kind: the
+                            fastest batched kind here) and momentum.tolerance / max_iterations /
+                            inner_tolerance / max_outer; 0 (default): three solves with
+                            `momentum` */
 } mi_flow_cfg;
 /* SimpleResiduals (src/solver/SimpleAlgorithm.hpp:59-63). */
 typedef struct {

@@ -2,25 +2,28 @@
 //
 // SimAll solves every scalar transport equation (energy, k/omega, species)
 // with its own lin_.solve on an identical pattern
-// (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232).  Here the
-// E equations of one pattern are solved together, each stage of the
-// iteration one launch for all of them.
+// (src/solver/ScalarTransport.cpp:84-242, the solve at :230-232), and the
+// three momentum components one after another on one matrix
+// (src/solver/SimpleAlgorithm.cpp:529-532).  Here the E equations of one
+// pattern are solved together, each stage of the iteration one launch for all
+// of them.
 //
 // Layout.  The Krylov working set is slot-interleaved: the equations being
 // iterated occupy W slots (1 <= W <= 64), and every per-equation
 // quantity of a nonzero k or a row i is a line of W consecutive values:
 // v32[k * W + s], r[i * W + s], ...  A row of the batched SpMV is handled by
 // W / V threads, each taking V consecutive slots (V = 4, 2 or 1, the widest
-// dividing W) with V-wide loads: one load of colIdx[k] serves every
-// equation, the values of nonzero k are one contiguous W-wide line, and the
-// gather of x at column c is one contiguous line too.  The shared pattern is
-// streamed once per launch and no gather is scattered across equations.  The
-// SpMV runs on an autotuned grid; the vector stages, norms and true
-// residuals on one workgroup per 256 rows.  When the
-// pattern has a SELL-64 form (short rows: 7-point operators) the nonzeros
-// are stored in its slot order, m = soff[slice] + 64 k + row-in-slice, so the
-// loads of colIdx and of the value lines of the rows of a workgroup step are
-// contiguous for any W.
+// dividing W under the autotuned cap) with V-wide loads: one load of
+// colIdx[k] serves every equation, the values of nonzero k are one
+// contiguous W-wide line (one value for all when the equations share one
+// matrix), and the gather of x at column c is one contiguous line too.  The
+// shared pattern is streamed once per launch and no gather is scattered
+// across equations.  The SpMV and the ILU(0) sweeps run on an autotuned grid
+// and row walk; the vector stages, norms and true residuals on one workgroup
+// per 256 rows.  When the pattern has a SELL-64 form (short rows: 7-point
+// operators) the nonzeros are stored in its slot order,
+// m = soff[slice] + 64 k + row-in-slice, so the loads of colIdx and of the
+// value lines of the rows of a workgroup step are contiguous for any W.
 //
 // As equations converge their slots go idle; at the next poll the live ones
 // are repacked into a layout exactly as wide as their number (the other of
@@ -29,16 +32,17 @@
 // equation lives equation-major ("home"): FP64 and FP32 values
 // v[e * nnz + k], b, x and the FP64 true residual [e * n + i].
 //
-// Arithmetic: Jacobi-PCG per equation (CGSolver::solve,
-// src/solver/LinearSolvers.cpp:420-452), or for nonsymmetric equations
-// (convection) Jacobi-preconditioned BiCGStab (BiCGSTABSolver::solve,
-// :360-404), with FP32 values, vectors and correction and FP64 dot products
-// and norms; the FP64 true residual with the FP64 values refines each
-// equation until ||b_e - A_e x_e|| <= tol ||b_e|| (x_e accumulates each
-// sweep's correction in FP64).  Each equation has its own scalars and
-// convergence flag.  One inner CG iteration is five launches (batched SpMV
-// + p.Ap, finalize, update, finalize, direction), one BiCGStab iteration
-// nine; `chunk` iterations are replayed from a HIP graph between polls.
+// Arithmetic: PCG (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452),
+// BiCGStab (BiCGSTABSolver::solve, :360-404) or GMRES(m) (GMRESmSolver::solve,
+// :266-340, one cycle per sweep) per equation, with Jacobi, ILU(0) (factorized
+// on the device under a two-colour ordering) or no preconditioner, FP32
+// values, vectors and correction and FP64 dot products and norms; the FP64
+// true residual with the FP64 values refines each equation until
+// ||b_e - A_e x_e|| <= tol ||b_e|| (x_e accumulates each sweep's correction in
+// FP64).  Each equation has its own scalars and convergence flag.  One inner
+// CG iteration is five launches (eight with ILU(0)), one BiCGStab iteration
+// nine (twelve); `chunk` iterations, or a whole GMRES cycle, are replayed
+// from a HIP graph between polls.
 #include "csr.hpp"
 #include "krylov.hpp"
 #include "reorder.hpp"
@@ -64,7 +68,10 @@ struct BEq
 {
     double rz, alpha, beta, r0, bnorm, tol_in, rel, last;
     double rho, omega; // BiCGStab: rho of the previous iteration (rz holds the current), omega
-    int iter, total, outer, done, fin, status, sconv, pad;
+    double gscale, gbeta; // GMRES: 1 / norm of the newest basis vector; the cycle's start norm
+    int iter, total, outer, done, fin, status, sconv;
+    int gj, gend;         // GMRES: Arnoldi steps of the cycle; the cycle has ended
+    int pad;
 };
 
 // Stages.  B_BI_*: BiCGStab; B_BI_T skips equations whose s already met the
@@ -77,7 +84,7 @@ enum BOp : int {
     B_BNORM = 0, B_TRUE = 1, B_INIT = 2, B_ALPHA = 3, B_UPDATE = 4, B_P = 5,
     B_BI_INIT = 6, B_BI_P = 7, B_BI_ALPHA = 8, B_BI_S = 9, B_BI_T = 10, B_BI_OMEGA = 11,
     B_BI_UPDATE = 12, B_UPD_X = 13, B_DOT_RZ = 14, B_INIT_Z = 15, B_PZ = 16, B_BI_PN = 17,
-    B_BI_SN = 18
+    B_BI_SN = 18, B_GM = 19
 };
 
 // Slot-layout stages.
@@ -133,6 +140,8 @@ __device__ __forceinline__ bool slot_on(const BArgs& a, int op, int e)
     const BEq& q = a.st[e];
     if (op == B_BI_T)
         return !q.done && !q.sconv;
+    if (op == B_GM)
+        return !q.done && !q.gend;
     return !q.done;
 }
 
@@ -254,9 +263,9 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
         any = any || on[u];
         all = all && on[u];
     }
-    const float* vin = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
+    const float* vin = (OP == B_ALPHA || OP == B_GM) ? a.p : OP == B_BI_ALPHA ? a.y : a.z;
     float* vout = OP == B_BI_T ? a.t : a.Ap;
-    const float* vw = OP == B_ALPHA ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
+    const float* vw = (OP == B_ALPHA || OP == B_GM) ? a.p : OP == B_BI_ALPHA ? a.rh : a.r;
     const RowWalk rw = row_walk(0, a.n, R, a.band);
     double d0[V], d1[V];
 #pragma unroll
@@ -544,6 +553,353 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
 // solve 3-4 % slower (519-525 vs 506 ms).
 using BVecKernel = void (*)(BArgs, int);
 BVecKernel bvec_kernel(int) { return k_bvec<1>; }
+
+// ---------------------------------------------------------------- GMRES(m)
+// Left-preconditioned GMRES(m) for every equation of the slot layout: one
+// refinement sweep is one cycle, the Arnoldi steps of GMRESmSolver::solve
+// (src/solver/LinearSolvers.cpp:266-340) with classical Gram-Schmidt applied
+// twice (the single-equation solver's orthogonalisation), per-equation
+// Hessenberg matrix, Givens rotations and least-squares right-hand side in
+// FP64 on the device.  The basis is m+1 slot-layout FP32 vectors.
+constexpr int kBGmMax = 32; // longest batched cycle (registers of the dot kernel)
+
+struct BGm
+{
+    float* V;      // basis: vector k at V + k * vs
+    size_t vs;     // n * W
+    int m;
+    double* small; // per equation, `ss` doubles: H, cs, sn, sv, y, h1, h2
+    int ss;
+    double* part;  // [W][m + 1][G] partials of the basis kernels
+};
+__device__ __forceinline__ double* gm_H(const BGm& g, int e) { return g.small + size_t(e) * g.ss; }
+__device__ __forceinline__ double* gm_cs(const BGm& g, int e) { return gm_H(g, e) + (g.m + 1) * g.m; }
+__device__ __forceinline__ double* gm_sn(const BGm& g, int e) { return gm_cs(g, e) + g.m; }
+__device__ __forceinline__ double* gm_sv(const BGm& g, int e) { return gm_sn(g, e) + g.m; }
+__device__ __forceinline__ double* gm_y(const BGm& g, int e) { return gm_sv(g, e) + g.m + 1; }
+__device__ __forceinline__ double* gm_h(const BGm& g, int e, int which)
+{
+    return gm_y(g, e) + g.m + (which == 2 ? g.m + 1 : 0);
+}
+inline int gm_small_stride(int m) { return (m + 1) * m + 3 * m + 3 * (m + 1); }
+
+__device__ __forceinline__ bool gm_on(const BArgs& a, int e)
+{
+    return e >= 0 && !a.st[e].done && !a.st[e].gend;
+}
+
+// Per-slot FP64 sums over a workgroup of nv values per thread (one slot per
+// thread, rows as k_bvec<1>), stored to g.part[(slot (m+1) + k) G + block],
+// in chunks of 8 values through LDS.
+template <int NMAX>
+__device__ __forceinline__ void gm_store_partials(const BArgs& a, const BGm& g, int P, int nv,
+                                                  const double (&acc)[NMAX])
+{
+    __shared__ double lds[8][kBlock];
+    const int W = a.W;
+    const int G = gridDim.x;
+    for (int k0 = 0; k0 < nv; k0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < NMAX; ++k)
+                if (k == k0 + u)
+                    v = acc[k];
+            lds[u][threadIdx.x] = v;
+        }
+        __syncthreads();
+        const int nk = min(8, nv - k0);
+        for (int t = threadIdx.x; t < nk * W; t += kBlock) {
+            const int u = t / W, sl = t % W;
+            double sum = 0.0;
+            for (int q = 0; q < P; ++q)
+                sum += lds[u][q * W + sl];
+            g.part[(size_t(sl) * (g.m + 1) + k0 + u) * G + blockIdx.x] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// Partials of v_k . z for k < nv <= NMAX, z = v_zslot.
+template <int NMAX>
+__global__ __launch_bounds__(kBlock) void k_bgm_dots(BArgs a, BGm g, int nv, int zslot)
+{
+    const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
+    const bool live = t < P * W;
+    const int sl = t % W, rr = t / W;
+    const bool on = live && gm_on(a, a.eq[sl]);
+    int64_t i0, i1;
+    block_range(a.n, P, i0, i1);
+    double acc[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k)
+        acc[k] = 0.0;
+    if (on) {
+        const float* z = g.V + size_t(zslot) * g.vs;
+        for (int64_t i = i0 + rr; i < i1; i += P) {
+            const size_t q = size_t(i) * W + sl;
+            const double zi = double(z[q]);
+#pragma unroll
+            for (int k = 0; k < NMAX; ++k)
+                if (k < nv)
+                    acc[k] += double(g.V[size_t(k) * g.vs + q]) * zi;
+        }
+    }
+    gm_store_partials<NMAX>(a, g, P, nv, acc);
+}
+
+// z = v_zslot; z -= sum_k h_k v_k (k < nv <= NMAX, h = h1 or h2 of the slot's
+// equation, accumulated in FP64 in order k), stored back; then the partials
+// of v_k . z (norm == 0) or of z . z (norm == 1, one value).
+template <int NMAX>
+__global__ __launch_bounds__(kBlock) void k_bgm_orth(BArgs a, BGm g, int nv, int zslot, int which,
+                                                     int norm)
+{
+    const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
+    const bool live = t < P * W;
+    const int sl = t % W, rr = t / W;
+    const int e = live ? a.eq[sl] : -1;
+    const bool on = live && gm_on(a, e);
+    int64_t i0, i1;
+    block_range(a.n, P, i0, i1);
+    double acc[NMAX];
+#pragma unroll
+    for (int k = 0; k < NMAX; ++k)
+        acc[k] = 0.0;
+    if (on) {
+        // the slot's coefficients in registers, and each row's basis values
+        // loaded once for the update and the dots
+        double h[NMAX];
+        const double* hg = gm_h(g, e, which);
+#pragma unroll
+        for (int k = 0; k < NMAX; ++k)
+            h[k] = k < nv ? hg[k] : 0.0;
+        float* z = g.V + size_t(zslot) * g.vs;
+        for (int64_t i = i0 + rr; i < i1; i += P) {
+            const size_t q = size_t(i) * W + sl;
+            float vk[NMAX];
+#pragma unroll
+            for (int k = 0; k < NMAX; ++k)
+                vk[k] = k < nv ? g.V[size_t(k) * g.vs + q] : 0.f;
+            double zr = double(z[q]);
+#pragma unroll
+            for (int k = 0; k < NMAX; ++k)
+                if (k < nv)
+                    zr -= h[k] * double(vk[k]);
+            const float zt = float(zr);
+            z[q] = zt;
+            const double zd = double(zt);
+            if (norm) {
+                acc[0] += zd * zd;
+            } else {
+#pragma unroll
+                for (int k = 0; k < NMAX; ++k)
+                    if (k < nv)
+                        acc[k] += double(vk[k]) * zd;
+            }
+        }
+    }
+    gm_store_partials<NMAX>(a, g, P, norm ? 1 : nv, acc);
+}
+
+using BGmDots = void (*)(BArgs, BGm, int, int);
+using BGmOrth = void (*)(BArgs, BGm, int, int, int, int);
+// The smallest register footprint holding nv basis vectors.
+inline BGmDots bgm_dots(int nv)
+{
+    return nv <= 8 ? k_bgm_dots<8> : nv <= 16 ? k_bgm_dots<16> : k_bgm_dots<kBGmMax + 1>;
+}
+inline BGmOrth bgm_orth(int nv)
+{
+    return nv <= 8 ? k_bgm_orth<8> : nv <= 16 ? k_bgm_orth<16> : k_bgm_orth<kBGmMax + 1>;
+}
+
+// h_which[k] of each slot's equation = the fold of its partials; one block
+// per (slot, k < nv).
+__global__ __launch_bounds__(kBlock) void k_bgm_fold(BArgs a, BGm g, int nv, int which, int G)
+{
+    const int sl = int(blockIdx.x) / nv, k = int(blockIdx.x) % nv;
+    const int e = a.eq[sl];
+    if (!gm_on(a, e))
+        return;
+    double tot[1] = {strided_sum(g.part + (size_t(sl) * (g.m + 1) + k) * G, G)};
+    __shared__ double lds[kBlock / kWave];
+    block_sum<1>(tot, lds);
+    if (threadIdx.x == 0)
+        gm_h(g, e, which)[k] = tot[0];
+}
+
+// Cycle start, one block per slot: beta = ||v_0|| (v_0 = M^-1 r of the
+// sweep), s = (beta, 0, ...), scale 1 / beta; a zero or non-finite beta ends
+// the slot's cycle at once (no correction).
+__global__ __launch_bounds__(kBlock) void k_bgm_begin(BArgs a, BGm g, int G)
+{
+    const int sl = int(blockIdx.x);
+    const int e = a.eq[sl];
+    if (e < 0 || a.st[e].done)
+        return;
+    double tot[1] = {strided_sum(g.part + size_t(sl) * (g.m + 1) * G, G)};
+    __shared__ double lds[kBlock / kWave];
+    block_sum<1>(tot, lds);
+    if (threadIdx.x != 0)
+        return;
+    BEq& q = a.st[e];
+    const double beta = sqrt(tot[0]);
+    q.gj = 0;
+    q.gbeta = beta;
+    double* sv = gm_sv(g, e);
+    for (int i = 0; i <= g.m; ++i)
+        sv[i] = 0.0;
+    sv[0] = beta;
+    q.gend = (beta > 0.0 && isfinite(beta)) ? 0 : 1;
+    q.gscale = beta > 0.0 ? 1.0 / beta : 1.0;
+}
+
+// v_slot *= the slot's scale
+__global__ __launch_bounds__(kBlock) void k_bgm_scale(BArgs a, BGm g, int vslot)
+{
+    const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
+    if (t >= P * W)
+        return;
+    const int sl = t % W, rr = t / W;
+    const int e = a.eq[sl];
+    if (!gm_on(a, e))
+        return;
+    const double sc = a.st[e].gscale;
+    float* v = g.V + size_t(vslot) * g.vs;
+    int64_t i0, i1;
+    block_range(a.n, P, i0, i1);
+    for (int64_t i = i0 + rr; i < i1; i += P) {
+        const size_t q = size_t(i) * W + sl;
+        v[q] = float(double(v[q]) * sc);
+    }
+}
+
+// Arnoldi step j, one block per slot: Hessenberg column j = h1 + h2 and
+// ||z||, the Givens rotations and the residual estimate of GMRESmSolver::solve
+// (LinearSolvers.cpp:303-326); the cycle ends when the estimate relative to
+// the cycle's start norm meets the sweep's inner tolerance, at m steps, at
+// the iteration limit or on a singular column.
+__global__ __launch_bounds__(kBlock) void k_bgm_step(BArgs a, BGm g, int j, int G, int max_iter)
+{
+    const int sl = int(blockIdx.x);
+    const int e = a.eq[sl];
+    if (!gm_on(a, e))
+        return;
+    double tot[1] = {strided_sum(g.part + size_t(sl) * (g.m + 1) * G, G)};
+    __shared__ double lds[kBlock / kWave];
+    block_sum<1>(tot, lds);
+    if (threadIdx.x != 0)
+        return;
+    BEq& q = a.st[e];
+    double* Hj = gm_H(g, e) + size_t(j) * size_t(g.m + 1);
+    double* cs = gm_cs(g, e);
+    double* sn = gm_sn(g, e);
+    double* sv = gm_sv(g, e);
+    const double* h1 = gm_h(g, e, 1);
+    const double* h2 = gm_h(g, e, 2);
+    for (int i = 0; i <= j; ++i)
+        Hj[i] = h1[i] + h2[i];
+    const double hn = sqrt(tot[0]);
+    Hj[j + 1] = hn;
+    q.gscale = hn != 0.0 ? 1.0 / hn : 1.0;
+    for (int i = 0; i < j; ++i) {
+        const double tt = cs[i] * Hj[i] + sn[i] * Hj[i + 1];
+        Hj[i + 1] = -sn[i] * Hj[i] + cs[i] * Hj[i + 1];
+        Hj[i] = tt;
+    }
+    const double dd = hypot(Hj[j], Hj[j + 1]);
+    if (!(dd > 0.0)) {
+        q.gend = 1; // singular column: the cycle keeps the steps before it
+        return;
+    }
+    cs[j] = Hj[j] / dd;
+    sn[j] = Hj[j + 1] / dd;
+    Hj[j] = dd;
+    Hj[j + 1] = 0.0;
+    const double tt = cs[j] * sv[j];
+    sv[j + 1] = -sn[j] * sv[j];
+    sv[j] = tt;
+    q.gj = j + 1;
+    ++q.iter;
+    ++q.total;
+    q.last = fabs(sv[j + 1]) / q.gbeta;
+    if (q.last < q.tol_in || j + 1 == g.m || q.total >= max_iter)
+        q.gend = 1;
+}
+
+// Cycle end, one thread per slot: y = H^-1 s over the cycle's steps.
+__global__ void k_bgm_backsolve(BArgs a, BGm g)
+{
+    const int sl = int(threadIdx.x);
+    if (sl >= a.W)
+        return;
+    const int e = a.eq[sl];
+    if (e < 0 || a.st[e].done)
+        return;
+    const int jc = a.st[e].gj;
+    const double* H = gm_H(g, e);
+    const double* sv = gm_sv(g, e);
+    double* y = gm_y(g, e);
+    const size_t ldh = size_t(g.m + 1);
+    for (int i = jc - 1; i >= 0; --i) {
+        double sum = sv[i];
+        for (int k = i + 1; k < jc; ++k)
+            sum -= H[size_t(k) * ldh + i] * y[k];
+        y[i] = sum / H[size_t(i) * ldh + i];
+    }
+}
+
+// xs += sum_{k < gj} y_k v_k for every slot of the sweep
+__global__ __launch_bounds__(kBlock) void k_bgm_xupdate(BArgs a, BGm g)
+{
+    const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
+    if (t >= P * W)
+        return;
+    const int sl = t % W, rr = t / W;
+    const int e = a.eq[sl];
+    if (e < 0 || a.st[e].done)
+        return;
+    const int jc = a.st[e].gj;
+    if (jc == 0)
+        return;
+    double y[kBGmMax];
+    const double* yg = gm_y(g, e);
+#pragma unroll
+    for (int k = 0; k < kBGmMax; ++k)
+        y[k] = k < jc ? yg[k] : 0.0;
+    int64_t i0, i1;
+    block_range(a.n, P, i0, i1);
+    for (int64_t i = i0 + rr; i < i1; i += P) {
+        const size_t q = size_t(i) * W + sl;
+        double xv = double(a.xs[q]);
+#pragma unroll
+        for (int k = 0; k < kBGmMax; ++k)
+            if (k < jc)
+                xv += y[k] * double(g.V[size_t(k) * g.vs + q]);
+        a.xs[q] = float(xv);
+    }
+}
+
+// out = D^-1 in (Jacobi, or the identity with invD = 1) for the slots of a
+// running GMRES cycle.
+__global__ __launch_bounds__(kBlock) void k_bgm_jacobi(BArgs a, const float* __restrict__ in,
+                                                       float* __restrict__ out)
+{
+    const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
+    if (t >= P * W)
+        return;
+    const int sl = t % W, rr = t / W;
+    if (!gm_on(a, a.eq[sl]))
+        return;
+    int64_t i0, i1;
+    block_range(a.n, P, i0, i1);
+    for (int64_t i = i0 + rr; i < i1; i += P) {
+        const size_t q = size_t(i) * W + sl;
+        out[q] = (a.shared ? a.invD[i] : a.invD[q]) * in[q];
+    }
+}
 
 // ILU(0) apply out = (LU)^-1 in under the two-colour ordering: rows [0, n0)
 // have no lower entries (their neighbours all come later), rows [n0, n) no
@@ -873,6 +1229,8 @@ __device__ void b_finalize(BEq& s, int op, const double (&tot)[2], const BFinArg
         } else {
             s.done = 0;
             s.iter = 0;
+            s.gj = 0;
+            s.gend = 0; // a GMRES cycle starts with the sweep
             ++s.outer;
             s.tol_in = fmax(f.inner_tol, 0.5 * f.tol * s.bnorm / rn);
         }
@@ -1210,6 +1568,11 @@ struct mi_batch
     bool slots_valid = false; // set[cur] holds the home values of its equations
     int pc = MI_PC_JACOBI;    // preconditioner the slots were packed for
     bool ext_alloc = false;   // rh, y, z, t allocated
+    // batched GMRES(m): basis ((m+1) slot-layout vectors of the widest
+    // layout), per-equation small FP64 arrays, partials of the basis kernels
+    int gm_m = 0;
+    DevBuf<float> gmV;
+    DevBuf<double> gmSmall, gmPart;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_chunk = 0, graph_kind = -1, graph_pc = -1, graph_set = -1, graph_W = 0;
@@ -1547,11 +1910,82 @@ int autotune(mi_batch* B)
     return MI_OK;
 }
 
+// The workspace of batched GMRES(m).
+int gm_alloc(mi_batch* B, int m)
+{
+    if (B->gm_m == m)
+        return MI_OK;
+    B->gmV = DevBuf<float>();
+    B->gmSmall = DevBuf<double>();
+    B->gmPart = DevBuf<double>();
+    const size_t nW = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Wmax);
+    MI_TRY(B->gmV.alloc(size_t(m + 1) * nW));
+    MI_TRY(B->gmSmall.alloc(size_t(B->E) * size_t(gm_small_stride(m))));
+    MI_TRY(B->gmPart.alloc(size_t(B->Wmax) * size_t(m + 1) * size_t(kBatchMaxGrid)));
+    B->gm_m = m;
+    return MI_OK;
+}
+
+BGm gm_args(const mi_batch* B, int W)
+{
+    BGm g{};
+    g.V = B->gmV.p;
+    g.vs = size_t(B->n) * size_t(W);
+    g.m = B->gm_m;
+    g.small = B->gmSmall.p;
+    g.ss = gm_small_stride(B->gm_m);
+    g.part = B->gmPart.p;
+    return g;
+}
+
+// z = M^-1 in for the slots of a running GMRES cycle.
+int gm_precond(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
+{
+    if (B->pc == MI_PC_ILU)
+        return ilu_apply(B, a, in, out, L);
+    return L(k_bgm_jacobi, unsigned(B->Gv), kBlock, a, in, out);
+}
+
+// One GMRES(m) cycle for every equation of the sweep, from r (FP32 residual
+// of the sweep) to xs += V y.
+int gm_sweep(mi_batch* B, const BArgs& a0, int max_iter, Launcher& L)
+{
+    const BGm g = gm_args(B, a0.W);
+    const unsigned Gv = unsigned(B->Gv);
+    const int m = g.m;
+    MI_TRY(gm_precond(B, a0, a0.r, g.V, L));
+    MI_TRY(L(bgm_orth(1), Gv, kBlock, a0, g, 0, 0, 1, 1));
+    MI_TRY(L(k_bgm_begin, unsigned(a0.W), kBlock, a0, g, int(Gv)));
+    MI_TRY(L(k_bgm_scale, Gv, kBlock, a0, g, 0));
+    for (int j = 0; j < m; ++j) {
+        BArgs aj = a0;
+        aj.p = g.V + size_t(j) * g.vs; // w = A v_j into Ap
+        float* vn = g.V + size_t(j + 1) * g.vs;
+        MI_TRY(L(bspmm_kernel<B_GM>(B->sell, B->shared, a0.W, B->Vmax), B->G, kBlock, aj));
+        MI_TRY(gm_precond(B, aj, aj.Ap, vn, L));
+        const unsigned fb = unsigned(a0.W * (j + 1));
+        MI_TRY(L(bgm_dots(j + 1), Gv, kBlock, a0, g, j + 1, j + 1));
+        MI_TRY(L(k_bgm_fold, fb, kBlock, a0, g, j + 1, 1, int(Gv)));
+        MI_TRY(L(bgm_orth(j + 1), Gv, kBlock, a0, g, j + 1, j + 1, 1, 0));
+        MI_TRY(L(k_bgm_fold, fb, kBlock, a0, g, j + 1, 2, int(Gv)));
+        MI_TRY(L(bgm_orth(j + 1), Gv, kBlock, a0, g, j + 1, j + 1, 2, 1));
+        MI_TRY(L(k_bgm_step, unsigned(a0.W), kBlock, a0, g, j, int(Gv), max_iter));
+        MI_TRY(L(k_bgm_scale, Gv, kBlock, a0, g, j + 1));
+    }
+    MI_TRY(L(k_bgm_backsolve, 1u, unsigned(kWave), a0, g));
+    return L(k_bgm_xupdate, Gv, kBlock, a0, g);
+}
+
 int check_cfg(const mi_batch* B, const mi_solver_cfg* c)
 {
-    if (c->kind != MI_SOLVER_CG && c->kind != MI_SOLVER_BICGSTAB)
-        return set_error(MI_ERR_UNSUPPORTED, "batched solve: CG or BiCGStab (kind %d given)",
-                         c->kind);
+    if (c->kind == MI_SOLVER_GMRES) {
+        if (c->restart > kBGmMax)
+            return set_error(MI_ERR_UNSUPPORTED, "batched GMRES: restart %d above %d", c->restart,
+                             kBGmMax);
+    } else if (c->kind != MI_SOLVER_CG && c->kind != MI_SOLVER_BICGSTAB) {
+        return set_error(MI_ERR_UNSUPPORTED,
+                         "batched solve: CG, BiCGStab or GMRES (kind %d given)", c->kind);
+    }
     if (c->preconditioner == MI_PC_ILU) {
         if (!B->ilu_order)
             return set_error(MI_ERR_UNSUPPORTED,
@@ -1607,6 +2041,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         return MI_OK;
     }
     const bool bicg = c.kind == MI_SOLVER_BICGSTAB;
+    const bool gmres = c.kind == MI_SOLVER_GMRES;
+    const int gm_m = std::max(1, c.restart);
     const int pc = c.preconditioner;
     if (pc != B->pc) {
         B->pc = pc;
@@ -1617,6 +2053,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         B->slots_valid = false;
     }
     const bool ext = bicg || pc == MI_PC_ILU;
+    if (gmres)
+        MI_TRY(gm_alloc(B, gm_m));
     if (ext && !B->ext_alloc) {
         MI_TRY(alloc_slots(B, B->set[0], true));
         MI_TRY(alloc_slots(B, B->set[1], true));
@@ -1626,7 +2064,11 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
     f.E = E;
     f.G = B->Gv;
     f.max_iter = c.max_iterations;
-    f.max_outer = c.max_outer > 0 ? c.max_outer : 20;
+    // a GMRES sweep is one cycle of at most m steps: by default as many
+    // sweeps as the iteration limit allows
+    f.max_outer = c.max_outer > 0 ? c.max_outer
+                  : gmres        ? std::max(20, c.max_iterations / gm_m + 2)
+                                 : 20;
     f.tol = c.tolerance;
     f.inner_tol = c.inner_tolerance > 0 ? c.inner_tolerance : 1e-5;
     const size_t m = size_t(n) * size_t(E);
@@ -1672,6 +2114,33 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             Launcher L{s};
             f.W = S.W;
             const BArgs a0 = B->args(B->cur);
+            if (gmres) {
+                // the sweep is one GMRES(m) cycle, one graph
+                if (use_graph) {
+                    if (!B->graph || B->graph_chunk != gm_m || B->graph_kind != c.kind ||
+                        B->graph_pc != pc || B->graph_set != B->cur || B->graph_W != a0.W ||
+                        std::memcmp(&B->graph_f, &f, sizeof(BFinArgs)) != 0) {
+                        if (B->graph) {
+                            (void)hipGraphExecDestroy(B->graph);
+                            B->graph = nullptr;
+                        }
+                        MI_TRY(build_graph(s, [&](Launcher& Lg) -> int {
+                            return gm_sweep(B, a0, c.max_iterations, Lg);
+                        }, &B->graph));
+                        B->graph_chunk = gm_m;
+                        B->graph_kind = c.kind;
+                        B->graph_pc = pc;
+                        B->graph_set = B->cur;
+                        B->graph_W = a0.W;
+                        B->graph_f = f;
+                    }
+                    MI_HIP(hipGraphLaunch(B->graph, s));
+                } else {
+                    MI_TRY(gm_sweep(B, a0, c.max_iterations, L));
+                }
+                MI_TRY(retire(B, S.W >= 64 ? ~0ull : ((1ull << S.W) - 1)));
+                continue;
+            }
             if (bicg) {
                 MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_BI_INIT)));
                 MI_TRY(fin(B, B_BI_INIT, f, B->Gv, L));

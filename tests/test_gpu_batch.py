@@ -237,3 +237,42 @@ def test_batch_bicgstab_reordered_path(gpu):
     for e in range(len(vals)):
         M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
         assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+
+
+def test_batch_config4_full_size(gpu):
+    """The 32-equation transport batch at its full size (100^3, 1M rows, 6.94M
+    nonzeros): every equation's FP64 true residual <= 1e-8, re-checked on the
+    host with scipy, uneven iteration counts, and three equations against the
+    single-equation solver (FP64 Jacobi-PCG solved tightly) to 1e-6."""
+    w = wl.transport(100, 32)
+    m = w.mesh
+    dm = capi.DeviceMesh(m, w.bc_of_zone)
+    A = dm.create_matrix()
+    rhs = capi.DeviceArray(m.n_cells)
+    phi = capi.DeviceArray.from_host(np.zeros(m.n_faces))
+    dt = capi.DeviceArray.from_host(np.full(m.n_cells, w.dt))
+    uold = capi.DeviceArray.from_host(np.zeros(m.n_cells))
+    rp, ci = A.pattern()
+    B = capi.Batch(A, 32)
+    vals = []
+    for e in range(32):
+        dm.assemble_momentum(A, rhs, 0, float(w.gammas[e]), 1.0, phi, d_dt=dt, d_Uold=uold)
+        B.set_equation(e, A)
+        vals.append(A.values())
+    b = np.stack(w.bs)
+    x, its, rel, rc = B.solve(b, tolerance=1e-8, max_iterations=2000, inner_tolerance=1e-5)
+    print(f"config 4 full size: iterations {its.min()}..{its.max()}, max rel {rel.max():.2e}")
+    assert rc == capi.MI_OK and np.all(rel <= 1e-8)
+    assert its.max() > 2 * its.min()
+    for e in range(32):
+        M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
+        assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e]), e
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_JACOBI, tolerance=1e-11,
+                    max_iterations=20000, true_residual=1)
+    for e in (0, 15, 31):
+        s.setup(capi.Csr(rp, ci, vals[e]))
+        xe, st, _ = s.solve(b[e])
+        assert st.converged
+        err = np.linalg.norm(x[e] - xe) / np.linalg.norm(xe)
+        print(f"equation {e}: {its[e]} iterations, |x - x_tight| / |x_tight| = {err:.2e}")
+        assert err <= 1e-6, e

@@ -273,3 +273,81 @@ def test_batch_ilu_refusals(conv4):
     h = capi.C.c_void_p()
     assert lib.mi_batch_create_ex(T.h, 1, ILU, capi.C.byref(h)) == capi.MI_ERR_UNSUPPORTED
     assert lib.mi_batch_create_ex(T.h, 1, 4, capi.C.byref(h)) == capi.MI_ERR_BAD_ARG
+
+
+@pytest.mark.parametrize("pc", [capi.MI_PC_ILU, capi.MI_PC_JACOBI])
+def test_batch_gmres_matches_oracle(conv4, pc):
+    """Batched GMRES(20) (one cycle per FP64 refinement sweep, classical
+    Gram-Schmidt twice): every FP64 true residual <= 1e-8, solutions within
+    1e-6 of the oracle's GMRES (GMRESmSolver::solve) solved tightly, a zero
+    right-hand side returns 0 in 0 iterations, and the single-equation mixed
+    GMRES with the same preconditioner needs about as many Arnoldi steps."""
+    dm, A, rp, ci, vals, bs = conv4
+    B = capi.Batch(A, len(vals), flags=ILU)
+    B.set_values(np.stack(vals))
+    b = bs.copy()
+    b[1] = 0.0
+    kw = dict(kind=capi.MI_SOLVER_GMRES, preconditioner=pc, restart=20, tolerance=1e-8,
+              max_iterations=2000)
+    x, its, rel, rc = B.solve(b, **kw)
+    print(f"batched GMRES(20) pc {pc}: steps {its.tolist()} rel {rel.max():.2e}")
+    assert rc == capi.MI_OK and np.all(rel <= 1e-8)
+    assert its[1] == 0 and np.all(x[1] == 0.0)
+    s = capi.Solver(kind=capi.MI_SOLVER_GMRES, preconditioner=capi.MI_PC_JACOBI, restart=20,
+                    tolerance=1e-8, max_iterations=2000, precision=capi.MI_PRECISION_MIXED,
+                    true_residual=1)
+    for e in range(len(vals)):
+        if e == 1:
+            continue
+        M = sp.csr_matrix((vals[e], ci, rp), shape=(A.n,) * 2)
+        assert np.linalg.norm(b[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+        xo, _, _ = oracle.gmres(oracle.Matrix(rp, ci, vals[e]), b[e], oracle.PC_JACOBI, 1e-13,
+                                5000, 30)
+        assert np.linalg.norm(x[e] - xo) <= 1e-6 * np.linalg.norm(xo), e
+        if pc == capi.MI_PC_JACOBI:
+            s.setup(capi.Csr(rp, ci, vals[e]))
+            _, st, _ = s.solve(b[e])
+            print(f"  eq {e}: batched {its[e]} steps, single-equation mixed {st.iterations}")
+            assert abs(int(its[e]) - st.iterations) <= max(3, st.iterations // 5)
+
+
+def test_batch_gmres_graph_replays_and_limits(gpu):
+    dm, A, rp, ci, vals, b = _convective(12, 3, seed=11)
+    B = capi.Batch(A, len(vals), flags=ILU)
+    B.set_values(np.stack(vals))
+    kw = dict(kind=capi.MI_SOLVER_GMRES, preconditioner=capi.MI_PC_ILU, restart=7,
+              tolerance=1e-8, max_iterations=500)
+    xe, itse, _, _ = B.solve(b, use_graph=0, **kw)
+    xg, itsg, _, _ = B.solve(b, use_graph=1, **kw)
+    xg2, itsg2, _, _ = B.solve(b, use_graph=1, **kw)
+    assert np.array_equal(itse, itsg) and np.array_equal(itsg, itsg2)
+    assert np.array_equal(xe, xg) and np.array_equal(xg, xg2)
+    # restart 1 still converges; an iteration limit of 3 reports not converged
+    x1, its1, rel1, rc1 = B.solve(b, **dict(kw, restart=1, max_iterations=3000))
+    assert rc1 == capi.MI_OK and np.all(rel1 <= 1e-8)
+    x3, its3, rel3, rc3 = B.solve(b, **dict(kw, max_iterations=3))
+    assert rc3 == capi.MI_ERR_NOT_CONVERGED and np.all(its3 <= 3)
+    lib = capi.load()
+    c = capi.default_cfg(kind=capi.MI_SOLVER_GMRES, restart=40)
+    db = capi.DeviceArray.from_host(b.ravel())
+    dx = capi.DeviceArray(b.size)
+    its = np.zeros(len(vals), np.int32)
+    rel = np.zeros(len(vals))
+    assert lib.mi_batch_solve_device(B.h, capi.C.byref(c), db.ptr, dx.ptr, capi.ptr(its),
+                                     capi.ptr(rel)) == capi.MI_ERR_UNSUPPORTED
+
+
+def test_batch_gmres_shared_matrix_large(gpu):
+    """48^3, three right-hand sides on one momentum matrix: GMRES(30)-ILU(0) as
+    the flow's momentum configuration, batched on the shared matrix."""
+    dm, A, rp, ci, vals, b = _convective(48, 1, seed=9)
+    bs = np.stack([wl.normal(80 + e, A.n) for e in range(3)])
+    B = capi.Batch(A, 3, flags=ILU | SHARED)
+    B.set_matrix(A)
+    x, its, rel, rc = B.solve(bs, kind=capi.MI_SOLVER_GMRES, preconditioner=capi.MI_PC_ILU,
+                              restart=30, tolerance=1e-8, max_iterations=3000)
+    print(f"48^3 shared GMRES(30)-ILU steps {its.tolist()}")
+    assert rc == capi.MI_OK
+    M = sp.csr_matrix((vals[0], ci, rp), shape=(A.n,) * 2)
+    for e in range(3):
+        assert np.linalg.norm(bs[e] - M @ x[e]) <= 1.01e-8 * np.linalg.norm(bs[e])
