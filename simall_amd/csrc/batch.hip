@@ -588,131 +588,149 @@ __device__ __forceinline__ bool gm_on(const BArgs& a, int e)
     return e >= 0 && !a.st[e].done && !a.st[e].gend;
 }
 
-// Per-slot FP64 sums over a workgroup of nv values per thread (one slot per
-// thread, rows as k_bvec<1>), stored to g.part[(slot (m+1) + k) G + block],
-// in chunks of 8 values through LDS.
-template <int NMAX>
-__device__ __forceinline__ void gm_store_partials(const BArgs& a, const BGm& g, int P, int nv,
-                                                  const double (&acc)[NMAX])
+// The basis kernels hold rows of z in registers and take the basis vectors
+// in groups of four (the single-equation GMRES kernels' arrangement): a
+// thread owns slot t % W of rows rg, rg + P, ..., rg + (kGmRows - 1) P of a
+// pass (rg = t / W, P = kBlock / W row groups), a workgroup walks the passes
+// of its contiguous row range, and every group of four dot products is
+// reduced per slot through LDS and stored as partials
+// g.part[(slot (m+1) + k) G + block].
+constexpr int kGmRows = 4;
+
+// Stores the per-slot workgroup sums of four values per thread (k0 .. k0+3,
+// those below nv).
+__device__ __forceinline__ void gm_store4(const BArgs& a, const BGm& g, int P, int k0, int nv,
+                                          const double (&acc)[4])
 {
-    __shared__ double lds[8][kBlock];
+    __shared__ double lds[4][kBlock];
     const int W = a.W;
     const int G = gridDim.x;
-    for (int k0 = 0; k0 < nv; k0 += 8) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            double v = 0.0;
+    for (int u = 0; u < 4; ++u)
+        lds[u][threadIdx.x] = acc[u];
+    __syncthreads();
+    const int nk = min(4, nv - k0);
+    for (int t = threadIdx.x; t < nk * W; t += kBlock) {
+        const int u = t / W, sl = t % W;
+        double sum = 0.0;
+        for (int q = 0; q < P; ++q)
+            sum += lds[u][q * W + sl];
+        g.part[(size_t(sl) * (g.m + 1) + k0 + u) * G + blockIdx.x] = sum;
+    }
+    __syncthreads();
+}
+
+// Partials of v_k . z for k < nv over this workgroup's rows, z read as
+// stored (one FP32 vector per call).
+__device__ __forceinline__ void gm_dots4(const BArgs& a, const BGm& g, int P, int rg, int sl,
+                                         bool on, int64_t i0, int64_t i1, int nv, const float* z)
+{
+    const int W = a.W;
+    for (int k0 = 0; k0 < nv; k0 += 4) {
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (on) {
+            for (int64_t base = i0; base < i1; base += int64_t(kGmRows) * P) {
+                double zr[kGmRows];
+                size_t q[kGmRows];
 #pragma unroll
-            for (int k = 0; k < NMAX; ++k)
-                if (k == k0 + u)
-                    v = acc[k];
-            lds[u][threadIdx.x] = v;
+                for (int r = 0; r < kGmRows; ++r) {
+                    const int64_t i = base + rg + int64_t(r) * P;
+                    const bool in = i < i1;
+                    q[r] = size_t(in ? i : i0) * W + sl; // clamped, dropped below
+                    zr[r] = in ? double(z[q[r]]) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = min(k0 + u, nv - 1);
+                    const float* v = g.V + size_t(k) * g.vs;
+                    float vv[kGmRows];
+#pragma unroll
+                    for (int r = 0; r < kGmRows; ++r)
+                        vv[r] = v[q[r]];
+#pragma unroll
+                    for (int r = 0; r < kGmRows; ++r)
+                        acc[u] += double(vv[r]) * zr[r];
+                }
+            }
         }
-        __syncthreads();
-        const int nk = min(8, nv - k0);
-        for (int t = threadIdx.x; t < nk * W; t += kBlock) {
-            const int u = t / W, sl = t % W;
-            double sum = 0.0;
-            for (int q = 0; q < P; ++q)
-                sum += lds[u][q * W + sl];
-            g.part[(size_t(sl) * (g.m + 1) + k0 + u) * G + blockIdx.x] = sum;
-        }
-        __syncthreads();
+        gm_store4(a, g, P, k0, nv, acc);
     }
 }
 
-// Partials of v_k . z for k < nv <= NMAX, z = v_zslot.
-template <int NMAX>
+// Partials of v_k . z for k < nv, z = v_zslot.
 __global__ __launch_bounds__(kBlock) void k_bgm_dots(BArgs a, BGm g, int nv, int zslot)
 {
     const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
     const bool live = t < P * W;
-    const int sl = t % W, rr = t / W;
+    const int sl = t % W, rg = t / W;
     const bool on = live && gm_on(a, a.eq[sl]);
     int64_t i0, i1;
-    block_range(a.n, P, i0, i1);
-    double acc[NMAX];
-#pragma unroll
-    for (int k = 0; k < NMAX; ++k)
-        acc[k] = 0.0;
-    if (on) {
-        const float* z = g.V + size_t(zslot) * g.vs;
-        for (int64_t i = i0 + rr; i < i1; i += P) {
-            const size_t q = size_t(i) * W + sl;
-            const double zi = double(z[q]);
-#pragma unroll
-            for (int k = 0; k < NMAX; ++k)
-                if (k < nv)
-                    acc[k] += double(g.V[size_t(k) * g.vs + q]) * zi;
-        }
-    }
-    gm_store_partials<NMAX>(a, g, P, nv, acc);
+    block_range(a.n, int64_t(kGmRows) * P, i0, i1);
+    gm_dots4(a, g, P, rg, sl, on, i0, i1, nv, g.V + size_t(zslot) * g.vs);
 }
 
-// z = v_zslot; z -= sum_k h_k v_k (k < nv <= NMAX, h = h1 or h2 of the slot's
+// z = v_zslot; z -= sum_k h_k v_k (k < nv, h = h1 or h2 of the slot's
 // equation, accumulated in FP64 in order k), stored back; then the partials
 // of v_k . z (norm == 0) or of z . z (norm == 1, one value).
-template <int NMAX>
 __global__ __launch_bounds__(kBlock) void k_bgm_orth(BArgs a, BGm g, int nv, int zslot, int which,
                                                      int norm)
 {
     const int W = a.W, P = kBlock / W, t = int(threadIdx.x);
     const bool live = t < P * W;
-    const int sl = t % W, rr = t / W;
+    const int sl = t % W, rg = t / W;
     const int e = live ? a.eq[sl] : -1;
     const bool on = live && gm_on(a, e);
     int64_t i0, i1;
-    block_range(a.n, P, i0, i1);
-    double acc[NMAX];
-#pragma unroll
-    for (int k = 0; k < NMAX; ++k)
-        acc[k] = 0.0;
+    block_range(a.n, int64_t(kGmRows) * P, i0, i1);
+    float* z = g.V + size_t(zslot) * g.vs;
+    double nrm = 0.0;
     if (on) {
-        // the slot's coefficients in registers, and each row's basis values
-        // loaded once for the update and the dots
-        double h[NMAX];
-        const double* hg = gm_h(g, e, which);
+        const double* h = gm_h(g, e, which);
+        for (int64_t base = i0; base < i1; base += int64_t(kGmRows) * P) {
+            double zr[kGmRows];
+            size_t q[kGmRows];
+            bool in[kGmRows];
 #pragma unroll
-        for (int k = 0; k < NMAX; ++k)
-            h[k] = k < nv ? hg[k] : 0.0;
-        float* z = g.V + size_t(zslot) * g.vs;
-        for (int64_t i = i0 + rr; i < i1; i += P) {
-            const size_t q = size_t(i) * W + sl;
-            float vk[NMAX];
+            for (int r = 0; r < kGmRows; ++r) {
+                const int64_t i = base + rg + int64_t(r) * P;
+                in[r] = i < i1;
+                q[r] = size_t(in[r] ? i : i0) * W + sl;
+                zr[r] = double(z[q[r]]);
+            }
+            for (int k0 = 0; k0 < nv; k0 += 4) {
 #pragma unroll
-            for (int k = 0; k < NMAX; ++k)
-                vk[k] = k < nv ? g.V[size_t(k) * g.vs + q] : 0.f;
-            double zr = double(z[q]);
+                for (int u = 0; u < 4; ++u) {
+                    const int k = k0 + u;
+                    const int kc = min(k, nv - 1);
+                    const double hk = k < nv ? h[kc] : 0.0;
+                    const float* v = g.V + size_t(kc) * g.vs;
+                    float vv[kGmRows];
 #pragma unroll
-            for (int k = 0; k < NMAX; ++k)
-                if (k < nv)
-                    zr -= h[k] * double(vk[k]);
-            const float zt = float(zr);
-            z[q] = zt;
-            const double zd = double(zt);
-            if (norm) {
-                acc[0] += zd * zd;
-            } else {
-#pragma unroll
-                for (int k = 0; k < NMAX; ++k)
+                    for (int r = 0; r < kGmRows; ++r)
+                        vv[r] = v[q[r]];
                     if (k < nv)
-                        acc[k] += double(vk[k]) * zd;
+#pragma unroll
+                        for (int r = 0; r < kGmRows; ++r)
+                            zr[r] -= hk * double(vv[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < kGmRows; ++r) {
+                if (!in[r])
+                    continue; // the clamped row's update is dropped
+                const float zt = float(zr[r]);
+                z[q[r]] = zt;
+                nrm += double(zt) * double(zt);
             }
         }
     }
-    gm_store_partials<NMAX>(a, g, P, norm ? 1 : nv, acc);
-}
-
-using BGmDots = void (*)(BArgs, BGm, int, int);
-using BGmOrth = void (*)(BArgs, BGm, int, int, int, int);
-// The smallest register footprint holding nv basis vectors.
-inline BGmDots bgm_dots(int nv)
-{
-    return nv <= 8 ? k_bgm_dots<8> : nv <= 16 ? k_bgm_dots<16> : k_bgm_dots<kBGmMax + 1>;
-}
-inline BGmOrth bgm_orth(int nv)
-{
-    return nv <= 8 ? k_bgm_orth<8> : nv <= 16 ? k_bgm_orth<16> : k_bgm_orth<kBGmMax + 1>;
+    if (norm) {
+        double acc[4] = {nrm, 0.0, 0.0, 0.0};
+        gm_store4(a, g, P, 0, 1, acc);
+        return;
+    }
+    // each thread reads back only the rows it wrote
+    gm_dots4(a, g, P, rg, sl, on, i0, i1, nv, z);
 }
 
 // h_which[k] of each slot's equation = the fold of its partials; one block
@@ -1954,7 +1972,7 @@ int gm_sweep(mi_batch* B, const BArgs& a0, int max_iter, Launcher& L)
     const unsigned Gv = unsigned(B->Gv);
     const int m = g.m;
     MI_TRY(gm_precond(B, a0, a0.r, g.V, L));
-    MI_TRY(L(bgm_orth(1), Gv, kBlock, a0, g, 0, 0, 1, 1));
+    MI_TRY(L(k_bgm_orth, Gv, kBlock, a0, g, 0, 0, 1, 1));
     MI_TRY(L(k_bgm_begin, unsigned(a0.W), kBlock, a0, g, int(Gv)));
     MI_TRY(L(k_bgm_scale, Gv, kBlock, a0, g, 0));
     for (int j = 0; j < m; ++j) {
@@ -1964,11 +1982,11 @@ int gm_sweep(mi_batch* B, const BArgs& a0, int max_iter, Launcher& L)
         MI_TRY(L(bspmm_kernel<B_GM>(B->sell, B->shared, a0.W, B->Vmax), B->G, kBlock, aj));
         MI_TRY(gm_precond(B, aj, aj.Ap, vn, L));
         const unsigned fb = unsigned(a0.W * (j + 1));
-        MI_TRY(L(bgm_dots(j + 1), Gv, kBlock, a0, g, j + 1, j + 1));
+        MI_TRY(L(k_bgm_dots, Gv, kBlock, a0, g, j + 1, j + 1));
         MI_TRY(L(k_bgm_fold, fb, kBlock, a0, g, j + 1, 1, int(Gv)));
-        MI_TRY(L(bgm_orth(j + 1), Gv, kBlock, a0, g, j + 1, j + 1, 1, 0));
+        MI_TRY(L(k_bgm_orth, Gv, kBlock, a0, g, j + 1, j + 1, 1, 0));
         MI_TRY(L(k_bgm_fold, fb, kBlock, a0, g, j + 1, 2, int(Gv)));
-        MI_TRY(L(bgm_orth(j + 1), Gv, kBlock, a0, g, j + 1, j + 1, 2, 1));
+        MI_TRY(L(k_bgm_orth, Gv, kBlock, a0, g, j + 1, j + 1, 2, 1));
         MI_TRY(L(k_bgm_step, unsigned(a0.W), kBlock, a0, g, j, int(Gv), max_iter));
         MI_TRY(L(k_bgm_scale, Gv, kBlock, a0, g, j + 1));
     }
