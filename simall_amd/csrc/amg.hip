@@ -358,8 +358,8 @@ template <class T> int DMat<T>::refresh(const double* src, double* tmp, hipStrea
 
 // Chebyshev step coefficients on [lmin, lmax] of D^-1 A, lmin = lmax * ratio
 // (ratio 1/2 for degree 1: damped Jacobi).
-static void chebyshev(double lmax_est, int ldeg, double lmin_ratio, std::vector<double>& alpha,
-                      std::vector<double>& beta)
+void chebyshev(double lmax_est, int ldeg, double lmin_ratio, std::vector<double>& alpha,
+               std::vector<double>& beta)
 {
     const double lmax = lmax_est > 0 ? lmax_est : 2.0;
     const double ratio = ldeg == 1 ? 0.5 : lmin_ratio;

@@ -10,6 +10,12 @@
 namespace mi
 {
 
+// Chebyshev coefficients of a degree-ldeg smoother on [lmin_ratio lmax, lmax]
+// of D^-1 A, lmax the setup's estimate (degree 1: damped Jacobi on
+// [lmax / 2, lmax]): step k is d = beta[k] d + alpha[k] D^-1 r.
+void chebyshev(double lmax_est, int ldeg, double lmin_ratio, std::vector<double>& alpha,
+               std::vector<double>& beta);
+
 template <class T> struct DMat
 {
     int nr = 0, nc = 0;

@@ -43,6 +43,7 @@
 // CG iteration is five launches (eight with ILU(0)), one BiCGStab iteration
 // nine (twelve); `chunk` iterations, or a whole GMRES cycle, are replayed
 // from a HIP graph between polls.
+#include "amg.hpp"
 #include "csr.hpp"
 #include "krylov.hpp"
 #include "reorder.hpp"
@@ -919,6 +920,113 @@ __global__ __launch_bounds__(kBlock) void k_bgm_jacobi(BArgs a, const float* __r
     }
 }
 
+// ------------------------------------------------------------ batched AMG
+// The smoothed-aggregation V-cycle of one shared matrix applied to every
+// slot of the layout at once: the sequence of the single-vector cycle
+// (DHierarchy::cycle: Chebyshev pre-smoothing from x = 0, restriction of the
+// residual, the dense FP64 coarsest solve, prolongation and post-smoothing)
+// on plain FP32 CSR operators, each streamed once per launch for all W
+// right-hand sides.  Vectors of level l are [n_l][W] lines.
+struct BCsr
+{
+    int nr;
+    const int* rp;
+    const int* ci;
+    const float* v;
+};
+
+// r = b, d = a0 D^-1 b, x = d over n W elements
+__global__ __launch_bounds__(kBlock) void k_bamg_init(int64_t n, int W, const float* __restrict__ b,
+                                                      const float* __restrict__ invD, float a0,
+                                                      float* __restrict__ x, float* __restrict__ r,
+                                                      float* __restrict__ d)
+{
+    const int64_t m = n * W;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
+        const float bi = b[q];
+        const float di = a0 * invD[q / W] * bi;
+        r[q] = bi;
+        d[q] = di;
+        x[q] = di;
+    }
+}
+
+// One thread per (row, slot), rows of the operator A over lines of W:
+//   mode 0 (Chebyshev step): r -= A din; dout = beta din + alpha D^-1 r; x += dout
+//   mode 1 (residual):       r -= A din
+//   mode 2 (restriction):    y = A din
+//   mode 3 (prolongation):   y = A din; x += y
+__global__ __launch_bounds__(kBlock) void k_bamg_rows(BCsr A, int W, int mode,
+                                                      const float* __restrict__ din,
+                                                      float* __restrict__ dout,
+                                                      float* __restrict__ r,
+                                                      const float* __restrict__ invD,
+                                                      float* __restrict__ x,
+                                                      float* __restrict__ y, float beta, float alpha)
+{
+    const int64_t m = int64_t(A.nr) * W;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
+        const int i = int(q / W), s = int(q - int64_t(i) * W);
+        const int kb = A.rp[i], ke = A.rp[i + 1];
+        float acc = 0.f;
+        int k = kb;
+        for (; k + 4 <= ke; k += 4) {
+            const int c0 = A.ci[k], c1 = A.ci[k + 1], c2 = A.ci[k + 2], c3 = A.ci[k + 3];
+            const float v0 = A.v[k], v1 = A.v[k + 1], v2 = A.v[k + 2], v3 = A.v[k + 3];
+            const float x0 = din[size_t(c0) * W + s], x1 = din[size_t(c1) * W + s];
+            const float x2 = din[size_t(c2) * W + s], x3 = din[size_t(c3) * W + s];
+            acc += v0 * x0;
+            acc += v1 * x1;
+            acc += v2 * x2;
+            acc += v3 * x3;
+        }
+        for (; k < ke; ++k)
+            acc += A.v[k] * din[size_t(A.ci[k]) * W + s];
+        if (mode <= 1) {
+            const float ri = r[q] - acc;
+            r[q] = ri;
+            if (mode == 0) {
+                const float dn = beta * din[q] + alpha * invD[i] * ri;
+                dout[q] = dn;
+                x[q] += dn;
+            }
+        } else {
+            y[q] = acc;
+            if (mode == 3)
+                x[q] += acc;
+        }
+    }
+}
+
+// x = Ainv b on the coarsest level (FP64 dense), one workgroup per row:
+// thread t sums the columns j = t / W (mod kBlock / W) of slot t % W.
+__global__ __launch_bounds__(kBlock) void k_bamg_coarse(int nc, int W,
+                                                        const double* __restrict__ inv,
+                                                        const float* __restrict__ b,
+                                                        float* __restrict__ x)
+{
+    __shared__ double lds[kBlock];
+    const int row = int(blockIdx.x);
+    const int P = kBlock / W, t = int(threadIdx.x);
+    const int s = t % W, jg = t / W;
+    double acc = 0.0;
+    if (t < P * W) {
+        const double* a = inv + size_t(row) * size_t(nc);
+        for (int j = jg; j < nc; j += P)
+            acc += a[j] * double(b[size_t(j) * W + s]);
+    }
+    lds[t] = acc;
+    __syncthreads();
+    if (t < W) {
+        double sum = 0.0;
+        for (int q = 0; q < P; ++q)
+            sum += lds[q * W + t];
+        x[size_t(row) * W + t] = float(sum);
+    }
+}
+
 // ILU(0) apply out = (LU)^-1 in under the two-colour ordering: rows [0, n0)
 // have no lower entries (their neighbours all come later), rows [n0, n) no
 // upper entries, and the factorization changes only the second colour's
@@ -1586,6 +1694,21 @@ struct mi_batch
     bool slots_valid = false; // set[cur] holds the home values of its equations
     int pc = MI_PC_JACOBI;    // preconditioner the slots were packed for
     bool ext_alloc = false;   // rh, y, z, t allocated
+    // batched AMG (shared matrix): the hierarchy of the last setup, FP32 CSR
+    // operators and [n][Wmax] work vectors per level, the dense FP64 inverse
+    // of the coarsest operator
+    struct ALevel
+    {
+        int n = 0, deg = 1;
+        DevBuf<int> arp, aci, prp, pci, rrp, rci;
+        DevBuf<float> av, pv, rv, invD;
+        DevBuf<float> r, d, d2, x, b, e;
+        std::vector<double> alpha, beta;
+    };
+    std::vector<ALevel> amg;
+    DevBuf<double> amg_cinv;
+    int amg_nc = 0;
+    bool amg_valid = false; // the hierarchy belongs to the current values
     // batched GMRES(m): basis ((m+1) slot-layout vectors of the widest
     // layout), per-equation small FP64 arrays, partials of the basis kernels
     int gm_m = 0;
@@ -1800,6 +1923,141 @@ int fin(mi_batch* B, int op, const BFinArgs& f, int G, Launcher& L)
              static_cast<const int*>(S.d_eq.p), B->st.p, op, g);
 }
 
+// The SA-AMG hierarchy of the shared matrix (host setup, the single-equation
+// solver's parameters), uploaded as FP32 CSR operators.
+int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
+{
+    const Pattern& P = *B->pat;
+    const int n = int(B->n);
+    HCsr A;
+    A.nr = A.nc = n;
+    A.rp = P.h_rp;
+    A.ci = P.h_ci;
+    A.v.resize(size_t(B->nnz));
+    if (B->nnz)
+        MI_HIP(hipMemcpy(A.v.data(), B->v64.p, size_t(B->nnz) * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    AmgParams p;
+    if (c.amg_max_levels > 0)
+        p.max_levels = c.amg_max_levels;
+    if (c.amg_coarse_size > 0)
+        p.coarse_size = c.amg_coarse_size;
+    if (c.amg_strength >= 0)
+        p.theta = c.amg_strength;
+    if (c.amg_jacobi_omega > 0)
+        p.omega_scale = c.amg_jacobi_omega;
+    if (c.amg_truncation >= 0)
+        p.truncation = c.amg_truncation;
+    AmgHierarchyHost H;
+    if (amg_build(A, p, H) != 0)
+        return set_error(MI_ERR_UNSUPPORTED, "batched AMG: hierarchy construction failed");
+    const int nl = int(H.levels.size());
+    const int deg0 = c.amg_sweeps > 0 ? c.amg_sweeps : 2;
+    const int degc = c.amg_coarse_sweeps > 0 ? c.amg_coarse_sweeps : deg0;
+    hipStream_t s = B->stream;
+    B->amg.clear();
+    B->amg.resize(size_t(nl));
+    auto up = [&](const HCsr& h, DevBuf<int>& rp, DevBuf<int>& ci, DevBuf<float>& v) -> int {
+        std::vector<float> f(h.v.size());
+        for (size_t k = 0; k < f.size(); ++k)
+            f[k] = float(h.v[k]);
+        MI_TRY(rp.upload(h.rp.data(), h.rp.size(), s));
+        MI_TRY(ci.upload(h.ci.data(), std::max<size_t>(h.ci.size(), 1), s));
+        MI_TRY(v.upload(f.data(), std::max<size_t>(f.size(), 1), s));
+        MI_HIP(hipStreamSynchronize(s)); // f is released on return
+        return MI_OK;
+    };
+    for (int l = 0; l < nl; ++l) {
+        const AmgLevelHost& h = H.levels[size_t(l)];
+        mi_batch::ALevel& d = B->amg[size_t(l)];
+        d.n = h.A.nr;
+        d.deg = l == 0 ? deg0 : degc;
+        MI_TRY(up(h.A, d.arp, d.aci, d.av));
+        if (l + 1 < nl) {
+            MI_TRY(up(h.P, d.prp, d.pci, d.pv));
+            MI_TRY(up(h.R, d.rrp, d.rci, d.rv));
+        }
+        std::vector<float> inv(size_t(std::max(d.n, 1)), 1.f);
+        for (int i = 0; i < d.n; ++i)
+            for (int k = h.A.rp[size_t(i)]; k < h.A.rp[size_t(i) + 1]; ++k)
+                if (h.A.ci[size_t(k)] == i && h.A.v[size_t(k)] != 0.0)
+                    inv[size_t(i)] = float(1.0 / h.A.v[size_t(k)]);
+        MI_TRY(d.invD.upload(inv.data(), inv.size(), s));
+        MI_HIP(hipStreamSynchronize(s));
+        const size_t nw = size_t(std::max(d.n, 1)) * size_t(B->Wmax);
+        for (DevBuf<float>* v : {&d.r, &d.d, &d.d2, &d.x, &d.b, &d.e})
+            MI_TRY(v->alloc(nw));
+        chebyshev(h.lmax, d.deg, 0.3, d.alpha, d.beta);
+    }
+    B->amg_nc = H.coarse_n;
+    MI_TRY(B->amg_cinv.upload(H.coarse_inverse.data(),
+                              std::max<size_t>(H.coarse_inverse.size(), 1), s));
+    MI_HIP(hipStreamSynchronize(s));
+    B->amg_valid = true;
+    return MI_OK;
+}
+
+// out = V-cycle(in) for every slot of the layout (width W): the launch
+// sequence of DHierarchy::cycle.
+int amg_apply_batch(mi_batch* B, int W, const float* in, float* out, Launcher& L)
+{
+    const int nl = int(B->amg.size());
+    auto grid = [&](int64_t rows) {
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows * W + kBlock - 1) / kBlock,
+                                                               16384)));
+    };
+    auto csr = [&](const DevBuf<int>& rp, const DevBuf<int>& ci, const DevBuf<float>& v, int nr) {
+        return BCsr{nr, rp.p, ci.p, v.p};
+    };
+    auto bptr = [&](int l) -> const float* { return l == 0 ? in : B->amg[size_t(l)].b.p; };
+    auto xptr = [&](int l) -> float* { return l == 0 ? out : B->amg[size_t(l)].x.p; };
+    const float* cnull = nullptr;
+    float* fnull = nullptr;
+    for (int l = 0; l + 1 < nl; ++l) {
+        mi_batch::ALevel& d = B->amg[size_t(l)];
+        const BCsr A = csr(d.arp, d.aci, d.av, d.n);
+        float* x = xptr(l);
+        MI_TRY(L(k_bamg_init, grid(d.n), kBlock, int64_t(d.n), W, bptr(l),
+                 static_cast<const float*>(d.invD.p), float(d.alpha[0]), x, d.r.p, d.d.p));
+        float* din = d.d.p;
+        float* dout = d.d2.p;
+        for (int k = 1; k < d.deg; ++k) {
+            MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(din), dout,
+                     d.r.p, static_cast<const float*>(d.invD.p), x, fnull,
+                     float(d.beta[size_t(k)]), float(d.alpha[size_t(k)])));
+            std::swap(din, dout);
+        }
+        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 1, static_cast<const float*>(din), fnull,
+                 d.r.p, static_cast<const float*>(d.invD.p), x, fnull, 0.f, 0.f));
+        const int nc = B->amg[size_t(l) + 1].n;
+        MI_TRY(L(k_bamg_rows, grid(nc), kBlock, csr(d.rrp, d.rci, d.rv, nc), W, 2,
+                 static_cast<const float*>(d.r.p), fnull, fnull, cnull, fnull,
+                 B->amg[size_t(l) + 1].b.p, 0.f, 0.f));
+    }
+    if (B->amg_nc > 0)
+        MI_TRY(L(k_bamg_coarse, unsigned(B->amg_nc), kBlock, B->amg_nc, W,
+                 static_cast<const double*>(B->amg_cinv.p), bptr(nl - 1), xptr(nl - 1)));
+    for (int l = nl - 2; l >= 0; --l) {
+        mi_batch::ALevel& d = B->amg[size_t(l)];
+        const BCsr A = csr(d.arp, d.aci, d.av, d.n);
+        float* x = xptr(l);
+        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, csr(d.prp, d.pci, d.pv, d.n), W, 3,
+                 static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f, 0.f));
+        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(d.e.p), d.d.p,
+                 d.r.p, static_cast<const float*>(d.invD.p), x, fnull, 0.f,
+                 float(d.alpha[0])));
+        float* din = d.d.p;
+        float* dout = d.d2.p;
+        for (int k = 1; k < d.deg; ++k) {
+            MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(din), dout,
+                     d.r.p, static_cast<const float*>(d.invD.p), x, fnull,
+                     float(d.beta[size_t(k)]), float(d.alpha[size_t(k)])));
+            std::swap(din, dout);
+        }
+    }
+    return MI_OK;
+}
+
 // z = M^-1 in for the iterating slots: the two sweeps of the ILU(0) apply.
 int ilu_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
 {
@@ -1808,15 +2066,24 @@ int ilu_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher
     return L(k, B->G, kBlock, a, in, out, 1, B->n0);
 }
 
+// A preconditioner applied by launches of its own (ILU(0), the AMG V-cycle).
+bool separate_pc(int pc) { return pc == MI_PC_ILU || pc == MI_PC_AMG; }
+int separate_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
+{
+    if (B->pc == MI_PC_AMG)
+        return amg_apply_batch(B, a.W, in, out, L);
+    return ilu_apply(B, a, in, out, L);
+}
+
 // One CG iteration for every iterating equation: five launches with Jacobi,
-// eight with ILU(0).
+// eight with ILU(0), the V-cycle's and seven with AMG.
 int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
     MI_TRY(L(bspmm_kernel<B_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_ALPHA, f, B->G, L));
-    if (B->pc == MI_PC_ILU) {
+    if (separate_pc(B->pc)) {
         MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_UPD_X)));
-        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+        MI_TRY(separate_apply(B, a, a.r, a.z, L));
         MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_DOT_RZ)));
         MI_TRY(fin(B, B_UPDATE, f, B->Gv, L));
         return L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_PZ));
@@ -1830,16 +2097,16 @@ int one_iteration(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 // Jacobi, twelve with ILU(0).
 int one_iteration_bicg(mi_batch* B, const BArgs& a, const BFinArgs& f, Launcher& L)
 {
-    const bool ilu = B->pc == MI_PC_ILU;
+    const bool ilu = separate_pc(B->pc);
     MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(ilu ? B_BI_PN : B_BI_P)));
     if (ilu)
-        MI_TRY(ilu_apply(B, a, a.p, a.y, L));
+        MI_TRY(separate_apply(B, a, a.p, a.y, L));
     MI_TRY(L(bspmm_kernel<B_BI_ALPHA>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_ALPHA, f, B->G, L));
     MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(ilu ? B_BI_SN : B_BI_S)));
     MI_TRY(fin(B, B_BI_S, f, B->Gv, L));
     if (ilu)
-        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+        MI_TRY(separate_apply(B, a, a.r, a.z, L));
     MI_TRY(L(bspmm_kernel<B_BI_T>(B->sell, B->shared, a.W, B->Vmax), B->G, kBlock, a));
     MI_TRY(fin(B, B_BI_OMEGA, f, B->G, L));
     MI_TRY(L(bvec_kernel(a.W), B->Gv, kBlock, a, int(B_BI_UPDATE)));
@@ -1959,8 +2226,8 @@ BGm gm_args(const mi_batch* B, int W)
 // z = M^-1 in for the slots of a running GMRES cycle.
 int gm_precond(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
 {
-    if (B->pc == MI_PC_ILU)
-        return ilu_apply(B, a, in, out, L);
+    if (separate_pc(B->pc))
+        return separate_apply(B, a, in, out, L);
     return L(k_bgm_jacobi, unsigned(B->Gv), kBlock, a, in, out);
 }
 
@@ -2008,9 +2275,13 @@ int check_cfg(const mi_batch* B, const mi_solver_cfg* c)
         if (!B->ilu_order)
             return set_error(MI_ERR_UNSUPPORTED,
                              "batched solve: ILU(0) needs a batch created with MI_BATCH_ILU0");
+    } else if (c->preconditioner == MI_PC_AMG) {
+        if (!B->shared)
+            return set_error(MI_ERR_UNSUPPORTED,
+                             "batched solve: AMG needs one shared matrix (MI_BATCH_SHARED_VALUES)");
     } else if (c->preconditioner != MI_PC_JACOBI && c->preconditioner != MI_PC_NONE) {
         return set_error(MI_ERR_UNSUPPORTED,
-                         "batched solve: Jacobi, ILU(0) or no preconditioner (%d given)",
+                         "batched solve: Jacobi, ILU(0), AMG or no preconditioner (%d given)",
                          c->preconditioner);
     }
     if (!(c->tolerance > 0.0) || c->max_iterations < 0)
@@ -2070,7 +2341,14 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         MI_TRY(factor_ilu(B));
         B->slots_valid = false;
     }
-    const bool ext = bicg || pc == MI_PC_ILU;
+    if (pc == MI_PC_AMG && !B->amg_valid) {
+        if (B->graph) { // its launches hold the previous hierarchy's buffers
+            (void)hipGraphExecDestroy(B->graph);
+            B->graph = nullptr;
+        }
+        MI_TRY(amg_setup_batch(B, c));
+    }
+    const bool ext = bicg || separate_pc(pc);
     if (gmres)
         MI_TRY(gm_alloc(B, gm_m));
     if (ext && !B->ext_alloc) {
@@ -2162,8 +2440,8 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             if (bicg) {
                 MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_BI_INIT)));
                 MI_TRY(fin(B, B_BI_INIT, f, B->Gv, L));
-            } else if (pc == MI_PC_ILU) {
-                MI_TRY(ilu_apply(B, a0, a0.r, a0.z, L));
+            } else if (separate_pc(pc)) {
+                MI_TRY(separate_apply(B, a0, a0.r, a0.z, L));
                 MI_TRY(L(bvec_kernel(a0.W), B->Gv, kBlock, a0, int(B_INIT_Z)));
                 MI_TRY(fin(B, B_INIT, f, B->Gv, L));
             } else {
@@ -2331,6 +2609,7 @@ int write_matrix(mi_batch* B, int e, const mi_csr* A)
     MI_HIP(hipStreamSynchronize(s));
     B->slots_valid = false;
     B->ilu_valid = false;
+    B->amg_valid = false;
     return MI_OK;
 }
 
@@ -2516,6 +2795,13 @@ MI_API int mi_batch_apply_preconditioner(mi_batch* B, int preconditioner, const 
     }
     if (preconditioner == MI_PC_ILU && !B->ilu_valid)
         MI_TRY(factor_ilu(B));
+    if (preconditioner == MI_PC_AMG && !B->amg_valid) {
+        if (B->graph) {
+            (void)hipGraphExecDestroy(B->graph);
+            B->graph = nullptr;
+        }
+        MI_TRY(amg_setup_batch(B, c));
+    }
     if (!B->ext_alloc) {
         MI_TRY(alloc_slots(B, B->set[0], true));
         MI_TRY(alloc_slots(B, B->set[1], true));
@@ -2538,8 +2824,8 @@ MI_API int mi_batch_apply_preconditioner(mi_batch* B, int preconditioner, const 
     BArgs a = B->args(B->cur);
     a.st = nullptr; // every slot
     Launcher L{s};
-    if (preconditioner == MI_PC_ILU) {
-        MI_TRY(ilu_apply(B, a, a.r, a.z, L));
+    if (separate_pc(preconditioner)) {
+        MI_TRY(separate_apply(B, a, a.r, a.z, L));
     } else {
         k_bscale<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(a, a.r, a.z);
         MI_HIP(hipGetLastError());

@@ -1,0 +1,177 @@
+"""Batched solves preconditioned by the SA-AMG V-cycle of one shared matrix.
+
+A batch created with MI_BATCH_SHARED_VALUES holds one pressure-like matrix for
+every right-hand side; MI_PC_AMG builds that matrix's hierarchy once (the
+single-equation solver's setup and parameters) and applies the V-cycle to
+every slot of the slot-interleaved layout in the same launches, each level's
+operators streamed once for all right-hand sides.
+
+Pinned here:
+  - one V-cycle against the numpy V-cycle (tests/amg_model.py, pinned to the
+    single-equation solver in test_gpu_amg_resetup.py) of the hierarchy the
+    host setup builds for the same matrix and parameters, per slot: the
+    device works in FP32 (the coarsest solve in FP64), so the bar is 1e-5 of
+    |z| (a dropped smoothing step, a wrong Chebyshev coefficient or a
+    transposed transfer moves z by 1e-2 or more);
+  - CG, BiCGStab and GMRES with AMG against scipy's direct solution (1e-6)
+    and by their FP64 true residual (<= 1e-8), with iteration counts near
+    the single-equation AMG-PCG's;
+  - graph replays equal the eager launches bit for bit;
+  - new values rebuild the hierarchy; batches without shared values refuse.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from simall_amd import capi
+from simall_amd import workload as wl
+
+from amg_model import VCycle
+
+pytestmark = pytest.mark.gpu
+
+SHARED = capi.MI_BATCH_SHARED_VALUES
+
+
+def poisson(N):
+    w = wl.poisson3d(N)
+    m = w.mesh
+    rp, ci = oracle.build_sparsity(m)
+    P = oracle.Matrix(rp, ci, np.zeros(ci.size))
+    vals, _, _ = oracle.assemble_pressure(m, P, w.bc_of_zone, gamma=w.gamma)
+    return sp.csr_matrix((vals, ci, rp), shape=(m.n_cells,) * 2)
+
+
+def rhs(n, n_eq, seed):
+    b = np.stack([wl.normal(seed + e, n) for e in range(n_eq)])
+    b[:, 0] = 0.0
+    return b
+
+
+@pytest.fixture(scope="module")
+def p24(gpu):
+    A = poisson(24)
+    return A, capi.Csr(A.indptr, A.indices, A.data)
+
+
+KW = dict(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-8,
+          max_iterations=300)
+
+
+def test_batched_vcycle_matches_the_model(p24):
+    A, M = p24
+    B = capi.Batch(M, 5, flags=SHARED)
+    B.set_matrix(M)
+    r = rhs(A.shape[0], 5, 80)
+    z = B.apply_preconditioner(r, capi.MI_PC_AMG)
+    H = capi.AmgHost(A.indptr, A.indices, A.data, cfg=capi.default_cfg())
+    V = VCycle(H, 2, 1)
+    for e in range(5):
+        zm = V(r[e])
+        err = np.linalg.norm(z[e] - zm) / np.linalg.norm(zm)
+        print(f"slot {e}: |z - model| / |model| = {err:.1e} ({H.levels} levels)")
+        assert err <= 1e-5, (e, err)
+    # the same right-hand side in every slot gives the same z in every slot
+    same = B.apply_preconditioner(np.repeat(r[:1], 5, axis=0), capi.MI_PC_AMG)
+    for e in range(1, 5):
+        assert np.array_equal(same[e], same[0])
+
+
+@pytest.mark.parametrize("kind", [capi.MI_SOLVER_CG, capi.MI_SOLVER_BICGSTAB,
+                                  capi.MI_SOLVER_GMRES])
+def test_batched_krylov_amg_solves(p24, kind):
+    A, M = p24
+    n_eq = 6
+    B = capi.Batch(M, n_eq, flags=SHARED)
+    B.set_matrix(M)
+    b = rhs(A.shape[0], n_eq, 90)
+    kw = dict(KW, kind=kind)
+    if kind == capi.MI_SOLVER_GMRES:
+        kw["restart"] = 20
+    x, its, rel, rc = B.solve(b, **kw)
+    s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-8,
+                    max_iterations=300, true_residual=1)
+    s.setup(M)
+    _, st, _ = s.solve(b[0])
+    print(f"kind {kind}: batched AMG iterations {its.tolist()}, single AMG-PCG {st.iterations}")
+    assert rc == capi.MI_OK
+    lu = spla.splu(A.tocsc())
+    for e in range(n_eq):
+        xs = lu.solve(b[e])
+        assert np.linalg.norm(b[e] - A @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+        assert np.linalg.norm(x[e] - xs) <= 1e-6 * np.linalg.norm(xs)
+    if kind == capi.MI_SOLVER_CG:
+        assert its.max() <= st.iterations + 5
+
+
+def test_batched_amg_graph_replays_match_eager(p24):
+    A, M = p24
+    B = capi.Batch(M, 4, flags=SHARED)
+    B.set_matrix(M)
+    b = rhs(A.shape[0], 4, 100)
+    kw = dict(KW, check_interval=3)
+    xe, itse, _, _ = B.solve(b, use_graph=0, **kw)
+    xg, itsg, _, _ = B.solve(b, use_graph=1, **kw)
+    xg2, itsg2, _, _ = B.solve(b, use_graph=1, **kw)
+    assert np.array_equal(itse, itsg) and np.array_equal(itsg, itsg2)
+    assert np.array_equal(xe, xg) and np.array_equal(xg, xg2)
+    kb = dict(KW, kind=capi.MI_SOLVER_BICGSTAB, check_interval=2)
+    be, _, _, _ = B.solve(b, use_graph=0, **kb)
+    bg, _, _, _ = B.solve(b, use_graph=1, **kb)
+    assert np.array_equal(be, bg)
+
+
+def test_batched_amg_new_values_rebuild_the_hierarchy(p24):
+    A, M = p24
+    # the same pattern, every diagonal entry raised by up to its own size
+    A2 = A.copy()
+    rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+    diag = rows == A.indices
+    A2.data[diag] *= 1.0 + np.linspace(0.0, 1.0, A.shape[0])[rows[diag]]
+    M2 = capi.Csr(A2.indptr, A2.indices, A2.data)
+    B = capi.Batch(M, 3, flags=SHARED)
+    B.set_matrix(M)
+    b = rhs(A.shape[0], 3, 110)
+    x1, _, _, rc1 = B.solve(b, **KW)
+    B.set_matrix(M2)
+    x2, _, _, rc2 = B.solve(b, **KW)
+    assert rc1 == rc2 == capi.MI_OK
+    for e in range(3):
+        assert np.linalg.norm(b[e] - A @ x1[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+        assert np.linalg.norm(b[e] - A2 @ x2[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+    # the rebuilt V-cycle is the one a fresh batch builds for the new values
+    r = rhs(A.shape[0], 3, 120)
+    Bf = capi.Batch(M2, 3, flags=SHARED)
+    Bf.set_matrix(M2)
+    assert np.array_equal(B.apply_preconditioner(r, capi.MI_PC_AMG),
+                          Bf.apply_preconditioner(r, capi.MI_PC_AMG))
+
+
+def test_batched_amg_needs_shared_values(p24):
+    A, M = p24
+    B = capi.Batch(M, 2)
+    B.set_values(np.stack([A.data, A.data]))
+    with pytest.raises(capi.MiError) as ei:
+        B.solve(rhs(A.shape[0], 2, 130), **KW)
+    assert ei.value.code == capi.MI_ERR_UNSUPPORTED
+    with pytest.raises(capi.MiError):
+        B.apply_preconditioner(rhs(A.shape[0], 2, 130), capi.MI_PC_AMG)
+
+
+def test_batched_amg_large_reordered(gpu):
+    """48^3 = 110592 rows (bandwidth-reduced inside the batch), eight
+    right-hand sides on one pressure matrix."""
+    A = poisson(48)
+    M = capi.Csr(A.indptr, A.indices, A.data)
+    n_eq = 8
+    B = capi.Batch(M, n_eq, flags=SHARED)
+    B.set_matrix(M)
+    b = rhs(A.shape[0], n_eq, 140)
+    x, its, rel, rc = B.solve(b, **KW)
+    print(f"48^3 batched CG-AMG iterations {its.tolist()}")
+    assert rc == capi.MI_OK
+    for e in range(n_eq):
+        assert np.linalg.norm(b[e] - A @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
+    assert its.max() <= 60
