@@ -813,6 +813,62 @@ def bench_batch_convective(capi, wl, lib, steps, log, N=96, n_eq=8):
     return out
 
 
+def bench_pressure_batched(capi, wl, lib, A, pinned, single, steps, log, n_eq=8):
+    """n_eq right-hand sides on the headline pressure matrix: one batched
+    AMG-PCG solve (mi_batch with shared values: one SA-AMG hierarchy, each
+    V-cycle launch streams a level's operators once for every right-hand
+    side) to 1e-8 true residual, against the headline mixed solver's
+    one-at-a-time solves of the same right-hand sides."""
+    import numpy as np
+
+    n = A.n
+    bs = []
+    for e in range(n_eq):
+        b = wl.remove_mean(wl.normal(200 + e, n))
+        if pinned:
+            b[0] = 0.0
+        bs.append(b)
+    db = capi.DeviceArray.from_host(np.concatenate(bs))
+    dx = capi.DeviceArray(n_eq * n)
+    B = capi.Batch(A, n_eq, flags=capi.MI_BATCH_SHARED_VALUES)
+    B.set_matrix(A)
+    kw = dict(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-8,
+              max_iterations=500)
+    lib.mi_device_synchronize()
+    t1 = time.perf_counter()
+    B.solve_device(db, dx, **kw)  # includes the host hierarchy setup
+    first_ms = 1e3 * (time.perf_counter() - t1)
+    times = []
+    for _ in range(steps):
+        lib.mi_device_synchronize()
+        t1 = time.perf_counter()
+        its, rel, rc = B.solve_device(db, dx, **kw)
+        times.append(time.perf_counter() - t1)
+    batched_ms = 1e3 * float(np.median(times))
+    one, its1 = 0.0, []
+    for e in range(n_eq):
+        dbe = capi.DeviceArray.from_host(bs[e])
+        dxe = capi.DeviceArray(n)
+        lib.mi_device_synchronize()
+        t1 = time.perf_counter()
+        st, rc1 = single.solve_device(dbe, dxe)
+        lib.mi_device_synchronize()
+        one += time.perf_counter() - t1
+        its1.append(st.as_dict()["iterations"])
+    B.close()
+    out = {"workload": f"{n_eq} right-hand sides on the {n}-row pressure matrix, CG with one "
+                       f"shared SA-AMG hierarchy, FP32 inner / FP64 refinement, to 1e-8",
+           "equations": n_eq, "ms": round(batched_ms, 3),
+           "first_solve_with_setup_ms": round(first_ms, 3),
+           "one_at_a_time_ms": round(1e3 * one, 3),
+           "speedup": round(1e3 * one / batched_ms, 3),
+           "iterations_min_max": [int(its.min()), int(its.max())],
+           "one_at_a_time_iterations_min_max": [int(min(its1)), int(max(its1))],
+           "max_true_relative_residual": float(rel.max()), "status": rc}
+    log(f"batched pressure: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -1025,6 +1081,11 @@ def main():
                    "faces": m.n_faces}
         log(f"adjoint: {adjoint}")
 
+    pressure_batched = None
+    if not args.no_batch and "mixed" in solvers:
+        pressure_batched = bench_pressure_batched(capi, wl, lib, A, pinned, solvers["mixed"],
+                                                  args.steps, log)
+
     momentum = None
     if not args.no_momentum:
         momentum = bench_momentum(capi, wl, lib, args.steps, log)
@@ -1037,6 +1098,7 @@ def main():
     if not args.no_batch:
         batch = bench_batch(capi, wl, lib, args.steps, log, traffic=batch_traffic)
         batch["convective"] = bench_batch_convective(capi, wl, lib, args.steps, log)
+        batch["pressure_amg"] = pressure_batched
 
     err = None
     if "fp64" in results and "mixed" in results:

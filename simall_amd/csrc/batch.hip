@@ -99,6 +99,7 @@ struct BArgs
     const int* eci;    // SELL-64 slot columns (diagonal flag, -1 padding)
     int shared;        // one matrix for every equation: v32[k], invD[i], lv[k]
     int band;          // row walk of the SpMM and ILU sweeps (row_walk)
+    int diff;          // v32 is the FP32 difference form (FP64 row sums on the diagonal)
     const float* v32;  // [nnz or padded SELL slots][W] ([..] when shared)
     const float* invD; // [n][W] ([n] when shared): 1/a_ii, or the ILU(0) 1/pivot
     const float* lv;   // ILU(0) lower factor l_ik at the entries of the second colour's rows
@@ -243,7 +244,12 @@ __device__ __forceinline__ void slot_partials_v(const BArgs& a, int R, int grp, 
 // V-wide loads of the values and of the gathered rows of `in`: kBlock V / W
 // rows per workgroup at a time; every slot's row is summed in CSR order.
 // kShared: one matrix for every equation, its value of a nonzero loaded once
-// and applied to the thread's V slots.
+// and applied to the thread's V slots.  With a.diff the FP32 values are the
+// difference form of the single-equation solver's inner operator: a_ij
+// (x_j - x_i) off the diagonal, the FP64 row sum times x_i on it (the same
+// product in exact arithmetic, without the cancellation that rounding a_ii
+// and a_ij apart leaves in A x for the nearly constant vectors of a pinned
+// pressure operator); without it x_i is 0 and the sums are the plain ones.
 template <int OP, bool kSell, int V, bool kShared>
 __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 {
@@ -280,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
             const size_t q = size_t(i) * W + s0;
             // the dot product's operand, loaded ahead of the row's streams
             const VT wq = *reinterpret_cast<const VT*>(vw + q);
+            const VT xi = a.diff ? *reinterpret_cast<const VT*>(vin + q) : VT(0.f);
             VT acc = VT(0.f);
             if constexpr (kSell) {
                 // slot k of row i: m0 + 64 k, in CSR order
@@ -310,7 +317,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 #pragma unroll
                     for (int j = 0; j < U; ++j)
                         if (c[j] >= 0)
-                            acc += v[j] * xv[j];
+                            acc += v[j] * ((c[j] & kSellDiag) ? xv[j] : xv[j] - xi);
                 }
             } else {
                 const int kb = a.rp[i], ke = a.rp[i + 1];
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void k_bspmm(BArgs a)
 #pragma unroll
                     for (int j = 0; j < U; ++j)
                         if (c[j] >= 0)
-                            acc += v[j] * xv[j];
+                            acc += v[j] * (c[j] == int(i) ? xv[j] : xv[j] - xi);
                 }
             }
             if (all) {
@@ -930,6 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_bgm_jacobi(BArgs a, const float* __r
 struct BCsr
 {
     int nr;
+    int diff; // square operator in the FP32 difference form (k_bspmm)
     const int* rp;
     const int* ci;
     const float* v;
@@ -970,20 +978,25 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(BCsr A, int W, int mode,
     for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
         const int i = int(q / W), s = int(q - int64_t(i) * W);
         const int kb = A.rp[i], ke = A.rp[i + 1];
+        const float xi = A.diff ? din[q] : 0.f;
+        auto term = [&](int c, float v) {
+            const float xc = din[size_t(c) * W + s];
+            return v * (c == i ? xc : xc - xi);
+        };
         float acc = 0.f;
         int k = kb;
         for (; k + 4 <= ke; k += 4) {
             const int c0 = A.ci[k], c1 = A.ci[k + 1], c2 = A.ci[k + 2], c3 = A.ci[k + 3];
             const float v0 = A.v[k], v1 = A.v[k + 1], v2 = A.v[k + 2], v3 = A.v[k + 3];
-            const float x0 = din[size_t(c0) * W + s], x1 = din[size_t(c1) * W + s];
-            const float x2 = din[size_t(c2) * W + s], x3 = din[size_t(c3) * W + s];
-            acc += v0 * x0;
-            acc += v1 * x1;
-            acc += v2 * x2;
-            acc += v3 * x3;
+            const float t0 = term(c0, v0), t1 = term(c1, v1), t2 = term(c2, v2);
+            const float t3 = term(c3, v3);
+            acc += t0;
+            acc += t1;
+            acc += t2;
+            acc += t3;
         }
         for (; k < ke; ++k)
-            acc += A.v[k] * din[size_t(A.ci[k]) * W + s];
+            acc += term(A.ci[k], A.v[k]);
         if (mode <= 1) {
             const float ri = r[q] - acc;
             r[q] = ri;
@@ -1592,6 +1605,25 @@ __global__ __launch_bounds__(kBlock) void k_bsell32(int64_t m, const int* __rest
     }
 }
 
+// The FP32 difference form's diagonal: v32 at the slot of a_ii (vpos: SELL
+// slot of each CSR entry, null for CSR order) = the FP64 sum of row i.
+__global__ __launch_bounds__(kBlock) void k_bdiffdiag(int n, const int* __restrict__ rp,
+                                                      const int* __restrict__ diag,
+                                                      const double* __restrict__ v64,
+                                                      const int* __restrict__ vpos,
+                                                      float* __restrict__ v32)
+{
+#pragma clang fp contract(off)
+    const int i = int(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    double sum = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+        sum += v64[k];
+    const int k = diag[i];
+    v32[vpos ? vpos[k] : k] = float(sum);
+}
+
 // invD_e[i] = 1 / a_ii (1 when missing or zero: LinearSolvers.cpp:54-65)
 __global__ __launch_bounds__(kBlock) void k_binvdiag(int n, const int* __restrict__ diag,
                                                      const double* __restrict__ val,
@@ -1667,6 +1699,7 @@ struct mi_batch
     std::shared_ptr<Pattern> pat; // working pattern: src, or src reordered (RCM / two colours)
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
+    bool diff = false; // FP32 values in the difference form (every row holds its diagonal)
     int E = 0, Wmax = 1;
     int G = 8;    // workgroups of the batched SpMM and ILU sweeps (autotuned)
     int Vmax = 4; // cap on slots per thread of those launches (autotuned)
@@ -1699,7 +1732,7 @@ struct mi_batch
     // of the coarsest operator
     struct ALevel
     {
-        int n = 0, deg = 1;
+        int n = 0, deg = 1, diff = 0;
         DevBuf<int> arp, aci, prp, pci, rrp, rci;
         DevBuf<float> av, pv, rv, invD;
         DevBuf<float> r, d, d2, x, b, e;
@@ -1742,6 +1775,7 @@ struct mi_batch
         a.eci = sell ? pat->sell.eci.p : nullptr;
         a.shared = shared ? 1 : 0;
         a.band = band;
+        a.diff = diff ? 1 : 0;
         if (shared) {
             a.v32 = v32h.p;
             a.invD = pc == MI_PC_ILU ? ipivh.p : pc == MI_PC_NONE ? ones.p : invDh.p;
@@ -1957,10 +1991,23 @@ int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
     hipStream_t s = B->stream;
     B->amg.clear();
     B->amg.resize(size_t(nl));
-    auto up = [&](const HCsr& h, DevBuf<int>& rp, DevBuf<int>& ci, DevBuf<float>& v) -> int {
+    // diff: the FP32 difference form of a square operator whose rows all hold
+    // their diagonal (the FP64 row sum in the diagonal's place)
+    auto up = [&](const HCsr& h, DevBuf<int>& rp, DevBuf<int>& ci, DevBuf<float>& v,
+                  bool diff) -> int {
         std::vector<float> f(h.v.size());
         for (size_t k = 0; k < f.size(); ++k)
             f[k] = float(h.v[k]);
+        for (int i = 0; diff && i < h.nr; ++i) {
+            double sum = 0.0;
+            int dk = -1;
+            for (int k = h.rp[size_t(i)]; k < h.rp[size_t(i) + 1]; ++k) {
+                sum += h.v[size_t(k)];
+                if (h.ci[size_t(k)] == i)
+                    dk = k;
+            }
+            f[size_t(dk)] = float(sum);
+        }
         MI_TRY(rp.upload(h.rp.data(), h.rp.size(), s));
         MI_TRY(ci.upload(h.ci.data(), std::max<size_t>(h.ci.size(), 1), s));
         MI_TRY(v.upload(f.data(), std::max<size_t>(f.size(), 1), s));
@@ -1972,10 +2019,18 @@ int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
         mi_batch::ALevel& d = B->amg[size_t(l)];
         d.n = h.A.nr;
         d.deg = l == 0 ? deg0 : degc;
-        MI_TRY(up(h.A, d.arp, d.aci, d.av));
+        bool all_diag = true;
+        for (int i = 0; i < h.A.nr && all_diag; ++i) {
+            bool has = false;
+            for (int k = h.A.rp[size_t(i)]; k < h.A.rp[size_t(i) + 1]; ++k)
+                has = has || h.A.ci[size_t(k)] == i;
+            all_diag = has;
+        }
+        d.diff = all_diag ? 1 : 0;
+        MI_TRY(up(h.A, d.arp, d.aci, d.av, all_diag));
         if (l + 1 < nl) {
-            MI_TRY(up(h.P, d.prp, d.pci, d.pv));
-            MI_TRY(up(h.R, d.rrp, d.rci, d.rv));
+            MI_TRY(up(h.P, d.prp, d.pci, d.pv, false));
+            MI_TRY(up(h.R, d.rrp, d.rci, d.rv, false));
         }
         std::vector<float> inv(size_t(std::max(d.n, 1)), 1.f);
         for (int i = 0; i < d.n; ++i)
@@ -2006,16 +2061,15 @@ int amg_apply_batch(mi_batch* B, int W, const float* in, float* out, Launcher& L
         return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows * W + kBlock - 1) / kBlock,
                                                                16384)));
     };
-    auto csr = [&](const DevBuf<int>& rp, const DevBuf<int>& ci, const DevBuf<float>& v, int nr) {
-        return BCsr{nr, rp.p, ci.p, v.p};
-    };
+    auto csr = [&](const DevBuf<int>& rp, const DevBuf<int>& ci, const DevBuf<float>& v, int nr,
+                   int diff = 0) { return BCsr{nr, diff, rp.p, ci.p, v.p}; };
     auto bptr = [&](int l) -> const float* { return l == 0 ? in : B->amg[size_t(l)].b.p; };
     auto xptr = [&](int l) -> float* { return l == 0 ? out : B->amg[size_t(l)].x.p; };
     const float* cnull = nullptr;
     float* fnull = nullptr;
     for (int l = 0; l + 1 < nl; ++l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
-        const BCsr A = csr(d.arp, d.aci, d.av, d.n);
+        const BCsr A = csr(d.arp, d.aci, d.av, d.n, d.diff);
         float* x = xptr(l);
         MI_TRY(L(k_bamg_init, grid(d.n), kBlock, int64_t(d.n), W, bptr(l),
                  static_cast<const float*>(d.invD.p), float(d.alpha[0]), x, d.r.p, d.d.p));
@@ -2039,7 +2093,7 @@ int amg_apply_batch(mi_batch* B, int W, const float* in, float* out, Launcher& L
                  static_cast<const double*>(B->amg_cinv.p), bptr(nl - 1), xptr(nl - 1)));
     for (int l = nl - 2; l >= 0; --l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
-        const BCsr A = csr(d.arp, d.aci, d.av, d.n);
+        const BCsr A = csr(d.arp, d.aci, d.av, d.n, d.diff);
         float* x = xptr(l);
         MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, csr(d.prp, d.pci, d.pv, d.n), W, 3,
                  static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f, 0.f));
@@ -2602,6 +2656,9 @@ int write_matrix(mi_batch* B, int e, const mi_csr* A)
     if (B->sell && B->vrows)
         k_bsell32<<<reduce_grid(B->vrows, 4), kBlock, 0, s>>>(B->vrows, B->pat->sell.eperm.p, v64,
                                                               v32);
+    if (B->n && B->diff)
+        k_bdiffdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
+            int(B->n), B->pat->rp.p, B->pat->diag.p, v64, B->sell ? B->d_vpos.p : nullptr, v32);
     if (B->n)
         k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
             int(B->n), B->pat->diag.p, v64, B->invDh.p + size_t(e) * size_t(B->n));
@@ -2661,6 +2718,14 @@ MI_API int mi_batch_create_ex(const mi_csr* A, int n_eq, int flags, mi_batch** o
     MI_HIP(hipHostMalloc(reinterpret_cast<void**>(&B->h_st), sizeof(BEq) * kMaxEq, 0));
     B->sell = B->pat->sell.built() && B->pat->sell.eperm.p != nullptr;
     B->vrows = B->sell ? B->pat->sell.padded : B->nnz;
+    B->diff = B->pat->all_diag;
+    if (B->sell && (B->diff || B->ilu_order)) {
+        // the SELL-64 slot of every CSR entry
+        MI_TRY(B->d_vpos.alloc(size_t(std::max<int64_t>(B->nnz, 1))));
+        k_bslot_of<<<reduce_grid(B->vrows, 4), kBlock, 0, B->stream>>>(
+            B->vrows, B->pat->sell.eperm.p, B->d_vpos.p);
+        MI_HIP(hipGetLastError());
+    }
     const size_t nE = size_t(std::max<int64_t>(B->n, 1)) * size_t(n_eq);
     const size_t nM = size_t(std::max<int64_t>(B->n, 1)) * size_t(B->Em);
     const size_t zM = size_t(std::max<int64_t>(B->nnz, 1)) * size_t(B->Em);
@@ -2686,12 +2751,7 @@ MI_API int mi_batch_create_ex(const mi_csr* A, int n_eq, int flags, mi_batch** o
         MI_HIP(hipMemsetAsync(B->lvh.p, 0, vM * sizeof(float), B->stream));
         MI_HIP(hipMemsetAsync(B->ipivh.p, 0, nM * sizeof(float), B->stream));
         MI_TRY(B->d_tpos.upload(tpos.data(), std::max<size_t>(tpos.size(), 1), B->stream));
-        if (B->sell) {
-            MI_TRY(B->d_vpos.alloc(size_t(std::max<int64_t>(B->nnz, 1))));
-            k_bslot_of<<<reduce_grid(B->vrows, 4), kBlock, 0, B->stream>>>(
-                B->vrows, B->pat->sell.eperm.p, B->d_vpos.p);
-            MI_HIP(hipGetLastError());
-        } else {
+        if (!B->sell) {
             std::vector<int> id(size_t(std::max<int64_t>(B->nnz, 1)));
             for (size_t k = 0; k < id.size(); ++k)
                 id[k] = int(k);
