@@ -346,6 +346,14 @@ MI_API int mi_batch_tuning(const mi_batch* b, int* rows_per_block, double* spmm_
 MI_API int mi_batch_tuning_shape(const mi_batch* b, int* slots_per_thread_max,
                                  int* spmm_workgroups, int* banded, int* vector_workgroups,
                                  double* vector_us);
+/* Replaces the autotuner's SpMM shape (the three values mi_batch_tuning_shape
+ * reports first): the shape fixes the order in which the dot products'
+ * partial sums are added, so two batches given the same shape, matrix and
+ * right-hand sides return the same bits, where the timed choice may differ
+ * from batch to batch.  slots_per_thread_max 1, 2 or 4; spmm_workgroups a
+ * multiple of 8 up to 4096; banded 0 or 1. */
+MI_API int mi_batch_set_shape(mi_batch* b, int slots_per_thread_max, int spmm_workgroups,
+                              int banded);
 /* Cost of a grid-wide barrier on this device: microseconds per cooperative
  * grid.sync() in a kernel of `workgroups` x 256 threads (each thread updates
  * one element between barriers), and per kernel boundary for the same work

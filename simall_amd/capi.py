@@ -194,6 +194,7 @@ PROTOTYPES = {
     "mi_batch_set_equation": (_i, [_vp, _i, _vp]),
     "mi_batch_set_matrix": (_i, [_vp, _vp]),
     "mi_batch_apply_preconditioner": (_i, [_vp, _i, _vp, _vp]),
+    "mi_batch_set_shape": (_i, [_vp, _i, _i, _i]),
     "mi_batch_tuning_shape": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i),
                                    C.POINTER(_i), C.POINTER(_d)]),
     "mi_time_grid_barrier": (_i, [_i, _i, C.POINTER(_d), C.POINTER(_d)]),
@@ -857,6 +858,11 @@ class Batch:
         return {"rows_per_block": rb.value, "spmm_us": us.value, "spmm_bytes": by.value,
                 "slots_per_thread_max": v.value, "spmm_workgroups": g.value,
                 "banded": bd.value, "vector_workgroups": gv.value, "vector_us": vus.value}
+
+    def set_shape(self, slots_per_thread_max, spmm_workgroups, banded):
+        """Pins the SpMM shape (mi_batch_set_shape): same shape, same bits."""
+        check(self.lib.mi_batch_set_shape(self.h, int(slots_per_thread_max),
+                                          int(spmm_workgroups), int(banded)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -64,6 +64,16 @@ constexpr int kMaxEq = 64;
 constexpr int kWaves = kBlock / kWave;
 constexpr int kBatchMaxGrid = 4096;
 constexpr int kSpmmUnroll = 8; // nonzeros of a row loaded ahead of their gathers
+// Operators whose mean diagonal-dominance margin (|a_ii| - sum_j!=i |a_ij|)
+// / |a_ii| is below this sum the sweep's correction in FP64: rows that
+// mostly sum to zero (a conservative diffusion operator without a time term,
+// the pressure equation) make a nearly singular operator, on which FP32
+// rounding of the correction costs the refinement sweeps iterations (the
+// 128^3 pressure batch: 55 against 66); an operator with a time term keeps
+// the FP32 correction, 8 bytes less per row and equation in each update
+// (the 100^3 transport batch, mean margins 0.03 and up: 1270 against 1259
+// iterations for its hardest equation, 7 % less time).
+constexpr double kX64Margin = 1e-3;
 
 struct BEq
 {
@@ -105,6 +115,7 @@ struct BArgs
     const float* lv;   // ILU(0) lower factor l_ik at the entries of the second colour's rows
     const int* eq;     // [W] equation of each slot, -1: empty
     float* xs;         // [n][W] correction of the current sweep
+    double* xs64;      // the same in FP64 when not null (weakly dominant operators)
     float* r;          // BiCGStab: r, then s in place
     float* p;
     float* Ap;         // BiCGStab: v
@@ -207,6 +218,19 @@ template <> struct SlotVec<1>
     using type = float;
     static __device__ __forceinline__ float at(float x, int) { return x; }
     static __device__ __forceinline__ void set(float& x, int, float v) { x = v; }
+};
+// The same V slots in FP64 (the sweep's correction).
+template <int V> struct SlotVec64
+{
+    using type = double __attribute__((ext_vector_type(V)));
+    static __device__ __forceinline__ void set(type& x, int u, double v) { x[u] = v; }
+    static __device__ __forceinline__ double at(const type& x, int u) { return x[u]; }
+};
+template <> struct SlotVec64<1>
+{
+    using type = double;
+    static __device__ __forceinline__ void set(double& x, int, double v) { x = v; }
+    static __device__ __forceinline__ double at(double x, int) { return x; }
 };
 
 // FP64 sums of up to two values per slot over a workgroup whose threads
@@ -414,11 +438,17 @@ template <int OP> BSpmmKernel<OP> bspmm_kernel(bool sell, bool shared, int W, in
 //   B_BI_SN     s = r - alpha v (in r); s.s
 // Threads take V consecutive slots of a row (V-wide loads and stores, like
 // the SpMM): W / V threads per row, R = kBlock V / W rows per pass; a slot of
-// an equation not iterating keeps its values.
+// an equation not iterating keeps its values.  With a.xs64 the sweep's
+// correction is summed in FP64 with the step lengths in FP64 (the products
+// of FP32 directions exact, as the single-equation solver's FP64 iterate):
+// rounding it to FP32 leaves a residual of order kappa * 6e-8, which on a
+// weakly dominant (ill-conditioned) operator costs the refinement sweeps
+// iterations; a strongly dominant one keeps the FP32 correction (a.xs).
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
 {
     using VT = typename SlotVec<V>::type;
+    using DT = typename SlotVec64<V>::type;
     const int W = a.W;
     const int TW = W / V;
     const int R = kBlock / TW;
@@ -429,16 +459,20 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
     bool on[V], keep[V];
     bool any = false, all = true, allk = true;
     VT al = VT(0.f), be = VT(0.f), om = VT(0.f);
+    double al64[V], om64[V];
 #pragma unroll
     for (int u = 0; u < V; ++u) {
         const int e = live ? a.eq[s0 + u] : -1;
         on[u] = slot_on(a, op, e);
         keep[u] = on[u];
+        al64[u] = om64[u] = 0.0;
         if (on[u]) {
             const BEq& st = a.st[e];
             SlotVec<V>::set(al, u, float(st.alpha));
             SlotVec<V>::set(be, u, float(st.beta));
             SlotVec<V>::set(om, u, float(st.omega));
+            al64[u] = st.alpha;
+            om64[u] = st.omega;
             if (op == B_BI_UPDATE && st.sconv)
                 keep[u] = false; // s met the tolerance: only x moves (:387-390)
         }
@@ -468,6 +502,26 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
                 }
             };
             auto dinv = [&]() { return sh ? VT(a.invD[i]) : ld(a.invD); };
+            // xs64 += dx for the slots in m
+            auto xs_add = [&](const DT& dx, const bool (&m)[V], bool every) {
+                double* px = a.xs64 + q;
+                if (every) {
+                    *reinterpret_cast<DT*>(px) = *reinterpret_cast<const DT*>(px) + dx;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < V; ++u)
+                        if (m[u])
+                            px[u] += SlotVec64<V>::at(dx, u);
+                }
+            };
+            // alpha p in FP64
+            auto step64 = [&](const VT& pv) {
+                DT dx;
+#pragma unroll
+                for (int u = 0; u < V; ++u)
+                    SlotVec64<V>::set(dx, u, al64[u] * double(SlotVec<V>::at(pv, u)));
+                return dx;
+            };
             auto acc = [&](double (&d)[V], const VT& x, const VT& y, const bool (&m)[V]) {
 #pragma unroll
                 for (int u = 0; u < V; ++u)
@@ -480,7 +534,10 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
                 acc(d0, r, z, on);
                 acc(d1, r, r, on);
             } else if (op == B_UPDATE) {
-                put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
+                if (a.xs64)
+                    xs_add(step64(ld(a.p)), on, all);
+                else
+                    put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
                 const VT r = ld(a.r) - al * ld(a.Ap);
                 put(a.r, r, on, all);
                 const VT z = dinv() * r;
@@ -489,7 +546,10 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
             } else if (op == B_P) {
                 put(a.p, dinv() * ld(a.r) + be * ld(a.p), on, all);
             } else if (op == B_UPD_X) {
-                put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
+                if (a.xs64)
+                    xs_add(step64(ld(a.p)), on, all);
+                else
+                    put(a.xs, ld(a.xs) + al * ld(a.p), on, all);
                 put(a.r, ld(a.r) - al * ld(a.Ap), on, all);
             } else if (op == B_DOT_RZ) {
                 const VT r = ld(a.r), z = ld(a.z);
@@ -524,19 +584,24 @@ __global__ __launch_bounds__(kBlock) void k_bvec(BArgs a, int op)
                     put(a.z, dinv() * sv, on, all);
                 acc(d0, sv, sv, on);
             } else { // B_BI_UPDATE (:387-397)
-                const VT y = ld(a.y);
-                VT xs = ld(a.xs) + al * y;
-                if (!allk) {
-                    // slots whose s met the tolerance take x += alpha y alone
-                    const VT z = ld(a.z);
+                // slots whose s met the tolerance take x += alpha y alone
+                const VT y = ld(a.y), z = ld(a.z);
+                if (a.xs64) {
+                    DT dx = step64(y);
+#pragma unroll
+                    for (int u = 0; u < V; ++u)
+                        if (keep[u])
+                            SlotVec64<V>::set(dx, u, SlotVec64<V>::at(dx, u) +
+                                                         om64[u] * double(SlotVec<V>::at(z, u)));
+                    xs_add(dx, on, all);
+                } else {
+                    VT xs = ld(a.xs) + al * y;
 #pragma unroll
                     for (int u = 0; u < V; ++u)
                         if (keep[u])
                             SlotVec<V>::set(xs, u, SlotVec<V>::at(xs, u) +
                                                        SlotVec<V>::at(om, u) * SlotVec<V>::at(z, u));
                     put(a.xs, xs, on, all);
-                } else {
-                    put(a.xs, xs + om * ld(a.z), on, all);
                 }
                 bool anyk = false;
 #pragma unroll
@@ -899,12 +964,15 @@ __global__ __launch_bounds__(kBlock) void k_bgm_xupdate(BArgs a, BGm g)
     block_range(a.n, P, i0, i1);
     for (int64_t i = i0 + rr; i < i1; i += P) {
         const size_t q = size_t(i) * W + sl;
-        double xv = double(a.xs[q]);
+        double xv = a.xs64 ? a.xs64[q] : double(a.xs[q]);
 #pragma unroll
         for (int k = 0; k < kBGmMax; ++k)
             if (k < jc)
                 xv += y[k] * double(g.V[size_t(k) * g.vs + q]);
-        a.xs[q] = float(xv);
+        if (a.xs64)
+            a.xs64[q] = xv;
+        else
+            a.xs[q] = float(xv);
     }
 }
 
@@ -934,13 +1002,14 @@ __global__ __launch_bounds__(kBlock) void k_bgm_jacobi(BArgs a, const float* __r
 // residual, the dense FP64 coarsest solve, prolongation and post-smoothing)
 // on plain FP32 CSR operators, each streamed once per launch for all W
 // right-hand sides.  Vectors of level l are [n_l][W] lines.
-struct BCsr
+struct BSell
 {
     int nr;
-    int diff; // square operator in the FP32 difference form (k_bspmm)
-    const int* rp;
-    const int* ci;
-    const float* v;
+    int diff; // square operator in the FP32 difference form (kSellDiag marks a_ii's slot)
+    int band; // row walk (row_walk)
+    const int* soff;  // slice offsets, 64 rows a slice
+    const int* eci;   // column per slot, kSellDiag on a diagonal, -1 padding
+    const float* v;   // value per slot
 };
 
 // r = b, d = a0 D^-1 b, x = d over n W elements
@@ -960,12 +1029,18 @@ __global__ __launch_bounds__(kBlock) void k_bamg_init(int64_t n, int W, const fl
     }
 }
 
-// One thread per (row, slot), rows of the operator A over lines of W:
+// One launch of the batched V-cycle over the rows of A and lines of W
+// slots, in the SpMM's thread layout: TW = W / V threads per row, each
+// taking V consecutive slots (V-wide loads of the gathered rows of din, one
+// load of a nonzero for all of them); every operator in SELL-64 slot order
+// (slot k of row i at soff[i / 64] + 64 k + i % 64, CSR order within the
+// row), so that the threads of a wave load consecutive slots:
 //   mode 0 (Chebyshev step): r -= A din; dout = beta din + alpha D^-1 r; x += dout
 //   mode 1 (residual):       r -= A din
 //   mode 2 (restriction):    y = A din
 //   mode 3 (prolongation):   y = A din; x += y
-__global__ __launch_bounds__(kBlock) void k_bamg_rows(BCsr A, int W, int mode,
+template <int V>
+__global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
                                                       const float* __restrict__ din,
                                                       float* __restrict__ dout,
                                                       float* __restrict__ r,
@@ -973,44 +1048,67 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(BCsr A, int W, int mode,
                                                       float* __restrict__ x,
                                                       float* __restrict__ y, float beta, float alpha)
 {
-    const int64_t m = int64_t(A.nr) * W;
-    const int64_t stride = int64_t(gridDim.x) * kBlock;
-    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
-        const int i = int(q / W), s = int(q - int64_t(i) * W);
-        const int kb = A.rp[i], ke = A.rp[i + 1];
-        const float xi = A.diff ? din[q] : 0.f;
-        auto term = [&](int c, float v) {
-            const float xc = din[size_t(c) * W + s];
-            return v * (c == i ? xc : xc - xi);
-        };
-        float acc = 0.f;
-        int k = kb;
-        for (; k + 4 <= ke; k += 4) {
-            const int c0 = A.ci[k], c1 = A.ci[k + 1], c2 = A.ci[k + 2], c3 = A.ci[k + 3];
-            const float v0 = A.v[k], v1 = A.v[k + 1], v2 = A.v[k + 2], v3 = A.v[k + 3];
-            const float t0 = term(c0, v0), t1 = term(c1, v1), t2 = term(c2, v2);
-            const float t3 = term(c3, v3);
-            acc += t0;
-            acc += t1;
-            acc += t2;
-            acc += t3;
+    using VT = typename SlotVec<V>::type;
+    constexpr int U = V == 1 ? kSpmmUnroll : kSpmmUnroll / 2; // row entries per pass
+    const int TW = W / V;
+    const int R = kBlock / TW;
+    const int t = int(threadIdx.x);
+    if (t >= R * TW) // no barrier below
+        return;
+    const int grp = t / TW;
+    const int s0 = (t % TW) * V;
+    const RowWalk rw = row_walk(0, A.nr, R, A.band);
+    auto ld = [&](const float* p, size_t q) { return *reinterpret_cast<const VT*>(p + q); };
+    auto st = [&](float* p, size_t q, VT v) { *reinterpret_cast<VT*>(p + q) = v; };
+    for (int64_t pass = rw.base; pass < rw.lim; pass += rw.stride) {
+        const int64_t i = pass + grp;
+        if (i >= rw.lim)
+            break;
+        const size_t q = size_t(i) * W + s0;
+        const VT xi = A.diff ? ld(din, q) : VT(0.f);
+        VT acc = VT(0.f);
+        const int slc = int(i / kSellSlice);
+        const int m0 = A.soff[slc] + int(i % kSellSlice);
+        const int w = (A.soff[slc + 1] - A.soff[slc]) / kSellSlice;
+        for (int k0 = 0; k0 < w; k0 += U) {
+            int c[U];
+            float v[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j) {
+                const size_t mk = size_t(m0) + size_t(k0 + j) * kSellSlice;
+                const bool in = k0 + j < w;
+                c[j] = in ? A.eci[mk] : -1;
+                v[j] = in ? A.v[mk] : 0.f;
+            }
+            VT xv[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j)
+                xv[j] = c[j] >= 0 ? ld(din, size_t(c[j] & kSellColMask) * W + s0) : VT(0.f);
+#pragma unroll
+            for (int j = 0; j < U; ++j)
+                if (c[j] >= 0)
+                    acc += VT(v[j]) * ((c[j] & kSellDiag) ? xv[j] : xv[j] - xi);
         }
-        for (; k < ke; ++k)
-            acc += term(A.ci[k], A.v[k]);
         if (mode <= 1) {
-            const float ri = r[q] - acc;
-            r[q] = ri;
+            const VT ri = ld(r, q) - acc;
+            st(r, q, ri);
             if (mode == 0) {
-                const float dn = beta * din[q] + alpha * invD[i] * ri;
-                dout[q] = dn;
-                x[q] += dn;
+                const VT dn = VT(beta) * ld(din, q) + VT(alpha * invD[i]) * ri;
+                st(dout, q, dn);
+                st(x, q, ld(x, q) + dn);
             }
         } else {
-            y[q] = acc;
+            st(y, q, acc);
             if (mode == 3)
-                x[q] += acc;
+                st(x, q, ld(x, q) + acc);
         }
     }
+}
+using BAmgRows = void (*)(int, BSell, int, const float*, float*, float*, const float*, float*,
+                          float*, float, float);
+inline BAmgRows bamg_rows_kernel(int V)
+{
+    return V == 4 ? k_bamg_rows<4> : V == 2 ? k_bamg_rows<2> : k_bamg_rows<1>;
 }
 
 // x = Ainv b on the coarsest level (FP64 dense), one workgroup per row:
@@ -1507,12 +1605,12 @@ __global__ __launch_bounds__(kBlock) void k_bfin_eq(const double* __restrict__ p
 //   mode 1: home[eq[s] * rows + i] += H(slot[i W + s]) for the slots set in
 //           `mask`
 constexpr int kTileRows = 64;
-template <class H>
+template <class H, class T>
 __global__ __launch_bounds__(kBlock) void k_btile(int64_t rows, int W, const int* __restrict__ eq,
                                                   uint64_t mask, int mode, int use_fill,
-                                                  float fill, H* home, float* slot)
+                                                  float fill, H* home, T* slot)
 {
-    __shared__ float tile[kTileRows * kMaxEq];
+    __shared__ T tile[kTileRows * kMaxEq];
     const int64_t ntile = (rows + kTileRows - 1) / kTileRows;
     for (int64_t tb = blockIdx.x; tb < ntile; tb += gridDim.x) {
         const int64_t i0 = tb * kTileRows;
@@ -1522,9 +1620,9 @@ __global__ __launch_bounds__(kBlock) void k_btile(int64_t rows, int W, const int
             for (int t = threadIdx.x; t < kTileRows * W; t += kBlock) {
                 const int sl = t / kTileRows, r = t % kTileRows;
                 const int e = eq[sl];
-                float v = fill;
+                T v = T(fill);
                 if (r < nr && e >= 0 && !use_fill)
-                    v = float(home[size_t(e) * rows + i0 + r]);
+                    v = T(home[size_t(e) * rows + i0 + r]);
                 tile[r * W + sl] = v;
             }
             __syncthreads();
@@ -1546,10 +1644,11 @@ __global__ __launch_bounds__(kBlock) void k_btile(int64_t rows, int W, const int
 }
 
 // Narrower slot layout from a wider one: dst[i W2 + s] = src[i W1 + map[s]].
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_brepack(int64_t rows, int W1, int W2,
                                                     const int* __restrict__ map,
-                                                    const float* __restrict__ src,
-                                                    float* __restrict__ dst)
+                                                    const T* __restrict__ src,
+                                                    T* __restrict__ dst)
 {
     const int64_t m = rows * W2;
     const int64_t stride = int64_t(gridDim.x) * kBlock;
@@ -1624,6 +1723,36 @@ __global__ __launch_bounds__(kBlock) void k_bdiffdiag(int n, const int* __restri
     v32[vpos ? vpos[k] : k] = float(sum);
 }
 
+// Sum of the row margins (|a_ii| - sum_j!=i |a_ij|) / |a_ii| (-1 for a zero
+// or missing diagonal) of each workgroup's rows: part[blockIdx].
+__global__ __launch_bounds__(kBlock) void k_bmargin(int n, const int* __restrict__ rp,
+                                                    const int* __restrict__ ci,
+                                                    const double* __restrict__ v,
+                                                    double* __restrict__ part)
+{
+    __shared__ double lds[kBlock];
+    double m = 0.0;
+    for (int i = int(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int(gridDim.x) * kBlock) {
+        double d = 0.0, off = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            if (ci[k] == i)
+                d = fabs(v[k]);
+            else
+                off += fabs(v[k]);
+        }
+        m += d > 0.0 ? (d - off) / d : -1.0;
+    }
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h /= 2) {
+        if (int(threadIdx.x) < h)
+            lds[threadIdx.x] += lds[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        part[blockIdx.x] = lds[0];
+}
+
 // invD_e[i] = 1 / a_ii (1 when missing or zero: LinearSolvers.cpp:54-65)
 __global__ __launch_bounds__(kBlock) void k_binvdiag(int n, const int* __restrict__ diag,
                                                      const double* __restrict__ val,
@@ -1691,6 +1820,7 @@ struct BSlots
     std::vector<int> eqs; // equation of each slot, -1 empty (size W)
     DevBuf<int> d_eq;
     DevBuf<float> v32, invD, lv, xs, r, p, Ap, rh, y, z, t;
+    DevBuf<double> xs64; // the sweep's correction in FP64 (weakly dominant operators)
 };
 
 struct mi_batch
@@ -1700,6 +1830,10 @@ struct mi_batch
     DevBuf<int> d_perm, d_vmap;   // perm[new] = old; vmap[k] = source entry of entry k
     bool reordered = false;
     bool diff = false; // FP32 values in the difference form (every row holds its diagonal)
+    // mean diagonal-dominance margin (|a_ii| - sum_j!=i |a_ij|) / |a_ii| of
+    // each matrix slot; below kX64Margin the sweeps sum their correction in FP64
+    std::vector<double> margin;
+    bool x64 = false;
     int E = 0, Wmax = 1;
     int G = 8;    // workgroups of the batched SpMM and ILU sweeps (autotuned)
     int Vmax = 4; // cap on slots per thread of those launches (autotuned)
@@ -1733,7 +1867,7 @@ struct mi_batch
     struct ALevel
     {
         int n = 0, deg = 1, diff = 0;
-        DevBuf<int> arp, aci, prp, pci, rrp, rci;
+        DevBuf<int> asoff, aeci, psoff, peci, rsoff, reci; // SELL-64 forms
         DevBuf<float> av, pv, rv, invD;
         DevBuf<float> r, d, d2, x, b, e;
         std::vector<double> alpha, beta;
@@ -1787,6 +1921,7 @@ struct mi_batch
         }
         a.eq = S.d_eq.p;
         a.xs = S.xs.p;
+        a.xs64 = x64 ? S.xs64.p : nullptr;
         a.r = S.r.p;
         a.p = S.p.p;
         a.Ap = S.Ap.p;
@@ -1839,6 +1974,10 @@ int alloc_slots(mi_batch* B, BSlots& S, bool ext)
         for (DevBuf<float>* v : {&S.xs, &S.r, &S.p, &S.Ap})
             MI_TRY(v->alloc(nW));
         MI_TRY(S.d_eq.alloc(size_t(kMaxEq)));
+    }
+    if (B->x64 && !S.xs64.p) {
+        MI_TRY(S.xs64.alloc(nW));
+        MI_HIP(hipMemsetAsync(S.xs64.p, 0, nW * sizeof(double), B->stream));
     }
     if (ext && !S.rh.p)
         for (DevBuf<float>* v : {&S.rh, &S.y, &S.z, &S.t})
@@ -1896,8 +2035,12 @@ int retire(mi_batch* B, uint64_t mask)
     const BSlots& S = B->set[B->cur];
     if (!mask || B->n == 0)
         return MI_OK;
-    k_btile<double><<<tile_grid(B->n), kBlock, 0, B->stream>>>(B->n, S.W, S.d_eq.p, mask, 1, 0,
-                                                               0.f, B->x.p, S.xs.p);
+    if (B->x64)
+        k_btile<double><<<tile_grid(B->n), kBlock, 0, B->stream>>>(B->n, S.W, S.d_eq.p, mask, 1,
+                                                                   0, 0.f, B->x.p, S.xs64.p);
+    else
+        k_btile<double><<<tile_grid(B->n), kBlock, 0, B->stream>>>(B->n, S.W, S.d_eq.p, mask, 1,
+                                                                   0, 0.f, B->x.p, S.xs.p);
     MI_HIP(hipGetLastError());
     return MI_OK;
 }
@@ -1920,7 +2063,7 @@ int repack(mi_batch* B, const std::vector<int>& keep, bool ext)
     DevBuf<int> d_map;
     MI_TRY(d_map.upload(map.data(), map.size(), B->stream));
     hipStream_t s = B->stream;
-    auto cp = [&](int64_t rows, const DevBuf<float>& from, DevBuf<float>& to) {
+    auto cp = [&](int64_t rows, const auto& from, auto& to) {
         if (rows > 0)
             k_brepack<<<reduce_grid(rows * S.W, 4), kBlock, 0, s>>>(rows, A.W, S.W, d_map.p,
                                                                     from.p, to.p);
@@ -1931,7 +2074,10 @@ int repack(mi_batch* B, const std::vector<int>& keep, bool ext)
         if (B->pc == MI_PC_ILU)
             cp(B->vrows, A.lv, S.lv);
     }
-    cp(B->n, A.xs, S.xs);
+    if (B->x64)
+        cp(B->n, A.xs64, S.xs64);
+    else
+        cp(B->n, A.xs, S.xs);
     cp(B->n, A.r, S.r);
     cp(B->n, A.p, S.p);
     cp(B->n, A.Ap, S.Ap);
@@ -1991,27 +2137,48 @@ int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
     hipStream_t s = B->stream;
     B->amg.clear();
     B->amg.resize(size_t(nl));
-    // diff: the FP32 difference form of a square operator whose rows all hold
-    // their diagonal (the FP64 row sum in the diagonal's place)
-    auto up = [&](const HCsr& h, DevBuf<int>& rp, DevBuf<int>& ci, DevBuf<float>& v,
-                  bool diff) -> int {
-        std::vector<float> f(h.v.size());
-        for (size_t k = 0; k < f.size(); ++k)
-            f[k] = float(h.v[k]);
-        for (int i = 0; diff && i < h.nr; ++i) {
-            double sum = 0.0;
-            int dk = -1;
-            for (int k = h.rp[size_t(i)]; k < h.rp[size_t(i) + 1]; ++k) {
-                sum += h.v[size_t(k)];
-                if (h.ci[size_t(k)] == i)
-                    dk = k;
-            }
-            f[size_t(dk)] = float(sum);
+    // SELL-64 form of an operator (slot k of row i at soff[i / 64] + 64 k +
+    // i % 64, CSR order within the row); square: kSellDiag on the diagonal
+    // slots; diff: the FP32 difference form (the FP64 row sum in the
+    // diagonal's place), for square operators whose rows all hold a diagonal
+    auto up = [&](const HCsr& h, DevBuf<int>& soff, DevBuf<int>& eci, DevBuf<float>& v,
+                  bool square, bool diff) -> int {
+        const int nr = h.nr;
+        const int nsl = (nr + kSellSlice - 1) / kSellSlice;
+        std::vector<int> off(size_t(nsl) + 1, 0);
+        for (int sl = 0; sl < nsl; ++sl) {
+            int w = 0;
+            for (int i = sl * kSellSlice; i < std::min(nr, (sl + 1) * kSellSlice); ++i)
+                w = std::max(w, h.rp[size_t(i) + 1] - h.rp[size_t(i)]);
+            off[size_t(sl) + 1] = off[size_t(sl)] + w * kSellSlice;
         }
-        MI_TRY(rp.upload(h.rp.data(), h.rp.size(), s));
-        MI_TRY(ci.upload(h.ci.data(), std::max<size_t>(h.ci.size(), 1), s));
-        MI_TRY(v.upload(f.data(), std::max<size_t>(f.size(), 1), s));
-        MI_HIP(hipStreamSynchronize(s)); // f is released on return
+        const size_t tot = std::max<size_t>(size_t(off[size_t(nsl)]), 1);
+        std::vector<int> c(tot, -1);
+        std::vector<float> f(tot, 0.f);
+        for (int i = 0; i < nr; ++i) {
+            const size_t base = size_t(off[size_t(i / kSellSlice)]) + size_t(i % kSellSlice);
+            double sum = 0.0;
+            size_t dpos = 0;
+            bool has = false;
+            for (int k = h.rp[size_t(i)], j = 0; k < h.rp[size_t(i) + 1]; ++k, ++j) {
+                const size_t m = base + size_t(j) * kSellSlice;
+                const int col = h.ci[size_t(k)];
+                const bool dg = square && col == i;
+                c[m] = dg ? (col | kSellDiag) : col;
+                f[m] = float(h.v[size_t(k)]);
+                sum += h.v[size_t(k)];
+                if (dg) {
+                    dpos = m;
+                    has = true;
+                }
+            }
+            if (diff && has)
+                f[dpos] = float(sum);
+        }
+        MI_TRY(soff.upload(off.data(), off.size(), s));
+        MI_TRY(eci.upload(c.data(), c.size(), s));
+        MI_TRY(v.upload(f.data(), f.size(), s));
+        MI_HIP(hipStreamSynchronize(s)); // the host arrays are released on return
         return MI_OK;
     };
     for (int l = 0; l < nl; ++l) {
@@ -2027,10 +2194,13 @@ int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
             all_diag = has;
         }
         d.diff = all_diag ? 1 : 0;
-        MI_TRY(up(h.A, d.arp, d.aci, d.av, all_diag));
+        // level 0 streams the batch's own SELL-64 operator when it has one;
+        // the coarsest level is solved with its dense inverse
+        if (l + 1 < nl && (l > 0 || !B->sell))
+            MI_TRY(up(h.A, d.asoff, d.aeci, d.av, true, all_diag));
         if (l + 1 < nl) {
-            MI_TRY(up(h.P, d.prp, d.pci, d.pv, false));
-            MI_TRY(up(h.R, d.rrp, d.rci, d.rv, false));
+            MI_TRY(up(h.P, d.psoff, d.peci, d.pv, false, false));
+            MI_TRY(up(h.R, d.rsoff, d.reci, d.rv, false, false));
         }
         std::vector<float> inv(size_t(std::max(d.n, 1)), 1.f);
         for (int i = 0; i < d.n; ++i)
@@ -2052,39 +2222,54 @@ int amg_setup_batch(mi_batch* B, const mi_solver_cfg& c)
     return MI_OK;
 }
 
-// out = V-cycle(in) for every slot of the layout (width W): the launch
-// sequence of DHierarchy::cycle.
-int amg_apply_batch(mi_batch* B, int W, const float* in, float* out, Launcher& L)
+// out = V-cycle(in) for every slot of the layout of a: the launch sequence
+// of DHierarchy::cycle.  Level 0 streams the batch's own SELL-64 operator
+// (the hierarchy's level-0 matrix, in the same FP32 form) when it has one.
+int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
 {
     const int nl = int(B->amg.size());
-    auto grid = [&](int64_t rows) {
+    const int W = a.W;
+    const int V = slots_per_thread(W, B->Vmax);
+    const int R = kBlock / (W / V);
+    const BAmgRows kr = bamg_rows_kernel(V);
+    auto rows_grid = [&](int64_t rows) {
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows + R - 1) / R, 16384)));
+    };
+    auto vec_grid = [&](int64_t rows) {
         return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows * W + kBlock - 1) / kBlock,
                                                                16384)));
     };
-    auto csr = [&](const DevBuf<int>& rp, const DevBuf<int>& ci, const DevBuf<float>& v, int nr,
-                   int diff = 0) { return BCsr{nr, diff, rp.p, ci.p, v.p}; };
+    auto sell = [&](const DevBuf<int>& soff, const DevBuf<int>& eci, const DevBuf<float>& v,
+                    int nr, int diff) { return BSell{nr, diff, 0, soff.p, eci.p, v.p}; };
     auto bptr = [&](int l) -> const float* { return l == 0 ? in : B->amg[size_t(l)].b.p; };
     auto xptr = [&](int l) -> float* { return l == 0 ? out : B->amg[size_t(l)].x.p; };
-    const float* cnull = nullptr;
     float* fnull = nullptr;
+    const float* cnull = nullptr;
+    // r -= A din, and for mode 0 the Chebyshev update, on level l's operator
+    // (level 0: the batch's own, with the SpMM's grid and row walk)
+    auto smooth = [&](int l, int mode, const float* din, float* dout, float* x, double beta,
+                      double alpha) -> int {
+        mi_batch::ALevel& d = B->amg[size_t(l)];
+        const bool own = l == 0 && B->sell;
+        const BSell A = own ? BSell{d.n, a.diff, a.band, a.soff, a.eci, a.v32}
+                            : sell(d.asoff, d.aeci, d.av, d.n, d.diff);
+        return L(kr, own ? unsigned(B->G) : rows_grid(d.n), kBlock, W, A, mode, din, dout, d.r.p,
+                 static_cast<const float*>(d.invD.p), x, fnull, float(beta), float(alpha));
+    };
     for (int l = 0; l + 1 < nl; ++l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
-        const BCsr A = csr(d.arp, d.aci, d.av, d.n, d.diff);
         float* x = xptr(l);
-        MI_TRY(L(k_bamg_init, grid(d.n), kBlock, int64_t(d.n), W, bptr(l),
+        MI_TRY(L(k_bamg_init, vec_grid(d.n), kBlock, int64_t(d.n), W, bptr(l),
                  static_cast<const float*>(d.invD.p), float(d.alpha[0]), x, d.r.p, d.d.p));
         float* din = d.d.p;
         float* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
-            MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(din), dout,
-                     d.r.p, static_cast<const float*>(d.invD.p), x, fnull,
-                     float(d.beta[size_t(k)]), float(d.alpha[size_t(k)])));
+            MI_TRY(smooth(l, 0, din, dout, x, d.beta[size_t(k)], d.alpha[size_t(k)]));
             std::swap(din, dout);
         }
-        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 1, static_cast<const float*>(din), fnull,
-                 d.r.p, static_cast<const float*>(d.invD.p), x, fnull, 0.f, 0.f));
+        MI_TRY(smooth(l, 1, din, nullptr, x, 0.0, 0.0));
         const int nc = B->amg[size_t(l) + 1].n;
-        MI_TRY(L(k_bamg_rows, grid(nc), kBlock, csr(d.rrp, d.rci, d.rv, nc), W, 2,
+        MI_TRY(L(kr, rows_grid(nc), kBlock, W, sell(d.rsoff, d.reci, d.rv, nc, 0), 2,
                  static_cast<const float*>(d.r.p), fnull, fnull, cnull, fnull,
                  B->amg[size_t(l) + 1].b.p, 0.f, 0.f));
     }
@@ -2093,19 +2278,15 @@ int amg_apply_batch(mi_batch* B, int W, const float* in, float* out, Launcher& L
                  static_cast<const double*>(B->amg_cinv.p), bptr(nl - 1), xptr(nl - 1)));
     for (int l = nl - 2; l >= 0; --l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
-        const BCsr A = csr(d.arp, d.aci, d.av, d.n, d.diff);
         float* x = xptr(l);
-        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, csr(d.prp, d.pci, d.pv, d.n), W, 3,
-                 static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f, 0.f));
-        MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(d.e.p), d.d.p,
-                 d.r.p, static_cast<const float*>(d.invD.p), x, fnull, 0.f,
-                 float(d.alpha[0])));
+        MI_TRY(L(kr, rows_grid(d.n), kBlock, W, sell(d.psoff, d.peci, d.pv, d.n, 0), 3,
+                 static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f,
+                 0.f));
+        MI_TRY(smooth(l, 0, d.e.p, d.d.p, x, 0.0, d.alpha[0]));
         float* din = d.d.p;
         float* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
-            MI_TRY(L(k_bamg_rows, grid(d.n), kBlock, A, W, 0, static_cast<const float*>(din), dout,
-                     d.r.p, static_cast<const float*>(d.invD.p), x, fnull,
-                     float(d.beta[size_t(k)]), float(d.alpha[size_t(k)])));
+            MI_TRY(smooth(l, 0, din, dout, x, d.beta[size_t(k)], d.alpha[size_t(k)]));
             std::swap(din, dout);
         }
     }
@@ -2125,7 +2306,7 @@ bool separate_pc(int pc) { return pc == MI_PC_ILU || pc == MI_PC_AMG; }
 int separate_apply(mi_batch* B, const BArgs& a, const float* in, float* out, Launcher& L)
 {
     if (B->pc == MI_PC_AMG)
-        return amg_apply_batch(B, a.W, in, out, L);
+        return amg_apply_batch(B, a, in, out, L);
     return ilu_apply(B, a, in, out, L);
 }
 
@@ -2403,6 +2584,21 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
         MI_TRY(amg_setup_batch(B, c));
     }
     const bool ext = bicg || separate_pc(pc);
+    {
+        double m = 1.0;
+        for (double v : B->margin)
+            m = std::min(m, v);
+        const bool x64 = m < kX64Margin;
+        if (x64 != B->x64) {
+            B->x64 = x64;
+            if (B->graph) { // its launches hold the other correction
+                (void)hipGraphExecDestroy(B->graph);
+                B->graph = nullptr;
+            }
+            for (BSlots& S : B->set)
+                MI_TRY(alloc_slots(B, S, false));
+        }
+    }
     if (gmres)
         MI_TRY(gm_alloc(B, gm_m));
     if (ext && !B->ext_alloc) {
@@ -2459,7 +2655,10 @@ int solve(mi_batch* B, const mi_solver_cfg& c, const double* b_dev, double* x_de
             BSlots& S = B->set[B->cur];
             k_btile<double><<<tile_grid(n), kBlock, 0, s>>>(n, S.W, S.d_eq.p, 0ull, 0, 0, 0.f,
                                                             B->rt.p, S.r.p);
-            k_bzero32<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(n * S.W, S.xs.p);
+            if (B->x64)
+                k_bfill<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(n * S.W, S.xs64.p);
+            else
+                k_bzero32<<<reduce_grid(n * S.W, 4), kBlock, 0, s>>>(n * S.W, S.xs.p);
             MI_HIP(hipGetLastError());
             Launcher L{s};
             f.W = S.W;
@@ -2662,8 +2861,18 @@ int write_matrix(mi_batch* B, int e, const mi_csr* A)
     if (B->n)
         k_binvdiag<<<grid_for(B->n, kBlock), kBlock, 0, s>>>(
             int(B->n), B->pat->diag.p, v64, B->invDh.p + size_t(e) * size_t(B->n));
+    constexpr int kMarginBlocks = 256;
+    double part[kMarginBlocks];
+    if (B->n)
+        k_bmargin<<<kMarginBlocks, kBlock, 0, s>>>(int(B->n), B->pat->rp.p, B->pat->ci.p, v64,
+                                                   B->partials.p);
     MI_HIP(hipGetLastError());
+    MI_HIP(hipMemcpyAsync(part, B->partials.p, sizeof(part), hipMemcpyDeviceToHost, s));
     MI_HIP(hipStreamSynchronize(s));
+    double m = 0.0;
+    for (int k = 0; B->n && k < kMarginBlocks; ++k)
+        m += part[k];
+    B->margin[size_t(e)] = B->n ? m / double(B->n) : 1.0;
     B->slots_valid = false;
     B->ilu_valid = false;
     B->amg_valid = false;
@@ -2694,6 +2903,7 @@ MI_API int mi_batch_create_ex(const mi_csr* A, int n_eq, int flags, mi_batch** o
     B->Wmax = n_eq;
     B->shared = (flags & MI_BATCH_SHARED_VALUES) != 0;
     B->Em = B->shared ? 1 : n_eq;
+    B->margin.assign(size_t(B->Em), -1.0); // no values yet
     B->ilu_order = (flags & MI_BATCH_ILU0) != 0;
     B->n = A->n();
     B->nnz = A->nnz();
@@ -2956,6 +3166,30 @@ MI_API int mi_batch_tuning_shape(const mi_batch* B, int* slots_per_thread_max,
         *vector_workgroups = B->Gv;
     if (vector_us)
         *vector_us = B->tune_vec_us;
+    return MI_OK;
+}
+
+MI_API int mi_batch_set_shape(mi_batch* B, int slots_per_thread_max, int spmm_workgroups,
+                              int banded)
+{
+    clear_error();
+    if (!B)
+        return set_error(MI_ERR_BAD_ARG, "null handle");
+    if (slots_per_thread_max != 1 && slots_per_thread_max != 2 && slots_per_thread_max != 4)
+        return set_error(MI_ERR_BAD_ARG, "slots per thread cap must be 1, 2 or 4 (%d given)",
+                         slots_per_thread_max);
+    if (spmm_workgroups < 8 || spmm_workgroups > kBatchMaxGrid || spmm_workgroups % 8 != 0)
+        return set_error(MI_ERR_BAD_ARG, "SpMM workgroups must be a multiple of 8 in [8, %d] (%d given)",
+                         kBatchMaxGrid, spmm_workgroups);
+    if (banded != 0 && banded != 1)
+        return set_error(MI_ERR_BAD_ARG, "banded must be 0 or 1");
+    B->Vmax = slots_per_thread_max;
+    B->G = spmm_workgroups;
+    B->band = banded;
+    if (B->graph) { // captured with the previous shape
+        (void)hipGraphExecDestroy(B->graph);
+        B->graph = nullptr;
+    }
     return MI_OK;
 }
 

@@ -117,6 +117,32 @@ def test_batch_tuning_is_reported(sys8):
     assert rc == capi.MI_OK and np.all(rel <= 1e-8)
 
 
+def test_batch_set_shape_pins_the_bits(sys8):
+    """Two batches given the same SpMM shape return the same bits; another
+    shape the same solution to the tolerance; bad shapes are refused."""
+    w, dm, A, rp, ci, vals = sys8
+    b = np.stack(w.bs[:4])
+    out = []
+    for shape in ((4, 512, 1), (1, 256, 0)):
+        xs = []
+        for _ in range(2):
+            B = capi.Batch(A, 4)
+            B.set_values(np.stack(vals[:4]))
+            B.set_shape(*shape)
+            t = B.tuning()
+            assert (t["slots_per_thread_max"], t["spmm_workgroups"], t["banded"]) == shape
+            x, its, rel, rc = B.solve(b, tolerance=1e-8, max_iterations=2000)
+            assert rc == capi.MI_OK
+            xs.append(x)
+        assert np.array_equal(xs[0], xs[1])
+        out.append(xs[0])
+    assert np.linalg.norm(out[0] - out[1]) <= 1e-6 * np.linalg.norm(out[1])
+    B = capi.Batch(A, 4)
+    for bad in ((3, 512, 0), (4, 100, 0), (4, 8192, 0), (4, 512, 2)):
+        with pytest.raises(capi.MiError):
+            B.set_shape(*bad)
+
+
 def test_grid_barrier_timing(gpu):
     """mi_time_grid_barrier reports a cooperative grid.sync() and a kernel
     boundary; a grid larger than can be resident is refused."""

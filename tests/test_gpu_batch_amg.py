@@ -14,8 +14,10 @@ Pinned here:
     |z| (a dropped smoothing step, a wrong Chebyshev coefficient or a
     transposed transfer moves z by 1e-2 or more);
   - CG, BiCGStab and GMRES with AMG against scipy's direct solution (1e-6)
-    and by their FP64 true residual (<= 1e-8), with iteration counts near
-    the single-equation AMG-PCG's;
+    and by their FP64 true residual (<= 1e-8); batched CG's iteration counts
+    within 2 of the single-equation mixed AMG-PCG's (the same V-cycle and
+    refinement; the sweeps' correction summed in FP64 on this nearly
+    singular operator, as the single-equation solver's iterate);
   - graph replays equal the eager launches bit for bit;
   - new values rebuild the hierarchy; batches without shared values refuse.
 """
@@ -91,11 +93,13 @@ def test_batched_krylov_amg_solves(p24, kind):
     if kind == capi.MI_SOLVER_GMRES:
         kw["restart"] = 20
     x, its, rel, rc = B.solve(b, **kw)
+    # the single-equation solver in the batch's arithmetic (FP32 inner, FP64
+    # refinement), one right-hand side at a time
     s = capi.Solver(kind=capi.MI_SOLVER_CG, preconditioner=capi.MI_PC_AMG, tolerance=1e-8,
-                    max_iterations=300, true_residual=1)
+                    max_iterations=300, true_residual=1, precision=capi.MI_PRECISION_MIXED)
     s.setup(M)
-    _, st, _ = s.solve(b[0])
-    print(f"kind {kind}: batched AMG iterations {its.tolist()}, single AMG-PCG {st.iterations}")
+    its1 = np.array([s.solve(b[e])[1].iterations for e in range(n_eq)])
+    print(f"kind {kind}: batched AMG iterations {its.tolist()}, single AMG-PCG {its1.tolist()}")
     assert rc == capi.MI_OK
     lu = spla.splu(A.tocsc())
     for e in range(n_eq):
@@ -103,7 +107,9 @@ def test_batched_krylov_amg_solves(p24, kind):
         assert np.linalg.norm(b[e] - A @ x[e]) <= 1.01e-8 * np.linalg.norm(b[e])
         assert np.linalg.norm(x[e] - xs) <= 1e-6 * np.linalg.norm(xs)
     if kind == capi.MI_SOLVER_CG:
-        assert its.max() <= st.iterations + 5
+        # the same preconditioner and refinement: the same counts but for the
+        # order of the dot products' sums
+        assert np.all(abs(its - its1) <= 2), (its, its1)
 
 
 def test_batched_amg_graph_replays_match_eager(p24):
