@@ -1000,8 +1000,9 @@ __global__ __launch_bounds__(kBlock) void k_bgm_jacobi(BArgs a, const float* __r
 // slot of the layout at once: the sequence of the single-vector cycle
 // (DHierarchy::cycle: Chebyshev pre-smoothing from x = 0, restriction of the
 // residual, the dense FP64 coarsest solve, prolongation and post-smoothing)
-// on plain FP32 CSR operators, each streamed once per launch for all W
-// right-hand sides.  Vectors of level l are [n_l][W] lines.
+// on FP32 SELL-64 operators (square ones in the difference form), each
+// streamed once per launch for all W right-hand sides.  Vectors of level l
+// are [n_l][W] lines.
 struct BSell
 {
     int nr;
@@ -1011,23 +1012,6 @@ struct BSell
     const int* eci;   // column per slot, kSellDiag on a diagonal, -1 padding
     const float* v;   // value per slot
 };
-
-// r = b, d = a0 D^-1 b, x = d over n W elements
-__global__ __launch_bounds__(kBlock) void k_bamg_init(int64_t n, int W, const float* __restrict__ b,
-                                                      const float* __restrict__ invD, float a0,
-                                                      float* __restrict__ x, float* __restrict__ r,
-                                                      float* __restrict__ d)
-{
-    const int64_t m = n * W;
-    const int64_t stride = int64_t(gridDim.x) * kBlock;
-    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < m; q += stride) {
-        const float bi = b[q];
-        const float di = a0 * invD[q / W] * bi;
-        r[q] = bi;
-        d[q] = di;
-        x[q] = di;
-    }
-}
 
 // One launch of the batched V-cycle over the rows of A and lines of W
 // slots, in the SpMM's thread layout: TW = W / V threads per row, each
@@ -1039,6 +1023,12 @@ __global__ __launch_bounds__(kBlock) void k_bamg_init(int64_t n, int W, const fl
 //   mode 1 (residual):       r -= A din
 //   mode 2 (restriction):    y = A din
 //   mode 3 (prolongation):   y = A din; x += y
+//   mode 4 (last step):      mode 0 without storing r and dout
+//   mode 5 (first step):     din = b, d0 = a0 D^-1 b: r = b - A d0;
+//                            dout = beta d0 + alpha D^-1 r; x = d0 + dout
+//   mode 6 (start, degree 1): din = b, d0 = a0 D^-1 b: r = b - A d0; x = d0
+// (modes 5 and 6 start the smoother from x = 0 in the same launch as its
+// first product, gathering a0 D^-1 b in place of d0).
 template <int V>
 __global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
                                                       const float* __restrict__ din,
@@ -1046,7 +1036,8 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
                                                       float* __restrict__ r,
                                                       const float* __restrict__ invD,
                                                       float* __restrict__ x,
-                                                      float* __restrict__ y, float beta, float alpha)
+                                                      float* __restrict__ y, float beta, float alpha,
+                                                      float a0)
 {
     using VT = typename SlotVec<V>::type;
     constexpr int U = V == 1 ? kSpmmUnroll : kSpmmUnroll / 2; // row entries per pass
@@ -1060,12 +1051,18 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
     const RowWalk rw = row_walk(0, A.nr, R, A.band);
     auto ld = [&](const float* p, size_t q) { return *reinterpret_cast<const VT*>(p + q); };
     auto st = [&](float* p, size_t q, VT v) { *reinterpret_cast<VT*>(p + q) = v; };
+    const bool from_b = mode >= 5;
+    // the vector the operator multiplies, at row j (entries q .. q + V - 1)
+    auto xin = [&](size_t qq, int64_t j) {
+        const VT v = ld(din, qq);
+        return from_b ? VT(a0 * invD[j]) * v : v;
+    };
     for (int64_t pass = rw.base; pass < rw.lim; pass += rw.stride) {
         const int64_t i = pass + grp;
         if (i >= rw.lim)
             break;
         const size_t q = size_t(i) * W + s0;
-        const VT xi = A.diff ? ld(din, q) : VT(0.f);
+        const VT xi = A.diff ? xin(q, i) : VT(0.f);
         VT acc = VT(0.f);
         const int slc = int(i / kSellSlice);
         const int m0 = A.soff[slc] + int(i % kSellSlice);
@@ -1082,14 +1079,32 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
             }
             VT xv[U];
 #pragma unroll
-            for (int j = 0; j < U; ++j)
-                xv[j] = c[j] >= 0 ? ld(din, size_t(c[j] & kSellColMask) * W + s0) : VT(0.f);
+            for (int j = 0; j < U; ++j) {
+                const int cj = c[j] & kSellColMask;
+                xv[j] = c[j] >= 0 ? xin(size_t(cj) * W + s0, cj) : VT(0.f);
+            }
 #pragma unroll
             for (int j = 0; j < U; ++j)
                 if (c[j] >= 0)
                     acc += VT(v[j]) * ((c[j] & kSellDiag) ? xv[j] : xv[j] - xi);
         }
-        if (mode <= 1) {
+        if (from_b) {
+            const VT b = ld(din, q);
+            const VT d0 = VT(a0 * invD[i]) * b;
+            const VT ri = b - acc;
+            st(r, q, ri);
+            if (mode == 5) {
+                const VT dn = VT(beta) * d0 + VT(alpha * invD[i]) * ri;
+                st(dout, q, dn);
+                st(x, q, d0 + dn);
+            } else {
+                st(x, q, d0);
+            }
+        } else if (mode == 4) {
+            const VT ri = ld(r, q) - acc;
+            const VT dn = VT(beta) * ld(din, q) + VT(alpha * invD[i]) * ri;
+            st(x, q, ld(x, q) + dn);
+        } else if (mode <= 1) {
             const VT ri = ld(r, q) - acc;
             st(r, q, ri);
             if (mode == 0) {
@@ -1105,7 +1120,7 @@ __global__ __launch_bounds__(kBlock) void k_bamg_rows(int W, BSell A, int mode,
     }
 }
 using BAmgRows = void (*)(int, BSell, int, const float*, float*, float*, const float*, float*,
-                          float*, float, float);
+                          float*, float, float, float);
 inline BAmgRows bamg_rows_kernel(int V)
 {
     return V == 4 ? k_bamg_rows<4> : V == 2 ? k_bamg_rows<2> : k_bamg_rows<1>;
@@ -2235,43 +2250,44 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
     auto rows_grid = [&](int64_t rows) {
         return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows + R - 1) / R, 16384)));
     };
-    auto vec_grid = [&](int64_t rows) {
-        return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows * W + kBlock - 1) / kBlock,
-                                                               16384)));
-    };
     auto sell = [&](const DevBuf<int>& soff, const DevBuf<int>& eci, const DevBuf<float>& v,
                     int nr, int diff) { return BSell{nr, diff, 0, soff.p, eci.p, v.p}; };
     auto bptr = [&](int l) -> const float* { return l == 0 ? in : B->amg[size_t(l)].b.p; };
     auto xptr = [&](int l) -> float* { return l == 0 ? out : B->amg[size_t(l)].x.p; };
     float* fnull = nullptr;
     const float* cnull = nullptr;
-    // r -= A din, and for mode 0 the Chebyshev update, on level l's operator
-    // (level 0: the batch's own, with the SpMM's grid and row walk)
+    // r -= A din, and for modes 0, 4, 5 the Chebyshev update, on level l's
+    // operator (level 0: the batch's own, with the SpMM's grid and row walk)
     auto smooth = [&](int l, int mode, const float* din, float* dout, float* x, double beta,
-                      double alpha) -> int {
+                      double alpha, double a0 = 0.0) -> int {
         mi_batch::ALevel& d = B->amg[size_t(l)];
         const bool own = l == 0 && B->sell;
         const BSell A = own ? BSell{d.n, a.diff, a.band, a.soff, a.eci, a.v32}
                             : sell(d.asoff, d.aeci, d.av, d.n, d.diff);
         return L(kr, own ? unsigned(B->G) : rows_grid(d.n), kBlock, W, A, mode, din, dout, d.r.p,
-                 static_cast<const float*>(d.invD.p), x, fnull, float(beta), float(alpha));
+                 static_cast<const float*>(d.invD.p), x, fnull, float(beta), float(alpha),
+                 float(a0));
     };
     for (int l = 0; l + 1 < nl; ++l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
         float* x = xptr(l);
-        MI_TRY(L(k_bamg_init, vec_grid(d.n), kBlock, int64_t(d.n), W, bptr(l),
-                 static_cast<const float*>(d.invD.p), float(d.alpha[0]), x, d.r.p, d.d.p));
-        float* din = d.d.p;
-        float* dout = d.d2.p;
-        for (int k = 1; k < d.deg; ++k) {
-            MI_TRY(smooth(l, 0, din, dout, x, d.beta[size_t(k)], d.alpha[size_t(k)]));
-            std::swap(din, dout);
+        // pre-smoothing from x = 0, its start fused with the first product
+        if (d.deg == 1) {
+            MI_TRY(smooth(l, 6, bptr(l), fnull, x, 0.0, 0.0, d.alpha[0]));
+        } else {
+            MI_TRY(smooth(l, 5, bptr(l), d.d.p, x, d.beta[1], d.alpha[1], d.alpha[0]));
+            float* din = d.d.p;
+            float* dout = d.d2.p;
+            for (int k = 2; k < d.deg; ++k) {
+                MI_TRY(smooth(l, 0, din, dout, x, d.beta[size_t(k)], d.alpha[size_t(k)]));
+                std::swap(din, dout);
+            }
+            MI_TRY(smooth(l, 1, din, nullptr, x, 0.0, 0.0));
         }
-        MI_TRY(smooth(l, 1, din, nullptr, x, 0.0, 0.0));
         const int nc = B->amg[size_t(l) + 1].n;
         MI_TRY(L(kr, rows_grid(nc), kBlock, W, sell(d.rsoff, d.reci, d.rv, nc, 0), 2,
                  static_cast<const float*>(d.r.p), fnull, fnull, cnull, fnull,
-                 B->amg[size_t(l) + 1].b.p, 0.f, 0.f));
+                 B->amg[size_t(l) + 1].b.p, 0.f, 0.f, 0.f));
     }
     if (B->amg_nc > 0)
         MI_TRY(L(k_bamg_coarse, unsigned(B->amg_nc), kBlock, B->amg_nc, W,
@@ -2281,12 +2297,14 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
         float* x = xptr(l);
         MI_TRY(L(kr, rows_grid(d.n), kBlock, W, sell(d.psoff, d.peci, d.pv, d.n, 0), 3,
                  static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f,
-                 0.f));
-        MI_TRY(smooth(l, 0, d.e.p, d.d.p, x, 0.0, d.alpha[0]));
+                 0.f, 0.f));
+        // post-smoothing; its last step stores only x
+        MI_TRY(smooth(l, d.deg == 1 ? 4 : 0, d.e.p, d.d.p, x, 0.0, d.alpha[0]));
         float* din = d.d.p;
         float* dout = d.d2.p;
         for (int k = 1; k < d.deg; ++k) {
-            MI_TRY(smooth(l, 0, din, dout, x, d.beta[size_t(k)], d.alpha[size_t(k)]));
+            MI_TRY(smooth(l, k + 1 == d.deg ? 4 : 0, din, dout, x, d.beta[size_t(k)],
+                          d.alpha[size_t(k)]));
             std::swap(din, dout);
         }
     }
