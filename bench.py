@@ -910,6 +910,9 @@ def main():
     ap.add_argument("--spmv-only", action="store_true",
                     help="time the working-matrix SpMV and a calibration copy only "
                          "(short run for hardware-counter passes)")
+    ap.add_argument("--pressure-batch-only", action="store_true",
+                    help="assemble the pressure matrix, then only the batched AMG-PCG solve of 8 "
+                         "right-hand sides against the mixed solver one at a time (for profiling)")
     ap.add_argument("--batch-spmm-only", action="store_true",
                     help="build the 32-equation transport batch, run three batched CG "
                          "iterations and a calibration copy only (hardware-counter passes)")
@@ -1000,6 +1003,12 @@ def main():
                         max_iterations=20000, precision=precision, true_residual=1, **kw)
         s.setup(A)
         return s
+
+    if args.pressure_batch_only:
+        out = bench_pressure_batched(capi, wl, lib, A, pinned,
+                                     make_solver(capi.MI_PRECISION_MIXED), args.steps, log)
+        print(json.dumps(out), flush=True)
+        return
 
     results, solvers = {}, {}
     modes = [m for m in (("fp64", capi.MI_PRECISION_FP64), ("mixed", capi.MI_PRECISION_MIXED))
