@@ -2265,9 +2265,15 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
     const int nl = int(B->amg.size());
     const int W = a.W;
     const int V = slots_per_thread(W, B->Vmax);
-    const int R = kBlock / (W / V);
-    const BAmgRows kr = bamg_rows_kernel(V);
+    // operators of few rows take one slot per thread (W threads a row): more
+    // workgroups for their latency-bound launches (8 pressure right-hand
+    // sides at 128^3: 68.9 -> 65.5 ms with levels 2 and 3 so; level 1's 451k
+    // rows as well: 69.4 ms)
+    constexpr int64_t kFewRows = 100000;
+    auto vfor = [&](int64_t rows) { return rows < kFewRows ? 1 : V; };
+    auto kern = [&](int64_t rows) { return bamg_rows_kernel(vfor(rows)); };
     auto rows_grid = [&](int64_t rows) {
+        const int R = kBlock / (W / vfor(rows));
         return unsigned(std::max<int64_t>(1, std::min<int64_t>((rows + R - 1) / R, 16384)));
     };
     auto vec_grid = [&](int64_t rows) {
@@ -2288,7 +2294,8 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
         const bool own = l == 0 && B->sell;
         const BSell A = own ? BSell{d.n, a.diff, a.band, a.soff, a.eci, a.v32}
                             : sell(d.asoff, d.aeci, d.av, d.n, d.diff);
-        return L(kr, own ? unsigned(B->G) : rows_grid(d.n), kBlock, W, A, mode, din, dout, d.r.p,
+        return L(own ? bamg_rows_kernel(V) : kern(d.n), own ? unsigned(B->G) : rows_grid(d.n),
+                 kBlock, W, A, mode, din, dout, d.r.p,
                  static_cast<const float*>(d.invD.p), x, fnull, float(beta), float(alpha),
                  float(a0));
     };
@@ -2320,7 +2327,7 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
             MI_TRY(smooth(l, 1, din, nullptr, x, 0.0, 0.0));
         }
         const int nc = B->amg[size_t(l) + 1].n;
-        MI_TRY(L(kr, rows_grid(nc), kBlock, W, sell(d.rsoff, d.reci, d.rv, nc, 0), 2,
+        MI_TRY(L(kern(nc), rows_grid(nc), kBlock, W, sell(d.rsoff, d.reci, d.rv, nc, 0), 2,
                  static_cast<const float*>(d.r.p), fnull, fnull, cnull, fnull,
                  B->amg[size_t(l) + 1].b.p, 0.f, 0.f, 0.f));
     }
@@ -2330,7 +2337,7 @@ int amg_apply_batch(mi_batch* B, const BArgs& a, const float* in, float* out, La
     for (int l = nl - 2; l >= 0; --l) {
         mi_batch::ALevel& d = B->amg[size_t(l)];
         float* x = xptr(l);
-        MI_TRY(L(kr, rows_grid(d.n), kBlock, W, sell(d.psoff, d.peci, d.pv, d.n, 0), 3,
+        MI_TRY(L(kern(d.n), rows_grid(d.n), kBlock, W, sell(d.psoff, d.peci, d.pv, d.n, 0), 3,
                  static_cast<const float*>(xptr(l + 1)), fnull, fnull, cnull, x, d.e.p, 0.f,
                  0.f, 0.f));
         // post-smoothing; its last step stores only x
