@@ -35,11 +35,14 @@
 // Arithmetic: PCG (CGSolver::solve, src/solver/LinearSolvers.cpp:420-452),
 // BiCGStab (BiCGSTABSolver::solve, :360-404) or GMRES(m) (GMRESmSolver::solve,
 // :266-340, one cycle per sweep) per equation, with Jacobi, ILU(0) (factorized
-// on the device under a two-colour ordering) or no preconditioner, FP32
-// values, vectors and correction and FP64 dot products and norms; the FP64
-// true residual with the FP64 values refines each equation until
-// ||b_e - A_e x_e|| <= tol ||b_e|| (x_e accumulates each sweep's correction in
-// FP64).  Each equation has its own scalars and convergence flag.  One inner
+// on the device under a two-colour ordering), on a shared matrix the SA-AMG
+// V-cycle of its one hierarchy, or no preconditioner; FP32 values (in the
+// difference form when every row holds its diagonal) and vectors, the sweep's
+// correction in FP32 or, for weakly dominant operators, FP64, and FP64 dot
+// products and norms; the FP64 true residual with the FP64 values refines
+// each equation until ||b_e - A_e x_e|| <= tol ||b_e|| (x_e accumulates each
+// sweep's correction in FP64).  Each equation has its own scalars and
+// convergence flag.  One inner
 // CG iteration is five launches (eight with ILU(0)), one BiCGStab iteration
 // nine (twelve); `chunk` iterations, or a whole GMRES cycle, are replayed
 // from a HIP graph between polls.
